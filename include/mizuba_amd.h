@@ -1,0 +1,186 @@
+/*
+ * mizuba_amd C ABI -- MI355X (gfx950) conjunction screening over a polyjectory.
+ *
+ * This is the drop-in boundary for the path
+ *     conjunctions(pj, conj_thresh, conj_det_interval, otypes)
+ * of bluescarni/mizuba 0.0.12. The reference has no plugin registry: the path
+ * sits behind the C++ class `mizuba::conjunctions` (mizuba/conjunctions.hpp:57-237),
+ * whose constructor runs `detect_conjunctions` (mizuba/detect_conjunctions.cpp:55-537)
+ * and whose accessors expose the per-step arrays. Each entry point below names
+ * the reference interface it replaces.
+ *
+ * Conventions
+ *  - Plain C types only; every entry point returns an int status (MZH_OK on
+ *    success). The message of the last failure of a context is read with
+ *    mzh_last_error(); errors raised before a context exists are read with
+ *    mzh_global_error(). No C++ exception crosses this boundary.
+ *  - Status kinds mirror the exception types of the reference:
+ *    MZH_EINVAL  <-> std::invalid_argument (ValueError in Python),
+ *    MZH_ERANGE  <-> std::out_of_range     (IndexError),
+ *    MZH_ERUNTIME<-> std::runtime_error / HIP failures (RuntimeError).
+ *  - A context owns one HIP stream on one device and all device buffers of
+ *    one problem. Calls on one context must not overlap; different contexts
+ *    may be used concurrently from different threads.
+ *  - Layouts and record sizes are the reference's:
+ *      aabbs, srt_aabbs : float   [C][N+1][2][4]   (row N = global box)
+ *      mcodes, srt_mcodes: uint64 [C][N];  srt_idx: uint32 [C][N]
+ *      cd_end_times     : double  [C]
+ *      mzh_bvh_node 48 B, mzh_aabb_collision 8 B, mzh_conj 120 B.
+ */
+#ifndef MIZUBA_AMD_H
+#define MIZUBA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MZH_OK 0
+#define MZH_EINVAL 1
+#define MZH_ERANGE 2
+#define MZH_ERUNTIME 3
+#define MZH_ENOMEM 4
+
+/* Precision modes of the screen (AABB -> broad phase).
+ * MZH_MODE_FP64 : drop-in arithmetic of the reference -- FP64 Cargo-Shisha
+ *                 enclosures converted outward to FP32 boxes; bit-exact with
+ *                 the CPU restatement.
+ * MZH_MODE_MIXED: FP32 enclosures with a rigorous a-priori error margin; every
+ *                 box contains the FP64-mode box, so the candidate set is a
+ *                 superset and the FP64 narrow phase returns the same
+ *                 conjunctions. */
+#define MZH_MODE_FP64 0
+#define MZH_MODE_MIXED 1
+
+/* conjunctions::bvh_node (mizuba/conjunctions.hpp:65-72). */
+typedef struct {
+    uint32_t begin, end;
+    int32_t left, right;
+    float lb[4], ub[4];
+} mzh_bvh_node;
+
+/* conjunctions::aabb_collision (mizuba/conjunctions.hpp:75-79). */
+typedef struct {
+    uint32_t i, j;
+} mzh_aabb_collision;
+
+/* conjunctions::conj (mizuba/conjunctions.hpp:82-94). */
+typedef struct {
+    uint32_t i, j;
+    double tca, dca;
+    double ri[3], vi[3], rj[3], vj[3];
+} mzh_conj;
+
+/* Narrow-phase counters in the order of conjunctions::np_report
+ * (mizuba/conjunctions.hpp:150-175). */
+#define MZH_NP_COUNT 7
+
+/* Stage timings reported by mzh_get_stage_times (seconds, device clock):
+ * aabbs, morton (codes + sort + gather), bvh, broad, narrow, total, then the
+ * AABB kernel alone and the (last) narrow-phase kernel alone. */
+#define MZH_STAGE_COUNT 8
+
+typedef struct mzh_ctx mzh_ctx;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Create a context on HIP device `device` with its own stream. No reference
+ * counterpart (the reference runs on the host); it stands for the per-call
+ * state that conjunctions::conjunctions builds (mizuba/conjunctions.cpp:263-360). */
+int mzh_create(int device, mzh_ctx **out);
+int mzh_destroy(mzh_ctx *ctx);
+/* Copy the message of the last failure on ctx into buf (NUL-terminated). */
+int mzh_last_error(const mzh_ctx *ctx, char *buf, size_t len);
+/* Message of the last failure of mzh_create (thread-local). */
+int mzh_global_error(char *buf, size_t len);
+/* Number of HIP devices visible (0 when none). */
+int mzh_device_count(int *count);
+
+/* ---- input: the polyjectory -------------------------------------------- */
+
+/* Upload a polyjectory to HBM. Replaces the reads through polyjectory::operator[]
+ * (mizuba/polyjectory.cpp:1006-1036) that every stage of the reference does:
+ *   traj      : double[S_tot][order+1][7], all objects' steps concatenated
+ *   time      : double[T_tot], per object nsteps+1 values (none if nsteps == 0)
+ *   traj_off  : per object, offset in doubles of its first step in traj
+ *   nsteps    : per object, number of trajectory steps
+ *   time_off  : per object, offset in doubles of its first time in time
+ *   maxT      : largest final time over all objects (polyjectory::get_maxT)
+ * The data must already satisfy the polyjectory invariants (finite values,
+ * strictly increasing non-negative times, order >= 2); the host front end
+ * checks them with the reference's messages. The upload also builds the
+ * step-major transposed coefficient copy used by the AABB kernels. */
+int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const double *traj, uint64_t traj_len,
+                           const double *time, uint64_t time_len, const uint64_t *traj_off, const uint64_t *nsteps,
+                           const uint64_t *time_off, double maxT);
+
+/* ---- the hot path ------------------------------------------------------ */
+
+/* Run conjunction screening over all conjunction steps. Replaces
+ * conjunctions::conjunctions (mizuba/conjunctions.cpp:263-360: validation,
+ * n_cd_steps = ceil(maxT / conj_det_interval)) followed by
+ * conjunctions::detect_conjunctions (mizuba/detect_conjunctions.cpp:55-537)
+ * and its five stages:
+ *   detect_conjunctions_aabbs        (mizuba/detect_conjunctions_aabbs.cpp:300-398)
+ *   detect_conjunctions_morton       (mizuba/detect_conjunctions_morton.cpp:113-224)
+ *   detect_conjunctions_bvh          (mizuba/detect_conjunctions_bvh.cpp:267-699)
+ *   detect_conjunctions_broad_phase  (mizuba/detect_conjunctions_broad_phase.cpp:153-339)
+ *   detect_conjunctions_narrow_phase (mizuba/detect_conjunctions_narrow_phase.cpp:167-769)
+ * otypes may be NULL (all primaries); otherwise it holds n_objs values in
+ * {1, 2, 4} (otype PRIMARY / SECONDARY / MASKED). Results stay in HBM and are
+ * read with the accessors below. Error messages are the reference's. */
+int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode);
+
+/* ---- accessors (mizuba/conjunctions.cpp:394-509) ------------------------ */
+
+/* conjunctions::get_n_cd_steps (conjunctions.cpp:457-460). */
+int mzh_get_n_cd_steps(const mzh_ctx *ctx, uint64_t *out);
+/* Number of objects of the uploaded polyjectory. */
+int mzh_get_n_objs(const mzh_ctx *ctx, uint64_t *out);
+/* conjunctions::get_cd_end_times (conjunctions.cpp:399-402): double[C]. */
+int mzh_copy_cd_end_times(mzh_ctx *ctx, double *dst);
+/* conjunctions::get_aabbs / get_srt_aabbs (conjunctions.cpp:394-397, 404-407):
+ * float[C][N+1][2][4] into dst. */
+int mzh_copy_aabbs(mzh_ctx *ctx, float *dst);
+int mzh_copy_srt_aabbs(mzh_ctx *ctx, float *dst);
+/* conjunctions::get_mcodes / get_srt_mcodes / get_srt_idx (conjunctions.cpp:409-422). */
+int mzh_copy_mcodes(mzh_ctx *ctx, uint64_t *dst);
+int mzh_copy_srt_mcodes(mzh_ctx *ctx, uint64_t *dst);
+int mzh_copy_srt_idx(mzh_ctx *ctx, uint32_t *dst);
+/* conjunctions::get_bvh_tree (conjunctions.cpp:424-439): size, then the nodes
+ * of step `step`. MZH_ERANGE with the reference's message if step >= C. */
+int mzh_get_bvh_tree_size(mzh_ctx *ctx, uint64_t step, uint64_t *size);
+int mzh_copy_bvh_tree(mzh_ctx *ctx, uint64_t step, mzh_bvh_node *dst);
+/* conjunctions::get_aabb_collisions (conjunctions.cpp:441-455): per-step
+ * broad-phase pairs, sorted by (i, j). */
+int mzh_get_aabb_collisions_size(mzh_ctx *ctx, uint64_t step, uint64_t *size);
+int mzh_copy_aabb_collisions(mzh_ctx *ctx, uint64_t step, mzh_aabb_collision *dst);
+/* (offset, size) of every step's pairs in the concatenated pair array, as
+ * the reference's bp_offsets (detect_conjunctions.cpp:141-152): uint64[C][2]. */
+int mzh_copy_bp_offsets(mzh_ctx *ctx, uint64_t *dst);
+/* conjunctions::get_conjunctions (conjunctions.cpp:478-491): all conjunctions,
+ * step by step, each step sorted by (tca, then lexicographically). */
+int mzh_get_n_conjunctions(mzh_ctx *ctx, uint64_t *out);
+int mzh_copy_conjunctions(mzh_ctx *ctx, mzh_conj *dst);
+/* np_report counters of the last screen (detect_conjunctions.cpp:490-514). */
+int mzh_get_np_report(mzh_ctx *ctx, uint64_t out[MZH_NP_COUNT]);
+/* Per-stage device times of the last screen (the trace breakdown of
+ * detect_conjunctions.cpp:444-454), seconds: see MZH_STAGE_COUNT. */
+int mzh_get_stage_times(mzh_ctx *ctx, double out[MZH_STAGE_COUNT]);
+
+/* Device-resident views (no reference counterpart; the reference's arrays are
+ * memory-mapped files): device pointer and byte size of an output array.
+ * which: 0 aabbs, 1 srt_aabbs, 2 mcodes, 3 srt_mcodes, 4 srt_idx,
+ *        5 bvh nodes (fixed capacity per step), 6 pairs, 7 conjunctions. */
+int mzh_device_view(mzh_ctx *ctx, int which, void **ptr, uint64_t *bytes);
+
+/* Wait for all work queued on the context's stream. */
+int mzh_synchronize(mzh_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,51 @@
+"""mizuba_amd -- MI355X-native conjunction screening over polyjectories.
+
+Drop-in for the conjunction-detection path of mizuba
+(`mizuba.conjunctions(pj, conj_thresh, conj_det_interval, otypes=None, ...)`):
+the same constructor arguments, accessors, array layouts and record dtypes,
+computed by hand-written gfx950 HIP kernels behind the C ABI declared in
+include/mizuba_amd.h.
+
+Extra keyword arguments of `conjunctions`: `mode` ("fp64": the reference's
+arithmetic, bit-exact screen; "mixed": FP32 enclosures with a rigorous margin,
+same conjunction list) and `device` (HIP device index).
+
+The native library must be built (`python -m mizuba_amd.build_native`); there
+is no CPU fallback: without a HIP device, constructing `conjunctions` raises
+RuntimeError.
+"""
+
+from __future__ import annotations
+
+import enum as _enum
+
+try:
+    from . import _core
+except ImportError as _e:  # pragma: no cover - depends on the build
+    raise ImportError(
+        "mizuba_amd native extension is not built; run `python -m mizuba_amd.build_native` first"
+    ) from _e
+
+polyjectory = _core.polyjectory
+conjunctions = _core.conjunctions
+Screener = _core.Screener
+make_synthetic_polyjectory = _core.make_synthetic_polyjectory
+device_count = _core.device_count
+
+
+class otype(_enum.IntEnum):
+    """Object types for conjunction screening (mizuba/__init__.py:61-66)."""
+
+    PRIMARY = 1
+    SECONDARY = 2
+    MASKED = 4
+
+
+__all__ = [
+    "polyjectory",
+    "conjunctions",
+    "otype",
+    "Screener",
+    "make_synthetic_polyjectory",
+    "device_count",
+]

@@ -1,0 +1,804 @@
+// C ABI of mizuba_amd and the host-side orchestrator of one screening problem.
+//
+// The orchestrator replaces conjunctions::conjunctions + detect_conjunctions
+// (mizuba/conjunctions.cpp:263-360, mizuba/detect_conjunctions.cpp:55-537):
+// instead of a TBB loop over conjunction steps in chunks of 128 with a writer
+// thread, every stage runs once over all steps as a batched gfx950 launch,
+// and the outputs stay resident in HBM until an accessor copies them out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mizuba_amd.h"
+#include "mzh_device.hpp"
+#include "mzh_kernels.hpp"
+
+namespace
+{
+
+using namespace mzh;
+
+// Shortest decimal text that round-trips a double (the {} formatting of the
+// reference's fmt messages).
+std::string fmt_double(double v)
+{
+    if (std::isnan(v)) return "nan";
+    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+    char buf[64];
+    for (int prec = 1; prec <= 17; ++prec) {
+        std::snprintf(buf, sizeof(buf), "%.*g", prec, v);
+        if (std::strtod(buf, nullptr) == v) break;
+    }
+    std::string s(buf);
+    // fmt prints integral doubles without exponent and without a trailing '.0'
+    return s;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t ensure(size_t b)
+    {
+        if (b <= bytes && p) return hipSuccess;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        const size_t alloc = b ? b : 256;
+        hipError_t e = hipMalloc(&p, alloc);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        bytes = alloc;
+        return hipSuccess;
+    }
+    template <typename T>
+    T *as() const
+    {
+        return static_cast<T *>(p);
+    }
+};
+
+struct StatusError {
+    int code;
+    std::string msg;
+};
+
+} // namespace
+
+struct mzh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+
+    // polyjectory
+    bool has_pj = false;
+    uint32_t n_objs = 0, order = 0;
+    double maxT = 0;
+    uint64_t n_steps_total = 0;
+    DevBuf traj, time, cf4, step0, nsteps, time_off;
+
+    // problem state
+    bool screened = false;
+    bool skip_cd = false;
+    uint64_t C = 0;
+    double conj_thresh = 0, conj_det_interval = 0;
+    int mode = MZH_MODE_FP64;
+    std::vector<double> h_cd_begin, h_cd_end;
+    DevBuf cd_begin, cd_end;
+    DevBuf aabbs, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, gbox;
+    uint64_t node_cap = 0;
+    DevBuf nodes, nolc, ps, tree_size;
+    DevBuf otypes, cnt, off, step_total, step_off, grand;
+    DevBuf pairs;
+    uint64_t n_pairs = 0;
+    DevBuf conj_raw, conj_step, conj_sorted, conj_count, np_report, conj_cnt, conj_start, conj_cursor, conj_pos,
+        conj_inv, conj_grand;
+    uint64_t conj_cap = 0, n_conj = 0;
+    DevBuf err;
+    hipEvent_t ev[10] = {};
+    double stage_s[MZH_STAGE_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t h_np[MZH_NP_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+
+    // host caches of per-step sizes (filled after a screen)
+    std::vector<uint32_t> h_tree_size;
+    std::vector<uint64_t> h_step_off;  // [C + 1]
+};
+
+namespace
+{
+
+thread_local std::string g_global_error;
+
+int fail(mzh_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+#define MZH_HIP(call)                                                                                                  \
+    do {                                                                                                               \
+        const hipError_t e_ = (call);                                                                                  \
+        if (e_ != hipSuccess)                                                                                          \
+            throw StatusError{e_ == hipErrorOutOfMemory ? MZH_ENOMEM : MZH_ERUNTIME,                                   \
+                              std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_)};                    \
+    } while (0)
+
+template <typename F>
+int guarded(mzh_ctx *ctx, F &&f)
+{
+    if (!ctx) return MZH_EINVAL;
+    try {
+        MZH_HIP(hipSetDevice(ctx->device));
+        f();
+        return MZH_OK;
+    } catch (const StatusError &e) {
+        return fail(ctx, e.code, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, MZH_ENOMEM, "host memory allocation failed");
+    } catch (...) {
+        return fail(ctx, MZH_ERUNTIME, "unknown error");
+    }
+}
+
+std::string dev_error_message(const DevError &e)
+{
+    const std::string i = std::to_string(e.i), j = std::to_string(e.j), v = fmt_double(e.value);
+    switch (e.code) {
+        case kErrAabbLb:
+            return "The computation of the bounding box for the object at index " + i
+                   + " produced the non-finite lower bound " + v;
+        case kErrAabbUb:
+            return "The computation of the bounding box for the object at index " + i
+                   + " produced the non-finite upper bound " + v;
+        case kErrGlobalBox:
+            return "A global bounding box with non-finite size was generated";
+        case kErrNpInterval:
+            return "During the narrow phase collision detection of objects " + i + " and " + j
+                   + ", an invalid time interval for polynomial root finding was generated";
+        case kErrNpDist2:
+            return "Non-finite value(s) detected during conjunction tracking for objects " + i + " and " + j;
+        case kErrNpRoot:
+            return "Polynomial root finding produced a non-finite root of " + v + " involving objects " + i + " and "
+                   + j;
+        case kErrNpRootDer:
+            return "Polynomial root finding produced the root " + v + " with nonfinite derivative involving objects "
+                   + i + " and " + j;
+        case kErrNpIsolation:
+            return "The polynomial root isolation algorithm failed: the working list size is " + i
+                   + " and the number of isolating intervals is " + j;
+        case kErrNpWorkList:
+            return "The polynomial root isolation algorithm failed: the working list size is "
+                   + std::to_string((uint64_t)e.value) + " and the number of isolating intervals is 0";
+        case kErrNpIters:
+            return "Polynomial root finding failed due to too many iterations";
+        case kErrNpErrno:
+            return "Polynomial root finding returned a nonzero errno with code '" + std::to_string((int)e.value) + "'";
+        case kErrNpConjDist2:
+            return "A non-finite conjunction distance square of " + v + " was computed for the objects at indices " + i
+                   + " and " + j;
+        case kErrNpBndDerUb:
+            return "An invalid value of " + v
+                   + " was computed for the derivative of the distance square function at the upper boundary of a "
+                     "trajectory";
+        case kErrNpBndDist2Ub:
+            return "An invalid value of " + v
+                   + " was computed for a candidate minimum of the distance square function at the upper boundary "
+                     "of a trajectory";
+        case kErrNpBndDerLb:
+            return "An invalid value of " + v
+                   + " was computed for the derivative of the distance square function at the lower boundary of a "
+                     "trajectory";
+        case kErrNpBndDist2Lb:
+            return "An invalid value of " + v
+                   + " was computed for a candidate minimum of the distance square function at the lower boundary "
+                     "of a trajectory";
+        default:
+            return "device error code " + std::to_string(e.code);
+    }
+}
+
+PjView pj_view(const mzh_ctx *ctx)
+{
+    PjView v;
+    v.traj = ctx->traj.as<const double>();
+    v.time = ctx->time.as<const double>();
+    v.cf4 = ctx->cf4.as<const double>();
+    v.step0 = ctx->step0.as<const uint64_t>();
+    v.nsteps = ctx->nsteps.as<const uint64_t>();
+    v.time_off = ctx->time_off.as<const uint64_t>();
+    v.n_objs = ctx->n_objs;
+    v.order = ctx->order;
+    return v;
+}
+
+// conjunctions::conjunctions validation + detect_conjunctions, batched.
+void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes_in, int mode)
+{
+    if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
+    if (mode != MZH_MODE_FP64 && mode != MZH_MODE_MIXED)
+        throw StatusError{MZH_EINVAL, "Invalid screening mode " + std::to_string(mode)};
+    // conjunctions.cpp:279-296
+    if (!std::isfinite(conj_thresh) || conj_thresh <= 0)
+        throw StatusError{MZH_EINVAL, "The conjunction threshold must be finite and positive, but instead a value of "
+                                          + fmt_double(conj_thresh) + " was provided"};
+    if (!std::isfinite(conj_thresh * conj_thresh))
+        throw StatusError{MZH_EINVAL, "A conjunction threshold of " + fmt_double(conj_thresh)
+                                          + " is too large and results in an overflow error"};
+    if (!std::isfinite(conj_det_interval) || conj_det_interval <= 0)
+        throw StatusError{MZH_EINVAL,
+                          "The conjunction detection interval must be finite and positive, but instead a value of "
+                              + fmt_double(conj_det_interval) + " was provided"};
+    const uint32_t N = ctx->n_objs;
+    // otypes (conjunctions.cpp:305-332)
+    std::vector<int32_t> ot(N, 1);
+    bool skip_cd = true;
+    if (otypes_in) {
+        for (uint32_t k = 0; k < N; ++k) {
+            const int32_t v = otypes_in[k];
+            if (v != 1 && v != 2 && v != 4)
+                throw StatusError{MZH_EINVAL, "The value of an object type must be one of [1, 2, 4], but a value of "
+                                                  + std::to_string(v) + " was detected instead"};
+            skip_cd = skip_cd && (v != 1);
+            ot[k] = v;
+        }
+    } else {
+        skip_cd = false;
+    }
+    // n_cd_steps (conjunctions.cpp:335)
+    const double cs = std::ceil(ctx->maxT / conj_det_interval);
+    if (!(cs >= 1) || cs > 4.0e9)
+        throw StatusError{MZH_EINVAL, "The number of conjunction steps " + fmt_double(cs) + " is not representable"};
+    const uint64_t C = (uint64_t)cs;
+    // get_cd_begin_end for every step (conjunctions.cpp:373-392)
+    ctx->h_cd_begin.resize(C);
+    ctx->h_cd_end.resize(C);
+    for (uint64_t k = 0; k < C; ++k) {
+        const double b = conj_det_interval * static_cast<double>(k);
+        const double e = (k == C - 1) ? ctx->maxT : conj_det_interval * static_cast<double>(k + 1);
+        if (!std::isfinite(b) || !std::isfinite(e) || !(e > b) || b < 0 || e > ctx->maxT)
+            throw StatusError{MZH_EINVAL, "Invalid conjunction step time range: [" + fmt_double(b) + ", "
+                                              + fmt_double(e) + ")"};
+        ctx->h_cd_begin[k] = b;
+        ctx->h_cd_end[k] = e;
+    }
+    const size_t boxes_bytes = sizeof(float) * 8u * (size_t)(N + 1) * C;
+    const size_t NC = (size_t)N * C;
+    hipStream_t st = ctx->stream;
+    ctx->screened = false;
+    ctx->C = C;
+    ctx->conj_thresh = conj_thresh;
+    ctx->conj_det_interval = conj_det_interval;
+    ctx->mode = mode;
+    ctx->skip_cd = skip_cd;
+    ctx->node_cap = 2ull * (N ? N : 1);
+
+    MZH_HIP(ctx->cd_begin.ensure(sizeof(double) * C));
+    MZH_HIP(ctx->cd_end.ensure(sizeof(double) * C));
+    MZH_HIP(ctx->aabbs.ensure(boxes_bytes));
+    MZH_HIP(ctx->srt_aabbs.ensure(boxes_bytes));
+    MZH_HIP(ctx->mcodes.ensure(sizeof(uint64_t) * NC));
+    MZH_HIP(ctx->srt_mcodes.ensure(sizeof(uint64_t) * NC));
+    MZH_HIP(ctx->keys_tmp.ensure(sizeof(uint64_t) * NC));
+    MZH_HIP(ctx->srt_idx.ensure(sizeof(uint32_t) * NC));
+    MZH_HIP(ctx->vals_tmp.ensure(sizeof(uint32_t) * NC));
+    MZH_HIP(ctx->n_eff.ensure(sizeof(uint32_t) * C));
+    MZH_HIP(ctx->gbox.ensure(sizeof(uint32_t) * 8u * C));
+    MZH_HIP(ctx->nodes.ensure(sizeof(mzh_bvh_node) * ctx->node_cap * C));
+    MZH_HIP(ctx->nolc.ensure(sizeof(uint32_t) * NC));
+    MZH_HIP(ctx->ps.ensure(sizeof(uint32_t) * NC));
+    MZH_HIP(ctx->tree_size.ensure(sizeof(uint32_t) * C));
+    MZH_HIP(ctx->otypes.ensure(sizeof(int32_t) * (N ? N : 1)));
+    MZH_HIP(ctx->cnt.ensure(sizeof(uint32_t) * NC));
+    MZH_HIP(ctx->off.ensure(sizeof(uint64_t) * NC));
+    MZH_HIP(ctx->step_total.ensure(sizeof(uint64_t) * C));
+    MZH_HIP(ctx->step_off.ensure(sizeof(uint64_t) * (C + 1)));
+    MZH_HIP(ctx->grand.ensure(sizeof(uint64_t)));
+    MZH_HIP(ctx->err.ensure(sizeof(DevError)));
+    MZH_HIP(ctx->conj_count.ensure(sizeof(unsigned long long)));
+    MZH_HIP(ctx->np_report.ensure(sizeof(unsigned long long) * MZH_NP_COUNT));
+    MZH_HIP(ctx->conj_cnt.ensure(sizeof(uint64_t) * C));
+    MZH_HIP(ctx->conj_start.ensure(sizeof(uint64_t) * (C + 1)));
+    MZH_HIP(ctx->conj_cursor.ensure(sizeof(uint32_t) * C));
+    MZH_HIP(ctx->conj_grand.ensure(sizeof(uint64_t)));
+
+    MZH_HIP(hipMemcpyAsync(ctx->cd_begin.p, ctx->h_cd_begin.data(), sizeof(double) * C, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemcpyAsync(ctx->cd_end.p, ctx->h_cd_end.data(), sizeof(double) * C, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemcpyAsync(ctx->otypes.p, ot.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemsetAsync(ctx->err.p, 0, sizeof(DevError), st));
+    MZH_HIP(hipMemsetAsync(ctx->np_report.p, 0, sizeof(unsigned long long) * MZH_NP_COUNT, st));
+
+    const PjView pj = pj_view(ctx);
+    StepGrid sg;
+    sg.cd_begin = ctx->cd_begin.as<const double>();
+    sg.cd_end = ctx->cd_end.as<const double>();
+    sg.n_steps = (uint32_t)C;
+    DevError *derr = ctx->err.as<DevError>();
+
+    MZH_HIP(hipEventRecord(ctx->ev[0], st));
+    // stage 1: AABBs
+    MZH_HIP(hipEventRecord(ctx->ev[6], st));
+    MZH_HIP(launch_aabbs(pj, sg, conj_thresh, mode, ctx->aabbs.as<float>(), ctx->gbox.as<uint32_t>(), derr, st));
+    MZH_HIP(hipEventRecord(ctx->ev[7], st));
+    MZH_HIP(launch_gbox_finalize(ctx->gbox.as<uint32_t>(), ctx->aabbs.as<float>(), N, (uint32_t)C, derr, st));
+    MZH_HIP(hipEventRecord(ctx->ev[1], st));
+    // stage 2: Morton codes, sort, gather
+    MZH_HIP(launch_morton(ctx->aabbs.as<float>(), ctx->mcodes.as<uint64_t>(), N, (uint32_t)C, st));
+    MZH_HIP(launch_sort(ctx->mcodes.as<uint64_t>(), ctx->aabbs.as<float>(), ctx->keys_tmp.as<uint64_t>(),
+                        ctx->vals_tmp.as<uint32_t>(), ctx->srt_mcodes.as<uint64_t>(), ctx->srt_idx.as<uint32_t>(),
+                        ctx->n_eff.as<uint32_t>(), N, (uint32_t)C, st));
+    MZH_HIP(launch_gather(ctx->aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->srt_aabbs.as<float>(), N,
+                          (uint32_t)C, st));
+    MZH_HIP(hipEventRecord(ctx->ev[2], st));
+    // stage 3: BVH
+    MZH_HIP(launch_bvh(ctx->srt_mcodes.as<uint64_t>(), ctx->srt_aabbs.as<float>(), ctx->n_eff.as<uint32_t>(),
+                       ctx->nodes.as<mzh_bvh_node>(), ctx->nolc.as<uint32_t>(), ctx->ps.as<uint32_t>(),
+                       ctx->tree_size.as<uint32_t>(), N, ctx->node_cap, (uint32_t)C, st));
+    MZH_HIP(hipEventRecord(ctx->ev[3], st));
+    ctx->n_pairs = 0;
+    ctx->n_conj = 0;
+    if (!skip_cd) {
+        // stage 4: broad phase
+        MZH_HIP(launch_bp_count(ctx->nodes.as<mzh_bvh_node>(), ctx->tree_size.as<uint32_t>(),
+                                ctx->srt_aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->n_eff.as<uint32_t>(),
+                                ctx->otypes.as<int32_t>(), ctx->cnt.as<uint32_t>(), N, ctx->node_cap, (uint32_t)C,
+                                st));
+        MZH_HIP(launch_scan_counts(ctx->cnt.as<uint32_t>(), ctx->off.as<uint64_t>(), ctx->step_total.as<uint64_t>(), N,
+                                   (uint32_t)C, st));
+        MZH_HIP(launch_scan_steps(ctx->step_total.as<uint64_t>(), ctx->step_off.as<uint64_t>(),
+                                  ctx->grand.as<uint64_t>(), (uint32_t)C, st));
+        uint64_t total = 0;
+        MZH_HIP(hipMemcpyAsync(&total, ctx->grand.p, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        MZH_HIP(hipStreamSynchronize(st));
+        ctx->n_pairs = total;
+        MZH_HIP(ctx->pairs.ensure(sizeof(mzh_aabb_collision) * (total ? total : 1)));
+        MZH_HIP(launch_bp_write(ctx->nodes.as<mzh_bvh_node>(), ctx->tree_size.as<uint32_t>(),
+                                ctx->srt_aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->n_eff.as<uint32_t>(),
+                                ctx->otypes.as<int32_t>(), ctx->off.as<uint64_t>(), ctx->step_off.as<uint64_t>(),
+                                ctx->pairs.as<mzh_aabb_collision>(), N, ctx->node_cap, (uint32_t)C, st));
+        MZH_HIP(hipEventRecord(ctx->ev[4], st));
+        // stage 5: narrow phase (re-run with a larger output buffer on overflow)
+        NarrowArgs na;
+        na.pj = pj;
+        na.sg = sg;
+        na.pairs = ctx->pairs.as<const mzh_aabb_collision>();
+        na.step_off = ctx->step_off.as<const uint64_t>();
+        na.n_pairs = total;
+        na.conj_thresh = conj_thresh;
+        const uint32_t op1 = ctx->order + 1;
+        const double pi = 3.141592653589793238462643383279502884;
+        for (int m = 0; m <= kMaxOrder; ++m) na.cheby[m] = 0;
+        for (uint32_t m = 0; m < op1; ++m) na.cheby[m] = (std::cos((2. * m + 1.) / (2. * op1) * pi) + 1.) / 2.;
+        na.np_report = ctx->np_report.as<unsigned long long>();
+        na.conj_count = ctx->conj_count.as<unsigned long long>();
+        na.err = derr;
+        if (ctx->conj_cap == 0) ctx->conj_cap = 1u << 16;
+        for (int attempt = 0; attempt < 3; ++attempt) {
+            MZH_HIP(ctx->conj_raw.ensure(sizeof(mzh_conj) * ctx->conj_cap));
+            MZH_HIP(ctx->conj_step.ensure(sizeof(uint32_t) * ctx->conj_cap));
+            MZH_HIP(hipMemsetAsync(ctx->conj_count.p, 0, sizeof(unsigned long long), st));
+            MZH_HIP(hipMemsetAsync(ctx->np_report.p, 0, sizeof(unsigned long long) * MZH_NP_COUNT, st));
+            na.conj = ctx->conj_raw.as<mzh_conj>();
+            na.conj_step = ctx->conj_step.as<uint32_t>();
+            na.conj_cap = ctx->conj_cap;
+            MZH_HIP(hipEventRecord(ctx->ev[8], st));
+            MZH_HIP(launch_narrow(na, st));
+            MZH_HIP(hipEventRecord(ctx->ev[9], st));
+            unsigned long long nc = 0;
+            MZH_HIP(hipMemcpyAsync(&nc, ctx->conj_count.p, sizeof(nc), hipMemcpyDeviceToHost, st));
+            MZH_HIP(hipStreamSynchronize(st));
+            ctx->n_conj = nc;
+            if (nc <= ctx->conj_cap) break;
+            ctx->conj_cap = nc + nc / 4 + 1024;
+        }
+        if (ctx->n_conj > ctx->conj_cap) throw StatusError{MZH_ERUNTIME, "conjunction buffer overflow"};
+        const uint64_t M = ctx->n_conj;
+        MZH_HIP(ctx->conj_sorted.ensure(sizeof(mzh_conj) * (M ? M : 1)));
+        MZH_HIP(ctx->conj_pos.ensure(sizeof(uint32_t) * (M ? M : 1)));
+        MZH_HIP(ctx->conj_inv.ensure(sizeof(uint32_t) * (M ? M : 1)));
+        MZH_HIP(launch_conj_hist(ctx->conj_step.as<uint32_t>(), M, ctx->conj_cnt.as<uint64_t>(), (uint32_t)C, st));
+        MZH_HIP(launch_scan_steps(ctx->conj_cnt.as<uint64_t>(), ctx->conj_start.as<uint64_t>(),
+                                  ctx->conj_grand.as<uint64_t>(), (uint32_t)C, st));
+        MZH_HIP(launch_conj_order(ctx->conj_raw.as<mzh_conj>(), ctx->conj_step.as<uint32_t>(), M,
+                                  ctx->conj_start.as<uint64_t>(), ctx->conj_cnt.as<uint64_t>(),
+                                  ctx->conj_cursor.as<uint32_t>(), ctx->conj_pos.as<uint32_t>(),
+                                  ctx->conj_inv.as<uint32_t>(), ctx->conj_sorted.as<mzh_conj>(), (uint32_t)C, st));
+    } else {
+        MZH_HIP(hipMemsetAsync(ctx->step_off.p, 0, sizeof(uint64_t) * (C + 1), st));
+        MZH_HIP(hipEventRecord(ctx->ev[4], st));
+        MZH_HIP(hipEventRecord(ctx->ev[8], st));
+        MZH_HIP(hipEventRecord(ctx->ev[9], st));
+    }
+    MZH_HIP(hipEventRecord(ctx->ev[5], st));
+    MZH_HIP(hipStreamSynchronize(st));
+
+    DevError herr;
+    MZH_HIP(hipMemcpy(&herr, ctx->err.p, sizeof(DevError), hipMemcpyDeviceToHost));
+    if (herr.code != kErrNone) throw StatusError{MZH_EINVAL, dev_error_message(herr)};
+
+    float ms[5];
+    for (int k = 0; k < 5; ++k) MZH_HIP(hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]));
+    for (int k = 0; k < 5; ++k) ctx->stage_s[k] = ms[k] * 1e-3;
+    float tot = 0;
+    MZH_HIP(hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[5]));
+    ctx->stage_s[5] = tot * 1e-3;
+    float ka = 0, kn = 0;
+    MZH_HIP(hipEventElapsedTime(&ka, ctx->ev[6], ctx->ev[7]));
+    MZH_HIP(hipEventElapsedTime(&kn, ctx->ev[8], ctx->ev[9]));
+    ctx->stage_s[6] = ka * 1e-3;
+    ctx->stage_s[7] = kn * 1e-3;
+    unsigned long long rep[MZH_NP_COUNT];
+    MZH_HIP(hipMemcpy(rep, ctx->np_report.p, sizeof(rep), hipMemcpyDeviceToHost));
+    for (int k = 0; k < MZH_NP_COUNT; ++k) ctx->h_np[k] = rep[k];
+
+    ctx->h_tree_size.resize(C);
+    MZH_HIP(hipMemcpy(ctx->h_tree_size.data(), ctx->tree_size.p, sizeof(uint32_t) * C, hipMemcpyDeviceToHost));
+    for (uint64_t k = 0; k < C; ++k) {
+        if (ctx->h_tree_size[k] > ctx->node_cap) throw StatusError{MZH_ERUNTIME, "BVH node capacity exceeded"};
+    }
+    ctx->h_step_off.resize(C + 1);
+    MZH_HIP(hipMemcpy(ctx->h_step_off.data(), ctx->step_off.p, sizeof(uint64_t) * (C + 1), hipMemcpyDeviceToHost));
+    ctx->screened = true;
+}
+
+void require_screened(const mzh_ctx *ctx)
+{
+    if (!ctx->screened) throw StatusError{MZH_EINVAL, "No screening results are available in this context"};
+}
+
+} // namespace
+
+extern "C" {
+
+int mzh_device_count(int *count)
+{
+    if (!count) return MZH_EINVAL;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return MZH_OK;
+}
+
+int mzh_global_error(char *buf, size_t len)
+{
+    if (!buf || len == 0) return MZH_EINVAL;
+    std::snprintf(buf, len, "%s", g_global_error.c_str());
+    return MZH_OK;
+}
+
+int mzh_create(int device, mzh_ctx **out)
+{
+    if (!out) return MZH_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        g_global_error = "No HIP device is available";
+        return MZH_ERUNTIME;
+    }
+    if (device < 0 || device >= n) {
+        g_global_error = "Invalid HIP device index " + std::to_string(device);
+        return MZH_EINVAL;
+    }
+    mzh_ctx *ctx = nullptr;
+    try {
+        ctx = new mzh_ctx();
+    } catch (...) {
+        g_global_error = "host memory allocation failed";
+        return MZH_ENOMEM;
+    }
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 10 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev[k]);
+    if (e != hipSuccess) {
+        g_global_error = std::string("HIP error while creating a context: ") + hipGetErrorString(e);
+        delete ctx;
+        return MZH_ERUNTIME;
+    }
+    *out = ctx;
+    return MZH_OK;
+}
+
+int mzh_destroy(mzh_ctx *ctx)
+{
+    if (!ctx) return MZH_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MZH_OK;
+}
+
+int mzh_last_error(const mzh_ctx *ctx, char *buf, size_t len)
+{
+    if (!ctx || !buf || len == 0) return MZH_EINVAL;
+    std::snprintf(buf, len, "%s", ctx->last_error.c_str());
+    return MZH_OK;
+}
+
+int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const double *traj, uint64_t traj_len,
+                           const double *time, uint64_t time_len, const uint64_t *traj_off, const uint64_t *nsteps,
+                           const uint64_t *time_off, double maxT)
+{
+    return guarded(ctx, [&] {
+        if (n_objs == 0) throw StatusError{MZH_EINVAL, "Cannot upload a polyjectory with zero objects"};
+        if (order < (uint32_t)kMinOrder || order > (uint32_t)kMaxOrder)
+            throw StatusError{MZH_EINVAL, "Polynomial order " + std::to_string(order)
+                                              + " is not supported (the supported range is [2, "
+                                              + std::to_string(kMaxOrder) + "])"};
+        if (!traj_off || !nsteps || !time_off || (!traj && traj_len) || (!time && time_len))
+            throw StatusError{MZH_EINVAL, "Null pointer passed to mzh_upload_polyjectory"};
+        if (!std::isfinite(maxT) || maxT <= 0) throw StatusError{MZH_EINVAL, "Invalid maxT"};
+        const uint64_t step_sz = 7ull * (order + 1);
+        std::vector<uint64_t> step0(n_objs);
+        uint64_t steps_total = traj_len / step_sz;
+        if (traj_len % step_sz) throw StatusError{MZH_EINVAL, "Trajectory data size is not a whole number of steps"};
+        for (uint32_t o = 0; o < n_objs; ++o) {
+            if (traj_off[o] % step_sz) throw StatusError{MZH_EINVAL, "Misaligned trajectory offset"};
+            step0[o] = traj_off[o] / step_sz;
+            if (step0[o] + nsteps[o] > steps_total) throw StatusError{MZH_EINVAL, "Trajectory offsets out of range"};
+            if (nsteps[o] && time_off[o] + nsteps[o] + 1 > time_len)
+                throw StatusError{MZH_EINVAL, "Time offsets out of range"};
+        }
+        hipStream_t st = ctx->stream;
+        ctx->has_pj = false;
+        ctx->screened = false;
+        MZH_HIP(ctx->traj.ensure(sizeof(double) * (traj_len ? traj_len : 1)));
+        MZH_HIP(ctx->time.ensure(sizeof(double) * (time_len ? time_len : 1)));
+        MZH_HIP(ctx->cf4.ensure(sizeof(double) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
+        MZH_HIP(ctx->step0.ensure(sizeof(uint64_t) * n_objs));
+        MZH_HIP(ctx->nsteps.ensure(sizeof(uint64_t) * n_objs));
+        MZH_HIP(ctx->time_off.ensure(sizeof(uint64_t) * n_objs));
+        if (traj_len) MZH_HIP(hipMemcpyAsync(ctx->traj.p, traj, sizeof(double) * traj_len, hipMemcpyHostToDevice, st));
+        if (time_len) MZH_HIP(hipMemcpyAsync(ctx->time.p, time, sizeof(double) * time_len, hipMemcpyHostToDevice, st));
+        MZH_HIP(hipMemcpyAsync(ctx->step0.p, step0.data(), sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
+        MZH_HIP(hipMemcpyAsync(ctx->nsteps.p, nsteps, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
+        MZH_HIP(hipMemcpyAsync(ctx->time_off.p, time_off, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
+        MZH_HIP(launch_transpose_cf4(ctx->traj.as<double>(), ctx->cf4.as<double>(), steps_total, order, st));
+        MZH_HIP(hipStreamSynchronize(st));
+        ctx->n_objs = n_objs;
+        ctx->order = order;
+        ctx->maxT = maxT;
+        ctx->n_steps_total = steps_total;
+        ctx->has_pj = true;
+    });
+}
+
+int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode)
+{
+    return guarded(ctx, [&] { run_screen(ctx, conj_thresh, conj_det_interval, otypes, mode); });
+}
+
+int mzh_get_n_cd_steps(const mzh_ctx *ctx, uint64_t *out)
+{
+    if (!ctx || !out) return MZH_EINVAL;
+    if (!ctx->screened) return fail(const_cast<mzh_ctx *>(ctx), MZH_EINVAL, "No screening results are available");
+    *out = ctx->C;
+    return MZH_OK;
+}
+
+int mzh_get_n_objs(const mzh_ctx *ctx, uint64_t *out)
+{
+    if (!ctx || !out) return MZH_EINVAL;
+    *out = ctx->n_objs;
+    return MZH_OK;
+}
+
+int mzh_copy_cd_end_times(mzh_ctx *ctx, double *dst)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        std::memcpy(dst, ctx->h_cd_end.data(), sizeof(double) * ctx->C);
+    });
+}
+
+namespace
+{
+int copy_dev(mzh_ctx *ctx, void *dst, const DevBuf &src, size_t bytes)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        if (!dst && bytes) throw StatusError{MZH_EINVAL, "Null destination"};
+        if (bytes) MZH_HIP(hipMemcpy(dst, src.p, bytes, hipMemcpyDeviceToHost));
+    });
+}
+} // namespace
+
+int mzh_copy_aabbs(mzh_ctx *ctx, float *dst)
+{
+    if (!ctx) return MZH_EINVAL;
+    return copy_dev(ctx, dst, ctx->aabbs, sizeof(float) * 8u * (size_t)(ctx->n_objs + 1) * ctx->C);
+}
+
+int mzh_copy_srt_aabbs(mzh_ctx *ctx, float *dst)
+{
+    if (!ctx) return MZH_EINVAL;
+    return copy_dev(ctx, dst, ctx->srt_aabbs, sizeof(float) * 8u * (size_t)(ctx->n_objs + 1) * ctx->C);
+}
+
+int mzh_copy_mcodes(mzh_ctx *ctx, uint64_t *dst)
+{
+    if (!ctx) return MZH_EINVAL;
+    return copy_dev(ctx, dst, ctx->mcodes, sizeof(uint64_t) * (size_t)ctx->n_objs * ctx->C);
+}
+
+int mzh_copy_srt_mcodes(mzh_ctx *ctx, uint64_t *dst)
+{
+    if (!ctx) return MZH_EINVAL;
+    return copy_dev(ctx, dst, ctx->srt_mcodes, sizeof(uint64_t) * (size_t)ctx->n_objs * ctx->C);
+}
+
+int mzh_copy_srt_idx(mzh_ctx *ctx, uint32_t *dst)
+{
+    if (!ctx) return MZH_EINVAL;
+    return copy_dev(ctx, dst, ctx->srt_idx, sizeof(uint32_t) * (size_t)ctx->n_objs * ctx->C);
+}
+
+int mzh_get_bvh_tree_size(mzh_ctx *ctx, uint64_t step, uint64_t *size)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        if (step >= ctx->C)
+            throw StatusError{MZH_ERANGE, "Cannot fetch the BVH tree for the conjunction timestep at index "
+                                              + std::to_string(step) + ": the total number of conjunction steps is only "
+                                              + std::to_string(ctx->C)};
+        *size = ctx->h_tree_size[step];
+    });
+}
+
+int mzh_copy_bvh_tree(mzh_ctx *ctx, uint64_t step, mzh_bvh_node *dst)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        if (step >= ctx->C)
+            throw StatusError{MZH_ERANGE, "Cannot fetch the BVH tree for the conjunction timestep at index "
+                                              + std::to_string(step) + ": the total number of conjunction steps is only "
+                                              + std::to_string(ctx->C)};
+        const size_t n = ctx->h_tree_size[step];
+        if (n)
+            MZH_HIP(hipMemcpy(dst, ctx->nodes.as<mzh_bvh_node>() + step * ctx->node_cap, sizeof(mzh_bvh_node) * n,
+                              hipMemcpyDeviceToHost));
+    });
+}
+
+int mzh_get_aabb_collisions_size(mzh_ctx *ctx, uint64_t step, uint64_t *size)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        if (step >= ctx->C)
+            throw StatusError{MZH_ERANGE,
+                              "Cannot fetch the list of AABB collisions for the conjunction timestep at index "
+                                  + std::to_string(step) + ": the total number of conjunction steps is only "
+                                  + std::to_string(ctx->C)};
+        *size = ctx->h_step_off[step + 1] - ctx->h_step_off[step];
+    });
+}
+
+int mzh_copy_aabb_collisions(mzh_ctx *ctx, uint64_t step, mzh_aabb_collision *dst)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        if (step >= ctx->C)
+            throw StatusError{MZH_ERANGE,
+                              "Cannot fetch the list of AABB collisions for the conjunction timestep at index "
+                                  + std::to_string(step) + ": the total number of conjunction steps is only "
+                                  + std::to_string(ctx->C)};
+        const uint64_t a = ctx->h_step_off[step], n = ctx->h_step_off[step + 1] - a;
+        if (n)
+            MZH_HIP(hipMemcpy(dst, ctx->pairs.as<mzh_aabb_collision>() + a, sizeof(mzh_aabb_collision) * n,
+                              hipMemcpyDeviceToHost));
+    });
+}
+
+int mzh_copy_bp_offsets(mzh_ctx *ctx, uint64_t *dst)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        for (uint64_t k = 0; k < ctx->C; ++k) {
+            dst[2 * k] = ctx->h_step_off[k];
+            dst[2 * k + 1] = ctx->h_step_off[k + 1] - ctx->h_step_off[k];
+        }
+    });
+}
+
+int mzh_get_n_conjunctions(mzh_ctx *ctx, uint64_t *out)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        *out = ctx->n_conj;
+    });
+}
+
+int mzh_copy_conjunctions(mzh_ctx *ctx, mzh_conj *dst)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        if (ctx->n_conj)
+            MZH_HIP(hipMemcpy(dst, ctx->conj_sorted.p, sizeof(mzh_conj) * ctx->n_conj, hipMemcpyDeviceToHost));
+    });
+}
+
+int mzh_get_np_report(mzh_ctx *ctx, uint64_t out[MZH_NP_COUNT])
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        for (int k = 0; k < MZH_NP_COUNT; ++k) out[k] = ctx->h_np[k];
+    });
+}
+
+int mzh_get_stage_times(mzh_ctx *ctx, double out[MZH_STAGE_COUNT])
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        for (int k = 0; k < MZH_STAGE_COUNT; ++k) out[k] = ctx->stage_s[k];
+    });
+}
+
+int mzh_device_view(mzh_ctx *ctx, int which, void **ptr, uint64_t *bytes)
+{
+    return guarded(ctx, [&] {
+        require_screened(ctx);
+        const size_t NC = (size_t)ctx->n_objs * ctx->C;
+        switch (which) {
+            case 0:
+                *ptr = ctx->aabbs.p;
+                *bytes = sizeof(float) * 8u * (size_t)(ctx->n_objs + 1) * ctx->C;
+                break;
+            case 1:
+                *ptr = ctx->srt_aabbs.p;
+                *bytes = sizeof(float) * 8u * (size_t)(ctx->n_objs + 1) * ctx->C;
+                break;
+            case 2:
+                *ptr = ctx->mcodes.p;
+                *bytes = sizeof(uint64_t) * NC;
+                break;
+            case 3:
+                *ptr = ctx->srt_mcodes.p;
+                *bytes = sizeof(uint64_t) * NC;
+                break;
+            case 4:
+                *ptr = ctx->srt_idx.p;
+                *bytes = sizeof(uint32_t) * NC;
+                break;
+            case 5:
+                *ptr = ctx->nodes.p;
+                *bytes = sizeof(mzh_bvh_node) * ctx->node_cap * ctx->C;
+                break;
+            case 6:
+                *ptr = ctx->pairs.p;
+                *bytes = sizeof(mzh_aabb_collision) * ctx->n_pairs;
+                break;
+            case 7:
+                *ptr = ctx->conj_sorted.p;
+                *bytes = sizeof(mzh_conj) * ctx->n_conj;
+                break;
+            default:
+                throw StatusError{MZH_EINVAL, "Invalid device view index " + std::to_string(which)};
+        }
+    });
+}
+
+int mzh_synchronize(mzh_ctx *ctx)
+{
+    return guarded(ctx, [&] { MZH_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+} // extern "C"

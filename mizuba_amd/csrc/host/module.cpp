@@ -1,0 +1,626 @@
+// pybind11 front end of mizuba_amd: the reference's Python API for the
+// conjunction-screening path (mizuba/expose_polyjectory.cpp:127-215,
+// mizuba/expose_conjunctions.cpp:62-203) in C++ over the mizuba_amd C ABI.
+//
+// Host code only: every device operation goes through include/mizuba_amd.h.
+#include <pybind11/eval.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/mizuba_amd.h"
+#include "synth.hpp"
+
+namespace py = pybind11;
+using namespace py::literals;
+
+namespace
+{
+
+std::string fmt_d(double v)
+{
+    if (std::isnan(v)) return "nan";
+    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+    char buf[64];
+    for (int prec = 1; prec <= 17; ++prec) {
+        std::snprintf(buf, sizeof(buf), "%.*g", prec, v);
+        if (std::strtod(buf, nullptr) == v) break;
+    }
+    return buf;
+}
+
+// Turn a C-ABI status into the exception type the reference throws.
+void check(int st, const mzh_ctx *ctx)
+{
+    if (st == MZH_OK) return;
+    char buf[1024] = {0};
+    if (ctx) {
+        mzh_last_error(ctx, buf, sizeof(buf));
+    } else {
+        mzh_global_error(buf, sizeof(buf));
+    }
+    const std::string msg(buf);
+    if (st == MZH_EINVAL) throw std::invalid_argument(msg);
+    if (st == MZH_ERANGE) throw std::out_of_range(msg);
+    if (st == MZH_ENOMEM) throw std::bad_alloc();
+    throw std::runtime_error(msg);
+}
+
+py::object numpy() { return py::module_::import("numpy"); }
+
+py::dtype bvh_node_dtype()
+{
+    return py::dtype::from_args(py::eval(
+        "[('begin', '<u4'), ('end', '<u4'), ('left', '<i4'), ('right', '<i4'), ('lb', '<f4', (4,)), ('ub', '<f4', (4,))]"));
+}
+
+py::dtype aabb_collision_dtype() { return py::dtype::from_args(py::eval("[('i', '<u4'), ('j', '<u4')]")); }
+
+py::dtype conj_dtype()
+{
+    return py::dtype::from_args(
+        py::eval("[('i', '<u4'), ('j', '<u4'), ('tca', '<f8'), ('dca', '<f8'), ('ri', '<f8', (3,)), "
+                 "('vi', '<f8', (3,)), ('rj', '<f8', (3,)), ('vj', '<f8', (3,))]"));
+}
+
+// Read-only numpy view of host memory owned by `base`.
+template <typename T>
+py::array ro_view(const py::object &base, const T *ptr, std::vector<py::ssize_t> shape)
+{
+    py::array a(py::dtype::of<T>(), shape, {}, ptr, base);
+    a.attr("setflags")("write"_a = false);
+    return a;
+}
+
+py::array ro_view_dtype(const py::object &base, const py::dtype &dt, const void *ptr, py::ssize_t n)
+{
+    py::array a(dt, {n}, {}, ptr, base);
+    a.attr("setflags")("write"_a = false);
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+// polyjectory: minimal in-memory equivalent of mizuba::polyjectory
+// ---------------------------------------------------------------------------
+
+struct PjData {
+    uint32_t n_objs = 0, order = 0;
+    double maxT = 0, epoch = 0, epoch2 = 0;
+    std::vector<double> traj, time;
+    std::vector<uint64_t> traj_off, nsteps, time_off, time_len;
+    std::vector<int32_t> status;
+};
+
+struct Polyjectory {
+    std::shared_ptr<PjData> d;
+};
+
+// Validation of polyjectory.cpp:375-447 and :559-599 with its messages.
+std::shared_ptr<PjData> build_pj(const std::vector<py::array_t<double>> &trajs,
+                                 const std::vector<py::array_t<double>> &times, const std::vector<int32_t> &status,
+                                 double epoch, double epoch2)
+{
+    const size_t n = trajs.size();
+    for (const auto &a : trajs) {
+        if (a.ndim() != 3)
+            throw std::invalid_argument("A trajectory array must have 3 dimensions, but instead "
+                                        + std::to_string(a.ndim()) + " dimension(s) were detected");
+        if (a.shape(2) != 7)
+            throw std::invalid_argument("A trajectory array must have a size of 7 in the third dimension, but instead "
+                                        "a size of "
+                                        + std::to_string(a.shape(2)) + " was detected");
+    }
+    for (const auto &a : times) {
+        if (a.ndim() != 1)
+            throw std::invalid_argument("A time array must have 1 dimension, but instead " + std::to_string(a.ndim())
+                                        + " dimension(s) were detected");
+    }
+    if (n == 0) throw std::invalid_argument("Cannot initialise a polyjectory object from an empty list of trajectories");
+    if (n != times.size())
+        throw std::invalid_argument("In the construction of a polyjectory, the number of objects deduced from the list "
+                                    "of trajectories ("
+                                    + std::to_string(n)
+                                    + ") is inconsistent with the number of objects deduced from the list of times ("
+                                    + std::to_string(times.size()) + ")");
+    if (n != status.size())
+        throw std::invalid_argument("In the construction of a polyjectory, the number of objects deduced from the list "
+                                    "of trajectories ("
+                                    + std::to_string(n)
+                                    + ") is inconsistent with the number of objects deduced from the status list ("
+                                    + std::to_string(status.size()) + ")");
+    if (!std::isfinite(epoch))
+        throw std::invalid_argument("The initial epoch of a polyjectory must be finite, but instead a value of "
+                                    + fmt_d(epoch) + " was provided");
+    if (!std::isfinite(epoch2))
+        throw std::invalid_argument("The second component of the initial epoch of a polyjectory must be finite, but "
+                                    "instead a value of "
+                                    + fmt_d(epoch2) + " was provided");
+    if (n > 0xffffffffull) throw std::overflow_error("Too many objects in a polyjectory");
+    auto d = std::make_shared<PjData>();
+    d->n_objs = (uint32_t)n;
+    d->epoch = epoch;
+    d->epoch2 = epoch2;
+    d->status = status;
+    uint64_t op1 = 0, tot_traj = 0, tot_time = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t cur = (uint64_t)trajs[i].shape(1);
+        if (i == 0) {
+            if (cur < 3)
+                throw std::invalid_argument(
+                    "The trajectory polynomial order for the first object is less than 2 - this is not allowed");
+            op1 = cur;
+        } else if (cur != op1) {
+            throw std::invalid_argument("The trajectory polynomial order for the object at index " + std::to_string(i)
+                                        + " is inconsistent with the polynomial order deduced from the first object ("
+                                        + std::to_string(op1 - 1) + ")");
+        }
+        tot_traj += (uint64_t)trajs[i].shape(0) * op1 * 7;
+    }
+    if (tot_traj == 0)
+        throw std::invalid_argument(
+            "All the trajectories in a polyjectory have a number of steps equal to zero: this is not allowed");
+    double maxT = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t ns = (uint64_t)trajs[i].shape(0), nt = (uint64_t)times[i].shape(0);
+        if (nt == 0) {
+            if (ns != 0)
+                throw std::invalid_argument("The trajectory of the object at index " + std::to_string(i)
+                                            + " has a nonzero number of steps but no associated time data");
+        } else if (nt - 1 != ns) {
+            throw std::invalid_argument("The number of steps for the trajectory of the object at index "
+                                        + std::to_string(i) + " is " + std::to_string(ns) + ", but the number of times is "
+                                        + std::to_string(nt)
+                                        + " (the number of times must be equal to the number of steps + 1)");
+        }
+        if (nt > 0) {
+            const double cT = times[i].at(nt - 1);
+            maxT = (cT > maxT) ? cT : maxT;
+        }
+        tot_time += nt;
+    }
+    d->order = (uint32_t)(op1 - 1);
+    d->traj.resize(tot_traj);
+    d->time.resize(tot_time);
+    d->traj_off.resize(n);
+    d->nsteps.resize(n);
+    d->time_off.resize(n);
+    d->time_len.resize(n);
+    uint64_t to = 0, mo = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const auto tr = trajs[i].unchecked<3>();
+        const uint64_t ns = (uint64_t)trajs[i].shape(0);
+        d->traj_off[i] = to;
+        d->nsteps[i] = ns;
+        for (uint64_t s = 0; s < ns; ++s)
+            for (uint64_t k = 0; k < op1; ++k)
+                for (uint64_t ch = 0; ch < 7; ++ch) {
+                    const double v = tr(s, k, ch);
+                    if (!std::isfinite(v))
+                        throw std::invalid_argument("A non-finite value was found in the trajectory at index "
+                                                    + std::to_string(i));
+                    d->traj[to++] = v;
+                }
+        const auto tm = times[i].unchecked<1>();
+        const uint64_t nt = (uint64_t)times[i].shape(0);
+        d->time_off[i] = mo;
+        d->time_len[i] = nt;
+        for (uint64_t j = 0; j < nt; ++j) {
+            const double v = tm(j);
+            if (!std::isfinite(v))
+                throw std::invalid_argument("A non-finite time coordinate was found for the object at index "
+                                            + std::to_string(i));
+            if (v < 0)
+                throw std::invalid_argument("A negative time coordinate was found for the object at index "
+                                            + std::to_string(i));
+            if (j > 0 && !(v > tm(j - 1)))
+                throw std::invalid_argument("The sequence of times for the object at index " + std::to_string(i)
+                                            + " is not monotonically increasing");
+            d->time[mo++] = v;
+        }
+    }
+    d->maxT = maxT;
+    return d;
+}
+
+std::shared_ptr<PjData> pj_from_synth(mzh_host::SynthOut &&s)
+{
+    auto d = std::make_shared<PjData>();
+    d->n_objs = (uint32_t)s.nsteps.size();
+    d->order = s.order;
+    d->maxT = s.maxT;
+    d->traj = std::move(s.traj);
+    d->time = std::move(s.time);
+    d->traj_off = std::move(s.traj_off);
+    d->nsteps = std::move(s.nsteps);
+    d->time_off = std::move(s.time_off);
+    d->time_len.resize(d->n_objs);
+    for (uint32_t i = 0; i < d->n_objs; ++i) d->time_len[i] = d->nsteps[i] ? d->nsteps[i] + 1 : 0;
+    d->status.assign(d->n_objs, 0);
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// device context wrapper
+// ---------------------------------------------------------------------------
+
+struct Ctx {
+    mzh_ctx *ctx = nullptr;
+    std::shared_ptr<PjData> pj;
+    explicit Ctx(int device)
+    {
+        const int st = mzh_create(device, &ctx);
+        check(st, nullptr);
+    }
+    ~Ctx()
+    {
+        if (ctx) mzh_destroy(ctx);
+    }
+    Ctx(const Ctx &) = delete;
+    Ctx &operator=(const Ctx &) = delete;
+
+    void upload(const std::shared_ptr<PjData> &d)
+    {
+        int st;
+        {
+            py::gil_scoped_release rel;
+            st = mzh_upload_polyjectory(ctx, d->n_objs, d->order, d->traj.data(), d->traj.size(), d->time.data(),
+                                        d->time.size(), d->traj_off.data(), d->nsteps.data(), d->time_off.data(),
+                                        d->maxT);
+        }
+        check(st, ctx);
+        pj = d;
+    }
+
+    void screen(double thresh, double interval, const std::optional<std::vector<int32_t>> &otypes, int mode)
+    {
+        if (otypes && otypes->size() != pj->n_objs)
+            throw std::invalid_argument("Invalid array of object types passed to the constructor of a conjunctions "
+                                        "objects: the expected size is "
+                                        + std::to_string(pj->n_objs) + ", but the actual size is "
+                                        + std::to_string(otypes->size()) + " instead");
+        int st;
+        {
+            py::gil_scoped_release rel;
+            st = mzh_screen(ctx, thresh, interval, otypes ? otypes->data() : nullptr, mode);
+        }
+        check(st, ctx);
+    }
+};
+
+int parse_mode(const std::string &m)
+{
+    if (m == "fp64") return MZH_MODE_FP64;
+    if (m == "mixed") return MZH_MODE_MIXED;
+    throw std::invalid_argument("Invalid screening mode '" + m + "': it must be 'fp64' or 'mixed'");
+}
+
+// ---------------------------------------------------------------------------
+// conjunctions
+// ---------------------------------------------------------------------------
+
+struct ConjImpl {
+    std::unique_ptr<Ctx> ctx;
+    std::shared_ptr<PjData> pj;
+    double conj_thresh = 0, conj_det_interval = 0;
+    int mode = MZH_MODE_FP64;
+    uint64_t C = 0, N = 0;
+    std::vector<int32_t> otypes;
+    std::vector<double> cd_end_times;
+    std::mutex m;
+    std::unique_ptr<std::vector<float>> aabbs, srt_aabbs;
+    std::unique_ptr<std::vector<uint64_t>> mcodes, srt_mcodes;
+    std::unique_ptr<std::vector<uint32_t>> srt_idx;
+    std::map<uint64_t, std::vector<mzh_bvh_node>> trees;
+    std::map<uint64_t, std::vector<mzh_aabb_collision>> bps;
+    std::unique_ptr<std::vector<mzh_conj>> conjs;
+};
+
+struct Conjunctions {
+    std::shared_ptr<ConjImpl> impl;
+};
+
+template <typename T, typename F>
+const std::vector<T> &lazy(ConjImpl &im, std::unique_ptr<std::vector<T>> &slot, size_t n, F &&fetch)
+{
+    std::lock_guard<std::mutex> lk(im.m);
+    if (!slot) {
+        auto v = std::make_unique<std::vector<T>>(n ? n : 1);
+        int st;
+        {
+            py::gil_scoped_release rel;
+            st = fetch(v->data());
+        }
+        check(st, im.ctx->ctx);
+        slot = std::move(v);
+    }
+    return *slot;
+}
+
+Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double conj_det_interval,
+                               std::optional<std::vector<int32_t>> otypes, std::optional<py::object> data_dir,
+                               std::optional<py::object> tmpdir, const std::string &mode, int device)
+{
+    if (data_dir && tmpdir && !data_dir->is_none() && !tmpdir->is_none())
+        throw std::invalid_argument(
+            "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time");
+    const int m = parse_mode(mode);
+    auto im = std::make_shared<ConjImpl>();
+    im->ctx = std::make_unique<Ctx>(device);
+    im->ctx->upload(pj.d);
+    im->ctx->screen(conj_thresh, conj_det_interval, otypes, m);
+    im->pj = pj.d;
+    im->conj_thresh = conj_thresh;
+    im->conj_det_interval = conj_det_interval;
+    im->mode = m;
+    im->N = pj.d->n_objs;
+    check(mzh_get_n_cd_steps(im->ctx->ctx, &im->C), im->ctx->ctx);
+    im->otypes = otypes ? *otypes : std::vector<int32_t>(pj.d->n_objs, 1);
+    im->cd_end_times.resize(im->C);
+    check(mzh_copy_cd_end_times(im->ctx->ctx, im->cd_end_times.data()), im->ctx->ctx);
+    return Conjunctions{im};
+}
+
+} // namespace
+
+PYBIND11_MODULE(_core, m)
+{
+    m.doc() = "mizuba_amd: MI355X-native conjunction screening over polyjectories";
+
+    // ---- polyjectory ---------------------------------------------------
+    py::class_<Polyjectory> pt(m, "polyjectory", py::dynamic_attr());
+    pt.def(py::init([](const py::iterable &trajs_, const py::iterable &times_, const py::iterable &status_, double epoch,
+                       double epoch2, std::optional<py::object> data_dir, bool persist,
+                       std::optional<py::object> tmpdir) {
+               (void)persist;
+               if (data_dir && tmpdir && !data_dir->is_none() && !tmpdir->is_none())
+                   throw std::invalid_argument(
+                       "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time");
+               auto np = numpy();
+               std::vector<py::array_t<double>> trajs, times;
+               for (auto o : trajs_)
+                   trajs.push_back(np.attr("ascontiguousarray")(py::reinterpret_borrow<py::object>(o), "dtype"_a = "f8")
+                                       .cast<py::array_t<double>>());
+               for (auto o : times_)
+                   times.push_back(np.attr("ascontiguousarray")(py::reinterpret_borrow<py::object>(o), "dtype"_a = "f8")
+                                       .cast<py::array_t<double>>());
+               py::array st_arr = np.attr("ascontiguousarray")(py::reinterpret_borrow<py::object>(status_),
+                                                               "dtype"_a = "i4");
+               if (st_arr.ndim() != 1)
+                   throw std::invalid_argument("A status array must have 1 dimension, but instead "
+                                               + std::to_string(st_arr.ndim()) + " dimension(s) were detected");
+               std::vector<int32_t> status(st_arr.shape(0));
+               std::memcpy(status.data(), st_arr.data(), sizeof(int32_t) * status.size());
+               return Polyjectory{build_pj(trajs, times, status, epoch, epoch2)};
+           }),
+           "trajs"_a, "times"_a, "status"_a, "epoch"_a = 0., "epoch2"_a = 0., "data_dir"_a = py::none(),
+           "persist"_a = false, "tmpdir"_a = py::none());
+    pt.def_property_readonly("n_objs", [](const Polyjectory &p) { return p.d->n_objs; });
+    pt.def_property_readonly("maxT", [](const Polyjectory &p) { return p.d->maxT; });
+    pt.def_property_readonly("epoch", [](const Polyjectory &p) { return py::make_tuple(p.d->epoch, p.d->epoch2); });
+    pt.def_property_readonly("poly_order", [](const Polyjectory &p) { return p.d->order; });
+    pt.def_property_readonly("status", [](const py::object &self) {
+        const auto &p = self.cast<const Polyjectory &>();
+        return ro_view(self, p.d->status.data(), {(py::ssize_t)p.d->n_objs});
+    });
+    pt.def("__getitem__", [](const py::object &self, py::ssize_t i) {
+        const auto &p = self.cast<const Polyjectory &>();
+        if (i < 0 || (uint64_t)i >= p.d->n_objs)
+            throw std::out_of_range("Invalid object index " + std::to_string(i)
+                                    + " specified - the total number of objects in the polyjectory is only "
+                                    + std::to_string(p.d->n_objs));
+        const uint64_t ns = p.d->nsteps[i];
+        const py::ssize_t op1 = p.d->order + 1;
+        auto tr = ro_view(self, p.d->traj.data() + p.d->traj_off[i], {(py::ssize_t)ns, op1, 7});
+        auto tm = ro_view(self, p.d->time.data() + p.d->time_off[i], {(py::ssize_t)(ns + (ns != 0 ? 1 : 0))});
+        return py::make_tuple(tr, tm, p.d->status[i]);
+    });
+    pt.def("_flat", [](const py::object &self) {
+        // flattened arrays (traj, time, traj_off, nsteps, time_off), for tests and tools
+        const auto &p = self.cast<const Polyjectory &>();
+        return py::make_tuple(ro_view(self, p.d->traj.data(), {(py::ssize_t)p.d->traj.size()}),
+                              ro_view(self, p.d->time.data(), {(py::ssize_t)p.d->time.size()}),
+                              ro_view(self, p.d->traj_off.data(), {(py::ssize_t)p.d->n_objs}),
+                              ro_view(self, p.d->nsteps.data(), {(py::ssize_t)p.d->n_objs}),
+                              ro_view(self, p.d->time_off.data(), {(py::ssize_t)p.d->n_objs}));
+    });
+
+    m.def(
+        "make_synthetic_polyjectory",
+        [](uint32_t n_objs, double days, uint64_t seed, double frac_high, uint32_t order, uint32_t n_shell) {
+            if (n_objs == 0) throw std::invalid_argument("n_objs must be positive");
+            if (!(days > 0) || !std::isfinite(days)) throw std::invalid_argument("days must be finite and positive");
+            if (order < 2 || order > 20) throw std::invalid_argument("order must be in [2, 20]");
+            mzh_host::SynthParams p;
+            p.n_objs = n_objs;
+            p.days = days;
+            p.seed = seed;
+            p.frac_high = frac_high;
+            p.order = order;
+            p.n_shell = n_shell;
+            mzh_host::SynthOut out;
+            {
+                py::gil_scoped_release rel;
+                out = mzh_host::make_synthetic(p);
+            }
+            return Polyjectory{pj_from_synth(std::move(out))};
+        },
+        "n_objs"_a, "days"_a, "seed"_a = 42, "frac_high"_a = 0.2, "order"_a = 8, "n_shell"_a = 0);
+
+    // ---- device context for repeated screening of one polyjectory ------
+    py::class_<Ctx>(m, "Screener")
+        .def(py::init<int>(), "device"_a = 0)
+        .def("upload", [](Ctx &c, const Polyjectory &p) { c.upload(p.d); })
+        .def(
+            "screen",
+            [](Ctx &c, double thresh, double interval, std::optional<std::vector<int32_t>> otypes,
+               const std::string &mode) {
+                if (!c.pj) throw std::invalid_argument("No polyjectory has been uploaded");
+                c.screen(thresh, interval, otypes, parse_mode(mode));
+            },
+            "conj_thresh"_a, "conj_det_interval"_a, "otypes"_a = py::none(), "mode"_a = "fp64")
+        .def("stage_times",
+             [](Ctx &c) {
+                 double t[MZH_STAGE_COUNT];
+                 check(mzh_get_stage_times(c.ctx, t), c.ctx);
+                 py::dict d;
+                 const char *names[] = {"aabbs", "morton", "bvh", "broad", "narrow", "total", "aabb_kernel", "narrow_kernel"};
+                 for (int k = 0; k < MZH_STAGE_COUNT; ++k) d[names[k]] = t[k];
+                 return d;
+             })
+        .def("np_report",
+             [](Ctx &c) {
+                 uint64_t r[MZH_NP_COUNT];
+                 check(mzh_get_np_report(c.ctx, r), c.ctx);
+                 py::dict d;
+                 const char *names[] = {"n_tot_conj_candidates", "n_dist2_check",      "n_poly_roots",
+                                        "n_fex_check",           "n_poly_no_roots",    "n_tot_dist_minima",
+                                        "n_tot_discarded_dist_minima"};
+                 for (int k = 0; k < MZH_NP_COUNT; ++k) d[names[k]] = r[k];
+                 return d;
+             })
+        .def("n_cd_steps",
+             [](Ctx &c) {
+                 uint64_t n;
+                 check(mzh_get_n_cd_steps(c.ctx, &n), c.ctx);
+                 return n;
+             })
+        .def("n_pairs",
+             [](Ctx &c) {
+                 uint64_t C;
+                 check(mzh_get_n_cd_steps(c.ctx, &C), c.ctx);
+                 std::vector<uint64_t> off(2 * C);
+                 check(mzh_copy_bp_offsets(c.ctx, off.data()), c.ctx);
+                 return C ? off[2 * (C - 1)] + off[2 * (C - 1) + 1] : 0;
+             })
+        .def("conjunctions", [](Ctx &c) {
+            uint64_t n;
+            check(mzh_get_n_conjunctions(c.ctx, &n), c.ctx);
+            py::array out(conj_dtype(), std::vector<py::ssize_t>{(py::ssize_t)n});
+            check(mzh_copy_conjunctions(c.ctx, static_cast<mzh_conj *>(out.mutable_data())), c.ctx);
+            return out;
+        });
+
+    m.def("device_count", []() {
+        int n = 0;
+        mzh_device_count(&n);
+        return n;
+    });
+
+    // ---- conjunctions -------------------------------------------------
+    py::class_<Conjunctions> cj(m, "conjunctions", py::dynamic_attr());
+    cj.def(py::init(&make_conjunctions), "pj"_a, "conj_thresh"_a, "conj_det_interval"_a, "otypes"_a = py::none(),
+           "data_dir"_a = py::none(), "tmpdir"_a = py::none(), "mode"_a = "fp64", "device"_a = 0);
+    cj.def_property_readonly("n_cd_steps", [](const Conjunctions &c) { return c.impl->C; });
+    cj.def_property_readonly("mode", [](const Conjunctions &c) {
+        return std::string(c.impl->mode == MZH_MODE_FP64 ? "fp64" : "mixed");
+    });
+    cj.def_property_readonly("aabbs", [](const py::object &self) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        const auto &v = lazy<float>(im, im.aabbs, im.C * (im.N + 1) * 8,
+                                    [&](float *p) { return mzh_copy_aabbs(im.ctx->ctx, p); });
+        return ro_view(self, v.data(), {(py::ssize_t)im.C, (py::ssize_t)im.N + 1, 2, 4});
+    });
+    cj.def_property_readonly("srt_aabbs", [](const py::object &self) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        const auto &v = lazy<float>(im, im.srt_aabbs, im.C * (im.N + 1) * 8,
+                                    [&](float *p) { return mzh_copy_srt_aabbs(im.ctx->ctx, p); });
+        return ro_view(self, v.data(), {(py::ssize_t)im.C, (py::ssize_t)im.N + 1, 2, 4});
+    });
+    cj.def_property_readonly("mcodes", [](const py::object &self) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        const auto &v = lazy<uint64_t>(im, im.mcodes, im.C * im.N,
+                                       [&](uint64_t *p) { return mzh_copy_mcodes(im.ctx->ctx, p); });
+        return ro_view(self, v.data(), {(py::ssize_t)im.C, (py::ssize_t)im.N});
+    });
+    cj.def_property_readonly("srt_mcodes", [](const py::object &self) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        const auto &v = lazy<uint64_t>(im, im.srt_mcodes, im.C * im.N,
+                                       [&](uint64_t *p) { return mzh_copy_srt_mcodes(im.ctx->ctx, p); });
+        return ro_view(self, v.data(), {(py::ssize_t)im.C, (py::ssize_t)im.N});
+    });
+    cj.def_property_readonly("srt_idx", [](const py::object &self) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        const auto &v = lazy<uint32_t>(im, im.srt_idx, im.C * im.N,
+                                       [&](uint32_t *p) { return mzh_copy_srt_idx(im.ctx->ctx, p); });
+        return ro_view(self, v.data(), {(py::ssize_t)im.C, (py::ssize_t)im.N});
+    });
+    cj.def_property_readonly("cd_end_times", [](const py::object &self) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        return ro_view(self, im.cd_end_times.data(), {(py::ssize_t)im.C});
+    });
+    cj.def("get_bvh_tree", [](const py::object &self, uint64_t i) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        std::lock_guard<std::mutex> lk(im.m);
+        auto it = im.trees.find(i);
+        if (it == im.trees.end()) {
+            uint64_t n = 0;
+            check(mzh_get_bvh_tree_size(im.ctx->ctx, i, &n), im.ctx->ctx);
+            std::vector<mzh_bvh_node> v(n ? n : 1);
+            check(mzh_copy_bvh_tree(im.ctx->ctx, i, v.data()), im.ctx->ctx);
+            v.resize(n);
+            it = im.trees.emplace(i, std::move(v)).first;
+        }
+        return ro_view_dtype(self, bvh_node_dtype(), it->second.data(), (py::ssize_t)it->second.size());
+    });
+    cj.def("get_aabb_collisions", [](const py::object &self, uint64_t i) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        std::lock_guard<std::mutex> lk(im.m);
+        auto it = im.bps.find(i);
+        if (it == im.bps.end()) {
+            uint64_t n = 0;
+            check(mzh_get_aabb_collisions_size(im.ctx->ctx, i, &n), im.ctx->ctx);
+            std::vector<mzh_aabb_collision> v(n ? n : 1);
+            check(mzh_copy_aabb_collisions(im.ctx->ctx, i, v.data()), im.ctx->ctx);
+            v.resize(n);
+            it = im.bps.emplace(i, std::move(v)).first;
+        }
+        return ro_view_dtype(self, aabb_collision_dtype(), it->second.data(), (py::ssize_t)it->second.size());
+    });
+    cj.def_property_readonly("conjunctions", [](const py::object &self) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        uint64_t n = 0;
+        check(mzh_get_n_conjunctions(im.ctx->ctx, &n), im.ctx->ctx);
+        const auto &v = lazy<mzh_conj>(im, im.conjs, n, [&](mzh_conj *p) { return mzh_copy_conjunctions(im.ctx->ctx, p); });
+        return ro_view_dtype(self, conj_dtype(), v.data(), (py::ssize_t)n);
+    });
+    cj.def_property_readonly("otypes", [](const py::object &self) {
+        auto &im = *self.cast<const Conjunctions &>().impl;
+        return ro_view(self, im.otypes.data(), {(py::ssize_t)im.N});
+    });
+    cj.def_property_readonly("conj_thresh", [](const Conjunctions &c) { return c.impl->conj_thresh; });
+    cj.def_property_readonly("conj_det_interval", [](const Conjunctions &c) { return c.impl->conj_det_interval; });
+    cj.def("hint_release", [](const Conjunctions &) {});
+    cj.def_property_readonly("np_report", [](const Conjunctions &c) {
+        uint64_t r[MZH_NP_COUNT];
+        check(mzh_get_np_report(c.impl->ctx->ctx, r), c.impl->ctx->ctx);
+        py::dict d;
+        const char *names[] = {"n_tot_conj_candidates", "n_dist2_check",     "n_poly_roots",
+                               "n_fex_check",           "n_poly_no_roots",   "n_tot_dist_minima",
+                               "n_tot_discarded_dist_minima"};
+        for (int k = 0; k < MZH_NP_COUNT; ++k) d[names[k]] = r[k];
+        return d;
+    });
+    cj.def_property_readonly("stage_times", [](const Conjunctions &c) {
+        double t[MZH_STAGE_COUNT];
+        check(mzh_get_stage_times(c.impl->ctx->ctx, t), c.impl->ctx->ctx);
+        py::dict d;
+        const char *names[] = {"aabbs", "morton", "bvh", "broad", "narrow", "total", "aabb_kernel", "narrow_kernel"};
+        for (int k = 0; k < MZH_STAGE_COUNT; ++k) d[names[k]] = t[k];
+        return d;
+    });
+    cj.def_property_readonly_static("bvh_node", [](const py::object &) { return bvh_node_dtype(); });
+    cj.def_property_readonly_static("aabb_collision", [](const py::object &) { return aabb_collision_dtype(); });
+    cj.def_property_readonly_static("conj", [](const py::object &) { return conj_dtype(); });
+}

@@ -1,0 +1,193 @@
+// Device-side helpers shared by the gfx950 kernels of mizuba_amd.
+//
+// All kernels are compiled with -ffp-contract=off: every FP64/FP32 multiply
+// and add is rounded separately, so the FP64 screen reproduces the CPU
+// restatement of the reference bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/mizuba_amd.h"
+
+namespace mzh
+{
+
+constexpr int kMaxOrder = 20;  // highest polynomial order with a compiled kernel
+constexpr int kMinOrder = 2;   // polyjectory.cpp:434-438 rejects order < 2
+constexpr uint32_t kBvhMaxLevels = 72;
+
+// Binomial coefficients C(n, k) for n, k <= kMaxOrder, exact in double.
+struct BinomTable {
+    double v[kMaxOrder + 1][kMaxOrder + 1];
+};
+
+constexpr BinomTable make_binom_table()
+{
+    BinomTable t{};
+    for (int n = 0; n <= kMaxOrder; ++n) {
+        for (int k = 0; k <= kMaxOrder; ++k) {
+            if (k > n) {
+                t.v[n][k] = 0.0;
+            } else {
+                unsigned long long r = 1;
+                const int kk = (k > n - k) ? n - k : k;
+                for (int i = 1; i <= kk; ++i) r = r * (unsigned long long)(n - kk + i) / (unsigned long long)i;
+                t.v[n][k] = (double)r;
+            }
+        }
+    }
+    return t;
+}
+
+// Per-problem parameters passed by value to the kernels.
+struct StepGrid {
+    const double *cd_begin; // [C]
+    const double *cd_end;   // [C]
+    uint32_t n_steps;       // C
+};
+
+// Polyjectory as laid out in HBM.
+struct PjView {
+    const double *traj;      // AoS [S][op1][7]
+    const double *time;      // per object nsteps + 1 values
+    const double *cf4;       // step-major transposed [S][4][op1], channels x y z r
+    const uint64_t *step0;   // global index of the object's first step
+    const uint64_t *nsteps;  // steps per object
+    const uint64_t *time_off;
+    uint32_t n_objs;
+    uint32_t order;
+};
+
+// Error word written by kernels; the host turns it into the reference's
+// exception message. Only the first error (lowest code wins ties) is kept.
+struct DevError {
+    int32_t code;      // 0 = none
+    uint32_t i, j;     // object indices involved
+    double value;      // offending value
+};
+
+enum DevErrCode : int32_t {
+    kErrNone = 0,
+    kErrAabbLb = 1,        // non-finite lower bound in an AABB
+    kErrAabbUb = 2,        // non-finite upper bound in an AABB
+    kErrGlobalBox = 3,     // non-finite global box size
+    kErrNpInterval = 10,   // invalid root-finding interval
+    kErrNpDist2 = 11,      // non-finite dist2 enclosure
+    kErrNpRoot = 12,       // non-finite root
+    kErrNpRootDer = 13,    // non-finite derivative at a root
+    kErrNpIsolation = 14,  // isolation limits exceeded (value = wlist size, i/j = sizes)
+    kErrNpIters = 15,      // TOMS 748 iteration limit
+    kErrNpErrno = 16,      // TOMS 748 domain error
+    kErrNpConjDist2 = 17,  // non-finite conjunction distance square
+    kErrNpBndDerUb = 18,
+    kErrNpBndDist2Ub = 19,
+    kErrNpBndDerLb = 20,
+    kErrNpBndDist2Lb = 21,
+    kErrNpWorkList = 22,   // device work-list capacity exceeded
+};
+
+__device__ inline void report_error(DevError *e, int32_t code, uint32_t i, uint32_t j, double v)
+{
+    if (atomicCAS(&e->code, 0, code) == 0) {
+        e->i = i;
+        e->j = j;
+        e->value = v;
+    }
+}
+
+// ---- IEEE helpers --------------------------------------------------------
+
+// nextafter(x, -inf) / nextafter(x, +inf) for float, exact bit stepping.
+__device__ inline float next_down_f(float x)
+{
+    if (x != x || x == -__builtin_inff()) return x;
+    if (x == 0.0f) return __uint_as_float(0x80000001u);
+    uint32_t u = __float_as_uint(x);
+    u = (u >> 31) ? u + 1u : u - 1u;
+    return __uint_as_float(u);
+}
+
+__device__ inline float next_up_f(float x)
+{
+    if (x != x || x == __builtin_inff()) return x;
+    if (x == 0.0f) return __uint_as_float(0x00000001u);
+    uint32_t u = __float_as_uint(x);
+    u = (u >> 31) ? u - 1u : u + 1u;
+    return __uint_as_float(u);
+}
+
+// nextafter(x, toward) for double (finite x).
+__device__ inline double next_toward_d(double x, double toward)
+{
+    if (x != x || toward != toward) return x + toward;
+    if (x == toward) return toward;
+    if (x == 0.0) {
+        const double tiny = __longlong_as_double(1ll);
+        return toward > 0 ? tiny : -tiny;
+    }
+    long long u = __double_as_longlong(x);
+    if ((x < toward) == (x > 0)) {
+        ++u;
+    } else {
+        --u;
+    }
+    return __longlong_as_double(u);
+}
+
+__device__ inline bool is_finite_f(float x) { return __builtin_isfinite(x); }
+__device__ inline bool is_finite_d(double x) { return __builtin_isfinite(x); }
+
+// Order-preserving float <-> uint32 map for atomic min/max on floats
+// (replaces the CAS loops of detail/atomic_minmax.hpp:29-57).
+__device__ inline uint32_t f2ord(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ inline float ord2f(uint32_t u)
+{
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// Spread the low 16 bits of x so that bit b lands on bit 4b.
+__device__ inline uint64_t spread4(uint64_t x)
+{
+    x &= 0xffffull;
+    x = (x | (x << 24)) & 0x000000ff000000ffull;
+    x = (x | (x << 12)) & 0x000f000f000f000full;
+    x = (x | (x << 6)) & 0x0303030303030303ull;
+    x = (x | (x << 3)) & 0x1111111111111111ull;
+    return x;
+}
+
+// ---- wave64 primitives ----------------------------------------------------
+
+__device__ inline uint64_t lanemask_lt()
+{
+    const uint32_t lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64u - lane));
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v)
+{
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= (uint32_t)off) v += t;
+    }
+    return v;
+}
+
+__device__ inline uint64_t wave_reduce_add_u64(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+} // namespace mzh

@@ -1,0 +1,80 @@
+// Host-side launchers of the gfx950 kernels (one declaration per stage).
+// Every launcher enqueues on `stream` and returns the hipError_t of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mzh_device.hpp"
+
+namespace mzh
+{
+
+// Upload-time layout transform: AoS [S][op1][7] -> step-major [S][4][op1]
+// holding channels x, y, z, r.
+hipError_t launch_transpose_cf4(const double *traj, double *cf4, uint64_t n_steps_total, uint32_t order,
+                                hipStream_t stream);
+
+// Stage 1: per-(step, object) AABBs plus the per-step global box.
+// mode: MZH_MODE_FP64 or MZH_MODE_MIXED.
+hipError_t launch_aabbs(const PjView &pj, const StepGrid &sg, double conj_thresh, int mode, float *aabbs,
+                        uint32_t *gbox_ord, DevError *err, hipStream_t stream);
+hipError_t launch_gbox_finalize(const uint32_t *gbox_ord, float *aabbs, uint32_t n_objs, uint32_t n_steps,
+                                DevError *err, hipStream_t stream);
+
+// Stage 2: Morton codes, segmented stable radix sort, gather.
+hipError_t launch_morton(const float *aabbs, uint64_t *mcodes, uint32_t n_objs, uint32_t n_steps, hipStream_t stream);
+hipError_t launch_sort(const uint64_t *mcodes, const float *aabbs, uint64_t *keys_tmp, uint32_t *vals_tmp,
+                       uint64_t *srt_mcodes, uint32_t *srt_idx, uint32_t *n_eff, uint32_t n_objs, uint32_t n_steps,
+                       hipStream_t stream);
+hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt_aabbs, uint32_t n_objs,
+                         uint32_t n_steps, hipStream_t stream);
+
+// Stage 3: BVH, one workgroup per step. nodes has node_cap entries per step.
+hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, mzh_bvh_node *nodes,
+                      uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap,
+                      uint32_t n_steps, hipStream_t stream);
+
+// Stage 4: broad phase (count, per-step scan, step scan, write + per-object sort).
+hipError_t launch_bp_count(const mzh_bvh_node *nodes, const uint32_t *tree_size, const float *srt_aabbs,
+                           const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, uint32_t *cnt,
+                           uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream);
+hipError_t launch_scan_counts(const uint32_t *cnt, uint64_t *off, uint64_t *step_total, uint32_t n_objs,
+                              uint32_t n_steps, hipStream_t stream);
+hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uint64_t *grand_total, uint32_t n_steps,
+                             hipStream_t stream);
+hipError_t launch_bp_write(const mzh_bvh_node *nodes, const uint32_t *tree_size, const float *srt_aabbs,
+                           const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, const uint64_t *off,
+                           const uint64_t *step_off, mzh_aabb_collision *pairs, uint32_t n_objs, uint64_t node_cap,
+                           uint32_t n_steps, hipStream_t stream);
+
+// Stage 5: narrow phase over all pairs of all steps.
+struct NarrowArgs {
+    PjView pj;
+    StepGrid sg;
+    const mzh_aabb_collision *pairs;
+    const uint64_t *step_off;  // [C + 1], exclusive prefix of the per-step pair counts
+    uint64_t n_pairs;
+    double conj_thresh;
+    double cheby[kMaxOrder + 1];  // (cos((2i+1)pi/(2 op1)) + 1) / 2, i = 0..order
+    mzh_conj *conj;               // output records (capacity conj_cap)
+    uint32_t *conj_step;          // step index of each record
+    uint64_t conj_cap;
+    unsigned long long *conj_count;
+    unsigned long long *np_report;  // [MZH_NP_COUNT]
+    DevError *err;
+};
+hipError_t launch_narrow(const NarrowArgs &a, hipStream_t stream);
+template <int ORDER>
+hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream);
+
+// Group conjunction records by step and sort each step by (tca, lexicographic):
+// launch_conj_hist counts records per step into cnt[C]; the caller scans cnt
+// into start[C + 1] (launch_scan_steps); launch_conj_order scatters and ranks.
+hipError_t launch_conj_hist(const uint32_t *in_step, uint64_t n, uint64_t *cnt, uint32_t n_steps, hipStream_t stream);
+hipError_t launch_conj_order(const mzh_conj *in, const uint32_t *in_step, uint64_t n, const uint64_t *start,
+                             const uint64_t *cnt, uint32_t *cursor, uint32_t *pos, uint32_t *inv, mzh_conj *out,
+                             uint32_t n_steps, hipStream_t stream);
+
+} // namespace mzh

@@ -1,0 +1,951 @@
+// Narrow phase for gfx950 (detect_conjunctions_narrow_phase.cpp:167-769).
+//
+// One lane per broad-phase pair of any step of the batch. For every pair of
+// overlapping trajectory steps the lane translates the position polynomials to
+// the common interval start, evaluates them at the order+1 Chebyshev nodes of
+// [0, rf_int], interpolates dist^2 with Bjorck-Pereyra, screens with a
+// Cargo-Shisha enclosure and, when the enclosure reaches below thresh^2,
+// isolates the minima (roots of d(dist^2)/dt with positive second derivative)
+// with the Vincent-Collins-Akritas bisection and refines them with TOMS 748.
+// All of it in FP64 with separately rounded operations, in the operation
+// order of the CPU restatement, so TCA/DCA match it bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mzh_device.hpp"
+#include "mzh_kernels.hpp"
+
+// Order range of the kernels compiled into this object; a standalone compile
+// builds every order and the host-side entry points.
+#ifndef MZH_NARROW_LO
+#define MZH_NARROW_LO 2
+#define MZH_NARROW_HI 20
+#define MZH_NARROW_HOST
+#endif
+
+namespace mzh
+{
+
+namespace
+{
+
+constexpr BinomTable kBinomN = make_binom_table();
+__constant__ BinomTable c_binom_np = kBinomN;
+
+constexpr int kWlCap = 48;  // device work-list capacity of the root isolation
+// capacity of the per-interval root list (isolated roots plus roots found
+// exactly at a work-list interval's lower end)
+template <int ORDER>
+constexpr int kMaxRoots = 2 * ORDER + 2;
+
+template <int ORDER>
+__device__ inline void halving_pows_d(double a, double *p)
+{
+    p[0] = 1.0;
+    p[1] = a;
+#pragma unroll
+    for (int i = 2; i <= ORDER; ++i) p[i] = (i % 2 == 0) ? p[i / 2] * p[i / 2] : p[i / 2 + 1] * p[i / 2];
+}
+
+// p(x) -> p(x + a) for one channel of an AoS trajectory step (stride 7)
+// (conjunctions_jit.cpp:68-115).
+template <int ORDER>
+__device__ inline void translate7(const double *__restrict__ src, double a, double *out)
+{
+    constexpr int OP1 = ORDER + 1;
+    double c[OP1];
+#pragma unroll
+    for (int k = 0; k < OP1; ++k) c[k] = src[k * 7];
+    if (a == 0.0) {
+#pragma unroll
+        for (int k = 0; k < OP1; ++k) out[k] = c[k];
+        return;
+    }
+    double ap[OP1];
+    halving_pows_d<ORDER>(a, ap);
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) {
+        double s = (c[i] * c_binom_np.v[i][0]) * ap[0];
+#pragma unroll
+        for (int k = i + 1; k < OP1; ++k) s = s + (c[k] * c_binom_np.v[k][k - i]) * ap[k - i];
+        out[i] = s;
+    }
+}
+
+template <int ORDER>
+__device__ inline double horner(const double *p, double x)
+{
+    double r = p[ORDER];
+#pragma unroll
+    for (int o = 1; o <= ORDER; ++o) r = p[ORDER - o] + r * x;
+    return r;
+}
+
+__device__ inline double hy_min_np(double a, double b) { return (b == b) ? ((b < a) ? b : a) : b; }
+__device__ inline double hy_max_np(double a, double b) { return (b == b) ? ((a < b) ? b : a) : b; }
+
+template <int N>
+__device__ inline double pwr_min(double *v)
+{
+    if constexpr (N == 1) {
+        return v[0];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) v[i] = hy_min_np(v[2 * i], v[2 * i + 1]);
+        if constexpr (N % 2) v[N / 2] = v[N - 1];
+        return pwr_min<(N + 1) / 2>(v);
+    }
+}
+
+template <int N>
+__device__ inline double pwr_max(double *v)
+{
+    if constexpr (N == 1) {
+        return v[0];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) v[i] = hy_max_np(v[2 * i], v[2 * i + 1]);
+        if constexpr (N % 2) v[N / 2] = v[N - 1];
+        return pwr_max<(N + 1) / 2>(v);
+    }
+}
+
+// Cargo-Shisha enclosure of p over [0, h] (cs_enc, conjunctions_jit.cpp:302-323).
+template <int ORDER>
+__device__ inline void cs_enc0(const double *p, double h, double &lo, double &hi)
+{
+    constexpr int OP1 = ORDER + 1;
+    double t[OP1], dp[OP1], bmin[OP1], bmax[OP1];
+    halving_pows_d<ORDER>(h, dp);
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) t[i] = (p[i] * dp[i]) / c_binom_np.v[ORDER][i];
+    bmin[0] = bmax[0] = t[0];
+#pragma unroll
+    for (int r = 1; r <= ORDER; ++r) {
+#pragma unroll
+        for (int k = 0; k + r <= ORDER; ++k) t[k] = t[k] + t[k + 1];
+        bmin[r] = bmax[r] = t[0];
+    }
+    lo = pwr_min<OP1>(bmin);
+    hi = pwr_max<OP1>(bmax);
+}
+
+// ---- root isolation + refinement (poly_utils.cpp:150-460) -----------------
+
+__device__ inline int sgn_d(double v) { return (0.0 < v) - (v < 0.0); }
+
+template <int ORDER>
+__device__ inline void translate1(const double *c, double *out)
+{
+    constexpr int OP1 = ORDER + 1;
+    double tmp[OP1];
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) {
+        double s = c[i];
+#pragma unroll
+        for (int k = i + 1; k < OP1; ++k) s = s + c[k] * c_binom_np.v[k][i];
+        tmp[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) out[i] = tmp[i];
+}
+
+template <int ORDER>
+__device__ inline uint32_t rtscc(const double *q)
+{
+    constexpr int OP1 = ORDER + 1;
+    double r[OP1], s[OP1];
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) r[i] = q[ORDER - i];
+    translate1<ORDER>(r, s);
+    uint32_t cnt = 0;
+    int last = sgn_d(s[0]);
+#pragma unroll
+    for (int i = 1; i < OP1; ++i) {
+        const int cur = sgn_d(s[i]);
+        cnt += (cur * last == -1) ? 1u : 0u;
+        if (cur != 0) last = cur;
+    }
+    return cnt;
+}
+
+template <int ORDER>
+__device__ inline bool fex_excludes(const double *p, double h)
+{
+    double lo = p[ORDER], hi = p[ORDER];
+#pragma unroll
+    for (int o = 1; o <= ORDER; ++o) {
+        const double a = lo * h, b = hi * h;
+        double nlo = a < b ? a : b, nhi = a < b ? b : a;
+        nlo = nlo < 0.0 ? nlo : 0.0;
+        nhi = nhi > 0.0 ? nhi : 0.0;
+        lo = nlo + p[ORDER - o];
+        hi = nhi + p[ORDER - o];
+    }
+    return (lo > 0.0) || (hi < 0.0);
+}
+
+template <int ORDER>
+__device__ inline double poly_eval_1(const double *a, double x)
+{
+    double r = a[ORDER] * (double)ORDER;
+#pragma unroll
+    for (int i = 1; i < ORDER; ++i) r = a[ORDER - i] * (double)(ORDER - i) + r * x;
+    return r;
+}
+
+// TOMS 748 (Alefeld, Potra & Shi 1995), eps tolerance 4 * 2^-52, same
+// step sequence as the CPU restatement.
+constexpr double kEps = 2.220446049250313080847263336181640625e-16;
+
+__device__ inline int bsign(double x) { return (x > 0) - (x < 0); }
+
+__device__ inline bool t_tol(double a, double b)
+{
+    const double m = fabs(a) < fabs(b) ? fabs(a) : fabs(b);
+    return fabs(a - b) <= (4.0 * kEps) * m;
+}
+
+__device__ inline double t_secant(double a, double b, double fa, double fb)
+{
+    const double tol = kEps * 5;
+    const double c = a - (fa / (fb - fa)) * (b - a);
+    if ((c <= a + fabs(a) * tol) || (c >= b - fabs(b) * tol)) return (a + b) / 2;
+    return c;
+}
+
+__device__ inline double t_safe_div(double num, double den, double r)
+{
+    if (fabs(den) < 1) {
+        if (fabs(den * 1.7976931348623157e308) <= fabs(num)) return r;
+    }
+    return num / den;
+}
+
+__device__ inline double t_quadratic(double a, double b, double d, double fa, double fb, double fd, int count)
+{
+    const double B = t_safe_div(fb - fa, b - a, 1.7976931348623157e308);
+    double A = t_safe_div(fd - fb, d - b, 1.7976931348623157e308);
+    A = t_safe_div(A - B, d - a, 0.0);
+    if (A == 0) return t_secant(a, b, fa, fb);
+    double c = (bsign(A) * bsign(fa) > 0) ? a : b;
+    for (int i = 1; i <= count; ++i) {
+        c -= t_safe_div(fa + (B + A * (c - b)) * (c - a), B + A * (2 * c - a - b), 1 + c - a);
+    }
+    if ((c <= a) || (c >= b)) c = t_secant(a, b, fa, fb);
+    return c;
+}
+
+__device__ inline double t_cubic(double a, double b, double d, double e, double fa, double fb, double fd, double fe)
+{
+    const double q11 = (d - e) * fd / (fe - fd);
+    const double q21 = (b - d) * fb / (fd - fb);
+    const double q31 = (a - b) * fa / (fb - fa);
+    const double d21 = (b - d) * fd / (fd - fb);
+    const double d31 = (a - b) * fb / (fb - fa);
+    const double q22 = (d21 - q11) * fb / (fe - fb);
+    const double q32 = (d31 - q21) * fa / (fd - fa);
+    const double d32 = (d31 - q21) * fd / (fd - fa);
+    const double q33 = (d32 - q22) * fa / (fe - fa);
+    double c = q31 + q32 + q33 + a;
+    if ((c <= a) || (c >= b)) c = t_quadratic(a, b, d, fa, fb, fd, 3);
+    return c;
+}
+
+__device__ inline bool t_prof(double fa, double fb, double fd, double fe)
+{
+    const double md = 2.2250738585072014e-308 * 32;
+    return (fabs(fa - fb) < md) || (fabs(fa - fd) < md) || (fabs(fa - fe) < md) || (fabs(fb - fd) < md)
+           || (fabs(fb - fe) < md) || (fabs(fd - fe) < md);
+}
+
+template <int ORDER>
+struct T748 {
+    const double *q;
+    __device__ inline double f(double x) const { return horner<ORDER>(q, x); }
+
+    __device__ inline void bracket(double &a, double &b, double c, double &fa, double &fb, double &d, double &fd) const
+    {
+        const double tol = kEps * 2;
+        if ((b - a) < 2 * tol * a) {
+            c = a + (b - a) / 2;
+        } else if (c <= a + fabs(a) * tol) {
+            c = a + fabs(a) * tol;
+        } else if (c >= b - fabs(b) * tol) {
+            c = b - fabs(b) * tol;
+        }
+        const double fc = f(c);
+        if (fc == 0) {
+            a = c;
+            fa = 0;
+            d = 0;
+            fd = 0;
+            return;
+        }
+        if (bsign(fa) * bsign(fc) < 0) {
+            d = b;
+            fd = fb;
+            b = c;
+            fb = fc;
+        } else {
+            d = a;
+            fd = fa;
+            a = c;
+            fa = fc;
+        }
+    }
+
+    // returns 0 on success, 33 (EDOM) on a non-bracketing interval; iters =
+    // number of function evaluations
+    __device__ inline int solve(double a, double b, unsigned max_iter, double &ra, double &rb, unsigned &iters) const
+    {
+        double fa = f(a), fb = f(b);
+        unsigned count = max_iter - 2;
+        const unsigned count0 = count;
+        if (a >= b) {
+            iters = 2;
+            return 33;
+        }
+        if (t_tol(a, b) || (fa == 0) || (fb == 0)) {
+            if (fa == 0) {
+                b = a;
+            } else if (fb == 0) {
+                a = b;
+            }
+            ra = a;
+            rb = b;
+            iters = 2;
+            return 0;
+        }
+        if (bsign(fa) * bsign(fb) > 0) {
+            iters = 2;
+            return 33;
+        }
+        double c, u, fu, a0, b0, d = 0, fd = 1e5, e = 1e5, fe = 1e5;
+        if (fa != 0) {
+            c = t_secant(a, b, fa, fb);
+            bracket(a, b, c, fa, fb, d, fd);
+            --count;
+            if (count && (fa != 0) && !t_tol(a, b)) {
+                c = t_quadratic(a, b, d, fa, fb, fd, 2);
+                e = d;
+                fe = fd;
+                bracket(a, b, c, fa, fb, d, fd);
+                --count;
+            }
+        }
+        while (count && (fa != 0) && !t_tol(a, b)) {
+            a0 = a;
+            b0 = b;
+            c = t_prof(fa, fb, fd, fe) ? t_quadratic(a, b, d, fa, fb, fd, 2) : t_cubic(a, b, d, e, fa, fb, fd, fe);
+            e = d;
+            fe = fd;
+            bracket(a, b, c, fa, fb, d, fd);
+            if ((0 == --count) || (fa == 0) || t_tol(a, b)) break;
+            c = t_prof(fa, fb, fd, fe) ? t_quadratic(a, b, d, fa, fb, fd, 3) : t_cubic(a, b, d, e, fa, fb, fd, fe);
+            bracket(a, b, c, fa, fb, d, fd);
+            if ((0 == --count) || (fa == 0) || t_tol(a, b)) break;
+            if (fabs(fa) < fabs(fb)) {
+                u = a;
+                fu = fa;
+            } else {
+                u = b;
+                fu = fb;
+            }
+            c = u - 2 * (fu / (fb - fa)) * (b - a);
+            if (fabs(c - u) > (b - a) / 2) c = a + (b - a) / 2;
+            e = d;
+            fe = fd;
+            bracket(a, b, c, fa, fb, d, fd);
+            if ((0 == --count) || (fa == 0) || t_tol(a, b)) break;
+            if ((b - a) < 0.5 * (b0 - a0)) continue;
+            e = d;
+            fe = fd;
+            bracket(a, b, a + (b - a) / 2, fa, fb, d, fd);
+            --count;
+        }
+        if (fa == 0) {
+            b = a;
+        } else if (fb == 0) {
+            a = b;
+        }
+        ra = a;
+        rb = b;
+        iters = (count0 - count) + 2;
+        return 0;
+    }
+};
+
+template <int ORDER>
+struct WLEntry {
+    double lb, ub;
+    double p[ORDER + 1];
+};
+
+// run_poly_root_finding with direction +1. Returns false (and reports) on error.
+template <int ORDER>
+__device__ bool find_minima(const double *poly, double rf_int, double *roots, int &nroots, bool &fex, uint32_t i,
+                            uint32_t j, DevError *err)
+{
+    constexpr int OP1 = ORDER + 1;
+    nroots = 0;
+    fex = false;
+    if (fex_excludes<ORDER>(poly, rf_int)) {
+        fex = true;
+        return true;
+    }
+    WLEntry<ORDER> wl[kWlCap];
+    double iso_lb[ORDER + 1], iso_ub[ORDER + 1];
+    int nwl = 0, niso = 0;
+
+    auto add_root = [&](double rt) -> bool {
+        if (!is_finite_d(rt)) {
+            report_error(err, kErrNpRoot, i, j, rt);
+            return false;
+        }
+        const double der = poly_eval_1<ORDER>(poly, rt);
+        if (!is_finite_d(der)) {
+            report_error(err, kErrNpRootDer, i, j, rt);
+            return false;
+        }
+        if (sgn_d(der) == 1) {
+            if (nroots >= kMaxRoots<ORDER>) {
+                report_error(err, kErrNpIsolation, (uint32_t)nwl, (uint32_t)niso, 0.0);
+                return false;
+            }
+            roots[nroots++] = rt;
+        }
+        return true;
+    };
+
+    {
+        double cf = 1.0;
+#pragma unroll
+        for (int k = 0; k < OP1; ++k) {
+            wl[0].p[k] = cf * poly[k];
+            cf *= rf_int;
+        }
+        wl[0].lb = 0;
+        wl[0].ub = 1;
+        nwl = 1;
+    }
+    do {
+        double q[OP1];
+        const double lb = wl[nwl - 1].lb, ub = wl[nwl - 1].ub;
+#pragma unroll
+        for (int k = 0; k < OP1; ++k) q[k] = wl[nwl - 1].p[k];
+        --nwl;
+        if (q[0] == 0) {
+            bool allf = true;
+#pragma unroll
+            for (int k = 1; k < OP1; ++k) allf = allf && is_finite_d(q[k]);
+            if (allf && !add_root(lb * rf_int)) return false;
+        }
+        const uint32_t nsc = rtscc<ORDER>(q);
+        if (nsc == 1) {
+            if (niso >= ORDER) {
+                report_error(err, kErrNpIsolation, (uint32_t)nwl, (uint32_t)(niso + 1), 0.0);
+                return false;
+            }
+            iso_lb[niso] = lb;
+            iso_ub[niso] = ub;
+            ++niso;
+        } else if (nsc > 1) {
+            if (nwl + 2 > kWlCap) {
+                report_error(err, kErrNpWorkList, i, j, (double)(nwl + 2));
+                return false;
+            }
+            double r[OP1], s[OP1];
+            double cf = 1.0;
+#pragma unroll
+            for (int k = 0; k < OP1; ++k) {
+                r[ORDER - k] = cf * q[ORDER - k];
+                cf *= 2;
+            }
+            translate1<ORDER>(r, s);
+            const double mid = (lb / 2) + (ub / 2);
+            wl[nwl].lb = lb;
+            wl[nwl].ub = mid;
+#pragma unroll
+            for (int k = 0; k < OP1; ++k) wl[nwl].p[k] = r[k];
+            ++nwl;
+            wl[nwl].lb = mid;
+            wl[nwl].ub = ub;
+#pragma unroll
+            for (int k = 0; k < OP1; ++k) wl[nwl].p[k] = s[k];
+            ++nwl;
+        }
+        if (nwl > 250) {
+            report_error(err, kErrNpIsolation, (uint32_t)nwl, (uint32_t)niso, 0.0);
+            return false;
+        }
+    } while (nwl != 0);
+
+    if (niso) {
+        double q[OP1];
+        double cf = 1.0;
+#pragma unroll
+        for (int k = 0; k < OP1; ++k) {
+            q[k] = cf * poly[k];
+            cf *= rf_int;
+        }
+        const T748<ORDER> solver{q};
+        for (int k = 0; k < niso; ++k) {
+            double lb = iso_lb[k], ub = iso_ub[k];
+            if (is_finite_d(lb) && is_finite_d(ub) && ub > lb) ub = next_toward_d(ub, lb);
+            double ra = 0, rb = 0;
+            unsigned it = 0;
+            const int e = solver.solve(lb, ub, 100, ra, rb, it);
+            if (e != 0) {
+                report_error(err, kErrNpErrno, i, j, (double)e);
+                return false;
+            }
+            if (it >= 100) {
+                report_error(err, kErrNpIters, i, j, 0.0);
+                return false;
+            }
+            if (!add_root(((ra / 2) + (rb / 2)) * rf_int)) return false;
+        }
+    }
+    return true;
+}
+
+// Bjorck-Pereyra interpolation in place: f -> monomial coefficients
+// (poly_utils.cpp:467-535).
+template <int ORDER>
+__device__ inline void vm_interp(const double *alpha, double *c)
+{
+#pragma unroll
+    for (int k = 0; k < ORDER; ++k) {
+#pragma unroll
+        for (int j = ORDER; j > k; --j) c[j] = (c[j] - c[j - 1]) / (alpha[j] - alpha[j - k - 1]);
+    }
+#pragma unroll
+    for (int idx = 0; idx < ORDER; ++idx) {
+        const int k = ORDER - idx - 1;
+#pragma unroll
+        for (int j = k; j < ORDER; ++j) c[j] = c[j] - alpha[k] * c[j + 1];
+    }
+}
+
+template <int ORDER>
+__device__ inline void emit_conj(const NarrowArgs &a, uint32_t c, uint32_t i, uint32_t j, double lb_rf, double t,
+                                 const double *Pi, double di, const double *Pj, double dj)
+{
+    constexpr int OP1 = ORDER + 1;
+    mzh_conj rec;
+    rec.i = i;
+    rec.j = j;
+    double tp[OP1];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        translate7<ORDER>(Pi + ch, di, tp);
+        rec.ri[ch] = horner<ORDER>(tp, t);
+        translate7<ORDER>(Pi + 3 + ch, di, tp);
+        rec.vi[ch] = horner<ORDER>(tp, t);
+        translate7<ORDER>(Pj + ch, dj, tp);
+        rec.rj[ch] = horner<ORDER>(tp, t);
+        translate7<ORDER>(Pj + 3 + ch, dj, tp);
+        rec.vj[ch] = horner<ORDER>(tp, t);
+    }
+    const double dx = rec.ri[0] - rec.rj[0], dy = rec.ri[1] - rec.rj[1], dz = rec.ri[2] - rec.rj[2];
+    rec.dca = sqrt((dx * dx) + (dy * dy) + (dz * dz));
+    rec.tca = lb_rf + t;
+    const unsigned long long slot = atomicAdd(a.conj_count, 1ull);
+    if (slot < a.conj_cap) {
+        a.conj[slot] = rec;
+        a.conj_step[slot] = c;
+    }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_narrow(NarrowArgs a)
+{
+    constexpr int OP1 = ORDER + 1;
+    const uint64_t p = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    unsigned long long rep[MZH_NP_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+    if (p < a.n_pairs) {
+        // step of this pair: last c with step_off[c] <= p
+        uint32_t lo = 0, n = a.sg.n_steps;
+        while (n > 0) {
+            const uint32_t h = n / 2;
+            if (a.step_off[lo + h + 1] <= p) {
+                lo += h + 1;
+                n -= h + 1;
+            } else {
+                n = h;
+            }
+        }
+        const uint32_t c = lo;
+        const uint32_t i = a.pairs[p].i, j = a.pairs[p].j;
+        const double cb = a.sg.cd_begin[c], ce = a.sg.cd_end[c];
+        const double thresh2 = a.conj_thresh * a.conj_thresh;
+        const uint64_t nsi = a.pj.nsteps[i], nsj = a.pj.nsteps[j];
+        const double *ti = a.pj.time + a.pj.time_off[i];
+        const double *tj = a.pj.time + a.pj.time_off[j];
+        // trajectory-step ranges overlapping [cb, ce) (same search as the AABBs)
+        auto range = [](const double *t, uint64_t ns, double cbv, double cev, uint64_t &kb, uint64_t &ke) {
+            uint64_t l = 0, m = ns;
+            while (m > 0) {
+                const uint64_t h = m / 2;
+                if (!(cbv < t[1 + l + h])) {
+                    l += h + 1;
+                    m -= h + 1;
+                } else {
+                    m = h;
+                }
+            }
+            kb = l;
+            m = ns - l;
+            while (m > 0) {
+                const uint64_t h = m / 2;
+                if (t[1 + l + h] < cev) {
+                    l += h + 1;
+                    m -= h + 1;
+                } else {
+                    m = h;
+                }
+            }
+            ke = l + (l != ns ? 1 : 0);
+        };
+        uint64_t kbi, kei, kbj, kej;
+        range(ti, nsi, cb, ce, kbi, kei);
+        range(tj, nsj, cb, ce, kbj, kej);
+        double nodes[OP1];
+        bool failed = false;
+        for (uint64_t ki = kbi, kj = kbj; ki < kei && kj < kej && !failed;) {
+            const double tsi = ti[ki], tei = ti[ki + 1], tsj = tj[kj], tej = tj[kj + 1];
+            const double lb_i = cb > tsi ? cb : tsi, ub_i = ce < tei ? ce : tei;
+            const double lb_j = cb > tsj ? cb : tsj, ub_j = ce < tej ? ce : tej;
+            if (ub_i > lb_j && lb_i < ub_j) {
+                ++rep[0];
+                const double lb_rf = lb_i > lb_j ? lb_i : lb_j;
+                const double ub_rf = ub_i < ub_j ? ub_i : ub_j;
+                const double di = lb_rf - tsi, dj = lb_rf - tsj;
+                const double rf_int = ub_rf - lb_rf;
+                if (!is_finite_d(di) || !is_finite_d(dj) || !is_finite_d(rf_int) || di < 0 || dj < 0 || rf_int < 0) {
+                    report_error(a.err, kErrNpInterval, i, j, rf_int);
+                    failed = true;
+                    break;
+                }
+                const double *Pi = a.pj.traj + (a.pj.step0[i] + ki) * (uint64_t)OP1 * 7u;
+                const double *Pj = a.pj.traj + (a.pj.step0[j] + kj) * (uint64_t)OP1 * 7u;
+#pragma unroll
+                for (int m = 0; m < OP1; ++m) nodes[OP1 - 1 - m] = a.cheby[m] * rf_int;
+                double d2[OP1];
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    double tp[OP1], vi[OP1];
+                    translate7<ORDER>(Pi + ch, di, tp);
+#pragma unroll
+                    for (int m = 0; m < OP1; ++m) vi[m] = horner<ORDER>(tp, nodes[m]);
+                    translate7<ORDER>(Pj + ch, dj, tp);
+#pragma unroll
+                    for (int m = 0; m < OP1; ++m) {
+                        const double dx = vi[m] - horner<ORDER>(tp, nodes[m]);
+                        d2[m] = (ch == 0) ? dx * dx : d2[m] + dx * dx;
+                    }
+                }
+                vm_interp<ORDER>(nodes, d2);  // d2 now holds the dist^2 coefficients
+                double elo, ehi;
+                cs_enc0<ORDER>(d2, rf_int, elo, ehi);
+                if (!is_finite_d(elo) || !is_finite_d(ehi)) {
+                    report_error(a.err, kErrNpDist2, i, j, elo);
+                    failed = true;
+                    break;
+                }
+                if (elo < thresh2) {
+                    double d2p[OP1];
+#pragma unroll
+                    for (int k = 0; k < ORDER; ++k) d2p[k] = (double)(k + 1) * d2[k + 1];
+                    d2p[ORDER] = 0;
+                    double roots[kMaxRoots<ORDER>];
+                    int nr = 0;
+                    bool fex = false;
+                    if (!find_minima<ORDER>(d2p, rf_int, roots, nr, fex, i, j, a.err)) {
+                        failed = true;
+                        break;
+                    }
+                    ++rep[2];
+                    rep[3] += fex ? 1 : 0;
+                    rep[4] += (nr == 0) ? 1 : 0;
+                    rep[5] += (unsigned long long)nr;
+                    for (int r = 0; r < nr; ++r) {
+                        const double cd2 = horner<ORDER>(d2, roots[r]);
+                        if (!is_finite_d(cd2)) {
+                            report_error(a.err, kErrNpConjDist2, i, j, cd2);
+                            failed = true;
+                            break;
+                        }
+                        if (cd2 < thresh2) {
+                            emit_conj<ORDER>(a, c, i, j, lb_rf, roots[r], Pi, di, Pj, dj);
+                        } else {
+                            ++rep[6];
+                        }
+                    }
+                    if (failed) break;
+                    // boundary minima (narrow_phase.cpp:629-714)
+                    if ((ki + 1 == nsi && tei == ub_rf) || (kj + 1 == nsj && tej == ub_rf)) {
+                        const double tmc = next_toward_d(rf_int, -1.0);
+                        const double dv = horner<ORDER>(d2p, tmc);
+                        if (!is_finite_d(dv)) {
+                            report_error(a.err, kErrNpBndDerUb, i, j, dv);
+                            failed = true;
+                            break;
+                        }
+                        if (dv < 0) {
+                            const double dd = horner<ORDER>(d2, tmc);
+                            if (!is_finite_d(dd)) {
+                                report_error(a.err, kErrNpBndDist2Ub, i, j, dd);
+                                failed = true;
+                                break;
+                            }
+                            if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, tmc, Pi, di, Pj, dj);
+                        }
+                    }
+                    if ((ki == 0 && lb_rf == ti[0]) || (kj == 0 && lb_rf == tj[0])) {
+                        const double dv = horner<ORDER>(d2p, 0.0);
+                        if (!is_finite_d(dv)) {
+                            report_error(a.err, kErrNpBndDerLb, i, j, dv);
+                            failed = true;
+                            break;
+                        }
+                        if (dv > 0) {
+                            const double dd = horner<ORDER>(d2, 0.0);
+                            if (!is_finite_d(dd)) {
+                                report_error(a.err, kErrNpBndDist2Lb, i, j, dd);
+                                failed = true;
+                                break;
+                            }
+                            if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, 0.0, Pi, di, Pj, dj);
+                        }
+                    }
+                } else {
+                    ++rep[1];
+                }
+            }
+            const bool inc_i = tei <= tej, inc_j = tej <= tei;
+            ki += inc_i ? 1 : 0;
+            kj += inc_j ? 1 : 0;
+        }
+    }
+    // wave-aggregated counter updates
+#pragma unroll
+    for (int k = 0; k < MZH_NP_COUNT; ++k) {
+        const unsigned long long s = wave_reduce_add_u64(rep[k]);
+        if (__lane_id() == 0 && s) atomicAdd(&a.np_report[k], s);
+    }
+}
+
+#ifdef MZH_NARROW_HOST
+// ---- conjunction ordering ---------------------------------------------------
+
+__global__ void k_conj_hist(const uint32_t *__restrict__ step, uint64_t n, uint64_t *__restrict__ cnt)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) atomicAdd((unsigned long long *)&cnt[step[k]], 1ull);
+}
+
+__global__ void k_conj_scatter(const uint32_t *__restrict__ step, uint64_t n, const uint64_t *__restrict__ start,
+                               uint32_t *__restrict__ cursor, uint32_t *__restrict__ pos)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        const uint32_t c = step[k];
+        pos[k] = (uint32_t)(start[c] + atomicAdd(&cursor[c], 1u));
+    }
+}
+
+// strict weak order (tca, then all fields lexicographically), with the input
+// index as the final tie-break so that equal records get distinct ranks
+__device__ inline bool conj_less(const mzh_conj &x, const mzh_conj &y)
+{
+    if (x.tca != y.tca) return x.tca < y.tca;
+    if (x.i != y.i) return x.i < y.i;
+    if (x.j != y.j) return x.j < y.j;
+    if (x.dca != y.dca) return x.dca < y.dca;
+    for (int k = 0; k < 3; ++k)
+        if (x.ri[k] != y.ri[k]) return x.ri[k] < y.ri[k];
+    for (int k = 0; k < 3; ++k)
+        if (x.vi[k] != y.vi[k]) return x.vi[k] < y.vi[k];
+    for (int k = 0; k < 3; ++k)
+        if (x.rj[k] != y.rj[k]) return x.rj[k] < y.rj[k];
+    for (int k = 0; k < 3; ++k)
+        if (x.vj[k] != y.vj[k]) return x.vj[k] < y.vj[k];
+    return false;
+}
+
+// rank of every record inside its step segment
+__global__ void k_conj_rank(const mzh_conj *__restrict__ in, const uint32_t *__restrict__ step, uint64_t n,
+                            const uint64_t *__restrict__ start, const uint64_t *__restrict__ cnt,
+                            const uint32_t *__restrict__ pos, uint32_t *__restrict__ inv, mzh_conj *__restrict__ out)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t c = step[k];
+    const uint64_t s0 = start[c], m = cnt[c];
+    const mzh_conj me = in[k];
+    uint64_t rank = 0;
+    for (uint64_t q = 0; q < m; ++q) {
+        const uint32_t other = inv[s0 + q];
+        if (other == (uint32_t)k) continue;
+        const mzh_conj &y = in[other];
+        if (conj_less(y, me) || (!conj_less(me, y) && other < (uint32_t)k)) ++rank;
+    }
+    out[s0 + rank] = me;
+}
+
+__global__ void k_conj_inv(const uint32_t *__restrict__ pos, uint64_t n, uint32_t *__restrict__ inv)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) inv[pos[k]] = (uint32_t)k;
+}
+#endif // MZH_NARROW_HOST
+
+} // namespace
+
+template <int ORDER>
+hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream)
+{
+    const uint64_t blocks = (a.n_pairs + 255) / 256;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_narrow<ORDER>, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+// Explicit instantiations for the orders compiled into this object (the build
+// splits the order range over several objects so that they compile in parallel).
+#define MZH_INST(N) template hipError_t launch_narrow_order<N>(const NarrowArgs &, hipStream_t);
+#if MZH_NARROW_LO <= 2 && 2 <= MZH_NARROW_HI
+MZH_INST(2)
+#endif
+#if MZH_NARROW_LO <= 3 && 3 <= MZH_NARROW_HI
+MZH_INST(3)
+#endif
+#if MZH_NARROW_LO <= 4 && 4 <= MZH_NARROW_HI
+MZH_INST(4)
+#endif
+#if MZH_NARROW_LO <= 5 && 5 <= MZH_NARROW_HI
+MZH_INST(5)
+#endif
+#if MZH_NARROW_LO <= 6 && 6 <= MZH_NARROW_HI
+MZH_INST(6)
+#endif
+#if MZH_NARROW_LO <= 7 && 7 <= MZH_NARROW_HI
+MZH_INST(7)
+#endif
+#if MZH_NARROW_LO <= 8 && 8 <= MZH_NARROW_HI
+MZH_INST(8)
+#endif
+#if MZH_NARROW_LO <= 9 && 9 <= MZH_NARROW_HI
+MZH_INST(9)
+#endif
+#if MZH_NARROW_LO <= 10 && 10 <= MZH_NARROW_HI
+MZH_INST(10)
+#endif
+#if MZH_NARROW_LO <= 11 && 11 <= MZH_NARROW_HI
+MZH_INST(11)
+#endif
+#if MZH_NARROW_LO <= 12 && 12 <= MZH_NARROW_HI
+MZH_INST(12)
+#endif
+#if MZH_NARROW_LO <= 13 && 13 <= MZH_NARROW_HI
+MZH_INST(13)
+#endif
+#if MZH_NARROW_LO <= 14 && 14 <= MZH_NARROW_HI
+MZH_INST(14)
+#endif
+#if MZH_NARROW_LO <= 15 && 15 <= MZH_NARROW_HI
+MZH_INST(15)
+#endif
+#if MZH_NARROW_LO <= 16 && 16 <= MZH_NARROW_HI
+MZH_INST(16)
+#endif
+#if MZH_NARROW_LO <= 17 && 17 <= MZH_NARROW_HI
+MZH_INST(17)
+#endif
+#if MZH_NARROW_LO <= 18 && 18 <= MZH_NARROW_HI
+MZH_INST(18)
+#endif
+#if MZH_NARROW_LO <= 19 && 19 <= MZH_NARROW_HI
+MZH_INST(19)
+#endif
+#if MZH_NARROW_LO <= 20 && 20 <= MZH_NARROW_HI
+MZH_INST(20)
+#endif
+#undef MZH_INST
+
+#ifdef MZH_NARROW_HOST
+hipError_t launch_narrow(const NarrowArgs &a, hipStream_t stream)
+{
+    switch (a.pj.order) {
+        case 2:
+            return launch_narrow_order<2>(a, stream);
+        case 3:
+            return launch_narrow_order<3>(a, stream);
+        case 4:
+            return launch_narrow_order<4>(a, stream);
+        case 5:
+            return launch_narrow_order<5>(a, stream);
+        case 6:
+            return launch_narrow_order<6>(a, stream);
+        case 7:
+            return launch_narrow_order<7>(a, stream);
+        case 8:
+            return launch_narrow_order<8>(a, stream);
+        case 9:
+            return launch_narrow_order<9>(a, stream);
+        case 10:
+            return launch_narrow_order<10>(a, stream);
+        case 11:
+            return launch_narrow_order<11>(a, stream);
+        case 12:
+            return launch_narrow_order<12>(a, stream);
+        case 13:
+            return launch_narrow_order<13>(a, stream);
+        case 14:
+            return launch_narrow_order<14>(a, stream);
+        case 15:
+            return launch_narrow_order<15>(a, stream);
+        case 16:
+            return launch_narrow_order<16>(a, stream);
+        case 17:
+            return launch_narrow_order<17>(a, stream);
+        case 18:
+            return launch_narrow_order<18>(a, stream);
+        case 19:
+            return launch_narrow_order<19>(a, stream);
+        case 20:
+            return launch_narrow_order<20>(a, stream);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+// Full conjunction ordering used by the orchestrator: histogram, scan (done by
+// the caller between the two halves), scatter, rank.
+hipError_t launch_conj_hist(const uint32_t *in_step, uint64_t n, uint64_t *cnt, uint32_t n_steps, hipStream_t stream)
+{
+    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint64_t) * n_steps, stream);
+    if (e != hipSuccess || n == 0) return e;
+    hipLaunchKernelGGL(k_conj_hist, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, in_step, n, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_conj_order(const mzh_conj *in, const uint32_t *in_step, uint64_t n, const uint64_t *start,
+                             const uint64_t *cnt, uint32_t *cursor, uint32_t *pos, uint32_t *inv, mzh_conj *out,
+                             uint32_t n_steps, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(cursor, 0, sizeof(uint32_t) * n_steps, stream);
+    if (e != hipSuccess) return e;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_conj_scatter, dim3(blocks), dim3(256), 0, stream, in_step, n, start, cursor, pos);
+    hipLaunchKernelGGL(k_conj_inv, dim3(blocks), dim3(256), 0, stream, pos, n, inv);
+    hipLaunchKernelGGL(k_conj_rank, dim3(blocks), dim3(256), 0, stream, in, in_step, n, start, cnt, pos, inv, out);
+    return hipGetLastError();
+}
+#endif // MZH_NARROW_HOST
+
+} // namespace mzh

@@ -1,0 +1,969 @@
+// Screening kernels for gfx950: AABBs, global box, Morton codes, segmented
+// radix sort, gather, BVH build and broad phase.
+//
+// Batching: every kernel covers all conjunction steps of a problem in one
+// launch (grid.y or grid.x = step), so a full-day screen is a fixed handful
+// of launches instead of per-step loops.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mzh_device.hpp"
+#include "mzh_kernels.hpp"
+
+namespace mzh
+{
+
+namespace
+{
+
+constexpr BinomTable kBinom = make_binom_table();
+__constant__ BinomTable c_binom = kBinom;
+
+// ---------------------------------------------------------------------------
+// upload-time transpose
+// ---------------------------------------------------------------------------
+
+__global__ void k_transpose_cf4(const double *__restrict__ traj, double *__restrict__ cf4, uint64_t n_steps_total,
+                                uint32_t op1)
+{
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t per_step = 4ull * op1;
+    if (idx >= n_steps_total * per_step) return;
+    const uint64_t s = idx / per_step;
+    const uint32_t r = (uint32_t)(idx - s * per_step);
+    const uint32_t ch = r / op1, k = r - ch * op1;
+    const uint32_t src_ch = (ch == 3) ? 6u : ch;
+    cf4[idx] = traj[(s * op1 + k) * 7u + src_ch];
+}
+
+// ---------------------------------------------------------------------------
+// stage 1: AABBs (detect_conjunctions_aabbs.cpp:64-252, conjunctions_jit.cpp:179-298)
+// ---------------------------------------------------------------------------
+
+// a^0..a^ORDER with the halving scheme of conjunctions_jit.cpp:83-95.
+template <int ORDER, typename T>
+__device__ inline void halving_pows(T a, T *p)
+{
+    p[0] = T(1);
+    if constexpr (ORDER >= 1) p[1] = a;
+#pragma unroll
+    for (int i = 2; i <= ORDER; ++i) p[i] = (i % 2 == 0) ? p[i / 2] * p[i / 2] : p[i / 2 + 1] * p[i / 2];
+}
+
+__device__ inline double hy_min(double a, double b) { return (b == b) ? ((b < a) ? b : a) : b; }
+__device__ inline double hy_max(double a, double b) { return (b == b) ? ((a < b) ? b : a) : b; }
+
+template <int N>
+__device__ inline double pw_min(double *v)
+{
+    if constexpr (N == 1) {
+        return v[0];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) v[i] = hy_min(v[2 * i], v[2 * i + 1]);
+        if constexpr (N % 2) v[N / 2] = v[N - 1];
+        return pw_min<(N + 1) / 2>(v);
+    }
+}
+
+template <int N>
+__device__ inline double pw_max(double *v)
+{
+    if constexpr (N == 1) {
+        return v[0];
+    } else {
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) v[i] = hy_max(v[2 * i], v[2 * i + 1]);
+        if constexpr (N % 2) v[N / 2] = v[N - 1];
+        return pw_max<(N + 1) / 2>(v);
+    }
+}
+
+// FP64 Cargo-Shisha enclosure of one polynomial (contiguous coefficients)
+// over [lb, ub]: translate by lb, scale by (ub - lb)^i, divide by C(n, i),
+// difference table, NaN-propagating pairwise min/max.
+template <int ORDER>
+__device__ inline void cs_enclosure_fp64(const double *__restrict__ cf, double lb, double ub, double &emin,
+                                         double &emax)
+{
+    constexpr int OP1 = ORDER + 1;
+    double c[OP1], t[OP1], ap[OP1], dp[OP1];
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) c[i] = cf[i];
+    halving_pows<ORDER>(lb, ap);
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) {
+        double s = (c[i] * c_binom.v[i][0]) * ap[0];
+#pragma unroll
+        for (int k = i + 1; k < OP1; ++k) s = s + (c[k] * c_binom.v[k][k - i]) * ap[k - i];
+        t[i] = s;
+    }
+    halving_pows<ORDER>(ub - lb, dp);
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) t[i] = t[i] * dp[i];
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) t[i] = t[i] / c_binom.v[ORDER][i];
+    double bmin[OP1], bmax[OP1];
+    bmin[0] = bmax[0] = t[0];
+#pragma unroll
+    for (int r = 1; r <= ORDER; ++r) {
+#pragma unroll
+        for (int k = 0; k + r <= ORDER; ++k) t[k] = t[k] + t[k + 1];
+        bmin[r] = bmax[r] = t[0];
+    }
+    emin = pw_min<OP1>(bmin);
+    emax = pw_max<OP1>(bmax);
+}
+
+// FP32 Cargo-Shisha enclosure with a rigorous margin: the same algorithm in
+// single precision on coefficients rounded to float, widened by an a-priori
+// bound on every rounding error made (coefficient rounding included), so that
+// [emin, emax] contains the exact range of the FP64 polynomial and the
+// result of the FP64 enclosure.
+//
+// Error model: with u = 2^-24 every float operation returns x(1 + d), |d| <= u.
+// Bernstein coefficients b_r are sums with non-negative weights of
+// t_i = c'_i (ub - lb)^i / C(n, i), so |fl(b_r) - b_r| <= gamma_K * B where
+// B = sum_i |t_i| C(r, i) bounds the magnitude of the whole computation and K
+// counts the rounding steps on any path (translation: 3 + order, scaling: order,
+// division: 1, difference table: order). The same bound covers the FP64
+// enclosure's own (smaller) errors, so the float interval [b - e, b + e]
+// contains the FP64 Bernstein coefficient. The margin is evaluated from
+// magnitudes accumulated alongside the values and inflated by a factor 2.
+template <int ORDER>
+__device__ inline void cs_enclosure_fp32(const double *__restrict__ cf, double lb_d, double ub_d, float &emin,
+                                         float &emax)
+{
+    constexpr int OP1 = ORDER + 1;
+    const float lb = (float)lb_d;
+    const float dl = (float)(ub_d - lb_d);
+    float t[OP1], m[OP1], ap[OP1], dp[OP1];
+    halving_pows<ORDER>(lb, ap);
+    halving_pows<ORDER>(dl, dp);
+    float cfl[OP1];
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) cfl[i] = (float)cf[i];
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) {
+        float s = cfl[i], ms = fabsf(cfl[i]);
+#pragma unroll
+        for (int k = i + 1; k < OP1; ++k) {
+            const float w = (float)c_binom.v[k][k - i] * ap[k - i];
+            s = s + cfl[k] * w;
+            ms = ms + fabsf(cfl[k]) * fabsf(w);
+        }
+        const float sc = dp[i] / (float)c_binom.v[ORDER][i];
+        t[i] = s * sc;
+        m[i] = ms * fabsf(sc);
+    }
+    float bmin = t[0], bmax = t[0];
+    float mag = m[0];
+#pragma unroll
+    for (int r = 1; r <= ORDER; ++r) {
+#pragma unroll
+        for (int k = 0; k + r <= ORDER; ++k) {
+            t[k] = t[k] + t[k + 1];
+            m[k] = m[k] + m[k + 1];
+        }
+        // NaN-propagating min/max, as in the FP64 enclosure
+        bmin = (t[0] == t[0]) ? (t[0] < bmin ? t[0] : bmin) : t[0];
+        bmax = (t[0] == t[0]) ? (bmax < t[0] ? t[0] : bmax) : t[0];
+        mag = m[0] > mag ? m[0] : mag;
+    }
+    // Rounding steps on any path, relative to the magnitude sum: coefficient
+    // and parameter conversion (3), powers of lb and dl (2 * order each, the
+    // inputs' own rounding included), binomial products and the translation
+    // sum (order + 2), scaling and division (2), difference table (order).
+    // The FP64 enclosure's error is bounded by the same count at u = 2^-53 and
+    // is absorbed by the factor 2.
+    constexpr float kSteps = (float)(6 * ORDER + 8);
+    constexpr float kU = 5.9604645e-08f;
+    const float gamma = kSteps * kU / (1.0f - kSteps * kU);
+    const float margin = 2.0f * gamma * mag + 1.0e-30f;
+    emin = bmin - margin;
+    emax = bmax + margin;
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double radius, int mode,
+                                               float *__restrict__ aabbs, uint32_t *__restrict__ gbox_ord,
+                                               DevError *err)
+{
+    constexpr int OP1 = ORDER + 1;
+    const uint32_t N = pj.n_objs;
+    const uint32_t bps = (N + 255u) / 256u;
+    const uint32_t c = blockIdx.x / bps;
+    const uint32_t o = (blockIdx.x - c * bps) * 256u + threadIdx.x;
+    float lb4[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float ub4[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    if (o < N) {
+        const uint64_t ns = pj.nsteps[o];
+        const double cb = sg.cd_begin[c], ce = sg.cd_end[c];
+        if (ns > 0) {
+            const double *t = pj.time + pj.time_off[o];
+            if (!(t[0] >= ce || t[ns] <= cb)) {
+                // upper_bound(t + 1, t + ns + 1, cb) -> index kb into t[1..ns]
+                uint64_t lo = 0, n = ns;
+                while (n > 0) {
+                    const uint64_t h = n / 2;
+                    if (!(cb < t[1 + lo + h])) {
+                        lo += h + 1;
+                        n -= h + 1;
+                    } else {
+                        n = h;
+                    }
+                }
+                const uint64_t kb = lo;
+                // lower_bound(ts_begin, t_end, ce)
+                n = ns - kb;
+                while (n > 0) {
+                    const uint64_t h = n / 2;
+                    if (t[1 + lo + h] < ce) {
+                        lo += h + 1;
+                        n -= h + 1;
+                    } else {
+                        n = h;
+                    }
+                }
+                const uint64_t ke = lo + (lo != ns ? 1 : 0);
+                const double *cfb = pj.cf4 + (pj.step0[o] * 4u) * OP1;
+                for (uint64_t k = kb; k < ke; ++k) {
+                    const double ss = t[k];
+                    const double ev_lb = cb > ss ? cb : ss;
+                    const double ev_ub = ce < t[k + 1] ? ce : t[k + 1];
+                    const double h_lb = ev_lb - ss, h_ub = ev_ub - ss;
+                    const double *cf = cfb + k * 4u * OP1;
+#pragma unroll
+                    for (int d = 0; d < 4; ++d) {
+                        float fl, fu;
+                        if (mode == MZH_MODE_FP64) {
+                            double emin, emax;
+                            cs_enclosure_fp64<ORDER>(cf + d * OP1, h_lb, h_ub, emin, emax);
+                            emin = emin - radius;
+                            emax = emax + radius;
+                            fl = next_down_f((float)emin);
+                            fu = next_up_f((float)emax);
+                        } else {
+                            float emin, emax;
+                            cs_enclosure_fp32<ORDER>(cf + d * OP1, h_lb, h_ub, emin, emax);
+                            // emin <= FP64 enclosure minimum (as reals); the same
+                            // monotone double subtraction and outward float
+                            // conversion then keep this box around the FP64 box.
+                            fl = next_down_f((float)((double)emin - radius));
+                            fu = next_up_f((float)((double)emax + radius));
+                        }
+                        if (!is_finite_f(fl)) {
+                            report_error(err, kErrAabbLb, o, 0, (double)fl);
+                            fl = -3.0e38f;
+                        }
+                        if (!is_finite_f(fu)) {
+                            report_error(err, kErrAabbUb, o, 0, (double)fu);
+                            fu = 3.0e38f;
+                        }
+                        lb4[d] = lb4[d] < fl ? lb4[d] : fl;
+                        ub4[d] = ub4[d] > fu ? ub4[d] : fu;
+                    }
+                }
+            }
+        }
+        float4 *dst = reinterpret_cast<float4 *>(aabbs + ((size_t)c * (N + 1) + o) * 8u);
+        dst[0] = make_float4(lb4[0], lb4[1], lb4[2], lb4[3]);
+        dst[1] = make_float4(ub4[0], ub4[1], ub4[2], ub4[3]);
+    }
+    // block-wide min/max of the boxes -> one atomic per block and component
+    uint32_t ol[4], ou[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        ol[d] = f2ord(lb4[d]);
+        ou[d] = f2ord(ub4[d]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t a = __shfl_xor(ol[d], off, 64), b = __shfl_xor(ou[d], off, 64);
+            ol[d] = a < ol[d] ? a : ol[d];
+            ou[d] = b > ou[d] ? b : ou[d];
+        }
+    }
+    __shared__ uint32_t s_l[4][4], s_u[4][4];
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    if (lane == 0) {
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            s_l[w][d] = ol[d];
+            s_u[w][d] = ou[d];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int d = threadIdx.x;
+        uint32_t l = s_l[0][d], u = s_u[0][d];
+        for (int k = 1; k < 4; ++k) {
+            l = s_l[k][d] < l ? s_l[k][d] : l;
+            u = s_u[k][d] > u ? s_u[k][d] : u;
+        }
+        atomicMin(&gbox_ord[c * 8u + d], l);
+        atomicMax(&gbox_ord[c * 8u + 4u + d], u);
+    }
+}
+
+__global__ void k_gbox_finalize(const uint32_t *__restrict__ gbox_ord, float *__restrict__ aabbs, uint32_t N,
+                                uint32_t C, DevError *err)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float *row = aabbs + ((size_t)c * (N + 1) + N) * 8u;
+    for (int d = 0; d < 4; ++d) {
+        const float l = ord2f(gbox_ord[c * 8u + d]), u = ord2f(gbox_ord[c * 8u + 4u + d]);
+        if (is_finite_f(l) && !is_finite_f(u - l)) report_error(err, kErrGlobalBox, c, 0, 0.0);
+        row[d] = l;
+        row[4 + d] = u;
+    }
+}
+
+__global__ void k_init_gbox(uint32_t *gbox_ord, uint32_t C)
+{
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= C * 8u) return;
+    const uint32_t d = idx & 7u;
+    gbox_ord[idx] = (d < 4) ? f2ord(__builtin_inff()) : f2ord(-__builtin_inff());
+}
+
+// ---------------------------------------------------------------------------
+// stage 2: Morton codes (detect_conjunctions_morton.cpp:70-175)
+// ---------------------------------------------------------------------------
+
+__device__ inline uint64_t disc_coord(float x, float mn, float mx)
+{
+    const float isize = mx - mn;
+    float rx = (x - mn) / isize;
+    rx = rx >= 0.f ? rx : 0.f;
+    rx *= 65536.f;
+    // values beyond 2^16 are clamped below; keep the conversion in range
+    rx = rx < 131072.f ? rx : 131072.f;
+    const uint64_t r = (uint64_t)rx;
+    return r < 65535ull ? r : 65535ull;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float *__restrict__ aabbs, uint64_t *__restrict__ mcodes,
+                                                uint32_t N)
+{
+    const uint32_t bps = (N + 255u) / 256u;
+    const uint32_t c = blockIdx.x / bps;
+    const uint32_t o = (blockIdx.x - c * bps) * 256u + threadIdx.x;
+    if (o >= N) return;
+    const float *step = aabbs + (size_t)c * (N + 1) * 8u;
+    const float4 blo = reinterpret_cast<const float4 *>(step + (size_t)o * 8u)[0];
+    const float4 bhi = reinterpret_cast<const float4 *>(step + (size_t)o * 8u)[1];
+    uint64_t code = ~0ull;
+    if (is_finite_f(blo.x)) {
+        const float4 glo = reinterpret_cast<const float4 *>(step + (size_t)N * 8u)[0];
+        const float4 ghi = reinterpret_cast<const float4 *>(step + (size_t)N * 8u)[1];
+        const uint64_t q0 = disc_coord(blo.x / 2 + bhi.x / 2, glo.x, ghi.x);
+        const uint64_t q1 = disc_coord(blo.y / 2 + bhi.y / 2, glo.y, ghi.y);
+        const uint64_t q2 = disc_coord(blo.z / 2 + bhi.z / 2, glo.z, ghi.z);
+        const uint64_t q3 = disc_coord(blo.w / 2 + bhi.w / 2, glo.w, ghi.w);
+        code = spread4(q0) | (spread4(q1) << 1) | (spread4(q2) << 2) | (spread4(q3) << 3);
+    }
+    mcodes[(size_t)c * N + o] = code;
+}
+
+// ---------------------------------------------------------------------------
+// stage 2b: segmented stable LSD radix sort, one workgroup per step
+// (replaces tbb::parallel_sort of detect_conjunctions_morton.cpp:179-196).
+// Keys are the 64-bit codes, values the object indices. Eight 8-bit passes
+// with wave64 ballot matching give a sort that is stable in the object index;
+// a final stable partition moves objects without data (code 2^64-1, infinite
+// box) behind finite objects that happen to share the all-ones code.
+// ---------------------------------------------------------------------------
+
+constexpr int kSortBlock = 1024;
+constexpr int kSortItems = 4;
+constexpr int kSortWaves = kSortBlock / 64;
+
+__device__ inline uint32_t block_excl_scan_1024(uint32_t v, uint32_t *s_w, uint32_t &total)
+{
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t x = lane < (blockDim.x >> 6) ? s_w[lane] : 0u;
+        const uint32_t xi = wave_inclusive_scan(x);
+        if (lane < (blockDim.x >> 6)) s_w[lane] = xi - x;
+        if (lane == 63) s_w[64] = xi;
+    }
+    __syncthreads();
+    const uint32_t r = s_w[w] + inc - v;
+    total = s_w[64];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_sort(const uint64_t *__restrict__ mcodes,
+                                                       const float *__restrict__ aabbs, uint64_t *__restrict__ keys_tmp,
+                                                       uint32_t *__restrict__ vals_tmp, uint64_t *__restrict__ keys_out,
+                                                       uint32_t *__restrict__ vals_out, uint32_t *__restrict__ n_eff,
+                                                       uint32_t N)
+{
+    const uint32_t c = blockIdx.x;
+    const size_t base = (size_t)c * N;
+    const uint64_t *kin = mcodes + base;
+    uint64_t *kt = keys_tmp + base;
+    uint32_t *vt = vals_tmp + base;
+    uint64_t *ko = keys_out + base;
+    uint32_t *vo = vals_out + base;
+    const float *boxes = aabbs + (size_t)c * (N + 1) * 8u;
+
+    __shared__ uint32_t hist[8][256];
+    __shared__ uint32_t wc[kSortWaves][256];
+    __shared__ uint32_t s_w[65];
+    __shared__ uint32_t s_nfin;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+
+    for (uint32_t i = tid; i < 8u * 256u; i += kSortBlock) (&hist[0][0])[i] = 0;
+    if (tid == 0) s_nfin = 0;
+    __syncthreads();
+    uint32_t nfin = 0;
+    for (uint32_t e = tid; e < N; e += kSortBlock) {
+        const uint64_t k = kin[e];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) atomicAdd(&hist[p][(uint32_t)(k >> (8 * p)) & 255u], 1u);
+        nfin += is_finite_f(boxes[(size_t)e * 8u]) ? 1u : 0u;
+    }
+    atomicAdd(&s_nfin, nfin);
+    __syncthreads();
+    // exclusive scans of the 8 histograms: wave p scans hist[p] (4 bins per lane)
+    if (w < 8) {
+        uint32_t v[4], s = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = hist[w][lane * 4 + k];
+            s += v[k];
+        }
+        const uint32_t inc = wave_inclusive_scan(s);
+        uint32_t run = inc - s;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            hist[w][lane * 4 + k] = run;
+            run += v[k];
+        }
+    }
+    __syncthreads();
+
+    const uint64_t lt = lanemask_lt();
+    for (int p = 0; p < 8; ++p) {
+        const uint64_t *ksrc = (p == 0) ? kin : ((p & 1) ? kt : ko);
+        const uint32_t *vsrc = (p & 1) ? vt : vo;
+        uint64_t *kdst = (p & 1) ? ko : kt;
+        uint32_t *vdst = (p & 1) ? vo : vt;
+        const uint32_t shift = 8u * (uint32_t)p;
+        for (uint32_t tb = 0; tb < N; tb += kSortBlock * kSortItems) {
+            // zero this wave's digit counters
+#pragma unroll
+            for (int k = 0; k < 4; ++k) wc[w][lane * 4 + k] = 0;
+            uint64_t key[kSortItems];
+            uint32_t val[kSortItems], dig[kSortItems], rank[kSortItems];
+            bool ok[kSortItems];
+#pragma unroll
+            for (int it = 0; it < kSortItems; ++it) {
+                const uint32_t e = tb + w * 64u * kSortItems + it * 64u + lane;
+                ok[it] = e < N;
+                key[it] = ok[it] ? ksrc[e] : 0ull;
+                val[it] = ok[it] ? (p == 0 ? e : vsrc[e]) : 0u;
+                dig[it] = (uint32_t)(key[it] >> shift) & 255u;
+                const uint64_t vmask = __ballot(ok[it]);
+                uint64_t peers = vmask;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const bool bit = (dig[it] >> b) & 1u;
+                    const uint64_t bal = __ballot(bit);
+                    peers &= bit ? bal : ~bal;
+                }
+                const uint32_t before = (uint32_t)__popcll(peers & lt);
+                const uint32_t old = wc[w][dig[it]];
+                rank[it] = old + before;
+                if (ok[it] && before == 0) wc[w][dig[it]] = old + (uint32_t)__popcll(peers);
+            }
+            __syncthreads();
+            uint32_t tile_tot = 0;
+            if (tid < 256) {
+                uint32_t run = 0;
+                for (int ww = 0; ww < kSortWaves; ++ww) {
+                    const uint32_t x = wc[ww][tid];
+                    wc[ww][tid] = run;
+                    run += x;
+                }
+                tile_tot = run;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int it = 0; it < kSortItems; ++it) {
+                if (ok[it]) {
+                    const uint32_t pos = hist[p][dig[it]] + wc[w][dig[it]] + rank[it];
+                    kdst[pos] = key[it];
+                    vdst[pos] = val[it];
+                }
+            }
+            __syncthreads();
+            if (tid < 256) hist[p][tid] += tile_tot;
+        }
+        __syncthreads();
+    }
+    // after 8 passes the result is in (ko, vo)
+    const uint32_t nf = s_nfin;
+    if (tid == 0) n_eff[c] = nf;
+    if (nf == N) return;
+    // first position whose code is all ones
+    uint32_t lo = 0, n = N;
+    while (n > 0) {
+        const uint32_t h = n / 2;
+        if (ko[lo + h] != ~0ull) {
+            lo += h + 1;
+            n -= h + 1;
+        } else {
+            n = h;
+        }
+    }
+    const uint32_t t0 = lo;
+    if (nf == t0) return;  // the tail holds only objects without data
+    // stable partition of the tail [t0, N): finite boxes first (into vt), then copy back
+    uint32_t fin_base = t0, inf_base = nf;
+    for (uint32_t cb = t0; cb < N; cb += kSortBlock) {
+        const uint32_t e = cb + tid;
+        uint32_t v = 0, isf = 0;
+        if (e < N) {
+            v = vo[e];
+            isf = is_finite_f(boxes[(size_t)v * 8u]) ? 1u : 0u;
+        }
+        uint32_t tot;
+        const uint32_t r = block_excl_scan_1024(isf, s_w, tot);
+        if (e < N) {
+            if (isf) {
+                vt[fin_base + r] = v;
+            } else {
+                vt[inf_base + (e - cb - r)] = v;
+            }
+        }
+        fin_base += tot;
+        inf_base += (N - cb < kSortBlock ? N - cb : kSortBlock) - tot;
+    }
+    __syncthreads();
+    for (uint32_t e = t0 + tid; e < N; e += kSortBlock) vo[e] = vt[e];
+}
+
+__global__ __launch_bounds__(256) void k_gather(const float *__restrict__ aabbs, const uint32_t *__restrict__ srt_idx,
+                                                float *__restrict__ srt_aabbs, uint32_t N)
+{
+    const uint32_t bps = (N + 1u + 255u) / 256u;
+    const uint32_t c = blockIdx.x / bps;
+    const uint32_t p = (blockIdx.x - c * bps) * 256u + threadIdx.x;
+    if (p > N) return;
+    const size_t sb = (size_t)c * (N + 1) * 8u;
+    const uint32_t src = (p == N) ? N : srt_idx[(size_t)c * N + p];
+    const float4 *s = reinterpret_cast<const float4 *>(aabbs + sb + (size_t)src * 8u);
+    float4 *d = reinterpret_cast<float4 *>(srt_aabbs + sb + (size_t)p * 8u);
+    d[0] = s[0];
+    d[1] = s[1];
+}
+
+// ---------------------------------------------------------------------------
+// stage 3: BVH (detect_conjunctions_bvh.cpp:267-699), one workgroup per step.
+// Level-synchronous top-down radix split; the split bit of a node is the
+// first bit in which its first and last codes differ, which is where the
+// reference's ++split_idx search stops. Children of a level are numbered by
+// an inclusive scan of the child counts (BFS order, adjacent siblings).
+// ---------------------------------------------------------------------------
+
+constexpr int kBvhBlock = 1024;
+
+__device__ inline void init_node(mzh_bvh_node &n, uint32_t b, uint32_t e)
+{
+    n.begin = b;
+    n.end = e;
+    n.left = -1;
+    n.right = -1;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        n.lb[d] = __builtin_inff();
+        n.ub[d] = -__builtin_inff();
+    }
+}
+
+__global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ srt_mcodes,
+                                                     const float *__restrict__ srt_aabbs,
+                                                     const uint32_t *__restrict__ n_eff, mzh_bvh_node *__restrict__ nodes,
+                                                     uint32_t *__restrict__ nolc_buf, uint32_t *__restrict__ ps_buf,
+                                                     uint32_t *__restrict__ tree_size, uint32_t N, uint64_t node_cap)
+{
+    const uint32_t c = blockIdx.x;
+    const uint32_t n = n_eff[c];
+    const uint32_t tid = threadIdx.x;
+    if (n == 0) {
+        if (tid == 0) tree_size[c] = 0;
+        return;
+    }
+    const uint64_t *codes = srt_mcodes + (size_t)c * N;
+    const float *boxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
+    mzh_bvh_node *T = nodes + (size_t)c * node_cap;
+    uint32_t *nolc = nolc_buf + (size_t)c * N;
+    uint32_t *ps = ps_buf + (size_t)c * N;
+    __shared__ uint32_t lvl[kBvhMaxLevels + 2];
+    __shared__ uint32_t s_w[65];
+
+    if (tid == 0) init_node(T[0], 0, n);
+    __syncthreads();
+    uint32_t lb = 0, le = 1, nlev = 0;
+    for (;;) {
+        if (tid == 0) lvl[nlev] = lb;
+        const uint32_t cnt = le - lb;
+        for (uint32_t k = tid; k < cnt; k += kBvhBlock) {
+            mzh_bvh_node &nd = T[lb + k];
+            const uint32_t b = nd.begin, e = nd.end;
+            uint32_t split = 0;
+            if (e - b > 1u) {
+                const uint64_t x = codes[b] ^ codes[e - 1];
+                if (x != 0) {
+                    const uint32_t bit = 63u - (uint32_t)__clzll(x);
+                    // first position in [b, e) whose code has `bit` set
+                    uint32_t lo = b, m = e - b;
+                    while (m > 0) {
+                        const uint32_t h = m / 2;
+                        if (((codes[lo + h] >> bit) & 1ull) == 0) {
+                            lo += h + 1;
+                            m -= h + 1;
+                        } else {
+                            m = h;
+                        }
+                    }
+                    split = lo - b;
+                }
+            }
+            nolc[k] = split;
+            if (split == 0) {
+                float l[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+                float u[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+                for (uint32_t ob = b; ob != e; ++ob) {
+                    const float4 bl = reinterpret_cast<const float4 *>(boxes + (size_t)ob * 8u)[0];
+                    const float4 bu = reinterpret_cast<const float4 *>(boxes + (size_t)ob * 8u)[1];
+                    l[0] = l[0] < bl.x ? l[0] : bl.x;
+                    l[1] = l[1] < bl.y ? l[1] : bl.y;
+                    l[2] = l[2] < bl.z ? l[2] : bl.z;
+                    l[3] = l[3] < bl.w ? l[3] : bl.w;
+                    u[0] = u[0] > bu.x ? u[0] : bu.x;
+                    u[1] = u[1] > bu.y ? u[1] : bu.y;
+                    u[2] = u[2] > bu.z ? u[2] : bu.z;
+                    u[3] = u[3] > bu.w ? u[3] : bu.w;
+                }
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    nd.lb[d] = l[d];
+                    nd.ub[d] = u[d];
+                }
+            }
+        }
+        __syncthreads();
+        // inclusive scan of the child counts (0 or 2)
+        uint32_t carry = 0;
+        for (uint32_t cb = 0; cb < cnt; cb += kBvhBlock) {
+            const uint32_t k = cb + tid;
+            const uint32_t v = (k < cnt && nolc[k] != 0) ? 2u : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan_1024(v, s_w, tot);
+            if (k < cnt) ps[k] = carry + ex + v;
+            carry += tot;
+        }
+        __syncthreads();
+        const uint32_t total = carry;
+        // A tree over n objects has at most 2n - 1 nodes, so this cannot fire
+        // with node_cap >= 2N; it keeps every write in bounds regardless.
+        if (le + total > node_cap) {
+            if (tid == 0) tree_size[c] = 0xffffffffu;
+            return;
+        }
+        for (uint32_t k = tid; k < cnt; k += kBvhBlock) {
+            const uint32_t sp = nolc[k];
+            if (sp != 0) {
+                mzh_bvh_node &nd = T[lb + k];
+                const uint32_t lc = le + ps[k] - 2u;
+                nd.left = (int32_t)lc;
+                nd.right = (int32_t)(lc + 1u);
+                init_node(T[lc], nd.begin, nd.begin + sp);
+                init_node(T[lc + 1], nd.begin + sp, nd.end);
+            }
+        }
+        __syncthreads();
+        ++nlev;
+        lb = le;
+        le = le + total;
+        if (total == 0 || nlev > kBvhMaxLevels) break;
+    }
+    if (tid == 0) lvl[nlev] = le;
+    __syncthreads();
+    // bottom-up refit, one level at a time
+    for (int j = (int)nlev - 2; j >= 0; --j) {
+        const uint32_t a = lvl[j], b = lvl[j + 1];
+        for (uint32_t ni = a + tid; ni < b; ni += kBvhBlock) {
+            mzh_bvh_node &nd = T[ni];
+            if (nd.left != -1) {
+                const mzh_bvh_node &L = T[nd.left];
+                const mzh_bvh_node &R = T[nd.right];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) {
+                    nd.lb[d] = L.lb[d] < R.lb[d] ? L.lb[d] : R.lb[d];
+                    nd.ub[d] = L.ub[d] > R.ub[d] ? L.ub[d] : R.ub[d];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) tree_size[c] = le;
+}
+
+// ---------------------------------------------------------------------------
+// stage 4: broad phase (detect_conjunctions_broad_phase.cpp:153-339)
+// One lane per (step, Morton-sorted query object); stack traversal with
+// closed-interval 4-D overlap; a pair (i, j) is emitted from the query of the
+// lower original index i. Counts go to cnt[step][i]; an exclusive scan over i
+// then places every query's pairs in (i, j) order without a global sort.
+// ---------------------------------------------------------------------------
+
+constexpr int kStackDepth = 72;
+
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_bp(const mzh_bvh_node *__restrict__ nodes,
+                                            const uint32_t *__restrict__ tree_size, const float *__restrict__ srt_aabbs,
+                                            const uint32_t *__restrict__ srt_idx, const uint32_t *__restrict__ n_eff,
+                                            const int32_t *__restrict__ otypes, uint32_t *__restrict__ cnt,
+                                            const uint64_t *__restrict__ off, const uint64_t *__restrict__ step_off,
+                                            mzh_aabb_collision *__restrict__ pairs, uint32_t N, uint64_t node_cap)
+{
+    const uint32_t bps = (N + 255u) / 256u;
+    const uint32_t c = blockIdx.x / bps;
+    const uint32_t o = (blockIdx.x - c * bps) * 256u + threadIdx.x;
+    if (o >= n_eff[c] || tree_size[c] == 0) return;
+    const mzh_bvh_node *T = nodes + (size_t)c * node_cap;
+    const uint32_t *vidx = srt_idx + (size_t)c * N;
+    const float4 ql = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[0];
+    const float4 qu = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[1];
+    const uint32_t orig = vidx[o];
+    const int32_t ot = otypes[orig];
+    uint32_t count = 0;
+    mzh_aabb_collision *out = nullptr;
+    if constexpr (WRITE) out = pairs + step_off[c] + off[(size_t)c * N + orig];
+    int32_t stack[kStackDepth];
+    int sp = 0;
+    stack[sp++] = 0;
+    do {
+        const mzh_bvh_node &nd = T[stack[--sp]];
+        const float4 nl = *reinterpret_cast<const float4 *>(nd.lb);
+        const float4 nu = *reinterpret_cast<const float4 *>(nd.ub);
+        const bool overlap = (qu.x >= nl.x && ql.x <= nu.x) && (qu.y >= nl.y && ql.y <= nu.y)
+                             && (qu.z >= nl.z && ql.z <= nu.z) && (qu.w >= nl.w && ql.w <= nu.w);
+        if (overlap) {
+            if (nd.left == -1) {
+                for (uint32_t k = nd.begin; k != nd.end; ++k) {
+                    const uint32_t oi = vidx[k];
+                    if (orig >= oi) continue;
+                    if (ot + otypes[oi] <= 3) {
+                        if constexpr (WRITE) out[count] = mzh_aabb_collision{orig, oi};
+                        ++count;
+                    }
+                }
+            } else if (sp + 2 <= kStackDepth) {
+                stack[sp++] = nd.left;
+                stack[sp++] = nd.right;
+            }
+        }
+    } while (sp != 0);
+    if constexpr (WRITE) {
+        // insertion sort of this query's pairs by j
+        for (uint32_t a = 1; a < count; ++a) {
+            const mzh_aabb_collision x = out[a];
+            uint32_t b = a;
+            while (b > 0 && out[b - 1].j > x.j) {
+                out[b] = out[b - 1];
+                --b;
+            }
+            out[b] = x;
+        }
+    } else {
+        cnt[(size_t)c * N + orig] = count;
+    }
+}
+
+// per-step exclusive scan of cnt[step][0..N) -> off, step totals
+__global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t *__restrict__ cnt, uint64_t *__restrict__ off,
+                                                      uint64_t *__restrict__ step_total, uint32_t N)
+{
+    const uint32_t c = blockIdx.x;
+    __shared__ uint32_t s_w[65];
+    uint64_t carry = 0;
+    for (uint32_t cb = 0; cb < N; cb += 1024) {
+        const uint32_t e = cb + threadIdx.x;
+        const uint32_t v = e < N ? cnt[(size_t)c * N + e] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan_1024(v, s_w, tot);
+        if (e < N) off[(size_t)c * N + e] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) step_total[c] = carry;
+}
+
+// exclusive scan over the step totals -> step_off[0..C], grand total
+__global__ __launch_bounds__(1024) void k_scan_steps(const uint64_t *__restrict__ step_total,
+                                                     uint64_t *__restrict__ step_off, uint64_t *__restrict__ grand,
+                                                     uint32_t C)
+{
+    __shared__ uint64_t s_w[17];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint64_t carry = 0;
+    for (uint32_t cb = 0; cb < C; cb += 1024) {
+        const uint32_t e = cb + threadIdx.x;
+        const uint64_t v = e < C ? step_total[e] : 0ull;
+        uint64_t inc = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint64_t t = __shfl_up(inc, d, 64);
+            if (lane >= (uint32_t)d) inc += t;
+        }
+        if (lane == 63) s_w[w] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t run = 0;
+            for (int k = 0; k < 16; ++k) {
+                const uint64_t x = s_w[k];
+                s_w[k] = run;
+                run += x;
+            }
+            s_w[16] = run;
+        }
+        __syncthreads();
+        if (e < C) step_off[e] = carry + s_w[w] + inc - v;
+        carry += s_w[16];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        step_off[C] = carry;
+        *grand = carry;
+    }
+}
+
+template <int ORDER>
+hipError_t launch_aabbs_order(const PjView &pj, const StepGrid &sg, double radius, int mode, float *aabbs,
+                              uint32_t *gbox_ord, DevError *err, hipStream_t stream)
+{
+    const dim3 grid(((pj.n_objs + 255u) / 256u) * sg.n_steps);
+    hipLaunchKernelGGL(k_aabbs<ORDER>, grid, dim3(256), 0, stream, pj, sg, radius, mode, aabbs, gbox_ord, err);
+    return hipGetLastError();
+}
+
+template <int ORDER>
+hipError_t dispatch_aabbs(int order, const PjView &pj, const StepGrid &sg, double radius, int mode, float *aabbs,
+                          uint32_t *gbox_ord, DevError *err, hipStream_t stream)
+{
+    if (order == ORDER) return launch_aabbs_order<ORDER>(pj, sg, radius, mode, aabbs, gbox_ord, err, stream);
+    if constexpr (ORDER < kMaxOrder) {
+        return dispatch_aabbs<ORDER + 1>(order, pj, sg, radius, mode, aabbs, gbox_ord, err, stream);
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+
+hipError_t launch_transpose_cf4(const double *traj, double *cf4, uint64_t n_steps_total, uint32_t order,
+                                hipStream_t stream)
+{
+    const uint64_t n = n_steps_total * 4ull * (order + 1u);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_transpose_cf4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, traj, cf4,
+                       n_steps_total, order + 1u);
+    return hipGetLastError();
+}
+
+hipError_t launch_aabbs(const PjView &pj, const StepGrid &sg, double conj_thresh, int mode, float *aabbs,
+                        uint32_t *gbox_ord, DevError *err, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_init_gbox, dim3((sg.n_steps * 8u + 255u) / 256u), dim3(256), 0, stream, gbox_ord,
+                       sg.n_steps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return dispatch_aabbs<kMinOrder>((int)pj.order, pj, sg, conj_thresh / 2, mode, aabbs, gbox_ord, err, stream);
+}
+
+hipError_t launch_gbox_finalize(const uint32_t *gbox_ord, float *aabbs, uint32_t n_objs, uint32_t n_steps,
+                                DevError *err, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_gbox_finalize, dim3((n_steps + 255u) / 256u), dim3(256), 0, stream, gbox_ord, aabbs, n_objs,
+                       n_steps, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_morton(const float *aabbs, uint64_t *mcodes, uint32_t n_objs, uint32_t n_steps, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_morton, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, aabbs, mcodes, n_objs);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort(const uint64_t *mcodes, const float *aabbs, uint64_t *keys_tmp, uint32_t *vals_tmp,
+                       uint64_t *srt_mcodes, uint32_t *srt_idx, uint32_t *n_eff, uint32_t n_objs, uint32_t n_steps,
+                       hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_sort, dim3(n_steps), dim3(kSortBlock), 0, stream, mcodes, aabbs, keys_tmp, vals_tmp,
+                       srt_mcodes, srt_idx, n_eff, n_objs);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt_aabbs, uint32_t n_objs,
+                         uint32_t n_steps, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_gather, dim3(((n_objs + 1u + 255u) / 256u) * n_steps), dim3(256), 0, stream, aabbs, srt_idx,
+                       srt_aabbs, n_objs);
+    return hipGetLastError();
+}
+
+hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, mzh_bvh_node *nodes,
+                      uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap,
+                      uint32_t n_steps, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_bvh, dim3(n_steps), dim3(kBvhBlock), 0, stream, srt_mcodes, srt_aabbs, n_eff, nodes,
+                       nolc_buf, ps_buf, tree_size, n_objs, node_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_count(const mzh_bvh_node *nodes, const uint32_t *tree_size, const float *srt_aabbs,
+                           const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, uint32_t *cnt,
+                           uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
+{
+    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (size_t)n_objs * n_steps, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bp<false>, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, nodes, tree_size,
+                       srt_aabbs, srt_idx, n_eff, otypes, cnt, nullptr, nullptr, nullptr, n_objs, node_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_counts(const uint32_t *cnt, uint64_t *off, uint64_t *step_total, uint32_t n_objs,
+                              uint32_t n_steps, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_scan_counts, dim3(n_steps), dim3(1024), 0, stream, cnt, off, step_total, n_objs);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uint64_t *grand_total, uint32_t n_steps,
+                             hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_scan_steps, dim3(1), dim3(1024), 0, stream, step_total, step_off, grand_total, n_steps);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_write(const mzh_bvh_node *nodes, const uint32_t *tree_size, const float *srt_aabbs,
+                           const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, const uint64_t *off,
+                           const uint64_t *step_off, mzh_aabb_collision *pairs, uint32_t n_objs, uint64_t node_cap,
+                           uint32_t n_steps, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_bp<true>, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, nodes, tree_size,
+                       srt_aabbs, srt_idx, n_eff, otypes, nullptr, off, step_off, pairs, n_objs, node_cap);
+    return hipGetLastError();
+}
+
+} // namespace mzh

@@ -522,7 +522,13 @@ static int root_finding_impl(const double *poly, uint32_t order, double rf_int, 
             }                                                                                                          \
             if (sgn_d(der_) != direction) accept_ = 0;                                                                 \
         }                                                                                                              \
-        if (accept_) roots[(*n_roots)++] = root_;                                                                      \
+        if (accept_) {                                                                                                 \
+            if (*n_roots >= 2 * order + 2) {                                                                           \
+                set_err(err, errlen, "Too many roots detected for objects %u and %u", i, j);                          \
+                return MZO_EINVAL;                                                                                     \
+            }                                                                                                          \
+            roots[(*n_roots)++] = root_;                                                                               \
+        }                                                                                                              \
     } while (0)
 
     /* rescale [0, rf_int) -> [0, 1) */
@@ -1082,7 +1088,7 @@ static int narrow_phase(const mzo_pj *pj, const mzo_aabb_collision *bp, size_t n
     const uint32_t order = pj->order, op1 = order + 1;
     const double thresh2 = conj_thresh * conj_thresh;
     double pti[MZO_MAX_OP1 * 7], ptj[MZO_MAX_OP1 * 7];
-    double nodes[MZO_MAX_OP1], d2v[MZO_MAX_OP1], ptd2[MZO_MAX_OP1], ptd2p[MZO_MAX_OP1], roots[MZO_MAX_OP1];
+    double nodes[MZO_MAX_OP1], d2v[MZO_MAX_OP1], ptd2[MZO_MAX_OP1], ptd2p[MZO_MAX_OP1], roots[2 * MZO_MAX_OP1 + 2];
     out->n = 0;
     for (size_t b = 0; b < n_bp; ++b) {
         const uint32_t i = bp[b].i, j = bp[b].j;

@@ -323,7 +323,7 @@ def vm_interp(alpha, f):
 def root_finding(poly, rf_int, direction=1):
     poly = np.ascontiguousarray(poly, dtype=np.float64)
     order = len(poly) - 1
-    roots = np.zeros(order + 2)
+    roots = np.zeros(2 * order + 2)
     nr = ct.c_uint32()
     fex = ct.c_int()
     st = lib().mzo_root_finding(_dptr(poly), order, float(rf_int), direction, _dptr(roots), ct.byref(nr), ct.byref(fex))

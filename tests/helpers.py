@@ -1,0 +1,103 @@
+"""Shared test helpers: reference closed-form trajectories and comparisons."""
+
+import numpy as np
+
+GOLDEN = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+def planar_circ():
+    g = np.load(GOLDEN / "planar_circ.npz")
+    return g["tcs"], g["times"]
+
+
+def linear_traj(tms, sgn):
+    """Objects on the x axis at +-1 moving towards the origin at unit speed
+    (test/test_conjunctions.py:1076-1324); order 3."""
+    out = []
+    for tm in tms[1:]:
+        td = np.zeros((7, 4))
+        td[0, 0] = sgn * (1.0 - (tm - 0.1))
+        td[0, 1] = -1.0 * sgn
+        td[3, 0] = -1.0 * sgn
+        out.append(td)
+    return np.ascontiguousarray(np.array(out).transpose((0, 2, 1)))
+
+
+TM21 = np.array([round(0.1 * k, 10) for k in range(21)])
+
+# (slice of object 0, slice of object 1, number of conjunctions expected)
+CD_BEGIN_END_CASES = [
+    ((0, 10), (11, 21), 0),
+    ((0, 12), (9, 21), 1),
+    ((11, 21), (12, 21), 0),
+    ((9, 21), (8, 21), 1),
+    ((0, 9), (0, 8), 0),
+    ((0, 11), (0, 12), 1),
+]
+
+
+def cd_begin_end_case(k):
+    (a0, b0), (a1, b1), exp = CD_BEGIN_END_CASES[k]
+    t0, t1 = TM21[a0:b0], TM21[a1:b1]
+    return [linear_traj(t0, 1), linear_traj(t1, -1)], [t0, t1], exp
+
+
+def boundary_incoming(shift=0.0):
+    """test/test_boundary_conjunctions.py:24-123."""
+    t0 = np.array([0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9])
+    t1 = np.array([round(0.1 * k, 10) for k in range(14)])
+    tr = [[], []]
+    for tm in t1[1:]:
+        td = np.zeros((7, 4))
+        td[0, 0] = 1.0 - (tm - 0.1)
+        td[0, 1] = -1.0
+        td[3, 0] = -1.0
+        if tm <= 0.9:
+            tr[0].append(td.T)
+        tr[1].append((-td).T)
+    return [np.ascontiguousarray(np.array(x)) for x in tr], [t0 + shift, t1 + shift]
+
+
+def boundary_outgoing(shift=0.0):
+    """test/test_boundary_conjunctions.py:125-198."""
+    tm_data = np.array([0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9])
+    tr = [[], []]
+    for tm in tm_data[1:]:
+        td = np.zeros((7, 4))
+        td[0, 0] = 0.1 + (tm - 0.1)
+        td[0, 1] = 1.0
+        td[3, 0] = 1.0
+        tr[0].append(td.T)
+        tr[1].append((-td).T)
+    return [np.ascontiguousarray(np.array(x)) for x in tr], [tm_data + shift, tm_data + shift]
+
+
+def zero_aabbs_pj():
+    """test/test_conjunctions.py:586-626."""
+    tdata = np.zeros((7, 6))
+    tdata[:3, 0] = 1.0
+    tdata[6, 0] = np.sqrt(3.0)
+    tdata = np.ascontiguousarray(tdata.transpose())
+    return [np.array([tdata, tdata, tdata])], [np.array([0.0, 1.0, 2.0, 3.0])]
+
+
+def axis_objects():
+    """Eight order-5 constant objects of test_bvh / test_no_traj_data
+    (test/test_conjunctions.py:628-784)."""
+    objs = []
+    for ch, v in [(0, 1.0), (0, -1.0), (1, 1.0), (1, -1.0), (2, 1.0), (2, -1.0)]:
+        t = np.zeros((6, 7))
+        t[0, ch] = v
+        objs.append(t)
+    objs.append(np.zeros((6, 7)))
+    t7 = np.zeros((6, 7))
+    t7[0, :] = 1
+    objs.append(t7)
+    return objs
+
+
+def flat_of(pj):
+    from oracle import pyoracle as po
+
+    traj, time_, toff, ns, tmo = pj._flat()
+    return po.Flat.from_arrays(pj.poly_order, traj, time_, ns, toff, tmo, pj.maxT)

@@ -1,0 +1,295 @@
+"""GPU parity: the HIP pipeline against the CPU oracle on the same inputs.
+
+Bar (FP64 drop-in mode): AABBs, Morton codes, sort order, BVH nodes and
+broad-phase pairs bit-exact; conjunction (i, j) lists identical with
+|dTCA| <= 1e-3 s and |dDCA| <= 1e-3 km (and, in practice, bit-exact too --
+checked separately). Mixed mode: boxes contain the FP64 boxes, the pair set is
+a superset and the conjunction list is identical.
+"""
+
+import numpy as np
+import pytest
+
+import mizuba_amd as mz
+
+from . import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+DAY_S = 86400.0
+TCA_TOL_DAYS = 1e-3 / DAY_S  # 1e-3 s
+DCA_TOL_KM = 1e-3
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint8)
+
+
+def assert_screen_equal(c, ref, trees=True, pairs=True):
+    C = c.n_cd_steps
+    assert C == ref["n_cd_steps"]
+    np.testing.assert_array_equal(c.cd_end_times, ref["cd_end_times"])
+    for name in ("aabbs", "srt_aabbs", "mcodes", "srt_mcodes", "srt_idx"):
+        g, r = getattr(c, name), ref[name]
+        assert g.shape == r.shape, name
+        if not np.array_equal(_bits(g), _bits(r)):
+            bad = np.argwhere(g != r)
+            raise AssertionError(f"{name} differs at {bad[:5].tolist()} ({len(bad)} entries)")
+    if trees:
+        from oracle import pyoracle as po
+
+        for k in range(C):
+            g, r = c.get_bvh_tree(k), po.get_bvh_tree(ref, k)
+            assert len(g) == len(r), (k, len(g), len(r))
+            assert np.array_equal(_bits(g), _bits(r)), f"tree of step {k} differs"
+    if pairs:
+        from oracle import pyoracle as po
+
+        for k in range(C):
+            g, r = c.get_aabb_collisions(k), po.get_aabb_collisions(ref, k)
+            assert len(g) == len(r), (k, len(g), len(r))
+            assert np.array_equal(_bits(g), _bits(r)), f"pairs of step {k} differ"
+
+
+def assert_conj_equal(g, r, bitwise=True):
+    assert len(g) == len(r), (len(g), len(r))
+    np.testing.assert_array_equal(g["i"], r["i"])
+    np.testing.assert_array_equal(g["j"], r["j"])
+    if len(g):
+        assert np.max(np.abs(g["tca"] - r["tca"])) <= TCA_TOL_DAYS
+        assert np.max(np.abs(g["dca"] - r["dca"])) <= DCA_TOL_KM
+    if bitwise:
+        assert np.array_equal(_bits(g), _bits(r)), "conjunction records differ in their bits"
+
+
+def run_both(trajs, times, thresh, interval, otypes=None, mode="fp64", status=None):
+    from oracle import pyoracle as po
+
+    pj = mz.polyjectory(trajs, times, status if status is not None else [0] * len(trajs))
+    c = mz.conjunctions(pj, thresh, interval, otypes=otypes, mode=mode)
+    ref = po.run((trajs, times), thresh, interval, otypes=otypes)
+    return c, ref
+
+
+# ---- reference closed-form cases -------------------------------------------
+
+
+def test_zero_aabbs_exact():
+    trajs, times = H.zero_aabbs_pj()
+    c, ref = run_both(trajs, times, float(np.finfo(float).eps), 0.1)
+    a = c.aabbs
+    assert np.all(a[:, :, 0, :3] == np.nextafter(np.single(1), np.single("-inf")))
+    assert np.all(a[:, :, 1, :3] == np.nextafter(np.single(1), np.single("+inf")))
+    assert np.all(a[:, :, 0, 3] == np.nextafter(np.single(np.sqrt(3.0)), np.single("-inf")))
+    assert np.all(a[:, :, 1, 3] == np.nextafter(np.single(np.sqrt(3.0)), np.single("+inf")))
+    assert_screen_equal(c, ref)
+
+
+@pytest.mark.parametrize("interval", [0.01, 0.1, 0.5, 2.0, 5.0, 7.0, 42.0])
+def test_planar_circ(interval):
+    tcs, tms = H.planar_circ()
+    c, ref = run_both([tcs], [tms], 0.1, interval)
+    assert c.cd_end_times[-1] == tms[-1]
+    a = c.aabbs
+    assert np.all(a[:, 0] == a[:, 1])
+    assert np.all(a == c.srt_aabbs)
+    assert np.all(a[:, 0, 0, 2] >= -0.05001) and np.all(a[:, 0, 1, 2] <= 0.05001)
+    assert np.all(a[:, 0, 0, 3] >= 1 - 0.05001) and np.all(a[:, 0, 1, 3] <= 1 + 0.05001)
+    assert len(c.conjunctions) == 0
+    assert_screen_equal(c, ref)
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_cd_begin_end(case):
+    trajs, times, exp = H.cd_begin_end_case(case)
+    c, ref = run_both(trajs, times, 1e-6, 2.0 / 3.0)
+    cj = c.conjunctions
+    assert_conj_equal(cj, ref["conjunctions"])
+    assert_screen_equal(c, ref)
+    if exp == 0:
+        assert len(cj) == 0
+    else:
+        assert len(cj) >= 1
+        assert np.all(cj["i"] == 0) and np.all(cj["j"] == 1)
+        assert abs(cj["tca"][0] - 1.0) < 1e-15
+        assert abs(cj["dca"][0]) < 1e-15
+        assert np.allclose(cj["vi"][0], [-1, 0, 0], atol=1e-15, rtol=0.0)
+        assert np.allclose(cj["vj"][0], [1, 0, 0], atol=1e-15, rtol=0.0)
+
+
+@pytest.mark.parametrize("shift", [0.0, 1.1])
+def test_boundary_incoming(shift):
+    trajs, times = H.boundary_incoming(shift)
+    c, ref = run_both(trajs, times, 0.25, 0.03)
+    cj = c.conjunctions
+    assert len(cj) == 1
+    assert abs(cj["tca"][0] - (0.9 + shift)) < 1e-15
+    assert abs(cj["dca"][0] - 0.2) < 1e-15
+    assert np.allclose(cj["ri"][0], [0.1, 0, 0], rtol=1e-15, atol=0.0)
+    assert np.allclose(cj["rj"][0], [-0.1, 0, 0], rtol=1e-15, atol=0.0)
+    assert_conj_equal(cj, ref["conjunctions"])
+    assert_screen_equal(c, ref)
+
+
+@pytest.mark.parametrize("shift", [0.0, 1.1])
+def test_boundary_outgoing(shift):
+    trajs, times = H.boundary_outgoing(shift)
+    c, ref = run_both(trajs, times, 0.25, 0.03)
+    cj = c.conjunctions
+    assert len(cj) == 1
+    assert abs(cj["tca"][0] - shift) < 1e-15
+    assert abs(cj["dca"][0] - 0.2) < 1e-15
+    assert np.allclose(cj["vi"][0], [1, 0, 0], rtol=1e-15, atol=0.0)
+    assert_conj_equal(cj, ref["conjunctions"])
+    assert_screen_equal(c, ref)
+
+
+def test_bvh_cases():
+    t = np.zeros((6, 7))
+    t[1, :] = 0.1
+    c, ref = run_both([np.array([t])], [np.array([0.0, 1.0])], 1e-16, 1.0)
+    assert len(c.get_bvh_tree(0)) == 1
+    with pytest.raises(IndexError) as ei:
+        c.get_bvh_tree(1)
+    assert (
+        "Cannot fetch the BVH tree for the conjunction timestep at index 1: the total number of conjunction steps is only 1"
+        in str(ei.value)
+    )
+    c, ref = run_both([np.array([t]), np.array([t])], [np.array([0.0, 1.0])] * 2, 1e-16, 1.0)
+    tr = c.get_bvh_tree(0)
+    assert len(tr) == 1 and tr[0]["begin"] == 0 and tr[0]["end"] == 2 and tr[0]["left"] == -1
+    assert_screen_equal(c, ref)
+    objs = H.axis_objects()
+    t8 = np.zeros((6, 7))
+    t8[0, :] = 1
+    t8[0, 0] = 1.0 - 2.1 / 2**16
+    objs.append(t8)
+    c, ref = run_both([np.array([o]) for o in objs], [np.array([0.0, 1.0])] * 9, 1e-16, 1.0)
+    assert c.mcodes[0, -2] == 2**64 - 1
+    assert c.mcodes[0, -1] == 2**64 - 2
+    assert c.srt_idx[0, -1] == 7 and c.srt_idx[0, -2] == 8
+    assert_screen_equal(c, ref)
+
+
+def test_no_traj_data():
+    objs = H.axis_objects()
+    trajs = [np.array([objs[0]])] * 10 + [np.array([o, o]) for o in objs]
+    times = [np.array([0.0, 1.0])] * 10 + [np.array([0.0, 1.0, 2.0])] * 8
+    c, ref = run_both(trajs, times, 1e-16, 1.0)
+    assert np.all(np.isinf(c.aabbs[1, :10]))
+    assert np.all(np.isinf(c.srt_aabbs[1, -11:-1]))
+    assert np.all(c.mcodes[1, :10] == 2**64 - 1)
+    assert np.all(c.srt_mcodes[1, -11:] == 2**64 - 1)
+    assert_screen_equal(c, ref)
+
+
+def test_empty_and_late_trajectories():
+    tcs, tms = H.planar_circ()
+    empty = np.zeros((0, tcs.shape[1], 7))
+    for trajs, times in [
+        ([tcs, empty], [tms, np.zeros(0)]),
+        ([empty, tcs, empty], [np.zeros(0), tms, np.zeros(0)]),
+        ([empty, empty, tcs], [np.zeros(0), np.zeros(0), tms]),
+        ([tcs], [tms + 1.0]),
+    ]:
+        c, ref = run_both(trajs, times, 1.0, 0.1)
+        assert len(c.conjunctions) == 0
+        assert_screen_equal(c, ref)
+
+
+# ---- synthetic catalogs ------------------------------------------------------
+
+
+@pytest.fixture(scope="module")
+def leo1000():
+    return mz.make_synthetic_polyjectory(1000, 1.0, seed=11, frac_high=0.0)
+
+
+def test_synthetic_1000_full_day(leo1000):
+    from oracle import pyoracle as po
+
+    pj = leo1000
+    c = mz.conjunctions(pj, 5.0, 3.0 / 1440.0)
+    ref = po.run(H.flat_of(pj), 5.0, 3.0 / 1440.0)
+    assert c.n_cd_steps == 480
+    assert_screen_equal(c, ref)
+    assert_conj_equal(c.conjunctions, ref["conjunctions"])
+    assert c.np_report == ref["np_report"]
+    assert len(c.conjunctions) > 0
+
+
+def test_synthetic_mix_otypes():
+    from oracle import pyoracle as po
+
+    pj = mz.make_synthetic_polyjectory(3000, 0.25, seed=5, frac_high=0.2)
+    rng = np.random.default_rng(3)
+    ot = rng.choice([1, 2, 4], size=pj.n_objs, p=[0.5, 0.4, 0.1]).astype(np.int32)
+    c = mz.conjunctions(pj, 10.0, 2.0 / 1440.0, otypes=ot)
+    ref = po.run(H.flat_of(pj), 10.0, 2.0 / 1440.0, otypes=ot)
+    assert_screen_equal(c, ref)
+    assert_conj_equal(c.conjunctions, ref["conjunctions"])
+    cj = c.conjunctions
+    assert np.all(ot[cj["i"]] + ot[cj["j"]] <= 3)
+    for ot_all in (np.full(pj.n_objs, 2, np.int32), np.full(pj.n_objs, 4, np.int32)):
+        c2 = mz.conjunctions(pj, 10.0, 2.0 / 1440.0, otypes=ot_all)
+        assert all(len(c2.get_aabb_collisions(k)) == 0 for k in range(c2.n_cd_steps))
+        assert len(c2.conjunctions) == 0
+
+
+def test_mixed_mode_superset(leo1000):
+    pj = leo1000
+    c64 = mz.conjunctions(pj, 5.0, 3.0 / 1440.0, mode="fp64")
+    cmx = mz.conjunctions(pj, 5.0, 3.0 / 1440.0, mode="mixed")
+    a64, amx = c64.aabbs, cmx.aabbs
+    fin = np.isfinite(a64)
+    assert np.array_equal(fin, np.isfinite(amx))
+    assert np.all(amx[:, :, 0][fin[:, :, 0]] <= a64[:, :, 0][fin[:, :, 0]])
+    assert np.all(amx[:, :, 1][fin[:, :, 1]] >= a64[:, :, 1][fin[:, :, 1]])
+    n64 = sum(len(c64.get_aabb_collisions(k)) for k in range(c64.n_cd_steps))
+    nmx = sum(len(cmx.get_aabb_collisions(k)) for k in range(cmx.n_cd_steps))
+    assert nmx >= n64
+    assert_conj_equal(cmx.conjunctions, c64.conjunctions)
+
+
+def test_errors():
+    tcs, tms = H.planar_circ()
+    pj = mz.polyjectory([tcs], [tms], [0])
+    with pytest.raises(ValueError, match="The conjunction threshold must be finite and positive"):
+        mz.conjunctions(pj, 0.0, 0.1)
+    with pytest.raises(ValueError, match="is too large and results in an overflow error"):
+        mz.conjunctions(pj, float(np.finfo(float).max), 0.1)
+    with pytest.raises(ValueError, match="The conjunction detection interval must be finite and positive"):
+        mz.conjunctions(pj, 1.0, float("nan"))
+    with pytest.raises(ValueError) as ei:
+        mz.conjunctions(pj, 0.1, 0.1, otypes=[])
+    assert (
+        "Invalid array of object types passed to the constructor of a conjunctions objects: the expected size is 1, "
+        "but the actual size is 0 instead" in str(ei.value)
+    )
+    with pytest.raises(ValueError) as ei:
+        mz.conjunctions(pj, 0.1, 0.1, otypes=[5])
+    assert "The value of an object type must be one of [1, 2, 4], but a value of 5 was detected instead" in str(ei.value)
+    c = mz.conjunctions(pj, 0.1, 0.1)
+    with pytest.raises(IndexError, match="Cannot fetch the list of AABB collisions"):
+        c.get_aabb_collisions(c.n_cd_steps)
+
+
+def test_accessors_readonly_and_refcount():
+    import sys
+
+    tcs, tms = H.planar_circ()
+    pj = mz.polyjectory([tcs], [tms], [0])
+    c = mz.conjunctions(pj, 1.0, 0.1)
+    for name in ("aabbs", "cd_end_times", "srt_aabbs", "mcodes", "srt_mcodes", "srt_idx", "otypes"):
+        rc = sys.getrefcount(c)
+        arr = getattr(c, name)
+        assert sys.getrefcount(c) == rc + 1
+        with pytest.raises(ValueError):
+            arr[:] = arr
+        with pytest.raises(AttributeError):
+            setattr(c, name, arr)
+    assert isinstance(c.bvh_node, np.dtype) and c.bvh_node.itemsize == 48
+    assert c.aabb_collision.itemsize == 8 and c.conj.itemsize == 120
+    assert c.conj_thresh == 1.0 and c.conj_det_interval == 0.1
+    c.hint_release()
