@@ -152,7 +152,9 @@ SynthOut make_synthetic(const SynthParams &p)
         tm.push_back(0.0);
         std::vector<double> x(op1), st(7), c(7 * (size_t)op1);
         while (t0 < p.days) {
-            double h = std::min(h0, p.days - t0);
+            // a remainder shorter than a fifth of a step is absorbed by the last
+            // step instead of becoming an ill-conditioned sliver
+            double h = (p.days - t0 < 1.2 * h0) ? p.days - t0 : h0;
             for (;;) {
                 for (int k = 0; k < op1; ++k) x[k] = kfrac[k] * h;
                 for (int k = 0; k < op1; ++k) {

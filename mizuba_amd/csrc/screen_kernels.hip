@@ -247,11 +247,18 @@ __global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double ra
                         } else {
                             float emin, emax;
                             cs_enclosure_fp32<ORDER>(cf + d * OP1, h_lb, h_ub, emin, emax);
-                            // emin <= FP64 enclosure minimum (as reals); the same
+                            double dmin = (double)emin, dmax = (double)emax;
+                            if (!is_finite_f(emin) || !is_finite_f(emax)) {
+                                // coefficients or intermediates outside the float
+                                // range (ill-conditioned short steps): use the FP64
+                                // enclosure for this piece
+                                cs_enclosure_fp64<ORDER>(cf + d * OP1, h_lb, h_ub, dmin, dmax);
+                            }
+                            // dmin <= FP64 enclosure minimum (as reals); the same
                             // monotone double subtraction and outward float
                             // conversion then keep this box around the FP64 box.
-                            fl = next_down_f((float)((double)emin - radius));
-                            fu = next_up_f((float)((double)emax + radius));
+                            fl = next_down_f((float)(dmin - radius));
+                            fu = next_up_f((float)(dmax + radius));
                         }
                         if (!is_finite_f(fl)) {
                             report_error(err, kErrAabbLb, o, 0, (double)fl);

@@ -252,6 +252,25 @@ def test_mixed_mode_superset(leo1000):
     assert_conj_equal(cmx.conjunctions, c64.conjunctions)
 
 
+def test_mixed_mode_ill_conditioned_step():
+    """A sliver trajectory step whose coefficients overflow FP32 takes the FP64
+    enclosure in mixed mode: boxes still contain the FP64 ones."""
+    tcs, tms = H.planar_circ()
+    sliver = np.zeros((1,) + tcs.shape[1:])
+    sliver[0, 0, :] = tcs[-1, 0, :]
+    sliver[0, -1, 0] = 1e41  # far outside the float range
+    trajs = [np.concatenate([tcs, sliver]), tcs]
+    times = [np.append(tms, tms[-1] + 1e-9), tms]
+    c64 = mz.conjunctions(mz.polyjectory(trajs, times, [0, 0]), 0.5, 0.7, mode="fp64")
+    cmx = mz.conjunctions(mz.polyjectory(trajs, times, [0, 0]), 0.5, 0.7, mode="mixed")
+    a64, amx = c64.aabbs, cmx.aabbs
+    fin = np.isfinite(a64)
+    assert np.array_equal(fin, np.isfinite(amx))
+    assert np.all(amx[:, :, 0][fin[:, :, 0]] <= a64[:, :, 0][fin[:, :, 0]])
+    assert np.all(amx[:, :, 1][fin[:, :, 1]] >= a64[:, :, 1][fin[:, :, 1]])
+    assert_conj_equal(cmx.conjunctions, c64.conjunctions)
+
+
 def test_errors():
     tcs, tms = H.planar_circ()
     pj = mz.polyjectory([tcs], [tms], [0])
@@ -289,6 +308,7 @@ def test_accessors_readonly_and_refcount():
             arr[:] = arr
         with pytest.raises(AttributeError):
             setattr(c, name, arr)
+        del arr
     assert isinstance(c.bvh_node, np.dtype) and c.bvh_node.itemsize == 48
     assert c.aabb_collision.itemsize == 8 and c.conj.itemsize == 120
     assert c.conj_thresh == 1.0 and c.conj_det_interval == 0.1
