@@ -104,13 +104,15 @@ struct mzh_ctx {
     DevBuf cd_begin, cd_end;
     DevBuf aabbs, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, gbox;
     uint64_t node_cap = 0;
-    DevBuf nodes, nolc, ps, tree_size;
+    DevBuf nodes, trav, nolc, ps, tree_size;
     DevBuf otypes, cnt, off, step_total, step_off, grand;
     DevBuf pairs;
     uint64_t n_pairs = 0;
     DevBuf conj_raw, conj_step, conj_sorted, conj_count, np_report, conj_cnt, conj_start, conj_cursor, conj_pos,
         conj_inv, conj_grand;
     uint64_t conj_cap = 0, n_conj = 0;
+    DevBuf cand, cand_count;
+    uint64_t cand_cap = 0;
     DevBuf err;
     hipEvent_t ev[10] = {};
     double stage_s[MZH_STAGE_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -301,8 +303,9 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(ctx->n_eff.ensure(sizeof(uint32_t) * C));
     MZH_HIP(ctx->gbox.ensure(sizeof(uint32_t) * 8u * C));
     MZH_HIP(ctx->nodes.ensure(sizeof(mzh_bvh_node) * ctx->node_cap * C));
-    MZH_HIP(ctx->nolc.ensure(sizeof(uint32_t) * NC));
-    MZH_HIP(ctx->ps.ensure(sizeof(uint32_t) * NC));
+    MZH_HIP(ctx->trav.ensure(sizeof(TravNode) * ctx->node_cap * C));
+    MZH_HIP(ctx->nolc.ensure(sizeof(uint32_t) * ctx->node_cap * C));
+    MZH_HIP(ctx->ps.ensure(sizeof(uint32_t) * ctx->node_cap * C));
     MZH_HIP(ctx->tree_size.ensure(sizeof(uint32_t) * C));
     MZH_HIP(ctx->otypes.ensure(sizeof(int32_t) * (N ? N : 1)));
     MZH_HIP(ctx->cnt.ensure(sizeof(uint32_t) * NC));
@@ -348,14 +351,15 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(hipEventRecord(ctx->ev[2], st));
     // stage 3: BVH
     MZH_HIP(launch_bvh(ctx->srt_mcodes.as<uint64_t>(), ctx->srt_aabbs.as<float>(), ctx->n_eff.as<uint32_t>(),
-                       ctx->nodes.as<mzh_bvh_node>(), ctx->nolc.as<uint32_t>(), ctx->ps.as<uint32_t>(),
+                       ctx->nodes.as<mzh_bvh_node>(), ctx->trav.as<TravNode>(), ctx->nolc.as<uint32_t>(),
+                       ctx->ps.as<uint32_t>(),
                        ctx->tree_size.as<uint32_t>(), N, ctx->node_cap, (uint32_t)C, st));
     MZH_HIP(hipEventRecord(ctx->ev[3], st));
     ctx->n_pairs = 0;
     ctx->n_conj = 0;
     if (!skip_cd) {
         // stage 4: broad phase
-        MZH_HIP(launch_bp_count(ctx->nodes.as<mzh_bvh_node>(), ctx->tree_size.as<uint32_t>(),
+        MZH_HIP(launch_bp_count(ctx->trav.as<TravNode>(), ctx->tree_size.as<uint32_t>(),
                                 ctx->srt_aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->n_eff.as<uint32_t>(),
                                 ctx->otypes.as<int32_t>(), ctx->cnt.as<uint32_t>(), N, ctx->node_cap, (uint32_t)C,
                                 st));
@@ -368,7 +372,7 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
         MZH_HIP(hipStreamSynchronize(st));
         ctx->n_pairs = total;
         MZH_HIP(ctx->pairs.ensure(sizeof(mzh_aabb_collision) * (total ? total : 1)));
-        MZH_HIP(launch_bp_write(ctx->nodes.as<mzh_bvh_node>(), ctx->tree_size.as<uint32_t>(),
+        MZH_HIP(launch_bp_write(ctx->trav.as<TravNode>(), ctx->tree_size.as<uint32_t>(),
                                 ctx->srt_aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->n_eff.as<uint32_t>(),
                                 ctx->otypes.as<int32_t>(), ctx->off.as<uint64_t>(), ctx->step_off.as<uint64_t>(),
                                 ctx->pairs.as<mzh_aabb_collision>(), N, ctx->node_cap, (uint32_t)C, st));
@@ -389,25 +393,42 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
         na.conj_count = ctx->conj_count.as<unsigned long long>();
         na.err = derr;
         if (ctx->conj_cap == 0) ctx->conj_cap = 1u << 16;
-        for (int attempt = 0; attempt < 3; ++attempt) {
+        // queued root-finding candidates: about a fifth of the pairs on LEO catalogs
+        const size_t cand_bytes = narrow_cand_bytes(ctx->order);
+        ctx->cand_cap = std::max<uint64_t>(ctx->cand_cap, total / 2 + 65536);
+        MZH_HIP(ctx->cand_count.ensure(sizeof(unsigned long long)));
+        na.cand_count = ctx->cand_count.as<unsigned long long>();
+        for (int attempt = 0; attempt < 4; ++attempt) {
             MZH_HIP(ctx->conj_raw.ensure(sizeof(mzh_conj) * ctx->conj_cap));
             MZH_HIP(ctx->conj_step.ensure(sizeof(uint32_t) * ctx->conj_cap));
+            MZH_HIP(ctx->cand.ensure(cand_bytes * ctx->cand_cap));
             MZH_HIP(hipMemsetAsync(ctx->conj_count.p, 0, sizeof(unsigned long long), st));
+            MZH_HIP(hipMemsetAsync(ctx->cand_count.p, 0, sizeof(unsigned long long), st));
             MZH_HIP(hipMemsetAsync(ctx->np_report.p, 0, sizeof(unsigned long long) * MZH_NP_COUNT, st));
             na.conj = ctx->conj_raw.as<mzh_conj>();
             na.conj_step = ctx->conj_step.as<uint32_t>();
             na.conj_cap = ctx->conj_cap;
+            na.cand = ctx->cand.p;
+            na.cand_cap = ctx->cand_cap;
             MZH_HIP(hipEventRecord(ctx->ev[8], st));
             MZH_HIP(launch_narrow(na, st));
             MZH_HIP(hipEventRecord(ctx->ev[9], st));
-            unsigned long long nc = 0;
-            MZH_HIP(hipMemcpyAsync(&nc, ctx->conj_count.p, sizeof(nc), hipMemcpyDeviceToHost, st));
+            unsigned long long cnts[2] = {0, 0};
+            MZH_HIP(hipMemcpyAsync(&cnts[0], ctx->conj_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            MZH_HIP(hipMemcpyAsync(&cnts[1], ctx->cand_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
             MZH_HIP(hipStreamSynchronize(st));
-            ctx->n_conj = nc;
-            if (nc <= ctx->conj_cap) break;
-            ctx->conj_cap = nc + nc / 4 + 1024;
+            ctx->n_conj = cnts[0];
+            const bool conj_ok = cnts[0] <= ctx->conj_cap, cand_ok = cnts[1] <= ctx->cand_cap;
+            if (conj_ok && cand_ok) break;
+            if (!conj_ok) ctx->conj_cap = cnts[0] + cnts[0] / 4 + 1024;
+            if (!cand_ok) ctx->cand_cap = cnts[1] + cnts[1] / 4 + 1024;
         }
         if (ctx->n_conj > ctx->conj_cap) throw StatusError{MZH_ERUNTIME, "conjunction buffer overflow"};
+        {
+            unsigned long long nq = 0;
+            MZH_HIP(hipMemcpy(&nq, ctx->cand_count.p, sizeof(nq), hipMemcpyDeviceToHost));
+            if (nq > ctx->cand_cap) throw StatusError{MZH_ERUNTIME, "narrow-phase candidate queue overflow"};
+        }
         const uint64_t M = ctx->n_conj;
         MZH_HIP(ctx->conj_sorted.ensure(sizeof(mzh_conj) * (M ? M : 1)));
         MZH_HIP(ctx->conj_pos.ensure(sizeof(uint32_t) * (M ? M : 1)));

@@ -60,6 +60,19 @@ struct PjView {
     uint32_t order;
 };
 
+// Traversal copy of a BVH node: the tree in depth-first (preorder) order with
+// an escape index (first node after the subtree), so that the broad phase
+// walks it without a stack. end == 0 marks an internal node.
+struct alignas(16) TravNode {
+    float lb[4];
+    float ub[4];
+    uint32_t escape;
+    uint32_t begin;
+    uint32_t end;
+    uint32_t pad;
+};
+static_assert(sizeof(TravNode) == 48, "TravNode layout");
+
 // Error word written by kernels; the host turns it into the reference's
 // exception message. Only the first error (lowest code wins ties) is kept.
 struct DevError {

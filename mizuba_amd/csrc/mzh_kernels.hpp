@@ -32,19 +32,21 @@ hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt
                          uint32_t n_steps, hipStream_t stream);
 
 // Stage 3: BVH, one workgroup per step. nodes has node_cap entries per step.
+// Also writes the depth-first traversal copy `trav` (node_cap per step);
+// nolc_buf and ps_buf are node_cap-sized scratch per step.
 hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, mzh_bvh_node *nodes,
-                      uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap,
-                      uint32_t n_steps, hipStream_t stream);
+                      TravNode *trav, uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *tree_size, uint32_t n_objs,
+                      uint64_t node_cap, uint32_t n_steps, hipStream_t stream);
 
 // Stage 4: broad phase (count, per-step scan, step scan, write + per-object sort).
-hipError_t launch_bp_count(const mzh_bvh_node *nodes, const uint32_t *tree_size, const float *srt_aabbs,
+hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
                            const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, uint32_t *cnt,
                            uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream);
 hipError_t launch_scan_counts(const uint32_t *cnt, uint64_t *off, uint64_t *step_total, uint32_t n_objs,
                               uint32_t n_steps, hipStream_t stream);
 hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uint64_t *grand_total, uint32_t n_steps,
                              hipStream_t stream);
-hipError_t launch_bp_write(const mzh_bvh_node *nodes, const uint32_t *tree_size, const float *srt_aabbs,
+hipError_t launch_bp_write(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
                            const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, const uint64_t *off,
                            const uint64_t *step_off, mzh_aabb_collision *pairs, uint32_t n_objs, uint64_t node_cap,
                            uint32_t n_steps, hipStream_t stream);
@@ -64,7 +66,13 @@ struct NarrowArgs {
     unsigned long long *conj_count;
     unsigned long long *np_report;  // [MZH_NP_COUNT]
     DevError *err;
+    // queue of step-pairs whose dist^2 enclosure reaches below thresh^2
+    void *cand;                   // records of cand_stride bytes
+    uint64_t cand_cap;
+    unsigned long long *cand_count;
 };
+// Bytes of one queued root-finding candidate for a polynomial order.
+size_t narrow_cand_bytes(uint32_t order);
 hipError_t launch_narrow(const NarrowArgs &a, hipStream_t stream);
 template <int ORDER>
 hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream);

@@ -11,6 +11,7 @@
 // order of the CPU restatement, so TCA/DCA match it bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "mzh_device.hpp"
@@ -538,7 +539,7 @@ __device__ inline void emit_conj(const NarrowArgs &a, uint32_t c, uint32_t i, ui
     rec.i = i;
     rec.j = j;
     double tp[OP1];
-#pragma unroll
+#pragma unroll 1
     for (int ch = 0; ch < 3; ++ch) {
         translate7<ORDER>(Pi + ch, di, tp);
         rec.ri[ch] = horner<ORDER>(tp, t);
@@ -559,14 +560,53 @@ __device__ inline void emit_conj(const NarrowArgs &a, uint32_t c, uint32_t i, ui
     }
 }
 
+// A step-pair whose Cargo-Shisha enclosure of dist^2 reaches below thresh^2:
+// everything the root finding and the conjunction records need.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_narrow(NarrowArgs a)
+struct NpCand {
+    double d2[ORDER + 1];  // dist^2 interpolant on [0, rf_int]
+    double lb_rf, rf_int, di, dj;
+    uint32_t c, i, j, flags;  // flags: bit 0 trajectory end reached, bit 1 trajectory begin
+    uint64_t ki, kj;          // trajectory-step indices of i and j
+};
+
+constexpr uint32_t kCandEnd = 1u, kCandBegin = 2u;
+
+// p(x) -> p(x + a) for one channel stored contiguously (the cf4 layout).
+template <int ORDER>
+__device__ inline void translate1s(const double *__restrict__ src, double a, double *out)
+{
+    constexpr int OP1 = ORDER + 1;
+    double c[OP1];
+#pragma unroll
+    for (int k = 0; k < OP1; ++k) c[k] = src[k];
+    if (a == 0.0) {
+#pragma unroll
+        for (int k = 0; k < OP1; ++k) out[k] = c[k];
+        return;
+    }
+    double ap[OP1];
+    halving_pows_d<ORDER>(a, ap);
+#pragma unroll
+    for (int i = 0; i < OP1; ++i) {
+        double s = (c[i] * c_binom_np.v[i][0]) * ap[0];
+#pragma unroll
+        for (int k = i + 1; k < OP1; ++k) s = s + (c[k] * c_binom_np.v[k][k - i]) * ap[k - i];
+        out[i] = s;
+    }
+}
+
+// Narrow phase, part 1: one lane per broad-phase pair. Merge-walk of the
+// overlapping trajectory steps (narrow_phase.cpp:357-434), translation of the
+// position polynomials (:472-488), dist^2 interpolation (:95-152) and the
+// Cargo-Shisha screen (:502-513); survivors go to the candidate queue.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
 {
     constexpr int OP1 = ORDER + 1;
     const uint64_t p = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    unsigned long long rep[MZH_NP_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+    unsigned long long n_cand = 0, n_rej = 0;
     if (p < a.n_pairs) {
-        // step of this pair: last c with step_off[c] <= p
         uint32_t lo = 0, n = a.sg.n_steps;
         while (n > 0) {
             const uint32_t h = n / 2;
@@ -584,7 +624,6 @@ __global__ __launch_bounds__(256) void k_narrow(NarrowArgs a)
         const uint64_t nsi = a.pj.nsteps[i], nsj = a.pj.nsteps[j];
         const double *ti = a.pj.time + a.pj.time_off[i];
         const double *tj = a.pj.time + a.pj.time_off[j];
-        // trajectory-step ranges overlapping [cb, ce) (same search as the AABBs)
         auto range = [](const double *t, uint64_t ns, double cbv, double cev, uint64_t &kb, uint64_t &ke) {
             uint64_t l = 0, m = ns;
             while (m > 0) {
@@ -612,117 +651,69 @@ __global__ __launch_bounds__(256) void k_narrow(NarrowArgs a)
         uint64_t kbi, kei, kbj, kej;
         range(ti, nsi, cb, ce, kbi, kei);
         range(tj, nsj, cb, ce, kbj, kej);
-        double nodes[OP1];
-        bool failed = false;
-        for (uint64_t ki = kbi, kj = kbj; ki < kei && kj < kej && !failed;) {
+        for (uint64_t ki = kbi, kj = kbj; ki < kei && kj < kej;) {
             const double tsi = ti[ki], tei = ti[ki + 1], tsj = tj[kj], tej = tj[kj + 1];
             const double lb_i = cb > tsi ? cb : tsi, ub_i = ce < tei ? ce : tei;
             const double lb_j = cb > tsj ? cb : tsj, ub_j = ce < tej ? ce : tej;
             if (ub_i > lb_j && lb_i < ub_j) {
-                ++rep[0];
+                ++n_cand;
                 const double lb_rf = lb_i > lb_j ? lb_i : lb_j;
                 const double ub_rf = ub_i < ub_j ? ub_i : ub_j;
                 const double di = lb_rf - tsi, dj = lb_rf - tsj;
                 const double rf_int = ub_rf - lb_rf;
                 if (!is_finite_d(di) || !is_finite_d(dj) || !is_finite_d(rf_int) || di < 0 || dj < 0 || rf_int < 0) {
                     report_error(a.err, kErrNpInterval, i, j, rf_int);
-                    failed = true;
                     break;
                 }
-                const double *Pi = a.pj.traj + (a.pj.step0[i] + ki) * (uint64_t)OP1 * 7u;
-                const double *Pj = a.pj.traj + (a.pj.step0[j] + kj) * (uint64_t)OP1 * 7u;
+                // positions from the step-major copy: channel ch of a step at cf4[(s * 4 + ch) * OP1]
+                const double *Qi = a.pj.cf4 + (a.pj.step0[i] + ki) * 4u * OP1;
+                const double *Qj = a.pj.cf4 + (a.pj.step0[j] + kj) * 4u * OP1;
+                double nodes[OP1];
 #pragma unroll
                 for (int m = 0; m < OP1; ++m) nodes[OP1 - 1 - m] = a.cheby[m] * rf_int;
                 double d2[OP1];
-#pragma unroll
+#pragma unroll 1
                 for (int ch = 0; ch < 3; ++ch) {
                     double tp[OP1], vi[OP1];
-                    translate7<ORDER>(Pi + ch, di, tp);
+                    translate1s<ORDER>(Qi + ch * OP1, di, tp);
 #pragma unroll
                     for (int m = 0; m < OP1; ++m) vi[m] = horner<ORDER>(tp, nodes[m]);
-                    translate7<ORDER>(Pj + ch, dj, tp);
+                    translate1s<ORDER>(Qj + ch * OP1, dj, tp);
 #pragma unroll
                     for (int m = 0; m < OP1; ++m) {
                         const double dx = vi[m] - horner<ORDER>(tp, nodes[m]);
                         d2[m] = (ch == 0) ? dx * dx : d2[m] + dx * dx;
                     }
                 }
-                vm_interp<ORDER>(nodes, d2);  // d2 now holds the dist^2 coefficients
+                vm_interp<ORDER>(nodes, d2);
                 double elo, ehi;
                 cs_enc0<ORDER>(d2, rf_int, elo, ehi);
                 if (!is_finite_d(elo) || !is_finite_d(ehi)) {
                     report_error(a.err, kErrNpDist2, i, j, elo);
-                    failed = true;
                     break;
                 }
                 if (elo < thresh2) {
-                    double d2p[OP1];
+                    uint32_t flags = 0;
+                    if ((ki + 1 == nsi && tei == ub_rf) || (kj + 1 == nsj && tej == ub_rf)) flags |= kCandEnd;
+                    if ((ki == 0 && lb_rf == ti[0]) || (kj == 0 && lb_rf == tj[0])) flags |= kCandBegin;
+                    const unsigned long long slot = atomicAdd(a.cand_count, 1ull);
+                    if (slot < a.cand_cap) {
+                        NpCand<ORDER> &q = static_cast<NpCand<ORDER> *>(a.cand)[slot];
 #pragma unroll
-                    for (int k = 0; k < ORDER; ++k) d2p[k] = (double)(k + 1) * d2[k + 1];
-                    d2p[ORDER] = 0;
-                    double roots[kMaxRoots<ORDER>];
-                    int nr = 0;
-                    bool fex = false;
-                    if (!find_minima<ORDER>(d2p, rf_int, roots, nr, fex, i, j, a.err)) {
-                        failed = true;
-                        break;
-                    }
-                    ++rep[2];
-                    rep[3] += fex ? 1 : 0;
-                    rep[4] += (nr == 0) ? 1 : 0;
-                    rep[5] += (unsigned long long)nr;
-                    for (int r = 0; r < nr; ++r) {
-                        const double cd2 = horner<ORDER>(d2, roots[r]);
-                        if (!is_finite_d(cd2)) {
-                            report_error(a.err, kErrNpConjDist2, i, j, cd2);
-                            failed = true;
-                            break;
-                        }
-                        if (cd2 < thresh2) {
-                            emit_conj<ORDER>(a, c, i, j, lb_rf, roots[r], Pi, di, Pj, dj);
-                        } else {
-                            ++rep[6];
-                        }
-                    }
-                    if (failed) break;
-                    // boundary minima (narrow_phase.cpp:629-714)
-                    if ((ki + 1 == nsi && tei == ub_rf) || (kj + 1 == nsj && tej == ub_rf)) {
-                        const double tmc = next_toward_d(rf_int, -1.0);
-                        const double dv = horner<ORDER>(d2p, tmc);
-                        if (!is_finite_d(dv)) {
-                            report_error(a.err, kErrNpBndDerUb, i, j, dv);
-                            failed = true;
-                            break;
-                        }
-                        if (dv < 0) {
-                            const double dd = horner<ORDER>(d2, tmc);
-                            if (!is_finite_d(dd)) {
-                                report_error(a.err, kErrNpBndDist2Ub, i, j, dd);
-                                failed = true;
-                                break;
-                            }
-                            if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, tmc, Pi, di, Pj, dj);
-                        }
-                    }
-                    if ((ki == 0 && lb_rf == ti[0]) || (kj == 0 && lb_rf == tj[0])) {
-                        const double dv = horner<ORDER>(d2p, 0.0);
-                        if (!is_finite_d(dv)) {
-                            report_error(a.err, kErrNpBndDerLb, i, j, dv);
-                            failed = true;
-                            break;
-                        }
-                        if (dv > 0) {
-                            const double dd = horner<ORDER>(d2, 0.0);
-                            if (!is_finite_d(dd)) {
-                                report_error(a.err, kErrNpBndDist2Lb, i, j, dd);
-                                failed = true;
-                                break;
-                            }
-                            if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, 0.0, Pi, di, Pj, dj);
-                        }
+                        for (int k = 0; k < OP1; ++k) q.d2[k] = d2[k];
+                        q.lb_rf = lb_rf;
+                        q.rf_int = rf_int;
+                        q.di = di;
+                        q.dj = dj;
+                        q.c = c;
+                        q.i = i;
+                        q.j = j;
+                        q.flags = flags;
+                        q.ki = ki;
+                        q.kj = kj;
                     }
                 } else {
-                    ++rep[1];
+                    ++n_rej;
                 }
             }
             const bool inc_i = tei <= tej, inc_j = tej <= tei;
@@ -730,11 +721,97 @@ __global__ __launch_bounds__(256) void k_narrow(NarrowArgs a)
             kj += inc_j ? 1 : 0;
         }
     }
-    // wave-aggregated counter updates
+    const unsigned long long s0 = wave_reduce_add_u64(n_cand), s1 = wave_reduce_add_u64(n_rej);
+    if (__lane_id() == 0) {
+        if (s0) atomicAdd(&a.np_report[0], s0);
+        if (s1) atomicAdd(&a.np_report[1], s1);
+    }
+}
+
+// Narrow phase, part 2: one lane per queued candidate (grid-stride over the
+// queue, whose length is read on the device). Minima of dist^2 from the roots
+// of its derivative (narrow_phase.cpp:519-605), boundary minima (:629-714).
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_np_roots(NarrowArgs a)
+{
+    constexpr int OP1 = ORDER + 1;
+    const unsigned long long avail = *a.cand_count;
+    const uint64_t n = avail < a.cand_cap ? avail : a.cand_cap;
+    const double thresh2 = a.conj_thresh * a.conj_thresh;
+    unsigned long long rep[5] = {0, 0, 0, 0, 0};
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x; q < n; q += stride) {
+        const NpCand<ORDER> &cd = static_cast<const NpCand<ORDER> *>(a.cand)[q];
+        double d2[OP1];
 #pragma unroll
-    for (int k = 0; k < MZH_NP_COUNT; ++k) {
+        for (int k = 0; k < OP1; ++k) d2[k] = cd.d2[k];
+        const double rf_int = cd.rf_int, lb_rf = cd.lb_rf;
+        const uint32_t i = cd.i, j = cd.j, c = cd.c, flags = cd.flags;
+        const double *Pi = a.pj.traj + (a.pj.step0[i] + cd.ki) * (uint64_t)OP1 * 7u;
+        const double *Pj = a.pj.traj + (a.pj.step0[j] + cd.kj) * (uint64_t)OP1 * 7u;
+        double d2p[OP1];
+#pragma unroll
+        for (int k = 0; k < ORDER; ++k) d2p[k] = (double)(k + 1) * d2[k + 1];
+        d2p[ORDER] = 0;
+        double roots[kMaxRoots<ORDER>];
+        int nr = 0;
+        bool fex = false;
+        if (!find_minima<ORDER>(d2p, rf_int, roots, nr, fex, i, j, a.err)) continue;
+        ++rep[0];
+        rep[1] += fex ? 1 : 0;
+        rep[2] += (nr == 0) ? 1 : 0;
+        rep[3] += (unsigned long long)nr;
+        bool failed = false;
+        for (int r = 0; r < nr; ++r) {
+            const double cd2 = horner<ORDER>(d2, roots[r]);
+            if (!is_finite_d(cd2)) {
+                report_error(a.err, kErrNpConjDist2, i, j, cd2);
+                failed = true;
+                break;
+            }
+            if (cd2 < thresh2) {
+                emit_conj<ORDER>(a, c, i, j, lb_rf, roots[r], Pi, cd.di, Pj, cd.dj);
+            } else {
+                ++rep[4];
+            }
+        }
+        if (failed) continue;
+        if (flags & kCandEnd) {
+            const double tmc = next_toward_d(rf_int, -1.0);
+            const double dv = horner<ORDER>(d2p, tmc);
+            if (!is_finite_d(dv)) {
+                report_error(a.err, kErrNpBndDerUb, i, j, dv);
+                continue;
+            }
+            if (dv < 0) {
+                const double dd = horner<ORDER>(d2, tmc);
+                if (!is_finite_d(dd)) {
+                    report_error(a.err, kErrNpBndDist2Ub, i, j, dd);
+                    continue;
+                }
+                if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, tmc, Pi, cd.di, Pj, cd.dj);
+            }
+        }
+        if (flags & kCandBegin) {
+            const double dv = horner<ORDER>(d2p, 0.0);
+            if (!is_finite_d(dv)) {
+                report_error(a.err, kErrNpBndDerLb, i, j, dv);
+                continue;
+            }
+            if (dv > 0) {
+                const double dd = horner<ORDER>(d2, 0.0);
+                if (!is_finite_d(dd)) {
+                    report_error(a.err, kErrNpBndDist2Lb, i, j, dd);
+                    continue;
+                }
+                if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, 0.0, Pi, cd.di, Pj, cd.dj);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
         const unsigned long long s = wave_reduce_add_u64(rep[k]);
-        if (__lane_id() == 0 && s) atomicAdd(&a.np_report[k], s);
+        if (__lane_id() == 0 && s) atomicAdd(&a.np_report[2 + k], s);
     }
 }
 
@@ -810,7 +887,12 @@ hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream)
 {
     const uint64_t blocks = (a.n_pairs + 255) / 256;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_narrow<ORDER>, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_np_screen<ORDER>, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the queue length is only known on the device: a fixed grid strides over it
+    const uint64_t qblocks = std::min<uint64_t>((a.cand_cap + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_np_roots<ORDER>, dim3((unsigned)qblocks), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 
@@ -877,6 +959,52 @@ MZH_INST(20)
 #undef MZH_INST
 
 #ifdef MZH_NARROW_HOST
+size_t narrow_cand_bytes(uint32_t order)
+{
+    switch (order) {
+        case 2:
+            return sizeof(NpCand<2>);
+        case 3:
+            return sizeof(NpCand<3>);
+        case 4:
+            return sizeof(NpCand<4>);
+        case 5:
+            return sizeof(NpCand<5>);
+        case 6:
+            return sizeof(NpCand<6>);
+        case 7:
+            return sizeof(NpCand<7>);
+        case 8:
+            return sizeof(NpCand<8>);
+        case 9:
+            return sizeof(NpCand<9>);
+        case 10:
+            return sizeof(NpCand<10>);
+        case 11:
+            return sizeof(NpCand<11>);
+        case 12:
+            return sizeof(NpCand<12>);
+        case 13:
+            return sizeof(NpCand<13>);
+        case 14:
+            return sizeof(NpCand<14>);
+        case 15:
+            return sizeof(NpCand<15>);
+        case 16:
+            return sizeof(NpCand<16>);
+        case 17:
+            return sizeof(NpCand<17>);
+        case 18:
+            return sizeof(NpCand<18>);
+        case 19:
+            return sizeof(NpCand<19>);
+        case 20:
+            return sizeof(NpCand<20>);
+        default:
+            return 0;
+    }
+}
+
 hipError_t launch_narrow(const NarrowArgs &a, hipStream_t stream)
 {
     switch (a.pj.order) {

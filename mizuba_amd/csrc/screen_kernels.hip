@@ -599,8 +599,9 @@ __device__ inline void init_node(mzh_bvh_node &n, uint32_t b, uint32_t e)
 __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ srt_mcodes,
                                                      const float *__restrict__ srt_aabbs,
                                                      const uint32_t *__restrict__ n_eff, mzh_bvh_node *__restrict__ nodes,
-                                                     uint32_t *__restrict__ nolc_buf, uint32_t *__restrict__ ps_buf,
-                                                     uint32_t *__restrict__ tree_size, uint32_t N, uint64_t node_cap)
+                                                     TravNode *__restrict__ trav, uint32_t *__restrict__ nolc_buf,
+                                                     uint32_t *__restrict__ ps_buf, uint32_t *__restrict__ tree_size,
+                                                     uint32_t N, uint64_t node_cap)
 {
     const uint32_t c = blockIdx.x;
     const uint32_t n = n_eff[c];
@@ -612,8 +613,8 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
     const uint64_t *codes = srt_mcodes + (size_t)c * N;
     const float *boxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
     mzh_bvh_node *T = nodes + (size_t)c * node_cap;
-    uint32_t *nolc = nolc_buf + (size_t)c * N;
-    uint32_t *ps = ps_buf + (size_t)c * N;
+    uint32_t *nolc = nolc_buf + (size_t)c * node_cap;
+    uint32_t *ps = ps_buf + (size_t)c * node_cap;
     __shared__ uint32_t lvl[kBvhMaxLevels + 2];
     __shared__ uint32_t s_w[65];
 
@@ -723,6 +724,45 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
         }
         __syncthreads();
     }
+    // depth-first traversal copy: subtree sizes bottom-up (into nolc), preorder
+    // positions top-down (into ps), then one scatter with escape indices
+    uint32_t *sz = nolc, *pos = ps;
+    for (int j = (int)nlev - 1; j >= 0; --j) {
+        const uint32_t a = lvl[j], b = lvl[j + 1];
+        for (uint32_t ni = a + tid; ni < b; ni += kBvhBlock) {
+            const mzh_bvh_node &nd = T[ni];
+            sz[ni] = (nd.left == -1) ? 1u : 1u + sz[nd.left] + sz[nd.right];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) pos[0] = 0;
+    __syncthreads();
+    for (uint32_t j = 0; j + 1 < nlev; ++j) {
+        const uint32_t a = lvl[j], b = lvl[j + 1];
+        for (uint32_t ni = a + tid; ni < b; ni += kBvhBlock) {
+            const mzh_bvh_node &nd = T[ni];
+            if (nd.left != -1) {
+                pos[nd.left] = pos[ni] + 1u;
+                pos[nd.right] = pos[ni] + 1u + sz[nd.left];
+            }
+        }
+        __syncthreads();
+    }
+    TravNode *TV = trav + (size_t)c * node_cap;
+    for (uint32_t ni = tid; ni < le; ni += kBvhBlock) {
+        const mzh_bvh_node &nd = T[ni];
+        TravNode tn;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            tn.lb[d] = nd.lb[d];
+            tn.ub[d] = nd.ub[d];
+        }
+        tn.escape = pos[ni] + sz[ni];
+        tn.begin = nd.begin;
+        tn.end = (nd.left == -1) ? nd.end : 0u;
+        tn.pad = 0;
+        TV[pos[ni]] = tn;
+    }
     if (tid == 0) tree_size[c] = le;
 }
 
@@ -734,10 +774,8 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
 // then places every query's pairs in (i, j) order without a global sort.
 // ---------------------------------------------------------------------------
 
-constexpr int kStackDepth = 72;
-
 template <bool WRITE>
-__global__ __launch_bounds__(256) void k_bp(const mzh_bvh_node *__restrict__ nodes,
+__global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
                                             const uint32_t *__restrict__ tree_size, const float *__restrict__ srt_aabbs,
                                             const uint32_t *__restrict__ srt_idx, const uint32_t *__restrict__ n_eff,
                                             const int32_t *__restrict__ otypes, uint32_t *__restrict__ cnt,
@@ -747,8 +785,9 @@ __global__ __launch_bounds__(256) void k_bp(const mzh_bvh_node *__restrict__ nod
     const uint32_t bps = (N + 255u) / 256u;
     const uint32_t c = blockIdx.x / bps;
     const uint32_t o = (blockIdx.x - c * bps) * 256u + threadIdx.x;
-    if (o >= n_eff[c] || tree_size[c] == 0) return;
-    const mzh_bvh_node *T = nodes + (size_t)c * node_cap;
+    const uint32_t nn = tree_size[c];
+    if (o >= n_eff[c] || nn == 0) return;
+    const TravNode *T = trav + (size_t)c * node_cap;
     const uint32_t *vidx = srt_idx + (size_t)c * N;
     const float4 ql = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[0];
     const float4 qu = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[1];
@@ -757,31 +796,30 @@ __global__ __launch_bounds__(256) void k_bp(const mzh_bvh_node *__restrict__ nod
     uint32_t count = 0;
     mzh_aabb_collision *out = nullptr;
     if constexpr (WRITE) out = pairs + step_off[c] + off[(size_t)c * N + orig];
-    int32_t stack[kStackDepth];
-    int sp = 0;
-    stack[sp++] = 0;
-    do {
-        const mzh_bvh_node &nd = T[stack[--sp]];
-        const float4 nl = *reinterpret_cast<const float4 *>(nd.lb);
-        const float4 nu = *reinterpret_cast<const float4 *>(nd.ub);
+    // stackless preorder walk: descend on overlap, jump to the escape index otherwise
+    uint32_t i = 0;
+    while (i < nn) {
+        const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
+        const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
+        const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
         const bool overlap = (qu.x >= nl.x && ql.x <= nu.x) && (qu.y >= nl.y && ql.y <= nu.y)
                              && (qu.z >= nl.z && ql.z <= nu.z) && (qu.w >= nl.w && ql.w <= nu.w);
+        if (overlap && meta.z == 0u) {
+            ++i;  // internal node: its left child follows it
+            continue;
+        }
         if (overlap) {
-            if (nd.left == -1) {
-                for (uint32_t k = nd.begin; k != nd.end; ++k) {
-                    const uint32_t oi = vidx[k];
-                    if (orig >= oi) continue;
-                    if (ot + otypes[oi] <= 3) {
-                        if constexpr (WRITE) out[count] = mzh_aabb_collision{orig, oi};
-                        ++count;
-                    }
+            for (uint32_t k = meta.y; k != meta.z; ++k) {
+                const uint32_t oi = vidx[k];
+                if (orig >= oi) continue;
+                if (ot + otypes[oi] <= 3) {
+                    if constexpr (WRITE) out[count] = mzh_aabb_collision{orig, oi};
+                    ++count;
                 }
-            } else if (sp + 2 <= kStackDepth) {
-                stack[sp++] = nd.left;
-                stack[sp++] = nd.right;
             }
         }
-    } while (sp != 0);
+        i = meta.x;
+    }
     if constexpr (WRITE) {
         // insertion sort of this query's pairs by j
         for (uint32_t a = 1; a < count; ++a) {
@@ -930,15 +968,15 @@ hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt
 }
 
 hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, mzh_bvh_node *nodes,
-                      uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap,
-                      uint32_t n_steps, hipStream_t stream)
+                      TravNode *trav, uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *tree_size, uint32_t n_objs,
+                      uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_bvh, dim3(n_steps), dim3(kBvhBlock), 0, stream, srt_mcodes, srt_aabbs, n_eff, nodes,
+    hipLaunchKernelGGL(k_bvh, dim3(n_steps), dim3(kBvhBlock), 0, stream, srt_mcodes, srt_aabbs, n_eff, nodes, trav,
                        nolc_buf, ps_buf, tree_size, n_objs, node_cap);
     return hipGetLastError();
 }
 
-hipError_t launch_bp_count(const mzh_bvh_node *nodes, const uint32_t *tree_size, const float *srt_aabbs,
+hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
                            const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, uint32_t *cnt,
                            uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
 {
@@ -963,7 +1001,7 @@ hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uin
     return hipGetLastError();
 }
 
-hipError_t launch_bp_write(const mzh_bvh_node *nodes, const uint32_t *tree_size, const float *srt_aabbs,
+hipError_t launch_bp_write(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
                            const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, const uint64_t *off,
                            const uint64_t *step_off, mzh_aabb_collision *pairs, uint32_t n_objs, uint64_t node_cap,
                            uint32_t n_steps, hipStream_t stream)
