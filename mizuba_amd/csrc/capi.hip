@@ -104,15 +104,15 @@ struct mzh_ctx {
     DevBuf cd_begin, cd_end;
     DevBuf aabbs, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, gbox;
     uint64_t node_cap = 0;
-    DevBuf nodes, trav, nolc, ps, tree_size;
+    DevBuf nodes, trav, nolc, ps, tpos, tree_size;
     DevBuf otypes, cnt, off, step_total, step_off, grand;
     DevBuf pairs;
     uint64_t n_pairs = 0;
     DevBuf conj_raw, conj_step, conj_sorted, conj_count, np_report, conj_cnt, conj_start, conj_cursor, conj_pos,
         conj_inv, conj_grand;
     uint64_t conj_cap = 0, n_conj = 0;
-    DevBuf cand, cand_count;
-    uint64_t cand_cap = 0;
+    DevBuf cand, cand_count, steps, steps_count;
+    uint64_t cand_cap = 0, steps_cap = 0;
     DevBuf err;
     hipEvent_t ev[10] = {};
     double stage_s[MZH_STAGE_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -306,6 +306,7 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(ctx->trav.ensure(sizeof(TravNode) * ctx->node_cap * C));
     MZH_HIP(ctx->nolc.ensure(sizeof(uint32_t) * ctx->node_cap * C));
     MZH_HIP(ctx->ps.ensure(sizeof(uint32_t) * ctx->node_cap * C));
+    MZH_HIP(ctx->tpos.ensure(sizeof(uint32_t) * ctx->node_cap * C));
     MZH_HIP(ctx->tree_size.ensure(sizeof(uint32_t) * C));
     MZH_HIP(ctx->otypes.ensure(sizeof(int32_t) * (N ? N : 1)));
     MZH_HIP(ctx->cnt.ensure(sizeof(uint32_t) * NC));
@@ -352,7 +353,7 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     // stage 3: BVH
     MZH_HIP(launch_bvh(ctx->srt_mcodes.as<uint64_t>(), ctx->srt_aabbs.as<float>(), ctx->n_eff.as<uint32_t>(),
                        ctx->nodes.as<mzh_bvh_node>(), ctx->trav.as<TravNode>(), ctx->nolc.as<uint32_t>(),
-                       ctx->ps.as<uint32_t>(),
+                       ctx->ps.as<uint32_t>(), ctx->tpos.as<uint32_t>(),
                        ctx->tree_size.as<uint32_t>(), N, ctx->node_cap, (uint32_t)C, st));
     MZH_HIP(hipEventRecord(ctx->ev[3], st));
     ctx->n_pairs = 0;
@@ -396,14 +397,22 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
         // queued root-finding candidates: about a fifth of the pairs on LEO catalogs
         const size_t cand_bytes = narrow_cand_bytes(ctx->order);
         ctx->cand_cap = std::max<uint64_t>(ctx->cand_cap, total / 2 + 65536);
+        // overlapping trajectory-step pairs: about 1.5 per pair for T/8 steps and 3-minute conjunction steps
+        ctx->steps_cap = std::max<uint64_t>(ctx->steps_cap, 2 * total + 65536);
         MZH_HIP(ctx->cand_count.ensure(sizeof(unsigned long long)));
+        MZH_HIP(ctx->steps_count.ensure(sizeof(unsigned long long)));
         na.cand_count = ctx->cand_count.as<unsigned long long>();
+        na.steps_count = ctx->steps_count.as<unsigned long long>();
         for (int attempt = 0; attempt < 4; ++attempt) {
             MZH_HIP(ctx->conj_raw.ensure(sizeof(mzh_conj) * ctx->conj_cap));
             MZH_HIP(ctx->conj_step.ensure(sizeof(uint32_t) * ctx->conj_cap));
             MZH_HIP(ctx->cand.ensure(cand_bytes * ctx->cand_cap));
+            MZH_HIP(ctx->steps.ensure(sizeof(StepPair) * ctx->steps_cap));
+            na.steps = ctx->steps.p;
+            na.steps_cap = ctx->steps_cap;
             MZH_HIP(hipMemsetAsync(ctx->conj_count.p, 0, sizeof(unsigned long long), st));
             MZH_HIP(hipMemsetAsync(ctx->cand_count.p, 0, sizeof(unsigned long long), st));
+            MZH_HIP(hipMemsetAsync(ctx->steps_count.p, 0, sizeof(unsigned long long), st));
             MZH_HIP(hipMemsetAsync(ctx->np_report.p, 0, sizeof(unsigned long long) * MZH_NP_COUNT, st));
             na.conj = ctx->conj_raw.as<mzh_conj>();
             na.conj_step = ctx->conj_step.as<uint32_t>();
@@ -413,21 +422,26 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
             MZH_HIP(hipEventRecord(ctx->ev[8], st));
             MZH_HIP(launch_narrow(na, st));
             MZH_HIP(hipEventRecord(ctx->ev[9], st));
-            unsigned long long cnts[2] = {0, 0};
+            unsigned long long cnts[3] = {0, 0, 0};
             MZH_HIP(hipMemcpyAsync(&cnts[0], ctx->conj_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
             MZH_HIP(hipMemcpyAsync(&cnts[1], ctx->cand_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            MZH_HIP(hipMemcpyAsync(&cnts[2], ctx->steps_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
             MZH_HIP(hipStreamSynchronize(st));
             ctx->n_conj = cnts[0];
-            const bool conj_ok = cnts[0] <= ctx->conj_cap, cand_ok = cnts[1] <= ctx->cand_cap;
-            if (conj_ok && cand_ok) break;
+            const bool conj_ok = cnts[0] <= ctx->conj_cap, cand_ok = cnts[1] <= ctx->cand_cap,
+                       steps_ok = cnts[2] <= ctx->steps_cap;
+            if (conj_ok && cand_ok && steps_ok) break;
             if (!conj_ok) ctx->conj_cap = cnts[0] + cnts[0] / 4 + 1024;
             if (!cand_ok) ctx->cand_cap = cnts[1] + cnts[1] / 4 + 1024;
+            if (!steps_ok) ctx->steps_cap = cnts[2] + cnts[2] / 4 + 1024;
         }
         if (ctx->n_conj > ctx->conj_cap) throw StatusError{MZH_ERUNTIME, "conjunction buffer overflow"};
         {
             unsigned long long nq = 0;
             MZH_HIP(hipMemcpy(&nq, ctx->cand_count.p, sizeof(nq), hipMemcpyDeviceToHost));
             if (nq > ctx->cand_cap) throw StatusError{MZH_ERUNTIME, "narrow-phase candidate queue overflow"};
+            MZH_HIP(hipMemcpy(&nq, ctx->steps_count.p, sizeof(nq), hipMemcpyDeviceToHost));
+            if (nq > ctx->steps_cap) throw StatusError{MZH_ERUNTIME, "narrow-phase step-pair list overflow"};
         }
         const uint64_t M = ctx->n_conj;
         MZH_HIP(ctx->conj_sorted.ensure(sizeof(mzh_conj) * (M ? M : 1)));

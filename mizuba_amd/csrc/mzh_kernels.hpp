@@ -33,10 +33,10 @@ hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt
 
 // Stage 3: BVH, one workgroup per step. nodes has node_cap entries per step.
 // Also writes the depth-first traversal copy `trav` (node_cap per step);
-// nolc_buf and ps_buf are node_cap-sized scratch per step.
+// nolc_buf, ps_buf and pos_buf are node_cap-sized scratch per step.
 hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, mzh_bvh_node *nodes,
-                      TravNode *trav, uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *tree_size, uint32_t n_objs,
-                      uint64_t node_cap, uint32_t n_steps, hipStream_t stream);
+                      TravNode *trav, uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *pos_buf, uint32_t *tree_size,
+                      uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream);
 
 // Stage 4: broad phase (count, per-step scan, step scan, write + per-object sort).
 hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
@@ -66,10 +66,18 @@ struct NarrowArgs {
     unsigned long long *conj_count;
     unsigned long long *np_report;  // [MZH_NP_COUNT]
     DevError *err;
+    // overlapping trajectory-step pairs of every broad-phase pair
+    void *steps;                  // StepPair records
+    uint64_t steps_cap;
+    unsigned long long *steps_count;
     // queue of step-pairs whose dist^2 enclosure reaches below thresh^2
-    void *cand;                   // records of cand_stride bytes
+    void *cand;                   // records of narrow_cand_bytes(order) bytes
     uint64_t cand_cap;
     unsigned long long *cand_count;
+};
+// One overlapping pair of trajectory steps of a broad-phase pair.
+struct StepPair {
+    uint32_t i, j, c, ki, kj;
 };
 // Bytes of one queued root-finding candidate for a polynomial order.
 size_t narrow_cand_bytes(uint32_t order);

@@ -596,16 +596,46 @@ __device__ inline void translate1s(const double *__restrict__ src, double a, dou
     }
 }
 
-// Narrow phase, part 1: one lane per broad-phase pair. Merge-walk of the
-// overlapping trajectory steps (narrow_phase.cpp:357-434), translation of the
-// position polynomials (:472-488), dist^2 interpolation (:95-152) and the
-// Cargo-Shisha screen (:502-513); survivors go to the candidate queue.
-template <int ORDER>
-__global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
+// trajectory-step range of an object overlapping [cb, ce) (the searches of
+// compute_object_aabb, detect_conjunctions_aabbs.cpp:130-142)
+__device__ inline void step_range(const double *t, uint64_t ns, double cb, double ce, uint64_t &kb, uint64_t &ke)
 {
-    constexpr int OP1 = ORDER + 1;
+    uint64_t l = 0, m = ns;
+    while (m > 0) {
+        const uint64_t h = m / 2;
+        if (!(cb < t[1 + l + h])) {
+            l += h + 1;
+            m -= h + 1;
+        } else {
+            m = h;
+        }
+    }
+    kb = l;
+    m = ns - l;
+    while (m > 0) {
+        const uint64_t h = m / 2;
+        if (t[1 + l + h] < ce) {
+            l += h + 1;
+            m -= h + 1;
+        } else {
+            m = h;
+        }
+    }
+    ke = l + (l != ns ? 1 : 0);
+}
+
+// Narrow phase, part 1a: one lane per broad-phase pair. Merge-walk of the
+// trajectory steps of i and j over the conjunction step
+// (narrow_phase.cpp:357-420); every temporally overlapping pair of steps is
+// appended to the step-pair list (wave-aggregated atomics).
+__global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
+{
     const uint64_t p = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    unsigned long long n_cand = 0, n_rej = 0;
+    bool more = false;
+    uint64_t ki = 0, kj = 0, kei = 0, kej = 0;
+    uint32_t i = 0, j = 0, c = 0;
+    const double *ti = nullptr, *tj = nullptr;
+    double cb = 0, ce = 0;
     if (p < a.n_pairs) {
         uint32_t lo = 0, n = a.sg.n_steps;
         while (n > 0) {
@@ -617,115 +647,138 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
                 n = h;
             }
         }
-        const uint32_t c = lo;
-        const uint32_t i = a.pairs[p].i, j = a.pairs[p].j;
+        c = lo;
+        i = a.pairs[p].i;
+        j = a.pairs[p].j;
+        cb = a.sg.cd_begin[c];
+        ce = a.sg.cd_end[c];
+        ti = a.pj.time + a.pj.time_off[i];
+        tj = a.pj.time + a.pj.time_off[j];
+        step_range(ti, a.pj.nsteps[i], cb, ce, ki, kei);
+        step_range(tj, a.pj.nsteps[j], cb, ce, kj, kej);
+        more = ki < kei && kj < kej;
+    }
+    // each round, every lane advances to its next overlapping step-pair (if
+    // any); rounds continue while any lane of the wave has steps left
+    while (__ballot(more) != 0ull) {
+        bool found = false;
+        uint32_t fki = 0, fkj = 0;
+        while (more && !found) {
+            const double tsi = ti[ki], tei = ti[ki + 1], tsj = tj[kj], tej = tj[kj + 1];
+            const double lb_i = cb > tsi ? cb : tsi, ub_i = ce < tei ? ce : tei;
+            const double lb_j = cb > tsj ? cb : tsj, ub_j = ce < tej ? ce : tej;
+            if (ub_i > lb_j && lb_i < ub_j) {
+                found = true;
+                fki = (uint32_t)ki;
+                fkj = (uint32_t)kj;
+            }
+            const bool inc_i = tei <= tej, inc_j = tej <= tei;
+            ki += inc_i ? 1 : 0;
+            kj += inc_j ? 1 : 0;
+            more = ki < kei && kj < kej;
+        }
+        // wave-aggregated append
+        const uint64_t bal = __ballot(found);
+        const uint32_t total = (uint32_t)__popcll(bal);
+        unsigned long long base = 0;
+        if (__lane_id() == 0 && total) {
+            base = atomicAdd(a.steps_count, (unsigned long long)total);
+            atomicAdd(&a.np_report[0], (unsigned long long)total);
+        }
+        base = __shfl(base, 0, 64);
+        if (found) {
+            const uint64_t slot = base + (uint64_t)__popcll(bal & lanemask_lt());
+            if (slot < a.steps_cap) static_cast<StepPair *>(a.steps)[slot] = StepPair{i, j, c, fki, fkj};
+        }
+    }
+}
+
+// Narrow phase, part 1b: one lane per overlapping step-pair. Root-finding
+// interval and translations (narrow_phase.cpp:404-488), dist^2
+// interpolation (:95-152) and the Cargo-Shisha screen (:502-513); survivors
+// go to the candidate queue.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
+{
+    constexpr int OP1 = ORDER + 1;
+    const unsigned long long avail = *a.steps_count;
+    const uint64_t nsp = avail < a.steps_cap ? avail : a.steps_cap;
+    const uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    unsigned long long n_rej = 0;
+    if (q < nsp) {
+        const StepPair sp = static_cast<const StepPair *>(a.steps)[q];
+        const uint32_t i = sp.i, j = sp.j, c = sp.c;
+        const uint64_t ki = sp.ki, kj = sp.kj;
         const double cb = a.sg.cd_begin[c], ce = a.sg.cd_end[c];
         const double thresh2 = a.conj_thresh * a.conj_thresh;
         const uint64_t nsi = a.pj.nsteps[i], nsj = a.pj.nsteps[j];
         const double *ti = a.pj.time + a.pj.time_off[i];
         const double *tj = a.pj.time + a.pj.time_off[j];
-        auto range = [](const double *t, uint64_t ns, double cbv, double cev, uint64_t &kb, uint64_t &ke) {
-            uint64_t l = 0, m = ns;
-            while (m > 0) {
-                const uint64_t h = m / 2;
-                if (!(cbv < t[1 + l + h])) {
-                    l += h + 1;
-                    m -= h + 1;
-                } else {
-                    m = h;
-                }
-            }
-            kb = l;
-            m = ns - l;
-            while (m > 0) {
-                const uint64_t h = m / 2;
-                if (t[1 + l + h] < cev) {
-                    l += h + 1;
-                    m -= h + 1;
-                } else {
-                    m = h;
-                }
-            }
-            ke = l + (l != ns ? 1 : 0);
-        };
-        uint64_t kbi, kei, kbj, kej;
-        range(ti, nsi, cb, ce, kbi, kei);
-        range(tj, nsj, cb, ce, kbj, kej);
-        for (uint64_t ki = kbi, kj = kbj; ki < kei && kj < kej;) {
-            const double tsi = ti[ki], tei = ti[ki + 1], tsj = tj[kj], tej = tj[kj + 1];
-            const double lb_i = cb > tsi ? cb : tsi, ub_i = ce < tei ? ce : tei;
-            const double lb_j = cb > tsj ? cb : tsj, ub_j = ce < tej ? ce : tej;
-            if (ub_i > lb_j && lb_i < ub_j) {
-                ++n_cand;
-                const double lb_rf = lb_i > lb_j ? lb_i : lb_j;
-                const double ub_rf = ub_i < ub_j ? ub_i : ub_j;
-                const double di = lb_rf - tsi, dj = lb_rf - tsj;
-                const double rf_int = ub_rf - lb_rf;
-                if (!is_finite_d(di) || !is_finite_d(dj) || !is_finite_d(rf_int) || di < 0 || dj < 0 || rf_int < 0) {
-                    report_error(a.err, kErrNpInterval, i, j, rf_int);
-                    break;
-                }
-                // positions from the step-major copy: channel ch of a step at cf4[(s * 4 + ch) * OP1]
-                const double *Qi = a.pj.cf4 + (a.pj.step0[i] + ki) * 4u * OP1;
-                const double *Qj = a.pj.cf4 + (a.pj.step0[j] + kj) * 4u * OP1;
-                double nodes[OP1];
+        const double tsi = ti[ki], tei = ti[ki + 1], tsj = tj[kj], tej = tj[kj + 1];
+        const double lb_i = cb > tsi ? cb : tsi, ub_i = ce < tei ? ce : tei;
+        const double lb_j = cb > tsj ? cb : tsj, ub_j = ce < tej ? ce : tej;
+        const double lb_rf = lb_i > lb_j ? lb_i : lb_j;
+        const double ub_rf = ub_i < ub_j ? ub_i : ub_j;
+        const double di = lb_rf - tsi, dj = lb_rf - tsj;
+        const double rf_int = ub_rf - lb_rf;
+        bool ok = true;
+        if (!is_finite_d(di) || !is_finite_d(dj) || !is_finite_d(rf_int) || di < 0 || dj < 0 || rf_int < 0) {
+            report_error(a.err, kErrNpInterval, i, j, rf_int);
+            ok = false;
+        }
+        if (ok) {
+            // positions from the step-major copy: channel ch of a step at cf4[(s * 4 + ch) * OP1]
+            const double *Qi = a.pj.cf4 + (a.pj.step0[i] + ki) * 4u * OP1;
+            const double *Qj = a.pj.cf4 + (a.pj.step0[j] + kj) * 4u * OP1;
+            double nodes[OP1];
 #pragma unroll
-                for (int m = 0; m < OP1; ++m) nodes[OP1 - 1 - m] = a.cheby[m] * rf_int;
-                double d2[OP1];
+            for (int m = 0; m < OP1; ++m) nodes[OP1 - 1 - m] = a.cheby[m] * rf_int;
+            double d2[OP1];
 #pragma unroll 1
-                for (int ch = 0; ch < 3; ++ch) {
-                    double tp[OP1], vi[OP1];
-                    translate1s<ORDER>(Qi + ch * OP1, di, tp);
+            for (int ch = 0; ch < 3; ++ch) {
+                double tp[OP1], vi[OP1];
+                translate1s<ORDER>(Qi + ch * OP1, di, tp);
 #pragma unroll
-                    for (int m = 0; m < OP1; ++m) vi[m] = horner<ORDER>(tp, nodes[m]);
-                    translate1s<ORDER>(Qj + ch * OP1, dj, tp);
+                for (int m = 0; m < OP1; ++m) vi[m] = horner<ORDER>(tp, nodes[m]);
+                translate1s<ORDER>(Qj + ch * OP1, dj, tp);
 #pragma unroll
-                    for (int m = 0; m < OP1; ++m) {
-                        const double dx = vi[m] - horner<ORDER>(tp, nodes[m]);
-                        d2[m] = (ch == 0) ? dx * dx : d2[m] + dx * dx;
-                    }
-                }
-                vm_interp<ORDER>(nodes, d2);
-                double elo, ehi;
-                cs_enc0<ORDER>(d2, rf_int, elo, ehi);
-                if (!is_finite_d(elo) || !is_finite_d(ehi)) {
-                    report_error(a.err, kErrNpDist2, i, j, elo);
-                    break;
-                }
-                if (elo < thresh2) {
-                    uint32_t flags = 0;
-                    if ((ki + 1 == nsi && tei == ub_rf) || (kj + 1 == nsj && tej == ub_rf)) flags |= kCandEnd;
-                    if ((ki == 0 && lb_rf == ti[0]) || (kj == 0 && lb_rf == tj[0])) flags |= kCandBegin;
-                    const unsigned long long slot = atomicAdd(a.cand_count, 1ull);
-                    if (slot < a.cand_cap) {
-                        NpCand<ORDER> &q = static_cast<NpCand<ORDER> *>(a.cand)[slot];
-#pragma unroll
-                        for (int k = 0; k < OP1; ++k) q.d2[k] = d2[k];
-                        q.lb_rf = lb_rf;
-                        q.rf_int = rf_int;
-                        q.di = di;
-                        q.dj = dj;
-                        q.c = c;
-                        q.i = i;
-                        q.j = j;
-                        q.flags = flags;
-                        q.ki = ki;
-                        q.kj = kj;
-                    }
-                } else {
-                    ++n_rej;
+                for (int m = 0; m < OP1; ++m) {
+                    const double dx = vi[m] - horner<ORDER>(tp, nodes[m]);
+                    d2[m] = (ch == 0) ? dx * dx : d2[m] + dx * dx;
                 }
             }
-            const bool inc_i = tei <= tej, inc_j = tej <= tei;
-            ki += inc_i ? 1 : 0;
-            kj += inc_j ? 1 : 0;
+            vm_interp<ORDER>(nodes, d2);
+            double elo, ehi;
+            cs_enc0<ORDER>(d2, rf_int, elo, ehi);
+            if (!is_finite_d(elo) || !is_finite_d(ehi)) {
+                report_error(a.err, kErrNpDist2, i, j, elo);
+            } else if (elo < thresh2) {
+                uint32_t flags = 0;
+                if ((ki + 1 == nsi && tei == ub_rf) || (kj + 1 == nsj && tej == ub_rf)) flags |= kCandEnd;
+                if ((ki == 0 && lb_rf == ti[0]) || (kj == 0 && lb_rf == tj[0])) flags |= kCandBegin;
+                const unsigned long long slot = atomicAdd(a.cand_count, 1ull);
+                if (slot < a.cand_cap) {
+                    NpCand<ORDER> &cd = static_cast<NpCand<ORDER> *>(a.cand)[slot];
+#pragma unroll
+                    for (int k = 0; k < OP1; ++k) cd.d2[k] = d2[k];
+                    cd.lb_rf = lb_rf;
+                    cd.rf_int = rf_int;
+                    cd.di = di;
+                    cd.dj = dj;
+                    cd.c = c;
+                    cd.i = i;
+                    cd.j = j;
+                    cd.flags = flags;
+                    cd.ki = ki;
+                    cd.kj = kj;
+                }
+            } else {
+                ++n_rej;
+            }
         }
     }
-    const unsigned long long s0 = wave_reduce_add_u64(n_cand), s1 = wave_reduce_add_u64(n_rej);
-    if (__lane_id() == 0) {
-        if (s0) atomicAdd(&a.np_report[0], s0);
-        if (s1) atomicAdd(&a.np_report[1], s1);
-    }
+    const unsigned long long s1 = wave_reduce_add_u64(n_rej);
+    if (__lane_id() == 0 && s1) atomicAdd(&a.np_report[1], s1);
 }
 
 // Narrow phase, part 2: one lane per queued candidate (grid-stride over the
@@ -887,8 +940,13 @@ hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream)
 {
     const uint64_t blocks = (a.n_pairs + 255) / 256;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_np_screen<ORDER>, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_np_enum, dim3((unsigned)blocks), dim3(256), 0, stream, a);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // one lane per step-pair; the list length is read on the device
+    const uint64_t sblocks = (a.steps_cap + 255) / 256;
+    hipLaunchKernelGGL(k_np_screen<ORDER>, dim3((unsigned)sblocks), dim3(256), 0, stream, a);
+    e = hipGetLastError();
     if (e != hipSuccess) return e;
     // the queue length is only known on the device: a fixed grid strides over it
     const uint64_t qblocks = std::min<uint64_t>((a.cand_cap + 255) / 256, 4096);

@@ -600,8 +600,8 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
                                                      const float *__restrict__ srt_aabbs,
                                                      const uint32_t *__restrict__ n_eff, mzh_bvh_node *__restrict__ nodes,
                                                      TravNode *__restrict__ trav, uint32_t *__restrict__ nolc_buf,
-                                                     uint32_t *__restrict__ ps_buf, uint32_t *__restrict__ tree_size,
-                                                     uint32_t N, uint64_t node_cap)
+                                                     uint32_t *__restrict__ ps_buf, uint32_t *__restrict__ pos_buf,
+                                                     uint32_t *__restrict__ tree_size, uint32_t N, uint64_t node_cap)
 {
     const uint32_t c = blockIdx.x;
     const uint32_t n = n_eff[c];
@@ -613,12 +613,32 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
     const uint64_t *codes = srt_mcodes + (size_t)c * N;
     const float *boxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
     mzh_bvh_node *T = nodes + (size_t)c * node_cap;
-    uint32_t *nolc = nolc_buf + (size_t)c * node_cap;
+    TravNode *TV = trav + (size_t)c * node_cap;
+    uint32_t *nolc = nolc_buf + (size_t)c * node_cap;  // [0, n): per-level split sizes
+    uint32_t *uq = nolc + N;                            // [N, N + n): unique-code boundary counts
     uint32_t *ps = ps_buf + (size_t)c * node_cap;
+    uint32_t *pos = pos_buf + (size_t)c * node_cap;     // preorder position of every node
     __shared__ uint32_t lvl[kBvhMaxLevels + 2];
     __shared__ uint32_t s_w[65];
 
-    if (tid == 0) init_node(T[0], 0, n);
+    // uq[q] = number of boundaries q' in [1, q] between different codes, so
+    // that a node over [b, e) has 1 + uq[e-1] - uq[b] leaves and
+    // 2 (1 + uq[e-1] - uq[b]) - 1 nodes in its subtree.
+    {
+        uint32_t carry = 0;
+        for (uint32_t cb = 0; cb < n; cb += kBvhBlock) {
+            const uint32_t q = cb + tid;
+            const uint32_t f = (q < n && q > 0 && codes[q - 1] != codes[q]) ? 1u : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan_1024(f, s_w, tot);
+            if (q < n) uq[q] = carry + ex + f;
+            carry += tot;
+        }
+    }
+    if (tid == 0) {
+        init_node(T[0], 0, n);
+        pos[0] = 0;
+    }
     __syncthreads();
     uint32_t lb = 0, le = 1, nlev = 0;
     for (;;) {
@@ -662,11 +682,20 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
                     u[2] = u[2] > bu.z ? u[2] : bu.z;
                     u[3] = u[3] > bu.w ? u[3] : bu.w;
                 }
+                TravNode tn;
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     nd.lb[d] = l[d];
                     nd.ub[d] = u[d];
+                    tn.lb[d] = l[d];
+                    tn.ub[d] = u[d];
                 }
+                const uint32_t pn = pos[lb + k];
+                tn.escape = pn + 1u;
+                tn.begin = b;
+                tn.end = e;
+                tn.pad = 0;
+                TV[pn] = tn;
             }
         }
         __syncthreads();
@@ -695,8 +724,13 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
                 const uint32_t lc = le + ps[k] - 2u;
                 nd.left = (int32_t)lc;
                 nd.right = (int32_t)(lc + 1u);
-                init_node(T[lc], nd.begin, nd.begin + sp);
-                init_node(T[lc + 1], nd.begin + sp, nd.end);
+                const uint32_t b = nd.begin, mid = nd.begin + sp;
+                init_node(T[lc], b, mid);
+                init_node(T[lc + 1], mid, nd.end);
+                const uint32_t pn = pos[lb + k];
+                const uint32_t left_size = 2u * (1u + uq[mid - 1] - uq[b]) - 1u;
+                pos[lc] = pn + 1u;
+                pos[lc + 1] = pn + 1u + left_size;
             }
         }
         __syncthreads();
@@ -707,7 +741,8 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
     }
     if (tid == 0) lvl[nlev] = le;
     __syncthreads();
-    // bottom-up refit, one level at a time
+    // bottom-up refit, one level at a time; internal nodes also write their
+    // traversal copy (escape = position + subtree size)
     for (int j = (int)nlev - 2; j >= 0; --j) {
         const uint32_t a = lvl[j], b = lvl[j + 1];
         for (uint32_t ni = a + tid; ni < b; ni += kBvhBlock) {
@@ -715,53 +750,23 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
             if (nd.left != -1) {
                 const mzh_bvh_node &L = T[nd.left];
                 const mzh_bvh_node &R = T[nd.right];
+                TravNode tn;
 #pragma unroll
                 for (int d = 0; d < 4; ++d) {
                     nd.lb[d] = L.lb[d] < R.lb[d] ? L.lb[d] : R.lb[d];
                     nd.ub[d] = L.ub[d] > R.ub[d] ? L.ub[d] : R.ub[d];
+                    tn.lb[d] = nd.lb[d];
+                    tn.ub[d] = nd.ub[d];
                 }
+                const uint32_t pn = pos[ni];
+                tn.escape = pn + 2u * (1u + uq[nd.end - 1] - uq[nd.begin]) - 1u;
+                tn.begin = nd.begin;
+                tn.end = 0;
+                tn.pad = 0;
+                TV[pn] = tn;
             }
         }
         __syncthreads();
-    }
-    // depth-first traversal copy: subtree sizes bottom-up (into nolc), preorder
-    // positions top-down (into ps), then one scatter with escape indices
-    uint32_t *sz = nolc, *pos = ps;
-    for (int j = (int)nlev - 1; j >= 0; --j) {
-        const uint32_t a = lvl[j], b = lvl[j + 1];
-        for (uint32_t ni = a + tid; ni < b; ni += kBvhBlock) {
-            const mzh_bvh_node &nd = T[ni];
-            sz[ni] = (nd.left == -1) ? 1u : 1u + sz[nd.left] + sz[nd.right];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) pos[0] = 0;
-    __syncthreads();
-    for (uint32_t j = 0; j + 1 < nlev; ++j) {
-        const uint32_t a = lvl[j], b = lvl[j + 1];
-        for (uint32_t ni = a + tid; ni < b; ni += kBvhBlock) {
-            const mzh_bvh_node &nd = T[ni];
-            if (nd.left != -1) {
-                pos[nd.left] = pos[ni] + 1u;
-                pos[nd.right] = pos[ni] + 1u + sz[nd.left];
-            }
-        }
-        __syncthreads();
-    }
-    TravNode *TV = trav + (size_t)c * node_cap;
-    for (uint32_t ni = tid; ni < le; ni += kBvhBlock) {
-        const mzh_bvh_node &nd = T[ni];
-        TravNode tn;
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            tn.lb[d] = nd.lb[d];
-            tn.ub[d] = nd.ub[d];
-        }
-        tn.escape = pos[ni] + sz[ni];
-        tn.begin = nd.begin;
-        tn.end = (nd.left == -1) ? nd.end : 0u;
-        tn.pad = 0;
-        TV[pos[ni]] = tn;
     }
     if (tid == 0) tree_size[c] = le;
 }
@@ -784,28 +789,44 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
 {
     const uint32_t bps = (N + 255u) / 256u;
     const uint32_t c = blockIdx.x / bps;
-    const uint32_t o = (blockIdx.x - c * bps) * 256u + threadIdx.x;
+    const uint32_t o0 = (blockIdx.x - c * bps) * 256u;
+    const uint32_t ne = n_eff[c];
     const uint32_t nn = tree_size[c];
-    if (o >= n_eff[c] || nn == 0) return;
+    // whole wave past the objects with data: nothing to do
+    const uint32_t wave0 = o0 + (threadIdx.x & ~63u);
+    if (wave0 >= ne || nn == 0) return;
+    const uint32_t o = o0 + threadIdx.x;
+    const bool valid = o < ne;
     const TravNode *T = trav + (size_t)c * node_cap;
     const uint32_t *vidx = srt_idx + (size_t)c * N;
-    const float4 ql = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[0];
-    const float4 qu = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[1];
-    const uint32_t orig = vidx[o];
-    const int32_t ot = otypes[orig];
+    float4 ql = make_float4(0.f, 0.f, 0.f, 0.f), qu = ql;
+    uint32_t orig = 0;
+    int32_t ot = 0;
+    if (valid) {
+        ql = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[0];
+        qu = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[1];
+        orig = vidx[o];
+        ot = otypes[orig];
+    }
     uint32_t count = 0;
     mzh_aabb_collision *out = nullptr;
-    if constexpr (WRITE) out = pairs + step_off[c] + off[(size_t)c * N + orig];
-    // stackless preorder walk: descend on overlap, jump to the escape index otherwise
+    if constexpr (WRITE) {
+        if (valid) out = pairs + step_off[c] + off[(size_t)c * N + orig];
+    }
+    // Packet traversal: the 64 lanes of a wave hold neighbouring queries in
+    // Morton order and walk the preorder node array together; a node is
+    // entered when any lane's box overlaps it. The node index is
+    // wave-uniform, so every node is one broadcast load for the whole wave.
     uint32_t i = 0;
     while (i < nn) {
         const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
         const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
         const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
-        const bool overlap = (qu.x >= nl.x && ql.x <= nu.x) && (qu.y >= nl.y && ql.y <= nu.y)
+        const bool overlap = valid && (qu.x >= nl.x && ql.x <= nu.x) && (qu.y >= nl.y && ql.y <= nu.y)
                              && (qu.z >= nl.z && ql.z <= nu.z) && (qu.w >= nl.w && ql.w <= nu.w);
-        if (overlap && meta.z == 0u) {
-            ++i;  // internal node: its left child follows it
+        const bool any = __ballot(overlap) != 0ull;
+        if (any && meta.z == 0u) {
+            ++i;  // internal node entered: its left child follows it
             continue;
         }
         if (overlap) {
@@ -820,6 +841,7 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
         }
         i = meta.x;
     }
+    if (!valid) return;
     if constexpr (WRITE) {
         // insertion sort of this query's pairs by j
         for (uint32_t a = 1; a < count; ++a) {
@@ -968,11 +990,11 @@ hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt
 }
 
 hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, mzh_bvh_node *nodes,
-                      TravNode *trav, uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *tree_size, uint32_t n_objs,
-                      uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
+                      TravNode *trav, uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *pos_buf, uint32_t *tree_size,
+                      uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_bvh, dim3(n_steps), dim3(kBvhBlock), 0, stream, srt_mcodes, srt_aabbs, n_eff, nodes, trav,
-                       nolc_buf, ps_buf, tree_size, n_objs, node_cap);
+                       nolc_buf, ps_buf, pos_buf, tree_size, n_objs, node_cap);
     return hipGetLastError();
 }
 
