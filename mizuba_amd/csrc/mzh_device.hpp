@@ -203,4 +203,37 @@ __device__ inline uint64_t wave_reduce_add_u64(uint64_t v)
     return v;
 }
 
+// Sum over a workgroup of blockDim.x <= 1024 threads; every thread gets the
+// total. s_w needs one slot per wave. Contains barriers: call from all threads.
+__device__ inline unsigned long long block_reduce_add_u64(unsigned long long v, unsigned long long *s_w)
+{
+    v = wave_reduce_add_u64(v);
+    const uint32_t w = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63u) == 0) s_w[w] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    for (uint32_t k = 0; k < nw; ++k) t += s_w[k];
+    return t;
+}
+
+// Exclusive prefix over a workgroup (blockDim.x <= 1024) of per-thread
+// counts, and the block total. s_w needs one slot per wave + 1.
+__device__ inline uint32_t block_excl_scan_u32(uint32_t v, uint32_t *s_w, uint32_t &total)
+{
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+    const uint32_t inc = wave_inclusive_scan(v);
+    __syncthreads();
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (uint32_t k = 0; k < nw; ++k) {
+        const uint32_t x = s_w[k];
+        before += (k < w) ? x : 0u;
+        tot += x;
+    }
+    total = tot;
+    return before + inc - v;
+}
+
 } // namespace mzh

@@ -631,8 +631,7 @@ __device__ inline void step_range(const double *t, uint64_t ns, double cb, doubl
 __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
 {
     const uint64_t p = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    bool more = false;
-    uint64_t ki = 0, kj = 0, kei = 0, kej = 0;
+    uint64_t kbi = 0, kbj = 0, kei = 0, kej = 0;
     uint32_t i = 0, j = 0, c = 0;
     const double *ti = nullptr, *tj = nullptr;
     double cb = 0, ce = 0;
@@ -654,43 +653,38 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
         ce = a.sg.cd_end[c];
         ti = a.pj.time + a.pj.time_off[i];
         tj = a.pj.time + a.pj.time_off[j];
-        step_range(ti, a.pj.nsteps[i], cb, ce, ki, kei);
-        step_range(tj, a.pj.nsteps[j], cb, ce, kj, kej);
-        more = ki < kei && kj < kej;
+        step_range(ti, a.pj.nsteps[i], cb, ce, kbi, kei);
+        step_range(tj, a.pj.nsteps[j], cb, ce, kbj, kej);
     }
-    // each round, every lane advances to its next overlapping step-pair (if
-    // any); rounds continue while any lane of the wave has steps left
-    while (__ballot(more) != 0ull) {
-        bool found = false;
-        uint32_t fki = 0, fkj = 0;
-        while (more && !found) {
+    // the merge-walk visits step pairs in order; pass 1 counts the overlapping
+    // ones, pass 2 writes them at this lane's offset in the block's range
+    auto walk = [&](auto &&emit) {
+        for (uint64_t ki = kbi, kj = kbj; ki < kei && kj < kej;) {
             const double tsi = ti[ki], tei = ti[ki + 1], tsj = tj[kj], tej = tj[kj + 1];
             const double lb_i = cb > tsi ? cb : tsi, ub_i = ce < tei ? ce : tei;
             const double lb_j = cb > tsj ? cb : tsj, ub_j = ce < tej ? ce : tej;
-            if (ub_i > lb_j && lb_i < ub_j) {
-                found = true;
-                fki = (uint32_t)ki;
-                fkj = (uint32_t)kj;
-            }
+            if (ub_i > lb_j && lb_i < ub_j) emit(ki, kj);
             const bool inc_i = tei <= tej, inc_j = tej <= tei;
             ki += inc_i ? 1 : 0;
             kj += inc_j ? 1 : 0;
-            more = ki < kei && kj < kej;
         }
-        // wave-aggregated append
-        const uint64_t bal = __ballot(found);
-        const uint32_t total = (uint32_t)__popcll(bal);
-        unsigned long long base = 0;
-        if (__lane_id() == 0 && total) {
-            base = atomicAdd(a.steps_count, (unsigned long long)total);
-            atomicAdd(&a.np_report[0], (unsigned long long)total);
-        }
-        base = __shfl(base, 0, 64);
-        if (found) {
-            const uint64_t slot = base + (uint64_t)__popcll(bal & lanemask_lt());
-            if (slot < a.steps_cap) static_cast<StepPair *>(a.steps)[slot] = StepPair{i, j, c, fki, fkj};
-        }
+    };
+    uint32_t cnt = 0;
+    walk([&](uint64_t, uint64_t) { ++cnt; });
+    __shared__ uint32_t s_w[17];
+    __shared__ unsigned long long s_base;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan_u32(cnt, s_w, total);
+    if (threadIdx.x == 0) {
+        s_base = total ? atomicAdd(a.steps_count, (unsigned long long)total) : 0ull;
+        if (total) atomicAdd(&a.np_report[0], (unsigned long long)total);
     }
+    __syncthreads();
+    uint64_t slot = s_base + ex;
+    walk([&](uint64_t ki, uint64_t kj) {
+        if (slot < a.steps_cap) static_cast<StepPair *>(a.steps)[slot] = StepPair{i, j, c, (uint32_t)ki, (uint32_t)kj};
+        ++slot;
+    });
 }
 
 // Narrow phase, part 1b: one lane per overlapping step-pair. Root-finding
@@ -705,10 +699,18 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
     const uint64_t nsp = avail < a.steps_cap ? avail : a.steps_cap;
     const uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     unsigned long long n_rej = 0;
+    bool keep = false;
+    uint32_t flags = 0, i = 0, j = 0, c = 0;
+    uint64_t ki = 0, kj = 0;
+    double lb_rf = 0, rf_int = 0, di = 0, dj = 0;
+    double d2[OP1];
     if (q < nsp) {
         const StepPair sp = static_cast<const StepPair *>(a.steps)[q];
-        const uint32_t i = sp.i, j = sp.j, c = sp.c;
-        const uint64_t ki = sp.ki, kj = sp.kj;
+        i = sp.i;
+        j = sp.j;
+        c = sp.c;
+        ki = sp.ki;
+        kj = sp.kj;
         const double cb = a.sg.cd_begin[c], ce = a.sg.cd_end[c];
         const double thresh2 = a.conj_thresh * a.conj_thresh;
         const uint64_t nsi = a.pj.nsteps[i], nsj = a.pj.nsteps[j];
@@ -717,10 +719,11 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
         const double tsi = ti[ki], tei = ti[ki + 1], tsj = tj[kj], tej = tj[kj + 1];
         const double lb_i = cb > tsi ? cb : tsi, ub_i = ce < tei ? ce : tei;
         const double lb_j = cb > tsj ? cb : tsj, ub_j = ce < tej ? ce : tej;
-        const double lb_rf = lb_i > lb_j ? lb_i : lb_j;
+        lb_rf = lb_i > lb_j ? lb_i : lb_j;
         const double ub_rf = ub_i < ub_j ? ub_i : ub_j;
-        const double di = lb_rf - tsi, dj = lb_rf - tsj;
-        const double rf_int = ub_rf - lb_rf;
+        di = lb_rf - tsi;
+        dj = lb_rf - tsj;
+        rf_int = ub_rf - lb_rf;
         bool ok = true;
         if (!is_finite_d(di) || !is_finite_d(dj) || !is_finite_d(rf_int) || di < 0 || dj < 0 || rf_int < 0) {
             report_error(a.err, kErrNpInterval, i, j, rf_int);
@@ -733,7 +736,6 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
             double nodes[OP1];
 #pragma unroll
             for (int m = 0; m < OP1; ++m) nodes[OP1 - 1 - m] = a.cheby[m] * rf_int;
-            double d2[OP1];
 #pragma unroll 1
             for (int ch = 0; ch < 3; ++ch) {
                 double tp[OP1], vi[OP1];
@@ -753,12 +755,32 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
             if (!is_finite_d(elo) || !is_finite_d(ehi)) {
                 report_error(a.err, kErrNpDist2, i, j, elo);
             } else if (elo < thresh2) {
-                uint32_t flags = 0;
+                flags = 0x80000000u;
                 if ((ki + 1 == nsi && tei == ub_rf) || (kj + 1 == nsj && tej == ub_rf)) flags |= kCandEnd;
                 if ((ki == 0 && lb_rf == ti[0]) || (kj == 0 && lb_rf == tj[0])) flags |= kCandBegin;
-                const unsigned long long slot = atomicAdd(a.cand_count, 1ull);
-                if (slot < a.cand_cap) {
-                    NpCand<ORDER> &cd = static_cast<NpCand<ORDER> *>(a.cand)[slot];
+                keep = true;
+            } else {
+                ++n_rej;
+            }
+        }
+    }
+    // block-aggregated append to the candidate queue
+    __shared__ uint32_t s_w[17];
+    __shared__ unsigned long long s_base;
+    __shared__ unsigned long long s_rw[4];
+    uint32_t total;
+    const uint32_t ex = block_excl_scan_u32(keep ? 1u : 0u, s_w, total);
+    const unsigned long long rej = block_reduce_add_u64(n_rej, s_rw);
+    if (threadIdx.x == 0) {
+        s_base = total ? atomicAdd(a.cand_count, (unsigned long long)total) : 0ull;
+        if (rej) atomicAdd(&a.np_report[1], rej);
+    }
+    __syncthreads();
+    if (keep) {
+        const unsigned long long slot = s_base + ex;
+        if (slot < a.cand_cap) {
+            flags &= 0x7fffffffu;
+            NpCand<ORDER> &cd = static_cast<NpCand<ORDER> *>(a.cand)[slot];
 #pragma unroll
                     for (int k = 0; k < OP1; ++k) cd.d2[k] = d2[k];
                     cd.lb_rf = lb_rf;
@@ -771,14 +793,8 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
                     cd.flags = flags;
                     cd.ki = ki;
                     cd.kj = kj;
-                }
-            } else {
-                ++n_rej;
-            }
         }
     }
-    const unsigned long long s1 = wave_reduce_add_u64(n_rej);
-    if (__lane_id() == 0 && s1) atomicAdd(&a.np_report[1], s1);
 }
 
 // Narrow phase, part 2: one lane per queued candidate (grid-stride over the

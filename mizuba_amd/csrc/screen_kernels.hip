@@ -789,44 +789,28 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
 {
     const uint32_t bps = (N + 255u) / 256u;
     const uint32_t c = blockIdx.x / bps;
-    const uint32_t o0 = (blockIdx.x - c * bps) * 256u;
-    const uint32_t ne = n_eff[c];
+    const uint32_t o = (blockIdx.x - c * bps) * 256u + threadIdx.x;
     const uint32_t nn = tree_size[c];
-    // whole wave past the objects with data: nothing to do
-    const uint32_t wave0 = o0 + (threadIdx.x & ~63u);
-    if (wave0 >= ne || nn == 0) return;
-    const uint32_t o = o0 + threadIdx.x;
-    const bool valid = o < ne;
+    if (o >= n_eff[c] || nn == 0) return;
     const TravNode *T = trav + (size_t)c * node_cap;
     const uint32_t *vidx = srt_idx + (size_t)c * N;
-    float4 ql = make_float4(0.f, 0.f, 0.f, 0.f), qu = ql;
-    uint32_t orig = 0;
-    int32_t ot = 0;
-    if (valid) {
-        ql = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[0];
-        qu = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[1];
-        orig = vidx[o];
-        ot = otypes[orig];
-    }
+    const float4 ql = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[0];
+    const float4 qu = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[1];
+    const uint32_t orig = vidx[o];
+    const int32_t ot = otypes[orig];
     uint32_t count = 0;
     mzh_aabb_collision *out = nullptr;
-    if constexpr (WRITE) {
-        if (valid) out = pairs + step_off[c] + off[(size_t)c * N + orig];
-    }
-    // Packet traversal: the 64 lanes of a wave hold neighbouring queries in
-    // Morton order and walk the preorder node array together; a node is
-    // entered when any lane's box overlaps it. The node index is
-    // wave-uniform, so every node is one broadcast load for the whole wave.
+    if constexpr (WRITE) out = pairs + step_off[c] + off[(size_t)c * N + orig];
+    // stackless preorder walk: descend on overlap, jump to the escape index otherwise
     uint32_t i = 0;
     while (i < nn) {
         const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
         const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
         const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
-        const bool overlap = valid && (qu.x >= nl.x && ql.x <= nu.x) && (qu.y >= nl.y && ql.y <= nu.y)
+        const bool overlap = (qu.x >= nl.x && ql.x <= nu.x) && (qu.y >= nl.y && ql.y <= nu.y)
                              && (qu.z >= nl.z && ql.z <= nu.z) && (qu.w >= nl.w && ql.w <= nu.w);
-        const bool any = __ballot(overlap) != 0ull;
-        if (any && meta.z == 0u) {
-            ++i;  // internal node entered: its left child follows it
+        if (overlap && meta.z == 0u) {
+            ++i;  // internal node: its left child follows it
             continue;
         }
         if (overlap) {
@@ -841,7 +825,6 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
         }
         i = meta.x;
     }
-    if (!valid) return;
     if constexpr (WRITE) {
         // insertion sort of this query's pairs by j
         for (uint32_t a = 1; a < count; ++a) {
