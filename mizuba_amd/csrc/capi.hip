@@ -105,7 +105,8 @@ struct mzh_ctx {
     DevBuf aabbs, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, gbox;
     uint64_t node_cap = 0;
     DevBuf nodes, trav, nolc, ps, tpos, tree_size;
-    DevBuf otypes, cnt, off, step_total, step_off, grand;
+    DevBuf otypes, cnt, off, step_total, step_off, grand, slots, ovf, ovf_count;
+    uint32_t ovf_cap = 0;
     DevBuf pairs;
     uint64_t n_pairs = 0;
     DevBuf conj_raw, conj_step, conj_sorted, conj_count, np_report, conj_cnt, conj_start, conj_cursor, conj_pos,
@@ -310,6 +311,10 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(ctx->tree_size.ensure(sizeof(uint32_t) * C));
     MZH_HIP(ctx->otypes.ensure(sizeof(int32_t) * (N ? N : 1)));
     MZH_HIP(ctx->cnt.ensure(sizeof(uint32_t) * NC));
+    MZH_HIP(ctx->slots.ensure(sizeof(uint32_t) * 4u * NC));
+    if (ctx->ovf_cap == 0) ctx->ovf_cap = (uint32_t)std::min<size_t>(NC / 8 + 4096, 0x7fffffffu);
+    MZH_HIP(ctx->ovf.ensure(sizeof(uint32_t) * ctx->ovf_cap));
+    MZH_HIP(ctx->ovf_count.ensure(sizeof(uint32_t)));
     MZH_HIP(ctx->off.ensure(sizeof(uint64_t) * NC));
     MZH_HIP(ctx->step_total.ensure(sizeof(uint64_t) * C));
     MZH_HIP(ctx->step_off.ensure(sizeof(uint64_t) * (C + 1)));
@@ -360,22 +365,35 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     ctx->n_conj = 0;
     if (!skip_cd) {
         // stage 4: broad phase
-        MZH_HIP(launch_bp_count(ctx->trav.as<TravNode>(), ctx->tree_size.as<uint32_t>(),
-                                ctx->srt_aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->n_eff.as<uint32_t>(),
-                                ctx->otypes.as<int32_t>(), ctx->cnt.as<uint32_t>(), N, ctx->node_cap, (uint32_t)C,
-                                st));
-        MZH_HIP(launch_scan_counts(ctx->cnt.as<uint32_t>(), ctx->off.as<uint64_t>(), ctx->step_total.as<uint64_t>(), N,
-                                   (uint32_t)C, st));
-        MZH_HIP(launch_scan_steps(ctx->step_total.as<uint64_t>(), ctx->step_off.as<uint64_t>(),
-                                  ctx->grand.as<uint64_t>(), (uint32_t)C, st));
+        if ((uint64_t)N * C >= 0xffffffffull)
+            throw StatusError{MZH_EINVAL, "Too many objects x conjunction steps for one screening"};
+        uint32_t n_ovf = 0;
+        for (int attempt = 0; attempt < 3; ++attempt) {
+            MZH_HIP(launch_bp_count(ctx->trav.as<TravNode>(), ctx->tree_size.as<uint32_t>(),
+                                    ctx->srt_aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->n_eff.as<uint32_t>(),
+                                    ctx->otypes.as<int32_t>(), ctx->cnt.as<uint32_t>(), ctx->slots.as<uint32_t>(),
+                                    ctx->ovf.as<uint32_t>(), ctx->ovf_count.as<uint32_t>(), ctx->ovf_cap, N,
+                                    ctx->node_cap, (uint32_t)C, st));
+            MZH_HIP(launch_scan_counts(ctx->cnt.as<uint32_t>(), ctx->off.as<uint64_t>(),
+                                       ctx->step_total.as<uint64_t>(), N, (uint32_t)C, st));
+            MZH_HIP(launch_scan_steps(ctx->step_total.as<uint64_t>(), ctx->step_off.as<uint64_t>(),
+                                      ctx->grand.as<uint64_t>(), (uint32_t)C, st));
+            MZH_HIP(hipMemcpyAsync(&n_ovf, ctx->ovf_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            MZH_HIP(hipStreamSynchronize(st));
+            if (n_ovf <= ctx->ovf_cap) break;
+            ctx->ovf_cap = n_ovf + n_ovf / 4 + 4096;
+            MZH_HIP(ctx->ovf.ensure(sizeof(uint32_t) * ctx->ovf_cap));
+        }
+        if (n_ovf > ctx->ovf_cap) throw StatusError{MZH_ERUNTIME, "broad-phase overflow list capacity exceeded"};
         uint64_t total = 0;
-        MZH_HIP(hipMemcpyAsync(&total, ctx->grand.p, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        MZH_HIP(hipStreamSynchronize(st));
+        MZH_HIP(hipMemcpy(&total, ctx->grand.p, sizeof(uint64_t), hipMemcpyDeviceToHost));
         ctx->n_pairs = total;
         MZH_HIP(ctx->pairs.ensure(sizeof(mzh_aabb_collision) * (total ? total : 1)));
         MZH_HIP(launch_bp_write(ctx->trav.as<TravNode>(), ctx->tree_size.as<uint32_t>(),
                                 ctx->srt_aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->n_eff.as<uint32_t>(),
-                                ctx->otypes.as<int32_t>(), ctx->off.as<uint64_t>(), ctx->step_off.as<uint64_t>(),
+                                ctx->otypes.as<int32_t>(), ctx->cnt.as<uint32_t>(), ctx->slots.as<uint32_t>(),
+                                ctx->ovf.as<uint32_t>(), ctx->ovf_count.as<uint32_t>(), ctx->ovf_cap, n_ovf,
+                                ctx->off.as<uint64_t>(), ctx->step_off.as<uint64_t>(),
                                 ctx->pairs.as<mzh_aabb_collision>(), N, ctx->node_cap, (uint32_t)C, st));
         MZH_HIP(hipEventRecord(ctx->ev[4], st));
         // stage 5: narrow phase (re-run with a larger output buffer on overflow)

@@ -176,6 +176,18 @@ __device__ inline uint64_t spread4(uint64_t x)
     return x;
 }
 
+// XCD-aware block order: the dispatcher hands block b to XCD b % 8, so the
+// logical block id x * q + b / 8 keeps consecutive logical blocks (those of
+// one conjunction step) on one XCD and its L2. The tail beyond 8 * q keeps
+// its own index, which makes the map a bijection.
+constexpr uint32_t kNumXcd = 8;
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nblocks)
+{
+    const uint32_t q = nblocks / kNumXcd;
+    if (b >= q * kNumXcd) return b;
+    return (b % kNumXcd) * q + b / kNumXcd;
+}
+
 // ---- wave64 primitives ----------------------------------------------------
 
 __device__ inline uint64_t lanemask_lt()

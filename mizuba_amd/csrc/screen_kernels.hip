@@ -779,17 +779,37 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
 // then places every query's pairs in (i, j) order without a global sort.
 // ---------------------------------------------------------------------------
 
-template <bool WRITE>
+constexpr uint32_t kBpSlots = 4;  // partners kept per query by the single traversal
+
+// Broad phase traversal. COLLECT: every (step, query) records its pair count
+// in cnt[step][i] and its first kBpSlots partners j in slots[step][i][:]; a
+// query with more partners is appended to the overflow list. !COLLECT: the
+// queries of the overflow list are walked again and write their pairs to
+// their final place.
+template <bool COLLECT>
 __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
                                             const uint32_t *__restrict__ tree_size, const float *__restrict__ srt_aabbs,
                                             const uint32_t *__restrict__ srt_idx, const uint32_t *__restrict__ n_eff,
                                             const int32_t *__restrict__ otypes, uint32_t *__restrict__ cnt,
+                                            uint32_t *__restrict__ slots, uint32_t *__restrict__ ovf,
+                                            uint32_t *__restrict__ ovf_count, uint32_t ovf_cap,
                                             const uint64_t *__restrict__ off, const uint64_t *__restrict__ step_off,
                                             mzh_aabb_collision *__restrict__ pairs, uint32_t N, uint64_t node_cap)
 {
-    const uint32_t bps = (N + 255u) / 256u;
-    const uint32_t c = blockIdx.x / bps;
-    const uint32_t o = (blockIdx.x - c * bps) * 256u + threadIdx.x;
+    uint32_t c, o;
+    if constexpr (COLLECT) {
+        const uint32_t bps = (N + 255u) / 256u;
+        const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+        c = lb / bps;
+        o = (lb - c * bps) * 256u + threadIdx.x;
+    } else {
+        const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+        const uint32_t n_ovf = *ovf_count;
+        if (k >= n_ovf || k >= ovf_cap) return;
+        const uint32_t packed = ovf[k];
+        c = packed / N;
+        o = packed - c * N;
+    }
     const uint32_t nn = tree_size[c];
     if (o >= n_eff[c] || nn == 0) return;
     const TravNode *T = trav + (size_t)c * node_cap;
@@ -799,8 +819,9 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
     const uint32_t orig = vidx[o];
     const int32_t ot = otypes[orig];
     uint32_t count = 0;
+    uint32_t *slot = slots + ((size_t)c * N + orig) * kBpSlots;
     mzh_aabb_collision *out = nullptr;
-    if constexpr (WRITE) out = pairs + step_off[c] + off[(size_t)c * N + orig];
+    if constexpr (!COLLECT) out = pairs + step_off[c] + off[(size_t)c * N + orig];
     // stackless preorder walk: descend on overlap, jump to the escape index otherwise
     uint32_t i = 0;
     while (i < nn) {
@@ -818,14 +839,24 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
                 const uint32_t oi = vidx[k];
                 if (orig >= oi) continue;
                 if (ot + otypes[oi] <= 3) {
-                    if constexpr (WRITE) out[count] = mzh_aabb_collision{orig, oi};
+                    if constexpr (COLLECT) {
+                        if (count < kBpSlots) slot[count] = oi;
+                    } else {
+                        out[count] = mzh_aabb_collision{orig, oi};
+                    }
                     ++count;
                 }
             }
         }
         i = meta.x;
     }
-    if constexpr (WRITE) {
+    if constexpr (COLLECT) {
+        cnt[(size_t)c * N + orig] = count;
+        if (count > kBpSlots) {
+            const uint32_t k = atomicAdd(ovf_count, 1u);
+            if (k < ovf_cap) ovf[k] = c * N + o;
+        }
+    } else {
         // insertion sort of this query's pairs by j
         for (uint32_t a = 1; a < count; ++a) {
             const mzh_aabb_collision x = out[a];
@@ -836,9 +867,39 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
             }
             out[b] = x;
         }
-    } else {
-        cnt[(size_t)c * N + orig] = count;
     }
+}
+
+// Place the pairs of every query with at most kBpSlots partners: sorted by j,
+// at the query's offset in (i, j) order.
+__global__ __launch_bounds__(256) void k_bp_place(const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ slots,
+                                                  const uint64_t *__restrict__ off,
+                                                  const uint64_t *__restrict__ step_off,
+                                                  mzh_aabb_collision *__restrict__ pairs, uint32_t N, uint32_t C)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (g >= (uint64_t)N * C) return;
+    const uint32_t n = cnt[g];
+    if (n == 0 || n > kBpSlots) return;
+    const uint32_t c = (uint32_t)(g / N), orig = (uint32_t)(g - (uint64_t)c * N);
+    uint32_t v[kBpSlots];
+#pragma unroll
+    for (uint32_t k = 0; k < kBpSlots; ++k) v[k] = (k < n) ? slots[g * kBpSlots + k] : 0xffffffffu;
+    // sorting network on kBpSlots = 4 values (padding sorts last)
+    auto cs = [](uint32_t &x, uint32_t &y) {
+        const uint32_t a = x < y ? x : y, b = x < y ? y : x;
+        x = a;
+        y = b;
+    };
+    cs(v[0], v[1]);
+    cs(v[2], v[3]);
+    cs(v[0], v[2]);
+    cs(v[1], v[3]);
+    cs(v[1], v[2]);
+    mzh_aabb_collision *out = pairs + step_off[c] + off[g];
+#pragma unroll
+    for (uint32_t k = 0; k < kBpSlots; ++k)
+        if (k < n) out[k] = mzh_aabb_collision{orig, v[k]};
 }
 
 // per-step exclusive scan of cnt[step][0..N) -> off, step totals
@@ -983,12 +1044,15 @@ hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const 
 
 hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
                            const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, uint32_t *cnt,
-                           uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
+                           uint32_t *slots, uint32_t *ovf, uint32_t *ovf_count, uint32_t ovf_cap, uint32_t n_objs,
+                           uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
 {
     hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (size_t)n_objs * n_steps, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(ovf_count, 0, sizeof(uint32_t), stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bp<false>, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, nodes, tree_size,
-                       srt_aabbs, srt_idx, n_eff, otypes, cnt, nullptr, nullptr, nullptr, n_objs, node_cap);
+    hipLaunchKernelGGL(k_bp<true>, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, nodes, tree_size,
+                       srt_aabbs, srt_idx, n_eff, otypes, cnt, slots, ovf, ovf_count, ovf_cap, nullptr, nullptr,
+                       nullptr, n_objs, node_cap);
     return hipGetLastError();
 }
 
@@ -1007,12 +1071,20 @@ hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uin
 }
 
 hipError_t launch_bp_write(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
-                           const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, const uint64_t *off,
-                           const uint64_t *step_off, mzh_aabb_collision *pairs, uint32_t n_objs, uint64_t node_cap,
-                           uint32_t n_steps, hipStream_t stream)
+                           const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, const uint32_t *cnt,
+                           const uint32_t *slots, const uint32_t *ovf, const uint32_t *ovf_count, uint32_t ovf_cap,
+                           uint32_t n_ovf_host, const uint64_t *off, const uint64_t *step_off,
+                           mzh_aabb_collision *pairs, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
+                           hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_bp<true>, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, nodes, tree_size,
-                       srt_aabbs, srt_idx, n_eff, otypes, nullptr, off, step_off, pairs, n_objs, node_cap);
+    const uint64_t nq = (uint64_t)n_objs * n_steps;
+    hipLaunchKernelGGL(k_bp_place, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, stream, cnt, slots, off, step_off,
+                       pairs, n_objs, n_steps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || n_ovf_host == 0) return e;
+    hipLaunchKernelGGL(k_bp<false>, dim3((n_ovf_host + 255u) / 256u), dim3(256), 0, stream, nodes, tree_size,
+                       srt_aabbs, srt_idx, n_eff, otypes, nullptr, nullptr, const_cast<uint32_t *>(ovf),
+                       const_cast<uint32_t *>(ovf_count), ovf_cap, off, step_off, pairs, n_objs, node_cap);
     return hipGetLastError();
 }
 
