@@ -104,7 +104,8 @@ struct mzh_ctx {
     DevBuf cd_begin, cd_end;
     DevBuf aabbs, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, gbox;
     uint64_t node_cap = 0;
-    DevBuf nodes, trav, nolc, ps, tpos, tree_size;
+    DevBuf nodes, trav, bvh_scr, tree_size;
+    bool nodes_ready = false;  // nodes holds the reference-layout export of trav
     DevBuf otypes, cnt, off, step_total, step_off, grand, slots, ovf, ovf_count;
     uint32_t ovf_cap = 0;
     DevBuf pairs;
@@ -303,11 +304,9 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(ctx->vals_tmp.ensure(sizeof(uint32_t) * NC));
     MZH_HIP(ctx->n_eff.ensure(sizeof(uint32_t) * C));
     MZH_HIP(ctx->gbox.ensure(sizeof(uint32_t) * 8u * C));
-    MZH_HIP(ctx->nodes.ensure(sizeof(mzh_bvh_node) * ctx->node_cap * C));
     MZH_HIP(ctx->trav.ensure(sizeof(TravNode) * ctx->node_cap * C));
-    MZH_HIP(ctx->nolc.ensure(sizeof(uint32_t) * ctx->node_cap * C));
-    MZH_HIP(ctx->ps.ensure(sizeof(uint32_t) * ctx->node_cap * C));
-    MZH_HIP(ctx->tpos.ensure(sizeof(uint32_t) * ctx->node_cap * C));
+    MZH_HIP(ctx->bvh_scr.ensure(bvh_scratch_stride(N) * C));
+    ctx->nodes_ready = false;
     MZH_HIP(ctx->tree_size.ensure(sizeof(uint32_t) * C));
     MZH_HIP(ctx->otypes.ensure(sizeof(int32_t) * (N ? N : 1)));
     MZH_HIP(ctx->cnt.ensure(sizeof(uint32_t) * NC));
@@ -357,9 +356,8 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(hipEventRecord(ctx->ev[2], st));
     // stage 3: BVH
     MZH_HIP(launch_bvh(ctx->srt_mcodes.as<uint64_t>(), ctx->srt_aabbs.as<float>(), ctx->n_eff.as<uint32_t>(),
-                       ctx->nodes.as<mzh_bvh_node>(), ctx->trav.as<TravNode>(), ctx->nolc.as<uint32_t>(),
-                       ctx->ps.as<uint32_t>(), ctx->tpos.as<uint32_t>(),
-                       ctx->tree_size.as<uint32_t>(), N, ctx->node_cap, (uint32_t)C, st));
+                       ctx->trav.as<TravNode>(), ctx->bvh_scr.as<uint8_t>(), ctx->tree_size.as<uint32_t>(), N,
+                       ctx->node_cap, (uint32_t)C, st));
     MZH_HIP(hipEventRecord(ctx->ev[3], st));
     ctx->n_pairs = 0;
     ctx->n_conj = 0;
@@ -513,6 +511,18 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
 void require_screened(const mzh_ctx *ctx)
 {
     if (!ctx->screened) throw StatusError{MZH_EINVAL, "No screening results are available in this context"};
+}
+
+// The trees are kept in the broad phase's layout; the reference's 48-byte
+// bvh_node records are produced on the first request after a screen.
+void export_nodes(mzh_ctx *ctx)
+{
+    if (ctx->nodes_ready) return;
+    MZH_HIP(ctx->nodes.ensure(sizeof(mzh_bvh_node) * ctx->node_cap * ctx->C));
+    MZH_HIP(launch_bvh_export(ctx->trav.as<TravNode>(), ctx->tree_size.as<uint32_t>(), ctx->nodes.as<mzh_bvh_node>(),
+                              ctx->node_cap, (uint32_t)ctx->C, ctx->stream));
+    MZH_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->nodes_ready = true;
 }
 
 } // namespace
@@ -726,9 +736,11 @@ int mzh_copy_bvh_tree(mzh_ctx *ctx, uint64_t step, mzh_bvh_node *dst)
                                               + std::to_string(step) + ": the total number of conjunction steps is only "
                                               + std::to_string(ctx->C)};
         const size_t n = ctx->h_tree_size[step];
-        if (n)
+        if (n) {
+            export_nodes(ctx);
             MZH_HIP(hipMemcpy(dst, ctx->nodes.as<mzh_bvh_node>() + step * ctx->node_cap, sizeof(mzh_bvh_node) * n,
                               hipMemcpyDeviceToHost));
+        }
     });
 }
 
@@ -832,6 +844,7 @@ int mzh_device_view(mzh_ctx *ctx, int which, void **ptr, uint64_t *bytes)
                 *bytes = sizeof(uint32_t) * NC;
                 break;
             case 5:
+                export_nodes(ctx);
                 *ptr = ctx->nodes.p;
                 *bytes = sizeof(mzh_bvh_node) * ctx->node_cap * ctx->C;
                 break;

@@ -60,16 +60,20 @@ struct PjView {
     uint32_t order;
 };
 
-// Traversal copy of a BVH node: the tree in depth-first (preorder) order with
-// an escape index (first node after the subtree), so that the broad phase
-// walks it without a stack. end == 0 marks an internal node.
+// BVH node as stored on the device: the reference's breadth-first order
+// (detect_conjunctions_bvh.cpp:521-580) with an escape index (the node that
+// follows the subtree in a depth-first walk), so that the broad phase walks the
+// tree without a stack. left is the index of the left child (the right child
+// is left + 1) and 0 for a leaf, since the root is nobody's child. The
+// escape of the root's right spine is kTravEnd.
+constexpr uint32_t kTravEnd = 0xffffffffu;
 struct alignas(16) TravNode {
     float lb[4];
     float ub[4];
     uint32_t escape;
     uint32_t begin;
     uint32_t end;
-    uint32_t pad;
+    uint32_t left;
 };
 static_assert(sizeof(TravNode) == 48, "TravNode layout");
 

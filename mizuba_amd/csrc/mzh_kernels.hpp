@@ -34,9 +34,14 @@ hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt
 // Stage 3: BVH, one workgroup per step. nodes has node_cap entries per step.
 // Also writes the depth-first traversal copy `trav` (node_cap per step);
 // nolc_buf, ps_buf and pos_buf are node_cap-sized scratch per step.
-hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, mzh_bvh_node *nodes,
-                      TravNode *trav, uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *pos_buf, uint32_t *tree_size,
-                      uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream);
+// Bytes of per-step BVH scratch in HBM (used when it does not fit in LDS).
+size_t bvh_scratch_stride(uint32_t n_objs);
+hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, TravNode *trav,
+                      uint8_t *scr_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
+                      hipStream_t stream);
+// The reference's bvh_node records of every step's tree, from the device layout.
+hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mzh_bvh_node *nodes, uint64_t node_cap,
+                             uint32_t n_steps, hipStream_t stream);
 
 // Stage 4: broad phase. launch_bp_count walks every query once, storing its
 // pair count, its first 4 partners (slots: 4 per object and step) and, for

@@ -582,193 +582,239 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ aabbs,
 // ---------------------------------------------------------------------------
 
 constexpr int kBvhBlock = 1024;
+constexpr size_t kBvhLdsMax = 64u * 1024u;
 
-__device__ inline void init_node(mzh_bvh_node &n, uint32_t b, uint32_t e)
+__device__ inline void box_union(float l[4], float u[4], const float *bx)
 {
-    n.begin = b;
-    n.end = e;
-    n.left = -1;
-    n.right = -1;
+    const float4 bl = reinterpret_cast<const float4 *>(bx)[0];
+    const float4 bu = reinterpret_cast<const float4 *>(bx)[1];
+    l[0] = l[0] < bl.x ? l[0] : bl.x;
+    l[1] = l[1] < bl.y ? l[1] : bl.y;
+    l[2] = l[2] < bl.z ? l[2] : bl.z;
+    l[3] = l[3] < bl.w ? l[3] : bl.w;
+    u[0] = u[0] > bu.x ? u[0] : bu.x;
+    u[1] = u[1] > bu.y ? u[1] : bu.y;
+    u[2] = u[2] > bu.z ? u[2] : bu.z;
+    u[3] = u[3] > bu.w ? u[3] : bu.w;
+}
+
+// Per-step scratch of the BVH build, in LDS when it fits, else in HBM.
+//   D[q]     highest differing bit of codes q-1 and q (-1 if equal), q in [1, n)
+//   dmax[k]  maximum of D over the 64 positions of chunk k
+struct BvhScratch {
+    int8_t *D;
+    int8_t *dmax;
+};
+
+__host__ __device__ inline size_t bvh_scratch_bytes(uint32_t n)
+{
+    const size_t nch = ((size_t)n + 63u) / 64u;
+    return nch * 64u + (nch + 15u) / 16u * 16u;
+}
+
+__device__ inline BvhScratch bvh_scratch_at(uint8_t *base, uint32_t n)
+{
+    const size_t nch = ((size_t)n + 63u) / 64u;
+    BvhScratch sc;
+    sc.D = reinterpret_cast<int8_t *>(base);
+    sc.dmax = reinterpret_cast<int8_t *>(base + nch * 64u);
+    return sc;
+}
+
+// Maximum of the signed bytes A[lo..hi] and the first position holding it,
+// folded into (best, at); reads whole aligned 16-byte groups.
+__device__ inline void max_bytes(const int8_t *A, uint32_t lo, uint32_t hi, int &best, uint32_t &at)
+{
+    for (uint32_t g = lo & ~15u; g <= hi; g += 16u) {
+        const uint4 w = *reinterpret_cast<const uint4 *>(A + g);
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        n.lb[d] = __builtin_inff();
-        n.ub[d] = -__builtin_inff();
+        for (uint32_t j = 0; j < 16u; ++j) {
+            const uint32_t p = g + j;
+            const int v = (int)(int8_t)(ws[j >> 2] >> (8u * (j & 3u)));
+            if (p >= lo && p <= hi && v > best) {
+                best = v;
+                at = p;
+            }
+        }
     }
 }
 
+// Position of the strongest boundary in [lo, hi] (lo <= hi), and its strength
+// (-1 if the codes over the range are all equal). Inside a BVH node the
+// strongest boundary is unique: its highest differing bit flips once.
+__device__ inline int bvh_split(const BvhScratch &sc, uint32_t lo, uint32_t hi, uint32_t &arg)
+{
+    int best = -1;
+    uint32_t at = lo;
+    const uint32_t c0 = (lo + 63u) >> 6, c1 = (hi + 1u) >> 6;  // full chunks [c0, c1)
+    if (c0 >= c1) {
+        max_bytes(sc.D, lo, hi, best, at);
+    } else {
+        if (lo < c0 * 64u) max_bytes(sc.D, lo, c0 * 64u - 1u, best, at);
+        int cb = -1;
+        uint32_t ck = c0;
+        max_bytes(sc.dmax, c0, c1 - 1u, cb, ck);
+        if (cb > best) {
+            int b2 = -1;
+            max_bytes(sc.D, ck * 64u, ck * 64u + 63u, b2, at);
+            best = cb;
+        }
+        if (hi >= c1 * 64u) max_bytes(sc.D, c1 * 64u, hi, best, at);
+    }
+    arg = at;
+    return best;
+}
+
+// BVH build over one conjunction step (detect_conjunctions_bvh.cpp:267-699).
+//
+// The tree is the binary radix tree of the sorted Morton codes: a node over
+// [b, e) whose end codes differ splits at the one position where its highest
+// differing bit flips, which is the strongest boundary (highest differing bit
+// of neighbouring codes) inside the range; a run of equal codes is one
+// multi-object leaf. Nodes are laid out level by level in the reference's
+// breadth-first order (children of a level appended in order, left before
+// right, detect_conjunctions_bvh.cpp:521-580) and refitted bottom-up
+// (:629-691). Boundary strengths and their per-chunk maxima live in LDS, so
+// the split search touches no global memory, and consecutive lanes read and
+// write consecutive node records. The escape index comes top-down: a left
+// child escapes to its right sibling, a right child to its parent's escape.
+//
+// While a node waits for its level, its record holds (escape, begin, end).
+template <bool LDS>
 __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ srt_mcodes,
                                                      const float *__restrict__ srt_aabbs,
-                                                     const uint32_t *__restrict__ n_eff, mzh_bvh_node *__restrict__ nodes,
-                                                     TravNode *__restrict__ trav, uint32_t *__restrict__ nolc_buf,
-                                                     uint32_t *__restrict__ ps_buf, uint32_t *__restrict__ pos_buf,
+                                                     const uint32_t *__restrict__ n_eff, TravNode *__restrict__ trav,
+                                                     uint8_t *__restrict__ scr_buf, size_t scr_stride,
                                                      uint32_t *__restrict__ tree_size, uint32_t N, uint64_t node_cap)
 {
+    extern __shared__ __align__(16) uint8_t s_dyn[];
     const uint32_t c = blockIdx.x;
     const uint32_t n = n_eff[c];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
     if (n == 0) {
         if (tid == 0) tree_size[c] = 0;
         return;
     }
     const uint64_t *codes = srt_mcodes + (size_t)c * N;
     const float *boxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
-    mzh_bvh_node *T = nodes + (size_t)c * node_cap;
-    TravNode *TV = trav + (size_t)c * node_cap;
-    uint32_t *nolc = nolc_buf + (size_t)c * node_cap;  // [0, n): per-level split sizes
-    uint32_t *uq = nolc + N;                            // [N, N + n): unique-code boundary counts
-    uint32_t *ps = ps_buf + (size_t)c * node_cap;
-    uint32_t *pos = pos_buf + (size_t)c * node_cap;     // preorder position of every node
+    TravNode *T = trav + (size_t)c * node_cap;
+    const BvhScratch sc = bvh_scratch_at(LDS ? s_dyn : scr_buf + (size_t)c * scr_stride, n);
+    const uint32_t nch = (n + 63u) / 64u;
     __shared__ uint32_t lvl[kBvhMaxLevels + 2];
     __shared__ uint32_t s_w[65];
 
-    // uq[q] = number of boundaries q' in [1, q] between different codes, so
-    // that a node over [b, e) has 1 + uq[e-1] - uq[b] leaves and
-    // 2 (1 + uq[e-1] - uq[b]) - 1 nodes in its subtree.
-    {
-        uint32_t carry = 0;
-        for (uint32_t cb = 0; cb < n; cb += kBvhBlock) {
-            const uint32_t q = cb + tid;
-            const uint32_t f = (q < n && q > 0 && codes[q - 1] != codes[q]) ? 1u : 0u;
-            uint32_t tot;
-            const uint32_t ex = block_excl_scan_1024(f, s_w, tot);
-            if (q < n) uq[q] = carry + ex + f;
-            carry += tot;
+    // ---- boundary strengths and chunk maxima (one wave per chunk) ----------
+    for (uint32_t q = tid; q < nch * 64u; q += kBvhBlock) {
+        int d = -1;
+        if (q > 0 && q < n) {
+            const uint64_t x = codes[q - 1] ^ codes[q];
+            d = x ? (int)(63u - (uint32_t)__clzll(x)) : -1;
         }
+        sc.D[q] = (int8_t)d;
+        int m = d;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const int o = __shfl_xor(m, off, 64);
+            m = m > o ? m : o;
+        }
+        if (lane == 0) sc.dmax[q >> 6] = (int8_t)m;
     }
-    if (tid == 0) {
-        init_node(T[0], 0, n);
-        pos[0] = 0;
-    }
+    if (tid == 0) reinterpret_cast<uint4 *>(&T[0])[2] = make_uint4(kTravEnd, 0u, n, 0u);
     __syncthreads();
+
+    // ---- top-down, one level at a time ------------------------------------
     uint32_t lb = 0, le = 1, nlev = 0;
     for (;;) {
         if (tid == 0) lvl[nlev] = lb;
         const uint32_t cnt = le - lb;
-        for (uint32_t k = tid; k < cnt; k += kBvhBlock) {
-            mzh_bvh_node &nd = T[lb + k];
-            const uint32_t b = nd.begin, e = nd.end;
-            uint32_t split = 0;
-            if (e - b > 1u) {
-                const uint64_t x = codes[b] ^ codes[e - 1];
-                if (x != 0) {
-                    const uint32_t bit = 63u - (uint32_t)__clzll(x);
-                    // first position in [b, e) whose code has `bit` set
-                    uint32_t lo = b, m = e - b;
-                    while (m > 0) {
-                        const uint32_t h = m / 2;
-                        if (((codes[lo + h] >> bit) & 1ull) == 0) {
-                            lo += h + 1;
-                            m -= h + 1;
-                        } else {
-                            m = h;
-                        }
-                    }
-                    split = lo - b;
-                }
-            }
-            nolc[k] = split;
-            if (split == 0) {
-                float l[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
-                float u[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-                for (uint32_t ob = b; ob != e; ++ob) {
-                    const float4 bl = reinterpret_cast<const float4 *>(boxes + (size_t)ob * 8u)[0];
-                    const float4 bu = reinterpret_cast<const float4 *>(boxes + (size_t)ob * 8u)[1];
-                    l[0] = l[0] < bl.x ? l[0] : bl.x;
-                    l[1] = l[1] < bl.y ? l[1] : bl.y;
-                    l[2] = l[2] < bl.z ? l[2] : bl.z;
-                    l[3] = l[3] < bl.w ? l[3] : bl.w;
-                    u[0] = u[0] > bu.x ? u[0] : bu.x;
-                    u[1] = u[1] > bu.y ? u[1] : bu.y;
-                    u[2] = u[2] > bu.z ? u[2] : bu.z;
-                    u[3] = u[3] > bu.w ? u[3] : bu.w;
-                }
-                TravNode tn;
-#pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    nd.lb[d] = l[d];
-                    nd.ub[d] = u[d];
-                    tn.lb[d] = l[d];
-                    tn.ub[d] = u[d];
-                }
-                const uint32_t pn = pos[lb + k];
-                tn.escape = pn + 1u;
-                tn.begin = b;
-                tn.end = e;
-                tn.pad = 0;
-                TV[pn] = tn;
-            }
-        }
-        __syncthreads();
-        // inclusive scan of the child counts (0 or 2)
         uint32_t carry = 0;
         for (uint32_t cb = 0; cb < cnt; cb += kBvhBlock) {
             const uint32_t k = cb + tid;
-            const uint32_t v = (k < cnt && nolc[k] != 0) ? 2u : 0u;
-            uint32_t tot;
-            const uint32_t ex = block_excl_scan_1024(v, s_w, tot);
-            if (k < cnt) ps[k] = carry + ex + v;
-            carry += tot;
-        }
-        __syncthreads();
-        const uint32_t total = carry;
-        // A tree over n objects has at most 2n - 1 nodes, so this cannot fire
-        // with node_cap >= 2N; it keeps every write in bounds regardless.
-        if (le + total > node_cap) {
-            if (tid == 0) tree_size[c] = 0xffffffffu;
-            return;
-        }
-        for (uint32_t k = tid; k < cnt; k += kBvhBlock) {
-            const uint32_t sp = nolc[k];
-            if (sp != 0) {
-                mzh_bvh_node &nd = T[lb + k];
-                const uint32_t lc = le + ps[k] - 2u;
-                nd.left = (int32_t)lc;
-                nd.right = (int32_t)(lc + 1u);
-                const uint32_t b = nd.begin, mid = nd.begin + sp;
-                init_node(T[lc], b, mid);
-                init_node(T[lc + 1], mid, nd.end);
-                const uint32_t pn = pos[lb + k];
-                const uint32_t left_size = 2u * (1u + uq[mid - 1] - uq[b]) - 1u;
-                pos[lc] = pn + 1u;
-                pos[lc + 1] = pn + 1u + left_size;
+            uint32_t esc = 0, b = 0, e = 0, q = 0;
+            bool inner = false;
+            if (k < cnt) {
+                const uint4 h = reinterpret_cast<const uint4 *>(&T[lb + k])[2];
+                esc = h.x;
+                b = h.y;
+                e = h.z;
+                if (e - b > 1u) inner = bvh_split(sc, b + 1u, e - 1u, q) >= 0;
             }
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan_1024(inner ? 2u : 0u, s_w, tot);
+            if ((uint64_t)le + carry + tot > node_cap) {
+                if (tid == 0) tree_size[c] = 0xffffffffu;
+                return;
+            }
+            if (k < cnt) {
+                uint4 *nd = reinterpret_cast<uint4 *>(&T[lb + k]);
+                if (inner) {
+                    const uint32_t lc = le + carry + ex;
+                    reinterpret_cast<uint4 *>(&T[lc])[2] = make_uint4(lc + 1u, b, q, 0u);
+                    reinterpret_cast<uint4 *>(&T[lc + 1u])[2] = make_uint4(esc, q, e, 0u);
+                    nd[2] = make_uint4(esc, b, e, lc);
+                } else {
+                    float l[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+                    float u[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+                    for (uint32_t o = b; o != e; ++o) box_union(l, u, boxes + (size_t)o * 8u);
+                    reinterpret_cast<float4 *>(nd)[0] = make_float4(l[0], l[1], l[2], l[3]);
+                    reinterpret_cast<float4 *>(nd)[1] = make_float4(u[0], u[1], u[2], u[3]);
+                    nd[2] = make_uint4(esc, b, e, 0u);
+                }
+            }
+            carry += tot;
         }
         __syncthreads();
         ++nlev;
         lb = le;
-        le = le + total;
-        if (total == 0 || nlev > kBvhMaxLevels) break;
+        le += carry;
+        if (carry == 0 || nlev > kBvhMaxLevels) break;
     }
     if (tid == 0) lvl[nlev] = le;
     __syncthreads();
-    // bottom-up refit, one level at a time; internal nodes also write their
-    // traversal copy (escape = position + subtree size)
+
+    // ---- bottom-up refit of the internal nodes ----------------------------
     for (int j = (int)nlev - 2; j >= 0; --j) {
-        const uint32_t a = lvl[j], b = lvl[j + 1];
-        for (uint32_t ni = a + tid; ni < b; ni += kBvhBlock) {
-            mzh_bvh_node &nd = T[ni];
-            if (nd.left != -1) {
-                const mzh_bvh_node &L = T[nd.left];
-                const mzh_bvh_node &R = T[nd.right];
-                TravNode tn;
-#pragma unroll
-                for (int d = 0; d < 4; ++d) {
-                    nd.lb[d] = L.lb[d] < R.lb[d] ? L.lb[d] : R.lb[d];
-                    nd.ub[d] = L.ub[d] > R.ub[d] ? L.ub[d] : R.ub[d];
-                    tn.lb[d] = nd.lb[d];
-                    tn.ub[d] = nd.ub[d];
-                }
-                const uint32_t pn = pos[ni];
-                tn.escape = pn + 2u * (1u + uq[nd.end - 1] - uq[nd.begin]) - 1u;
-                tn.begin = nd.begin;
-                tn.end = 0;
-                tn.pad = 0;
-                TV[pn] = tn;
+        const uint32_t a = lvl[j], z = lvl[j + 1];
+        for (uint32_t ni = a + tid; ni < z; ni += kBvhBlock) {
+            const uint32_t left = T[ni].left;
+            if (left != 0u) {
+                float l[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+                float u[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+                box_union(l, u, T[left].lb);
+                box_union(l, u, T[left + 1u].lb);
+                reinterpret_cast<float4 *>(&T[ni])[0] = make_float4(l[0], l[1], l[2], l[3]);
+                reinterpret_cast<float4 *>(&T[ni])[1] = make_float4(u[0], u[1], u[2], u[3]);
             }
         }
         __syncthreads();
     }
     if (tid == 0) tree_size[c] = le;
+}
+
+// The reference's node records (conjunctions.hpp:65-72) from the device
+// layout, for the steps' trees as exposed by get_bvh_tree.
+__global__ __launch_bounds__(256) void k_bvh_export(const TravNode *__restrict__ trav,
+                                                    const uint32_t *__restrict__ tree_size,
+                                                    mzh_bvh_node *__restrict__ nodes, uint64_t node_cap,
+                                                    uint32_t blocks_per_step)
+{
+    const uint32_t c = blockIdx.x / blocks_per_step;
+    const uint32_t i = (blockIdx.x - c * blocks_per_step) * 256u + threadIdx.x;
+    const uint32_t nn = tree_size[c];
+    if (i >= nn) return;
+    const TravNode t = trav[(size_t)c * node_cap + i];
+    mzh_bvh_node r;
+    r.begin = t.begin;
+    r.end = t.end;
+    r.left = t.left ? (int32_t)t.left : -1;
+    r.right = t.left ? (int32_t)(t.left + 1u) : -1;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        r.lb[d] = t.lb[d];
+        r.ub[d] = t.ub[d];
+    }
+    nodes[(size_t)c * node_cap + i] = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -822,7 +868,7 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
     uint32_t *slot = slots + ((size_t)c * N + orig) * kBpSlots;
     mzh_aabb_collision *out = nullptr;
     if constexpr (!COLLECT) out = pairs + step_off[c] + off[(size_t)c * N + orig];
-    // stackless preorder walk: descend on overlap, jump to the escape index otherwise
+    // stackless depth-first walk: descend on overlap, jump to the escape index otherwise
     uint32_t i = 0;
     while (i < nn) {
         const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
@@ -830,8 +876,8 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
         const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
         const bool overlap = (qu.x >= nl.x && ql.x <= nu.x) && (qu.y >= nl.y && ql.y <= nu.y)
                              && (qu.z >= nl.z && ql.z <= nu.z) && (qu.w >= nl.w && ql.w <= nu.w);
-        if (overlap && meta.z == 0u) {
-            ++i;  // internal node: its left child follows it
+        if (overlap && meta.w != 0u) {
+            i = meta.w;  // internal node: descend to the left child
             continue;
         }
         if (overlap) {
@@ -1033,12 +1079,30 @@ hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt
     return hipGetLastError();
 }
 
-hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, mzh_bvh_node *nodes,
-                      TravNode *trav, uint32_t *nolc_buf, uint32_t *ps_buf, uint32_t *pos_buf, uint32_t *tree_size,
-                      uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
+size_t bvh_scratch_stride(uint32_t n_objs) { return (bvh_scratch_bytes(n_objs) + 255u) / 256u * 256u; }
+
+hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, TravNode *trav,
+                      uint8_t *scr_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
+                      hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_bvh, dim3(n_steps), dim3(kBvhBlock), 0, stream, srt_mcodes, srt_aabbs, n_eff, nodes, trav,
-                       nolc_buf, ps_buf, pos_buf, tree_size, n_objs, node_cap);
+    // LDS when two workgroups per CU still fit; otherwise the scratch is in HBM
+    const size_t bytes = bvh_scratch_bytes(n_objs);
+    if (bytes <= kBvhLdsMax) {
+        hipLaunchKernelGGL(k_bvh<true>, dim3(n_steps), dim3(kBvhBlock), bytes, stream, srt_mcodes, srt_aabbs, n_eff,
+                           trav, scr_buf, bvh_scratch_stride(n_objs), tree_size, n_objs, node_cap);
+    } else {
+        hipLaunchKernelGGL(k_bvh<false>, dim3(n_steps), dim3(kBvhBlock), 0, stream, srt_mcodes, srt_aabbs, n_eff,
+                           trav, scr_buf, bvh_scratch_stride(n_objs), tree_size, n_objs, node_cap);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mzh_bvh_node *nodes, uint64_t node_cap,
+                             uint32_t n_steps, hipStream_t stream)
+{
+    const uint32_t bps = (uint32_t)((node_cap + 255u) / 256u);
+    if (n_steps == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bvh_export, dim3(bps * n_steps), dim3(256), 0, stream, trav, tree_size, nodes, node_cap, bps);
     return hipGetLastError();
 }
 
