@@ -58,6 +58,51 @@ def aabb_bytes_per_launch(pj, interval):
     return pieces * per_piece + C * pj.n_objs * 32, pieces / (C * pj.n_objs)
 
 
+def broad_bytes_per_launch(n_objs, n_steps, n_pairs):
+    """Algorithmic bytes of one broad-phase traversal launch: per (object,
+    conjunction step) the query box (32 B), its sorted-to-original index and
+    object type (8 B), its pair count (4 B) and its share of the step's tree
+    read once (2 nodes x 48 B), plus one 4 B partner index per pair."""
+    return n_steps * n_objs * (32 + 8 + 4 + 96) + 4 * n_pairs
+
+
+def bvh_bytes_per_launch(n_objs, n_steps):
+    """Algorithmic bytes of one BVH launch: per (object, conjunction step) the
+    sorted code (8 B) and box (32 B) read once and two 48 B nodes written."""
+    return n_steps * n_objs * (8 + 32 + 96)
+
+
+def hbm_traffic(kernel_prefix):
+    """Measured HBM bytes per dispatch of a kernel, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel calibration
+    recorded in the same file), or None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "screen_30k_1day_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].startswith(kernel_prefix):
+            try:
+                return int(float(cells[4]))
+            except ValueError:
+                return None
+    return None
+
+
+def copy_calibration(mib):
+    """Plain streaming kernel of known size for the PMC calibration: reads and
+    writes `mib` MiB three times (an elementwise torch kernel, not a DMA copy)."""
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,8 +111,9 @@ def main():
     ap.add_argument("--n-objs", type=int, default=30000)
     ap.add_argument("--days", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=2025)
-    ap.add_argument("--cpu-sample-steps", type=int, default=64)
+    ap.add_argument("--cpu-sample-steps", type=int, default=0, help="conjunction steps timed on the CPU (0: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0, help="only run the PMC copy calibration")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -77,6 +123,10 @@ def main():
             file=sys.stderr,
         )
         sys.exit(2)
+
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
+        return
 
     import mizuba_amd as mz
 
@@ -124,24 +174,42 @@ def main():
     def avg(stages, key):
         return float(np.mean([s[key] for s in stages]))
 
-    # dominant kernel + roofline
+    # per-kernel device times (HIP events on the library's stream) and rooflines
     stage_keys = ("aabbs", "morton", "bvh", "broad", "narrow")
-    st_avg = {k: avg(st_mixed, k) for k in stage_keys + ("total", "aabb_kernel", "narrow_kernel")}
-    st_avg_fp64 = {k: avg(st_fp64, k) for k in stage_keys + ("total", "aabb_kernel", "narrow_kernel")}
+    kern_keys = ("aabb_kernel", "narrow_kernel", "broad_kernel")
+    st_avg = {k: avg(st_mixed, k) for k in stage_keys + ("total",) + kern_keys}
+    st_avg_fp64 = {k: avg(st_fp64, k) for k in stage_keys + ("total",) + kern_keys}
+    C = int(np.ceil(pj.maxT / interval))
     aabb_bytes, kbar = aabb_bytes_per_launch(pj, interval)
-    aabb_ms = st_avg_fp64["aabb_kernel"] * 1e3
-    aabb_gbs = aabb_bytes / st_avg_fp64["aabb_kernel"] / 1e9
+    kernels = {
+        "k_aabbs<8>": (st_avg_fp64["aabb_kernel"], aabb_bytes),
+        "k_bvh": (st_avg_fp64["bvh"], bvh_bytes_per_launch(pj.n_objs, C)),
+        "k_bp<true>": (st_avg_fp64["broad_kernel"], broad_bytes_per_launch(pj.n_objs, C, pairs_fp64)),
+    }
+    per_kernel = {
+        name: {
+            "avg_launch_ms": round(t * 1e3, 4),
+            "algorithmic_bytes": b,
+            "achieved_gbs": round(b / t / 1e9, 1),
+            "frac_hbm": round(b / t / 1e9 / HBM_PEAK_GBS, 4),
+        }
+        for name, (t, b) in kernels.items()
+    }
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    t_dom, b_dom = kernels[dom]
+    gbs = b_dom / t_dom / 1e9
     roofline = {
-        "kernel": "k_aabbs<8> (FP64 drop-in AABB enclosures)",
+        "kernel": dom,
         "bound": "hbm",
-        "achieved": round(aabb_gbs, 1),
+        "achieved": round(gbs, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(aabb_gbs / HBM_PEAK_GBS, 4),
-        "traffic": None,
-        "algorithmic_bytes": aabb_bytes,
-        "avg_launch_ms": round(aabb_ms, 4),
+        "frac": round(gbs / HBM_PEAK_GBS, 4),
+        "traffic": hbm_traffic(dom),
+        "algorithmic_bytes": b_dom,
+        "avg_launch_ms": round(t_dom * 1e3, 4),
         "kbar": round(kbar, 4),
+        "kernels": per_kernel,
     }
 
     cpu = None
@@ -152,7 +220,7 @@ def main():
         traj, tm, toff, ns, tmo = pj._flat()
         flat = po.Flat.from_arrays(pj.poly_order, traj, tm, ns, toff, tmo, pj.maxT)
         threads = po.n_threads_default()
-        nsamp = min(args.cpu_sample_steps, int(np.ceil(pj.maxT / interval)))
+        nsamp = C if args.cpu_sample_steps <= 0 else min(args.cpu_sample_steps, C)
         t = time.perf_counter()
         ref = po.run(flat, thresh, interval, step_end=nsamp, n_threads=threads)
         dt_cpu = time.perf_counter() - t
@@ -162,8 +230,8 @@ def main():
             "unit": "object-days/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"first {nsamp} of {int(np.ceil(pj.maxT / interval))} conjunction steps "
-            f"({sample_days:.4f} days) of the same 30k-object polyjectory, wall {dt_cpu:.2f}s",
+            "sample": f"first {nsamp} of {C} conjunction steps ({sample_days:.4f} days) of the same "
+            f"{pj.n_objs}-object polyjectory, wall {dt_cpu:.2f}s",
         }
         # TCA/DCA error of the mixed GPU pipeline against the FP64 oracle on the sample
         t_end = ref["cd_end_times"][-1]

@@ -79,8 +79,9 @@ typedef struct {
 
 /* Stage timings reported by mzh_get_stage_times (seconds, device clock):
  * aabbs, morton (codes + sort + gather), bvh, broad, narrow, total, then the
- * AABB kernel alone and the (last) narrow-phase kernel alone. */
-#define MZH_STAGE_COUNT 8
+ * AABB kernel alone, the narrow-phase kernels alone and the broad-phase
+ * traversal kernel alone. */
+#define MZH_STAGE_COUNT 9
 
 typedef struct mzh_ctx mzh_ctx;
 

@@ -116,8 +116,8 @@ struct mzh_ctx {
     DevBuf cand, cand_count, steps, steps_count;
     uint64_t cand_cap = 0, steps_cap = 0;
     DevBuf err;
-    hipEvent_t ev[10] = {};
-    double stage_s[MZH_STAGE_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0};
+    hipEvent_t ev[12] = {};
+    double stage_s[MZH_STAGE_COUNT] = {};
     uint64_t h_np[MZH_NP_COUNT] = {0, 0, 0, 0, 0, 0, 0};
 
     // host caches of per-step sizes (filled after a screen)
@@ -367,11 +367,13 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
             throw StatusError{MZH_EINVAL, "Too many objects x conjunction steps for one screening"};
         uint32_t n_ovf = 0;
         for (int attempt = 0; attempt < 3; ++attempt) {
+            MZH_HIP(hipEventRecord(ctx->ev[10], st));
             MZH_HIP(launch_bp_count(ctx->trav.as<TravNode>(), ctx->tree_size.as<uint32_t>(),
                                     ctx->srt_aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->n_eff.as<uint32_t>(),
                                     ctx->otypes.as<int32_t>(), ctx->cnt.as<uint32_t>(), ctx->slots.as<uint32_t>(),
                                     ctx->ovf.as<uint32_t>(), ctx->ovf_count.as<uint32_t>(), ctx->ovf_cap, N,
                                     ctx->node_cap, (uint32_t)C, st));
+            MZH_HIP(hipEventRecord(ctx->ev[11], st));
             MZH_HIP(launch_scan_counts(ctx->cnt.as<uint32_t>(), ctx->off.as<uint64_t>(),
                                        ctx->step_total.as<uint64_t>(), N, (uint32_t)C, st));
             MZH_HIP(launch_scan_steps(ctx->step_total.as<uint64_t>(), ctx->step_off.as<uint64_t>(),
@@ -494,6 +496,9 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(hipEventElapsedTime(&kn, ctx->ev[8], ctx->ev[9]));
     ctx->stage_s[6] = ka * 1e-3;
     ctx->stage_s[7] = kn * 1e-3;
+    float kb = 0;
+    if (!skip_cd) MZH_HIP(hipEventElapsedTime(&kb, ctx->ev[10], ctx->ev[11]));
+    ctx->stage_s[8] = kb * 1e-3;
     unsigned long long rep[MZH_NP_COUNT];
     MZH_HIP(hipMemcpy(rep, ctx->np_report.p, sizeof(rep), hipMemcpyDeviceToHost));
     for (int k = 0; k < MZH_NP_COUNT; ++k) ctx->h_np[k] = rep[k];
@@ -568,7 +573,7 @@ int mzh_create(int device, mzh_ctx **out)
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    for (int k = 0; k < 10 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev[k]);
+    for (int k = 0; k < 12 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev[k]);
     if (e != hipSuccess) {
         g_global_error = std::string("HIP error while creating a context: ") + hipGetErrorString(e);
         delete ctx;

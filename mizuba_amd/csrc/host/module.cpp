@@ -475,7 +475,7 @@ PYBIND11_MODULE(_core, m)
                  double t[MZH_STAGE_COUNT];
                  check(mzh_get_stage_times(c.ctx, t), c.ctx);
                  py::dict d;
-                 const char *names[] = {"aabbs", "morton", "bvh", "broad", "narrow", "total", "aabb_kernel", "narrow_kernel"};
+                 const char *names[] = {"aabbs", "morton", "bvh", "broad", "narrow", "total", "aabb_kernel", "narrow_kernel", "broad_kernel"};
                  for (int k = 0; k < MZH_STAGE_COUNT; ++k) d[names[k]] = t[k];
                  return d;
              })
@@ -616,7 +616,7 @@ PYBIND11_MODULE(_core, m)
         double t[MZH_STAGE_COUNT];
         check(mzh_get_stage_times(c.impl->ctx->ctx, t), c.impl->ctx->ctx);
         py::dict d;
-        const char *names[] = {"aabbs", "morton", "bvh", "broad", "narrow", "total", "aabb_kernel", "narrow_kernel"};
+        const char *names[] = {"aabbs", "morton", "bvh", "broad", "narrow", "total", "aabb_kernel", "narrow_kernel", "broad_kernel"};
         for (int k = 0; k < MZH_STAGE_COUNT; ++k) d[names[k]] = t[k];
         return d;
     });
