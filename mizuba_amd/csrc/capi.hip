@@ -102,7 +102,7 @@ struct mzh_ctx {
     int mode = MZH_MODE_FP64;
     std::vector<double> h_cd_begin, h_cd_end;
     DevBuf cd_begin, cd_end;
-    DevBuf aabbs, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, gbox;
+    DevBuf aabbs, kbke, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, gbox;
     uint64_t node_cap = 0;
     DevBuf nodes, trav, bvh_scr, tree_size;
     bool nodes_ready = false;  // nodes holds the reference-layout export of trav
@@ -296,6 +296,7 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(ctx->cd_begin.ensure(sizeof(double) * C));
     MZH_HIP(ctx->cd_end.ensure(sizeof(double) * C));
     MZH_HIP(ctx->aabbs.ensure(boxes_bytes));
+    MZH_HIP(ctx->kbke.ensure(sizeof(uint2) * NC));
     MZH_HIP(ctx->srt_aabbs.ensure(boxes_bytes));
     MZH_HIP(ctx->mcodes.ensure(sizeof(uint64_t) * NC));
     MZH_HIP(ctx->srt_mcodes.ensure(sizeof(uint64_t) * NC));
@@ -342,7 +343,8 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(hipEventRecord(ctx->ev[0], st));
     // stage 1: AABBs
     MZH_HIP(hipEventRecord(ctx->ev[6], st));
-    MZH_HIP(launch_aabbs(pj, sg, conj_thresh, mode, ctx->aabbs.as<float>(), ctx->gbox.as<uint32_t>(), derr, st));
+    MZH_HIP(launch_aabbs(pj, sg, conj_thresh, mode, ctx->aabbs.as<float>(), ctx->kbke.as<uint2>(),
+                         ctx->gbox.as<uint32_t>(), derr, st));
     MZH_HIP(hipEventRecord(ctx->ev[7], st));
     MZH_HIP(launch_gbox_finalize(ctx->gbox.as<uint32_t>(), ctx->aabbs.as<float>(), N, (uint32_t)C, derr, st));
     MZH_HIP(hipEventRecord(ctx->ev[1], st));
@@ -400,6 +402,7 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
         NarrowArgs na;
         na.pj = pj;
         na.sg = sg;
+        na.kbke = ctx->kbke.as<const uint2>();
         na.pairs = ctx->pairs.as<const mzh_aabb_collision>();
         na.step_off = ctx->step_off.as<const uint64_t>();
         na.n_pairs = total;
@@ -623,6 +626,8 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
             if (traj_off[o] % step_sz) throw StatusError{MZH_EINVAL, "Misaligned trajectory offset"};
             step0[o] = traj_off[o] / step_sz;
             if (step0[o] + nsteps[o] > steps_total) throw StatusError{MZH_EINVAL, "Trajectory offsets out of range"};
+            if (nsteps[o] >= 0xffffffffull)
+                throw StatusError{MZH_EINVAL, "Too many trajectory steps for one object"};
             if (nsteps[o] && time_off[o] + nsteps[o] + 1 > time_len)
                 throw StatusError{MZH_EINVAL, "Time offsets out of range"};
         }

@@ -18,8 +18,10 @@ hipError_t launch_transpose_cf4(const double *traj, double *cf4, uint64_t n_step
 
 // Stage 1: per-(step, object) AABBs plus the per-step global box.
 // mode: MZH_MODE_FP64 or MZH_MODE_MIXED.
+// Also stores, per (step, object), the range [kb, ke) of the object's
+// trajectory steps overlapping the conjunction step, for the narrow phase.
 hipError_t launch_aabbs(const PjView &pj, const StepGrid &sg, double conj_thresh, int mode, float *aabbs,
-                        uint32_t *gbox_ord, DevError *err, hipStream_t stream);
+                        uint2 *kbke, uint32_t *gbox_ord, DevError *err, hipStream_t stream);
 hipError_t launch_gbox_finalize(const uint32_t *gbox_ord, float *aabbs, uint32_t n_objs, uint32_t n_steps,
                                 DevError *err, hipStream_t stream);
 
@@ -67,6 +69,7 @@ hipError_t launch_bp_write(const TravNode *nodes, const uint32_t *tree_size, con
 struct NarrowArgs {
     PjView pj;
     StepGrid sg;
+    const uint2 *kbke;  // [C][N] overlapping trajectory-step ranges from the AABB stage
     const mzh_aabb_collision *pairs;
     const uint64_t *step_off;  // [C + 1], exclusive prefix of the per-step pair counts
     uint64_t n_pairs;

@@ -596,34 +596,6 @@ __device__ inline void translate1s(const double *__restrict__ src, double a, dou
     }
 }
 
-// trajectory-step range of an object overlapping [cb, ce) (the searches of
-// compute_object_aabb, detect_conjunctions_aabbs.cpp:130-142)
-__device__ inline void step_range(const double *t, uint64_t ns, double cb, double ce, uint64_t &kb, uint64_t &ke)
-{
-    uint64_t l = 0, m = ns;
-    while (m > 0) {
-        const uint64_t h = m / 2;
-        if (!(cb < t[1 + l + h])) {
-            l += h + 1;
-            m -= h + 1;
-        } else {
-            m = h;
-        }
-    }
-    kb = l;
-    m = ns - l;
-    while (m > 0) {
-        const uint64_t h = m / 2;
-        if (t[1 + l + h] < ce) {
-            l += h + 1;
-            m -= h + 1;
-        } else {
-            m = h;
-        }
-    }
-    ke = l + (l != ns ? 1 : 0);
-}
-
 // Narrow phase, part 1a: one lane per broad-phase pair. Merge-walk of the
 // trajectory steps of i and j over the conjunction step
 // (narrow_phase.cpp:357-420); every temporally overlapping pair of steps is
@@ -653,8 +625,12 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
         ce = a.sg.cd_end[c];
         ti = a.pj.time + a.pj.time_off[i];
         tj = a.pj.time + a.pj.time_off[j];
-        step_range(ti, a.pj.nsteps[i], cb, ce, kbi, kei);
-        step_range(tj, a.pj.nsteps[j], cb, ce, kbj, kej);
+        // overlapping trajectory steps of both objects, as found by the AABB stage
+        const uint2 ri = a.kbke[(size_t)c * a.pj.n_objs + i], rj = a.kbke[(size_t)c * a.pj.n_objs + j];
+        kbi = ri.x;
+        kei = ri.y;
+        kbj = rj.x;
+        kej = rj.y;
     }
     // the merge-walk visits step pairs in order; pass 1 counts the overlapping
     // ones, pass 2 writes them at this lane's offset in the block's range

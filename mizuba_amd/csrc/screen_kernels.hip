@@ -185,74 +185,89 @@ __device__ inline void cs_enclosure_fp32(const double *__restrict__ cf, double l
     emax = bmax + margin;
 }
 
+// Conjunction steps handled by one thread of the AABB kernel: the thread walks
+// its object's time array once across the group instead of searching it per
+// step, and consecutive steps reuse the trajectory pieces it has just read.
+constexpr uint32_t kAabbGroup = 16;
+
+// Stage 1: per (object, conjunction step) boxes (detect_conjunctions_aabbs.cpp:64-252)
+// and the per-step global box (:314-397). One thread per object and group of
+// kAabbGroup consecutive steps; the 256 lanes of a block are 256 consecutive
+// objects, so each step's boxes are written coalesced and reduced per block.
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double radius, int mode,
-                                               float *__restrict__ aabbs, uint32_t *__restrict__ gbox_ord,
-                                               DevError *err)
+                                               float *__restrict__ aabbs, uint2 *__restrict__ kbke,
+                                               uint32_t *__restrict__ gbox_ord, DevError *err)
 {
-    constexpr int OP1 = ORDER + 1;
     const uint32_t N = pj.n_objs;
-    const uint32_t bps = (N + 255u) / 256u;
-    const uint32_t c = blockIdx.x / bps;
-    const uint32_t o = (blockIdx.x - c * bps) * 256u + threadIdx.x;
-    float lb4[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
-    float ub4[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+    const uint32_t bpo = (N + 255u) / 256u;
+    const uint32_t g = blockIdx.x / bpo;
+    const uint32_t o = (blockIdx.x - g * bpo) * 256u + threadIdx.x;
+    const uint32_t c0 = g * kAabbGroup, c1 = min(sg.n_steps, c0 + kAabbGroup);
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    __shared__ uint32_t s_l[4][4], s_u[4][4];
+
+    uint64_t ns = 0, kb = 0;
+    const double *t = nullptr;
+    const double *cfb = nullptr;
     if (o < N) {
-        const uint64_t ns = pj.nsteps[o];
-        const double cb = sg.cd_begin[c], ce = sg.cd_end[c];
+        ns = pj.nsteps[o];
         if (ns > 0) {
-            const double *t = pj.time + pj.time_off[o];
-            if (!(t[0] >= ce || t[ns] <= cb)) {
-                // upper_bound(t + 1, t + ns + 1, cb) -> index kb into t[1..ns]
-                uint64_t lo = 0, n = ns;
-                while (n > 0) {
-                    const uint64_t h = n / 2;
-                    if (!(cb < t[1 + lo + h])) {
-                        lo += h + 1;
-                        n -= h + 1;
-                    } else {
-                        n = h;
-                    }
+            t = pj.time + pj.time_off[o];
+            cfb = pj.cf4 + (pj.step0[o] * 4u) * (ORDER + 1);
+            // upper_bound(t + 1, t + ns + 1, cd_begin[c0]): first piece ending after the group's start
+            const double cb = sg.cd_begin[c0];
+            uint64_t lo = 0, n = ns;
+            while (n > 0) {
+                const uint64_t h = n / 2;
+                if (!(cb < t[1 + lo + h])) {
+                    lo += h + 1;
+                    n -= h + 1;
+                } else {
+                    n = h;
                 }
-                const uint64_t kb = lo;
-                // lower_bound(ts_begin, t_end, ce)
-                n = ns - kb;
-                while (n > 0) {
-                    const uint64_t h = n / 2;
-                    if (t[1 + lo + h] < ce) {
-                        lo += h + 1;
-                        n -= h + 1;
-                    } else {
-                        n = h;
-                    }
-                }
-                const uint64_t ke = lo + (lo != ns ? 1 : 0);
-                const double *cfb = pj.cf4 + (pj.step0[o] * 4u) * OP1;
+            }
+            kb = lo;
+        }
+    }
+    for (uint32_t c = c0; c < c1; ++c) {
+        float lb4[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
+        float ub4[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+        if (o < N) {
+            const double cb = sg.cd_begin[c], ce = sg.cd_end[c];
+            uint2 range = make_uint2(0u, 0u);
+            if (ns > 0 && !(t[0] >= ce || t[ns] <= cb)) {
+                // kb: upper_bound of cb (monotone in c); ke: lower_bound of ce from kb, plus one
+                while (kb < ns && !(cb < t[1 + kb])) ++kb;
+                uint64_t ke = kb;
+                while (ke < ns && t[1 + ke] < ce) ++ke;
+                ke += (ke != ns) ? 1 : 0;
+                range = make_uint2((uint32_t)kb, (uint32_t)ke);
                 for (uint64_t k = kb; k < ke; ++k) {
                     const double ss = t[k];
                     const double ev_lb = cb > ss ? cb : ss;
                     const double ev_ub = ce < t[k + 1] ? ce : t[k + 1];
                     const double h_lb = ev_lb - ss, h_ub = ev_ub - ss;
-                    const double *cf = cfb + k * 4u * OP1;
+                    const double *cf = cfb + k * 4u * (ORDER + 1);
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
                         float fl, fu;
                         if (mode == MZH_MODE_FP64) {
                             double emin, emax;
-                            cs_enclosure_fp64<ORDER>(cf + d * OP1, h_lb, h_ub, emin, emax);
+                            cs_enclosure_fp64<ORDER>(cf + d * (ORDER + 1), h_lb, h_ub, emin, emax);
                             emin = emin - radius;
                             emax = emax + radius;
                             fl = next_down_f((float)emin);
                             fu = next_up_f((float)emax);
                         } else {
                             float emin, emax;
-                            cs_enclosure_fp32<ORDER>(cf + d * OP1, h_lb, h_ub, emin, emax);
+                            cs_enclosure_fp32<ORDER>(cf + d * (ORDER + 1), h_lb, h_ub, emin, emax);
                             double dmin = (double)emin, dmax = (double)emax;
                             if (!is_finite_f(emin) || !is_finite_f(emax)) {
                                 // coefficients or intermediates outside the float
                                 // range (ill-conditioned short steps): use the FP64
                                 // enclosure for this piece
-                                cs_enclosure_fp64<ORDER>(cf + d * OP1, h_lb, h_ub, dmin, dmax);
+                                cs_enclosure_fp64<ORDER>(cf + d * (ORDER + 1), h_lb, h_ub, dmin, dmax);
                             }
                             // dmin <= FP64 enclosure minimum (as reals); the same
                             // monotone double subtraction and outward float
@@ -273,43 +288,43 @@ __global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double ra
                     }
                 }
             }
+            float4 *dst = reinterpret_cast<float4 *>(aabbs + ((size_t)c * (N + 1) + o) * 8u);
+            dst[0] = make_float4(lb4[0], lb4[1], lb4[2], lb4[3]);
+            dst[1] = make_float4(ub4[0], ub4[1], ub4[2], ub4[3]);
+            kbke[(size_t)c * N + o] = range;
         }
-        float4 *dst = reinterpret_cast<float4 *>(aabbs + ((size_t)c * (N + 1) + o) * 8u);
-        dst[0] = make_float4(lb4[0], lb4[1], lb4[2], lb4[3]);
-        dst[1] = make_float4(ub4[0], ub4[1], ub4[2], ub4[3]);
-    }
-    // block-wide min/max of the boxes -> one atomic per block and component
-    uint32_t ol[4], ou[4];
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        ol[d] = f2ord(lb4[d]);
-        ou[d] = f2ord(ub4[d]);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const uint32_t a = __shfl_xor(ol[d], off, 64), b = __shfl_xor(ou[d], off, 64);
-            ol[d] = a < ol[d] ? a : ol[d];
-            ou[d] = b > ou[d] ? b : ou[d];
-        }
-    }
-    __shared__ uint32_t s_l[4][4], s_u[4][4];
-    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    if (lane == 0) {
+        // block-wide min/max of this step's boxes -> one atomic per block and component
+        uint32_t ol[4], ou[4];
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-            s_l[w][d] = ol[d];
-            s_u[w][d] = ou[d];
+            ol[d] = f2ord(lb4[d]);
+            ou[d] = f2ord(ub4[d]);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint32_t x = __shfl_xor(ol[d], off, 64), y = __shfl_xor(ou[d], off, 64);
+                ol[d] = x < ol[d] ? x : ol[d];
+                ou[d] = y > ou[d] ? y : ou[d];
+            }
         }
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        const int d = threadIdx.x;
-        uint32_t l = s_l[0][d], u = s_u[0][d];
-        for (int k = 1; k < 4; ++k) {
-            l = s_l[k][d] < l ? s_l[k][d] : l;
-            u = s_u[k][d] > u ? s_u[k][d] : u;
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                s_l[w][d] = ol[d];
+                s_u[w][d] = ou[d];
+            }
         }
-        atomicMin(&gbox_ord[c * 8u + d], l);
-        atomicMax(&gbox_ord[c * 8u + 4u + d], u);
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            const int d = threadIdx.x;
+            uint32_t l = s_l[0][d], u = s_u[0][d];
+            for (int k = 1; k < 4; ++k) {
+                l = s_l[k][d] < l ? s_l[k][d] : l;
+                u = s_u[k][d] > u ? s_u[k][d] : u;
+            }
+            atomicMin(&gbox_ord[c * 8u + d], l);
+            atomicMax(&gbox_ord[c * 8u + 4u + d], u);
+        }
     }
 }
 
@@ -1007,20 +1022,20 @@ __global__ __launch_bounds__(1024) void k_scan_steps(const uint64_t *__restrict_
 
 template <int ORDER>
 hipError_t launch_aabbs_order(const PjView &pj, const StepGrid &sg, double radius, int mode, float *aabbs,
-                              uint32_t *gbox_ord, DevError *err, hipStream_t stream)
+                              uint2 *kbke, uint32_t *gbox_ord, DevError *err, hipStream_t stream)
 {
-    const dim3 grid(((pj.n_objs + 255u) / 256u) * sg.n_steps);
-    hipLaunchKernelGGL(k_aabbs<ORDER>, grid, dim3(256), 0, stream, pj, sg, radius, mode, aabbs, gbox_ord, err);
+    const dim3 grid(((pj.n_objs + 255u) / 256u) * ((sg.n_steps + kAabbGroup - 1u) / kAabbGroup));
+    hipLaunchKernelGGL(k_aabbs<ORDER>, grid, dim3(256), 0, stream, pj, sg, radius, mode, aabbs, kbke, gbox_ord, err);
     return hipGetLastError();
 }
 
 template <int ORDER>
 hipError_t dispatch_aabbs(int order, const PjView &pj, const StepGrid &sg, double radius, int mode, float *aabbs,
-                          uint32_t *gbox_ord, DevError *err, hipStream_t stream)
+                          uint2 *kbke, uint32_t *gbox_ord, DevError *err, hipStream_t stream)
 {
-    if (order == ORDER) return launch_aabbs_order<ORDER>(pj, sg, radius, mode, aabbs, gbox_ord, err, stream);
+    if (order == ORDER) return launch_aabbs_order<ORDER>(pj, sg, radius, mode, aabbs, kbke, gbox_ord, err, stream);
     if constexpr (ORDER < kMaxOrder) {
-        return dispatch_aabbs<ORDER + 1>(order, pj, sg, radius, mode, aabbs, gbox_ord, err, stream);
+        return dispatch_aabbs<ORDER + 1>(order, pj, sg, radius, mode, aabbs, kbke, gbox_ord, err, stream);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1039,13 +1054,14 @@ hipError_t launch_transpose_cf4(const double *traj, double *cf4, uint64_t n_step
 }
 
 hipError_t launch_aabbs(const PjView &pj, const StepGrid &sg, double conj_thresh, int mode, float *aabbs,
-                        uint32_t *gbox_ord, DevError *err, hipStream_t stream)
+                        uint2 *kbke, uint32_t *gbox_ord, DevError *err, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_init_gbox, dim3((sg.n_steps * 8u + 255u) / 256u), dim3(256), 0, stream, gbox_ord,
                        sg.n_steps);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return dispatch_aabbs<kMinOrder>((int)pj.order, pj, sg, conj_thresh / 2, mode, aabbs, gbox_ord, err, stream);
+    return dispatch_aabbs<kMinOrder>((int)pj.order, pj, sg, conj_thresh / 2, mode, aabbs, kbke, gbox_ord, err,
+                                     stream);
 }
 
 hipError_t launch_gbox_finalize(const uint32_t *gbox_ord, float *aabbs, uint32_t n_objs, uint32_t n_steps,
