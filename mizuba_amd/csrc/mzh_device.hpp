@@ -64,7 +64,9 @@ struct PjView {
 // (detect_conjunctions_bvh.cpp:521-580) with an escape index (the node that
 // follows the subtree in a depth-first walk), so that the broad phase walks the
 // tree without a stack. left is the index of the left child (the right child
-// is left + 1) and 0 for a leaf, since the root is nobody's child. The
+// is left + 1) and 0 for a leaf, since the root is nobody's child. A leaf
+// holds its sorted range [begin, end); an internal node holds begin and, in
+// place of its end, its split (the first position of its right child). The
 // escape of the root's right spine is kTravEnd.
 constexpr uint32_t kTravEnd = 0xffffffffu;
 struct alignas(16) TravNode {
@@ -72,7 +74,7 @@ struct alignas(16) TravNode {
     float ub[4];
     uint32_t escape;
     uint32_t begin;
-    uint32_t end;
+    uint32_t end;   // leaf: end; internal node: split
     uint32_t left;
 };
 static_assert(sizeof(TravNode) == 48, "TravNode layout");

@@ -695,7 +695,8 @@ __device__ inline int bvh_split(const BvhScratch &sc, uint32_t lo, uint32_t hi, 
 // write consecutive node records. The escape index comes top-down: a left
 // child escapes to its right sibling, a right child to its parent's escape.
 //
-// While a node waits for its level, its record holds (escape, begin, end).
+// While a node waits for its level, its record holds (escape, begin, end); an
+// internal node then keeps its split in place of its end.
 template <bool LDS>
 __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ srt_mcodes,
                                                      const float *__restrict__ srt_aabbs,
@@ -767,7 +768,7 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
                     const uint32_t lc = le + carry + ex;
                     reinterpret_cast<uint4 *>(&T[lc])[2] = make_uint4(lc + 1u, b, q, 0u);
                     reinterpret_cast<uint4 *>(&T[lc + 1u])[2] = make_uint4(esc, q, e, 0u);
-                    nd[2] = make_uint4(esc, b, e, lc);
+                    nd[2] = make_uint4(esc, b, q, lc);
                 } else {
                     float l[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
                     float u[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
@@ -818,10 +819,18 @@ __global__ __launch_bounds__(256) void k_bvh_export(const TravNode *__restrict__
     const uint32_t i = (blockIdx.x - c * blocks_per_step) * 256u + threadIdx.x;
     const uint32_t nn = tree_size[c];
     if (i >= nn) return;
-    const TravNode t = trav[(size_t)c * node_cap + i];
+    const TravNode *T = trav + (size_t)c * node_cap;
+    const TravNode t = T[i];
+    uint32_t end = t.end;
+    if (t.left != 0u) {
+        // an internal node ends where the last leaf of its right spine ends
+        uint32_t x = t.left + 1u;
+        while (T[x].left != 0u) x = T[x].left + 1u;
+        end = T[x].end;
+    }
     mzh_bvh_node r;
     r.begin = t.begin;
-    r.end = t.end;
+    r.end = end;
     r.left = t.left ? (int32_t)t.left : -1;
     r.right = t.left ? (int32_t)(t.left + 1u) : -1;
 #pragma unroll
@@ -842,19 +851,58 @@ __global__ __launch_bounds__(256) void k_bvh_export(const TravNode *__restrict__
 
 constexpr uint32_t kBpSlots = 4;  // partners kept per query by the single traversal
 
-// Broad phase traversal. COLLECT: every (step, query) records its pair count
-// in cnt[step][i] and its first kBpSlots partners j in slots[step][i][:]; a
-// query with more partners is appended to the overflow list. !COLLECT: the
-// queries of the overflow list are walked again and write their pairs to
-// their final place.
+__device__ inline bool box_overlap(const float4 &ql, const float4 &qu, const float4 &nl, const float4 &nu)
+{
+    return (qu.x >= nl.x && ql.x <= nu.x) && (qu.y >= nl.y && ql.y <= nu.y) && (qu.z >= nl.z && ql.z <= nu.z)
+           && (qu.w >= nl.w && ql.w <= nu.w);
+}
+
+// Record the broad-phase pair (i, j), i < j: count it for i, keep the first
+// kBpSlots partners, and queue i for the overflow walk when it gets more.
+__device__ inline void bp_emit(uint32_t c, uint32_t i, uint32_t j, uint32_t N, uint32_t *cnt, uint32_t *slots,
+                               uint32_t *ovf, uint32_t *ovf_count, uint32_t ovf_cap)
+{
+    const size_t g = (size_t)c * N + i;
+    const uint32_t k = atomicAdd(&cnt[g], 1u);
+    if (k < kBpSlots) {
+        slots[g * kBpSlots + k] = j;
+    } else if (k == kBpSlots) {
+        const uint32_t q = atomicAdd(ovf_count, 1u);
+        if (q < ovf_cap) ovf[q] = c * N + i;
+    }
+}
+
+// Broad phase (detect_conjunctions_broad_phase.cpp:153-339). The reference
+// walks the tree from every object i and emits (i, j), i < j, for every member
+// j of every leaf whose box overlaps box_i (closed intervals, :269-271), with
+// otype_i + otype_j <= 3 (:294-310). Equivalently: (i, j) is a pair iff box_i
+// overlaps the box of the leaf holding j.
+//
+// COLLECT finds every pair once. The query at sorted position o walks only the
+// part of the tree to the right of its own run of equal codes [rb, re): at an
+// internal node it goes straight to the right child when the split is at or
+// before re, so no subtree left of the run is ever read. Its query box is the
+// box of its own leaf, which contains box_o, so every pair between o and a
+// later leaf is reached from o whichever of the two has the lower original
+// index; the reference's condition is then checked exactly for the pair
+// (min, max) of the two original indices. Pairs inside o's own run are
+// emitted by o for the members after it. Each pair is counted for its lower
+// index i with an atomic; the first kBpSlots partners go to slots[step][i][:],
+// objects with more are queued for the overflow walk.
+//
+// !COLLECT is the reference's own walk for the queued objects: from the
+// object's sorted position, every leaf overlapping its box, members with a
+// higher original index, written in place and sorted by j.
 template <bool COLLECT>
 __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
                                             const uint32_t *__restrict__ tree_size, const float *__restrict__ srt_aabbs,
-                                            const uint32_t *__restrict__ srt_idx, const uint32_t *__restrict__ n_eff,
-                                            const int32_t *__restrict__ otypes, uint32_t *__restrict__ cnt,
-                                            uint32_t *__restrict__ slots, uint32_t *__restrict__ ovf,
-                                            uint32_t *__restrict__ ovf_count, uint32_t ovf_cap,
-                                            const uint64_t *__restrict__ off, const uint64_t *__restrict__ step_off,
+                                            const uint64_t *__restrict__ srt_mcodes,
+                                            const uint32_t *__restrict__ srt_idx, uint32_t *__restrict__ inv,
+                                            const uint32_t *__restrict__ n_eff, const int32_t *__restrict__ otypes,
+                                            uint32_t *__restrict__ cnt, uint32_t *__restrict__ slots,
+                                            uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_count,
+                                            uint32_t ovf_cap, const uint64_t *__restrict__ off,
+                                            const uint64_t *__restrict__ step_off,
                                             mzh_aabb_collision *__restrict__ pairs, uint32_t N, uint64_t node_cap)
 {
     uint32_t c, o;
@@ -869,56 +917,103 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
         if (k >= n_ovf || k >= ovf_cap) return;
         const uint32_t packed = ovf[k];
         c = packed / N;
-        o = packed - c * N;
+        o = inv[packed];
     }
+    const uint32_t n = n_eff[c];
     const uint32_t nn = tree_size[c];
-    if (o >= n_eff[c] || nn == 0) return;
+    if (o >= n || nn == 0) return;
     const TravNode *T = trav + (size_t)c * node_cap;
     const uint32_t *vidx = srt_idx + (size_t)c * N;
-    const float4 ql = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[0];
-    const float4 qu = reinterpret_cast<const float4 *>(srt_aabbs + ((size_t)c * (N + 1) + o) * 8u)[1];
+    const float *boxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
+    const float4 bl = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[0];
+    const float4 bu = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[1];
     const uint32_t orig = vidx[o];
     const int32_t ot = otypes[orig];
-    uint32_t count = 0;
-    uint32_t *slot = slots + ((size_t)c * N + orig) * kBpSlots;
-    mzh_aabb_collision *out = nullptr;
-    if constexpr (!COLLECT) out = pairs + step_off[c] + off[(size_t)c * N + orig];
-    // stackless depth-first walk: descend on overlap, jump to the escape index otherwise
-    uint32_t i = 0;
-    while (i < nn) {
-        const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
-        const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
-        const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
-        const bool overlap = (qu.x >= nl.x && ql.x <= nu.x) && (qu.y >= nl.y && ql.y <= nu.y)
-                             && (qu.z >= nl.z && ql.z <= nu.z) && (qu.w >= nl.w && ql.w <= nu.w);
-        if (overlap && meta.w != 0u) {
-            i = meta.w;  // internal node: descend to the left child
-            continue;
-        }
-        if (overlap) {
-            for (uint32_t k = meta.y; k != meta.z; ++k) {
-                const uint32_t oi = vidx[k];
-                if (orig >= oi) continue;
-                if (ot + otypes[oi] <= 3) {
-                    if constexpr (COLLECT) {
-                        if (count < kBpSlots) slot[count] = oi;
-                    } else {
-                        out[count] = mzh_aabb_collision{orig, oi};
-                    }
-                    ++count;
-                }
+    if constexpr (COLLECT) {
+        inv[(size_t)c * N + orig] = o;
+        if (ot + 1 > 3) return;  // no object type can pair with this one
+        // own run of equal codes [rb, re) and its leaf box
+        const uint64_t *codes = srt_mcodes + (size_t)c * N;
+        const uint64_t code = codes[o];
+        uint32_t rb = o, re = o + 1u;
+        while (rb > 0 && codes[rb - 1u] == code) --rb;
+        while (re < n && codes[re] == code) ++re;
+        const bool multi = re - rb > 1u;
+        float4 ql = bl, qu = bu;
+        if (multi) {
+            float l[4] = {bl.x, bl.y, bl.z, bl.w}, u[4] = {bu.x, bu.y, bu.z, bu.w};
+            for (uint32_t m = rb; m != re; ++m) box_union(l, u, boxes + (size_t)m * 8u);
+            ql = make_float4(l[0], l[1], l[2], l[3]);
+            qu = make_float4(u[0], u[1], u[2], u[3]);
+            // pairs inside the run: every member overlaps the leaf box
+            for (uint32_t m = o + 1u; m < re; ++m) {
+                const uint32_t oi = vidx[m];
+                if (ot + otypes[oi] <= 3)
+                    bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, ovf_count, ovf_cap);
             }
         }
-        i = meta.x;
-    }
-    if constexpr (COLLECT) {
-        cnt[(size_t)c * N + orig] = count;
-        if (count > kBpSlots) {
-            const uint32_t k = atomicAdd(ovf_count, 1u);
-            if (k < ovf_cap) ovf[k] = c * N + o;
+        uint32_t i = 0;
+        while (i < nn) {
+            const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
+            const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
+            const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
+            if (!box_overlap(ql, qu, nl, nu)) {
+                i = meta.x;
+                continue;
+            }
+            if (meta.w != 0u) {
+                // internal node: skip the left child when it ends at or before the run
+                i = meta.z <= re ? meta.w + 1u : meta.w;
+                continue;
+            }
+            if (meta.y >= re) {
+                const bool single = meta.z - meta.y == 1u;
+                // box_o against the leaf box, needed when the query box is larger than box_o
+                const bool o_hits_leaf = !multi || box_overlap(bl, bu, nl, nu);
+                for (uint32_t m = meta.y; m != meta.z; ++m) {
+                    const uint32_t oi = vidx[m];
+                    if (ot + otypes[oi] > 3) continue;
+                    bool hit;
+                    if (orig < oi) {
+                        hit = o_hits_leaf;  // box_o against the leaf of m
+                    } else if (!multi && single) {
+                        hit = true;  // box_m is the leaf box and the query box is box_o
+                    } else {
+                        // box_m against the leaf of o
+                        const float4 ml = reinterpret_cast<const float4 *>(boxes + (size_t)m * 8u)[0];
+                        const float4 mu = reinterpret_cast<const float4 *>(boxes + (size_t)m * 8u)[1];
+                        hit = box_overlap(ml, mu, ql, qu);
+                    }
+                    if (hit) bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, ovf_count,
+                                     ovf_cap);
+                }
+            }
+            i = meta.x;
         }
     } else {
-        // insertion sort of this query's pairs by j
+        mzh_aabb_collision *out = pairs + step_off[c] + off[(size_t)c * N + orig];
+        const uint32_t limit = cnt[(size_t)c * N + orig];  // counted by the collect pass
+        uint32_t count = 0;
+        uint32_t i = 0;
+        while (i < nn) {
+            const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
+            const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
+            const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
+            const bool overlap = box_overlap(bl, bu, nl, nu);
+            if (overlap && meta.w != 0u) {
+                i = meta.w;  // internal node: descend to the left child
+                continue;
+            }
+            if (overlap) {
+                for (uint32_t k = meta.y; k != meta.z; ++k) {
+                    const uint32_t oi = vidx[k];
+                    if (orig >= oi) continue;
+                    if (ot + otypes[oi] <= 3 && count < limit) out[count++] = mzh_aabb_collision{orig, oi};
+                }
+            }
+            i = meta.x;
+        }
+        // insertion sort of this object's pairs by j
         for (uint32_t a = 1; a < count; ++a) {
             const mzh_aabb_collision x = out[a];
             uint32_t b = a;
@@ -1123,16 +1218,16 @@ hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mz
 }
 
 hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
-                           const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, uint32_t *cnt,
-                           uint32_t *slots, uint32_t *ovf, uint32_t *ovf_count, uint32_t ovf_cap, uint32_t n_objs,
-                           uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
+                           const uint64_t *srt_mcodes, const uint32_t *srt_idx, uint32_t *inv, const uint32_t *n_eff,
+                           const int32_t *otypes, uint32_t *cnt, uint32_t *slots, uint32_t *ovf, uint32_t *ovf_count,
+                           uint32_t ovf_cap, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
 {
     hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (size_t)n_objs * n_steps, stream);
     if (e == hipSuccess) e = hipMemsetAsync(ovf_count, 0, sizeof(uint32_t), stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_bp<true>, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, nodes, tree_size,
-                       srt_aabbs, srt_idx, n_eff, otypes, cnt, slots, ovf, ovf_count, ovf_cap, nullptr, nullptr,
-                       nullptr, n_objs, node_cap);
+                       srt_aabbs, srt_mcodes, srt_idx, inv, n_eff, otypes, cnt, slots, ovf, ovf_count, ovf_cap,
+                       nullptr, nullptr, nullptr, n_objs, node_cap);
     return hipGetLastError();
 }
 
@@ -1151,7 +1246,8 @@ hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uin
 }
 
 hipError_t launch_bp_write(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
-                           const uint32_t *srt_idx, const uint32_t *n_eff, const int32_t *otypes, const uint32_t *cnt,
+                           const uint32_t *srt_idx, uint32_t *inv, const uint32_t *n_eff, const int32_t *otypes,
+                           const uint32_t *cnt,
                            const uint32_t *slots, const uint32_t *ovf, const uint32_t *ovf_count, uint32_t ovf_cap,
                            uint32_t n_ovf_host, const uint64_t *off, const uint64_t *step_off,
                            mzh_aabb_collision *pairs, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
@@ -1163,7 +1259,8 @@ hipError_t launch_bp_write(const TravNode *nodes, const uint32_t *tree_size, con
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || n_ovf_host == 0) return e;
     hipLaunchKernelGGL(k_bp<false>, dim3((n_ovf_host + 255u) / 256u), dim3(256), 0, stream, nodes, tree_size,
-                       srt_aabbs, srt_idx, n_eff, otypes, nullptr, nullptr, const_cast<uint32_t *>(ovf),
+                       srt_aabbs, nullptr, srt_idx, inv, n_eff, otypes, const_cast<uint32_t *>(cnt), nullptr,
+                       const_cast<uint32_t *>(ovf),
                        const_cast<uint32_t *>(ovf_count), ovf_cap, off, step_off, pairs, n_objs, node_cap);
     return hipGetLastError();
 }
