@@ -46,26 +46,26 @@ hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mz
                              uint32_t n_steps, hipStream_t stream);
 
 // Stage 4: broad phase. launch_bp_count finds every pair once (each query
-// walks the tree to the right of its own code) and counts it for the pair's
-// lower original index, keeping the first 4 partners (slots: 4 per object and
-// step) and queueing objects with more in the overflow list; inv receives the
-// sorted position of every object. After the scans, launch_bp_write places the
-// stored pairs and walks the n_ovf_host overflow objects again.
+// walks the tree to the right of its own code run) and counts it for the
+// pair's lower original index i, keeping i's first 4 partners inline (slots: 4
+// per object and step) and pushing the others on i's list in the overflow
+// pool (ovf entries {partner, next}, head per object and step). After the
+// scans, launch_bp_place writes every object's pairs sorted by partner.
+// launch_bp_reset clears the counts, list heads and pool counter first.
+hipError_t launch_bp_reset(uint32_t *cnt, uint32_t *head, uint32_t *ovf_count, uint32_t n_objs, uint32_t n_steps,
+                           hipStream_t stream);
 hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
-                           const uint64_t *srt_mcodes, const uint32_t *srt_idx, uint32_t *inv, const uint32_t *n_eff,
-                           const int32_t *otypes, uint32_t *cnt, uint32_t *slots, uint32_t *ovf, uint32_t *ovf_count,
-                           uint32_t ovf_cap, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream);
+                           const uint64_t *srt_mcodes, const uint32_t *srt_idx, const uint32_t *n_eff,
+                           const int32_t *otypes, uint32_t *cnt, uint32_t *slots, uint2 *ovf, uint32_t *head,
+                           uint32_t *ovf_count, uint32_t ovf_cap, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
+                           hipStream_t stream);
 hipError_t launch_scan_counts(const uint32_t *cnt, uint64_t *off, uint64_t *step_total, uint32_t n_objs,
                               uint32_t n_steps, hipStream_t stream);
 hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uint64_t *grand_total, uint32_t n_steps,
                              hipStream_t stream);
-hipError_t launch_bp_write(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
-                           const uint32_t *srt_idx, uint32_t *inv, const uint32_t *n_eff, const int32_t *otypes,
-                           const uint32_t *cnt,
-                           const uint32_t *slots, const uint32_t *ovf, const uint32_t *ovf_count, uint32_t ovf_cap,
-                           uint32_t n_ovf_host, const uint64_t *off, const uint64_t *step_off,
-                           mzh_aabb_collision *pairs, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
-                           hipStream_t stream);
+hipError_t launch_bp_place(const uint32_t *cnt, const uint32_t *slots, const uint2 *ovf, const uint32_t *head,
+                           uint32_t ovf_cap, const uint64_t *off, const uint64_t *step_off, mzh_aabb_collision *pairs,
+                           uint32_t n_objs, uint32_t n_steps, hipStream_t stream);
 
 // Stage 5: narrow phase over all pairs of all steps.
 struct NarrowArgs {

@@ -849,7 +849,7 @@ __global__ __launch_bounds__(256) void k_bvh_export(const TravNode *__restrict__
 // then places every query's pairs in (i, j) order without a global sort.
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t kBpSlots = 4;  // partners kept per query by the single traversal
+constexpr uint32_t kBpSlots = 4;  // partners kept inline per object and step
 
 __device__ inline bool box_overlap(const float4 &ql, const float4 &qu, const float4 &nl, const float4 &nu)
 {
@@ -858,17 +858,21 @@ __device__ inline bool box_overlap(const float4 &ql, const float4 &qu, const flo
 }
 
 // Record the broad-phase pair (i, j), i < j: count it for i, keep the first
-// kBpSlots partners, and queue i for the overflow walk when it gets more.
+// kBpSlots partners inline, and push the others on i's list in the overflow
+// pool (entry = partner, next entry; head[step][i] = latest entry).
 __device__ inline void bp_emit(uint32_t c, uint32_t i, uint32_t j, uint32_t N, uint32_t *cnt, uint32_t *slots,
-                               uint32_t *ovf, uint32_t *ovf_count, uint32_t ovf_cap)
+                               uint2 *ovf, uint32_t *head, uint32_t *ovf_count, uint32_t ovf_cap)
 {
     const size_t g = (size_t)c * N + i;
     const uint32_t k = atomicAdd(&cnt[g], 1u);
     if (k < kBpSlots) {
         slots[g * kBpSlots + k] = j;
-    } else if (k == kBpSlots) {
-        const uint32_t q = atomicAdd(ovf_count, 1u);
-        if (q < ovf_cap) ovf[q] = c * N + i;
+    } else {
+        const uint32_t e = atomicAdd(ovf_count, 1u);
+        if (e < ovf_cap) {
+            const uint32_t next = atomicExch(&head[g], e);
+            ovf[e] = make_uint2(j, next);
+        }
     }
 }
 
@@ -878,47 +882,29 @@ __device__ inline void bp_emit(uint32_t c, uint32_t i, uint32_t j, uint32_t N, u
 // otype_i + otype_j <= 3 (:294-310). Equivalently: (i, j) is a pair iff box_i
 // overlaps the box of the leaf holding j.
 //
-// COLLECT finds every pair once. The query at sorted position o walks only the
-// part of the tree to the right of its own run of equal codes [rb, re): at an
-// internal node it goes straight to the right child when the split is at or
-// before re, so no subtree left of the run is ever read. Its query box is the
-// box of its own leaf, which contains box_o, so every pair between o and a
-// later leaf is reached from o whichever of the two has the lower original
-// index; the reference's condition is then checked exactly for the pair
-// (min, max) of the two original indices. Pairs inside o's own run are
+// Here every pair is found once. The query at sorted position o walks only
+// the part of the tree to the right of its own run of equal codes [rb, re):
+// at an internal node it goes straight to the right child when the split is
+// at or before re, so no subtree left of the run is ever read. Its query box
+// is the box of its own leaf, which contains box_o, so every pair between o
+// and a later leaf is reached from o whichever of the two has the lower
+// original index; the reference's condition is then checked exactly for the
+// pair (min, max) of the two original indices. Pairs inside o's own run are
 // emitted by o for the members after it. Each pair is counted for its lower
-// index i with an atomic; the first kBpSlots partners go to slots[step][i][:],
-// objects with more are queued for the overflow walk.
-//
-// !COLLECT is the reference's own walk for the queued objects: from the
-// object's sorted position, every leaf overlapping its box, members with a
-// higher original index, written in place and sorted by j.
-template <bool COLLECT>
-__global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
-                                            const uint32_t *__restrict__ tree_size, const float *__restrict__ srt_aabbs,
-                                            const uint64_t *__restrict__ srt_mcodes,
-                                            const uint32_t *__restrict__ srt_idx, uint32_t *__restrict__ inv,
-                                            const uint32_t *__restrict__ n_eff, const int32_t *__restrict__ otypes,
-                                            uint32_t *__restrict__ cnt, uint32_t *__restrict__ slots,
-                                            uint32_t *__restrict__ ovf, uint32_t *__restrict__ ovf_count,
-                                            uint32_t ovf_cap, const uint64_t *__restrict__ off,
-                                            const uint64_t *__restrict__ step_off,
-                                            mzh_aabb_collision *__restrict__ pairs, uint32_t N, uint64_t node_cap)
+// index i (bp_emit); the order within i's partners is fixed when they are
+// placed.
+__global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav, const uint32_t *__restrict__ tree_size,
+                                            const float *__restrict__ srt_aabbs, const uint64_t *__restrict__ srt_mcodes,
+                                            const uint32_t *__restrict__ srt_idx, const uint32_t *__restrict__ n_eff,
+                                            const int32_t *__restrict__ otypes, uint32_t *__restrict__ cnt,
+                                            uint32_t *__restrict__ slots, uint2 *__restrict__ ovf,
+                                            uint32_t *__restrict__ head, uint32_t *__restrict__ ovf_count,
+                                            uint32_t ovf_cap, uint32_t N, uint64_t node_cap)
 {
-    uint32_t c, o;
-    if constexpr (COLLECT) {
-        const uint32_t bps = (N + 255u) / 256u;
-        const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-        c = lb / bps;
-        o = (lb - c * bps) * 256u + threadIdx.x;
-    } else {
-        const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-        const uint32_t n_ovf = *ovf_count;
-        if (k >= n_ovf || k >= ovf_cap) return;
-        const uint32_t packed = ovf[k];
-        c = packed / N;
-        o = inv[packed];
-    }
+    const uint32_t bps = (N + 255u) / 256u;
+    const uint32_t lblk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t c = lblk / bps;
+    const uint32_t o = (lblk - c * bps) * 256u + threadIdx.x;
     const uint32_t n = n_eff[c];
     const uint32_t nn = tree_size[c];
     if (o >= n || nn == 0) return;
@@ -929,133 +915,115 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav,
     const float4 bu = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[1];
     const uint32_t orig = vidx[o];
     const int32_t ot = otypes[orig];
-    if constexpr (COLLECT) {
-        inv[(size_t)c * N + orig] = o;
-        if (ot + 1 > 3) return;  // no object type can pair with this one
-        // own run of equal codes [rb, re) and its leaf box
-        const uint64_t *codes = srt_mcodes + (size_t)c * N;
-        const uint64_t code = codes[o];
-        uint32_t rb = o, re = o + 1u;
-        while (rb > 0 && codes[rb - 1u] == code) --rb;
-        while (re < n && codes[re] == code) ++re;
-        const bool multi = re - rb > 1u;
-        float4 ql = bl, qu = bu;
-        if (multi) {
-            float l[4] = {bl.x, bl.y, bl.z, bl.w}, u[4] = {bu.x, bu.y, bu.z, bu.w};
-            for (uint32_t m = rb; m != re; ++m) box_union(l, u, boxes + (size_t)m * 8u);
-            ql = make_float4(l[0], l[1], l[2], l[3]);
-            qu = make_float4(u[0], u[1], u[2], u[3]);
-            // pairs inside the run: every member overlaps the leaf box
-            for (uint32_t m = o + 1u; m < re; ++m) {
+    if (ot + 1 > 3) return;  // no object type can pair with this one
+    // own run of equal codes [rb, re) and its leaf box
+    const uint64_t *codes = srt_mcodes + (size_t)c * N;
+    const uint64_t code = codes[o];
+    uint32_t rb = o, re = o + 1u;
+    while (rb > 0 && codes[rb - 1u] == code) --rb;
+    while (re < n && codes[re] == code) ++re;
+    const bool multi = re - rb > 1u;
+    float4 ql = bl, qu = bu;
+    if (multi) {
+        float l[4] = {bl.x, bl.y, bl.z, bl.w}, u[4] = {bu.x, bu.y, bu.z, bu.w};
+        for (uint32_t m = rb; m != re; ++m) box_union(l, u, boxes + (size_t)m * 8u);
+        ql = make_float4(l[0], l[1], l[2], l[3]);
+        qu = make_float4(u[0], u[1], u[2], u[3]);
+        // pairs inside the run: every member overlaps the leaf box
+        for (uint32_t m = o + 1u; m < re; ++m) {
+            const uint32_t oi = vidx[m];
+            if (ot + otypes[oi] <= 3)
+                bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, head, ovf_count, ovf_cap);
+        }
+    }
+    uint32_t i = 0;
+    while (i < nn) {
+        const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
+        const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
+        const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
+        if (!box_overlap(ql, qu, nl, nu)) {
+            i = meta.x;
+            continue;
+        }
+        if (meta.w != 0u) {
+            // internal node: skip the left child when it ends at or before the run
+            i = meta.z <= re ? meta.w + 1u : meta.w;
+            continue;
+        }
+        if (meta.y >= re) {
+            const bool single = meta.z - meta.y == 1u;
+            // box_o against the leaf box, needed when the query box is larger than box_o
+            const bool o_hits_leaf = !multi || box_overlap(bl, bu, nl, nu);
+            for (uint32_t m = meta.y; m != meta.z; ++m) {
                 const uint32_t oi = vidx[m];
-                if (ot + otypes[oi] <= 3)
-                    bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, ovf_count, ovf_cap);
-            }
-        }
-        uint32_t i = 0;
-        while (i < nn) {
-            const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
-            const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
-            const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
-            if (!box_overlap(ql, qu, nl, nu)) {
-                i = meta.x;
-                continue;
-            }
-            if (meta.w != 0u) {
-                // internal node: skip the left child when it ends at or before the run
-                i = meta.z <= re ? meta.w + 1u : meta.w;
-                continue;
-            }
-            if (meta.y >= re) {
-                const bool single = meta.z - meta.y == 1u;
-                // box_o against the leaf box, needed when the query box is larger than box_o
-                const bool o_hits_leaf = !multi || box_overlap(bl, bu, nl, nu);
-                for (uint32_t m = meta.y; m != meta.z; ++m) {
-                    const uint32_t oi = vidx[m];
-                    if (ot + otypes[oi] > 3) continue;
-                    bool hit;
-                    if (orig < oi) {
-                        hit = o_hits_leaf;  // box_o against the leaf of m
-                    } else if (!multi && single) {
-                        hit = true;  // box_m is the leaf box and the query box is box_o
-                    } else {
-                        // box_m against the leaf of o
-                        const float4 ml = reinterpret_cast<const float4 *>(boxes + (size_t)m * 8u)[0];
-                        const float4 mu = reinterpret_cast<const float4 *>(boxes + (size_t)m * 8u)[1];
-                        hit = box_overlap(ml, mu, ql, qu);
-                    }
-                    if (hit) bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, ovf_count,
-                                     ovf_cap);
+                if (ot + otypes[oi] > 3) continue;
+                bool hit;
+                if (orig < oi) {
+                    hit = o_hits_leaf;  // box_o against the leaf of m
+                } else if (!multi && single) {
+                    hit = true;  // box_m is the leaf box and the query box is box_o
+                } else {
+                    // box_m against the leaf of o
+                    const float4 ml = reinterpret_cast<const float4 *>(boxes + (size_t)m * 8u)[0];
+                    const float4 mu = reinterpret_cast<const float4 *>(boxes + (size_t)m * 8u)[1];
+                    hit = box_overlap(ml, mu, ql, qu);
                 }
+                if (hit)
+                    bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, head, ovf_count,
+                            ovf_cap);
             }
-            i = meta.x;
         }
-    } else {
-        mzh_aabb_collision *out = pairs + step_off[c] + off[(size_t)c * N + orig];
-        const uint32_t limit = cnt[(size_t)c * N + orig];  // counted by the collect pass
-        uint32_t count = 0;
-        uint32_t i = 0;
-        while (i < nn) {
-            const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
-            const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
-            const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
-            const bool overlap = box_overlap(bl, bu, nl, nu);
-            if (overlap && meta.w != 0u) {
-                i = meta.w;  // internal node: descend to the left child
-                continue;
-            }
-            if (overlap) {
-                for (uint32_t k = meta.y; k != meta.z; ++k) {
-                    const uint32_t oi = vidx[k];
-                    if (orig >= oi) continue;
-                    if (ot + otypes[oi] <= 3 && count < limit) out[count++] = mzh_aabb_collision{orig, oi};
-                }
-            }
-            i = meta.x;
-        }
-        // insertion sort of this object's pairs by j
-        for (uint32_t a = 1; a < count; ++a) {
-            const mzh_aabb_collision x = out[a];
-            uint32_t b = a;
-            while (b > 0 && out[b - 1].j > x.j) {
-                out[b] = out[b - 1];
-                --b;
-            }
-            out[b] = x;
-        }
+        i = meta.x;
     }
 }
 
-// Place the pairs of every query with at most kBpSlots partners: sorted by j,
-// at the query's offset in (i, j) order.
+// Place every object's pairs at its offset in (i, j) order: the inline
+// partners and, past kBpSlots, those on its overflow list, sorted by j.
 __global__ __launch_bounds__(256) void k_bp_place(const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ slots,
-                                                  const uint64_t *__restrict__ off,
+                                                  const uint2 *__restrict__ ovf, const uint32_t *__restrict__ head,
+                                                  uint32_t ovf_cap, const uint64_t *__restrict__ off,
                                                   const uint64_t *__restrict__ step_off,
                                                   mzh_aabb_collision *__restrict__ pairs, uint32_t N, uint32_t C)
 {
     const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (g >= (uint64_t)N * C) return;
     const uint32_t n = cnt[g];
-    if (n == 0 || n > kBpSlots) return;
+    if (n == 0) return;
     const uint32_t c = (uint32_t)(g / N), orig = (uint32_t)(g - (uint64_t)c * N);
-    uint32_t v[kBpSlots];
-#pragma unroll
-    for (uint32_t k = 0; k < kBpSlots; ++k) v[k] = (k < n) ? slots[g * kBpSlots + k] : 0xffffffffu;
-    // sorting network on kBpSlots = 4 values (padding sorts last)
-    auto cs = [](uint32_t &x, uint32_t &y) {
-        const uint32_t a = x < y ? x : y, b = x < y ? y : x;
-        x = a;
-        y = b;
-    };
-    cs(v[0], v[1]);
-    cs(v[2], v[3]);
-    cs(v[0], v[2]);
-    cs(v[1], v[3]);
-    cs(v[1], v[2]);
     mzh_aabb_collision *out = pairs + step_off[c] + off[g];
+    if (n <= kBpSlots) {
+        uint32_t v[kBpSlots];
 #pragma unroll
-    for (uint32_t k = 0; k < kBpSlots; ++k)
-        if (k < n) out[k] = mzh_aabb_collision{orig, v[k]};
+        for (uint32_t k = 0; k < kBpSlots; ++k) v[k] = (k < n) ? slots[g * kBpSlots + k] : 0xffffffffu;
+        // sorting network on kBpSlots = 4 values (padding sorts last)
+        auto cs = [](uint32_t &x, uint32_t &y) {
+            const uint32_t a = x < y ? x : y, b = x < y ? y : x;
+            x = a;
+            y = b;
+        };
+        cs(v[0], v[1]);
+        cs(v[2], v[3]);
+        cs(v[0], v[2]);
+        cs(v[1], v[3]);
+        cs(v[1], v[2]);
+#pragma unroll
+        for (uint32_t k = 0; k < kBpSlots; ++k)
+            if (k < n) out[k] = mzh_aabb_collision{orig, v[k]};
+        return;
+    }
+    // gather, then insertion sort in place (objects with many partners are rare)
+    uint32_t k = 0;
+    for (; k < kBpSlots; ++k) out[k] = mzh_aabb_collision{orig, slots[g * kBpSlots + k]};
+    for (uint32_t e = head[g]; e < ovf_cap && k < n; e = ovf[e].y) out[k++] = mzh_aabb_collision{orig, ovf[e].x};
+    for (uint32_t a = 1; a < k; ++a) {
+        const mzh_aabb_collision x = out[a];
+        uint32_t b = a;
+        while (b > 0 && out[b - 1].j > x.j) {
+            out[b] = out[b - 1];
+            --b;
+        }
+        out[b] = x;
+    }
 }
 
 // per-step exclusive scan of cnt[step][0..N) -> off, step totals
@@ -1217,17 +1185,25 @@ hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mz
     return hipGetLastError();
 }
 
-hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
-                           const uint64_t *srt_mcodes, const uint32_t *srt_idx, uint32_t *inv, const uint32_t *n_eff,
-                           const int32_t *otypes, uint32_t *cnt, uint32_t *slots, uint32_t *ovf, uint32_t *ovf_count,
-                           uint32_t ovf_cap, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps, hipStream_t stream)
+hipError_t launch_bp_reset(uint32_t *cnt, uint32_t *head, uint32_t *ovf_count, uint32_t n_objs, uint32_t n_steps,
+                           hipStream_t stream)
 {
-    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint32_t) * (size_t)n_objs * n_steps, stream);
+    const size_t nq = (size_t)n_objs * n_steps;
+    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nq, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(head, 0xff, sizeof(uint32_t) * nq, stream);
     if (e == hipSuccess) e = hipMemsetAsync(ovf_count, 0, sizeof(uint32_t), stream);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bp<true>, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, nodes, tree_size,
-                       srt_aabbs, srt_mcodes, srt_idx, inv, n_eff, otypes, cnt, slots, ovf, ovf_count, ovf_cap,
-                       nullptr, nullptr, nullptr, n_objs, node_cap);
+    return e;
+}
+
+hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
+                           const uint64_t *srt_mcodes, const uint32_t *srt_idx, const uint32_t *n_eff,
+                           const int32_t *otypes, uint32_t *cnt, uint32_t *slots, uint2 *ovf, uint32_t *head,
+                           uint32_t *ovf_count, uint32_t ovf_cap, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
+                           hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_bp, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, nodes, tree_size,
+                       srt_aabbs, srt_mcodes, srt_idx, n_eff, otypes, cnt, slots, ovf, head, ovf_count, ovf_cap,
+                       n_objs, node_cap);
     return hipGetLastError();
 }
 
@@ -1245,23 +1221,13 @@ hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uin
     return hipGetLastError();
 }
 
-hipError_t launch_bp_write(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
-                           const uint32_t *srt_idx, uint32_t *inv, const uint32_t *n_eff, const int32_t *otypes,
-                           const uint32_t *cnt,
-                           const uint32_t *slots, const uint32_t *ovf, const uint32_t *ovf_count, uint32_t ovf_cap,
-                           uint32_t n_ovf_host, const uint64_t *off, const uint64_t *step_off,
-                           mzh_aabb_collision *pairs, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
-                           hipStream_t stream)
+hipError_t launch_bp_place(const uint32_t *cnt, const uint32_t *slots, const uint2 *ovf, const uint32_t *head,
+                           uint32_t ovf_cap, const uint64_t *off, const uint64_t *step_off, mzh_aabb_collision *pairs,
+                           uint32_t n_objs, uint32_t n_steps, hipStream_t stream)
 {
     const uint64_t nq = (uint64_t)n_objs * n_steps;
-    hipLaunchKernelGGL(k_bp_place, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, stream, cnt, slots, off, step_off,
-                       pairs, n_objs, n_steps);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || n_ovf_host == 0) return e;
-    hipLaunchKernelGGL(k_bp<false>, dim3((n_ovf_host + 255u) / 256u), dim3(256), 0, stream, nodes, tree_size,
-                       srt_aabbs, nullptr, srt_idx, inv, n_eff, otypes, const_cast<uint32_t *>(cnt), nullptr,
-                       const_cast<uint32_t *>(ovf),
-                       const_cast<uint32_t *>(ovf_count), ovf_cap, off, step_off, pairs, n_objs, node_cap);
+    hipLaunchKernelGGL(k_bp_place, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, stream, cnt, slots, ovf, head,
+                       ovf_cap, off, step_off, pairs, n_objs, n_steps);
     return hipGetLastError();
 }
 

@@ -237,6 +237,32 @@ def test_synthetic_mix_otypes():
         assert len(c2.conjunctions) == 0
 
 
+def test_dense_duplicates_many_partners():
+    """Copies of the same trajectory (runs of equal Morton codes, i.e.
+    multi-object leaves) and a 100 km threshold: many objects have more
+    partners than are kept inline, and pairs inside a run of equal codes must
+    come out as the reference's (i, j) order gives them."""
+    from oracle import pyoracle as po
+
+    pj = mz.make_synthetic_polyjectory(400, 0.1, seed=9, frac_high=0.3)
+    traj, tm, toff, ns, tmo = pj._flat()
+    op1 = pj.poly_order + 1
+    rng = np.random.default_rng(1)
+    idx = np.concatenate([np.arange(400), rng.integers(0, 400, 300), np.repeat([3, 17], 12)])
+    trajs, times = [], []
+    for o in idx:
+        n = int(ns[o])
+        trajs.append(traj[int(toff[o]) : int(toff[o]) + n * op1 * 7].reshape(n, op1, 7))
+        times.append(tm[int(tmo[o]) : int(tmo[o]) + n + 1])
+    ot = rng.choice([1, 2], size=len(idx), p=[0.7, 0.3]).astype(np.int32)
+    for otypes in (None, ot):
+        c, ref = run_both(trajs, times, 100.0, 3.0 / 1440.0, otypes=otypes)
+        assert_screen_equal(c, ref)
+        assert_conj_equal(c.conjunctions, ref["conjunctions"])
+        counts = np.bincount(c.get_aabb_collisions(0)["i"], minlength=len(idx))
+        assert counts.max() > 8
+
+
 def test_mixed_mode_superset(leo1000):
     pj = leo1000
     c64 = mz.conjunctions(pj, 5.0, 3.0 / 1440.0, mode="fp64")
