@@ -102,7 +102,7 @@ struct mzh_ctx {
     int mode = MZH_MODE_FP64;
     std::vector<double> h_cd_begin, h_cd_end;
     DevBuf cd_begin, cd_end;
-    DevBuf aabbs, kbke, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, gbox;
+    DevBuf aabbs, kbke, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, sort_todo, gbox;
     uint64_t node_cap = 0;
     DevBuf nodes, trav, bvh_scr, tree_size;
     bool nodes_ready = false;  // nodes holds the reference-layout export of trav
@@ -304,6 +304,7 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(ctx->srt_idx.ensure(sizeof(uint32_t) * NC));
     MZH_HIP(ctx->vals_tmp.ensure(sizeof(uint32_t) * NC));
     MZH_HIP(ctx->n_eff.ensure(sizeof(uint32_t) * C));
+    MZH_HIP(ctx->sort_todo.ensure(sizeof(uint32_t) * C));
     MZH_HIP(ctx->gbox.ensure(sizeof(uint32_t) * 8u * C));
     MZH_HIP(ctx->trav.ensure(sizeof(TravNode) * ctx->node_cap * C));
     MZH_HIP(ctx->bvh_scr.ensure(bvh_scratch_stride(N) * C));
@@ -350,12 +351,10 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(launch_gbox_finalize(ctx->gbox.as<uint32_t>(), ctx->aabbs.as<float>(), N, (uint32_t)C, derr, st));
     MZH_HIP(hipEventRecord(ctx->ev[1], st));
     // stage 2: Morton codes, sort, gather
-    MZH_HIP(launch_morton(ctx->aabbs.as<float>(), ctx->mcodes.as<uint64_t>(), N, (uint32_t)C, st));
-    MZH_HIP(launch_sort(ctx->mcodes.as<uint64_t>(), ctx->aabbs.as<float>(), ctx->keys_tmp.as<uint64_t>(),
-                        ctx->vals_tmp.as<uint32_t>(), ctx->srt_mcodes.as<uint64_t>(), ctx->srt_idx.as<uint32_t>(),
-                        ctx->n_eff.as<uint32_t>(), N, (uint32_t)C, st));
-    MZH_HIP(launch_gather(ctx->aabbs.as<float>(), ctx->srt_idx.as<uint32_t>(), ctx->srt_aabbs.as<float>(), N,
-                          (uint32_t)C, st));
+    MZH_HIP(launch_morton_sort(ctx->aabbs.as<float>(), ctx->mcodes.as<uint64_t>(), ctx->keys_tmp.as<uint64_t>(),
+                               ctx->vals_tmp.as<uint32_t>(), ctx->srt_mcodes.as<uint64_t>(),
+                               ctx->srt_idx.as<uint32_t>(), ctx->srt_aabbs.as<float>(), ctx->n_eff.as<uint32_t>(),
+                               ctx->sort_todo.as<uint32_t>(), N, (uint32_t)C, st));
     MZH_HIP(hipEventRecord(ctx->ev[2], st));
     // stage 3: BVH
     MZH_HIP(launch_bvh(ctx->srt_mcodes.as<uint64_t>(), ctx->srt_aabbs.as<float>(), ctx->n_eff.as<uint32_t>(),

@@ -25,13 +25,12 @@ hipError_t launch_aabbs(const PjView &pj, const StepGrid &sg, double conj_thresh
 hipError_t launch_gbox_finalize(const uint32_t *gbox_ord, float *aabbs, uint32_t n_objs, uint32_t n_steps,
                                 DevError *err, hipStream_t stream);
 
-// Stage 2: Morton codes, segmented stable radix sort, gather.
-hipError_t launch_morton(const float *aabbs, uint64_t *mcodes, uint32_t n_objs, uint32_t n_steps, hipStream_t stream);
-hipError_t launch_sort(const uint64_t *mcodes, const float *aabbs, uint64_t *keys_tmp, uint32_t *vals_tmp,
-                       uint64_t *srt_mcodes, uint32_t *srt_idx, uint32_t *n_eff, uint32_t n_objs, uint32_t n_steps,
-                       hipStream_t stream);
-hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt_aabbs, uint32_t n_objs,
-                         uint32_t n_steps, hipStream_t stream);
+// Stage 2: Morton codes, sort by (no data, code, index), gather of the boxes
+// in sorted order; n_eff = number of objects with data per step. todo
+// (one word per step) marks the steps left to the radix sort fallback.
+hipError_t launch_morton_sort(const float *aabbs, uint64_t *mcodes, uint64_t *keys_tmp, uint32_t *vals_tmp,
+                              uint64_t *srt_mcodes, uint32_t *srt_idx, float *srt_aabbs, uint32_t *n_eff,
+                              uint32_t *todo, uint32_t n_objs, uint32_t n_steps, hipStream_t stream);
 
 // Stage 3: BVH, one workgroup per step. nodes has node_cap entries per step.
 // Also writes the depth-first traversal copy `trav` (node_cap per step);

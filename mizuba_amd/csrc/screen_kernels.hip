@@ -366,27 +366,17 @@ __device__ inline uint64_t disc_coord(float x, float mn, float mx)
     return r < 65535ull ? r : 65535ull;
 }
 
-__global__ __launch_bounds__(256) void k_morton(const float *__restrict__ aabbs, uint64_t *__restrict__ mcodes,
-                                                uint32_t N)
+// Morton code of a box (detect_conjunctions_morton.cpp:70-175): centre lb/2 +
+// ub/2 of each dimension quantized against the global box, bit b of dimension
+// d at bit 4b + d; all ones when the box is empty (no data).
+__device__ inline uint64_t morton_code(const float4 &blo, const float4 &bhi, const float4 &glo, const float4 &ghi)
 {
-    const uint32_t bps = (N + 255u) / 256u;
-    const uint32_t c = blockIdx.x / bps;
-    const uint32_t o = (blockIdx.x - c * bps) * 256u + threadIdx.x;
-    if (o >= N) return;
-    const float *step = aabbs + (size_t)c * (N + 1) * 8u;
-    const float4 blo = reinterpret_cast<const float4 *>(step + (size_t)o * 8u)[0];
-    const float4 bhi = reinterpret_cast<const float4 *>(step + (size_t)o * 8u)[1];
-    uint64_t code = ~0ull;
-    if (is_finite_f(blo.x)) {
-        const float4 glo = reinterpret_cast<const float4 *>(step + (size_t)N * 8u)[0];
-        const float4 ghi = reinterpret_cast<const float4 *>(step + (size_t)N * 8u)[1];
-        const uint64_t q0 = disc_coord(blo.x / 2 + bhi.x / 2, glo.x, ghi.x);
-        const uint64_t q1 = disc_coord(blo.y / 2 + bhi.y / 2, glo.y, ghi.y);
-        const uint64_t q2 = disc_coord(blo.z / 2 + bhi.z / 2, glo.z, ghi.z);
-        const uint64_t q3 = disc_coord(blo.w / 2 + bhi.w / 2, glo.w, ghi.w);
-        code = spread4(q0) | (spread4(q1) << 1) | (spread4(q2) << 2) | (spread4(q3) << 3);
-    }
-    mcodes[(size_t)c * N + o] = code;
+    if (!is_finite_f(blo.x)) return ~0ull;
+    const uint64_t q0 = disc_coord(blo.x / 2 + bhi.x / 2, glo.x, ghi.x);
+    const uint64_t q1 = disc_coord(blo.y / 2 + bhi.y / 2, glo.y, ghi.y);
+    const uint64_t q2 = disc_coord(blo.z / 2 + bhi.z / 2, glo.z, ghi.z);
+    const uint64_t q3 = disc_coord(blo.w / 2 + bhi.w / 2, glo.w, ghi.w);
+    return spread4(q0) | (spread4(q1) << 1) | (spread4(q2) << 2) | (spread4(q3) << 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -425,9 +415,10 @@ __global__ __launch_bounds__(kSortBlock) void k_sort(const uint64_t *__restrict_
                                                        const float *__restrict__ aabbs, uint64_t *__restrict__ keys_tmp,
                                                        uint32_t *__restrict__ vals_tmp, uint64_t *__restrict__ keys_out,
                                                        uint32_t *__restrict__ vals_out, uint32_t *__restrict__ n_eff,
-                                                       uint32_t N)
+                                                       const uint32_t *__restrict__ todo, uint32_t N)
 {
     const uint32_t c = blockIdx.x;
+    if (!todo[c]) return;  // sorted by the bucket sort
     const size_t base = (size_t)c * N;
     const uint64_t *kin = mcodes + base;
     uint64_t *kt = keys_tmp + base;
@@ -574,10 +565,12 @@ __global__ __launch_bounds__(kSortBlock) void k_sort(const uint64_t *__restrict_
 }
 
 __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ aabbs, const uint32_t *__restrict__ srt_idx,
-                                                float *__restrict__ srt_aabbs, uint32_t N)
+                                                float *__restrict__ srt_aabbs, const uint32_t *__restrict__ todo,
+                                                uint32_t N)
 {
     const uint32_t bps = (N + 1u + 255u) / 256u;
     const uint32_t c = blockIdx.x / bps;
+    if (!todo[c]) return;  // gathered by the bucket sort
     const uint32_t p = (blockIdx.x - c * bps) * 256u + threadIdx.x;
     if (p > N) return;
     const size_t sb = (size_t)c * (N + 1) * 8u;
@@ -586,6 +579,161 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ aabbs,
     float4 *d = reinterpret_cast<float4 *>(srt_aabbs + sb + (size_t)p * 8u);
     d[0] = s[0];
     d[1] = s[1];
+}
+
+// ---------------------------------------------------------------------------
+// stage 2: Morton codes, sort and gather in one workgroup per step
+// (detect_conjunctions_morton.cpp:70-223). The sort key is (box without
+// data, code, object index): the reference's comparator (:179-196) with its
+// unspecified tie order fixed to the object index. Objects are distributed
+// into 2^12 buckets by the top code bits (LDS counters); then each window of
+// 1024 sorted positions, widened to whole buckets, is staged in LDS and every
+// object finds its place by counting the smaller keys of its bucket. Objects
+// without data go after the finite ones in index order. A step with a bucket
+// larger than kBktMax (a tight cluster of objects) is left to the radix sort
+// (k_sort + k_gather), flagged in todo.
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kBktBits = 12;
+constexpr uint32_t kBkts = 1u << kBktBits;
+constexpr uint32_t kBktMax = 1024;
+constexpr uint32_t kBktWin = 1024 + 2 * kBktMax;  // a window and the buckets it cuts
+
+__device__ inline bool key_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb)
+{
+    return ka < kb || (ka == kb && va < vb);
+}
+
+__global__ __launch_bounds__(1024) void k_bucket_sort(const float *__restrict__ aabbs, uint64_t *__restrict__ mcodes,
+                                                      uint64_t *__restrict__ keys_tmp, uint32_t *__restrict__ vals_tmp,
+                                                      uint64_t *__restrict__ srt_mcodes, uint32_t *__restrict__ srt_idx,
+                                                      float *__restrict__ srt_aabbs, uint32_t *__restrict__ n_eff,
+                                                      uint32_t *__restrict__ todo, uint32_t N)
+{
+    const uint32_t c = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const float *boxes = aabbs + (size_t)c * (N + 1) * 8u;
+    float *sboxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
+    uint64_t *codes = mcodes + (size_t)c * N;
+    uint64_t *kt = keys_tmp + (size_t)c * N;
+    uint32_t *vt = vals_tmp + (size_t)c * N;
+    uint64_t *ko = srt_mcodes + (size_t)c * N;
+    uint32_t *vo = srt_idx + (size_t)c * N;
+    __shared__ uint32_t s_off[kBkts];
+    __shared__ uint64_t s_key[kBktWin];
+    __shared__ uint32_t s_val[kBktWin];
+    __shared__ uint32_t s_w[65];
+    __shared__ uint32_t s_big, s_win[2];
+    const float4 glo = reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[0];
+    const float4 ghi = reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[1];
+
+    for (uint32_t k = tid; k < kBkts; k += 1024u) s_off[k] = 0;
+    if (tid == 0) s_big = 0;
+    __syncthreads();
+    // codes and bucket sizes
+    for (uint32_t o = tid; o < N; o += 1024u) {
+        const float4 blo = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[0];
+        const float4 bhi = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[1];
+        const uint64_t code = morton_code(blo, bhi, glo, ghi);
+        codes[o] = code;
+        if (is_finite_f(blo.x)) atomicAdd(&s_off[(uint32_t)(code >> (64u - kBktBits))], 1u);
+    }
+    __syncthreads();
+    // exclusive offsets of the buckets (16 per thread), the largest bucket
+    constexpr uint32_t kPer = kBkts / 1024u;
+    uint32_t cnt[kPer], sum = 0, mx = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        cnt[k] = s_off[tid * kPer + k];
+        sum += cnt[k];
+        mx = cnt[k] > mx ? cnt[k] : mx;
+    }
+    uint32_t nfin;
+    uint32_t run = block_excl_scan_1024(sum, s_w, nfin);
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        s_off[tid * kPer + k] = run;
+        run += cnt[k];
+    }
+    if (mx > kBktMax) atomicMax(&s_big, mx);
+    __syncthreads();
+    if (s_big > kBktMax) {
+        if (tid == 0) todo[c] = 1u;
+        return;
+    }
+    if (tid == 0) {
+        todo[c] = 0u;
+        n_eff[c] = nfin;
+        // row N: the global box
+        reinterpret_cast<float4 *>(sboxes + (size_t)N * 8u)[0] = glo;
+        reinterpret_cast<float4 *>(sboxes + (size_t)N * 8u)[1] = ghi;
+    }
+    // finite objects into their buckets (any order inside a bucket); objects
+    // without data straight to their final place, in index order
+    uint32_t inf_carry = 0;
+    for (uint32_t base = 0; base < N; base += 1024u) {
+        const uint32_t o = base + tid;
+        bool fin = false, inf = false;
+        float4 blo, bhi;
+        if (o < N) {
+            blo = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[0];
+            bhi = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[1];
+            fin = is_finite_f(blo.x);
+            inf = !fin;
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan_1024(inf ? 1u : 0u, s_w, tot);
+        if (fin) {
+            const uint64_t code = codes[o];
+            const uint32_t pos = atomicAdd(&s_off[(uint32_t)(code >> (64u - kBktBits))], 1u);
+            kt[pos] = code;
+            vt[pos] = o;
+        } else if (inf) {
+            const uint32_t p = nfin + inf_carry + ex;
+            ko[p] = ~0ull;
+            vo[p] = o;
+            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] = blo;
+            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] = bhi;
+        }
+        inf_carry += tot;
+    }
+    __syncthreads();
+    __syncthreads();
+    // s_off[b] is now the end of bucket b. Windows of 1024 positions, widened
+    // to the buckets cut by their ends, are ranked from LDS.
+    for (uint32_t base = 0; base < nfin; base += 1024u) {
+        if (tid == 0) {
+            const uint32_t last = min(base + 1023u, nfin - 1u);
+            const uint32_t b0 = (uint32_t)(kt[base] >> (64u - kBktBits));
+            const uint32_t b1 = (uint32_t)(kt[last] >> (64u - kBktBits));
+            s_win[0] = b0 ? s_off[b0 - 1u] : 0u;
+            s_win[1] = s_off[b1];
+        }
+        __syncthreads();
+        const uint32_t wb = s_win[0], we = s_win[1];
+        for (uint32_t e = wb + tid; e < we; e += 1024u) {
+            s_key[e - wb] = kt[e];
+            s_val[e - wb] = vt[e];
+        }
+        __syncthreads();
+        const uint32_t e = base + tid;
+        if (e < nfin) {
+            const uint64_t key = s_key[e - wb];
+            const uint32_t val = s_val[e - wb];
+            const uint32_t b = (uint32_t)(key >> (64u - kBktBits));
+            const uint32_t beg = b ? s_off[b - 1u] : 0u, end = s_off[b];
+            uint32_t rank = 0;
+            for (uint32_t m = beg; m < end; ++m) rank += key_less(s_key[m - wb], s_val[m - wb], key, val) ? 1u : 0u;
+            const uint32_t p = beg + rank;
+            ko[p] = key;
+            vo[p] = val;
+            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] =
+                reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[0];
+            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] =
+                reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1135,26 +1283,21 @@ hipError_t launch_gbox_finalize(const uint32_t *gbox_ord, float *aabbs, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_morton(const float *aabbs, uint64_t *mcodes, uint32_t n_objs, uint32_t n_steps, hipStream_t stream)
+hipError_t launch_morton_sort(const float *aabbs, uint64_t *mcodes, uint64_t *keys_tmp, uint32_t *vals_tmp,
+                              uint64_t *srt_mcodes, uint32_t *srt_idx, float *srt_aabbs, uint32_t *n_eff,
+                              uint32_t *todo, uint32_t n_objs, uint32_t n_steps, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_morton, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, aabbs, mcodes, n_objs);
-    return hipGetLastError();
-}
-
-hipError_t launch_sort(const uint64_t *mcodes, const float *aabbs, uint64_t *keys_tmp, uint32_t *vals_tmp,
-                       uint64_t *srt_mcodes, uint32_t *srt_idx, uint32_t *n_eff, uint32_t n_objs, uint32_t n_steps,
-                       hipStream_t stream)
-{
+    hipLaunchKernelGGL(k_bucket_sort, dim3(n_steps), dim3(1024), 0, stream, aabbs, mcodes, keys_tmp, vals_tmp,
+                       srt_mcodes, srt_idx, srt_aabbs, n_eff, todo, n_objs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // steps with an oversized bucket: stable radix sort and gather
     hipLaunchKernelGGL(k_sort, dim3(n_steps), dim3(kSortBlock), 0, stream, mcodes, aabbs, keys_tmp, vals_tmp,
-                       srt_mcodes, srt_idx, n_eff, n_objs);
-    return hipGetLastError();
-}
-
-hipError_t launch_gather(const float *aabbs, const uint32_t *srt_idx, float *srt_aabbs, uint32_t n_objs,
-                         uint32_t n_steps, hipStream_t stream)
-{
+                       srt_mcodes, srt_idx, n_eff, todo, n_objs);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gather, dim3(((n_objs + 1u + 255u) / 256u) * n_steps), dim3(256), 0, stream, aabbs, srt_idx,
-                       srt_aabbs, n_objs);
+                       srt_aabbs, todo, n_objs);
     return hipGetLastError();
 }
 

@@ -263,6 +263,30 @@ def test_dense_duplicates_many_partners():
         assert counts.max() > 8
 
 
+@pytest.mark.parametrize("copies", [300, 900, 4300])
+def test_sort_large_buckets(copies):
+    """Hundreds, then thousands, of copies of one object share a Morton code:
+    the sort must handle a bucket far larger than a wave and, past 1024 keys,
+    the radix-sort path; equal codes stay in index order. The copies are
+    masked so that the pair count stays small."""
+    from oracle import pyoracle as po
+
+    pj = mz.make_synthetic_polyjectory(200, 0.02, seed=4, frac_high=0.3)
+    traj, tm, toff, ns, tmo = pj._flat()
+    op1 = pj.poly_order + 1
+    idx = np.concatenate([np.arange(200), np.full(copies, 7)])
+    trajs, times = [], []
+    for o in idx:
+        n = int(ns[o])
+        trajs.append(traj[int(toff[o]) : int(toff[o]) + n * op1 * 7].reshape(n, op1, 7))
+        times.append(tm[int(tmo[o]) : int(tmo[o]) + n + 1])
+    ot = np.ones(len(idx), np.int32)
+    ot[200:] = 4
+    c, ref = run_both(trajs, times, 5.0, 3.0 / 1440.0, otypes=ot)
+    assert_screen_equal(c, ref)
+    assert_conj_equal(c.conjunctions, ref["conjunctions"])
+
+
 def test_mixed_mode_superset(leo1000):
     pj = leo1000
     c64 = mz.conjunctions(pj, 5.0, 3.0 / 1440.0, mode="fp64")
