@@ -134,6 +134,19 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
  * read with the accessors below. Error messages are the reference's. */
 int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode);
 
+/* Screen n independent problems together: ctxs[k] (each with its own uploaded
+ * polyjectory) is screened with conj_thresh[k], conj_det_interval[k] and
+ * otypes[k] (otypes may be NULL, as may any otypes[k]), exactly as
+ * mzh_screen would. The problems run concurrently, each on its own context's
+ * stream, so that many small problems fill the GPU together. status[k]
+ * receives each problem's return code (its message through mzh_last_error on
+ * ctxs[k]); the function returns the first non-zero code, MZH_OK if none, or
+ * MZH_EINVAL if the arguments are malformed (a null or repeated context).
+ * The reference has no batched entry: it is conjunctions::conjunctions
+ * (mizuba/conjunctions.cpp:263-360) called once per problem. */
+int mzh_screen_batch(mzh_ctx *const *ctxs, uint64_t n, const double *conj_thresh, const double *conj_det_interval,
+                     const int32_t *const *otypes, int mode, int *status);
+
 /* ---- accessors (mizuba/conjunctions.cpp:394-509) ------------------------ */
 
 /* conjunctions::get_n_cd_steps (conjunctions.cpp:457-460). */

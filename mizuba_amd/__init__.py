@@ -9,6 +9,9 @@ include/mizuba_amd.h.
 Extra keyword arguments of `conjunctions`: `mode` ("fp64": the reference's
 arithmetic, bit-exact screen; "mixed": FP32 enclosures with a rigorous margin,
 same conjunction list) and `device` (HIP device index).
+`conjunctions_batch(pjs, conj_threshs, conj_det_intervals, otypes=None, mode=...)`
+screens several independent problems together and returns one `conjunctions`
+object per problem, identical to constructing them one by one.
 
 The native library must be built (`python -m mizuba_amd.build_native`); there
 is no CPU fallback: without a HIP device, constructing `conjunctions` raises
@@ -28,6 +31,7 @@ except ImportError as _e:  # pragma: no cover - depends on the build
 
 polyjectory = _core.polyjectory
 conjunctions = _core.conjunctions
+conjunctions_batch = _core.conjunctions_batch
 Screener = _core.Screener
 make_synthetic_polyjectory = _core.make_synthetic_polyjectory
 device_count = _core.device_count
@@ -44,6 +48,7 @@ class otype(_enum.IntEnum):
 __all__ = [
     "polyjectory",
     "conjunctions",
+    "conjunctions_batch",
     "otype",
     "Screener",
     "make_synthetic_polyjectory",

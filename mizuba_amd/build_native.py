@@ -102,7 +102,7 @@ def build(verbose: bool = False, jobs: int | None = None, force: bool = True) ->
         for r in ex.map(_run, cmds):
             if verbose and r.stderr.strip():
                 print(r.stderr, file=sys.stderr)
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(LIB), *map(str, objs)])
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", str(LIB), *map(str, objs)])
 
     # pybind11 front end (host compiler only)
     import pybind11

@@ -8,12 +8,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mizuba_amd.h"
@@ -66,6 +69,26 @@ struct DevBuf {
             return e;
         }
         bytes = alloc;
+        return hipSuccess;
+    }
+    // Grow to at least b bytes, keeping the first `keep` bytes.
+    hipError_t grow(size_t b, size_t keep, hipStream_t st)
+    {
+        if (b <= bytes && p) return hipSuccess;
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, b);
+        if (e != hipSuccess) return e;
+        if (p && keep) {
+            e = hipMemcpyAsync(q, p, std::min(keep, bytes), hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                (void)hipFree(q);
+                return e;
+            }
+        }
+        if (p) (void)hipFree(p);
+        p = q;
+        bytes = b;
         return hipSuccess;
     }
     template <typename T>
@@ -398,14 +421,16 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
                                 ctx->head.as<uint32_t>(), ctx->ovf_cap, ctx->off.as<uint64_t>(),
                                 ctx->step_off.as<uint64_t>(), ctx->pairs.as<mzh_aabb_collision>(), N, (uint32_t)C, st));
         MZH_HIP(hipEventRecord(ctx->ev[4], st));
-        // stage 5: narrow phase (re-run with a larger output buffer on overflow)
+        // stage 5: narrow phase, in passes over at most pass_pairs broad-phase
+        // pairs (64 Mi, or MIZUBA_AMD_NARROW_PASS_PAIRS) so that the step-pair
+        // and candidate queues stay bounded; a pass whose queues or output
+        // overflow is re-run with larger buffers
         NarrowArgs na;
         na.pj = pj;
         na.sg = sg;
         na.kbke = ctx->kbke.as<const uint2>();
         na.pairs = ctx->pairs.as<const mzh_aabb_collision>();
         na.step_off = ctx->step_off.as<const uint64_t>();
-        na.n_pairs = total;
         na.conj_thresh = conj_thresh;
         const uint32_t op1 = ctx->order + 1;
         const double pi = 3.141592653589793238462643383279502884;
@@ -414,56 +439,79 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
         na.np_report = ctx->np_report.as<unsigned long long>();
         na.conj_count = ctx->conj_count.as<unsigned long long>();
         na.err = derr;
+        uint64_t pass_pairs = 64ull << 20;  // bounds the queues to a few GB
+        if (const char *env = std::getenv("MIZUBA_AMD_NARROW_PASS_PAIRS")) {
+            const unsigned long long v = std::strtoull(env, nullptr, 10);
+            if (v > 0) pass_pairs = v;
+        }
+        const uint64_t chunk = std::min<uint64_t>(total, pass_pairs);
         if (ctx->conj_cap == 0) ctx->conj_cap = 1u << 16;
         // queued root-finding candidates: about a fifth of the pairs on LEO catalogs
         const size_t cand_bytes = narrow_cand_bytes(ctx->order);
-        ctx->cand_cap = std::max<uint64_t>(ctx->cand_cap, total / 2 + 65536);
+        ctx->cand_cap = std::max<uint64_t>(ctx->cand_cap, chunk / 2 + 65536);
         // overlapping trajectory-step pairs: about 1.5 per pair for T/8 steps and 3-minute conjunction steps
-        ctx->steps_cap = std::max<uint64_t>(ctx->steps_cap, 2 * total + 65536);
+        ctx->steps_cap = std::max<uint64_t>(ctx->steps_cap, 2 * chunk + 65536);
         MZH_HIP(ctx->cand_count.ensure(sizeof(unsigned long long)));
         MZH_HIP(ctx->steps_count.ensure(sizeof(unsigned long long)));
         na.cand_count = ctx->cand_count.as<unsigned long long>();
         na.steps_count = ctx->steps_count.as<unsigned long long>();
-        for (int attempt = 0; attempt < 4; ++attempt) {
-            MZH_HIP(ctx->conj_raw.ensure(sizeof(mzh_conj) * ctx->conj_cap));
-            MZH_HIP(ctx->conj_step.ensure(sizeof(uint32_t) * ctx->conj_cap));
-            MZH_HIP(ctx->cand.ensure(cand_bytes * ctx->cand_cap));
-            MZH_HIP(ctx->steps.ensure(sizeof(StepPair) * ctx->steps_cap));
-            na.steps = ctx->steps.p;
-            na.steps_cap = ctx->steps_cap;
-            MZH_HIP(hipMemsetAsync(ctx->conj_count.p, 0, sizeof(unsigned long long), st));
-            MZH_HIP(hipMemsetAsync(ctx->cand_count.p, 0, sizeof(unsigned long long), st));
-            MZH_HIP(hipMemsetAsync(ctx->steps_count.p, 0, sizeof(unsigned long long), st));
-            MZH_HIP(hipMemsetAsync(ctx->np_report.p, 0, sizeof(unsigned long long) * MZH_NP_COUNT, st));
-            na.conj = ctx->conj_raw.as<mzh_conj>();
-            na.conj_step = ctx->conj_step.as<uint32_t>();
-            na.conj_cap = ctx->conj_cap;
-            na.cand = ctx->cand.p;
-            na.cand_cap = ctx->cand_cap;
-            MZH_HIP(hipEventRecord(ctx->ev[8], st));
-            MZH_HIP(launch_narrow(na, st));
-            MZH_HIP(hipEventRecord(ctx->ev[9], st));
-            unsigned long long cnts[3] = {0, 0, 0};
-            MZH_HIP(hipMemcpyAsync(&cnts[0], ctx->conj_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-            MZH_HIP(hipMemcpyAsync(&cnts[1], ctx->cand_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-            MZH_HIP(hipMemcpyAsync(&cnts[2], ctx->steps_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-            MZH_HIP(hipStreamSynchronize(st));
-            ctx->n_conj = cnts[0];
-            const bool conj_ok = cnts[0] <= ctx->conj_cap, cand_ok = cnts[1] <= ctx->cand_cap,
-                       steps_ok = cnts[2] <= ctx->steps_cap;
-            if (conj_ok && cand_ok && steps_ok) break;
-            if (!conj_ok) ctx->conj_cap = cnts[0] + cnts[0] / 4 + 1024;
-            if (!cand_ok) ctx->cand_cap = cnts[1] + cnts[1] / 4 + 1024;
-            if (!steps_ok) ctx->steps_cap = cnts[2] + cnts[2] / 4 + 1024;
+        MZH_HIP(ctx->conj_raw.grow(sizeof(mzh_conj) * ctx->conj_cap, 0, st));
+        MZH_HIP(ctx->conj_step.grow(sizeof(uint32_t) * ctx->conj_cap, 0, st));
+        MZH_HIP(hipMemsetAsync(ctx->conj_count.p, 0, sizeof(unsigned long long), st));
+        MZH_HIP(hipMemsetAsync(ctx->np_report.p, 0, sizeof(unsigned long long) * MZH_NP_COUNT, st));
+        // state after the last completed pass: conjunction count, counters
+        unsigned long long done_conj = 0, done_rep[MZH_NP_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+        MZH_HIP(hipEventRecord(ctx->ev[8], st));
+        for (uint64_t pb = 0; pb < total; pb += chunk) {
+            na.pair_begin = pb;
+            na.pair_end = std::min<uint64_t>(total, pb + chunk);
+            bool ok = false;
+            for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
+                MZH_HIP(ctx->conj_raw.grow(sizeof(mzh_conj) * ctx->conj_cap, sizeof(mzh_conj) * done_conj, st));
+                MZH_HIP(ctx->conj_step.grow(sizeof(uint32_t) * ctx->conj_cap, sizeof(uint32_t) * done_conj, st));
+                MZH_HIP(ctx->cand.ensure(cand_bytes * ctx->cand_cap));
+                MZH_HIP(ctx->steps.ensure(sizeof(StepPair) * ctx->steps_cap));
+                if (attempt > 0) {
+                    // undo the overflowed pass
+                    MZH_HIP(hipMemcpyAsync(ctx->conj_count.p, &done_conj, sizeof(done_conj), hipMemcpyHostToDevice,
+                                           st));
+                    MZH_HIP(hipMemcpyAsync(ctx->np_report.p, done_rep, sizeof(done_rep), hipMemcpyHostToDevice, st));
+                }
+                MZH_HIP(hipMemsetAsync(ctx->cand_count.p, 0, sizeof(unsigned long long), st));
+                MZH_HIP(hipMemsetAsync(ctx->steps_count.p, 0, sizeof(unsigned long long), st));
+                na.steps = ctx->steps.p;
+                na.steps_cap = ctx->steps_cap;
+                na.conj = ctx->conj_raw.as<mzh_conj>();
+                na.conj_step = ctx->conj_step.as<uint32_t>();
+                na.conj_cap = ctx->conj_cap;
+                na.cand = ctx->cand.p;
+                na.cand_cap = ctx->cand_cap;
+                MZH_HIP(launch_narrow(na, st));
+                unsigned long long cnts[3] = {0, 0, 0}, rep[MZH_NP_COUNT];
+                MZH_HIP(hipMemcpyAsync(&cnts[0], ctx->conj_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                       st));
+                MZH_HIP(hipMemcpyAsync(&cnts[1], ctx->cand_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                       st));
+                MZH_HIP(hipMemcpyAsync(&cnts[2], ctx->steps_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                       st));
+                MZH_HIP(hipMemcpyAsync(rep, ctx->np_report.p, sizeof(rep), hipMemcpyDeviceToHost, st));
+                MZH_HIP(hipStreamSynchronize(st));
+                const bool conj_ok = cnts[0] <= ctx->conj_cap, cand_ok = cnts[1] <= ctx->cand_cap,
+                           steps_ok = cnts[2] <= ctx->steps_cap;
+                ok = conj_ok && cand_ok && steps_ok;
+                if (ok) {
+                    done_conj = cnts[0];
+                    std::memcpy(done_rep, rep, sizeof(rep));
+                    break;
+                }
+                if (!conj_ok) ctx->conj_cap = cnts[0] + cnts[0] / 4 + 1024;
+                if (!cand_ok) ctx->cand_cap = cnts[1] + cnts[1] / 4 + 1024;
+                if (!steps_ok) ctx->steps_cap = cnts[2] + cnts[2] / 4 + 1024;
+            }
+            if (!ok) throw StatusError{MZH_ERUNTIME, "narrow-phase buffers overflowed after resizing"};
         }
-        if (ctx->n_conj > ctx->conj_cap) throw StatusError{MZH_ERUNTIME, "conjunction buffer overflow"};
-        {
-            unsigned long long nq = 0;
-            MZH_HIP(hipMemcpy(&nq, ctx->cand_count.p, sizeof(nq), hipMemcpyDeviceToHost));
-            if (nq > ctx->cand_cap) throw StatusError{MZH_ERUNTIME, "narrow-phase candidate queue overflow"};
-            MZH_HIP(hipMemcpy(&nq, ctx->steps_count.p, sizeof(nq), hipMemcpyDeviceToHost));
-            if (nq > ctx->steps_cap) throw StatusError{MZH_ERUNTIME, "narrow-phase step-pair list overflow"};
-        }
+        MZH_HIP(hipEventRecord(ctx->ev[9], st));
+        ctx->n_conj = done_conj;
         const uint64_t M = ctx->n_conj;
         MZH_HIP(ctx->conj_sorted.ensure(sizeof(mzh_conj) * (M ? M : 1)));
         MZH_HIP(ctx->conj_pos.ensure(sizeof(uint32_t) * (M ? M : 1)));
@@ -658,6 +706,38 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
 int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode)
 {
     return guarded(ctx, [&] { run_screen(ctx, conj_thresh, conj_det_interval, otypes, mode); });
+}
+
+int mzh_screen_batch(mzh_ctx *const *ctxs, uint64_t n, const double *conj_thresh, const double *conj_det_interval,
+                     const int32_t *const *otypes, int mode, int *status)
+{
+    if (n == 0) return MZH_OK;
+    if (!ctxs || !conj_thresh || !conj_det_interval || !status) return MZH_EINVAL;
+    {
+        std::vector<const mzh_ctx *> seen(ctxs, ctxs + n);
+        std::sort(seen.begin(), seen.end());
+        if (!seen.front() || std::adjacent_find(seen.begin(), seen.end()) != seen.end()) return MZH_EINVAL;
+    }
+    // One host thread per problem in flight: each drives its own context's
+    // stream, including the two synchronisations that size the pair and
+    // conjunction buffers, so the problems overlap on the device.
+    std::atomic<uint64_t> next{0};
+    auto worker = [&] {
+        for (uint64_t k = next.fetch_add(1); k < n; k = next.fetch_add(1))
+            status[k] = mzh_screen(ctxs[k], conj_thresh[k], conj_det_interval[k], otypes ? otypes[k] : nullptr, mode);
+    };
+    const uint64_t nthreads = std::min<uint64_t>(n, 8);
+    std::vector<std::thread> pool;
+    try {
+        for (uint64_t t = 1; t < nthreads; ++t) pool.emplace_back(worker);
+    } catch (...) {
+        // fewer threads: the remaining problems are taken by those that started
+    }
+    worker();
+    for (auto &th : pool) th.join();
+    for (uint64_t k = 0; k < n; ++k)
+        if (status[k] != MZH_OK) return status[k];
+    return MZH_OK;
 }
 
 int mzh_get_n_cd_steps(const mzh_ctx *ctx, uint64_t *out)

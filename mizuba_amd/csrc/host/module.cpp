@@ -348,6 +348,30 @@ const std::vector<T> &lazy(ConjImpl &im, std::unique_ptr<std::vector<T>> &slot, 
     return *slot;
 }
 
+// Fill the accessor state of a screened problem.
+void finish_impl(ConjImpl &im, const Polyjectory &pj, double conj_thresh, double conj_det_interval,
+                 const std::optional<std::vector<int32_t>> &otypes, int m)
+{
+    im.pj = pj.d;
+    im.conj_thresh = conj_thresh;
+    im.conj_det_interval = conj_det_interval;
+    im.mode = m;
+    im.N = pj.d->n_objs;
+    check(mzh_get_n_cd_steps(im.ctx->ctx, &im.C), im.ctx->ctx);
+    im.otypes = otypes ? *otypes : std::vector<int32_t>(pj.d->n_objs, 1);
+    im.cd_end_times.resize(im.C);
+    check(mzh_copy_cd_end_times(im.ctx->ctx, im.cd_end_times.data()), im.ctx->ctx);
+}
+
+void check_otypes_size(const std::optional<std::vector<int32_t>> &otypes, uint64_t n_objs)
+{
+    if (otypes && otypes->size() != n_objs)
+        throw std::invalid_argument("Invalid array of object types passed to the constructor of a conjunctions "
+                                    "objects: the expected size is "
+                                    + std::to_string(n_objs) + ", but the actual size is "
+                                    + std::to_string(otypes->size()) + " instead");
+}
+
 Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double conj_det_interval,
                                std::optional<std::vector<int32_t>> otypes, std::optional<py::object> data_dir,
                                std::optional<py::object> tmpdir, const std::string &mode, int device)
@@ -360,16 +384,49 @@ Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double
     im->ctx = std::make_unique<Ctx>(device);
     im->ctx->upload(pj.d);
     im->ctx->screen(conj_thresh, conj_det_interval, otypes, m);
-    im->pj = pj.d;
-    im->conj_thresh = conj_thresh;
-    im->conj_det_interval = conj_det_interval;
-    im->mode = m;
-    im->N = pj.d->n_objs;
-    check(mzh_get_n_cd_steps(im->ctx->ctx, &im->C), im->ctx->ctx);
-    im->otypes = otypes ? *otypes : std::vector<int32_t>(pj.d->n_objs, 1);
-    im->cd_end_times.resize(im->C);
-    check(mzh_copy_cd_end_times(im->ctx->ctx, im->cd_end_times.data()), im->ctx->ctx);
+    finish_impl(*im, pj, conj_thresh, conj_det_interval, otypes, m);
     return Conjunctions{im};
+}
+
+// Several independent problems screened together (mzh_screen_batch); the
+// reference would construct one conjunctions object per problem.
+std::vector<Conjunctions> make_conjunctions_batch(const std::vector<Polyjectory> &pjs,
+                                                  const std::vector<double> &conj_threshs,
+                                                  const std::vector<double> &conj_det_intervals,
+                                                  std::optional<std::vector<std::optional<std::vector<int32_t>>>> otypes,
+                                                  const std::string &mode, int device)
+{
+    const size_t n = pjs.size();
+    if (conj_threshs.size() != n || conj_det_intervals.size() != n || (otypes && otypes->size() != n))
+        throw std::invalid_argument("conjunctions_batch: the polyjectories, thresholds, intervals and object types "
+                                    "must come in lists of the same length");
+    const int m = parse_mode(mode);
+    std::vector<std::shared_ptr<ConjImpl>> ims(n);
+    std::vector<mzh_ctx *> ctxs(n);
+    std::vector<const int32_t *> ots(n, nullptr);
+    for (size_t k = 0; k < n; ++k) {
+        if (otypes) check_otypes_size((*otypes)[k], pjs[k].d->n_objs);
+        ims[k] = std::make_shared<ConjImpl>();
+        ims[k]->ctx = std::make_unique<Ctx>(device);
+        ims[k]->ctx->upload(pjs[k].d);
+        ctxs[k] = ims[k]->ctx->ctx;
+        if (otypes && (*otypes)[k]) ots[k] = (*otypes)[k]->data();
+    }
+    std::vector<int> status(n, MZH_OK);
+    {
+        py::gil_scoped_release rel;
+        (void)mzh_screen_batch(ctxs.data(), n, conj_threshs.data(), conj_det_intervals.data(), ots.data(), m,
+                               status.data());
+    }
+    std::vector<Conjunctions> out;
+    out.reserve(n);
+    for (size_t k = 0; k < n; ++k) {
+        check(status[k], ctxs[k]);
+        finish_impl(*ims[k], pjs[k], conj_threshs[k], conj_det_intervals[k],
+                    otypes ? (*otypes)[k] : std::optional<std::vector<int32_t>>{}, m);
+        out.push_back(Conjunctions{ims[k]});
+    }
+    return out;
 }
 
 } // namespace
@@ -511,6 +568,11 @@ PYBIND11_MODULE(_core, m)
             check(mzh_copy_conjunctions(c.ctx, static_cast<mzh_conj *>(out.mutable_data())), c.ctx);
             return out;
         });
+
+    m.def("conjunctions_batch", &make_conjunctions_batch, "pjs"_a, "conj_threshs"_a, "conj_det_intervals"_a,
+          "otypes"_a = py::none(), "mode"_a = "fp64", "device"_a = 0,
+          "Screen several independent polyjectories together on one GPU; returns one conjunctions object per "
+          "problem, identical to constructing them one by one.");
 
     m.def("device_count", []() {
         int n = 0;

@@ -148,6 +148,11 @@ SynthOut make_synthetic(const SynthParams &p)
         const double h0 = T_days / 8;
         const double hmin = 5.0 / kDay;
         std::vector<double> &tr = traj_o[oi], &tm = time_o[oi];
+        // one allocation per object: growing these vectors step by step from
+        // many threads at once serialises on the allocator's memory maps
+        const size_t est_steps = (size_t)std::ceil(p.days / h0) + 16;
+        tr.reserve(est_steps * 7 * (size_t)op1);
+        tm.reserve(est_steps + 1);
         double t0 = 0;
         tm.push_back(0.0);
         std::vector<double> x(op1), st(7), c(7 * (size_t)op1);

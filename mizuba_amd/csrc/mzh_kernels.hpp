@@ -66,14 +66,15 @@ hipError_t launch_bp_place(const uint32_t *cnt, const uint32_t *slots, const uin
                            uint32_t ovf_cap, const uint64_t *off, const uint64_t *step_off, mzh_aabb_collision *pairs,
                            uint32_t n_objs, uint32_t n_steps, hipStream_t stream);
 
-// Stage 5: narrow phase over all pairs of all steps.
+// Stage 5: narrow phase over the broad-phase pairs [pair_begin, pair_end) of
+// the concatenated pair list (all steps).
 struct NarrowArgs {
     PjView pj;
     StepGrid sg;
     const uint2 *kbke;  // [C][N] overlapping trajectory-step ranges from the AABB stage
     const mzh_aabb_collision *pairs;
     const uint64_t *step_off;  // [C + 1], exclusive prefix of the per-step pair counts
-    uint64_t n_pairs;
+    uint64_t pair_begin, pair_end;
     double conj_thresh;
     double cheby[kMaxOrder + 1];  // (cos((2i+1)pi/(2 op1)) + 1) / 2, i = 0..order
     mzh_conj *conj;               // output records (capacity conj_cap)

@@ -602,12 +602,12 @@ __device__ inline void translate1s(const double *__restrict__ src, double a, dou
 // appended to the step-pair list (wave-aggregated atomics).
 __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
 {
-    const uint64_t p = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t p = a.pair_begin + (uint64_t)blockIdx.x * 256u + threadIdx.x;
     uint64_t kbi = 0, kbj = 0, kei = 0, kej = 0;
     uint32_t i = 0, j = 0, c = 0;
     const double *ti = nullptr, *tj = nullptr;
     double cb = 0, ce = 0;
-    if (p < a.n_pairs) {
+    if (p < a.pair_end) {
         uint32_t lo = 0, n = a.sg.n_steps;
         while (n > 0) {
             const uint32_t h = n / 2;
@@ -930,7 +930,7 @@ __global__ void k_conj_inv(const uint32_t *__restrict__ pos, uint64_t n, uint32_
 template <int ORDER>
 hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream)
 {
-    const uint64_t blocks = (a.n_pairs + 255) / 256;
+    const uint64_t blocks = (a.pair_end - a.pair_begin + 255) / 256;
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_np_enum, dim3((unsigned)blocks), dim3(256), 0, stream, a);
     hipError_t e = hipGetLastError();

@@ -1,0 +1,90 @@
+"""GPU runs at the full sizes of the larger configurations: 30 000 objects over
+7 days (3 360 conjunction steps) and the dense-shell stress case (100 000
+objects, 70 000 of them in five near-circular shells, 10 km, 2-minute steps).
+
+The CPU oracle cannot screen these in a test's time, so they are checked
+through properties that hold at any size, and against the oracle on windows
+of conjunction steps (the oracle screens any step range on its own):
+- the FP64 screen equals the oracle bit for bit on the sampled windows (boxes,
+  codes, sort order, trees, pairs, conjunctions);
+- the mixed-precision conjunction list equals the FP64 one bit for bit;
+- pairs are sorted, i < j; conjunctions have DCA below the threshold and DCA
+  equal to the distance of their recorded positions;
+- the narrow-phase counters add up: candidates = distance rejects + root
+  findings, minima - discarded minima = conjunctions.
+"""
+
+import numpy as np
+import pytest
+
+import mizuba_amd as mz
+
+from . import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint8)
+
+
+def check_window(c, pj, thresh, interval, k0, k1):
+    from oracle import pyoracle as po
+
+    ref = po.run(H.flat_of(pj), thresh, interval, step_begin=k0, step_end=k1)
+    for name in ("aabbs", "srt_aabbs", "mcodes", "srt_mcodes", "srt_idx"):
+        g = getattr(c, name)[k0:k1]
+        assert np.array_equal(_bits(g), _bits(ref[name])), (name, k0, k1)
+    for k in range(k1 - k0):
+        assert np.array_equal(_bits(c.get_bvh_tree(k0 + k)), _bits(po.get_bvh_tree(ref, k))), k0 + k
+        assert np.array_equal(_bits(c.get_aabb_collisions(k0 + k)), _bits(po.get_aabb_collisions(ref, k))), k0 + k
+    # conjunctions found in the window's steps
+    t0 = 0.0 if k0 == 0 else c.cd_end_times[k0 - 1]
+    t1 = c.cd_end_times[k1 - 1]
+    cj = c.conjunctions
+    g = cj[(cj["tca"] >= t0) & (cj["tca"] < t1)]
+    assert np.array_equal(_bits(g), _bits(ref["conjunctions"])), (k0, k1, len(g), len(ref["conjunctions"]))
+
+
+def check_properties(c64, cmx, thresh):
+    # mixed = FP64, bit for bit
+    assert np.array_equal(_bits(c64.conjunctions), _bits(cmx.conjunctions))
+    cj = c64.conjunctions
+    assert len(cj) > 0
+    assert np.all(cj["i"] < cj["j"])
+    assert np.all(cj["dca"] < thresh)
+    d = np.sqrt(np.sum((cj["ri"] - cj["rj"]) ** 2, axis=1))
+    np.testing.assert_allclose(cj["dca"], d, rtol=1e-12, atol=1e-12)
+    rep = c64.np_report
+    assert rep["n_tot_conj_candidates"] == rep["n_dist2_check"] + rep["n_poly_roots"]
+    assert rep["n_tot_dist_minima"] - rep["n_tot_discarded_dist_minima"] == len(cj)
+    # pairs of a few steps: sorted, i < j
+    C = c64.n_cd_steps
+    for k in (0, C // 3, C - 1):
+        p = c64.get_aabb_collisions(k)
+        assert np.all(p["i"] < p["j"])
+        key = p["i"].astype(np.uint64) << np.uint64(32) | p["j"].astype(np.uint64)
+        assert np.all(np.diff(key.astype(np.int64)) > 0)
+
+
+def test_config_30k_7days():
+    pj = mz.make_synthetic_polyjectory(30000, 7.0, seed=2026, frac_high=0.2)
+    thresh, interval = 5.0, 3.0 / 1440.0
+    c64 = mz.conjunctions(pj, thresh, interval, mode="fp64")
+    cmx = mz.conjunctions(pj, thresh, interval, mode="mixed")
+    assert c64.n_cd_steps == 3360
+    check_properties(c64, cmx, thresh)
+    check_window(c64, pj, thresh, interval, 0, 3)
+    check_window(c64, pj, thresh, interval, 1680, 1683)
+    check_window(c64, pj, thresh, interval, 3357, 3360)
+
+
+def test_config_dense_shells_100k():
+    pj = mz.make_synthetic_polyjectory(100000, 1.0, seed=2027, frac_high=0.0, n_shell=70000)
+    thresh, interval = 10.0, 2.0 / 1440.0
+    c64 = mz.conjunctions(pj, thresh, interval, mode="fp64")
+    cmx = mz.conjunctions(pj, thresh, interval, mode="mixed")
+    assert c64.n_cd_steps == 720
+    check_properties(c64, cmx, thresh)
+    check_window(c64, pj, thresh, interval, 0, 2)
+    check_window(c64, pj, thresh, interval, 718, 720)

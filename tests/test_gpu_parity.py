@@ -287,6 +287,46 @@ def test_sort_large_buckets(copies):
     assert_conj_equal(c.conjunctions, ref["conjunctions"])
 
 
+def test_batch_equals_individual():
+    """Problems of uneven size, threshold and horizon screened together give
+    exactly what screening them one by one gives; a failing problem reports
+    its own error."""
+    rng = np.random.default_rng(8)
+    pjs, threshs, intervals, ots = [], [], [], []
+    for k, (n, days) in enumerate([(300, 0.3), (2500, 0.2), (900, 0.5), (50, 0.1), (1500, 0.25), (4000, 0.1)]):
+        pjs.append(mz.make_synthetic_polyjectory(n, days, seed=100 + k, frac_high=0.2))
+        threshs.append(float(rng.uniform(1, 10)))
+        intervals.append(float(rng.choice([1.0, 2.0, 3.0])) / 1440.0)
+        ots.append(rng.choice([1, 2, 4], size=n, p=[0.6, 0.3, 0.1]).astype(np.int32) if k % 2 else None)
+    for mode in ("fp64", "mixed"):
+        batch = mz.conjunctions_batch(pjs, threshs, intervals, otypes=ots, mode=mode)
+        assert len(batch) == len(pjs)
+        for pj, th, iv, ot, cb in zip(pjs, threshs, intervals, ots, batch):
+            ci = mz.conjunctions(pj, th, iv, otypes=ot, mode=mode)
+            assert cb.n_cd_steps == ci.n_cd_steps
+            assert np.array_equal(_bits(cb.srt_aabbs), _bits(ci.srt_aabbs))
+            assert np.array_equal(_bits(cb.srt_idx), _bits(ci.srt_idx))
+            for k in (0, ci.n_cd_steps - 1):
+                assert np.array_equal(_bits(cb.get_bvh_tree(k)), _bits(ci.get_bvh_tree(k)))
+                assert np.array_equal(_bits(cb.get_aabb_collisions(k)), _bits(ci.get_aabb_collisions(k)))
+            assert np.array_equal(_bits(cb.conjunctions), _bits(ci.conjunctions))
+            assert cb.np_report == ci.np_report
+    with pytest.raises(ValueError, match="The conjunction threshold must be finite and positive"):
+        mz.conjunctions_batch(pjs[:2], [threshs[0], -1.0], intervals[:2])
+
+
+def test_narrow_phase_in_passes(leo1000, monkeypatch):
+    """The narrow phase split into many passes over the pair list (a few
+    thousand pairs each, so steps straddle passes) gives the same records and
+    counters as one pass."""
+    c1 = mz.conjunctions(leo1000, 5.0, 3.0 / 1440.0)
+    monkeypatch.setenv("MIZUBA_AMD_NARROW_PASS_PAIRS", "3001")
+    c2 = mz.conjunctions(leo1000, 5.0, 3.0 / 1440.0)
+    assert len(c2.conjunctions) > 0
+    assert np.array_equal(_bits(c1.conjunctions), _bits(c2.conjunctions))
+    assert c1.np_report == c2.np_report
+
+
 def test_mixed_mode_superset(leo1000):
     pj = leo1000
     c64 = mz.conjunctions(pj, 5.0, 3.0 / 1440.0, mode="fp64")

@@ -43,6 +43,28 @@ def test_c_abi_without_context():
     assert lib.mzh_last_error(None, buf, 256) == 1
 
 
+def test_screen_batch_argument_checks():
+    lib = ctypes.CDLL(str(ROOT / "mizuba_amd" / "libmizuba_amd.so"))
+    one = (ctypes.c_double * 2)(1.0, 1.0)
+    st = (ctypes.c_int * 2)()
+    # nothing to do
+    assert lib.mzh_screen_batch(None, ctypes.c_uint64(0), None, None, None, 0, None) == 0
+    # null context, repeated context, missing arrays: MZH_EINVAL before anything runs
+    ctxs = (ctypes.c_void_p * 2)(None, None)
+    assert lib.mzh_screen_batch(ctxs, ctypes.c_uint64(2), one, one, None, 0, st) == 1
+    fake = ctypes.c_void_p(0x1000)
+    ctxs = (ctypes.c_void_p * 2)(fake, fake)
+    assert lib.mzh_screen_batch(ctxs, ctypes.c_uint64(2), one, one, None, 0, st) == 1
+    assert lib.mzh_screen_batch(ctxs, ctypes.c_uint64(2), None, one, None, 0, st) == 1
+
+
+def test_conjunctions_batch_length_mismatch():
+    tcs, tms = H.planar_circ()
+    pj = mz.polyjectory([tcs], [tms], [0])
+    with pytest.raises(ValueError, match="lists of the same length"):
+        mz.conjunctions_batch([pj, pj], [1.0], [0.1, 0.1])
+
+
 @pytest.mark.skipif(mz.device_count() > 0, reason="checks the behaviour on a host without a HIP device")
 def test_no_device_fails_loudly():
     tcs, tms = H.planar_circ()
