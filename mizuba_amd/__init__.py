@@ -33,6 +33,7 @@ polyjectory = _core.polyjectory
 conjunctions = _core.conjunctions
 conjunctions_batch = _core.conjunctions_batch
 Screener = _core.Screener
+screen_batch = _core.screen_batch
 make_synthetic_polyjectory = _core.make_synthetic_polyjectory
 device_count = _core.device_count
 
@@ -51,6 +52,7 @@ __all__ = [
     "conjunctions_batch",
     "otype",
     "Screener",
+    "screen_batch",
     "make_synthetic_polyjectory",
     "device_count",
 ]

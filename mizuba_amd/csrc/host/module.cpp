@@ -569,6 +569,30 @@ PYBIND11_MODULE(_core, m)
             return out;
         });
 
+    m.def(
+        "screen_batch",
+        [](const std::vector<Ctx *> &scrs, const std::vector<double> &threshs, const std::vector<double> &intervals,
+           const std::string &mode) {
+            const size_t n = scrs.size();
+            if (threshs.size() != n || intervals.size() != n)
+                throw std::invalid_argument("screen_batch: the screeners, thresholds and intervals must come in lists "
+                                            "of the same length");
+            std::vector<mzh_ctx *> ctxs(n);
+            for (size_t k = 0; k < n; ++k) {
+                if (!scrs[k] || !scrs[k]->pj) throw std::invalid_argument("No polyjectory has been uploaded");
+                ctxs[k] = scrs[k]->ctx;
+            }
+            std::vector<int> status(n, MZH_OK);
+            {
+                py::gil_scoped_release rel;
+                (void)mzh_screen_batch(ctxs.data(), n, threshs.data(), intervals.data(), nullptr, parse_mode(mode),
+                                       status.data());
+            }
+            for (size_t k = 0; k < n; ++k) check(status[k], ctxs[k]);
+        },
+        "screeners"_a, "conj_threshs"_a, "conj_det_intervals"_a, "mode"_a = "fp64",
+        "Screen the polyjectories resident in several Screener objects together (mzh_screen_batch).");
+
     m.def("conjunctions_batch", &make_conjunctions_batch, "pjs"_a, "conj_threshs"_a, "conj_det_intervals"_a,
           "otypes"_a = py::none(), "mode"_a = "fp64", "device"_a = 0,
           "Screen several independent polyjectories together on one GPU; returns one conjunctions object per "
