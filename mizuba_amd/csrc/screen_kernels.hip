@@ -83,14 +83,14 @@ __device__ inline double pw_max(double *v)
 // FP64 Cargo-Shisha enclosure of one polynomial (contiguous coefficients)
 // over [lb, ub]: translate by lb, scale by (ub - lb)^i, divide by C(n, i),
 // difference table, NaN-propagating pairwise min/max.
-template <int ORDER>
+template <int ORDER, int STRIDE = 1>
 __device__ inline void cs_enclosure_fp64(const double *__restrict__ cf, double lb, double ub, double &emin,
                                          double &emax)
 {
     constexpr int OP1 = ORDER + 1;
     double c[OP1], t[OP1], ap[OP1], dp[OP1];
 #pragma unroll
-    for (int i = 0; i < OP1; ++i) c[i] = cf[i];
+    for (int i = 0; i < OP1; ++i) c[i] = cf[i * STRIDE];
     halving_pows<ORDER>(lb, ap);
 #pragma unroll
     for (int i = 0; i < OP1; ++i) {
@@ -131,7 +131,7 @@ __device__ inline void cs_enclosure_fp64(const double *__restrict__ cf, double l
 // enclosure's own (smaller) errors, so the float interval [b - e, b + e]
 // contains the FP64 Bernstein coefficient. The margin is evaluated from
 // magnitudes accumulated alongside the values and inflated by a factor 2.
-template <int ORDER>
+template <int ORDER, int STRIDE = 1>
 __device__ inline void cs_enclosure_fp32(const double *__restrict__ cf, double lb_d, double ub_d, float &emin,
                                          float &emax)
 {
@@ -143,7 +143,7 @@ __device__ inline void cs_enclosure_fp32(const double *__restrict__ cf, double l
     halving_pows<ORDER>(dl, dp);
     float cfl[OP1];
 #pragma unroll
-    for (int i = 0; i < OP1; ++i) cfl[i] = (float)cf[i];
+    for (int i = 0; i < OP1; ++i) cfl[i] = (float)cf[i * STRIDE];
 #pragma unroll
     for (int i = 0; i < OP1; ++i) {
         float s = cfl[i], ms = fabsf(cfl[i]);
@@ -206,6 +206,13 @@ __global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double ra
     const uint32_t c0 = g * kAabbGroup, c1 = min(sg.n_steps, c0 + kAabbGroup);
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     __shared__ uint32_t s_l[4][4], s_u[4][4];
+    // The lane's current trajectory piece (4 channels x (ORDER + 1)
+    // coefficients) is kept in LDS, lane-minor: a piece spans several
+    // conjunction steps and is read from HBM once. Up to order 8 (72 KB).
+    constexpr int kCf = 4 * (ORDER + 1);
+    constexpr bool kCache = kCf * 256 * 8 <= 80 * 1024;
+    __shared__ double s_cf[kCache ? kCf : 1][256];
+    uint64_t k_cached = ~0ull;
 
     uint64_t ns = 0, kb = 0;
     const double *t = nullptr;
@@ -248,26 +255,36 @@ __global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double ra
                     const double ev_lb = cb > ss ? cb : ss;
                     const double ev_ub = ce < t[k + 1] ? ce : t[k + 1];
                     const double h_lb = ev_lb - ss, h_ub = ev_ub - ss;
-                    const double *cf = cfb + k * 4u * (ORDER + 1);
+                    const double *cf = cfb + k * (uint64_t)kCf;
+                    constexpr int kStride = kCache ? 256 : 1;
+                    if constexpr (kCache) {
+                        if (k != k_cached) {
+#pragma unroll
+                            for (int j = 0; j < kCf; ++j) s_cf[j][threadIdx.x] = cf[j];
+                            k_cached = k;
+                        }
+                        cf = &s_cf[0][threadIdx.x];
+                    }
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
                         float fl, fu;
+                        const double *cfd = cf + d * (ORDER + 1) * kStride;
                         if (mode == MZH_MODE_FP64) {
                             double emin, emax;
-                            cs_enclosure_fp64<ORDER>(cf + d * (ORDER + 1), h_lb, h_ub, emin, emax);
+                            cs_enclosure_fp64<ORDER, kStride>(cfd, h_lb, h_ub, emin, emax);
                             emin = emin - radius;
                             emax = emax + radius;
                             fl = next_down_f((float)emin);
                             fu = next_up_f((float)emax);
                         } else {
                             float emin, emax;
-                            cs_enclosure_fp32<ORDER>(cf + d * (ORDER + 1), h_lb, h_ub, emin, emax);
+                            cs_enclosure_fp32<ORDER, kStride>(cfd, h_lb, h_ub, emin, emax);
                             double dmin = (double)emin, dmax = (double)emax;
                             if (!is_finite_f(emin) || !is_finite_f(emax)) {
                                 // coefficients or intermediates outside the float
                                 // range (ill-conditioned short steps): use the FP64
                                 // enclosure for this piece
-                                cs_enclosure_fp64<ORDER>(cf + d * (ORDER + 1), h_lb, h_ub, dmin, dmax);
+                                cs_enclosure_fp64<ORDER, kStride>(cfd, h_lb, h_ub, dmin, dmax);
                             }
                             // dmin <= FP64 enclosure minimum (as reals); the same
                             // monotone double subtraction and outward float
