@@ -173,6 +173,136 @@ def batch_workload(args):
     print(json.dumps(out))
 
 
+def eval_bytes_per_launch(pj, tgrid):
+    """Algorithmic bytes of one k_state_eval launch over a shared time grid:
+    the 7-channel state written per evaluation (56 B), the grid read once
+    (8 B per time) and the coefficient record of every (object, trajectory
+    step) the grid touches read once (7 x (order + 1) x 8 B); the time arrays
+    searched are counted once per object (8 B per step end)."""
+    traj, time_, traj_off, nsteps, time_off = pj._flat()
+    op1 = pj.poly_order + 1
+    touched = 0
+    searched = 0
+    for o in range(pj.n_objs):
+        ns = int(nsteps[o])
+        if ns == 0:
+            continue
+        t = time_[int(time_off[o]) : int(time_off[o]) + ns + 1]
+        inside = tgrid[(tgrid >= t[0]) & (tgrid < t[-1])]
+        touched += len(np.unique(np.searchsorted(t[1:], inside, side="right")))
+        searched += ns + 1
+    n_eval = pj.n_objs * len(tgrid)
+    return 56 * n_eval + 8 * len(tgrid) + touched * 7 * op1 * 8 + 8 * searched
+
+
+def eval_hbm_traffic():
+    path = os.path.join(ROOT, "profiles", "state_eval_30k_2048_times.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == "k_state_eval":
+            try:
+                return int(float(cells[4]))
+            except ValueError:
+                return None
+    return None
+
+
+def state_eval_workload(args):
+    """Batched polyjectory evaluation (polyjectory::state_meval with one time
+    grid for all objects): the 30 000-object 1-day polyjectory at 2048 times
+    spread over the day, 61 M evaluations of the 7-channel state, times and
+    results resident in HBM (mzh_state_eval_device). The oracle's evaluation
+    is timed on a sample of the objects."""
+    import mizuba_amd as mz
+    from oracle import pyoracle as po
+
+    n_t = 2048
+    pj = mz.make_synthetic_polyjectory(args.n_objs, args.days, seed=args.seed, frac_high=0.2, order=8)
+    N = pj.n_objs
+    tgrid = np.arange(n_t, dtype=np.float64) * (pj.maxT / n_t)
+    scr = mz.Screener(0)
+    scr.upload(pj)
+    # times and results in HBM, in buffers of the context's device
+    d_tm = scr.device_buffer(tgrid.nbytes)
+    d_tm.write(tgrid)
+    d_out = scr.device_buffer(N * n_t * 7 * 8)
+
+    def run():
+        scr.state_eval_device(d_tm.ptr, 2, 0, N, n_t, d_out.ptr)
+
+    for _ in range(args.warmup):
+        run()
+    steps = max(args.steps, 1) * 4
+    kt = []
+    t = time.perf_counter()
+    for _ in range(steps):
+        run()
+        kt.append(scr.eval_time())
+    dt = (time.perf_counter() - t) / steps
+    kavg = sum(kt) / len(kt)
+    n_eval = N * n_t
+    # parity on a sample of rows against the oracle, bit for bit (NaN included)
+    traj, time_, traj_off, nsteps, time_off = pj._flat()
+    flat = po.Flat.from_arrays(pj.poly_order, traj, time_, nsteps, traj_off, time_off, pj.maxT)
+    rng = np.random.default_rng(args.seed)
+    rows = np.sort(rng.choice(N, 64, replace=False)).astype(np.uint64)
+    st, ref, _ = po.state_meval(flat, tgrid, po.TIME_SHARED, sel=rows, n_times=n_t)
+    # the sampled rows again through a selector, and a spot check of the full
+    # run's output for the same rows
+    d_sel = scr.device_buffer(rows.nbytes)
+    d_sel.write(rows)
+    d_smp = scr.device_buffer(len(rows) * n_t * 7 * 8)
+    scr.state_eval_device(d_tm.ptr, 2, d_sel.ptr, len(rows), n_t, d_smp.ptr)
+    got = np.empty((len(rows), n_t, 7))
+    d_smp.read(got)
+    full = np.empty((N, n_t, 7))
+    d_out.read(full)
+    same_rows = np.array_equal(full[rows.astype(np.int64)].view(np.uint64), got.view(np.uint64))
+    del full
+    identical = bool(st == 0 and same_rows and np.array_equal(got.view(np.uint64), ref.view(np.uint64)))
+    alg = eval_bytes_per_launch(pj, tgrid)
+    traffic = eval_hbm_traffic()
+    out = {
+        "metric": "polyjectory state evaluations/s (state_meval, one time grid for all objects)",
+        "value": round(n_eval / dt, 1),
+        "unit": "evaluations/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded J2-secular Keplerian catalog, order-8 Chebyshev-fitted polyjectory)",
+        "config": {"workload": f"{N} objects x {args.days:g} day, {n_t} shared times", "n_objs": N, "n_times": n_t,
+                   "seed": args.seed},
+        "identical_to_oracle_sample": identical,
+        "roofline": {
+            "kernel": "k_state_eval",
+            "bound": "hbm",
+            "achieved": round(alg / kavg / 1e9, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(alg / kavg / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes": int(alg),
+            "avg_launch_ms": round(kavg * 1e3, 4),
+        },
+    }
+    if not args.no_cpu_baseline:
+        n_s = N  # the whole workload: about 2 s of one core
+        t = time.perf_counter()
+        po.state_meval(flat, tgrid, po.TIME_SHARED, sel=np.arange(n_s, dtype=np.uint64), n_times=n_t)
+        dt_cpu = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(n_s * n_t / dt_cpu, 1), "unit": "evaluations/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"first {n_s} objects x {n_t} times, wall {dt_cpu:.2f}s"}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,7 +314,7 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=0, help="conjunction steps timed on the CPU (0: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0, help="only run the PMC copy calibration")
-    ap.add_argument("--workload", choices=("headline", "batch64"), default="headline")
+    ap.add_argument("--workload", choices=("headline", "batch64", "state_eval"), default="headline")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -200,6 +330,9 @@ def main():
         return
     if args.workload == "batch64":
         batch_workload(args)
+        return
+    if args.workload == "state_eval":
+        state_eval_workload(args)
         return
 
     import mizuba_amd as mz

@@ -117,6 +117,46 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
                            const double *time, uint64_t time_len, const uint64_t *traj_off, const uint64_t *nsteps,
                            const uint64_t *time_off, double maxT);
 
+/* ---- evaluation of the uploaded polyjectory ----------------------------- */
+
+/* Where the evaluation time of row r (object sel[r], or r) and column k comes
+ * from in mzh_state_eval: */
+#define MZH_EVAL_TIME_SCALAR 0     /* tm[0] for every evaluation: state_eval(t) */
+#define MZH_EVAL_TIME_PER_OBJECT 1 /* tm[r]: state_eval(t[n_rows]) */
+#define MZH_EVAL_TIME_SHARED 2     /* tm[k], the same times for every row: state_meval(t[n_times]) */
+#define MZH_EVAL_TIME_PER_EVAL 3   /* tm[r * n_times + k]: state_meval(t[n_rows][n_times]) */
+
+/* Evaluate the uploaded polyjectory: out[r][k][0..6] = (x, y, z, vx, vy, vz, r)
+ * of object sel[r] (object r when sel is NULL, and then n_rows must be n_objs)
+ * at the time given by time_mode. Replaces polyjectory::state_eval and
+ * polyjectory::state_meval (mizuba/polyjectory.hpp:184-204,
+ * mizuba/polyjectory.cpp:1094-1366): Horner over the trajectory step whose end
+ * time is the first one after t; NaN where the object has no data or t lies
+ * outside [t_begin, t_end). n_times must be 1 for the SCALAR and PER_OBJECT
+ * modes. Errors (the reference's messages, reported for the first failing
+ * evaluation in (r, k) order): a non-finite time -> MZH_EINVAL, an object
+ * index >= n_objs -> MZH_ERANGE.
+ * mzh_state_eval takes host pointers (copied through HBM);
+ * mzh_state_eval_device takes device pointers and leaves out in HBM. Both
+ * return once the results are in out. */
+int mzh_state_eval(mzh_ctx *ctx, const double *tm, int time_mode, const uint64_t *sel, uint64_t n_rows,
+                   uint64_t n_times, double *out);
+int mzh_state_eval_device(mzh_ctx *ctx, const double *tm, int time_mode, const uint64_t *sel, uint64_t n_rows,
+                          uint64_t n_times, double *out);
+/* Device memory on the context's device, for callers of the device-pointer
+ * entry points that have no HIP allocator of their own. No reference
+ * counterpart. mzh_copy_to_device / mzh_copy_from_device copy between host
+ * and such memory and return once the copy is complete. */
+int mzh_device_alloc(mzh_ctx *ctx, uint64_t bytes, void **ptr);
+int mzh_device_free(mzh_ctx *ctx, void *ptr);
+int mzh_copy_to_device(mzh_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int mzh_copy_from_device(mzh_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* Device time (seconds, HIP events on the context's stream) of the evaluation
+ * kernel of the last mzh_state_eval / mzh_state_eval_device call. No
+ * reference counterpart: measurement only. */
+int mzh_get_eval_time(const mzh_ctx *ctx, double *seconds);
+
 /* ---- the hot path ------------------------------------------------------ */
 
 /* Run conjunction screening over all conjunction steps. Replaces

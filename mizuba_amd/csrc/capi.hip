@@ -17,6 +17,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <limits>
 #include <vector>
 
 #include "../../include/mizuba_amd.h"
@@ -139,6 +140,10 @@ struct mzh_ctx {
     DevBuf cand, cand_count, steps, steps_count;
     uint64_t cand_cap = 0, steps_cap = 0;
     DevBuf err;
+    // polyjectory evaluation: staged times, selector and results, first bad element
+    DevBuf ev_tm, ev_sel, ev_out, ev_bad;
+    hipEvent_t ev_eval[2] = {};
+    double eval_s = 0;  // device time of the last evaluation kernel
     hipEvent_t ev[12] = {};
     double stage_s[MZH_STAGE_COUNT] = {};
     uint64_t h_np[MZH_NP_COUNT] = {0, 0, 0, 0, 0, 0, 0};
@@ -625,6 +630,7 @@ int mzh_create(int device, mzh_ctx **out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     for (int k = 0; k < 12 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev[k]);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev_eval[k]);
     if (e != hipSuccess) {
         g_global_error = std::string("HIP error while creating a context: ") + hipGetErrorString(e);
         delete ctx;
@@ -640,6 +646,8 @@ int mzh_destroy(mzh_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->ev_eval)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -701,6 +709,146 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         ctx->n_steps_total = steps_total;
         ctx->has_pj = true;
     });
+}
+
+namespace
+{
+
+// polyjectory::state_eval / state_meval (polyjectory.cpp:1094-1366).
+void state_eval_impl(mzh_ctx *ctx, const double *tm, int time_mode, const uint64_t *sel, uint64_t n_rows,
+                     uint64_t n_times, double *out, bool on_device)
+{
+    if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
+    if (time_mode < MZH_EVAL_TIME_SCALAR || time_mode > MZH_EVAL_TIME_PER_EVAL)
+        throw StatusError{MZH_EINVAL, "Invalid time mode " + std::to_string(time_mode) + " passed to state_eval()"};
+    if ((time_mode == MZH_EVAL_TIME_SCALAR || time_mode == MZH_EVAL_TIME_PER_OBJECT) && n_times != 1)
+        throw StatusError{MZH_EINVAL, "state_eval() evaluates one time per object, but " + std::to_string(n_times)
+                                          + " time evaluations per object were requested"};
+    if (!sel && n_rows != ctx->n_objs)
+        throw StatusError{MZH_EINVAL, "Invalid output array passed to state_eval(): the number of objects is "
+                                          + std::to_string(ctx->n_objs)
+                                          + " but the size of the first dimension of the array is "
+                                          + std::to_string(n_rows) + " (the two numbers must be equal)"};
+    if (n_times && n_rows > std::numeric_limits<uint64_t>::max() / 7u / n_times)
+        throw StatusError{MZH_EINVAL, "Too many evaluations requested from state_eval()"};
+    const uint64_t n_eval = n_rows * n_times;
+    if (n_eval == 0) return;
+    if (!tm || !out) throw StatusError{MZH_EINVAL, "Null pointer passed to state_eval()"};
+    const uint64_t n_tm = time_mode == MZH_EVAL_TIME_SCALAR       ? 1u
+                          : time_mode == MZH_EVAL_TIME_PER_OBJECT ? n_rows
+                          : time_mode == MZH_EVAL_TIME_SHARED     ? n_times
+                                                                  : n_eval;
+    hipStream_t st = ctx->stream;
+    const double *d_tm = tm;
+    const uint64_t *d_sel = sel;
+    double *d_out = out;
+    if (!on_device) {
+        MZH_HIP(ctx->ev_tm.ensure(sizeof(double) * n_tm));
+        MZH_HIP(ctx->ev_out.ensure(sizeof(double) * 7u * n_eval));
+        MZH_HIP(hipMemcpyAsync(ctx->ev_tm.p, tm, sizeof(double) * n_tm, hipMemcpyHostToDevice, st));
+        d_tm = ctx->ev_tm.as<const double>();
+        d_out = ctx->ev_out.as<double>();
+        if (sel) {
+            MZH_HIP(ctx->ev_sel.ensure(sizeof(uint64_t) * n_rows));
+            MZH_HIP(hipMemcpyAsync(ctx->ev_sel.p, sel, sizeof(uint64_t) * n_rows, hipMemcpyHostToDevice, st));
+            d_sel = ctx->ev_sel.as<const uint64_t>();
+        }
+    }
+    MZH_HIP(ctx->ev_bad.ensure(sizeof(unsigned long long)));
+    MZH_HIP(hipMemsetAsync(ctx->ev_bad.p, 0xff, sizeof(unsigned long long), st));
+    MZH_HIP(hipEventRecord(ctx->ev_eval[0], st));
+    MZH_HIP(launch_state_eval(pj_view(ctx), d_tm, time_mode, d_sel, n_rows, n_times, d_out,
+                              ctx->ev_bad.as<unsigned long long>(), st));
+    MZH_HIP(hipEventRecord(ctx->ev_eval[1], st));
+    unsigned long long bad = 0;
+    MZH_HIP(hipMemcpyAsync(&bad, ctx->ev_bad.p, sizeof(bad), hipMemcpyDeviceToHost, st));
+    MZH_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    MZH_HIP(hipEventElapsedTime(&ms, ctx->ev_eval[0], ctx->ev_eval[1]));
+    ctx->eval_s = ms * 1e-3;
+    if (bad != ~0ull) {
+        // the first failing evaluation: its time is checked before its index
+        const uint64_t r = bad / n_times, k = bad - r * n_times;
+        const uint64_t ti = time_mode == MZH_EVAL_TIME_SCALAR       ? 0u
+                            : time_mode == MZH_EVAL_TIME_PER_OBJECT ? r
+                            : time_mode == MZH_EVAL_TIME_SHARED     ? k
+                                                                    : bad;
+        double t = 0;
+        uint64_t obj = r;
+        if (on_device) {
+            MZH_HIP(hipMemcpy(&t, tm + ti, sizeof(double), hipMemcpyDeviceToHost));
+            if (sel) MZH_HIP(hipMemcpy(&obj, sel + r, sizeof(uint64_t), hipMemcpyDeviceToHost));
+        } else {
+            t = tm[ti];
+            if (sel) obj = sel[r];
+        }
+        if (!std::isfinite(t))
+            throw StatusError{MZH_EINVAL, "An non-finite evaluation time of " + fmt_double(t)
+                                              + " was detected during the evaluation of a polyjectory"};
+        throw StatusError{MZH_ERANGE, "Invalid object index " + std::to_string(obj)
+                                          + " specified - the total number of objects in the polyjectory is only "
+                                          + std::to_string(ctx->n_objs)};
+    }
+    if (!on_device)
+        MZH_HIP(hipMemcpy(out, ctx->ev_out.p, sizeof(double) * 7u * n_eval, hipMemcpyDeviceToHost));
+}
+
+} // namespace
+
+int mzh_state_eval(mzh_ctx *ctx, const double *tm, int time_mode, const uint64_t *sel, uint64_t n_rows,
+                   uint64_t n_times, double *out)
+{
+    return guarded(ctx, [&] { state_eval_impl(ctx, tm, time_mode, sel, n_rows, n_times, out, false); });
+}
+
+int mzh_state_eval_device(mzh_ctx *ctx, const double *tm, int time_mode, const uint64_t *sel, uint64_t n_rows,
+                          uint64_t n_times, double *out)
+{
+    return guarded(ctx, [&] { state_eval_impl(ctx, tm, time_mode, sel, n_rows, n_times, out, true); });
+}
+
+int mzh_device_alloc(mzh_ctx *ctx, uint64_t bytes, void **ptr)
+{
+    if (!ptr) return MZH_EINVAL;
+    *ptr = nullptr;
+    return guarded(ctx, [&] { MZH_HIP(hipMalloc(ptr, bytes ? bytes : 1)); });
+}
+
+int mzh_device_free(mzh_ctx *ctx, void *ptr)
+{
+    return guarded(ctx, [&] {
+        if (ptr) {
+            MZH_HIP(hipStreamSynchronize(ctx->stream));
+            MZH_HIP(hipFree(ptr));
+        }
+    });
+}
+
+int mzh_copy_to_device(mzh_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    return guarded(ctx, [&] {
+        if (!bytes) return;
+        if (!dst || !src) throw StatusError{MZH_EINVAL, "Null pointer passed to mzh_copy_to_device"};
+        MZH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        MZH_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mzh_copy_from_device(mzh_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    return guarded(ctx, [&] {
+        if (!bytes) return;
+        if (!dst || !src) throw StatusError{MZH_EINVAL, "Null pointer passed to mzh_copy_from_device"};
+        MZH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        MZH_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mzh_get_eval_time(const mzh_ctx *ctx, double *seconds)
+{
+    if (!ctx || !seconds) return MZH_EINVAL;
+    *seconds = ctx->eval_s;
+    return MZH_OK;
 }
 
 int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode)

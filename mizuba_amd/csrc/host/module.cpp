@@ -18,6 +18,7 @@
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <variant>
 #include <vector>
 
 #include "../../../include/mizuba_amd.h"
@@ -95,12 +96,27 @@ py::array ro_view_dtype(const py::object &base, const py::dtype &dt, const void 
 // polyjectory: minimal in-memory equivalent of mizuba::polyjectory
 // ---------------------------------------------------------------------------
 
+// Device copy of a polyjectory for state_eval / state_meval, made on the
+// first evaluation and kept for the polyjectory's lifetime.
+struct PjDevice {
+    mzh_ctx *ctx = nullptr;
+    PjDevice() = default;
+    PjDevice(const PjDevice &) = delete;
+    PjDevice &operator=(const PjDevice &) = delete;
+    ~PjDevice()
+    {
+        if (ctx) mzh_destroy(ctx);
+    }
+};
+
 struct PjData {
     uint32_t n_objs = 0, order = 0;
     double maxT = 0, epoch = 0, epoch2 = 0;
     std::vector<double> traj, time;
     std::vector<uint64_t> traj_off, nsteps, time_off, time_len;
     std::vector<int32_t> status;
+    std::mutex eval_m;
+    std::unique_ptr<PjDevice> eval_dev;
 };
 
 struct Polyjectory {
@@ -299,6 +315,242 @@ struct Ctx {
     }
 };
 
+// ---------------------------------------------------------------------------
+// polyjectory evaluation (polyjectory.cpp:1094-1366, expose_polyjectory.cpp:60-95, 308-520)
+// ---------------------------------------------------------------------------
+
+// Evaluations run on the GPU through mzh_state_eval, on a device copy of the
+// polyjectory made on first use (device 0).
+void pj_state_eval(PjData &d, const double *tm, int mode, const uint64_t *sel, uint64_t n_rows, uint64_t n_times,
+                   double *out)
+{
+    if (n_rows == 0 || n_times == 0) return;
+    py::gil_scoped_release rel;
+    std::lock_guard<std::mutex> lk(d.eval_m);
+    if (!d.eval_dev) {
+        auto dev = std::make_unique<PjDevice>();
+        check(mzh_create(0, &dev->ctx), nullptr);
+        check(mzh_upload_polyjectory(dev->ctx, d.n_objs, d.order, d.traj.data(), d.traj.size(), d.time.data(),
+                                     d.time.size(), d.traj_off.data(), d.nsteps.data(), d.time_off.data(), d.maxT),
+              dev->ctx);
+        d.eval_dev = std::move(dev);
+    }
+    check(mzh_state_eval(d.eval_dev->ctx, tm, mode, sel, n_rows, n_times, out), d.eval_dev->ctx);
+}
+
+void check_cc_aligned(const py::array &a, const char *msg)
+{
+    const py::object fl = a.attr("flags");
+    if (!fl.attr("c_contiguous").cast<bool>() || !fl.attr("aligned").cast<bool>()) throw std::invalid_argument(msg);
+}
+
+bool may_share_memory(const py::array &a, const py::array &b)
+{
+    return numpy().attr("may_share_memory")(a, b).cast<bool>();
+}
+
+using EvalSelector = std::optional<std::variant<std::size_t, py::array_t<std::size_t>>>;
+
+// The selector argument (expose_polyjectory.cpp:66-95): one index or a 1-D array.
+struct SelView {
+    bool present = false;
+    uint64_t single = 0;
+    const uint64_t *ptr = nullptr;
+    uint64_t n = 0;
+    const py::array *arr = nullptr;
+};
+
+SelView setup_selector(const EvalSelector &selector)
+{
+    SelView s;
+    if (!selector) return s;
+    s.present = true;
+    if (const auto *idx = std::get_if<std::size_t>(&*selector)) {
+        s.single = *idx;
+        s.ptr = &s.single;
+        s.n = 1;
+        return s;
+    }
+    const auto &v = std::get<py::array_t<std::size_t>>(*selector);
+    check_cc_aligned(v, "The selector array passed to state_eval() must be C contiguous and properly aligned");
+    if (v.ndim() != 1)
+        throw std::invalid_argument("The selector array passed to state_eval() must have 1 dimension, but the number "
+                                    "of dimensions is "
+                                    + std::to_string(v.ndim()) + " instead");
+    s.ptr = reinterpret_cast<const uint64_t *>(v.data());
+    s.n = (uint64_t)v.shape(0);
+    s.arr = &v;
+    return s;
+}
+
+py::array_t<double> state_eval(const py::object &self, const std::variant<double, py::array_t<double>> &tm,
+                               const std::optional<py::array_t<double>> &out_, const EvalSelector &selector)
+{
+    auto &d = *self.cast<const Polyjectory &>().d;
+    SelView sel = setup_selector(selector);
+    if (sel.present && !sel.arr) sel.ptr = &sel.single;
+    const auto *tm_arr = std::get_if<py::array_t<double>>(&tm);
+    py::array_t<double> out;
+    if (out_) {
+        out = *out_;
+        check_cc_aligned(out, "The output array passed to state_eval() must be C contiguous and properly aligned");
+        if (out.ndim() != 2)
+            throw std::invalid_argument("The output array passed to state_eval() must have 2 dimensions, but the "
+                                        "number of dimensions is "
+                                        + std::to_string(out.ndim()) + " instead");
+        if (out.shape(1) != 7)
+            throw std::invalid_argument("The output array passed to state_eval() must have a size of 7 in the second "
+                                        "dimension, but the size in the second dimension is "
+                                        + std::to_string(out.shape(1)) + " instead");
+        if (tm_arr && may_share_memory(out, *tm_arr))
+            throw std::invalid_argument(
+                "Potential memory overlap detected between the output array passed to state_eval() and the time array");
+        if (sel.arr && may_share_memory(out, *sel.arr))
+            throw std::invalid_argument("Potential memory overlap detected between the output array passed to "
+                                        "state_eval() and the array of object indices");
+    } else {
+        out = py::array_t<double>(std::vector<py::ssize_t>{(py::ssize_t)(sel.present ? sel.n : d.n_objs), 7});
+    }
+    int mode = MZH_EVAL_TIME_SCALAR;
+    const double *tm_ptr = std::get_if<double>(&tm);
+    uint64_t n_tm = 1;
+    if (tm_arr) {
+        check_cc_aligned(*tm_arr, "The time array passed to state_eval() must be C contiguous and properly aligned");
+        if (tm_arr->ndim() != 1)
+            throw std::invalid_argument("The time array passed to state_eval() must have 1 dimension, but the number "
+                                        "of dimensions is "
+                                        + std::to_string(tm_arr->ndim()) + " instead");
+        mode = MZH_EVAL_TIME_PER_OBJECT;
+        tm_ptr = tm_arr->data();
+        n_tm = (uint64_t)tm_arr->shape(0);
+    }
+    // polyjectory.cpp:1158-1225
+    const uint64_t n_out = (uint64_t)out.shape(0);
+    if (sel.present) {
+        if (n_out != sel.n)
+            throw std::invalid_argument(
+                "Invalid output array passed to state_eval(): the number of objects selected for evaluation is "
+                + std::to_string(sel.n) + " but the size of the first dimension of the array is "
+                + std::to_string(n_out) + " (the two numbers must be equal)");
+        if (tm_arr && n_tm != sel.n)
+            throw std::invalid_argument("Invalid time array passed to state_eval(): the number of selected objects is "
+                                        + std::to_string(sel.n) + " but the size of the array is "
+                                        + std::to_string(n_tm) + " (the two numbers must be equal)");
+    } else {
+        if (n_out != d.n_objs)
+            throw std::invalid_argument("Invalid output array passed to state_eval(): the number of objects is "
+                                        + std::to_string(d.n_objs)
+                                        + " but the size of the first dimension of the array is "
+                                        + std::to_string(n_out) + " (the two numbers must be equal)");
+        if (tm_arr && n_tm != d.n_objs)
+            throw std::invalid_argument("Invalid time array passed to state_eval(): the number of objects is "
+                                        + std::to_string(d.n_objs) + " but the size of the array is "
+                                        + std::to_string(n_tm) + " (the two numbers must be equal)");
+    }
+    pj_state_eval(d, tm_ptr, mode, sel.present ? sel.ptr : nullptr, n_out, 1, out.mutable_data());
+    return out;
+}
+
+py::array_t<double> state_meval(const py::object &self, const py::array_t<double> &tm,
+                                const std::optional<py::array_t<double>> &out_, const EvalSelector &selector)
+{
+    auto &d = *self.cast<const Polyjectory &>().d;
+    SelView sel = setup_selector(selector);
+    if (sel.present && !sel.arr) sel.ptr = &sel.single;
+    check_cc_aligned(tm, "The time array passed to state_meval() must be C contiguous and properly aligned");
+    if (tm.ndim() != 1 && tm.ndim() != 2)
+        throw std::invalid_argument("The time array passed to state_meval() must have either 1 or 2 dimensions, but "
+                                    "the number of dimensions is "
+                                    + std::to_string(tm.ndim()) + " instead");
+    const uint64_t n_time_evals = (uint64_t)(tm.ndim() == 1 ? tm.shape(0) : tm.shape(1));
+    py::array_t<double> out;
+    if (out_) {
+        out = *out_;
+        check_cc_aligned(out, "The output array passed to state_meval() must be C contiguous and properly aligned");
+        if (out.ndim() != 3)
+            throw std::invalid_argument("The output array passed to state_meval() must have 3 dimensions, but the "
+                                        "number of dimensions is "
+                                        + std::to_string(out.ndim()) + " instead");
+        if (out.shape(2) != 7)
+            throw std::invalid_argument("The output array passed to state_meval() must have a size of 7 in the third "
+                                        "dimension, but the size in the third dimension is "
+                                        + std::to_string(out.shape(2)) + " instead");
+        if (may_share_memory(out, tm))
+            throw std::invalid_argument("Potential memory overlap detected between the output array passed to "
+                                        "state_meval() and the time array");
+        if (sel.arr && may_share_memory(out, *sel.arr))
+            throw std::invalid_argument("Potential memory overlap detected between the output array passed to "
+                                        "state_meval() and the array of object indices");
+    } else {
+        out = py::array_t<double>(std::vector<py::ssize_t>{(py::ssize_t)(sel.present ? sel.n : d.n_objs),
+                                                           (py::ssize_t)n_time_evals, 7});
+    }
+    // polyjectory.cpp:1255-1327
+    const uint64_t n_out = (uint64_t)out.shape(0);
+    if ((uint64_t)out.shape(1) != n_time_evals)
+        throw std::invalid_argument("Invalid output array passed to state_meval(): the number of time evaluations per "
+                                    "object is "
+                                    + std::to_string(n_time_evals)
+                                    + " but the size of the second dimension of the array is "
+                                    + std::to_string(out.shape(1)) + " (the two numbers must be equal)");
+    if (sel.present) {
+        if (n_out != sel.n)
+            throw std::invalid_argument("Invalid output array passed to state_meval(): the number of selected objects "
+                                        "is "
+                                        + std::to_string(sel.n) + " but the size of the first dimension of the array is "
+                                        + std::to_string(n_out) + " (the two numbers must be equal)");
+        if (tm.ndim() == 2 && (uint64_t)tm.shape(0) != sel.n)
+            throw std::invalid_argument("Invalid time array passed to state_meval(): the number of selected objects "
+                                        "is "
+                                        + std::to_string(sel.n) + " but the size of the first dimension of the array is "
+                                        + std::to_string(tm.shape(0)) + " (the two numbers must be equal)");
+    } else {
+        if (n_out != d.n_objs)
+            throw std::invalid_argument("Invalid output array passed to state_meval(): the number of objects is "
+                                        + std::to_string(d.n_objs)
+                                        + " but the size of the first dimension of the array is "
+                                        + std::to_string(n_out) + " (the two numbers must be equal)");
+        if (tm.ndim() == 2 && (uint64_t)tm.shape(0) != d.n_objs)
+            throw std::invalid_argument("Invalid time array passed to state_meval(): the number of objects is "
+                                        + std::to_string(d.n_objs)
+                                        + " but the size of the first dimension of the array is "
+                                        + std::to_string(tm.shape(0)) + " (the two numbers must be equal)");
+    }
+    pj_state_eval(d, tm.data(), tm.ndim() == 1 ? MZH_EVAL_TIME_SHARED : MZH_EVAL_TIME_PER_EVAL,
+                  sel.present ? sel.ptr : nullptr, n_out, n_time_evals, out.mutable_data());
+    return out;
+}
+
+// Device memory of a Screener's context (mzh_device_alloc), for the
+// device-pointer entry points; freed with the object.
+struct DeviceBuffer {
+    std::shared_ptr<Ctx> owner;
+    void *p = nullptr;
+    uint64_t nbytes = 0;
+    DeviceBuffer(std::shared_ptr<Ctx> c, uint64_t n) : owner(std::move(c)), nbytes(n)
+    {
+        check(mzh_device_alloc(owner->ctx, n, &p), owner->ctx);
+    }
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer()
+    {
+        if (p) mzh_device_free(owner->ctx, p);
+    }
+    void write(const py::array &a)
+    {
+        if (!(a.flags() & py::array::c_style) || (uint64_t)a.nbytes() > nbytes)
+            throw std::invalid_argument("write() needs a C-contiguous array of at most the buffer's size");
+        check(mzh_copy_to_device(owner->ctx, p, a.data(), (uint64_t)a.nbytes()), owner->ctx);
+    }
+    void read(py::array &a)
+    {
+        if (!(a.flags() & py::array::c_style) || !a.writeable() || (uint64_t)a.nbytes() > nbytes)
+            throw std::invalid_argument("read() needs a writeable C-contiguous array of at most the buffer's size");
+        check(mzh_copy_from_device(owner->ctx, a.mutable_data(), p, (uint64_t)a.nbytes()), owner->ctx);
+    }
+};
+
 int parse_mode(const std::string &m)
 {
     if (m == "fp64") return MZH_MODE_FP64;
@@ -483,6 +735,8 @@ PYBIND11_MODULE(_core, m)
         auto tm = ro_view(self, p.d->time.data() + p.d->time_off[i], {(py::ssize_t)(ns + (ns != 0 ? 1 : 0))});
         return py::make_tuple(tr, tm, p.d->status[i]);
     });
+    pt.def("state_eval", &state_eval, "time"_a, "out"_a.noconvert() = py::none(), "obj_idx"_a = py::none());
+    pt.def("state_meval", &state_meval, "time"_a, "out"_a.noconvert() = py::none(), "obj_idx"_a = py::none());
     pt.def("_flat", [](const py::object &self) {
         // flattened arrays (traj, time, traj_off, nsteps, time_off), for tests and tools
         const auto &p = self.cast<const Polyjectory &>();
@@ -516,7 +770,13 @@ PYBIND11_MODULE(_core, m)
         "n_objs"_a, "days"_a, "seed"_a = 42, "frac_high"_a = 0.2, "order"_a = 8, "n_shell"_a = 0);
 
     // ---- device context for repeated screening of one polyjectory ------
-    py::class_<Ctx>(m, "Screener")
+    py::class_<DeviceBuffer, std::shared_ptr<DeviceBuffer>>(m, "DeviceBuffer")
+        .def_property_readonly("ptr", [](const DeviceBuffer &b) { return reinterpret_cast<uintptr_t>(b.p); })
+        .def_property_readonly("nbytes", [](const DeviceBuffer &b) { return b.nbytes; })
+        .def("write", &DeviceBuffer::write, "array"_a)
+        .def("read", &DeviceBuffer::read, "array"_a);
+
+    py::class_<Ctx, std::shared_ptr<Ctx>>(m, "Screener")
         .def(py::init<int>(), "device"_a = 0)
         .def("upload", [](Ctx &c, const Polyjectory &p) { c.upload(p.d); })
         .def(
@@ -560,6 +820,30 @@ PYBIND11_MODULE(_core, m)
                  std::vector<uint64_t> off(2 * C);
                  check(mzh_copy_bp_offsets(c.ctx, off.data()), c.ctx);
                  return C ? off[2 * (C - 1)] + off[2 * (C - 1) + 1] : 0;
+             })
+        .def(
+            "state_eval_device",
+            [](Ctx &c, uintptr_t tm, int time_mode, uintptr_t sel, uint64_t n_rows, uint64_t n_times, uintptr_t out) {
+                // device pointers (e.g. torch tensors' data_ptr()); sel may be 0
+                if (!c.pj) throw std::invalid_argument("No polyjectory has been uploaded");
+                int st;
+                {
+                    py::gil_scoped_release rel;
+                    st = mzh_state_eval_device(c.ctx, reinterpret_cast<const double *>(tm), time_mode,
+                                               reinterpret_cast<const uint64_t *>(sel), n_rows, n_times,
+                                               reinterpret_cast<double *>(out));
+                }
+                check(st, c.ctx);
+            },
+            "tm"_a, "time_mode"_a, "sel"_a, "n_rows"_a, "n_times"_a, "out"_a)
+        .def("device_buffer",
+             [](const std::shared_ptr<Ctx> &c, uint64_t nbytes) { return std::make_shared<DeviceBuffer>(c, nbytes); },
+             "nbytes"_a)
+        .def("eval_time",
+             [](Ctx &c) {
+                 double s = 0;
+                 check(mzh_get_eval_time(c.ctx, &s), c.ctx);
+                 return s;
              })
         .def("conjunctions", [](Ctx &c) {
             uint64_t n;

@@ -110,4 +110,12 @@ hipError_t launch_conj_order(const mzh_conj *in, const uint32_t *in_step, uint64
                              const uint64_t *cnt, uint32_t *cursor, uint32_t *pos, uint32_t *inv, mzh_conj *out,
                              uint32_t n_steps, hipStream_t stream);
 
+// Polyjectory evaluation: out[r][k][7] for n_rows x n_times evaluations,
+// times and rows as in mzh_state_eval (time_mode = MZH_EVAL_TIME_*). The
+// smallest flat index r * n_times + k of an evaluation with a non-finite time
+// or an object index >= n_objs is atomicMin-ed into first_bad.
+hipError_t launch_state_eval(const PjView &pj, const double *tm, int time_mode, const uint64_t *sel,
+                             uint64_t n_rows, uint64_t n_times, double *out, unsigned long long *first_bad,
+                             hipStream_t stream);
+
 } // namespace mzh

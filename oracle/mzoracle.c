@@ -1532,3 +1532,75 @@ int mzo_conjunctions(const mzo_pj *pj, double conj_thresh, double conj_det_inter
     out->stage_seconds[4] = st_narrow;
     return out->status;
 }
+
+/* ------------------------------------------------------------------------ */
+/* polyjectory evaluation (polyjectory.cpp:1094-1366)                        */
+/* ------------------------------------------------------------------------ */
+
+int mzo_state_eval(const mzo_pj *pj, uint64_t obj, double tm, double out[7])
+{
+    /* polyjectory.cpp:1097-1100: the time is checked first */
+    if (!isfinite(tm)) return MZO_EINVAL;
+    /* polyjectory::operator[] (:1009-1014) */
+    if (obj >= pj->n_objs) return MZO_ERANGE;
+    const uint64_t nsteps = pj->nsteps[obj];
+    const double *t = pj->time + pj->time_off[obj];
+    /* no data, before the start or at/after the end: NaN (:1108-1120) */
+    if (nsteps == 0 || tm < t[0] || tm >= t[nsteps]) {
+        for (int ch = 0; ch < 7; ++ch) out[ch] = NAN;
+        return MZO_OK;
+    }
+    /* first end time > tm among t[1..nsteps] (upper_bound, :1136-1138) */
+    uint64_t lo = 1, hi = nsteps;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (t[mid] > tm) {
+            hi = mid;
+        } else {
+            lo = mid + 1;
+        }
+    }
+    const uint64_t step = lo - 1;
+    const double h = tm - t[step]; /* :1141 */
+    const uint32_t order = pj->order;
+    const double *c = pj->traj + pj->traj_off[obj] + step * (uint64_t)(order + 1) * 7u;
+    /* horner_eval with stride 7 (detail/poly_utils.hpp:41-51) */
+    for (int ch = 0; ch < 7; ++ch) {
+        double r = c[(uint64_t)order * 7u + ch];
+        for (uint32_t o = 1; o <= order; ++o) r = c[(uint64_t)(order - o) * 7u + ch] + r * h;
+        out[ch] = r;
+    }
+    return MZO_OK;
+}
+
+int mzo_state_meval(const mzo_pj *pj, const double *tm, int time_mode, const uint64_t *sel, uint64_t n_rows,
+                    uint64_t n_times, double *out, uint64_t *bad)
+{
+    for (uint64_t r = 0; r < n_rows; ++r) {
+        const uint64_t obj = sel ? sel[r] : r;
+        for (uint64_t k = 0; k < n_times; ++k) {
+            const uint64_t e = r * n_times + k;
+            double t;
+            switch (time_mode) {
+                case MZO_TIME_SCALAR:
+                    t = tm[0];
+                    break;
+                case MZO_TIME_PER_OBJECT:
+                    t = tm[r];
+                    break;
+                case MZO_TIME_SHARED:
+                    t = tm[k];
+                    break;
+                default:
+                    t = tm[e];
+                    break;
+            }
+            const int st = mzo_state_eval(pj, obj, t, out + e * 7u);
+            if (st != MZO_OK) {
+                if (bad) *bad = e;
+                return st;
+            }
+        }
+    }
+    return MZO_OK;
+}

@@ -140,6 +140,26 @@ int mzo_root_finding(const double *poly, uint32_t order, double rf_int, int dire
  * returns 0 on success (poly_utils.cpp:213-268). */
 int mzo_bracketed_root(const double *poly, uint32_t order, double lb, double ub, double *root, uint32_t *n_iter);
 
+/* ---- polyjectory evaluation (polyjectory.cpp:1094-1366) ---- */
+
+/* State of object `obj` at time tm (pj_eval_obj_state, polyjectory.cpp:1094-1149):
+ * the 7 channels by Horner over the trajectory step holding tm (the first step
+ * whose end time is > tm). NaN if the object has no data or tm is outside
+ * [t_0, t_nsteps). Returns MZO_EINVAL for a non-finite tm, MZO_ERANGE for
+ * obj >= n_objs (the out_of_range of polyjectory::operator[], :1009-1014). */
+int mzo_state_eval(const mzo_pj *pj, uint64_t obj, double tm, double out[7]);
+
+/* Time arguments of mzo_state_meval. */
+enum { MZO_TIME_SCALAR = 0, MZO_TIME_PER_OBJECT = 1, MZO_TIME_SHARED = 2, MZO_TIME_PER_EVAL = 3 };
+
+/* Batched evaluation (state_eval_impl / state_meval_impl, polyjectory.cpp:1151-1350):
+ * out[r][k][7] = state of object sel[r] (r if sel is NULL) at the time
+ * tm[0] (SCALAR), tm[r] (PER_OBJECT), tm[k] (SHARED) or tm[r * n_times + k]
+ * (PER_EVAL). On error returns the status of the first failing element in
+ * (r, k) order and stores its flat index r * n_times + k in *bad. */
+int mzo_state_meval(const mzo_pj *pj, const double *tm, int time_mode, const uint64_t *sel, uint64_t n_rows,
+                    uint64_t n_times, double *out, uint64_t *bad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -145,6 +145,17 @@ def lib():
         L.mzo_bracketed_root.restype = ct.c_int
         L.mzo_cd_begin_end.argtypes = [ct.c_double, ct.c_uint64, ct.c_double, ct.c_uint64, dp, dp]
         L.mzo_cd_begin_end.restype = ct.c_int
+        L.mzo_state_meval.argtypes = [
+            ct.POINTER(_PJ),
+            dp,
+            ct.c_int,
+            ct.POINTER(ct.c_uint64),
+            ct.c_uint64,
+            ct.c_uint64,
+            dp,
+            ct.POINTER(ct.c_uint64),
+        ]
+        L.mzo_state_meval.restype = ct.c_int
         _lib = L
     return _lib
 
@@ -348,3 +359,35 @@ def cd_begin_end(maxT, k, interval, n):
 
 def n_threads_default() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+TIME_SCALAR, TIME_PER_OBJECT, TIME_SHARED, TIME_PER_EVAL = 0, 1, 2, 3
+
+
+def state_meval(pj, tm, time_mode, sel=None, n_rows=None, n_times=1):
+    """Batched polyjectory evaluation (polyjectory.cpp:1094-1366).
+
+    Returns (status, out[n_rows][n_times][7], flat index of the first failing
+    element or None)."""
+    if not isinstance(pj, Flat):
+        pj = Flat(*pj)
+    tm = np.ascontiguousarray(np.atleast_1d(tm), dtype=np.float64)
+    if sel is not None:
+        sel = np.ascontiguousarray(sel, dtype=np.uint64)
+        n_rows = len(sel)
+    elif n_rows is None:
+        n_rows = pj.n_objs
+    out = np.empty((n_rows, n_times, 7))
+    bad = ct.c_uint64(0)
+    view = pj.view()
+    st = lib().mzo_state_meval(
+        ct.byref(view),
+        _dptr(tm),
+        int(time_mode),
+        sel.ctypes.data_as(ct.POINTER(ct.c_uint64)) if sel is not None else None,
+        n_rows,
+        n_times,
+        _dptr(out),
+        ct.byref(bad),
+    )
+    return st, out, (int(bad.value) if st else None)

@@ -101,3 +101,11 @@ def flat_of(pj):
 
     traj, time_, toff, ns, tmo = pj._flat()
     return po.Flat.from_arrays(pj.poly_order, traj, time_, ns, toff, tmo, pj.maxT)
+
+
+def eval_pj_arrays():
+    """The two-object polyjectory of the reference's test_eval
+    (test/test_polyjectory.py:792-844): x(t) = +-(1 - (t - 0.1)) at unit speed,
+    object 0 over [0, 0.9], object 1 over [1.1, 2.0]."""
+    tm = np.array([round(0.1 * k, 10) for k in range(21)])
+    return [linear_traj(tm[:10], 1), linear_traj(tm[11:], -1)], [tm[:10], tm[11:]]
