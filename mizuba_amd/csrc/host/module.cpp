@@ -7,6 +7,7 @@
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
+#include <pybind11/stl/filesystem.h>
 
 #include <cmath>
 #include <cstdint>
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "../../../include/mizuba_amd.h"
+#include "pj_store.hpp"
 #include "synth.hpp"
 
 namespace py = pybind11;
@@ -66,6 +68,8 @@ py::dtype bvh_node_dtype()
     return py::dtype::from_args(py::eval(
         "[('begin', '<u4'), ('end', '<u4'), ('left', '<i4'), ('right', '<i4'), ('lb', '<f4', (4,)), ('ub', '<f4', (4,))]"));
 }
+
+py::dtype traj_offset_dtype() { return py::dtype::from_args(py::eval("[('offset', '<u8'), ('n_steps', '<u8')]")); }
 
 py::dtype aabb_collision_dtype() { return py::dtype::from_args(py::eval("[('i', '<u4'), ('j', '<u4')]")); }
 
@@ -117,11 +121,97 @@ struct PjData {
     std::vector<int32_t> status;
     std::mutex eval_m;
     std::unique_ptr<PjDevice> eval_dev;
+    // data dir holding the polyjectory's files (empty: in memory only); a
+    // non-persistent one is removed with the data (polyjectory.cpp:245-290)
+    std::filesystem::path data_dir;
+    bool persist = false;
+    PjData() = default;
+    PjData(const PjData &) = delete;
+    PjData &operator=(const PjData &) = delete;
+    ~PjData()
+    {
+        if (!data_dir.empty() && !persist) {
+            std::error_code ec;
+            std::filesystem::remove_all(data_dir, ec);
+        }
+    }
 };
 
 struct Polyjectory {
     std::shared_ptr<PjData> d;
 };
+
+// polyjectory::check_attached (polyjectory.cpp:333-338).
+PjData &attached(const Polyjectory &p)
+{
+    if (!p.d) throw std::invalid_argument("Cannot operate on a detached polyjectory");
+    return *p.d;
+}
+
+std::shared_ptr<PjData> pj_from_arrays(mzh_host::PjArrays &&a)
+{
+    auto d = std::make_shared<PjData>();
+    d->n_objs = a.n_objs;
+    d->order = a.order;
+    d->maxT = a.maxT;
+    d->epoch = a.epoch;
+    d->epoch2 = a.epoch2;
+    d->traj = std::move(a.traj);
+    d->time = std::move(a.time);
+    d->traj_off = std::move(a.traj_off);
+    d->nsteps = std::move(a.nsteps);
+    d->time_off = std::move(a.time_off);
+    d->status = std::move(a.status);
+    d->time_len.resize(d->n_objs);
+    for (uint32_t i = 0; i < d->n_objs; ++i) d->time_len[i] = d->nsteps[i] ? d->nsteps[i] + 1 : 0;
+    return d;
+}
+
+// Write the files of an in-memory polyjectory into a new data dir (the user's
+// data_dir, or a temporary dir in tmpdir), as the reference does for every
+// polyjectory (polyjectory.cpp:389-392, 460-519). The data dir is removed
+// again if writing fails.
+void store_pj(PjData &d, const std::optional<std::filesystem::path> &data_dir,
+              const std::optional<std::filesystem::path> &tmpdir, bool persist)
+{
+    const auto dir = (data_dir && !data_dir->empty()) ? mzh_host::create_dir_0700(*data_dir)
+                                                      : mzh_host::create_temp_dir(tmpdir);
+    try {
+        mzh_host::PjArrays a;
+        a.n_objs = d.n_objs;
+        a.order = d.order;
+        a.maxT = d.maxT;
+        a.epoch = d.epoch;
+        a.epoch2 = d.epoch2;
+        // the file writer reads the arrays: lend them and take them back
+        a.traj.swap(d.traj);
+        a.time.swap(d.time);
+        a.traj_off.swap(d.traj_off);
+        a.nsteps.swap(d.nsteps);
+        a.time_off.swap(d.time_off);
+        a.status.swap(d.status);
+        struct Back {
+            mzh_host::PjArrays &a;
+            PjData &d;
+            ~Back()
+            {
+                a.traj.swap(d.traj);
+                a.time.swap(d.time);
+                a.traj_off.swap(d.traj_off);
+                a.nsteps.swap(d.nsteps);
+                a.time_off.swap(d.time_off);
+                a.status.swap(d.status);
+            }
+        } back{a, d};
+        mzh_host::write_pj_files(dir, a, persist);
+    } catch (...) {
+        std::error_code ec;
+        std::filesystem::remove_all(dir, ec);
+        throw;
+    }
+    d.data_dir = dir;
+    d.persist = persist;
+}
 
 // Validation of polyjectory.cpp:375-447 and :559-599 with its messages.
 std::shared_ptr<PjData> build_pj(const std::vector<py::array_t<double>> &trajs,
@@ -138,10 +228,15 @@ std::shared_ptr<PjData> build_pj(const std::vector<py::array_t<double>> &trajs,
                                         "a size of "
                                         + std::to_string(a.shape(2)) + " was detected");
     }
+    for (const auto &a : trajs)
+        if (!(a.flags() & py::array::c_style))
+            throw std::invalid_argument("All trajectory arrays must be C contiguous and properly aligned");
     for (const auto &a : times) {
         if (a.ndim() != 1)
             throw std::invalid_argument("A time array must have 1 dimension, but instead " + std::to_string(a.ndim())
                                         + " dimension(s) were detected");
+        if (!(a.flags() & py::array::c_style))
+            throw std::invalid_argument("All time arrays must be C contiguous and properly aligned");
     }
     if (n == 0) throw std::invalid_argument("Cannot initialise a polyjectory object from an empty list of trajectories");
     if (n != times.size())
@@ -166,8 +261,9 @@ std::shared_ptr<PjData> build_pj(const std::vector<py::array_t<double>> &trajs,
     if (n > 0xffffffffull) throw std::overflow_error("Too many objects in a polyjectory");
     auto d = std::make_shared<PjData>();
     d->n_objs = (uint32_t)n;
-    d->epoch = epoch;
-    d->epoch2 = epoch2;
+    const auto ep = mzh_host::normalise_epoch(epoch, epoch2);
+    d->epoch = ep.first;
+    d->epoch2 = ep.second;
     d->status = status;
     uint64_t op1 = 0, tot_traj = 0, tot_time = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -386,7 +482,7 @@ SelView setup_selector(const EvalSelector &selector)
 py::array_t<double> state_eval(const py::object &self, const std::variant<double, py::array_t<double>> &tm,
                                const std::optional<py::array_t<double>> &out_, const EvalSelector &selector)
 {
-    auto &d = *self.cast<const Polyjectory &>().d;
+    auto &d = attached(self.cast<const Polyjectory &>());
     SelView sel = setup_selector(selector);
     if (sel.present && !sel.arr) sel.ptr = &sel.single;
     const auto *tm_arr = std::get_if<py::array_t<double>>(&tm);
@@ -454,7 +550,7 @@ py::array_t<double> state_eval(const py::object &self, const std::variant<double
 py::array_t<double> state_meval(const py::object &self, const py::array_t<double> &tm,
                                 const std::optional<py::array_t<double>> &out_, const EvalSelector &selector)
 {
-    auto &d = *self.cast<const Polyjectory &>().d;
+    auto &d = attached(self.cast<const Polyjectory &>());
     SelView sel = setup_selector(selector);
     if (sel.present && !sel.arr) sel.ptr = &sel.single;
     check_cc_aligned(tm, "The time array passed to state_meval() must be C contiguous and properly aligned");
@@ -632,6 +728,7 @@ Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double
         throw std::invalid_argument(
             "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time");
     const int m = parse_mode(mode);
+    (void)attached(pj);
     auto im = std::make_shared<ConjImpl>();
     im->ctx = std::make_unique<Ctx>(device);
     im->ctx->upload(pj.d);
@@ -657,7 +754,7 @@ std::vector<Conjunctions> make_conjunctions_batch(const std::vector<Polyjectory>
     std::vector<mzh_ctx *> ctxs(n);
     std::vector<const int32_t *> ots(n, nullptr);
     for (size_t k = 0; k < n; ++k) {
-        if (otypes) check_otypes_size((*otypes)[k], pjs[k].d->n_objs);
+        if (otypes) check_otypes_size((*otypes)[k], attached(pjs[k]).n_objs);
         ims[k] = std::make_shared<ConjImpl>();
         ims[k]->ctx = std::make_unique<Ctx>(device);
         ims[k]->ctx->upload(pjs[k].d);
@@ -689,63 +786,122 @@ PYBIND11_MODULE(_core, m)
 
     // ---- polyjectory ---------------------------------------------------
     py::class_<Polyjectory> pt(m, "polyjectory", py::dynamic_attr());
+    using OptPath = std::optional<std::filesystem::path>;
     pt.def(py::init([](const py::iterable &trajs_, const py::iterable &times_, const py::iterable &status_, double epoch,
-                       double epoch2, std::optional<py::object> data_dir, bool persist,
-                       std::optional<py::object> tmpdir) {
-               (void)persist;
-               if (data_dir && tmpdir && !data_dir->is_none() && !tmpdir->is_none())
+                       double epoch2, OptPath data_dir, bool persist, OptPath tmpdir) {
+               if (data_dir && tmpdir)
                    throw std::invalid_argument(
                        "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time");
-               auto np = numpy();
-               std::vector<py::array_t<double>> trajs, times;
-               for (auto o : trajs_)
-                   trajs.push_back(np.attr("ascontiguousarray")(py::reinterpret_borrow<py::object>(o), "dtype"_a = "f8")
-                                       .cast<py::array_t<double>>());
-               for (auto o : times_)
-                   times.push_back(np.attr("ascontiguousarray")(py::reinterpret_borrow<py::object>(o), "dtype"_a = "f8")
-                                       .cast<py::array_t<double>>());
-               py::array st_arr = np.attr("ascontiguousarray")(py::reinterpret_borrow<py::object>(status_),
-                                                               "dtype"_a = "i4");
+               // expose_polyjectory.cpp:129-212: arrays are converted only when
+               // their type differs; a float64 or int32 array is checked as it is
+               py::array_t<int32_t> st_arr = status_.cast<py::array_t<int32_t>>();
                if (st_arr.ndim() != 1)
                    throw std::invalid_argument("A status array must have 1 dimension, but instead "
                                                + std::to_string(st_arr.ndim()) + " dimension(s) were detected");
-               std::vector<int32_t> status(st_arr.shape(0));
-               std::memcpy(status.data(), st_arr.data(), sizeof(int32_t) * status.size());
-               return Polyjectory{build_pj(trajs, times, status, epoch, epoch2)};
+               if (!(st_arr.flags() & py::array::c_style))
+                   throw std::invalid_argument("The status array must be C contiguous and properly aligned");
+               std::vector<py::array_t<double>> trajs, times;
+               for (auto o : trajs_) trajs.push_back(o.cast<py::array_t<double>>());
+               for (auto o : times_) times.push_back(o.cast<py::array_t<double>>());
+               std::vector<int32_t> status(st_arr.data(), st_arr.data() + st_arr.shape(0));
+               auto d = build_pj(trajs, times, status, epoch, epoch2);
+               // files are written when a data dir, a temporary dir or persistence
+               // is asked for; otherwise the polyjectory lives in memory only
+               if ((data_dir && !data_dir->empty()) || tmpdir || persist) store_pj(*d, data_dir, tmpdir, persist);
+               return Polyjectory{d};
            }),
            "trajs"_a, "times"_a, "status"_a, "epoch"_a = 0., "epoch2"_a = 0., "data_dir"_a = py::none(),
            "persist"_a = false, "tmpdir"_a = py::none());
-    pt.def_property_readonly("n_objs", [](const Polyjectory &p) { return p.d->n_objs; });
-    pt.def_property_readonly("maxT", [](const Polyjectory &p) { return p.d->maxT; });
-    pt.def_property_readonly("epoch", [](const Polyjectory &p) { return py::make_tuple(p.d->epoch, p.d->epoch2); });
-    pt.def_property_readonly("poly_order", [](const Polyjectory &p) { return p.d->order; });
+    pt.def_static(
+        "from_datafiles",
+        [](const std::filesystem::path &traj_file, const std::filesystem::path &time_file, uint32_t order,
+           const py::object &traj_offsets_, const std::vector<int32_t> &status, double epoch, double epoch2,
+           OptPath data_dir, bool persist, OptPath tmpdir) {
+            // expose_polyjectory.cpp:216-249
+            py::array offs = numpy().attr("asarray")(traj_offsets_, "dtype"_a = traj_offset_dtype());
+            if (offs.ndim() != 1)
+                throw std::invalid_argument("The array of trajectory offsets passed to 'from_datafiles()' must have 1 "
+                                            "dimension, but the number of dimensions is "
+                                            + std::to_string(offs.ndim()) + " instead");
+            offs = numpy().attr("ascontiguousarray")(offs);
+            std::vector<mzh_host::TrajOffset> tv((size_t)offs.shape(0));
+            if (!tv.empty()) std::memcpy(tv.data(), offs.data(), tv.size() * sizeof(mzh_host::TrajOffset));
+            mzh_host::PjArrays a;
+            std::filesystem::path dir;
+            {
+                py::gil_scoped_release rel;
+                dir = mzh_host::pj_from_datafiles(traj_file, time_file, order, tv, status, epoch, epoch2, data_dir,
+                                                  persist, tmpdir, a);
+            }
+            auto d = pj_from_arrays(std::move(a));
+            d->data_dir = dir;
+            d->persist = persist;
+            return Polyjectory{d};
+        },
+        "traj_file"_a, "time_file"_a, "order"_a, "traj_offsets"_a, "status"_a, "epoch"_a = 0., "epoch2"_a = 0.,
+        "data_dir"_a = py::none(), "persist"_a = false, "tmpdir"_a = py::none());
+    pt.def_static(
+        "mount",
+        [](const std::filesystem::path &path) {
+            mzh_host::PjArrays a;
+            std::filesystem::path dir;
+            {
+                py::gil_scoped_release rel;
+                dir = mzh_host::pj_mount(path, a);
+            }
+            auto d = pj_from_arrays(std::move(a));
+            d->data_dir = dir;
+            d->persist = true;  // only persistent data can be mounted
+            return Polyjectory{d};
+        },
+        "data_dir"_a);
+    pt.def_property_readonly_static("traj_offset", [](const py::object &) { return traj_offset_dtype(); });
+    pt.def_property_readonly("n_objs", [](const Polyjectory &p) { return attached(p).n_objs; });
+    pt.def_property_readonly("maxT", [](const Polyjectory &p) { return attached(p).maxT; });
+    pt.def_property_readonly("epoch", [](const Polyjectory &p) {
+        const auto &d = attached(p);
+        return py::make_tuple(d.epoch, d.epoch2);
+    });
+    pt.def_property_readonly("poly_order", [](const Polyjectory &p) { return attached(p).order; });
+    pt.def_property_readonly("data_dir", [](const Polyjectory &p) -> py::object {
+        const auto &d = attached(p);
+        if (d.data_dir.empty()) return py::none();  // in memory only
+        return py::cast(d.data_dir);
+    });
+    pt.def_property_readonly("persist", [](const Polyjectory &p) { return attached(p).persist; });
+    pt.def("detach", [](Polyjectory &p) { p.d.reset(); });
+    pt.def_property_readonly("is_detached", [](const Polyjectory &p) { return !p.d; });
+    pt.def("hint_release", [](const Polyjectory &p) {
+        // the data is held in memory, not in mapped files: nothing to release
+        (void)attached(p);
+    });
     pt.def_property_readonly("status", [](const py::object &self) {
-        const auto &p = self.cast<const Polyjectory &>();
-        return ro_view(self, p.d->status.data(), {(py::ssize_t)p.d->n_objs});
+        const auto &d = attached(self.cast<const Polyjectory &>());
+        return ro_view(self, d.status.data(), {(py::ssize_t)d.n_objs});
     });
     pt.def("__getitem__", [](const py::object &self, py::ssize_t i) {
-        const auto &p = self.cast<const Polyjectory &>();
-        if (i < 0 || (uint64_t)i >= p.d->n_objs)
+        const auto &d = attached(self.cast<const Polyjectory &>());
+        if (i < 0 || (uint64_t)i >= d.n_objs)
             throw std::out_of_range("Invalid object index " + std::to_string(i)
                                     + " specified - the total number of objects in the polyjectory is only "
-                                    + std::to_string(p.d->n_objs));
-        const uint64_t ns = p.d->nsteps[i];
-        const py::ssize_t op1 = p.d->order + 1;
-        auto tr = ro_view(self, p.d->traj.data() + p.d->traj_off[i], {(py::ssize_t)ns, op1, 7});
-        auto tm = ro_view(self, p.d->time.data() + p.d->time_off[i], {(py::ssize_t)(ns + (ns != 0 ? 1 : 0))});
-        return py::make_tuple(tr, tm, p.d->status[i]);
+                                    + std::to_string(d.n_objs));
+        const uint64_t ns = d.nsteps[i];
+        const py::ssize_t op1 = d.order + 1;
+        auto tr = ro_view(self, d.traj.data() + d.traj_off[i], {(py::ssize_t)ns, op1, 7});
+        auto tm = ro_view(self, d.time.data() + d.time_off[i], {(py::ssize_t)(ns + (ns != 0 ? 1 : 0))});
+        return py::make_tuple(tr, tm, d.status[i]);
+    });
+    pt.def("_flat", [](const py::object &self) {
+        // flattened arrays (traj, time, traj_off, nsteps, time_off), for tests and tools
+        const auto &d = attached(self.cast<const Polyjectory &>());
+        return py::make_tuple(ro_view(self, d.traj.data(), {(py::ssize_t)d.traj.size()}),
+                              ro_view(self, d.time.data(), {(py::ssize_t)d.time.size()}),
+                              ro_view(self, d.traj_off.data(), {(py::ssize_t)d.n_objs}),
+                              ro_view(self, d.nsteps.data(), {(py::ssize_t)d.n_objs}),
+                              ro_view(self, d.time_off.data(), {(py::ssize_t)d.n_objs}));
     });
     pt.def("state_eval", &state_eval, "time"_a, "out"_a.noconvert() = py::none(), "obj_idx"_a = py::none());
     pt.def("state_meval", &state_meval, "time"_a, "out"_a.noconvert() = py::none(), "obj_idx"_a = py::none());
-    pt.def("_flat", [](const py::object &self) {
-        // flattened arrays (traj, time, traj_off, nsteps, time_off), for tests and tools
-        const auto &p = self.cast<const Polyjectory &>();
-        return py::make_tuple(ro_view(self, p.d->traj.data(), {(py::ssize_t)p.d->traj.size()}),
-                              ro_view(self, p.d->time.data(), {(py::ssize_t)p.d->time.size()}),
-                              ro_view(self, p.d->traj_off.data(), {(py::ssize_t)p.d->n_objs}),
-                              ro_view(self, p.d->nsteps.data(), {(py::ssize_t)p.d->n_objs}),
-                              ro_view(self, p.d->time_off.data(), {(py::ssize_t)p.d->n_objs}));
-    });
 
     m.def(
         "make_synthetic_polyjectory",
@@ -778,7 +934,11 @@ PYBIND11_MODULE(_core, m)
 
     py::class_<Ctx, std::shared_ptr<Ctx>>(m, "Screener")
         .def(py::init<int>(), "device"_a = 0)
-        .def("upload", [](Ctx &c, const Polyjectory &p) { c.upload(p.d); })
+        .def("upload",
+             [](Ctx &c, const Polyjectory &p) {
+                 (void)attached(p);
+                 c.upload(p.d);
+             })
         .def(
             "screen",
             [](Ctx &c, double thresh, double interval, std::optional<std::vector<int32_t>> otypes,
