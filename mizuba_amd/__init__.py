@@ -13,7 +13,7 @@ same conjunction list) and `device` (HIP device index).
 screens several independent problems together and returns one `conjunctions`
 object per problem, identical to constructing them one by one.
 
-The native library must be built (`python -m mizuba_amd.build_native`); there
+The native library must be built (`python mizuba_amd/build_native.py`); there
 is no CPU fallback: without a HIP device, constructing `conjunctions` raises
 RuntimeError.
 """
@@ -26,7 +26,7 @@ try:
     from . import _core
 except ImportError as _e:  # pragma: no cover - depends on the build
     raise ImportError(
-        "mizuba_amd native extension is not built; run `python -m mizuba_amd.build_native` first"
+        "mizuba_amd native extension is not built; run `python mizuba_amd/build_native.py` first"
     ) from _e
 
 polyjectory = _core.polyjectory
@@ -36,6 +36,8 @@ Screener = _core.Screener
 screen_batch = _core.screen_batch
 make_synthetic_polyjectory = _core.make_synthetic_polyjectory
 device_count = _core.device_count
+get_tmpdir = _core.get_tmpdir
+set_tmpdir = _core.set_tmpdir
 
 
 class otype(_enum.IntEnum):
@@ -55,4 +57,6 @@ __all__ = [
     "screen_batch",
     "make_synthetic_polyjectory",
     "device_count",
+    "get_tmpdir",
+    "set_tmpdir",
 ]

@@ -673,6 +673,19 @@ struct ConjImpl {
     std::map<uint64_t, std::vector<mzh_bvh_node>> trees;
     std::map<uint64_t, std::vector<mzh_aabb_collision>> bps;
     std::unique_ptr<std::vector<mzh_conj>> conjs;
+    // data dir holding the result files, removed with the object
+    // (conjunctions.cpp:200-233); empty: results in HBM and host caches only
+    std::filesystem::path data_dir;
+    ConjImpl() = default;
+    ConjImpl(const ConjImpl &) = delete;
+    ConjImpl &operator=(const ConjImpl &) = delete;
+    ~ConjImpl()
+    {
+        if (!data_dir.empty()) {
+            std::error_code ec;
+            std::filesystem::remove_all(data_dir, ec);
+        }
+    }
 };
 
 struct Conjunctions {
@@ -711,6 +724,71 @@ void finish_impl(ConjImpl &im, const Polyjectory &pj, double conj_thresh, double
     check(mzh_copy_cd_end_times(im.ctx->ctx, im.cd_end_times.data()), im.ctx->ctx);
 }
 
+// The result files of detect_conjunctions (detect_conjunctions.cpp:91-126,
+// 455-534) in `dir`: aabbs, srt_aabbs, mcodes, srt_mcodes, vidx, bvh (the
+// steps' trees concatenated), bp (the steps' pairs concatenated; one zeroed
+// record when there are none) and conjunctions (a single byte when there are
+// none), all read-only.
+void store_conjunctions(ConjImpl &im, const std::filesystem::path &dir)
+{
+    mzh_ctx *ctx = im.ctx->ctx;
+    const uint64_t C = im.C, N = im.N;
+    {
+        std::vector<float> v(C * (N + 1) * 8);
+        check(mzh_copy_aabbs(ctx, v.data()), ctx);
+        mzh_host::write_file_ro(dir / "aabbs", v.data(), v.size() * sizeof(float));
+        check(mzh_copy_srt_aabbs(ctx, v.data()), ctx);
+        mzh_host::write_file_ro(dir / "srt_aabbs", v.data(), v.size() * sizeof(float));
+    }
+    {
+        std::vector<uint64_t> v(C * N);
+        check(mzh_copy_mcodes(ctx, v.data()), ctx);
+        mzh_host::write_file_ro(dir / "mcodes", v.data(), v.size() * sizeof(uint64_t));
+        check(mzh_copy_srt_mcodes(ctx, v.data()), ctx);
+        mzh_host::write_file_ro(dir / "srt_mcodes", v.data(), v.size() * sizeof(uint64_t));
+    }
+    {
+        std::vector<uint32_t> v(C * N);
+        check(mzh_copy_srt_idx(ctx, v.data()), ctx);
+        mzh_host::write_file_ro(dir / "vidx", v.data(), v.size() * sizeof(uint32_t));
+    }
+    {
+        std::vector<mzh_bvh_node> all;
+        for (uint64_t k = 0; k < C; ++k) {
+            uint64_t n = 0;
+            check(mzh_get_bvh_tree_size(ctx, k, &n), ctx);
+            const size_t at = all.size();
+            all.resize(at + n);
+            if (n) check(mzh_copy_bvh_tree(ctx, k, all.data() + at), ctx);
+        }
+        mzh_host::write_file_ro(dir / "bvh", all.data(), all.size() * sizeof(mzh_bvh_node));
+    }
+    {
+        std::vector<mzh_aabb_collision> all;
+        for (uint64_t k = 0; k < C; ++k) {
+            uint64_t n = 0;
+            check(mzh_get_aabb_collisions_size(ctx, k, &n), ctx);
+            const size_t at = all.size();
+            all.resize(at + n);
+            if (n) check(mzh_copy_aabb_collisions(ctx, k, all.data() + at), ctx);
+        }
+        if (all.empty()) all.push_back(mzh_aabb_collision{});
+        mzh_host::write_file_ro(dir / "bp", all.data(), all.size() * sizeof(mzh_aabb_collision));
+    }
+    {
+        uint64_t n = 0;
+        check(mzh_get_n_conjunctions(ctx, &n), ctx);
+        if (n) {
+            std::vector<mzh_conj> v(n);
+            check(mzh_copy_conjunctions(ctx, v.data()), ctx);
+            mzh_host::write_file_ro(dir / "conjunctions", v.data(), v.size() * sizeof(mzh_conj));
+        } else {
+            const char empty = 0;
+            mzh_host::write_file_ro(dir / "conjunctions", &empty, 1);
+        }
+    }
+}
+
 void check_otypes_size(const std::optional<std::vector<int32_t>> &otypes, uint64_t n_objs)
 {
     if (otypes && otypes->size() != n_objs)
@@ -721,19 +799,31 @@ void check_otypes_size(const std::optional<std::vector<int32_t>> &otypes, uint64
 }
 
 Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double conj_det_interval,
-                               std::optional<std::vector<int32_t>> otypes, std::optional<py::object> data_dir,
-                               std::optional<py::object> tmpdir, const std::string &mode, int device)
+                               std::optional<std::vector<int32_t>> otypes,
+                               std::optional<std::filesystem::path> data_dir,
+                               std::optional<std::filesystem::path> tmpdir, const std::string &mode, int device)
 {
-    if (data_dir && tmpdir && !data_dir->is_none() && !tmpdir->is_none())
+    if (data_dir && tmpdir)
         throw std::invalid_argument(
             "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time");
     const int m = parse_mode(mode);
     (void)attached(pj);
     auto im = std::make_shared<ConjImpl>();
+    // the result files are written when a data dir or a temporary dir is
+    // given; the dir is made first, as the reference does (conjunctions.cpp:336-341)
+    if (data_dir && !data_dir->empty()) {
+        im->data_dir = mzh_host::create_dir_0700(*data_dir);
+    } else if (tmpdir) {
+        im->data_dir = mzh_host::create_temp_dir(tmpdir, "mizuba_conjunctions");
+    }
     im->ctx = std::make_unique<Ctx>(device);
     im->ctx->upload(pj.d);
     im->ctx->screen(conj_thresh, conj_det_interval, otypes, m);
     finish_impl(*im, pj, conj_thresh, conj_det_interval, otypes, m);
+    if (!im->data_dir.empty()) {
+        py::gil_scoped_release rel;
+        store_conjunctions(*im, im->data_dir);
+    }
     return Conjunctions{im};
 }
 
@@ -902,6 +992,17 @@ PYBIND11_MODULE(_core, m)
     });
     pt.def("state_eval", &state_eval, "time"_a, "out"_a.noconvert() = py::none(), "obj_idx"_a = py::none());
     pt.def("state_meval", &state_meval, "time"_a, "out"_a.noconvert() = py::none(), "obj_idx"_a = py::none());
+
+    // tmpdir getter/setter (core.cpp:63-86): None stands for no global dir
+    m.def("get_tmpdir", []() -> std::optional<std::filesystem::path> {
+        auto p = mzh_host::get_global_tmpdir();
+        if (p.empty()) return std::nullopt;
+        return p;
+    });
+    m.def(
+        "set_tmpdir",
+        [](std::optional<std::filesystem::path> p) { mzh_host::set_global_tmpdir(p ? *p : std::filesystem::path()); },
+        "path"_a);
 
     m.def(
         "make_synthetic_polyjectory",
@@ -1132,6 +1233,10 @@ PYBIND11_MODULE(_core, m)
     cj.def_property_readonly("conj_thresh", [](const Conjunctions &c) { return c.impl->conj_thresh; });
     cj.def_property_readonly("conj_det_interval", [](const Conjunctions &c) { return c.impl->conj_det_interval; });
     cj.def("hint_release", [](const Conjunctions &) {});
+    cj.def_property_readonly("data_dir", [](const Conjunctions &c) -> py::object {
+        if (c.impl->data_dir.empty()) return py::none();
+        return py::cast(c.impl->data_dir);
+    });
     cj.def_property_readonly("np_report", [](const Conjunctions &c) {
         uint64_t r[MZH_NP_COUNT];
         check(mzh_get_np_report(c.impl->ctx->ctx, r), c.impl->ctx->ctx);

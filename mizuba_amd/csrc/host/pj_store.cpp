@@ -3,9 +3,11 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <limits>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -37,6 +39,12 @@ static_assert(sizeof(PjDesc) == 48 && offsetof(PjDesc, persist) == 4 && offsetof
 static_assert(sizeof(TrajOffset) == 16, "traj_offset layout");
 
 void mark_read_only(const fs::path &p) { fs::permissions(p, fs::perms::owner_read, fs::perm_options::replace); }
+
+std::mutex g_tmpdir_m;
+fs::path g_tmpdir = [] {
+    const char *e = std::getenv("MIZUBA_TMPDIR");
+    return e ? fs::path(e) : fs::path();
+}();
 
 // Write `bytes` bytes to a new file, then mark it read-only
 // (polyjectory.cpp:310-326).
@@ -143,9 +151,29 @@ fs::path create_dir_0700(const fs::path &path)
     }
 }
 
-fs::path create_temp_dir(const std::optional<fs::path> &tmpdir)
+fs::path get_global_tmpdir()
 {
-    fs::path base = (tmpdir && !tmpdir->empty()) ? *tmpdir : fs::temp_directory_path();
+    std::lock_guard<std::mutex> lk(g_tmpdir_m);
+    return g_tmpdir;
+}
+
+void set_global_tmpdir(fs::path p)
+{
+    std::lock_guard<std::mutex> lk(g_tmpdir_m);
+    g_tmpdir = std::move(p);
+}
+
+void write_file_ro(const fs::path &p, const void *data, size_t bytes) { write_file(p, data, bytes); }
+
+fs::path create_temp_dir(const std::optional<fs::path> &tmpdir, const char *prefix)
+{
+    fs::path base;
+    if (tmpdir && !tmpdir->empty()) {
+        base = *tmpdir;
+    } else {
+        base = get_global_tmpdir();
+        if (base.empty()) base = fs::temp_directory_path();
+    }
     try {
         base = fs::canonical(base);
     } catch (...) {
@@ -155,8 +183,8 @@ fs::path create_temp_dir(const std::optional<fs::path> &tmpdir)
     std::mt19937_64 rng(std::random_device{}());
     for (int attempt = 0; attempt < 10; ++attempt) {
         const fs::path p = base
-                           / ("mizuba_polyjectory-" + hex_group(rng) + "-" + hex_group(rng) + "-" + hex_group(rng) + "-"
-                              + hex_group(rng));
+                           / (std::string(prefix) + "-" + hex_group(rng) + "-" + hex_group(rng) + "-" + hex_group(rng)
+                              + "-" + hex_group(rng));
         if (fs::create_directory(p)) {
             try {
                 fs::permissions(p, fs::perms::owner_all, fs::perm_options::replace);

@@ -42,9 +42,17 @@ std::pair<double, double> normalise_epoch(double hi, double lo);
 // A new directory with owner-only permissions; it must not exist
 // (detail/file_utils.cpp:165-198). Returns its canonical path.
 fs::path create_dir_0700(const fs::path &path);
-// A new uniquely named directory "mizuba_polyjectory-XXXX-XXXX-XXXX-XXXX" in
-// tmpdir, or in the system's temporary directory (detail/file_utils.cpp:90-163).
-fs::path create_temp_dir(const std::optional<fs::path> &tmpdir);
+// A new uniquely named directory "<prefix>-XXXX-XXXX-XXXX-XXXX" in tmpdir, else
+// in the global temporary dir (set_tmpdir, initially $MIZUBA_TMPDIR), else in
+// the system's temporary directory (detail/file_utils.cpp:90-163).
+fs::path create_temp_dir(const std::optional<fs::path> &tmpdir, const char *prefix = "mizuba_polyjectory");
+
+// The global temporary dir (mizuba/tmpdir.cpp:40-78); empty: none.
+fs::path get_global_tmpdir();
+void set_global_tmpdir(fs::path p);
+
+// Write `bytes` bytes into a new file and mark it read-only.
+void write_file_ro(const fs::path &p, const void *data, size_t bytes);
 
 // Write the six files of `a` into the existing directory `dir`.
 void write_pj_files(const fs::path &dir, const PjArrays &a, bool persist);

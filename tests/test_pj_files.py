@@ -254,3 +254,37 @@ def test_mount_synthetic_round_trip():
         for a, b in zip(flat, m._flat()):
             assert np.array_equal(np.asarray(a), np.asarray(b))
         assert m.maxT == pj.maxT and m.n_objs == pj.n_objs
+
+
+def test_global_tmpdir():
+    orig = mz.get_tmpdir()
+    try:
+        with tempfile.TemporaryDirectory() as d:
+            mz.set_tmpdir(d)
+            assert mz.get_tmpdir() == Path(d)
+            pj = two_obj(persist=False, tmpdir=None, data_dir=None)
+            assert pj.data_dir is None  # in memory: no files asked for
+            pj = polyjectory(trajs=[np.zeros((1, 8, 7))], times=[np.array([0.0, 1.0])], status=[0], persist=True)
+            assert pj.data_dir.parent == Path(d).resolve()
+            mz.set_tmpdir(None)
+            assert mz.get_tmpdir() is None
+            # a tmpdir argument wins over the global one
+            mz.set_tmpdir(__file__)
+            with tempfile.TemporaryDirectory() as d2:
+                pj2 = two_obj(tmpdir=d2)
+                assert pj2.data_dir.parent == Path(d2).resolve()
+                del pj2
+    finally:
+        mz.set_tmpdir(orig)
+
+
+def test_conjunctions_data_dir_errors_before_screening():
+    trajs, times = [np.zeros((1, 3, 7))], [np.array([0.0, 1.0])]
+    pj = polyjectory(trajs, times, [0])
+    with tempfile.TemporaryDirectory() as d:
+        dd = Path(d) / "data_dir"
+        os.mkdir(dd)
+        raises(RuntimeError, "Error while creating the directory",
+               lambda: mz.conjunctions(pj, conj_thresh=0.1, conj_det_interval=0.01, data_dir=dd))
+    raises(ValueError, "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time",
+           lambda: mz.conjunctions(pj, conj_thresh=0.1, conj_det_interval=0.01, tmpdir="", data_dir=""))
