@@ -128,7 +128,7 @@ struct mzh_ctx {
     DevBuf cd_begin, cd_end;
     DevBuf aabbs, kbke, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, sort_todo, gbox;
     uint64_t node_cap = 0;
-    DevBuf nodes, trav, bvh_scr, tree_size;
+    DevBuf nodes, trav, bvh_hdr, bvh_scr, tree_size;
     bool nodes_ready = false;  // nodes holds the reference-layout export of trav
     DevBuf otypes, cnt, head, off, step_total, step_off, grand, slots, ovf, ovf_count;
     uint32_t ovf_cap = 0;
@@ -335,6 +335,7 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(ctx->sort_todo.ensure(sizeof(uint32_t) * C));
     MZH_HIP(ctx->gbox.ensure(sizeof(uint32_t) * 8u * C));
     MZH_HIP(ctx->trav.ensure(sizeof(TravNode) * ctx->node_cap * C));
+    MZH_HIP(ctx->bvh_hdr.ensure(sizeof(uint4) * ctx->node_cap * C));
     MZH_HIP(ctx->bvh_scr.ensure(bvh_scratch_stride(N) * C));
     ctx->nodes_ready = false;
     MZH_HIP(ctx->tree_size.ensure(sizeof(uint32_t) * C));
@@ -386,8 +387,8 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(hipEventRecord(ctx->ev[2], st));
     // stage 3: BVH
     MZH_HIP(launch_bvh(ctx->srt_mcodes.as<uint64_t>(), ctx->srt_aabbs.as<float>(), ctx->n_eff.as<uint32_t>(),
-                       ctx->trav.as<TravNode>(), ctx->bvh_scr.as<uint8_t>(), ctx->tree_size.as<uint32_t>(), N,
-                       ctx->node_cap, (uint32_t)C, st));
+                       ctx->trav.as<TravNode>(), ctx->bvh_hdr.as<uint4>(), ctx->bvh_scr.as<uint8_t>(),
+                       ctx->tree_size.as<uint32_t>(), N, ctx->node_cap, (uint32_t)C, st));
     MZH_HIP(hipEventRecord(ctx->ev[3], st));
     ctx->n_pairs = 0;
     ctx->n_conj = 0;

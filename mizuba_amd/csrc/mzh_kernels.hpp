@@ -37,9 +37,10 @@ hipError_t launch_morton_sort(const float *aabbs, uint64_t *mcodes, uint64_t *ke
 // nolc_buf, ps_buf and pos_buf are node_cap-sized scratch per step.
 // Bytes of per-step BVH scratch in HBM (used when it does not fit in LDS).
 size_t bvh_scratch_stride(uint32_t n_objs);
+// hdr: node_cap uint4 headers per step, scratch of the build.
 hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, TravNode *trav,
-                      uint8_t *scr_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
-                      hipStream_t stream);
+                      uint4 *hdr, uint8_t *scr_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap,
+                      uint32_t n_steps, hipStream_t stream);
 // The reference's bvh_node records of every step's tree, from the device layout.
 hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mzh_bvh_node *nodes, uint64_t node_cap,
                              uint32_t n_steps, hipStream_t stream);

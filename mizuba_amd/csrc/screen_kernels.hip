@@ -860,14 +860,19 @@ __device__ inline int bvh_split(const BvhScratch &sc, uint32_t lo, uint32_t hi, 
 // write consecutive node records. The escape index comes top-down: a left
 // child escapes to its right sibling, a right child to its parent's escape.
 //
-// While a node waits for its level, its record holds (escape, begin, end); an
-// internal node then keeps its split in place of its end.
+// The headers (escape, begin, end or split, left) live in a scratch array hdr
+// while the tree is built: a node's header is made by its parent and completed
+// at its own level, where an internal node keeps its split in place of its
+// end. Every node record is then written once, whole: a leaf's at its level,
+// an internal node's in the refit, so that consecutive lanes write full cache
+// lines instead of revisiting parts of records.
 template <bool LDS>
 __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ srt_mcodes,
                                                      const float *__restrict__ srt_aabbs,
                                                      const uint32_t *__restrict__ n_eff, TravNode *__restrict__ trav,
-                                                     uint8_t *__restrict__ scr_buf, size_t scr_stride,
-                                                     uint32_t *__restrict__ tree_size, uint32_t N, uint64_t node_cap)
+                                                     uint4 *__restrict__ hdr, uint8_t *__restrict__ scr_buf,
+                                                     size_t scr_stride, uint32_t *__restrict__ tree_size, uint32_t N,
+                                                     uint64_t node_cap)
 {
     extern __shared__ __align__(16) uint8_t s_dyn[];
     const uint32_t c = blockIdx.x;
@@ -880,6 +885,7 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
     const uint64_t *codes = srt_mcodes + (size_t)c * N;
     const float *boxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
     TravNode *T = trav + (size_t)c * node_cap;
+    uint4 *H = hdr + (size_t)c * node_cap;
     const BvhScratch sc = bvh_scratch_at(LDS ? s_dyn : scr_buf + (size_t)c * scr_stride, n);
     const uint32_t nch = (n + 63u) / 64u;
     __shared__ uint32_t lvl[kBvhMaxLevels + 2];
@@ -901,7 +907,7 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
         }
         if (lane == 0) sc.dmax[q >> 6] = (int8_t)m;
     }
-    if (tid == 0) reinterpret_cast<uint4 *>(&T[0])[2] = make_uint4(kTravEnd, 0u, n, 0u);
+    if (tid == 0) H[0] = make_uint4(kTravEnd, 0u, n, 0u);
     __syncthreads();
 
     // ---- top-down, one level at a time ------------------------------------
@@ -915,7 +921,7 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
             uint32_t esc = 0, b = 0, e = 0, q = 0;
             bool inner = false;
             if (k < cnt) {
-                const uint4 h = reinterpret_cast<const uint4 *>(&T[lb + k])[2];
+                const uint4 h = H[lb + k];
                 esc = h.x;
                 b = h.y;
                 e = h.z;
@@ -928,18 +934,20 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
                 return;
             }
             if (k < cnt) {
-                uint4 *nd = reinterpret_cast<uint4 *>(&T[lb + k]);
                 if (inner) {
                     const uint32_t lc = le + carry + ex;
-                    reinterpret_cast<uint4 *>(&T[lc])[2] = make_uint4(lc + 1u, b, q, 0u);
-                    reinterpret_cast<uint4 *>(&T[lc + 1u])[2] = make_uint4(esc, q, e, 0u);
-                    nd[2] = make_uint4(esc, b, q, lc);
+                    H[lc] = make_uint4(lc + 1u, b, q, 0u);
+                    H[lc + 1u] = make_uint4(esc, q, e, 0u);
+                    H[lb + k] = make_uint4(esc, b, q, lc);
                 } else {
                     float l[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
                     float u[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
                     for (uint32_t o = b; o != e; ++o) box_union(l, u, boxes + (size_t)o * 8u);
-                    reinterpret_cast<float4 *>(nd)[0] = make_float4(l[0], l[1], l[2], l[3]);
-                    reinterpret_cast<float4 *>(nd)[1] = make_float4(u[0], u[1], u[2], u[3]);
+                    uint4 *nd = reinterpret_cast<uint4 *>(&T[lb + k]);
+                    nd[0] = make_uint4(__float_as_uint(l[0]), __float_as_uint(l[1]), __float_as_uint(l[2]),
+                                       __float_as_uint(l[3]));
+                    nd[1] = make_uint4(__float_as_uint(u[0]), __float_as_uint(u[1]), __float_as_uint(u[2]),
+                                       __float_as_uint(u[3]));
                     nd[2] = make_uint4(esc, b, e, 0u);
                 }
             }
@@ -958,14 +966,19 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
     for (int j = (int)nlev - 2; j >= 0; --j) {
         const uint32_t a = lvl[j], z = lvl[j + 1];
         for (uint32_t ni = a + tid; ni < z; ni += kBvhBlock) {
-            const uint32_t left = T[ni].left;
-            if (left != 0u) {
+            const uint4 h = H[ni];
+            if (h.w != 0u) {
+                const uint32_t left = h.w;
                 float l[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
                 float u[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
                 box_union(l, u, T[left].lb);
                 box_union(l, u, T[left + 1u].lb);
-                reinterpret_cast<float4 *>(&T[ni])[0] = make_float4(l[0], l[1], l[2], l[3]);
-                reinterpret_cast<float4 *>(&T[ni])[1] = make_float4(u[0], u[1], u[2], u[3]);
+                uint4 *nd = reinterpret_cast<uint4 *>(&T[ni]);
+                nd[0] = make_uint4(__float_as_uint(l[0]), __float_as_uint(l[1]), __float_as_uint(l[2]),
+                                   __float_as_uint(l[3]));
+                nd[1] = make_uint4(__float_as_uint(u[0]), __float_as_uint(u[1]), __float_as_uint(u[2]),
+                                   __float_as_uint(u[3]));
+                nd[2] = h;
             }
         }
         __syncthreads();
@@ -1321,17 +1334,17 @@ hipError_t launch_morton_sort(const float *aabbs, uint64_t *mcodes, uint64_t *ke
 size_t bvh_scratch_stride(uint32_t n_objs) { return (bvh_scratch_bytes(n_objs) + 255u) / 256u * 256u; }
 
 hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, TravNode *trav,
-                      uint8_t *scr_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
-                      hipStream_t stream)
+                      uint4 *hdr, uint8_t *scr_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap,
+                      uint32_t n_steps, hipStream_t stream)
 {
     // LDS when two workgroups per CU still fit; otherwise the scratch is in HBM
     const size_t bytes = bvh_scratch_bytes(n_objs);
     if (bytes <= kBvhLdsMax) {
         hipLaunchKernelGGL(k_bvh<true>, dim3(n_steps), dim3(kBvhBlock), bytes, stream, srt_mcodes, srt_aabbs, n_eff,
-                           trav, scr_buf, bvh_scratch_stride(n_objs), tree_size, n_objs, node_cap);
+                           trav, hdr, scr_buf, bvh_scratch_stride(n_objs), tree_size, n_objs, node_cap);
     } else {
         hipLaunchKernelGGL(k_bvh<false>, dim3(n_steps), dim3(kBvhBlock), 0, stream, srt_mcodes, srt_aabbs, n_eff,
-                           trav, scr_buf, bvh_scratch_stride(n_objs), tree_size, n_objs, node_cap);
+                           trav, hdr, scr_buf, bvh_scratch_stride(n_objs), tree_size, n_objs, node_cap);
     }
     return hipGetLastError();
 }
