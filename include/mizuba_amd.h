@@ -117,6 +117,63 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
                            const double *time, uint64_t time_len, const uint64_t *traj_off, const uint64_t *nsteps,
                            const uint64_t *time_off, double maxT);
 
+/* ---- polyjectories from SGP4 element sets ------------------------------- */
+
+/* A general perturbations element set (mizuba/make_sgp4_polyjectory.hpp:35-47):
+ * UTC Julian epoch as two parts, mean motion (rad/min), eccentricity,
+ * inclination, right ascension of the ascending node, argument of perigee,
+ * mean anomaly (rad) and the B* drag term (1 / Earth radii). */
+typedef struct mzh_gpe {
+    uint64_t norad_id;
+    double epoch_jd1, epoch_jd2;
+    double n0, e0, i0, node0, omega0, m0, bstar;
+} mzh_gpe;
+
+/* Satellite status codes of a generated polyjectory, beside the SGP4 error
+ * codes 1-6 (make_sgp4_polyjectory.cpp:478-913): */
+#define MZH_SGP4_NONFINITE 10  /* non-finite values in the interpolation */
+#define MZH_SGP4_REENTRY 11    /* below the reentry radius */
+#define MZH_SGP4_EXIT 12       /* at or beyond the exit radius */
+#define MZH_SGP4_INTERPOLATION 14 /* more than 20 pieces needed for one nominal step */
+#define MZH_SGP4_DEEP_SPACE 20 /* deep-space element set: not propagated by this build */
+
+/* SGP4 (near-earth model, WGS-72) states of n_gpes element sets at n_times
+ * times each: out[g][k] = {x, y, z, vx, vy, vz} (km, km/s, TEME) at tsince[g][k]
+ * minutes from the epoch of gpes[g], err[g][k] = the SGP4 error code (0: none,
+ * MZH_SGP4_DEEP_SPACE for a deep-space set; an error of the propagation to the
+ * epoch, which the published initialisation performs, is reported for every
+ * time). The arithmetic is that of the published SGP4 code, including its
+ * stopping rule for Kepler's equation. Host pointers. No reference
+ * counterpart as a function: it is the propagation step of
+ * make_sgp4_polyjectory (make_sgp4_polyjectory.cpp:722-758, 822-835). */
+int mzh_sgp4_propagate(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, const double *tsince, uint64_t n_times,
+                       double *out, int32_t *err);
+
+/* Build, on the GPU, the polyjectory of the satellites of gpes (sorted by
+ * NORAD ID, then epoch) over the UTC Julian dates [jd_begin, jd_end), and make
+ * it the context's polyjectory, as mzh_upload_polyjectory would. Replaces
+ * make_sgp4_polyjectory (mizuba/make_sgp4_polyjectory.cpp:1460-1605) with the
+ * reference's checks, GPE selection and status codes. Each GPE's stretch of
+ * a trajectory is cut into nominal steps of a twentieth of the orbital period
+ * (shortened for eccentric orbits by the perigee's angular rate), each
+ * interpolated at the Chebyshev nodes and halved until the positions are
+ * within 1e-7 km of SGP4 between the nodes (pieces shorter than 5 s are kept
+ * as they are; more than 20 pieces is MZH_SGP4_INTERPOLATION). A satellite's
+ * trajectory ends at its first failing step, whose code becomes its status.
+ * Deep-space element sets get MZH_SGP4_DEEP_SPACE and no data. Time in days
+ * from jd_begin in TAI; the interpolated model solves Kepler's equation to
+ * convergence. order: the polynomial order (8 in this build). */
+int mzh_make_sgp4_polyjectory(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double jd_begin, double jd_end,
+                              double reentry_radius, double exit_radius, uint32_t order);
+
+/* The context's polyjectory: sizes, then the arrays (host pointers) in the
+ * layout of mzh_upload_polyjectory, the per-object status (0 for uploaded
+ * polyjectories) and the epoch of a generated one (TAI Julian date, two parts). */
+int mzh_get_polyjectory_info(const mzh_ctx *ctx, uint64_t *n_objs, uint32_t *order, uint64_t *traj_len,
+                             uint64_t *time_len, double *maxT, double epoch[2]);
+int mzh_copy_polyjectory(mzh_ctx *ctx, double *traj, double *time, uint64_t *traj_off, uint64_t *nsteps,
+                         uint64_t *time_off, int32_t *status);
+
 /* ---- evaluation of the uploaded polyjectory ----------------------------- */
 
 /* Where the evaluation time of row r (object sel[r], or r) and column k comes

@@ -13,6 +13,9 @@ same conjunction list) and `device` (HIP device index).
 screens several independent problems together and returns one `conjunctions`
 object per problem, identical to constructing them one by one.
 
+`make_sgp4_polyjectory(gpes, jd_begin, jd_end, ...)` generates the
+polyjectory of a set of SGP4 element sets on the GPU (near-earth satellites).
+
 The native library must be built (`python mizuba_amd/build_native.py`); there
 is no CPU fallback: without a HIP device, constructing `conjunctions` raises
 RuntimeError.
@@ -39,6 +42,8 @@ device_count = _core.device_count
 get_tmpdir = _core.get_tmpdir
 set_tmpdir = _core.set_tmpdir
 
+from ._sgp4_polyjectory import gpe_dtype, make_sgp4_polyjectory, sgp4_pj_status, sgp4_propagate  # noqa: E402
+
 
 class otype(_enum.IntEnum):
     """Object types for conjunction screening (mizuba/__init__.py:61-66)."""
@@ -59,4 +64,8 @@ __all__ = [
     "device_count",
     "get_tmpdir",
     "set_tmpdir",
+    "gpe_dtype",
+    "make_sgp4_polyjectory",
+    "sgp4_pj_status",
+    "sgp4_propagate",
 ]

@@ -117,6 +117,11 @@ struct mzh_ctx {
     double maxT = 0;
     uint64_t n_steps_total = 0;
     DevBuf traj, time, cf4, step0, nsteps, time_off;
+    // host copies of its layout, statuses and epoch (TAI Julian date) for copy-out
+    std::vector<uint64_t> h_traj_off, h_nsteps, h_time_off;
+    std::vector<int32_t> h_status;
+    uint64_t traj_len = 0, time_len = 0;
+    double pj_epoch[2] = {0, 0};
 
     // problem state
     bool screened = false;
@@ -708,7 +713,336 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         ctx->order = order;
         ctx->maxT = maxT;
         ctx->n_steps_total = steps_total;
+        ctx->h_traj_off.assign(traj_off, traj_off + n_objs);
+        ctx->h_nsteps.assign(nsteps, nsteps + n_objs);
+        ctx->h_time_off.assign(time_off, time_off + n_objs);
+        ctx->h_status.assign(n_objs, 0);
+        ctx->traj_len = traj_len;
+        ctx->time_len = time_len;
+        ctx->pj_epoch[0] = ctx->pj_epoch[1] = 0;
         ctx->has_pj = true;
+    });
+}
+
+namespace
+{
+
+// Double-length sum (Knuth's two-sum), normalised.
+struct DL {
+    double hi, lo;
+};
+DL dl_norm(double a, double b)
+{
+    const double s = a + b, bp = s - a, ap = s - bp;
+    return DL{s, (a - ap) + (b - bp)};
+}
+bool dl_less(const DL &a, const DL &b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }
+
+// TAI - UTC (leap seconds) from 1972 on, as published by the IERS: (UTC
+// Julian date from which it applies, seconds).
+constexpr double kLeap[][2] = {
+    {2441317.5, 10}, {2441499.5, 11}, {2441683.5, 12}, {2442048.5, 13}, {2442413.5, 14}, {2442778.5, 15},
+    {2443144.5, 16}, {2443509.5, 17}, {2443874.5, 18}, {2444239.5, 19}, {2444786.5, 20}, {2445151.5, 21},
+    {2445516.5, 22}, {2446247.5, 23}, {2447161.5, 24}, {2447892.5, 25}, {2448257.5, 26}, {2448804.5, 27},
+    {2449169.5, 28}, {2449534.5, 29}, {2450083.5, 30}, {2450630.5, 31}, {2451179.5, 32}, {2453736.5, 33},
+    {2454832.5, 34}, {2456109.5, 35}, {2457204.5, 36}, {2457754.5, 37}};
+
+// TAI - UTC in seconds at the UTC Julian date jd (>= 1972).
+double tai_minus_utc(double jd)
+{
+    double dat = kLeap[0][1];
+    for (const auto &e : kLeap)
+        if (jd >= e[0]) dat = e[1];
+    return dat;
+}
+
+// UTC Julian date (two parts) -> TAI Julian date, double-length, as the IAU
+// SOFA routine utctai states it (make_sgp4_polyjectory.cpp:115-121 converts
+// through heyoka's jd_utc_to_tai): the fraction of the day is scaled by
+// (86400 + leap) / 86400 on a day that ends with a leap second, then TAI - UTC
+// at the day's start is added.
+DL utc_to_tai(double jd1, double jd2)
+{
+    const DL a = dl_norm(jd1, jd2);
+    if (!(a.hi >= kLeap[0][0]))
+        throw StatusError{MZH_EINVAL, "UTC dates before 1972 are not supported by make_sgp4_polyjectory()"};
+    double day0 = std::floor(a.hi - 0.5) + 0.5;
+    double fd = (a.hi - day0) + a.lo;
+    if (fd < 0.0) {
+        day0 -= 1.0;
+        fd += 1.0;
+    } else if (fd >= 1.0) {
+        day0 += 1.0;
+        fd -= 1.0;
+    }
+    const double dat0 = tai_minus_utc(day0), leap = tai_minus_utc(day0 + 1.0) - dat0;
+    return dl_norm(day0, fd * ((86400.0 + leap) / 86400.0) + dat0 / 86400.0);
+}
+
+// b - a in days, for nearby double-length dates.
+double dl_diff(const DL &b, const DL &a) { return (b.hi - a.hi) + (b.lo - a.lo); }
+
+DL gpe_epoch(const mzh_gpe &g) { return dl_norm(g.epoch_jd1, g.epoch_jd2); }
+
+std::string fmt_u64(uint64_t v) { return std::to_string(v); }
+
+// make_sgp4_polyjectory (make_sgp4_polyjectory.cpp:1460-1605).
+void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double jd_begin, double jd_end,
+                    double reentry_radius, double exit_radius, uint32_t order)
+{
+    if (!std::isfinite(jd_begin) || !std::isfinite(jd_end) || !(jd_begin < jd_end))
+        throw StatusError{MZH_EINVAL, "Invalid Julian date interval [" + fmt_double(jd_begin) + ", " + fmt_double(jd_end)
+                                          + ") supplied to make_sgp4_polyjectory(): the begin/end dates must be "
+                                            "finite and the end date must be strictly after the begin date"};
+    if (std::isnan(reentry_radius) || std::isnan(exit_radius))
+        throw StatusError{MZH_EINVAL, "The reentry/exit radiuses in make_sgp4_polyjectory() cannot be NaN"};
+    if (!(reentry_radius < exit_radius))
+        throw StatusError{MZH_EINVAL, "The reentry radius (" + fmt_double(reentry_radius)
+                                          + ") must be less than the exit radius (" + fmt_double(exit_radius)
+                                          + ") in make_sgp4_polyjectory()"};
+    if (n_gpes == 0 || !gpes)
+        throw StatusError{MZH_EINVAL, "make_sgp4_polyjectory() requires a non-empty array of GPEs in input"};
+    if (order != 8)
+        throw StatusError{MZH_EINVAL, "make_sgp4_polyjectory() builds polyjectories of order 8 in this build, but "
+                                      "the order "
+                                          + std::to_string(order) + " was requested"};
+    // finiteness (make_sgp4_polyjectory.cpp:123-147), then the order (:150-186)
+    for (uint64_t i = 0; i < n_gpes; ++i) {
+        const mzh_gpe &g = gpes[i];
+        const std::pair<const char *, double> f[] = {{"epoch_jd1", g.epoch_jd1}, {"epoch_jd2", g.epoch_jd2},
+                                                     {"n0", g.n0},               {"e0", g.e0},
+                                                     {"i0", g.i0},               {"node0", g.node0},
+                                                     {"omega0", g.omega0},       {"m0", g.m0},
+                                                     {"bstar", g.bstar}};
+        for (const auto &x : f)
+            if (!std::isfinite(x.second))
+                throw StatusError{MZH_EINVAL, std::string("Invalid GPE detected by make_sgp4_polyjectory(): the '")
+                                                  + x.first + "' of the GPE at index " + std::to_string(i)
+                                                  + " is not finite"};
+    }
+    for (uint64_t i = 1; i < n_gpes; ++i) {
+        const mzh_gpe &a = gpes[i - 1], &b = gpes[i];
+        if (a.norad_id < b.norad_id) continue;
+        if (a.norad_id == b.norad_id) {
+            const DL ea = gpe_epoch(a), eb = gpe_epoch(b);
+            if (dl_less(ea, eb)) continue;
+            if (!dl_less(eb, ea))
+                throw StatusError{MZH_EINVAL, "Two GPEs with identical NORAD ID " + fmt_u64(a.norad_id) + " and epoch ("
+                                                  + fmt_double(a.epoch_jd1) + ", " + fmt_double(a.epoch_jd2)
+                                                  + ") were identified by make_sgp4_polyjectory() - this is not "
+                                                    "allowed"};
+        }
+        throw StatusError{MZH_EINVAL, "The set of GPEs passed to make_sgp4_polyjectory() is not sorted correctly: it "
+                                      "must be ordered first by NORAD ID and then by epoch"};
+    }
+    // satellites, the GPE each starts with (the latest at or before jd_begin,
+    // else the first: make_sgp4_polyjectory.cpp:1308-1335) and the segments:
+    // each GPE from the later of jd_begin and its epoch to the next GPE's
+    // epoch, or to jd_end when there is no next GPE before it (:1340-1375).
+    // Times in days from the polyjectory's epoch (jd_begin in TAI).
+    const DL pj_epoch = utc_to_tai(jd_begin, 0.0);
+    const double t_end = dl_diff(utc_to_tai(jd_end, 0.0), pj_epoch);
+    const DL begin_utc = dl_norm(jd_begin, 0.0), end_utc = dl_norm(jd_end, 0.0);
+    if (n_gpes > 0xfffffff0ull) throw StatusError{MZH_EINVAL, "Too many GPEs for make_sgp4_polyjectory()"};
+    std::vector<Sgp4Seg> segs;
+    std::vector<uint32_t> sat_seg0;
+    for (uint64_t i = 0; i < n_gpes;) {
+        uint64_t j = i;
+        while (j < n_gpes && gpes[j].norad_id == gpes[i].norad_id) ++j;
+        uint64_t start = i;
+        for (uint64_t k = i; k < j; ++k)
+            if (!dl_less(begin_utc, gpe_epoch(gpes[k]))) start = k;
+        const uint32_t sat = (uint32_t)sat_seg0.size();
+        sat_seg0.push_back((uint32_t)segs.size());
+        double t_b = 0.0;
+        for (uint64_t g = start; g < j; ++g) {
+            const bool last = g + 1u == j || !dl_less(gpe_epoch(gpes[g + 1u]), end_utc);
+            Sgp4Seg sg{};
+            sg.eoff = dl_diff(utc_to_tai(gpes[g].epoch_jd1, gpes[g].epoch_jd2), pj_epoch);
+            sg.t_b = t_b;
+            sg.t_e = last ? t_end : dl_diff(utc_to_tai(gpes[g + 1u].epoch_jd1, gpes[g + 1u].epoch_jd2), pj_epoch);
+            sg.sat = sat;
+            sg.gpe = (uint32_t)g;
+            segs.push_back(sg);
+            if (last) break;
+            t_b = sg.t_e;
+        }
+        i = j;
+    }
+    const uint32_t n_sats = (uint32_t)sat_seg0.size();
+    const uint32_t n_segs = (uint32_t)segs.size();
+    sat_seg0.push_back(n_segs);
+    const uint32_t op1 = order + 1u;
+    hipStream_t st = ctx->stream;
+    DevBuf d_gpes, d_segs, d_sats, d_satseg, d_segstep, d_res, d_out, d_np, d_status, d_tlast;
+    MZH_HIP(d_gpes.ensure(sizeof(mzh_gpe) * n_gpes));
+    MZH_HIP(d_segs.ensure(sizeof(Sgp4Seg) * n_segs));
+    MZH_HIP(d_sats.ensure(sizeof(Sgp4Sat) * n_segs));
+    MZH_HIP(d_satseg.ensure(sizeof(uint32_t) * (n_sats + 1u)));
+    MZH_HIP(d_segstep.ensure(sizeof(uint64_t) * (n_segs + 1u)));
+    MZH_HIP(d_np.ensure(sizeof(uint64_t) * n_sats));
+    MZH_HIP(d_status.ensure(sizeof(int32_t) * n_sats));
+    MZH_HIP(d_tlast.ensure(sizeof(double) * n_sats));
+    MZH_HIP(hipMemcpyAsync(d_gpes.p, gpes, sizeof(mzh_gpe) * n_gpes, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemcpyAsync(d_segs.p, segs.data(), sizeof(Sgp4Seg) * n_segs, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemcpyAsync(d_satseg.p, sat_seg0.data(), sizeof(uint32_t) * (n_sats + 1u), hipMemcpyHostToDevice, st));
+    Sgp4Gen g{};
+    g.gpes = d_gpes.as<const mzh_gpe>();
+    g.segs = d_segs.as<Sgp4Seg>();
+    g.sats = d_sats.as<Sgp4Sat>();
+    g.n_segs = n_segs;
+    g.n_sats = n_sats;
+    g.sat_seg0 = d_satseg.as<const uint32_t>();
+    g.reentry_r2 = reentry_radius * reentry_radius;
+    g.exit_r2 = std::isinf(exit_radius) ? std::numeric_limits<double>::infinity() : exit_radius * exit_radius;
+    MZH_HIP(launch_sgp4_seg_init(g, st));
+    MZH_HIP(hipMemcpyAsync(segs.data(), d_segs.p, sizeof(Sgp4Seg) * n_segs, hipMemcpyDeviceToHost, st));
+    MZH_HIP(hipStreamSynchronize(st));
+    // nominal steps of each segment (none for an empty stretch of a valid GPE)
+    std::vector<uint64_t> seg_step0(n_segs + 1u);
+    uint64_t n_steps = 0;
+    for (uint32_t k = 0; k < n_segs; ++k) {
+        if (segs[k].status == 0 && !(segs[k].t_e > segs[k].t_b)) segs[k].n_steps = 0;
+        segs[k].step0 = n_steps;
+        seg_step0[k] = n_steps;
+        n_steps += segs[k].n_steps;
+    }
+    seg_step0[n_segs] = n_steps;
+    MZH_HIP(hipMemcpyAsync(d_segs.p, segs.data(), sizeof(Sgp4Seg) * n_segs, hipMemcpyHostToDevice, st));
+    MZH_HIP(d_segstep.ensure(sizeof(uint64_t) * (n_segs + 1u)));
+    MZH_HIP(hipMemcpyAsync(d_segstep.p, seg_step0.data(), sizeof(uint64_t) * (n_segs + 1u), hipMemcpyHostToDevice,
+                           st));
+    MZH_HIP(d_res.ensure(sizeof(int32_t) * (n_steps ? n_steps : 1)));
+    MZH_HIP(d_out.ensure(sizeof(uint32_t) * (n_steps ? n_steps : 1)));
+    g.seg_step0 = d_segstep.as<const uint64_t>();
+    g.n_steps = n_steps;
+    g.step_res = d_res.as<int32_t>();
+    g.step_out = d_out.as<uint32_t>();
+    g.sat_npolys = d_np.as<uint64_t>();
+    g.sat_status = d_status.as<int32_t>();
+    g.sat_tlast = d_tlast.as<double>();
+    MZH_HIP(launch_sgp4_count(g, order, st));
+    MZH_HIP(launch_sgp4_scan(g, st));
+    std::vector<uint64_t> nsteps(n_sats);
+    std::vector<int32_t> status(n_sats);
+    std::vector<double> t_last(n_sats);
+    MZH_HIP(hipMemcpyAsync(nsteps.data(), d_np.p, sizeof(uint64_t) * n_sats, hipMemcpyDeviceToHost, st));
+    MZH_HIP(hipMemcpyAsync(status.data(), d_status.p, sizeof(int32_t) * n_sats, hipMemcpyDeviceToHost, st));
+    MZH_HIP(hipMemcpyAsync(t_last.data(), d_tlast.p, sizeof(double) * n_sats, hipMemcpyDeviceToHost, st));
+    MZH_HIP(hipStreamSynchronize(st));
+    // layout of the polyjectory
+    std::vector<uint64_t> step0(n_sats), traj_off(n_sats), time_off(n_sats);
+    uint64_t steps_total = 0, time_len = 0;
+    double maxT = 0;
+    for (uint32_t k = 0; k < n_sats; ++k) {
+        if (nsteps[k] >= 0xffffffffull) throw StatusError{MZH_EINVAL, "Too many trajectory steps for one object"};
+        step0[k] = steps_total;
+        traj_off[k] = steps_total * op1 * 7u;
+        time_off[k] = time_len;
+        steps_total += nsteps[k];
+        time_len += nsteps[k] ? nsteps[k] + 1u : 0u;
+        if (nsteps[k]) maxT = std::max(maxT, t_last[k]);
+    }
+    if (steps_total == 0)
+        throw StatusError{MZH_EINVAL,
+                          "All the trajectories in a polyjectory have a number of steps equal to zero: this is not "
+                          "allowed"};
+    ctx->has_pj = false;
+    ctx->screened = false;
+    MZH_HIP(ctx->traj.ensure(sizeof(double) * steps_total * op1 * 7u));
+    MZH_HIP(ctx->time.ensure(sizeof(double) * time_len));
+    MZH_HIP(ctx->cf4.ensure(sizeof(double) * 4u * op1 * steps_total));
+    MZH_HIP(ctx->step0.ensure(sizeof(uint64_t) * n_sats));
+    MZH_HIP(ctx->nsteps.ensure(sizeof(uint64_t) * n_sats));
+    MZH_HIP(ctx->time_off.ensure(sizeof(uint64_t) * n_sats));
+    MZH_HIP(hipMemcpyAsync(ctx->step0.p, step0.data(), sizeof(uint64_t) * n_sats, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemcpyAsync(ctx->nsteps.p, nsteps.data(), sizeof(uint64_t) * n_sats, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemcpyAsync(ctx->time_off.p, time_off.data(), sizeof(uint64_t) * n_sats, hipMemcpyHostToDevice, st));
+    g.traj = ctx->traj.as<double>();
+    g.time = ctx->time.as<double>();
+    g.sat_poly0 = ctx->step0.as<const uint64_t>();
+    g.sat_time0 = ctx->time_off.as<const uint64_t>();
+    MZH_HIP(launch_sgp4_write(g, order, st));
+    MZH_HIP(launch_transpose_cf4(ctx->traj.as<double>(), ctx->cf4.as<double>(), steps_total, order, st));
+    MZH_HIP(hipStreamSynchronize(st));
+    ctx->n_objs = n_sats;
+    ctx->order = order;
+    ctx->maxT = maxT;
+    ctx->n_steps_total = steps_total;
+    ctx->h_traj_off = std::move(traj_off);
+    ctx->h_nsteps = std::move(nsteps);
+    ctx->h_time_off = std::move(time_off);
+    ctx->h_status = std::move(status);
+    ctx->traj_len = steps_total * op1 * 7u;
+    ctx->time_len = time_len;
+    ctx->pj_epoch[0] = pj_epoch.hi;
+    ctx->pj_epoch[1] = pj_epoch.lo;
+    ctx->has_pj = true;
+}
+
+} // namespace
+
+int mzh_sgp4_propagate(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, const double *tsince, uint64_t n_times,
+                       double *out, int32_t *err)
+{
+    return guarded(ctx, [&] {
+        const uint64_t n = n_gpes * n_times;
+        if (n == 0) return;
+        if (!gpes || !tsince || !out || !err) throw StatusError{MZH_EINVAL, "Null pointer passed to mzh_sgp4_propagate"};
+        if (n_times && n_gpes > (uint64_t)1 << 40) throw StatusError{MZH_EINVAL, "Too many propagations"};
+        hipStream_t st = ctx->stream;
+        DevBuf d_g, d_t, d_o, d_e;
+        MZH_HIP(d_g.ensure(sizeof(mzh_gpe) * n_gpes));
+        MZH_HIP(d_t.ensure(sizeof(double) * n));
+        MZH_HIP(d_o.ensure(sizeof(double) * 6u * n));
+        MZH_HIP(d_e.ensure(sizeof(int32_t) * n));
+        MZH_HIP(hipMemcpyAsync(d_g.p, gpes, sizeof(mzh_gpe) * n_gpes, hipMemcpyHostToDevice, st));
+        MZH_HIP(hipMemcpyAsync(d_t.p, tsince, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        MZH_HIP(launch_sgp4_propagate(d_g.as<const mzh_gpe>(), n_gpes, d_t.as<const double>(), n_times,
+                                      d_o.as<double>(), d_e.as<int32_t>(), st));
+        MZH_HIP(hipMemcpyAsync(out, d_o.p, sizeof(double) * 6u * n, hipMemcpyDeviceToHost, st));
+        MZH_HIP(hipMemcpyAsync(err, d_e.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+        MZH_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int mzh_make_sgp4_polyjectory(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double jd_begin, double jd_end,
+                              double reentry_radius, double exit_radius, uint32_t order)
+{
+    return guarded(ctx,
+                   [&] { make_sgp4_impl(ctx, gpes, n_gpes, jd_begin, jd_end, reentry_radius, exit_radius, order); });
+}
+
+int mzh_get_polyjectory_info(const mzh_ctx *ctx, uint64_t *n_objs, uint32_t *order, uint64_t *traj_len,
+                             uint64_t *time_len, double *maxT, double epoch[2])
+{
+    if (!ctx || !ctx->has_pj) return MZH_EINVAL;
+    if (n_objs) *n_objs = ctx->n_objs;
+    if (order) *order = ctx->order;
+    if (traj_len) *traj_len = ctx->traj_len;
+    if (time_len) *time_len = ctx->time_len;
+    if (maxT) *maxT = ctx->maxT;
+    if (epoch) {
+        epoch[0] = ctx->pj_epoch[0];
+        epoch[1] = ctx->pj_epoch[1];
+    }
+    return MZH_OK;
+}
+
+int mzh_copy_polyjectory(mzh_ctx *ctx, double *traj, double *time, uint64_t *traj_off, uint64_t *nsteps,
+                         uint64_t *time_off, int32_t *status)
+{
+    return guarded(ctx, [&] {
+        if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
+        if (traj && ctx->traj_len)
+            MZH_HIP(hipMemcpy(traj, ctx->traj.p, sizeof(double) * ctx->traj_len, hipMemcpyDeviceToHost));
+        if (time && ctx->time_len)
+            MZH_HIP(hipMemcpy(time, ctx->time.p, sizeof(double) * ctx->time_len, hipMemcpyDeviceToHost));
+        const size_t n = ctx->n_objs;
+        if (traj_off) std::copy(ctx->h_traj_off.begin(), ctx->h_traj_off.end(), traj_off);
+        if (nsteps) std::copy(ctx->h_nsteps.begin(), ctx->h_nsteps.end(), nsteps);
+        if (time_off) std::copy(ctx->h_time_off.begin(), ctx->h_time_off.end(), time_off);
+        if (status) std::copy(ctx->h_status.begin(), ctx->h_status.begin() + n, status);
     });
 }
 

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -70,6 +71,13 @@ py::dtype bvh_node_dtype()
 }
 
 py::dtype traj_offset_dtype() { return py::dtype::from_args(py::eval("[('offset', '<u8'), ('n_steps', '<u8')]")); }
+
+py::dtype gpe_dtype()
+{
+    return py::dtype::from_args(py::eval(
+        "[('norad_id', '<u8'), ('epoch_jd1', '<f8'), ('epoch_jd2', '<f8'), ('n0', '<f8'), ('e0', '<f8'), "
+        "('i0', '<f8'), ('node0', '<f8'), ('omega0', '<f8'), ('m0', '<f8'), ('bstar', '<f8')]"));
+}
 
 py::dtype aabb_collision_dtype() { return py::dtype::from_args(py::eval("[('i', '<u4'), ('j', '<u4')]")); }
 
@@ -647,6 +655,49 @@ struct DeviceBuffer {
     }
 };
 
+// GPEs from a NumPy array of gpe_dtype (expose_make_sgp4_polyjectory.cpp:55-70).
+std::vector<mzh_gpe> gpes_from_array(const py::array &a)
+{
+    if (!a.dtype().equal(gpe_dtype()))
+        throw py::type_error("When passing the 'gpes' argument to make_sgp4_polyjectory() as a NumPy array, the dtype "
+                             "must be 'gpe_dtype', but it is '"
+                             + py::str(a.dtype()).cast<std::string>() + "' instead");
+    if (a.ndim() != 1)
+        throw std::invalid_argument("The array of gpes passed to make_sgp4_polyjectory() must have 1 dimension, but "
+                                    "the number of dimensions is "
+                                    + std::to_string(a.ndim()) + " instead");
+    check_cc_aligned(a, "The array of gpes passed to make_sgp4_polyjectory() must be C contiguous and properly aligned");
+    static_assert(sizeof(mzh_gpe) == 80, "gpe layout");
+    std::vector<mzh_gpe> v((size_t)a.shape(0));
+    if (!v.empty()) std::memcpy(v.data(), a.data(), v.size() * sizeof(mzh_gpe));
+    return v;
+}
+
+// The context's polyjectory as a host polyjectory (statuses and epoch included).
+std::shared_ptr<PjData> pj_from_ctx(mzh_ctx *ctx)
+{
+    uint64_t n = 0, tl = 0, ml = 0;
+    uint32_t order = 0;
+    double maxT = 0, ep[2] = {0, 0};
+    check(mzh_get_polyjectory_info(ctx, &n, &order, &tl, &ml, &maxT, ep), ctx);
+    mzh_host::PjArrays a;
+    a.n_objs = (uint32_t)n;
+    a.order = order;
+    a.maxT = maxT;
+    a.epoch = ep[0];
+    a.epoch2 = ep[1];
+    a.traj.resize(tl);
+    a.time.resize(ml);
+    a.traj_off.resize(n);
+    a.nsteps.resize(n);
+    a.time_off.resize(n);
+    a.status.resize(n);
+    check(mzh_copy_polyjectory(ctx, a.traj.data(), a.time.data(), a.traj_off.data(), a.nsteps.data(),
+                               a.time_off.data(), a.status.data()),
+          ctx);
+    return pj_from_arrays(std::move(a));
+}
+
 int parse_mode(const std::string &m)
 {
     if (m == "fp64") return MZH_MODE_FP64;
@@ -993,6 +1044,64 @@ PYBIND11_MODULE(_core, m)
     pt.def("state_eval", &state_eval, "time"_a, "out"_a.noconvert() = py::none(), "obj_idx"_a = py::none());
     pt.def("state_meval", &state_meval, "time"_a, "out"_a.noconvert() = py::none(), "obj_idx"_a = py::none());
 
+    // ---- polyjectories from SGP4 element sets ---------------------------
+    m.attr("gpe_dtype") = gpe_dtype();
+    m.def(
+        "_make_sgp4_polyjectory",
+        [](const py::array &gpes_, double jd_begin, double jd_end, double reentry_radius, double exit_radius,
+           std::optional<std::filesystem::path> data_dir, bool persist, std::optional<std::filesystem::path> tmpdir,
+           uint32_t order, int device) {
+            if (data_dir && tmpdir)
+                throw std::invalid_argument(
+                    "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time");
+            const auto gpes = gpes_from_array(gpes_);
+            Ctx c(device);
+            {
+                py::gil_scoped_release rel;
+                check(mzh_make_sgp4_polyjectory(c.ctx, gpes.data(), gpes.size(), jd_begin, jd_end, reentry_radius,
+                                                exit_radius, order),
+                      c.ctx);
+            }
+            auto d = pj_from_ctx(c.ctx);
+            if ((data_dir && !data_dir->empty()) || tmpdir || persist) store_pj(*d, data_dir, tmpdir, persist);
+            return Polyjectory{d};
+        },
+        "gpes"_a, "jd_begin"_a, "jd_end"_a, "reentry_radius"_a, "exit_radius"_a, "data_dir"_a, "persist"_a,
+        "tmpdir"_a, "order"_a, "device"_a);
+    m.def(
+        "sgp4_propagate",
+        [](const py::array &gpes_, const py::array_t<double> &tsince_, int device) {
+            // states of every GPE at the minutes since its epoch tsince[g, k]
+            // (or tsince[k] for all): (out [n_gpes][n_times][6], err [n_gpes][n_times])
+            const auto gpes = gpes_from_array(gpes_);
+            const uint64_t ng = gpes.size();
+            py::array_t<double> tsince = numpy().attr("ascontiguousarray")(tsince_);
+            uint64_t nt = 0;
+            std::vector<double> t;
+            if (tsince.ndim() == 1) {
+                nt = (uint64_t)tsince.shape(0);
+                t.resize(ng * nt);
+                for (uint64_t g = 0; g < ng; ++g) std::memcpy(t.data() + g * nt, tsince.data(), nt * sizeof(double));
+            } else if (tsince.ndim() == 2 && (uint64_t)tsince.shape(0) == ng) {
+                nt = (uint64_t)tsince.shape(1);
+                t.assign(tsince.data(), tsince.data() + ng * nt);
+            } else {
+                throw std::invalid_argument("sgp4_propagate(): the times must be a 1-D array or a 2-D array with one "
+                                            "row per GPE");
+            }
+            py::array_t<double> out(std::vector<py::ssize_t>{(py::ssize_t)ng, (py::ssize_t)nt, 6});
+            py::array_t<int32_t> err(std::vector<py::ssize_t>{(py::ssize_t)ng, (py::ssize_t)nt});
+            Ctx c(device);
+            {
+                py::gil_scoped_release rel;
+                check(mzh_sgp4_propagate(c.ctx, gpes.data(), ng, t.data(), nt, out.mutable_data(),
+                                         err.mutable_data()),
+                      c.ctx);
+            }
+            return py::make_tuple(out, err);
+        },
+        "gpes"_a, "tsince"_a, "device"_a = 0);
+
     // tmpdir getter/setter (core.cpp:63-86): None stands for no global dir
     m.def("get_tmpdir", []() -> std::optional<std::filesystem::path> {
         auto p = mzh_host::get_global_tmpdir();
@@ -1097,6 +1206,26 @@ PYBIND11_MODULE(_core, m)
                 check(st, c.ctx);
             },
             "tm"_a, "time_mode"_a, "sel"_a, "n_rows"_a, "n_times"_a, "out"_a)
+        .def(
+            "make_sgp4",
+            [](Ctx &c, const py::array &gpes_, double jd_begin, double jd_end, double reentry_radius,
+               double exit_radius, uint32_t order) {
+                // generate the polyjectory of the GPEs into this context, where it
+                // stays for screening; returns the per-satellite statuses
+                const auto gpes = gpes_from_array(gpes_);
+                {
+                    py::gil_scoped_release rel;
+                    check(mzh_make_sgp4_polyjectory(c.ctx, gpes.data(), gpes.size(), jd_begin, jd_end,
+                                                    reentry_radius, exit_radius, order),
+                          c.ctx);
+                }
+                c.pj = pj_from_ctx(c.ctx);
+                py::array_t<int32_t> status((py::ssize_t)c.pj->n_objs);
+                std::copy(c.pj->status.begin(), c.pj->status.end(), status.mutable_data());
+                return status;
+            },
+            "gpes"_a, "jd_begin"_a, "jd_end"_a, "reentry_radius"_a = 0.0,
+            "exit_radius"_a = std::numeric_limits<double>::infinity(), "order"_a = 8)
         .def("device_buffer",
              [](const std::shared_ptr<Ctx> &c, uint64_t nbytes) { return std::make_shared<DeviceBuffer>(c, nbytes); },
              "nbytes"_a)

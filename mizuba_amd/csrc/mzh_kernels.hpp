@@ -111,6 +111,60 @@ hipError_t launch_conj_order(const mzh_conj *in, const uint32_t *in_step, uint64
                              const uint64_t *cnt, uint32_t *cursor, uint32_t *pos, uint32_t *inv, mzh_conj *out,
                              uint32_t n_steps, hipStream_t stream);
 
+// ---- SGP4 (near-earth model) ------------------------------------------------
+constexpr int kSgp4DeepSpace = MZH_SGP4_DEEP_SPACE;
+// The initialised near-earth model of one element set.
+struct Sgp4Sat {
+    double bstar, ecco, argpo, inclo, mo, nodeo, no_unkozai;
+    double aycof, con41, cc1, cc4, cc5, d2, d3, d4, delmo, eta, argpdot, omgcof, sinmao, t2cof, t3cof, t4cof, t5cof,
+        x1mth2, x7thm1, mdot, nodedot, xlcof, xmcof, nodecf;
+    int isimp, error, deep;
+};
+hipError_t launch_sgp4_propagate(const mzh_gpe *gpes, uint64_t n_gpes, const double *tsince, uint64_t n_times,
+                                 double *out, int32_t *err, hipStream_t stream);
+// The generator of polyjectories from GPEs. A satellite's trajectory is a
+// sequence of segments, one per GPE (from the later of the range's beginning
+// and the GPE's epoch to the next GPE's epoch or the range's end); a segment
+// is cut into nominal steps of a twelfth of the orbital period, each of which
+// is interpolated independently (bisected where the interpolation is not
+// accurate enough), one lane per nominal step.
+struct Sgp4Seg {
+    double t_b, t_e;    // the segment, days from the polyjectory's epoch
+    double eoff;        // the GPE's epoch, days from the polyjectory's epoch
+    double h0;          // nominal step, days (k_sgp4_seg_init)
+    uint64_t step0;     // index of its first nominal step (host)
+    uint32_t n_steps;   // nominal steps (k_sgp4_seg_init; 1 if status != 0)
+    uint32_t sat, gpe;  // satellite and GPE indices
+    int32_t status;     // initialisation status (0, an SGP4 error code, or deep space)
+};
+struct Sgp4Gen {
+    const mzh_gpe *gpes;
+    Sgp4Seg *segs;
+    Sgp4Sat *sats;                 // [n_segs] the initialised model of each segment
+    uint32_t n_segs, n_sats;
+    const uint32_t *sat_seg0;      // [n_sats + 1] segments of each satellite
+    const uint64_t *seg_step0;     // [n_segs + 1] first nominal step of each segment
+    uint64_t n_steps;              // nominal steps in total
+    double reentry_r2, exit_r2;    // squared radii (km^2)
+    int32_t *step_res;             // [n_steps] polynomials of the step (> 0), or -status
+    uint32_t *step_out;            // [n_steps] index of its first polynomial within the satellite, or ~0u
+    uint64_t *sat_npolys;          // [n_sats]
+    int32_t *sat_status;           // [n_sats]
+    double *sat_tlast;             // [n_sats] end of the last polynomial
+    // output (k_sgp4_write)
+    double *traj, *time;
+    const uint64_t *sat_poly0, *sat_time0;
+};
+// Initialise every segment's model, its nominal step and step count.
+hipError_t launch_sgp4_seg_init(const Sgp4Gen &g, hipStream_t stream);
+// Interpolate every nominal step, counting its polynomials (or its status).
+hipError_t launch_sgp4_count(const Sgp4Gen &g, uint32_t order, hipStream_t stream);
+// Per satellite: the first failing step ends the trajectory; offsets of the
+// kept steps, polynomial counts, statuses and end times.
+hipError_t launch_sgp4_scan(const Sgp4Gen &g, hipStream_t stream);
+// Interpolate the kept steps again and write their polynomials and times.
+hipError_t launch_sgp4_write(const Sgp4Gen &g, uint32_t order, hipStream_t stream);
+
 // Polyjectory evaluation: out[r][k][7] for n_rows x n_times evaluations,
 // times and rows as in mzh_state_eval (time_mode = MZH_EVAL_TIME_*). The
 // smallest flat index r * n_times + k of an evaluation with a non-finite time

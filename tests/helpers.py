@@ -109,3 +109,72 @@ def eval_pj_arrays():
     object 0 over [0, 0.9], object 1 over [1.1, 2.0]."""
     tm = np.array([round(0.1 * k, 10) for k in range(21)])
     return [linear_traj(tm[:10], 1), linear_traj(tm[11:], -1)], [tm[:10], tm[11:]]
+
+
+# ---- SGP4 -------------------------------------------------------------------
+
+GPE_FIELDS = ["norad_id", "epoch_jd1", "epoch_jd2", "n0", "e0", "i0", "node0", "omega0", "m0", "bstar"]
+
+
+def sgp4_golden():
+    """The SGP4 fixture (tests/golden/make_sgp4_golden.py): the reference's
+    GPE files with the vendored Vallado SGP4's states."""
+    return np.load(GOLDEN / "sgp4_vallado.npz")
+
+
+def gpe_array(rows):
+    """[n, 10] float64 GPE fields -> a gpe_dtype array."""
+    import mizuba_amd as mz
+
+    rows = np.asarray(rows, dtype=np.float64).reshape(-1, 10)
+    out = np.zeros(len(rows), dtype=mz.gpe_dtype)
+    for k, f in enumerate(GPE_FIELDS):
+        out[f] = rows[:, k]
+    return out
+
+
+def golden_group(name):
+    """The GPEs of one file of the SGP4 fixture, as a gpe_dtype array."""
+    g = sgp4_golden()
+    gi = list(g["groups"]).index(name)
+    return gpe_array(g["gpes"][g["group"] == gi])
+
+
+def _jd_jan1(year):
+    # Julian date of January 1, 0h (the Gregorian calendar formula of
+    # Vallado's jday with month 1, day 1)
+    return 367.0 * year - np.floor(7.0 * (year + np.floor(10.0 / 12.0)) * 0.25) + np.floor(275.0 / 9.0) + 1.0 + 1721013.5
+
+
+def gpe_from_tle(line1, line2, norad_id=None):
+    """A GPE from a two-line element set (the published column layout: epoch
+    year and day at 1:18-32, B* at 1:53-61, inclination, node, eccentricity,
+    argument of perigee, mean anomaly and mean motion at 2:8-63)."""
+    yy = int(line1[18:20])
+    year = 2000 + yy if yy < 57 else 1900 + yy
+    doy = float(line1[20:32])
+    b = line1[53:61]
+    bstar = float(b[0] + "0." + b[1:6].strip()) * 10.0 ** int(b[6:8])
+    deg = np.pi / 180.0
+    row = [
+        float(int(line1[2:7]) if norad_id is None else norad_id),
+        _jd_jan1(year) + np.floor(doy) - 1.0,
+        doy - np.floor(doy),
+        float(line2[52:63]) * 2.0 * np.pi / 1440.0,
+        float("0." + line2[26:33].strip()),
+        float(line2[8:16]) * deg,
+        float(line2[17:25]) * deg,
+        float(line2[34:42]) * deg,
+        float(line2[43:51]) * deg,
+        bstar,
+    ]
+    return row
+
+
+def poly_eval(cfs, x):
+    """Horner of the [op1, 7] coefficients of one step at x -> [len(x), 7]."""
+    x = np.asarray(x, dtype=np.float64)
+    r = np.zeros((len(x), 7)) + cfs[-1]
+    for k in range(cfs.shape[0] - 2, -1, -1):
+        r = cfs[k] + r * x[:, None]
+    return r
