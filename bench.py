@@ -216,7 +216,6 @@ def state_eval_workload(args):
     results resident in HBM (mzh_state_eval_device). The oracle's evaluation
     is timed on a sample of the objects."""
     import mizuba_amd as mz
-    from oracle import pyoracle as po
 
     n_t = 2048
     pj = mz.make_synthetic_polyjectory(args.n_objs, args.days, seed=args.seed, frac_high=0.2, order=8)
@@ -243,14 +242,10 @@ def state_eval_workload(args):
     dt = (time.perf_counter() - t) / steps
     kavg = sum(kt) / len(kt)
     n_eval = N * n_t
-    # parity on a sample of rows against the oracle, bit for bit (NaN included)
-    traj, time_, traj_off, nsteps, time_off = pj._flat()
-    flat = po.Flat.from_arrays(pj.poly_order, traj, time_, nsteps, traj_off, time_off, pj.maxT)
+    # a sample of rows again through a selector, and a spot check of the full
+    # run's output for the same rows
     rng = np.random.default_rng(args.seed)
     rows = np.sort(rng.choice(N, 64, replace=False)).astype(np.uint64)
-    st, ref, _ = po.state_meval(flat, tgrid, po.TIME_SHARED, sel=rows, n_times=n_t)
-    # the sampled rows again through a selector, and a spot check of the full
-    # run's output for the same rows
     d_sel = scr.device_buffer(rows.nbytes)
     d_sel.write(rows)
     d_smp = scr.device_buffer(len(rows) * n_t * 7 * 8)
@@ -261,7 +256,6 @@ def state_eval_workload(args):
     d_out.read(full)
     same_rows = np.array_equal(full[rows.astype(np.int64)].view(np.uint64), got.view(np.uint64))
     del full
-    identical = bool(st == 0 and same_rows and np.array_equal(got.view(np.uint64), ref.view(np.uint64)))
     alg = eval_bytes_per_launch(pj, tgrid)
     traffic = eval_hbm_traffic()
     out = {
@@ -279,7 +273,7 @@ def state_eval_workload(args):
         "data": "synthetic (seeded J2-secular Keplerian catalog, order-8 Chebyshev-fitted polyjectory)",
         "config": {"workload": f"{N} objects x {args.days:g} day, {n_t} shared times", "n_objs": N, "n_times": n_t,
                    "seed": args.seed},
-        "identical_to_oracle_sample": identical,
+        "selector_rows_match_full_run": bool(same_rows),
         "roofline": {
             "kernel": "k_state_eval",
             "bound": "hbm",
@@ -293,6 +287,14 @@ def state_eval_workload(args):
         },
     }
     if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+
+        traj, time_, traj_off, nsteps, time_off = pj._flat()
+        flat = po.Flat.from_arrays(pj.poly_order, traj, time_, nsteps, traj_off, time_off, pj.maxT)
+        # the checker: the sampled rows against the oracle, bit for bit (NaN included)
+        st, ref, _ = po.state_meval(flat, tgrid, po.TIME_SHARED, sel=rows, n_times=n_t)
+        out["identical_to_oracle_sample"] = bool(st == 0 and np.array_equal(got.view(np.uint64),
+                                                                            ref.view(np.uint64)))
         n_s = N  # the whole workload: about 2 s of one core
         t = time.perf_counter()
         po.state_meval(flat, tgrid, po.TIME_SHARED, sel=np.arange(n_s, dtype=np.uint64), n_times=n_t)
@@ -300,6 +302,178 @@ def state_eval_workload(args):
         out["cpu_baseline"] = {"value": round(n_s * n_t / dt_cpu, 1), "unit": "evaluations/s", "cores": 1,
                                "kind": "port",
                                "sample": f"first {n_s} objects x {n_t} times, wall {dt_cpu:.2f}s"}
+    print(json.dumps(out))
+
+
+def synthetic_gpes(n, jd_begin, seed):
+    """A seeded near-earth catalog: mean motions of 11-16 rev/day (most above
+    14), small eccentricities (a few up to 0.2, perigee above 200 km
+    altitude), inclinations up to 108 deg, uniform angles, B* around 1e-4 and
+    epochs up to 2 days before jd_begin; one GPE per satellite."""
+    import mizuba_amd as mz
+
+    rng = np.random.default_rng(seed)
+    g = np.zeros(n, dtype=mz.gpe_dtype)
+    g["norad_id"] = np.arange(1, n + 1)
+    rev = np.where(rng.uniform(size=n) < 0.7, rng.uniform(14.0, 16.0, n), rng.uniform(11.0, 14.0, n))
+    n0 = rev * 2.0 * np.pi / 1440.0
+    a = (398600.8 / (n0 / 60.0) ** 2) ** (1.0 / 3.0)
+    e_max = np.clip(1.0 - 6578.0 / a, 1e-4, 0.2)
+    e = np.where(rng.uniform(size=n) < 0.95, rng.exponential(0.003, n), rng.uniform(0.0, 0.2, n))
+    g["e0"] = np.clip(e, 1e-5, e_max)
+    g["n0"] = n0
+    g["i0"] = rng.uniform(0.0, 0.6 * np.pi, n)
+    for f in ("node0", "omega0", "m0"):
+        g[f] = rng.uniform(0.0, 2.0 * np.pi, n)
+    g["bstar"] = 1e-4 * rng.lognormal(0.0, 1.0, n)
+    ep = jd_begin - rng.uniform(0.0, 2.0, n)
+    g["epoch_jd1"] = np.floor(ep - 0.5) + 0.5
+    g["epoch_jd2"] = ep - g["epoch_jd1"]
+    return g
+
+
+def ref_sgp4_lib():
+    """The reference's vendored SGP4 built from its sources into oracle/_ref
+    (None where it was not built)."""
+    import ctypes
+
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_sgp4.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.ref_sgp4_propagate.argtypes = [dp, dp, ctypes.c_int64, dp, ctypes.POINTER(ctypes.c_int)]
+    lib.ref_sgp4_propagate.restype = ctypes.c_int
+    return lib
+
+
+def ref_sgp4(lib, gpe, tsince):
+    import ctypes
+
+    dp = ctypes.POINTER(ctypes.c_double)
+    row = np.ascontiguousarray([float(gpe[f]) for f in gpe.dtype.names], dtype=np.float64)
+    ts = np.ascontiguousarray(tsince, dtype=np.float64)
+    out = np.zeros((len(ts), 6))
+    err = np.zeros(len(ts), dtype=np.int32)
+    rc = lib.ref_sgp4_propagate(row.ctypes.data_as(dp), ts.ctypes.data_as(dp), len(ts), out.ctypes.data_as(dp),
+                                err.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return rc, out, err
+
+
+def sgp4_flops_per_launch():
+    # FP64 operations of one k_sgp4_count launch on this workload, from the
+    # PMC profile (the counters of the VALU's FP64 instructions x 64 lanes)
+    path = os.path.join(ROOT, "profiles", "sgp4_30k_1day.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 3 and cells[0] == "fp64_ops_per_launch":
+            try:
+                return float(cells[1])
+            except ValueError:
+                return None
+    return None
+
+
+def sgp4_workload(args):
+    """Polyjectory generation from GPEs (make_sgp4_polyjectory) on the GPU:
+    a seeded near-earth catalog of --n-objs satellites over --days days, the
+    polyjectory left in HBM for screening. A sample of the generated
+    trajectories is checked against the reference's vendored SGP4."""
+    import mizuba_amd as mz
+
+    jd0 = 2460496.5
+    N = args.n_objs
+    gpes = synthetic_gpes(N, jd0, args.seed)
+    scr = mz.Screener(0)
+    for _ in range(args.warmup):
+        scr.make_sgp4(gpes, jd0, jd0 + args.days)
+    steps = max(args.steps, 1)
+    wall, kt = [], []
+    for _ in range(steps):
+        t = time.perf_counter()
+        status = scr.make_sgp4(gpes, jd0, jd0 + args.days)
+        wall.append(time.perf_counter() - t)
+        kt.append(scr.sgp4_times())
+    dt = sum(wall) / steps
+    info = scr.pj_info()
+    stage = {k: sum(d[k] for d in kt) / steps for k in kt[0]}
+    n_steps = int(info["n_steps"])
+    # a sample of 32 satellites, copied out and evaluated at 16 random times,
+    # against SGP4 (the GPU propagator here; the reference's in the CPU leg)
+    rng = np.random.default_rng(args.seed)
+    smp = np.sort(rng.choice(N, 32, replace=False))
+    pj = mz.make_sgp4_polyjectory(gpes[smp], jd0, jd0 + args.days)[0]
+    tm = np.sort(rng.uniform(0.0, args.days * 0.999, 16))
+    ev = pj.state_meval(tm)
+
+    def sample_err(prop):
+        worst_p = worst_v = 0.0
+        for k, i in enumerate(smp):
+            if pj.status[k] != 0:
+                continue
+            e_off = (gpes["epoch_jd1"][i] - jd0) + gpes["epoch_jd2"][i]
+            ref, err = prop(i, (tm - e_off) * 1440.0)
+            ok = err == 0
+            worst_p = max(worst_p, float(np.abs(ev[k, ok, :3] - ref[ok, :3]).max()))
+            worst_v = max(worst_v, float(np.abs(ev[k, ok, 3:6] - ref[ok, 3:]).max()))
+        return {"position_km": worst_p, "velocity_km_s": worst_v}
+
+    def gpu_prop(i, ts):
+        ref, err = mz.sgp4_propagate(gpes[i : i + 1], ts)
+        return ref[0], err[0]
+
+    flops = sgp4_flops_per_launch()
+    kc = stage["count"]
+    out = {
+        "metric": "SGP4 polyjectory generation throughput (make_sgp4_polyjectory)",
+        "value": round(N * args.days / dt, 1),
+        "unit": "satellite-days/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded near-earth GPE catalog, one GPE per satellite)",
+        "config": {"workload": f"{N} near-earth satellites x {args.days:g} day, order 8, data left in HBM",
+                   "n_sats": N, "days": args.days, "seed": args.seed},
+        "trajectory_steps": n_steps,
+        "satellites_with_data": int(np.sum(status == 0)),
+        "stage_ms": {k: round(v * 1e3, 3) for k, v in stage.items()},
+        "sample_max_err_vs_gpu_sgp4": sample_err(gpu_prop),
+        "roofline": {
+            "kernel": "k_sgp4_count",
+            "bound": "valu-fp64",
+            "achieved": None if flops is None else round(flops / kc / 1e12, 2),
+            "peak": 78.6,
+            "unit": "TFLOP/s",
+            "frac": None if flops is None else round(flops / kc / 1e12 / 78.6, 4),
+            "traffic": None,
+            "avg_launch_ms": round(kc * 1e3, 3),
+        },
+    }
+    lib = None if args.no_cpu_baseline else ref_sgp4_lib()
+    if lib is not None:
+        # the checker: the sample against the reference's SGP4
+        out["sample_max_err_vs_reference_sgp4"] = sample_err(lambda i, ts: ref_sgp4(lib, gpes[i], ts)[1:])
+        # the reference's SGP4 on one core, the evaluations a CPU generator with
+        # the same steps would make (19 per step: the step's beginning, 9 nodes
+        # and 9 error points), on a sample of satellites
+        per_sat = n_steps / max(int(np.sum(status == 0)), 1)
+        n_s = 1000
+        n_eval = int(per_sat * 19)
+        t = time.perf_counter()
+        for i in range(n_s):
+            ref_sgp4(lib, gpes[i], np.linspace(0.0, args.days * 1440.0, n_eval))
+        dt_cpu = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(n_s * args.days / dt_cpu, 1), "unit": "satellite-days/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": f"{n_s} satellites x {n_eval} SGP4 evaluations of the reference's vendored "
+                                         f"SGP4 (19 per step), interpolation not counted, wall {dt_cpu:.2f}s"}
     print(json.dumps(out))
 
 
@@ -314,7 +488,7 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=0, help="conjunction steps timed on the CPU (0: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0, help="only run the PMC copy calibration")
-    ap.add_argument("--workload", choices=("headline", "batch64", "state_eval"), default="headline")
+    ap.add_argument("--workload", choices=("headline", "batch64", "state_eval", "sgp4"), default="headline")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -330,6 +504,9 @@ def main():
         return
     if args.workload == "batch64":
         batch_workload(args)
+        return
+    if args.workload == "sgp4":
+        sgp4_workload(args)
         return
     if args.workload == "state_eval":
         state_eval_workload(args)

@@ -166,9 +166,15 @@ int mzh_sgp4_propagate(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, const
 int mzh_make_sgp4_polyjectory(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double jd_begin, double jd_end,
                               double reentry_radius, double exit_radius, uint32_t order);
 
+/* Device times (seconds, HIP events on the context's stream) of the kernels
+ * of the last mzh_make_sgp4_polyjectory: segment set-up, count, scan, write
+ * and the transposed copy for screening. No reference counterpart. */
+int mzh_get_sgp4_times(const mzh_ctx *ctx, double seconds[5]);
+
 /* The context's polyjectory: sizes, then the arrays (host pointers) in the
  * layout of mzh_upload_polyjectory, the per-object status (0 for uploaded
- * polyjectories) and the epoch of a generated one (TAI Julian date, two parts). */
+ * polyjectories) and the epoch of a generated one (TAI Julian date, two parts).
+ * A NULL destination skips that array. */
 int mzh_get_polyjectory_info(const mzh_ctx *ctx, uint64_t *n_objs, uint32_t *order, uint64_t *traj_len,
                              uint64_t *time_len, double *maxT, double epoch[2]);
 int mzh_copy_polyjectory(mzh_ctx *ctx, double *traj, double *time, uint64_t *traj_off, uint64_t *nsteps,

@@ -149,6 +149,10 @@ struct mzh_ctx {
     DevBuf ev_tm, ev_sel, ev_out, ev_bad;
     hipEvent_t ev_eval[2] = {};
     double eval_s = 0;  // device time of the last evaluation kernel
+    // SGP4 generation: device times of its kernels (segment set-up, count,
+    // scan, write, transpose) in the last mzh_make_sgp4_polyjectory
+    hipEvent_t ev_sgp4[8] = {};
+    double sgp4_s[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev[12] = {};
     double stage_s[MZH_STAGE_COUNT] = {};
     uint64_t h_np[MZH_NP_COUNT] = {0, 0, 0, 0, 0, 0, 0};
@@ -637,6 +641,7 @@ int mzh_create(int device, mzh_ctx **out)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     for (int k = 0; k < 12 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev[k]);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev_eval[k]);
+    for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev_sgp4[k]);
     if (e != hipSuccess) {
         g_global_error = std::string("HIP error while creating a context: ") + hipGetErrorString(e);
         delete ctx;
@@ -654,6 +659,8 @@ int mzh_destroy(mzh_ctx *ctx)
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->ev_eval)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->ev_sgp4)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -895,7 +902,10 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     g.sat_seg0 = d_satseg.as<const uint32_t>();
     g.reentry_r2 = reentry_radius * reentry_radius;
     g.exit_r2 = std::isinf(exit_radius) ? std::numeric_limits<double>::infinity() : exit_radius * exit_radius;
+    hipEvent_t *E = ctx->ev_sgp4;
+    MZH_HIP(hipEventRecord(E[0], st));
     MZH_HIP(launch_sgp4_seg_init(g, st));
+    MZH_HIP(hipEventRecord(E[1], st));
     MZH_HIP(hipMemcpyAsync(segs.data(), d_segs.p, sizeof(Sgp4Seg) * n_segs, hipMemcpyDeviceToHost, st));
     MZH_HIP(hipStreamSynchronize(st));
     // nominal steps of each segment (none for an empty stretch of a valid GPE)
@@ -921,8 +931,11 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     g.sat_npolys = d_np.as<uint64_t>();
     g.sat_status = d_status.as<int32_t>();
     g.sat_tlast = d_tlast.as<double>();
+    MZH_HIP(hipEventRecord(E[2], st));
     MZH_HIP(launch_sgp4_count(g, order, st));
+    MZH_HIP(hipEventRecord(E[3], st));
     MZH_HIP(launch_sgp4_scan(g, st));
+    MZH_HIP(hipEventRecord(E[4], st));
     std::vector<uint64_t> nsteps(n_sats);
     std::vector<int32_t> status(n_sats);
     std::vector<double> t_last(n_sats);
@@ -962,9 +975,20 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     g.time = ctx->time.as<double>();
     g.sat_poly0 = ctx->step0.as<const uint64_t>();
     g.sat_time0 = ctx->time_off.as<const uint64_t>();
+    MZH_HIP(hipEventRecord(E[5], st));
     MZH_HIP(launch_sgp4_write(g, order, st));
+    MZH_HIP(hipEventRecord(E[6], st));
     MZH_HIP(launch_transpose_cf4(ctx->traj.as<double>(), ctx->cf4.as<double>(), steps_total, order, st));
+    MZH_HIP(hipEventRecord(E[7], st));
     MZH_HIP(hipStreamSynchronize(st));
+    {
+        const int pairs[5][2] = {{0, 1}, {2, 3}, {3, 4}, {5, 6}, {6, 7}};
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0;
+            MZH_HIP(hipEventElapsedTime(&ms, E[pairs[k][0]], E[pairs[k][1]]));
+            ctx->sgp4_s[k] = ms * 1e-3;
+        }
+    }
     ctx->n_objs = n_sats;
     ctx->order = order;
     ctx->maxT = maxT;
@@ -1177,6 +1201,13 @@ int mzh_copy_from_device(mzh_ctx *ctx, void *dst, const void *src, uint64_t byte
         MZH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
         MZH_HIP(hipStreamSynchronize(ctx->stream));
     });
+}
+
+int mzh_get_sgp4_times(const mzh_ctx *ctx, double seconds[5])
+{
+    if (!ctx || !seconds) return MZH_EINVAL;
+    for (int k = 0; k < 5; ++k) seconds[k] = ctx->sgp4_s[k];
+    return MZH_OK;
 }
 
 int mzh_get_eval_time(const mzh_ctx *ctx, double *seconds)

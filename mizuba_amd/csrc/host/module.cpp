@@ -1219,9 +1219,20 @@ PYBIND11_MODULE(_core, m)
                                                     reentry_radius, exit_radius, order),
                           c.ctx);
                 }
-                c.pj = pj_from_ctx(c.ctx);
-                py::array_t<int32_t> status((py::ssize_t)c.pj->n_objs);
-                std::copy(c.pj->status.begin(), c.pj->status.end(), status.mutable_data());
+                // the data stay in HBM: only the sizes and statuses come back
+                auto d = std::make_shared<PjData>();
+                uint64_t n = 0, tl = 0, ml = 0;
+                double ep[2] = {0, 0};
+                check(mzh_get_polyjectory_info(c.ctx, &n, &d->order, &tl, &ml, &d->maxT, ep), c.ctx);
+                d->n_objs = (uint32_t)n;
+                d->epoch = ep[0];
+                d->epoch2 = ep[1];
+                d->status.resize(n);
+                check(mzh_copy_polyjectory(c.ctx, nullptr, nullptr, nullptr, nullptr, nullptr, d->status.data()),
+                      c.ctx);
+                c.pj = d;
+                py::array_t<int32_t> status((py::ssize_t)n);
+                std::copy(d->status.begin(), d->status.end(), status.mutable_data());
                 return status;
             },
             "gpes"_a, "jd_begin"_a, "jd_end"_a, "reentry_radius"_a = 0.0,
@@ -1229,6 +1240,30 @@ PYBIND11_MODULE(_core, m)
         .def("device_buffer",
              [](const std::shared_ptr<Ctx> &c, uint64_t nbytes) { return std::make_shared<DeviceBuffer>(c, nbytes); },
              "nbytes"_a)
+        .def("pj_info",
+             [](Ctx &c) {
+                 // sizes of the context's polyjectory
+                 uint64_t n = 0, tl = 0, ml = 0;
+                 uint32_t order = 0;
+                 double maxT = 0, ep[2] = {0, 0};
+                 check(mzh_get_polyjectory_info(c.ctx, &n, &order, &tl, &ml, &maxT, ep), c.ctx);
+                 py::dict d;
+                 d["n_objs"] = n;
+                 d["order"] = order;
+                 d["n_steps"] = tl / (7u * (order + 1u));
+                 d["maxT"] = maxT;
+                 d["epoch"] = py::make_tuple(ep[0], ep[1]);
+                 return d;
+             })
+        .def("sgp4_times",
+             [](Ctx &c) {
+                 double t[5];
+                 check(mzh_get_sgp4_times(c.ctx, t), c.ctx);
+                 py::dict d;
+                 const char *names[5] = {"seg_init", "count", "scan", "write", "transpose"};
+                 for (int k = 0; k < 5; ++k) d[names[k]] = t[k];
+                 return d;
+             })
         .def("eval_time",
              [](Ctx &c) {
                  double s = 0;
