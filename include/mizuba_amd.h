@@ -135,14 +135,12 @@ typedef struct mzh_gpe {
 #define MZH_SGP4_REENTRY 11    /* below the reentry radius */
 #define MZH_SGP4_EXIT 12       /* at or beyond the exit radius */
 #define MZH_SGP4_INTERPOLATION 14 /* more than 20 pieces needed for one nominal step */
-#define MZH_SGP4_DEEP_SPACE 20 /* deep-space element set: not propagated by this build */
 
-/* SGP4 (near-earth model, WGS-72) states of n_gpes element sets at n_times
+/* SGP4 / SDP4 (WGS-72, AFSPC mode) states of n_gpes element sets at n_times
  * times each: out[g][k] = {x, y, z, vx, vy, vz} (km, km/s, TEME) at tsince[g][k]
- * minutes from the epoch of gpes[g], err[g][k] = the SGP4 error code (0: none,
- * MZH_SGP4_DEEP_SPACE for a deep-space set; an error of the propagation to the
- * epoch, which the published initialisation performs, is reported for every
- * time). The arithmetic is that of the published SGP4 code, including its
+ * minutes from the epoch of gpes[g], err[g][k] = the SGP4 error code (0: none;
+ * an error of the propagation to the epoch, which the published
+ * initialisation performs, is reported for every time). The arithmetic is that of the published SGP4 code, including its
  * stopping rule for Kepler's equation. Host pointers. No reference
  * counterpart as a function: it is the propagation step of
  * make_sgp4_polyjectory (make_sgp4_polyjectory.cpp:722-758, 822-835). */
@@ -160,7 +158,7 @@ int mzh_sgp4_propagate(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, const
  * within 1e-7 km of SGP4 between the nodes (pieces shorter than 5 s are kept
  * as they are; more than 20 pieces is MZH_SGP4_INTERPOLATION). A satellite's
  * trajectory ends at its first failing step, whose code becomes its status.
- * Deep-space element sets get MZH_SGP4_DEEP_SPACE and no data. Time in days
+ * Deep-space element sets use SDP4's lunar-solar and resonance terms. Time in days
  * from jd_begin in TAI; the interpolated model solves Kepler's equation to
  * convergence. order: the polynomial order (8 in this build). */
 int mzh_make_sgp4_polyjectory(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double jd_begin, double jd_end,

@@ -14,10 +14,9 @@ Accepted forms of `gpes` (mizuba/_sgp4_polyjectory.py:150-200):
   `jdsatepoch`, `jdsatepochF`, `no_kozai`, `ecco`, `inclo`, `nodeo`,
   `argpo`, `mo`, `bstar`).
 
-Scope of the GPU generator: near-earth satellites (orbital period below
-225 minutes) are propagated with SGP4 (WGS-72, AFSPC mode) and interpolated
-with order-8 polynomials; deep-space satellites get status
-`sgp4_pj_status.DEEP_SPACE` and no trajectory.
+The GPU generator propagates near-earth and deep-space element sets with
+SGP4 / SDP4 (WGS-72, AFSPC mode) and interpolates them with order-8
+polynomials.
 """
 
 from __future__ import annotations
@@ -37,8 +36,6 @@ class sgp4_pj_status(_enum.IntEnum):
     """Per-satellite status of an SGP4 polyjectory (mizuba/_sgp4_polyjectory.py:31-47).
 
     1-6 are the SGP4 propagator's own error codes; 10-14 are the generator's.
-    DEEP_SPACE is this library's own: a satellite with an orbital period of
-    225 minutes or more, which the GPU generator does not propagate.
     """
 
     OK = 0
@@ -52,7 +49,6 @@ class sgp4_pj_status(_enum.IntEnum):
     EXIT = 12
     ERROR_SMALL_STEP = 13
     ERROR_INTERPOLATION = 14
-    DEEP_SPACE = 20
 
 
 _gpe_fields = ["norad_id", "epoch_jd1", "epoch_jd2", "n0", "e0", "i0", "node0", "omega0", "m0", "bstar"]
@@ -150,8 +146,7 @@ def sgp4_propagate(gpes: np.ndarray, tsince, device: int = 0):
 
     `tsince` is 1-D (the same times for every GPE) or 2-D (one row per GPE).
     Returns `(states, errors)`: float64 [n_gpes, n_times, 6] (km, km/s, TEME)
-    and int32 [n_gpes, n_times] SGP4 error codes (0: none; 20: deep space,
-    not propagated).
+    and int32 [n_gpes, n_times] SGP4 error codes (0: none).
     """
     if not isinstance(gpes, np.ndarray) or gpes.dtype != gpe_dtype:
         raise TypeError("sgp4_propagate() requires a NumPy array of gpe_dtype")

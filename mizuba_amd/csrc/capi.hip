@@ -911,7 +911,9 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     // nominal steps of each segment (none for an empty stretch of a valid GPE)
     std::vector<uint64_t> seg_step0(n_segs + 1u);
     uint64_t n_steps = 0;
+    g.any_deep = false;
     for (uint32_t k = 0; k < n_segs; ++k) {
+        if (segs[k].status == 0 && segs[k].deep) g.any_deep = true;
         if (segs[k].status == 0 && !(segs[k].t_e > segs[k].t_b)) segs[k].n_steps = 0;
         segs[k].step0 = n_steps;
         seg_step0[k] = n_steps;

@@ -112,13 +112,25 @@ hipError_t launch_conj_order(const mzh_conj *in, const uint32_t *in_step, uint64
                              uint32_t n_steps, hipStream_t stream);
 
 // ---- SGP4 (near-earth model) ------------------------------------------------
-constexpr int kSgp4DeepSpace = MZH_SGP4_DEEP_SPACE;
-// The initialised near-earth model of one element set.
+// Deep-space (SDP4) terms of an element set with a period of 225 minutes or
+// more: the Greenwich sidereal angle at epoch, the lunar and solar periodic
+// coefficients, the secular rates and the geopotential resonance terms.
+struct Sdp4Terms {
+    double gsto;
+    double e3, ee2, peo, pgho, pho, pinco, plo, se2, se3, sgh2, sgh3, sgh4, sh2, sh3, si2, si3, sl2, sl3, sl4;
+    double xgh2, xgh3, xgh4, xh2, xh3, xi2, xi3, xl2, xl3, xl4, zmol, zmos;
+    double dedt, didt, dmdt, dnodt, domdt;
+    double d2201, d2211, d3210, d3222, d4410, d4422, d5220, d5232, d5421, d5433;
+    double del1, del2, del3, xfact, xlamo;
+    int irez;  // 0 none, 1 one-day (synchronous), 2 half-day resonance
+};
+// The initialised model of one element set.
 struct Sgp4Sat {
     double bstar, ecco, argpo, inclo, mo, nodeo, no_unkozai;
     double aycof, con41, cc1, cc4, cc5, d2, d3, d4, delmo, eta, argpdot, omgcof, sinmao, t2cof, t3cof, t4cof, t5cof,
         x1mth2, x7thm1, mdot, nodedot, xlcof, xmcof, nodecf;
     int isimp, error, deep;
+    Sdp4Terms ds;  // deep != 0 only
 };
 hipError_t launch_sgp4_propagate(const mzh_gpe *gpes, uint64_t n_gpes, const double *tsince, uint64_t n_times,
                                  double *out, int32_t *err, hipStream_t stream);
@@ -135,7 +147,8 @@ struct Sgp4Seg {
     uint64_t step0;     // index of its first nominal step (host)
     uint32_t n_steps;   // nominal steps (k_sgp4_seg_init; 1 if status != 0)
     uint32_t sat, gpe;  // satellite and GPE indices
-    int32_t status;     // initialisation status (0, an SGP4 error code, or deep space)
+    int32_t status;     // initialisation status (0 or an SGP4 error code)
+    int32_t deep;       // deep-space model (k_sgp4_seg_init)
 };
 struct Sgp4Gen {
     const mzh_gpe *gpes;
@@ -145,6 +158,7 @@ struct Sgp4Gen {
     const uint32_t *sat_seg0;      // [n_sats + 1] segments of each satellite
     const uint64_t *seg_step0;     // [n_segs + 1] first nominal step of each segment
     uint64_t n_steps;              // nominal steps in total
+    bool any_deep;                 // a segment uses the deep-space model
     double reentry_r2, exit_r2;    // squared radii (km^2)
     int32_t *step_res;             // [n_steps] polynomials of the step (> 0), or -status
     uint32_t *step_out;            // [n_steps] index of its first polynomial within the satellite, or ~0u

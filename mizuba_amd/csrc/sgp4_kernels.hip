@@ -2,11 +2,12 @@
 // and the construction of polyjectories from them, the upstream producer of
 // the screening path (mizuba/make_sgp4_polyjectory.cpp:478-913).
 //
-// The propagator is the near-earth SGP4 model of Spacetrack Report #3 (Hoots &
-// Roehrich, 1980) with the corrections of Vallado, Crawford, Hujsak & Kelso,
-// "Revisiting Spacetrack Report #3" (AIAA 2006-6753), WGS-72 constants, in
-// FP64. Deep-space element sets (orbital period of 225 minutes or more) are
-// recognised and reported, not propagated.
+// The propagator is SGP4 / SDP4 of Spacetrack Report #3 (Hoots & Roehrich,
+// 1980) with the corrections of Vallado, Crawford, Hujsak & Kelso,
+// "Revisiting Spacetrack Report #3" (AIAA 2006-6753), WGS-72 constants, AFSPC
+// operation mode ('a'), in FP64: the near-earth model, and for element sets
+// with an orbital period of 225 minutes or more the deep-space lunar-solar
+// and resonance terms.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,6 +31,251 @@ constexpr double kJ4 = -0.00000165597;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 __device__ inline double xke() { return 60.0 / sqrt(kRe * kRe * kRe / kMuE); }
+
+constexpr double kPi = kTwoPi / 2.0;
+constexpr double kRptim = 4.37526908801129966e-3;  // Earth's rotation, rad/min
+
+// Deep-space initialisation (Hoots & Roehrich 1980, Spacetrack Report #3,
+// with the corrections of Vallado et al. 2006): the Greenwich sidereal angle
+// at epoch (the AFSPC formula of opsmode 'a'), the lunar and solar terms of
+// the orbit's geometry, their periodic coefficients, the secular rates, and
+// the geopotential resonance terms of one-day (synchronous) and half-day
+// (e >= 0.5) orbits. epoch: days from 1949 December 31 0h.
+__device__ void sdp4_init(double epoch, double eccsq, Sgp4Sat &s)
+{
+    Sdp4Terms &d = s.ds;
+    // sidereal angle at epoch
+    {
+        const double ts70 = epoch - 7305.0;
+        const double ds70 = floor(ts70 + 1.0e-8);
+        const double tfrac = ts70 - ds70;
+        const double c1 = 1.72027916940703639e-2, thgr70 = 1.7321343856509374, fk5r = 5.07551419432269442e-15;
+        double g = fmod(thgr70 + c1 * ds70 + (c1 + kTwoPi) * tfrac + ts70 * ts70 * fk5r, kTwoPi);
+        if (g < 0.0) g += kTwoPi;
+        d.gsto = g;
+    }
+    // lunar and solar terms at epoch
+    const double zes = 0.01675, zel = 0.05490, zns = 1.19459e-5, znl = 1.5835218e-4;
+    const double nm = s.no_unkozai, em = s.ecco;
+    const double snodm = sin(s.nodeo), cnodm = cos(s.nodeo);
+    const double sinomm = sin(s.argpo), cosomm = cos(s.argpo);
+    const double sinim = sin(s.inclo), cosim = cos(s.inclo);
+    const double emsq = em * em, betasq = 1.0 - emsq, rtemsq = sqrt(betasq);
+    d.peo = d.pinco = d.plo = d.pgho = d.pho = 0.0;
+    const double day = epoch + 18261.5;
+    const double xnodce = fmod(4.5236020 - 9.2422029e-4 * day, kTwoPi);
+    const double stem = sin(xnodce), ctem = cos(xnodce);
+    const double zcosil = 0.91375164 - 0.03568096 * ctem;
+    const double zsinil = sqrt(1.0 - zcosil * zcosil);
+    const double zsinhl = 0.089683511 * stem / zsinil;
+    const double zcoshl = sqrt(1.0 - zsinhl * zsinhl);
+    const double gam = 5.8351514 + 0.0019443680 * day;
+    double zx = 0.39785416 * stem / zsinil;
+    const double zy = zcoshl * ctem + 0.91744867 * zsinhl * stem;
+    zx = atan2(zx, zy);
+    zx = gam + zx - xnodce;
+    const double zcosgl = cos(zx), zsingl = sin(zx);
+    // the sun first (k = 0), then the moon (k = 1)
+    double S[2][7], Z[2][12];  // s1..s7; z1 z2 z3 z11 z12 z13 z21 z22 z23 z31 z32 z33
+    double zcosg = 0.1945905, zsing = -0.98088458, zcosi = 0.91744867, zsini = 0.39785416;
+    double zcosh = cnodm, zsinh = snodm, cc = 2.9864797e-6;
+    const double xnoi = 1.0 / nm;
+    for (int k = 0; k < 2; ++k) {
+        const double a1 = zcosg * zcosh + zsing * zcosi * zsinh;
+        const double a3 = -zsing * zcosh + zcosg * zcosi * zsinh;
+        const double a7 = -zcosg * zsinh + zsing * zcosi * zcosh;
+        const double a8 = zsing * zsini;
+        const double a9 = zsing * zsinh + zcosg * zcosi * zcosh;
+        const double a10 = zcosg * zsini;
+        const double a2 = cosim * a7 + sinim * a8;
+        const double a4 = cosim * a9 + sinim * a10;
+        const double a5 = -sinim * a7 + cosim * a8;
+        const double a6 = -sinim * a9 + cosim * a10;
+        const double x1 = a1 * cosomm + a2 * sinomm;
+        const double x2 = a3 * cosomm + a4 * sinomm;
+        const double x3 = -a1 * sinomm + a2 * cosomm;
+        const double x4 = -a3 * sinomm + a4 * cosomm;
+        const double x5 = a5 * sinomm, x6 = a6 * sinomm, x7 = a5 * cosomm, x8 = a6 * cosomm;
+        double *z = Z[k], *sv = S[k];
+        z[9] = 12.0 * x1 * x1 - 3.0 * x3 * x3;
+        z[10] = 24.0 * x1 * x2 - 6.0 * x3 * x4;
+        z[11] = 12.0 * x2 * x2 - 3.0 * x4 * x4;
+        z[0] = 3.0 * (a1 * a1 + a2 * a2) + z[9] * emsq;
+        z[1] = 6.0 * (a1 * a3 + a2 * a4) + z[10] * emsq;
+        z[2] = 3.0 * (a3 * a3 + a4 * a4) + z[11] * emsq;
+        z[3] = -6.0 * a1 * a5 + emsq * (-24.0 * x1 * x7 - 6.0 * x3 * x5);
+        z[4] = -6.0 * (a1 * a6 + a3 * a5) + emsq * (-24.0 * (x2 * x7 + x1 * x8) - 6.0 * (x3 * x6 + x4 * x5));
+        z[5] = -6.0 * a3 * a6 + emsq * (-24.0 * x2 * x8 - 6.0 * x4 * x6);
+        z[6] = 6.0 * a2 * a5 + emsq * (24.0 * x1 * x5 - 6.0 * x3 * x7);
+        z[7] = 6.0 * (a4 * a5 + a2 * a6) + emsq * (24.0 * (x2 * x5 + x1 * x6) - 6.0 * (x4 * x7 + x3 * x8));
+        z[8] = 6.0 * a4 * a6 + emsq * (24.0 * x2 * x6 - 6.0 * x4 * x8);
+        z[0] = z[0] + z[0] + betasq * z[9];
+        z[1] = z[1] + z[1] + betasq * z[10];
+        z[2] = z[2] + z[2] + betasq * z[11];
+        sv[2] = cc * xnoi;
+        sv[1] = -0.5 * sv[2] / rtemsq;
+        sv[3] = sv[2] * rtemsq;
+        sv[0] = -15.0 * em * sv[3];
+        sv[4] = x1 * x3 + x2 * x4;
+        sv[5] = x2 * x3 + x1 * x4;
+        sv[6] = x2 * x4 - x1 * x3;
+        // the moon's geometry for the second pass
+        zcosg = zcosgl;
+        zsing = zsingl;
+        zcosi = zcosil;
+        zsini = zsinil;
+        zcosh = zcoshl * cnodm + zsinhl * snodm;
+        zsinh = snodm * zcoshl - cnodm * zsinhl;
+        cc = 4.7968065e-7;
+    }
+    d.zmol = fmod(4.7199672 + 0.22997150 * day - gam, kTwoPi);
+    d.zmos = fmod(6.2565837 + 0.017201977 * day, kTwoPi);
+    const double *ss = S[0], *sz = Z[0], *sl = S[1], *zl = Z[1];
+    // solar periodic coefficients
+    d.se2 = 2.0 * ss[0] * ss[5];
+    d.se3 = 2.0 * ss[0] * ss[6];
+    d.si2 = 2.0 * ss[1] * sz[4];
+    d.si3 = 2.0 * ss[1] * (sz[5] - sz[3]);
+    d.sl2 = -2.0 * ss[2] * sz[1];
+    d.sl3 = -2.0 * ss[2] * (sz[2] - sz[0]);
+    d.sl4 = -2.0 * ss[2] * (-21.0 - 9.0 * emsq) * zes;
+    d.sgh2 = 2.0 * ss[3] * sz[10];
+    d.sgh3 = 2.0 * ss[3] * (sz[11] - sz[9]);
+    d.sgh4 = -18.0 * ss[3] * zes;
+    d.sh2 = -2.0 * ss[1] * sz[7];
+    d.sh3 = -2.0 * ss[1] * (sz[8] - sz[6]);
+    // lunar periodic coefficients
+    d.ee2 = 2.0 * sl[0] * sl[5];
+    d.e3 = 2.0 * sl[0] * sl[6];
+    d.xi2 = 2.0 * sl[1] * zl[4];
+    d.xi3 = 2.0 * sl[1] * (zl[5] - zl[3]);
+    d.xl2 = -2.0 * sl[2] * zl[1];
+    d.xl3 = -2.0 * sl[2] * (zl[2] - zl[0]);
+    d.xl4 = -2.0 * sl[2] * (-21.0 - 9.0 * emsq) * zel;
+    d.xgh2 = 2.0 * sl[3] * zl[10];
+    d.xgh3 = 2.0 * sl[3] * (zl[11] - zl[9]);
+    d.xgh4 = -18.0 * sl[3] * zel;
+    d.xh2 = -2.0 * sl[1] * zl[7];
+    d.xh3 = -2.0 * sl[1] * (zl[8] - zl[6]);
+
+    // secular rates from the sun and the moon
+    const bool polar_flat = s.inclo < 5.2359877e-2 || s.inclo > kPi - 5.2359877e-2;
+    const double ses = ss[0] * zns * ss[4];
+    const double sis = ss[1] * zns * (sz[3] + sz[5]);
+    const double sls = -zns * ss[2] * (sz[0] + sz[2] - 14.0 - 6.0 * emsq);
+    const double sghs = ss[3] * zns * (sz[9] + sz[11] - 6.0);
+    double shs = -zns * ss[1] * (sz[6] + sz[8]);
+    if (polar_flat) shs = 0.0;
+    if (sinim != 0.0) shs = shs / sinim;
+    const double sgs = sghs - cosim * shs;
+    d.dedt = ses + sl[0] * znl * sl[4];
+    d.didt = sis + sl[1] * znl * (zl[3] + zl[5]);
+    d.dmdt = sls - znl * sl[2] * (zl[0] + zl[2] - 14.0 - 6.0 * emsq);
+    const double sghl = sl[3] * znl * (zl[9] + zl[11] - 6.0);
+    double shll = -znl * sl[1] * (zl[6] + zl[8]);
+    if (polar_flat) shll = 0.0;
+    d.domdt = sgs + sghl;
+    d.dnodt = shs;
+    if (sinim != 0.0) {
+        d.domdt = d.domdt - cosim / sinim * shll;
+        d.dnodt = d.dnodt + shll / sinim;
+    }
+
+    // geopotential resonances
+    d.irez = 0;
+    if (nm < 0.0052359877 && nm > 0.0034906585) d.irez = 1;
+    if (nm >= 8.26e-3 && nm <= 9.24e-3 && em >= 0.5) d.irez = 2;
+    d.d2201 = d.d2211 = d.d3210 = d.d3222 = d.d4410 = d.d4422 = 0.0;
+    d.d5220 = d.d5232 = d.d5421 = d.d5433 = 0.0;
+    d.del1 = d.del2 = d.del3 = d.xfact = d.xlamo = 0.0;
+    if (d.irez == 0) return;
+    const double theta = d.gsto;
+    const double aonv = pow(nm / xke(), 2.0 / 3.0);
+    if (d.irez == 2) {
+        // half-day orbits
+        const double cosisq = cosim * cosim;
+        const double e = s.ecco, esq = eccsq, eoc = e * esq;
+        const double g201 = -0.306 - (e - 0.64) * 0.440;
+        double g211, g310, g322, g410, g422, g520, g521, g532, g533;
+        if (e <= 0.65) {
+            g211 = 3.616 - 13.2470 * e + 16.2900 * esq;
+            g310 = -19.302 + 117.3900 * e - 228.4190 * esq + 156.5910 * eoc;
+            g322 = -18.9068 + 109.7927 * e - 214.6334 * esq + 146.5816 * eoc;
+            g410 = -41.122 + 242.6940 * e - 471.0940 * esq + 313.9530 * eoc;
+            g422 = -146.407 + 841.8800 * e - 1629.014 * esq + 1083.4350 * eoc;
+            g520 = -532.114 + 3017.977 * e - 5740.032 * esq + 3708.2760 * eoc;
+        } else {
+            g211 = -72.099 + 331.819 * e - 508.738 * esq + 266.724 * eoc;
+            g310 = -346.844 + 1582.851 * e - 2415.925 * esq + 1246.113 * eoc;
+            g322 = -342.585 + 1554.908 * e - 2366.899 * esq + 1215.972 * eoc;
+            g410 = -1052.797 + 4758.686 * e - 7193.992 * esq + 3651.957 * eoc;
+            g422 = -3581.690 + 16178.110 * e - 24462.770 * esq + 12422.520 * eoc;
+            if (e > 0.715)
+                g520 = -5149.66 + 29936.92 * e - 54087.36 * esq + 31324.56 * eoc;
+            else
+                g520 = 1464.74 - 4664.75 * e + 3763.64 * esq;
+        }
+        if (e < 0.7) {
+            g533 = -919.22770 + 4988.6100 * e - 9064.7700 * esq + 5542.21 * eoc;
+            g521 = -822.71072 + 4568.6173 * e - 8491.4146 * esq + 5337.524 * eoc;
+            g532 = -853.66600 + 4690.2500 * e - 8624.7700 * esq + 5341.4 * eoc;
+        } else {
+            g533 = -37995.780 + 161616.52 * e - 229838.20 * esq + 109377.94 * eoc;
+            g521 = -51752.104 + 218913.95 * e - 309468.16 * esq + 146349.42 * eoc;
+            g532 = -40023.880 + 170470.89 * e - 242699.48 * esq + 115605.82 * eoc;
+        }
+        const double sini2 = sinim * sinim;
+        const double f220 = 0.75 * (1.0 + 2.0 * cosim + cosisq);
+        const double f221 = 1.5 * sini2;
+        const double f321 = 1.875 * sinim * (1.0 - 2.0 * cosim - 3.0 * cosisq);
+        const double f322 = -1.875 * sinim * (1.0 + 2.0 * cosim - 3.0 * cosisq);
+        const double f441 = 35.0 * sini2 * f220;
+        const double f442 = 39.3750 * sini2 * sini2;
+        const double f522 = 9.84375 * sinim
+                            * (sini2 * (1.0 - 2.0 * cosim - 5.0 * cosisq) + 0.33333333 * (-2.0 + 4.0 * cosim + 6.0 * cosisq));
+        const double f523 = sinim * (4.92187512 * sini2 * (-2.0 - 4.0 * cosim + 10.0 * cosisq)
+                                     + 6.56250012 * (1.0 + 2.0 * cosim - 3.0 * cosisq));
+        const double f542 = 29.53125 * sinim * (2.0 - 8.0 * cosim + cosisq * (-12.0 + 8.0 * cosim + 10.0 * cosisq));
+        const double f543 = 29.53125 * sinim * (-2.0 - 8.0 * cosim + cosisq * (12.0 + 8.0 * cosim - 10.0 * cosisq));
+        const double xno2 = nm * nm, ainv2 = aonv * aonv;
+        double temp1 = 3.0 * xno2 * ainv2;
+        double temp = temp1 * 1.7891679e-6;
+        d.d2201 = temp * f220 * g201;
+        d.d2211 = temp * f221 * g211;
+        temp1 = temp1 * aonv;
+        temp = temp1 * 3.7393792e-7;
+        d.d3210 = temp * f321 * g310;
+        d.d3222 = temp * f322 * g322;
+        temp1 = temp1 * aonv;
+        temp = 2.0 * temp1 * 7.3636953e-9;
+        d.d4410 = temp * f441 * g410;
+        d.d4422 = temp * f442 * g422;
+        temp1 = temp1 * aonv;
+        temp = temp1 * 1.1428639e-7;
+        d.d5220 = temp * f522 * g520;
+        d.d5232 = temp * f523 * g532;
+        temp = 2.0 * temp1 * 2.1765803e-9;
+        d.d5421 = temp * f542 * g521;
+        d.d5433 = temp * f543 * g533;
+        d.xlamo = fmod(s.mo + s.nodeo + s.nodeo - theta - theta, kTwoPi);
+        d.xfact = s.mdot + d.dmdt + 2.0 * (s.nodedot + d.dnodt - kRptim) - s.no_unkozai;
+    } else {
+        // one-day (synchronous) orbits
+        const double g200 = 1.0 + emsq * (-2.5 + 0.8125 * emsq);
+        const double g310 = 1.0 + 2.0 * emsq;
+        const double g300 = 1.0 + emsq * (-6.0 + 6.60937 * emsq);
+        const double f220 = 0.75 * (1.0 + cosim) * (1.0 + cosim);
+        const double f311 = 0.9375 * sinim * sinim * (1.0 + 3.0 * cosim) - 0.75 * (1.0 + cosim);
+        double f330 = 1.0 + cosim;
+        f330 = 1.875 * f330 * f330 * f330;
+        const double del1 = 3.0 * nm * nm * aonv * aonv;
+        d.del2 = 2.0 * del1 * f220 * g200 * 1.7891679e-6;
+        d.del3 = 3.0 * del1 * f330 * g300 * 2.2123015e-7 * aonv;
+        d.del1 = del1 * f311 * g310 * 2.1460748e-6 * aonv;
+        d.xlamo = fmod(s.mo + s.nodeo + s.argpo - theta, kTwoPi);
+        d.xfact = s.mdot + (s.argpdot + s.nodedot) - kRptim + d.dmdt + d.domdt + d.dnodt - s.no_unkozai;
+    }
+}
 
 // Initialise the near-earth model from the mean elements of a GPE (the
 // initialisation of the Report #3 / 2006 formulation: recovery of the
@@ -70,11 +316,6 @@ __device__ int sgp4_init(const mzh_gpe &g, Sgp4Sat &s)
     const double posq = po * po;
     const double rp = ao * (1.0 - s.ecco);
     if (omeosq < 0.0 && s.no_unkozai < 0.0) return 0;
-    // deep space: period of 225 minutes or more
-    if (kTwoPi / s.no_unkozai >= 225.0) {
-        s.deep = 1;
-        return 0;
-    }
     if (rp < 220.0 / kRe + 1.0) s.isimp = 1;
     double sfour = 78.0 / kRe + 1.0;
     const double qzms2ttemp = (120.0 - 78.0) / kRe;
@@ -133,6 +374,12 @@ __device__ int sgp4_init(const mzh_gpe &g, Sgp4Sat &s)
     s.delmo = delmotemp * delmotemp * delmotemp;
     s.sinmao = sin(s.mo);
     s.x7thm1 = 7.0 * cosio2 - 1.0;
+    // deep space: period of 225 minutes or more
+    if (kTwoPi / s.no_unkozai >= 225.0) {
+        s.deep = 1;
+        s.isimp = 1;
+        sdp4_init((g.epoch_jd1 - 2433281.5) + g.epoch_jd2, eccsq, s);
+    }
     if (s.isimp != 1) {
         const double cc1sq = s.cc1 * s.cc1;
         s.d2 = 4.0 * ao * tsi * cc1sq;
@@ -146,20 +393,160 @@ __device__ int sgp4_init(const mzh_gpe &g, Sgp4Sat &s)
     return 0;
 }
 
+// Deep-space secular effects at t minutes: the lunar-solar secular rates,
+// then, for resonant orbits, the resonance terms integrated from epoch with
+// the fixed 720-minute Euler-Maclaurin steps of the published model (the
+// same steps from epoch at every call, so the result does not depend on
+// earlier calls). Returns false for a resonant orbit at a time that is not
+// finite or beyond 5e7 minutes (about 95 years) of its epoch, where the
+// integration would not end in reasonable time.
+__device__ bool sdp4_secular(const Sgp4Sat &s, double t, double &em, double &argpm, double &inclm, double &mm,
+                             double &nodem, double &nm)
+{
+    const Sdp4Terms &d = s.ds;
+    const double theta = fmod(d.gsto + t * kRptim, kTwoPi);
+    em = em + d.dedt * t;
+    inclm = inclm + d.didt * t;
+    argpm = argpm + d.domdt * t;
+    nodem = nodem + d.dnodt * t;
+    mm = mm + d.dmdt * t;
+    if (d.irez == 0) return true;
+    if (!(fabs(t) <= 5.0e7)) return false;
+    const double step = t > 0.0 ? 720.0 : -720.0, step2 = 259200.0;
+    double atime = 0.0, xni = s.no_unkozai, xli = d.xlamo, ft = 0.0;
+    double xndt, xldot, xnddt;
+    for (;;) {
+        if (d.irez != 2) {
+            const double fasx2 = 0.13130908, fasx4 = 2.8843198, fasx6 = 0.37448087;
+            xndt = d.del1 * sin(xli - fasx2) + d.del2 * sin(2.0 * (xli - fasx4)) + d.del3 * sin(3.0 * (xli - fasx6));
+            xldot = xni + d.xfact;
+            xnddt = d.del1 * cos(xli - fasx2) + 2.0 * d.del2 * cos(2.0 * (xli - fasx4))
+                    + 3.0 * d.del3 * cos(3.0 * (xli - fasx6));
+            xnddt = xnddt * xldot;
+        } else {
+            const double g22 = 5.7686396, g32 = 0.95240898, g44 = 1.8014998, g52 = 1.0508330, g54 = 4.4108898;
+            const double xomi = s.argpo + s.argpdot * atime;
+            const double x2omi = xomi + xomi, x2li = xli + xli;
+            xndt = d.d2201 * sin(x2omi + xli - g22) + d.d2211 * sin(xli - g22) + d.d3210 * sin(xomi + xli - g32)
+                   + d.d3222 * sin(-xomi + xli - g32) + d.d4410 * sin(x2omi + x2li - g44) + d.d4422 * sin(x2li - g44)
+                   + d.d5220 * sin(xomi + xli - g52) + d.d5232 * sin(-xomi + xli - g52)
+                   + d.d5421 * sin(xomi + x2li - g54) + d.d5433 * sin(-xomi + x2li - g54);
+            xldot = xni + d.xfact;
+            xnddt = d.d2201 * cos(x2omi + xli - g22) + d.d2211 * cos(xli - g22) + d.d3210 * cos(xomi + xli - g32)
+                    + d.d3222 * cos(-xomi + xli - g32) + d.d5220 * cos(xomi + xli - g52)
+                    + d.d5232 * cos(-xomi + xli - g52)
+                    + 2.0 * (d.d4410 * cos(x2omi + x2li - g44) + d.d4422 * cos(x2li - g44)
+                             + d.d5421 * cos(xomi + x2li - g54) + d.d5433 * cos(-xomi + x2li - g54));
+            xnddt = xnddt * xldot;
+        }
+        if (fabs(t - atime) >= 720.0) {
+            xli = xli + xldot * step + xndt * step2;
+            xni = xni + xndt * step + xnddt * step2;
+            atime = atime + step;
+        } else {
+            ft = t - atime;
+            break;
+        }
+    }
+    nm = xni + xndt * ft + xnddt * ft * ft * 0.5;
+    const double xl = xli + xldot * ft + xndt * ft * ft * 0.5;
+    if (d.irez != 1)
+        mm = xl - 2.0 * nodem + 2.0 * theta;
+    else
+        mm = xl - nodem - argpm + theta;
+    const double dndt = nm - s.no_unkozai;
+    nm = s.no_unkozai + dndt;
+    return true;
+}
+
+// Deep-space lunar-solar periodics at t minutes, applied to the elements
+// unless `at_init` (the published initialisation's own propagation to epoch,
+// which leaves them out). Inclinations below 0.2 rad use Lyddane's form, with
+// the AFSPC (opsmode 'a') quadrant fixes of the node.
+__device__ void sdp4_periodics(const Sdp4Terms &d, double t, bool at_init, double &ep, double &inclp, double &nodep,
+                               double &argpp, double &mp)
+{
+    const double zns = 1.19459e-5, zes = 0.01675, znl = 1.5835218e-4, zel = 0.05490;
+    double zm = at_init ? d.zmos : d.zmos + zns * t;
+    double zf = zm + 2.0 * zes * sin(zm);
+    double sinzf = sin(zf);
+    double f2 = 0.5 * sinzf * sinzf - 0.25;
+    double f3 = -0.5 * sinzf * cos(zf);
+    const double ses = d.se2 * f2 + d.se3 * f3;
+    const double sis = d.si2 * f2 + d.si3 * f3;
+    const double sls = d.sl2 * f2 + d.sl3 * f3 + d.sl4 * sinzf;
+    const double sghs = d.sgh2 * f2 + d.sgh3 * f3 + d.sgh4 * sinzf;
+    const double shs = d.sh2 * f2 + d.sh3 * f3;
+    zm = at_init ? d.zmol : d.zmol + znl * t;
+    zf = zm + 2.0 * zel * sin(zm);
+    sinzf = sin(zf);
+    f2 = 0.5 * sinzf * sinzf - 0.25;
+    f3 = -0.5 * sinzf * cos(zf);
+    const double sel = d.ee2 * f2 + d.e3 * f3;
+    const double sil = d.xi2 * f2 + d.xi3 * f3;
+    const double sll = d.xl2 * f2 + d.xl3 * f3 + d.xl4 * sinzf;
+    const double sghl = d.xgh2 * f2 + d.xgh3 * f3 + d.xgh4 * sinzf;
+    const double shll = d.xh2 * f2 + d.xh3 * f3;
+    if (at_init) return;
+    const double pe = (ses + sel) - d.peo;
+    const double pinc = (sis + sil) - d.pinco;
+    const double pl = (sls + sll) - d.plo;
+    double pgh = (sghs + sghl) - d.pgho;
+    double ph = (shs + shll) - d.pho;
+    inclp = inclp + pinc;
+    ep = ep + pe;
+    const double sinip = sin(inclp), cosip = cos(inclp);
+    if (inclp >= 0.2) {
+        ph = ph / sinip;
+        pgh = pgh - cosip * ph;
+        argpp = argpp + pgh;
+        nodep = nodep + ph;
+        mp = mp + pl;
+    } else {
+        const double sinop = sin(nodep), cosop = cos(nodep);
+        double alfdp = sinip * sinop, betdp = sinip * cosop;
+        const double dalf = ph * cosop + pinc * cosip * sinop;
+        const double dbet = -ph * sinop + pinc * cosip * cosop;
+        alfdp = alfdp + dalf;
+        betdp = betdp + dbet;
+        nodep = fmod(nodep, kTwoPi);
+        if (nodep < 0.0) nodep = nodep + kTwoPi;
+        double xls = mp + argpp + cosip * nodep;
+        const double dls = pl + pgh - pinc * nodep * sinip;
+        xls = xls + dls;
+        const double xnoh = nodep;
+        nodep = atan2(alfdp, betdp);
+        if (nodep < 0.0) nodep = nodep + kTwoPi;
+        if (fabs(xnoh - nodep) > kPi) {
+            if (nodep < xnoh)
+                nodep = nodep + kTwoPi;
+            else
+                nodep = nodep - kTwoPi;
+        }
+        mp = mp + pl;
+        argpp = xls - mp - cosip * nodep;
+    }
+}
+
 } // namespace
 
 // Position (km) and velocity (km/s) in TEME at tsince minutes from the epoch.
 // Returns 0, or the model's error code: 1 mean eccentricity out of range, 2
-// negative mean motion, 4 negative semi-latus rectum, 6 decayed (radius below
-// one Earth radius).
+// negative mean motion, 3 perturbed eccentricity out of range (deep space), 4
+// negative semi-latus rectum, 6 decayed (radius below one Earth radius); 10
+// for a resonant deep-space orbit at a time beyond 5e7 minutes of its epoch.
+// at_init leaves out the deep-space periodics, as the published
+// initialisation's propagation to epoch does.
 // SMOOTH = false follows the published code exactly: its solution of Kepler's
 // equation stops once a Newton correction is below 1e-12 rad and keeps the
 // sine and cosine from before that correction, so the state jumps by up to
 // about 1e-12 rad (some 1e-8 km) where the iteration count changes. SMOOTH =
 // true takes them at the converged anomaly instead: the model without those
 // jumps, for interpolation.
-template <bool SMOOTH>
-__device__ int sgp4_prop(const Sgp4Sat &s, double t, double r[3], double v[3])
+// DEEP selects the model at compile time (the element set's own `deep` flag
+// must match), so that near-earth code carries none of the deep-space terms.
+template <bool SMOOTH, bool DEEP>
+__device__ int sgp4_prop(const Sgp4Sat &s, double t, double r[3], double v[3], bool at_init = false)
 {
     const double x2o3 = 2.0 / 3.0;
     const double ke = xke();
@@ -190,7 +577,10 @@ __device__ int sgp4_prop(const Sgp4Sat &s, double t, double r[3], double v[3])
     }
     double nm = s.no_unkozai;
     double em = s.ecco;
-    const double inclm = s.inclo;
+    double inclm = s.inclo;
+    if constexpr (DEEP) {
+        if (!sdp4_secular(s, t, em, argpm, inclm, mm, nodem, nm)) return MZH_SGP4_NONFINITE;
+    }
     if (nm <= 0.0) return 2;
     const double am = pow(ke / nm, x2o3) * tempa * tempa;
     nm = ke / pow(am, 1.5);
@@ -203,13 +593,33 @@ __device__ int sgp4_prop(const Sgp4Sat &s, double t, double r[3], double v[3])
     argpm = fmod(argpm, kTwoPi);
     xlm = fmod(xlm, kTwoPi);
     mm = fmod(xlm - argpm - nodem, kTwoPi);
-    const double sinim = sin(inclm), cosim = cos(inclm);
+    // lunar-solar periodics (deep space)
+    double ep = em, xincp = inclm, argpp = argpm, nodep = nodem, mp = mm;
+    double sinip, cosip, aycof = s.aycof, xlcof = s.xlcof;
+    if constexpr (DEEP) {
+        sdp4_periodics(s.ds, t, at_init, ep, xincp, nodep, argpp, mp);
+        if (xincp < 0.0) {
+            xincp = -xincp;
+            nodep = nodep + kPi;
+            argpp = argpp - kPi;
+        }
+        if (ep < 0.0 || ep > 1.0) return 3;
+        sinip = sin(xincp);
+        cosip = cos(xincp);
+        aycof = -0.5 * (kJ3 / kJ2) * sinip;
+        if (fabs(cosip + 1.0) > 1.5e-12)
+            xlcof = -0.25 * (kJ3 / kJ2) * sinip * (3.0 + 5.0 * cosip) / (1.0 + cosip);
+        else
+            xlcof = -0.25 * (kJ3 / kJ2) * sinip * (3.0 + 5.0 * cosip) / 1.5e-12;
+    } else {
+        sinip = sin(inclm);
+        cosip = cos(inclm);
+    }
     // long-period periodics
-    const double ep = em, xincp = inclm, argpp = argpm, nodep = nodem, mp = mm;
     const double axnl = ep * cos(argpp);
     double temp = 1.0 / (am * (1.0 - ep * ep));
-    const double aynl = ep * sin(argpp) + temp * s.aycof;
-    const double xl = mp + argpp + nodep + temp * s.xlcof * axnl;
+    const double aynl = ep * sin(argpp) + temp * aycof;
+    const double xl = mp + argpp + nodep + temp * xlcof * axnl;
     // Kepler's equation for the eccentric longitude
     const double u = fmod(xl - nodep, kTwoPi);
     double eo1 = u, tem5 = 9999.9, sineo1 = 0.0, coseo1 = 0.0;
@@ -244,12 +654,19 @@ __device__ int sgp4_prop(const Sgp4Sat &s, double t, double r[3], double v[3])
     temp = 1.0 / pl;
     const double temp1 = 0.5 * kJ2 * temp;
     const double temp2 = temp1 * temp;
-    const double mrt = rl * (1.0 - 1.5 * temp2 * betal * s.con41) + 0.5 * temp1 * s.x1mth2 * cos2u;
-    su = su - 0.25 * temp2 * s.x7thm1 * sin2u;
-    const double xnode = nodep + 1.5 * temp2 * cosim * sin2u;
-    const double xinc = xincp + 1.5 * temp2 * cosim * sinim * cos2u;
-    const double mvt = rdotl - nm * temp1 * s.x1mth2 * sin2u / ke;
-    const double rvdot = rvdotl + nm * temp1 * (s.x1mth2 * cos2u + 1.5 * s.con41) / ke;
+    double con41 = s.con41, x1mth2 = s.x1mth2, x7thm1 = s.x7thm1;
+    if constexpr (DEEP) {
+        const double cosisq = cosip * cosip;
+        con41 = 3.0 * cosisq - 1.0;
+        x1mth2 = 1.0 - cosisq;
+        x7thm1 = 7.0 * cosisq - 1.0;
+    }
+    const double mrt = rl * (1.0 - 1.5 * temp2 * betal * con41) + 0.5 * temp1 * x1mth2 * cos2u;
+    su = su - 0.25 * temp2 * x7thm1 * sin2u;
+    const double xnode = nodep + 1.5 * temp2 * cosip * sin2u;
+    const double xinc = xincp + 1.5 * temp2 * cosip * sinip * cos2u;
+    const double mvt = rdotl - nm * temp1 * x1mth2 * sin2u / ke;
+    const double rvdot = rvdotl + nm * temp1 * (x1mth2 * cos2u + 1.5 * con41) / ke;
     // orientation vectors
     const double sinsu = sin(su), cossu = cos(su), snod = sin(xnode), cnod = cos(xnode), sini = sin(xinc),
                  cosi = cos(xinc);
@@ -268,14 +685,13 @@ __device__ int sgp4_prop(const Sgp4Sat &s, double t, double r[3], double v[3])
 }
 
 // The model of a GPE, checked as the published initialisation does by a
-// propagation to the epoch: 0, an SGP4 error code, or deep space.
+// propagation to the epoch: 0 or an SGP4 error code.
 __device__ int sgp4_setup(const mzh_gpe &g, Sgp4Sat &s)
 {
     int st = sgp4_init(g, s);
-    if (st == 0 && s.deep) return kSgp4DeepSpace;
     if (st == 0) {
         double r[3], v[3];
-        st = sgp4_prop<false>(s, 0.0, r, v);
+        st = s.deep ? sgp4_prop<false, true>(s, 0.0, r, v, true) : sgp4_prop<false, false>(s, 0.0, r, v, true);
     }
     return st;
 }
@@ -294,7 +710,7 @@ __global__ __launch_bounds__(256) void k_sgp4_propagate(const mzh_gpe *__restric
     Sgp4Sat s;
     int st = sgp4_setup(gpes[g], s);
     double r[3] = {0, 0, 0}, v[3] = {0, 0, 0};
-    if (st == 0) st = sgp4_prop<false>(s, tsince[e], r, v);
+    if (st == 0) st = s.deep ? sgp4_prop<false, true>(s, tsince[e], r, v) : sgp4_prop<false, false>(s, tsince[e], r, v);
     for (int i = 0; i < 3; ++i) {
         out[e * 6 + i] = r[i];
         out[e * 6 + 3 + i] = v[i];
@@ -356,6 +772,7 @@ __global__ __launch_bounds__(256) void k_sgp4_seg_init(Sgp4Gen a)
     sg.h0 = h0;
     sg.n_steps = n;
     sg.status = st;
+    sg.deep = s.deep;
     a.segs[i] = sg;
     a.sats[i] = s;
 }
@@ -391,7 +808,7 @@ __device__ inline uint32_t seg_of_step(const uint64_t *seg_step0, uint32_t n_seg
 // [b + (e - b) i / 2^d, b + (e - b) (i + 1) / 2^d).
 // Returns the number of pieces (> 0), or -status. With WRITE, piece j (for j <
 // max_out) goes to traj[j] / time[j + 1], and time[0] gets b when first.
-template <int ORDER, bool WRITE>
+template <int ORDER, bool WRITE, bool DEEP>
 __device__ int interp_step(const Sgp4Sat &sat, double eoff, double b, double e, double reentry_r2, double exit_r2,
                            double *__restrict__ traj, double *__restrict__ time, bool first, int max_out)
 {
@@ -399,7 +816,7 @@ __device__ int interp_step(const Sgp4Sat &sat, double eoff, double b, double e, 
     double kfrac[OP1];
     for (int i = 0; i < OP1; ++i) kfrac[OP1 - 1 - i] = (cos((2.0 * i + 1.0) / (2.0 * OP1) * (kTwoPi / 2.0)) + 1.0) / 2.0;
     double r[3], v[3];
-    int st = sgp4_prop<true>(sat, (b - eoff) * 1440.0, r, v);
+    int st = sgp4_prop<true, DEEP>(sat, (b - eoff) * 1440.0, r, v);
     if (st != 0) return -st;
     const double rr2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
     if (rr2 < reentry_r2) return -MZH_SGP4_REENTRY;
@@ -418,7 +835,7 @@ __device__ int interp_step(const Sgp4Sat &sat, double eoff, double b, double e, 
         double x[OP1], c[7][OP1];
         for (int k = 0; k < OP1; ++k) {
             x[k] = kfrac[k] * dur;
-            st = sgp4_prop<true>(sat, (base + x[k]) * 1440.0, r, v);
+            st = sgp4_prop<true, DEEP>(sat, (base + x[k]) * 1440.0, r, v);
             if (st != 0) return -st;
             for (int ch = 0; ch < 3; ++ch) {
                 c[ch][k] = r[ch];
@@ -432,7 +849,7 @@ __device__ int interp_step(const Sgp4Sat &sat, double eoff, double b, double e, 
             double ep2 = 0.0;
             for (int k = 0; k < OP1; ++k) {
                 const double lo = k == 0 ? 0.0 : x[k - 1], xm = 0.5 * lo + 0.5 * x[k];
-                st = sgp4_prop<true>(sat, (base + xm) * 1440.0, r, v);
+                st = sgp4_prop<true, DEEP>(sat, (base + xm) * 1440.0, r, v);
                 if (st != 0) return -st;
                 double p2 = 0.0;
                 for (int ch = 0; ch < 3; ++ch) {
@@ -473,21 +890,26 @@ __device__ int interp_step(const Sgp4Sat &sat, double eoff, double b, double e, 
 }
 
 // One lane per nominal step: its number of polynomials, or -status.
-template <int ORDER>
+// The near-earth instantiation (DEEP = false) takes the steps of near-earth
+// segments and of failed ones, the deep-space one the others.
+__device__ inline bool takes(const Sgp4Seg &sg, bool deep) { return deep == (sg.status == 0 && sg.deep != 0); }
+
+template <int ORDER, bool DEEP>
 __global__ __launch_bounds__(256) void k_sgp4_count(Sgp4Gen a)
 {
     const uint64_t step = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (step >= a.n_steps) return;
     const uint32_t si = seg_of_step(a.seg_step0, a.n_segs, step);
     const Sgp4Seg sg = a.segs[si];
+    if (!takes(sg, DEEP)) return;
     int res;
     if (sg.status != 0) {
         res = -sg.status;
     } else {
         double b, e;
         nominal_step(sg, (uint32_t)(step - a.seg_step0[si]), b, e);
-        res = interp_step<ORDER, false>(a.sats[si], sg.eoff, b, e, a.reentry_r2, a.exit_r2, nullptr, nullptr, false,
-                                        0);
+        res = interp_step<ORDER, false, DEEP>(a.sats[si], sg.eoff, b, e, a.reentry_r2, a.exit_r2, nullptr, nullptr,
+                                              false, 0);
     }
     a.step_res[step] = res;
 }
@@ -526,7 +948,7 @@ __global__ __launch_bounds__(256) void k_sgp4_scan(Sgp4Gen a)
 
 // One lane per kept nominal step: interpolate again (the same arithmetic as
 // k_sgp4_count) and write the polynomials and their end times.
-template <int ORDER>
+template <int ORDER, bool DEEP>
 __global__ __launch_bounds__(256) void k_sgp4_write(Sgp4Gen a)
 {
     constexpr int OP1 = ORDER + 1;
@@ -538,11 +960,12 @@ __global__ __launch_bounds__(256) void k_sgp4_write(Sgp4Gen a)
     if (cnt <= 0) return;
     const uint32_t si = seg_of_step(a.seg_step0, a.n_segs, step);
     const Sgp4Seg sg = a.segs[si];
+    if (!takes(sg, DEEP)) return;
     double b, e;
     nominal_step(sg, (uint32_t)(step - a.seg_step0[si]), b, e);
     double *traj = a.traj + (a.sat_poly0[sg.sat] + out) * (uint64_t)(OP1 * 7);
     double *time = a.time + a.sat_time0[sg.sat] + out;
-    interp_step<ORDER, true>(a.sats[si], sg.eoff, b, e, a.reentry_r2, a.exit_r2, traj, time, out == 0u, cnt);
+    interp_step<ORDER, true, DEEP>(a.sats[si], sg.eoff, b, e, a.reentry_r2, a.exit_r2, traj, time, out == 0u, cnt);
 }
 
 } // namespace
@@ -558,7 +981,9 @@ hipError_t launch_sgp4_count(const Sgp4Gen &g, uint32_t order, hipStream_t strea
 {
     if (g.n_steps == 0) return hipSuccess;
     if (order != 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_sgp4_count<8>, dim3((unsigned)((g.n_steps + 255u) / 256u)), dim3(256), 0, stream, g);
+    const dim3 grid((unsigned)((g.n_steps + 255u) / 256u));
+    hipLaunchKernelGGL((k_sgp4_count<8, false>), grid, dim3(256), 0, stream, g);
+    if (g.any_deep) hipLaunchKernelGGL((k_sgp4_count<8, true>), grid, dim3(256), 0, stream, g);
     return hipGetLastError();
 }
 
@@ -573,7 +998,9 @@ hipError_t launch_sgp4_write(const Sgp4Gen &g, uint32_t order, hipStream_t strea
 {
     if (g.n_steps == 0) return hipSuccess;
     if (order != 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_sgp4_write<8>, dim3((unsigned)((g.n_steps + 255u) / 256u)), dim3(256), 0, stream, g);
+    const dim3 grid((unsigned)((g.n_steps + 255u) / 256u));
+    hipLaunchKernelGGL((k_sgp4_write<8, false>), grid, dim3(256), 0, stream, g);
+    if (g.any_deep) hipLaunchKernelGGL((k_sgp4_write<8, true>), grid, dim3(256), 0, stream, g);
     return hipGetLastError();
 }
 

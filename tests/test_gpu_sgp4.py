@@ -67,17 +67,19 @@ def test_propagate_matches_vallado():
     st, err = mz.sgp4_propagate(gpes, g["tsince"])
     deep = g["deep"] == 1
     ref_err = g["errors"]
-    # deep-space sets are recognised and not propagated
-    assert np.all(err[deep] == mz.sgp4_pj_status.DEEP_SPACE)
-    near = ~deep
-    assert np.array_equal(err[near], ref_err[near])
-    ok = near[:, None] & (ref_err == 0)
-    dp = np.abs(st[..., :3] - g["states"][..., :3])[ok]
-    dv = np.abs(st[..., 3:] - g["states"][..., 3:])[ok]
-    # nearly every state to the last bits; a few may sit across a change of
-    # the Kepler iteration count
-    assert np.quantile(dp, 0.99) < POS_TOL_TYPICAL and np.quantile(dv, 0.99) < VEL_TOL_TYPICAL
-    assert dp.max() < 2e-8 and dv.max() < 2e-11
+    assert np.array_equal(err, ref_err)
+    for kind, sel in (("near-earth", ~deep), ("deep-space", deep)):
+        ok = sel[:, None] & (ref_err == 0)
+        dp = np.abs(st[..., :3] - g["states"][..., :3])[ok]
+        dv = np.abs(st[..., 3:] - g["states"][..., 3:])[ok]
+        print(f"{kind}: {ok.sum()} states, position q99 {np.quantile(dp, 0.99):.2e} max {dp.max():.2e} km, "
+              f"velocity q99 {np.quantile(dv, 0.99):.2e} max {dv.max():.2e} km/s")
+        # nearly every state to the last bits; a few may sit across a change
+        # of the Kepler iteration count (1e-12 rad times the radius)
+        rmax = np.linalg.norm(g["states"][..., :3], axis=2)[ok].max()
+        assert np.quantile(dp, 0.99) < POS_TOL_TYPICAL * rmax / 7000.0
+        assert np.quantile(dv, 0.99) < VEL_TOL_TYPICAL * rmax / 7000.0
+        assert dp.max() < 2e-12 * rmax + 1e-8 and dv.max() < 2e-11
 
 
 def test_propagate_shapes_and_errors():
@@ -217,22 +219,38 @@ def test_malformed_data():
         mz.make_sgp4_polyjectory(arr, 2460669.0, 2460670.0)
 
 
-def test_deep_space_status():
-    # a deep-space satellite gets its status and no data; the near-earth one
-    # beside it is generated
+def test_single_gpe_ds():
+    # a deep-space GPE (test_make_sgp4_polyjectory.py:116-153: 1e-7 km, 1e-10 km/s)
+    gpes = H.golden_group("single_gpe_ds")
+    jd_begin = 2460669.0
+    pj = mz.make_sgp4_polyjectory(gpes, jd_begin, jd_begin + 1)[0]
+    assert pj[0][1][0] == 0.0 and pj.status[0] == 0
+    e_off = (gpes["epoch_jd1"][0] - jd_begin) + gpes["epoch_jd2"][0]
+    _compare_steps(pj, 0, gpes, np.random.default_rng(42), pos_tol=1e-7, vel_tol=1e-10, ep_off=e_off)
+
+
+def test_single_gpe_syncom():
+    # a deep-space GPE whose SGP4 derivatives spike now and then
+    # (test_make_sgp4_polyjectory.py:155-197: 10 days, 1e-6 km, 1e-10 km/s)
+    gpes = H.golden_group("syncom_gpe")
+    jd_begin = 2460666.5
+    pj = mz.make_sgp4_polyjectory(gpes, jd_begin, jd_begin + 10)[0]
+    assert pj[0][1][0] == 0.0 and pj.status[0] == 0
+    e_off = (gpes["epoch_jd1"][0] - jd_begin) + gpes["epoch_jd2"][0]
+    _compare_steps(pj, 0, gpes, np.random.default_rng(24), pos_tol=1e-6, vel_tol=1e-10, ep_off=e_off)
+
+
+def test_near_and_deep_together():
+    # near-earth and deep-space satellites in one polyjectory come out as
+    # they do alone
     near = H.golden_group("single_gpe")
     ds = H.golden_group("single_gpe_ds")
     both = np.concatenate([near, ds]) if near["norad_id"][0] < ds["norad_id"][0] else np.concatenate([ds, near])
     pj, ids = mz.make_sgp4_polyjectory(both, 2460669.0, 2460670.0)
-    assert np.array_equal(ids, both["norad_id"])
-    st = dict(zip(ids, pj.status))
-    assert st[int(ds["norad_id"][0])] == mz.sgp4_pj_status.DEEP_SPACE
-    assert st[int(near["norad_id"][0])] == 0
-    i_ds = list(ids).index(int(ds["norad_id"][0]))
-    assert len(pj[i_ds][1]) == 0
-    # only deep-space sets: no data at all (the polyjectory constructor's rule)
-    with pytest.raises(ValueError, match="number of steps equal to zero"):
-        mz.make_sgp4_polyjectory(ds, 2460669.0, 2460670.0)
+    assert np.array_equal(ids, both["norad_id"]) and np.all(pj.status == 0)
+    for k, one in enumerate(both):
+        alone = mz.make_sgp4_polyjectory(both[k : k + 1], 2460669.0, 2460670.0)[0]
+        assert np.array_equal(alone[0][0], pj[k][0]) and np.array_equal(alone[0][1], pj[k][1])
 
 
 _S_8000 = ("1 00011U 59001A   24187.51496924  .00001069  00000-0  55482-3 0  9992",
@@ -268,7 +286,6 @@ def test_nz_status_and_catalog():
     for cfs, times, status in pj:
         if times.shape[0] == 0:
             assert status != 0
-    assert np.all(pj.status[g["deep"][g["group"] == gi] == 1] == mz.sgp4_pj_status.DEEP_SPACE)
     tspan = np.linspace(0.0, np.nextafter(1 / 6.0, -1.0), 10)
     ev = pj.state_meval(tspan)
     e_off = (gpes["epoch_jd1"] - begin) + gpes["epoch_jd2"]
@@ -278,9 +295,12 @@ def test_nz_status_and_catalog():
     # the last time may lie past the trajectories' end (jd_end is rounded)
     assert ok[:, :-1].sum() > 0.8 * ok[:, :-1].size
     assert d[ok].max() < 1e-6
-    # and, where nothing failed, at the reference's step tolerance
+    # and, where nothing failed, at the reference's step tolerance, scaled by
+    # the radius for deep-space orbits (the published Kepler stopping rule
+    # moves states by up to 1e-12 rad times the radius)
     good = (pj.status == 0)[:, None] & ok
-    assert d[good].max() < POS_TOL
+    r = np.linalg.norm(ref[:, :, :3], axis=2)
+    assert np.all(d[good] < POS_TOL * np.maximum(1.0, r[good] / 7000.0))
 
 
 def test_disc_gpe_isolated():
@@ -293,10 +313,7 @@ def test_disc_gpe_isolated():
     pj = mz.make_sgp4_polyjectory(np.concatenate([other, gpes]), jd_begin, jd_begin + 5.0)[0]
     cfs, times, status = pj[1]
     dur = np.diff(times) * 86400.0
-    print("status", status, "steps", len(times) - 1, "durations (s)", np.unique(np.round(dur, 1)))
-    if status == mz.sgp4_pj_status.DEEP_SPACE:
-        assert len(times) == 0
-        pytest.skip("disc_gpe_01 is a deep-space element set, which the GPU generator does not propagate")
+    print("status", status, "steps", len(times) - 1, "shortest steps (s)", np.sort(dur)[:5])
     assert status == 0
     e_off = (gpes["epoch_jd1"][0] - jd_begin) + gpes["epoch_jd2"][0]
     assert pj.status[0] == 0
