@@ -926,6 +926,26 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
                            st));
     MZH_HIP(d_res.ensure(sizeof(int32_t) * (n_steps ? n_steps : 1)));
     MZH_HIP(d_out.ensure(sizeof(uint32_t) * (n_steps ? n_steps : 1)));
+    // stage single-piece steps when that takes at most a quarter of the free
+    // device memory (else the write pass interpolates every step again)
+    DevBuf d_stage, d_stage_t, d_redo, d_redo_n;
+    g.stage = g.stage_t = nullptr;
+    MZH_HIP(d_redo.ensure(sizeof(uint32_t) * (n_steps ? n_steps : 1)));
+    MZH_HIP(d_redo_n.ensure(sizeof(unsigned long long)));
+    MZH_HIP(hipMemsetAsync(d_redo_n.p, 0, sizeof(unsigned long long), st));
+    g.redo = d_redo.as<uint32_t>();
+    g.redo_count = d_redo_n.as<unsigned long long>();
+    if (n_steps > 0xffffffffull) throw StatusError{MZH_EINVAL, "Too many trajectory steps for make_sgp4_polyjectory()"};
+    {
+        size_t free_b = 0, total_b = 0;
+        const uint64_t stage_b = n_steps * sizeof(double) * 7u * (order + 1u);
+        if (n_steps && hipMemGetInfo(&free_b, &total_b) == hipSuccess && stage_b <= free_b / 4u) {
+            MZH_HIP(d_stage.ensure(stage_b));
+            MZH_HIP(d_stage_t.ensure(sizeof(double) * (n_steps + 1u)));
+            g.stage = d_stage.as<double>();
+            g.stage_t = d_stage_t.as<double>();
+        }
+    }
     g.seg_step0 = d_segstep.as<const uint64_t>();
     g.n_steps = n_steps;
     g.step_res = d_res.as<int32_t>();
@@ -941,6 +961,8 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     std::vector<uint64_t> nsteps(n_sats);
     std::vector<int32_t> status(n_sats);
     std::vector<double> t_last(n_sats);
+    unsigned long long n_redo = 0;
+    MZH_HIP(hipMemcpyAsync(&n_redo, d_redo_n.p, sizeof(n_redo), hipMemcpyDeviceToHost, st));
     MZH_HIP(hipMemcpyAsync(nsteps.data(), d_np.p, sizeof(uint64_t) * n_sats, hipMemcpyDeviceToHost, st));
     MZH_HIP(hipMemcpyAsync(status.data(), d_status.p, sizeof(int32_t) * n_sats, hipMemcpyDeviceToHost, st));
     MZH_HIP(hipMemcpyAsync(t_last.data(), d_tlast.p, sizeof(double) * n_sats, hipMemcpyDeviceToHost, st));
@@ -978,7 +1000,7 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     g.sat_poly0 = ctx->step0.as<const uint64_t>();
     g.sat_time0 = ctx->time_off.as<const uint64_t>();
     MZH_HIP(hipEventRecord(E[5], st));
-    MZH_HIP(launch_sgp4_write(g, order, st));
+    MZH_HIP(launch_sgp4_write(g, order, n_redo, st));
     MZH_HIP(hipEventRecord(E[6], st));
     MZH_HIP(launch_transpose_cf4(ctx->traj.as<double>(), ctx->cf4.as<double>(), steps_total, order, st));
     MZH_HIP(hipEventRecord(E[7], st));

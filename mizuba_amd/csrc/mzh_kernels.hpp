@@ -165,7 +165,16 @@ struct Sgp4Gen {
     uint64_t *sat_npolys;          // [n_sats]
     int32_t *sat_status;           // [n_sats]
     double *sat_tlast;             // [n_sats] end of the last polynomial
-    // output (k_sgp4_write)
+    // staging of single-piece steps (null: none): the count pass keeps
+    // coefficient k of the step's polynomial in stage[k * n_steps + step]
+    // (coefficient-major, so that a wave's stores are contiguous) and its end
+    // time in stage_t[step + 1]; k_sgp4_copy moves them into place
+    double *stage, *stage_t;
+    // the kept steps to interpolate again (multi-piece ones, or all when not
+    // staging), listed by k_sgp4_scan
+    uint32_t *redo;
+    unsigned long long *redo_count;
+    // output (k_sgp4_copy, k_sgp4_write)
     double *traj, *time;
     const uint64_t *sat_poly0, *sat_time0;
 };
@@ -176,8 +185,9 @@ hipError_t launch_sgp4_count(const Sgp4Gen &g, uint32_t order, hipStream_t strea
 // Per satellite: the first failing step ends the trajectory; offsets of the
 // kept steps, polynomial counts, statuses and end times.
 hipError_t launch_sgp4_scan(const Sgp4Gen &g, hipStream_t stream);
-// Interpolate the kept steps again and write their polynomials and times.
-hipError_t launch_sgp4_write(const Sgp4Gen &g, uint32_t order, hipStream_t stream);
+// Move the staged single-piece steps into place; interpolate the n_redo
+// listed steps again and write their polynomials and times.
+hipError_t launch_sgp4_write(const Sgp4Gen &g, uint32_t order, uint64_t n_redo, hipStream_t stream);
 
 // Polyjectory evaluation: out[r][k][7] for n_rows x n_times evaluations,
 // times and rows as in mzh_state_eval (time_mode = MZH_EVAL_TIME_*). The

@@ -546,7 +546,7 @@ __device__ void sdp4_periodics(const Sdp4Terms &d, double t, bool at_init, doubl
 // DEEP selects the model at compile time (the element set's own `deep` flag
 // must match), so that near-earth code carries none of the deep-space terms.
 template <bool SMOOTH, bool DEEP>
-__device__ int sgp4_prop(const Sgp4Sat &s, double t, double r[3], double v[3], bool at_init = false)
+__device__ __attribute__((always_inline)) int sgp4_prop(const Sgp4Sat &s, double t, double r[3], double v[3], bool at_init = false)
 {
     const double x2o3 = 2.0 / 3.0;
     const double ke = xke();
@@ -807,10 +807,13 @@ __device__ inline uint32_t seg_of_step(const uint64_t *seg_step0, uint32_t n_seg
 // accepted as they are; more than 20 pieces with more to do is status 14. The pieces are visited in time order, without a stack: piece (d, i) is
 // [b + (e - b) i / 2^d, b + (e - b) (i + 1) / 2^d).
 // Returns the number of pieces (> 0), or -status. With WRITE, piece j (for j <
-// max_out) goes to traj[j] / time[j + 1], and time[0] gets b when first.
+// max_out) goes to traj[j * 7 (order + 1) + (k * 7 + ch) * cstride] for
+// coefficient k of channel ch, its end time to time[j + 1], and time[0] gets b
+// when first.
 template <int ORDER, bool WRITE, bool DEEP>
-__device__ int interp_step(const Sgp4Sat &sat, double eoff, double b, double e, double reentry_r2, double exit_r2,
-                           double *__restrict__ traj, double *__restrict__ time, bool first, int max_out)
+__device__ __attribute__((always_inline)) int interp_step(const Sgp4Sat &sat, double eoff, double b, double e, double reentry_r2, double exit_r2,
+                           double *__restrict__ traj, double *__restrict__ time, bool first, int max_out,
+                           uint64_t cstride = 1)
 {
     constexpr int OP1 = ORDER + 1;
     double kfrac[OP1];
@@ -868,7 +871,7 @@ __device__ int interp_step(const Sgp4Sat &sat, double eoff, double b, double e, 
             if (WRITE && npolys < max_out) {
                 double *out = traj + (size_t)npolys * OP1 * 7u;
                 for (int k = 0; k < OP1; ++k)
-                    for (int ch = 0; ch < 7; ++ch) out[k * 7 + ch] = c[ch][k];
+                    for (int ch = 0; ch < 7; ++ch) out[(k * 7 + ch) * cstride] = c[ch][k];
                 time[npolys + 1] = pe;
             }
             ++npolys;
@@ -894,7 +897,7 @@ __device__ int interp_step(const Sgp4Sat &sat, double eoff, double b, double e, 
 // segments and of failed ones, the deep-space one the others.
 __device__ inline bool takes(const Sgp4Seg &sg, bool deep) { return deep == (sg.status == 0 && sg.deep != 0); }
 
-template <int ORDER, bool DEEP>
+template <int ORDER, bool DEEP, bool STAGE>
 __global__ __launch_bounds__(256) void k_sgp4_count(Sgp4Gen a)
 {
     const uint64_t step = (uint64_t)blockIdx.x * 256u + threadIdx.x;
@@ -908,10 +911,59 @@ __global__ __launch_bounds__(256) void k_sgp4_count(Sgp4Gen a)
     } else {
         double b, e;
         nominal_step(sg, (uint32_t)(step - a.seg_step0[si]), b, e);
-        res = interp_step<ORDER, false, DEEP>(a.sats[si], sg.eoff, b, e, a.reentry_r2, a.exit_r2, nullptr, nullptr,
-                                              false, 0);
+        if constexpr (STAGE)
+            // the first piece goes to the staging slot of the step
+            res = interp_step<ORDER, true, DEEP>(a.sats[si], sg.eoff, b, e, a.reentry_r2, a.exit_r2, a.stage + step,
+                                                 a.stage_t + step, false, 1, a.n_steps);
+        else
+            res = interp_step<ORDER, false, DEEP>(a.sats[si], sg.eoff, b, e, a.reentry_r2, a.exit_r2, nullptr,
+                                                  nullptr, false, 0);
     }
     a.step_res[step] = res;
+}
+
+// 64 steps per workgroup: the staged polynomials of the kept single-piece
+// steps go through an LDS tile from the coefficient-major staging array to
+// their records in the polyjectory, both sides read and written contiguously;
+// their end times (and a satellite's start time) go along.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_sgp4_copy(Sgp4Gen a)
+{
+    constexpr uint32_t NC = 7u * (ORDER + 1u);
+    __shared__ double tile[NC * 65u];
+    __shared__ uint64_t dst[64];
+    const uint64_t s0 = (uint64_t)blockIdx.x * 64u;
+    const uint32_t tid = threadIdx.x;
+    if (tid < 64u) {
+        const uint64_t step = s0 + tid;
+        uint64_t d = ~0ull;
+        if (step < a.n_steps && a.step_res[step] == 1) {
+            const uint32_t out = a.step_out[step];
+            if (out != ~0u) {
+                const uint32_t si = seg_of_step(a.seg_step0, a.n_segs, step);
+                const Sgp4Seg sg = a.segs[si];
+                d = (a.sat_poly0[sg.sat] + out) * NC;
+                double *time = a.time + a.sat_time0[sg.sat] + out;
+                time[1] = a.stage_t[step + 1u];
+                if (out == 0u) {
+                    double b, e;
+                    nominal_step(sg, (uint32_t)(step - a.seg_step0[si]), b, e);
+                    time[0] = b;
+                }
+            }
+        }
+        dst[tid] = d;
+    }
+    for (uint32_t i = tid; i < NC * 64u; i += 256u) {
+        const uint32_t k = i / 64u, j = i % 64u;
+        if (s0 + j < a.n_steps) tile[k * 65u + j] = a.stage[k * a.n_steps + s0 + j];
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < NC * 64u; i += 256u) {
+        const uint32_t j = i / NC, k = i % NC;
+        const uint64_t d = dst[j];
+        if (d != ~0ull) a.traj[d + k] = tile[k * 65u + j];
+    }
 }
 
 // One lane per satellite: its steps in time order up to the first failing one
@@ -936,6 +988,7 @@ __global__ __launch_bounds__(256) void k_sgp4_scan(Sgp4Gen a)
             }
             a.step_out[step] = (uint32_t)acc;
             acc += (uint64_t)res;
+            if (res > 1 || !a.stage) a.redo[atomicAdd(a.redo_count, 1ull)] = (uint32_t)step;
             double b, e;
             nominal_step(sg, k, b, e);
             t_last = e;
@@ -949,15 +1002,15 @@ __global__ __launch_bounds__(256) void k_sgp4_scan(Sgp4Gen a)
 // One lane per kept nominal step: interpolate again (the same arithmetic as
 // k_sgp4_count) and write the polynomials and their end times.
 template <int ORDER, bool DEEP>
-__global__ __launch_bounds__(256) void k_sgp4_write(Sgp4Gen a)
+__global__ __launch_bounds__(256) void k_sgp4_write(Sgp4Gen a, uint64_t n_redo)
 {
     constexpr int OP1 = ORDER + 1;
-    const uint64_t step = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    if (step >= a.n_steps) return;
+    const uint64_t r = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (r >= n_redo) return;
+    const uint32_t step = a.redo[r];
     const uint32_t out = a.step_out[step];
-    if (out == ~0u) return;
     const int32_t cnt = a.step_res[step];
-    if (cnt <= 0) return;
+    if (out == ~0u || cnt <= 0) return;
     const uint32_t si = seg_of_step(a.seg_step0, a.n_segs, step);
     const Sgp4Seg sg = a.segs[si];
     if (!takes(sg, DEEP)) return;
@@ -982,8 +1035,13 @@ hipError_t launch_sgp4_count(const Sgp4Gen &g, uint32_t order, hipStream_t strea
     if (g.n_steps == 0) return hipSuccess;
     if (order != 8) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((g.n_steps + 255u) / 256u));
-    hipLaunchKernelGGL((k_sgp4_count<8, false>), grid, dim3(256), 0, stream, g);
-    if (g.any_deep) hipLaunchKernelGGL((k_sgp4_count<8, true>), grid, dim3(256), 0, stream, g);
+    if (g.stage) {
+        hipLaunchKernelGGL((k_sgp4_count<8, false, true>), grid, dim3(256), 0, stream, g);
+        if (g.any_deep) hipLaunchKernelGGL((k_sgp4_count<8, true, true>), grid, dim3(256), 0, stream, g);
+    } else {
+        hipLaunchKernelGGL((k_sgp4_count<8, false, false>), grid, dim3(256), 0, stream, g);
+        if (g.any_deep) hipLaunchKernelGGL((k_sgp4_count<8, true, false>), grid, dim3(256), 0, stream, g);
+    }
     return hipGetLastError();
 }
 
@@ -994,13 +1052,20 @@ hipError_t launch_sgp4_scan(const Sgp4Gen &g, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launch_sgp4_write(const Sgp4Gen &g, uint32_t order, hipStream_t stream)
+hipError_t launch_sgp4_write(const Sgp4Gen &g, uint32_t order, uint64_t n_redo, hipStream_t stream)
 {
     if (g.n_steps == 0) return hipSuccess;
     if (order != 8) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((g.n_steps + 255u) / 256u));
-    hipLaunchKernelGGL((k_sgp4_write<8, false>), grid, dim3(256), 0, stream, g);
-    if (g.any_deep) hipLaunchKernelGGL((k_sgp4_write<8, true>), grid, dim3(256), 0, stream, g);
+    if (g.stage) {
+        const uint64_t blocks = (g.n_steps + 63u) / 64u;
+        if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_sgp4_copy<8>), dim3((unsigned)blocks), dim3(256), 0, stream, g);
+    }
+    if (n_redo) {
+        const dim3 grid((unsigned)((n_redo + 255u) / 256u));
+        hipLaunchKernelGGL((k_sgp4_write<8, false>), grid, dim3(256), 0, stream, g, n_redo);
+        if (g.any_deep) hipLaunchKernelGGL((k_sgp4_write<8, true>), grid, dim3(256), 0, stream, g, n_redo);
+    }
     return hipGetLastError();
 }
 
