@@ -21,8 +21,12 @@ Stored arrays:
              initialisation fails)
   deep       [n] int32, 1 for a deep-space GPE
 
-Run only where the reference tree is present; the .npz is the committed
-fixture.
+It also writes the two full space-track catalogs of the reference's tests
+(strack_20240705.npz and strack_20240917.npz: `gpes`, [n, 10] float64 in the
+field order above), the data of test_make_sgp4_polyjectory.py:755-817.
+
+Run only where the reference tree is present; the .npz files are the
+committed fixtures.
 """
 
 import ctypes
@@ -81,6 +85,11 @@ def main():
         errors[i] = err
     np.savez_compressed(OUT, gpes=gpes, group=np.array(group, dtype=np.int32), groups=np.array(GROUPS),
                         tsince=tsince, states=states, errors=errors, deep=deep)
+    for name in ("strack_20240705", "strack_20240917"):
+        t = pq.read_table(REF_TEST / f"{name}.parquet")
+        arr = np.stack([np.asarray(t[f].to_numpy(), dtype=np.float64) for f in FIELDS], axis=1)
+        np.savez_compressed(HERE / f"{name}.npz", gpes=arr)
+        print(HERE / f"{name}.npz", arr.shape)
     print(OUT, n, "GPEs,", int(deep.sum()), "deep space,", int((errors != 0).sum()), "error entries")
 
 

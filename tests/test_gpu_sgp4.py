@@ -303,6 +303,31 @@ def test_nz_status_and_catalog():
     assert np.all(d[good] < POS_TOL * np.maximum(1.0, r[good] / 7000.0))
 
 
+@pytest.mark.parametrize("name,begin", [("strack_20240705", 2460496.5), ("strack_20240917", 2460569.5)])
+def test_strack_catalogs(name, begin):
+    # the reference's full space-track catalogs over 4 hours
+    # (test_make_sgp4_polyjectory.py:755-817): some decayed satellites hit the
+    # bisection limit (status 14), and the states of the others agree with
+    # SGP4 within 1e-6 km at 10 times; satellites without data have a status
+    gpes = H.gpe_array(np.load(H.GOLDEN / f"{name}.npz")["gpes"])
+    prop_time = 1 / 6.0
+    pj, ids = mz.make_sgp4_polyjectory(gpes, begin, begin + prop_time)
+    assert pj.n_objs == len(ids) == len(np.unique(gpes["norad_id"]))
+    st = np.asarray(pj.status)
+    assert np.any(st == 14)
+    for k in np.nonzero(st != 0)[0][:50]:
+        assert pj[int(k)][2] == st[k]
+    tspan = np.linspace(0.0, np.nextafter(prop_time, -1.0), 10)
+    ev = pj.state_meval(tspan)
+    e_off = (gpes["epoch_jd1"] - begin) + gpes["epoch_jd2"]
+    ref, err = mz.sgp4_propagate(gpes, (tspan[None, :] - e_off[:, None]) * 1440.0)
+    d = np.linalg.norm(ev[:, :, :3] - ref[:, :, :3], axis=2)
+    ok = ~np.isnan(d)
+    print(name, dict(zip(*[a.tolist() for a in np.unique(st, return_counts=True)])), "max", d[ok].max())
+    assert d[ok].max() < 1e-6
+    assert np.all(st[np.all(~ok, axis=1)] != 0)
+
+
 def test_disc_gpe_isolated():
     # a GPE with known discontinuities: inaccurate interpolation is confined
     # to steps shorter than 5 s (test_make_sgp4_polyjectory.py:609-665)

@@ -31,6 +31,18 @@ def test_fixture_shape():
     assert r.min() > 6378.0 and r.max() < 20000.0 and v.max() < 11.2
 
 
+def test_strack_fixtures():
+    # the reference's two space-track catalogs, sorted by NORAD ID then epoch
+    for name, n in (("strack_20240705", 26389), ("strack_20240917", 26905)):
+        g = np.load(H.GOLDEN / f"{name}.npz")["gpes"]
+        assert g.shape == (n, 10) and np.all(np.isfinite(g))
+        ids = g[:, 0]
+        assert np.all(np.diff(ids) >= 0)
+        same = np.diff(ids) == 0
+        ep = g[:, 1] + g[:, 2]
+        assert np.all(np.diff(ep)[same] > 0)
+
+
 @pytest.mark.skipif(not REF_SGP4.exists(), reason="oracle/_ref not built (make -C oracle ref)")
 def test_fixture_is_the_reference_sgp4():
     # the fixture's states are the reference's vendored SGP4 bit for bit
