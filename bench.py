@@ -2,7 +2,10 @@
 
 Workload: 30 000 synthetic objects (80 % LEO perigee 400-1200 km e < 0.01,
 20 % perigee 1200-2000 km e < 0.05, i in [0, 100] deg, seeded) over 1 day,
-order-8 FP64 polyjectory, conj_thresh 5 km, conj_det_interval 3 min.
+order-8 FP64 polyjectory, conj_thresh 5 km, conj_det_interval 3 min. The
+objects are SGP4 mean elements made into a polyjectory by this library's GPU
+make_sgp4_polyjectory (`--catalog sgp4`, the default; `--catalog kepler`
+takes J2-secular Keplerian orbits fitted by make_synthetic_polyjectory).
 One step = one full conjunction screening of that polyjectory (480
 conjunction steps: AABBs -> Morton/sort -> BVH -> broad -> narrow phase), with
 the polyjectory already resident in HBM. `value` is the mixed-precision
@@ -489,6 +492,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0, help="only run the PMC copy calibration")
     ap.add_argument("--workload", choices=("headline", "batch64", "state_eval", "sgp4"), default="headline")
+    ap.add_argument("--catalog", choices=("kepler", "sgp4"), default="sgp4",
+                    help="headline catalog: J2-secular Keplerian fits, or SGP4 mean elements through make_sgp4_polyjectory")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -516,7 +521,12 @@ def main():
 
     thresh, interval = 5.0, 3.0 / 1440.0
     t0 = time.perf_counter()
-    pj = mz.make_synthetic_polyjectory(args.n_objs, args.days, seed=args.seed, frac_high=0.2, order=8)
+    if args.catalog == "sgp4":
+        jd0 = 2460496.5
+        gpes = mz.make_synthetic_gpes(args.n_objs, seed=args.seed, frac_high=0.2, epoch_jd=jd0)
+        pj = mz.make_sgp4_polyjectory(gpes, jd0, jd0 + args.days)[0]
+    else:
+        pj = mz.make_synthetic_polyjectory(args.n_objs, args.days, seed=args.seed, frac_high=0.2, order=8)
     print(f"[bench] synthetic polyjectory: {pj.n_objs} objects in {time.perf_counter() - t0:.1f}s", file=sys.stderr)
 
     scr = mz.Screener(0)
@@ -643,13 +653,18 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 screen + f64 narrow phase",
-        "data": "synthetic (seeded J2-secular Keplerian catalog, order-8 Chebyshev-fitted polyjectory)",
+        "data": ("synthetic (seeded SGP4 mean elements, polyjectory from this library's GPU make_sgp4_polyjectory, "
+                 "order 8)" if args.catalog == "sgp4" else
+                 "synthetic (seeded J2-secular Keplerian catalog, order-8 Chebyshev-fitted polyjectory)"),
         "config": {
-            "workload": f"{pj.n_objs} objects x {pj.maxT:g} day, order 8, conj_thresh 5 km, conj_det_interval 3 min",
+            "workload": f"{pj.n_objs} objects x {args.days:g} day, order 8, conj_thresh 5 km, conj_det_interval 3 min",
             "n_objs": pj.n_objs,
-            "days": pj.maxT,
+            "days": args.days,
             "n_cd_steps": int(np.ceil(pj.maxT / interval)),
-            "mix": "80% LEO / 20% high",
+            "mix": ("80% LEO (perigee 400-1200 km, e < 0.01) / 20% perigee 1200-2000 km, e < 0.05; "
+                    "inclination 0-100 deg; bstar 1e-5-1e-4"),
+            "catalog": args.catalog,
+            "trajectory_steps": int(pj._flat()[3].sum()),
             "seed": args.seed,
         },
         "fp64_dropin": {

@@ -43,6 +43,7 @@ get_tmpdir = _core.get_tmpdir
 set_tmpdir = _core.set_tmpdir
 
 from ._sgp4_polyjectory import gpe_dtype, make_sgp4_polyjectory, sgp4_pj_status, sgp4_propagate  # noqa: E402
+from ._synthetic_gpes import make_synthetic_gpes  # noqa: E402
 
 
 class otype(_enum.IntEnum):
@@ -68,4 +69,5 @@ __all__ = [
     "make_sgp4_polyjectory",
     "sgp4_pj_status",
     "sgp4_propagate",
+    "make_synthetic_gpes",
 ]

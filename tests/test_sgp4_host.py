@@ -134,3 +134,21 @@ def test_tle_helper():
         "2 00045  66.6943  81.3521 0257384 317.3173  40.8180 14.34783636277898",
     )
     assert neg[9] == pytest.approx(-0.14841e-3, rel=1e-15)
+
+
+def test_synthetic_gpes():
+    # the configurations' element distributions, as SGP4 mean elements
+    g = mz.make_synthetic_gpes(20000, seed=7)
+    assert g.dtype == mz.gpe_dtype and np.array_equal(g["norad_id"], np.arange(1, 20001))
+    a = (398600.8 / (g["n0"] / 60.0) ** 2) ** (1.0 / 3.0)
+    hp = a * (1.0 - g["e0"]) - 6378.135
+    assert hp.min() > 400.0 - 1e-6 and hp.max() < 2000.0 + 1e-6
+    low = hp < 1200.0
+    assert 0.75 < low.mean() < 0.85 and g["e0"][low].max() <= 0.01 and g["e0"].max() <= 0.05
+    assert np.degrees(g["i0"]).max() <= 100.0 and g["bstar"].min() >= 1e-5 and g["bstar"].max() <= 1e-4
+    assert np.array_equal(mz.make_synthetic_gpes(100, seed=7), mz.make_synthetic_gpes(100, seed=7))
+    s = mz.make_synthetic_gpes(1000, seed=1, n_shell=720)
+    a_s = (398600.8 / (s["n0"][:720] / 60.0) ** 2) ** (1.0 / 3.0)
+    alt = a_s * (1.0 - s["e0"][:720]) - 6378.135
+    assert np.allclose(np.sort(np.unique(np.round(alt))), [540, 550, 560, 570, 580])
+
