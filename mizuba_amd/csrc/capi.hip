@@ -924,6 +924,8 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     MZH_HIP(d_segstep.ensure(sizeof(uint64_t) * (n_segs + 1u)));
     MZH_HIP(hipMemcpyAsync(d_segstep.p, seg_step0.data(), sizeof(uint64_t) * (n_segs + 1u), hipMemcpyHostToDevice,
                            st));
+    // step indices are 32-bit in the generator's lists
+    if (n_steps > 0xffffffffull) throw StatusError{MZH_EINVAL, "Too many trajectory steps for make_sgp4_polyjectory()"};
     MZH_HIP(d_res.ensure(sizeof(int32_t) * (n_steps ? n_steps : 1)));
     MZH_HIP(d_out.ensure(sizeof(uint32_t) * (n_steps ? n_steps : 1)));
     // stage single-piece steps when that takes at most a quarter of the free
@@ -935,7 +937,6 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     MZH_HIP(hipMemsetAsync(d_redo_n.p, 0, sizeof(unsigned long long), st));
     g.redo = d_redo.as<uint32_t>();
     g.redo_count = d_redo_n.as<unsigned long long>();
-    if (n_steps > 0xffffffffull) throw StatusError{MZH_EINVAL, "Too many trajectory steps for make_sgp4_polyjectory()"};
     {
         size_t free_b = 0, total_b = 0;
         const uint64_t stage_b = n_steps * sizeof(double) * 7u * (order + 1u);
