@@ -1,6 +1,9 @@
 """GPU runs at the full sizes of the larger configurations: 30 000 objects over
 7 days (3 360 conjunction steps) and the dense-shell stress case (100 000
-objects, 70 000 of them in five near-circular shells, 10 km, 2-minute steps).
+objects, 70 000 of them in five near-circular shells, 10 km, 2-minute steps);
+and, with catalogs made the configurations' way (seeded SGP4 mean elements
+through make_sgp4_polyjectory), the FP64 drop-in at 20 000 objects checked
+against the oracle on every step, and the dense-shell case again.
 
 The CPU oracle cannot screen these in a test's time, so they are checked
 through properties that hold at any size, and against the oracle on windows
@@ -94,3 +97,45 @@ def test_config_dense_shells_100k():
     check_properties(c64, cmx, thresh)
     check_window(c64, pj, thresh, interval, 0, 2)
     check_window(c64, pj, thresh, interval, 718, 720)
+
+
+def _sgp4_catalog(n, days, seed, n_shell=0, frac_high=0.2):
+    jd0 = 2460496.5
+    gpes = mz.make_synthetic_gpes(n, seed=seed, frac_high=frac_high, n_shell=n_shell, epoch_jd=jd0)
+    pj = mz.make_sgp4_polyjectory(gpes, jd0, jd0 + days)[0]
+    assert np.all(pj.status == 0)
+    return pj
+
+
+def test_config_20k_sgp4_fp64_every_step():
+    # the all-FP64 drop-in configuration: 20 000 objects (80% LEO, 20% higher),
+    # 1 day, order 8; boxes, codes, sort order, trees, pairs and conjunctions
+    # equal the oracle's bit for bit on all 480 steps, and mixed mode gives
+    # the same conjunctions
+    from oracle import pyoracle as po
+
+    pj = _sgp4_catalog(20000, 1.0, seed=2028)
+    thresh, interval = 5.0, 3.0 / 1440.0
+    c64 = mz.conjunctions(pj, thresh, interval, mode="fp64")
+    cmx = mz.conjunctions(pj, thresh, interval, mode="mixed")
+    check_properties(c64, cmx, thresh)
+    ref = po.run(H.flat_of(pj), thresh, interval)
+    for name in ("aabbs", "srt_aabbs", "mcodes", "srt_mcodes", "srt_idx"):
+        assert np.array_equal(_bits(getattr(c64, name)), _bits(ref[name])), name
+    for k in range(c64.n_cd_steps):
+        assert np.array_equal(_bits(c64.get_bvh_tree(k)), _bits(po.get_bvh_tree(ref, k))), k
+        assert np.array_equal(_bits(c64.get_aabb_collisions(k)), _bits(po.get_aabb_collisions(ref, k))), k
+    assert np.array_equal(_bits(c64.conjunctions), _bits(ref["conjunctions"]))
+
+
+def test_config_dense_shells_100k_sgp4():
+    pj = _sgp4_catalog(100000, 1.0, seed=2029, n_shell=70000, frac_high=0.0)
+    thresh, interval = 10.0, 2.0 / 1440.0
+    c64 = mz.conjunctions(pj, thresh, interval, mode="fp64")
+    cmx = mz.conjunctions(pj, thresh, interval, mode="mixed")
+    assert c64.n_cd_steps == 720
+    check_properties(c64, cmx, thresh)
+    check_window(c64, pj, thresh, interval, 0, 2)
+    check_window(c64, pj, thresh, interval, 359, 361)
+    check_window(c64, pj, thresh, interval, 718, 720)
+
