@@ -498,8 +498,8 @@ def main():
 
     if args.gpus != 1:
         print(
-            f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU; "
-            "the multi-GPU design (conjunction steps sharded across GPUs) is described in DESIGN.md",
+            f"bench.py: --gpus {args.gpus} is not supported: the screening runs on one GPU "
+            "(conjunction steps are independent, so several GPUs would each take a contiguous range of them)",
             file=sys.stderr,
         )
         sys.exit(2)
