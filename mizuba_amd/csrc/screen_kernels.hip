@@ -1101,6 +1101,27 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav, c
     while (rb > 0 && codes[rb - 1u] == code) --rb;
     while (re < n && codes[re] == code) ++re;
     const bool multi = re - rb > 1u;
+    // partners found during the walk wait in registers and are recorded
+    // together (all lanes at once) when the buffer fills and at the end
+    constexpr uint32_t kBuf = 8;
+    uint32_t buf[kBuf];
+    uint32_t nbuf = 0;
+    auto flush = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < kBuf; ++k)
+            if (k < nbuf) {
+                const uint32_t oi = buf[k];
+                bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, head, ovf_count,
+                        ovf_cap);
+            }
+        nbuf = 0;
+    };
+    auto add = [&](uint32_t oi) {
+#pragma unroll
+        for (uint32_t k = 0; k < kBuf; ++k)
+            if (k == nbuf) buf[k] = oi;
+        if (++nbuf == kBuf) flush();
+    };
     float4 ql = bl, qu = bu;
     if (multi) {
         float l[4] = {bl.x, bl.y, bl.z, bl.w}, u[4] = {bu.x, bu.y, bu.z, bu.w};
@@ -1110,8 +1131,7 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav, c
         // pairs inside the run: every member overlaps the leaf box
         for (uint32_t m = o + 1u; m < re; ++m) {
             const uint32_t oi = vidx[m];
-            if (ot + otypes[oi] <= 3)
-                bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, head, ovf_count, ovf_cap);
+            if (ot + otypes[oi] <= 3) add(oi);
         }
     }
     uint32_t i = 0;
@@ -1146,13 +1166,12 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav, c
                     const float4 mu = reinterpret_cast<const float4 *>(boxes + (size_t)m * 8u)[1];
                     hit = box_overlap(ml, mu, ql, qu);
                 }
-                if (hit)
-                    bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, head, ovf_count,
-                            ovf_cap);
+                if (hit) add(oi);
             }
         }
         i = meta.x;
     }
+    flush();
 }
 
 // Place every object's pairs at its offset in (i, j) order: the inline
