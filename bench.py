@@ -371,7 +371,7 @@ def sgp4_flops_per_launch():
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 3 and cells[0] == "fp64_ops_per_launch":
+        if len(cells) >= 2 and cells[0] == "fp64_ops_per_launch":
             try:
                 return float(cells[1])
             except ValueError:

@@ -140,3 +140,15 @@ def test_synthetic_generator():
 def test_bench_rejects_multi_gpu():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_bench_reads_the_committed_profiles():
+    # the roofline fields of every bench leg come from rows of the committed
+    # rocprofv3 summaries; a row the parser misses would leave them null
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    for k in ("k_bp", "k_aabbs", "k_bvh"):
+        assert bench.hbm_traffic(k) > 0
+    assert bench.eval_hbm_traffic() > 0
+    assert bench.sgp4_flops_per_launch() > 1e9
