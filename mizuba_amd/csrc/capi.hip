@@ -464,8 +464,9 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
         // queued root-finding candidates: about a fifth of the pairs on LEO catalogs
         const size_t cand_bytes = narrow_cand_bytes(ctx->order);
         ctx->cand_cap = std::max<uint64_t>(ctx->cand_cap, chunk / 2 + 65536);
-        // overlapping trajectory-step pairs: about 1.5 per pair for T/8 steps and 3-minute conjunction steps
-        ctx->steps_cap = std::max<uint64_t>(ctx->steps_cap, 2 * chunk + 65536);
+        // overlapping trajectory-step pairs: about 1.5 per pair for T/8 steps, 2.2 for the T/20 steps
+        // of SGP4 polyjectories, with 3-minute conjunction steps (a larger need re-runs the pass)
+        ctx->steps_cap = std::max<uint64_t>(ctx->steps_cap, 3 * chunk + 65536);
         MZH_HIP(ctx->cand_count.ensure(sizeof(unsigned long long)));
         MZH_HIP(ctx->steps_count.ensure(sizeof(unsigned long long)));
         na.cand_count = ctx->cand_count.as<unsigned long long>();
