@@ -235,6 +235,16 @@ int mzh_get_eval_time(const mzh_ctx *ctx, double *seconds);
  * read with the accessors below. Error messages are the reference's. */
 int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode);
 
+/* The validation of conjunctions::conjunctions (mizuba/conjunctions.cpp:263-360)
+ * without a context or a device: the threshold, the interval, the object
+ * types (otypes may be NULL) and the number of conjunction steps
+ * ceil(maxT / conj_det_interval), returned in *n_cd_steps. Also checks this
+ * build's size limit: n_objs * n_cd_steps must not exceed 2^32 - 1. Same
+ * status and message (mzh_global_error) as mzh_screen for the same
+ * arguments. */
+int mzh_check_screen(uint64_t n_objs, double maxT, double conj_thresh, double conj_det_interval,
+                     const int32_t *otypes, int mode, uint64_t *n_cd_steps);
+
 /* Screen n independent problems together: ctxs[k] (each with its own uploaded
  * polyjectory) is screened with conj_thresh[k], conj_det_interval[k] and
  * otypes[k] (otypes may be NULL, as may any otypes[k]), exactly as
@@ -247,6 +257,17 @@ int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const
  * (mizuba/conjunctions.cpp:263-360) called once per problem. */
 int mzh_screen_batch(mzh_ctx *const *ctxs, uint64_t n, const double *conj_thresh, const double *conj_det_interval,
                      const int32_t *const *otypes, int mode, int *status);
+
+/* The narrow phase's root finder applied to n free-standing polynomials:
+ * run_poly_root_finding with direction +1 (mizuba/detail/poly_utils.cpp:272-460),
+ * the minima of the narrow phase. polys[k][0..order] (ascending powers) on
+ * [0, rf_int[k]); roots[k][0 .. 2*order+1] receives the n_roots[k] roots
+ * found, flags[k] (may be NULL) bit 0 = the fast exclusion check fired, bit 1
+ * = the isolation outgrew the private work list and ran with the 250-entry
+ * list. Host pointers. An isolation failure is MZH_EINVAL with the
+ * reference's message. */
+int mzh_root_finding(mzh_ctx *ctx, uint32_t order, uint64_t n, const double *polys, const double *rf_int,
+                     double *roots, uint32_t *n_roots, uint32_t *flags);
 
 /* ---- accessors (mizuba/conjunctions.cpp:394-509) ------------------------ */
 

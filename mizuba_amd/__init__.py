@@ -39,6 +39,7 @@ Screener = _core.Screener
 screen_batch = _core.screen_batch
 make_synthetic_polyjectory = _core.make_synthetic_polyjectory
 device_count = _core.device_count
+check_screen = _core.check_screen
 get_tmpdir = _core.get_tmpdir
 set_tmpdir = _core.set_tmpdir
 
@@ -63,6 +64,7 @@ __all__ = [
     "screen_batch",
     "make_synthetic_polyjectory",
     "device_count",
+    "check_screen",
     "get_tmpdir",
     "set_tmpdir",
     "gpe_dtype",

@@ -45,6 +45,18 @@ std::string fmt_double(double v)
     return s;
 }
 
+// Positive integer from the environment, or dflt. Test hooks: the initial
+// sizes of the growable screening buffers can be made small so that the
+// resize-and-rerun paths run on small problems.
+uint64_t env_u64(const char *name, uint64_t dflt)
+{
+    if (const char *v = std::getenv(name)) {
+        const unsigned long long x = std::strtoull(v, nullptr, 10);
+        if (x > 0) return x;
+    }
+    return dflt;
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -139,11 +151,10 @@ struct mzh_ctx {
     uint32_t ovf_cap = 0;
     DevBuf pairs;
     uint64_t n_pairs = 0;
-    DevBuf conj_raw, conj_step, conj_sorted, conj_count, np_report, conj_cnt, conj_start, conj_cursor, conj_pos,
-        conj_inv, conj_grand;
+    DevBuf conj_raw, conj_step, conj_sorted, conj_count, np_report, conj_keys;
     uint64_t conj_cap = 0, n_conj = 0;
-    DevBuf cand, cand_count, steps, steps_count;
-    uint64_t cand_cap = 0, steps_cap = 0;
+    DevBuf cand, cand_count, steps, steps_count, slow, slow_count, wl_slow;
+    uint64_t cand_cap = 0, steps_cap = 0, slow_cap = 0;
     DevBuf err;
     // polyjectory evaluation: staged times, selector and results, first bad element
     DevBuf ev_tm, ev_sel, ev_out, ev_bad;
@@ -224,9 +235,6 @@ std::string dev_error_message(const DevError &e)
         case kErrNpIsolation:
             return "The polynomial root isolation algorithm failed: the working list size is " + i
                    + " and the number of isolating intervals is " + j;
-        case kErrNpWorkList:
-            return "The polynomial root isolation algorithm failed: the working list size is "
-                   + std::to_string((uint64_t)e.value) + " and the number of isolating intervals is 0";
         case kErrNpIters:
             return "Polynomial root finding failed due to too many iterations";
         case kErrNpErrno:
@@ -269,10 +277,22 @@ PjView pj_view(const mzh_ctx *ctx)
     return v;
 }
 
-// conjunctions::conjunctions validation + detect_conjunctions, batched.
-void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes_in, int mode)
+// The arguments of one screening after validation.
+struct ScreenPlan {
+    uint64_t C = 0;
+    bool skip_cd = true;
+    std::vector<int32_t> ot;
+    std::vector<double> cd_begin, cd_end;
+};
+
+// conjunctions::conjunctions validation (conjunctions.cpp:279-335) with the
+// reference's messages, the step ranges of get_cd_begin_end
+// (conjunctions.cpp:373-392), and this build's size limit: the per-step
+// stages index (step, object) with 32-bit counts, so N * C must stay below
+// 2^32. Everything is checked before any device memory is touched.
+ScreenPlan plan_screen(uint32_t N, double maxT, double conj_thresh, double conj_det_interval,
+                       const int32_t *otypes_in, int mode)
 {
-    if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
     if (mode != MZH_MODE_FP64 && mode != MZH_MODE_MIXED)
         throw StatusError{MZH_EINVAL, "Invalid screening mode " + std::to_string(mode)};
     // conjunctions.cpp:279-296
@@ -286,39 +306,56 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
         throw StatusError{MZH_EINVAL,
                           "The conjunction detection interval must be finite and positive, but instead a value of "
                               + fmt_double(conj_det_interval) + " was provided"};
-    const uint32_t N = ctx->n_objs;
+    ScreenPlan pl;
     // otypes (conjunctions.cpp:305-332)
-    std::vector<int32_t> ot(N, 1);
-    bool skip_cd = true;
+    pl.ot.assign(N, 1);
     if (otypes_in) {
         for (uint32_t k = 0; k < N; ++k) {
             const int32_t v = otypes_in[k];
             if (v != 1 && v != 2 && v != 4)
                 throw StatusError{MZH_EINVAL, "The value of an object type must be one of [1, 2, 4], but a value of "
                                                   + std::to_string(v) + " was detected instead"};
-            skip_cd = skip_cd && (v != 1);
-            ot[k] = v;
+            pl.skip_cd = pl.skip_cd && (v != 1);
+            pl.ot[k] = v;
         }
     } else {
-        skip_cd = false;
+        pl.skip_cd = false;
     }
     // n_cd_steps (conjunctions.cpp:335)
-    const double cs = std::ceil(ctx->maxT / conj_det_interval);
+    const double cs = std::ceil(maxT / conj_det_interval);
     if (!(cs >= 1) || cs > 4.0e9)
         throw StatusError{MZH_EINVAL, "The number of conjunction steps " + fmt_double(cs) + " is not representable"};
-    const uint64_t C = (uint64_t)cs;
+    pl.C = (uint64_t)cs;
+    if ((uint64_t)N * pl.C > 0xffffffffull)
+        throw StatusError{MZH_EINVAL, "A screening of " + std::to_string(N) + " objects over " + std::to_string(pl.C)
+                                          + " conjunction steps is too large: the number of objects times the number "
+                                            "of conjunction steps must not exceed 4294967295"};
     // get_cd_begin_end for every step (conjunctions.cpp:373-392)
-    ctx->h_cd_begin.resize(C);
-    ctx->h_cd_end.resize(C);
-    for (uint64_t k = 0; k < C; ++k) {
+    pl.cd_begin.resize(pl.C);
+    pl.cd_end.resize(pl.C);
+    for (uint64_t k = 0; k < pl.C; ++k) {
         const double b = conj_det_interval * static_cast<double>(k);
-        const double e = (k == C - 1) ? ctx->maxT : conj_det_interval * static_cast<double>(k + 1);
-        if (!std::isfinite(b) || !std::isfinite(e) || !(e > b) || b < 0 || e > ctx->maxT)
+        const double e = (k == pl.C - 1) ? maxT : conj_det_interval * static_cast<double>(k + 1);
+        if (!std::isfinite(b) || !std::isfinite(e) || !(e > b) || b < 0 || e > maxT)
             throw StatusError{MZH_EINVAL, "Invalid conjunction step time range: [" + fmt_double(b) + ", "
                                               + fmt_double(e) + ")"};
-        ctx->h_cd_begin[k] = b;
-        ctx->h_cd_end[k] = e;
+        pl.cd_begin[k] = b;
+        pl.cd_end[k] = e;
     }
+    return pl;
+}
+
+// conjunctions::conjunctions validation + detect_conjunctions, batched.
+void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes_in, int mode)
+{
+    if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
+    const uint32_t N = ctx->n_objs;
+    ScreenPlan pl = plan_screen(N, ctx->maxT, conj_thresh, conj_det_interval, otypes_in, mode);
+    const uint64_t C = pl.C;
+    const bool skip_cd = pl.skip_cd;
+    const std::vector<int32_t> &ot = pl.ot;
+    ctx->h_cd_begin = std::move(pl.cd_begin);
+    ctx->h_cd_end = std::move(pl.cd_end);
     const size_t boxes_bytes = sizeof(float) * 8u * (size_t)(N + 1) * C;
     const size_t NC = (size_t)N * C;
     hipStream_t st = ctx->stream;
@@ -352,7 +389,8 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(ctx->cnt.ensure(sizeof(uint32_t) * NC));
     MZH_HIP(ctx->head.ensure(sizeof(uint32_t) * NC));
     MZH_HIP(ctx->slots.ensure(sizeof(uint32_t) * 4u * NC));
-    if (ctx->ovf_cap == 0) ctx->ovf_cap = (uint32_t)std::min<size_t>(NC / 8 + 4096, 0x7fffffffu);
+    if (ctx->ovf_cap == 0)
+        ctx->ovf_cap = (uint32_t)std::min<uint64_t>(env_u64("MIZUBA_AMD_INIT_OVF_CAP", NC / 8 + 4096), 0x7fffffffu);
     MZH_HIP(ctx->ovf.ensure(sizeof(uint2) * ctx->ovf_cap));
     MZH_HIP(ctx->ovf_count.ensure(sizeof(uint32_t)));
     MZH_HIP(ctx->off.ensure(sizeof(uint64_t) * NC));
@@ -362,10 +400,6 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     MZH_HIP(ctx->err.ensure(sizeof(DevError)));
     MZH_HIP(ctx->conj_count.ensure(sizeof(unsigned long long)));
     MZH_HIP(ctx->np_report.ensure(sizeof(unsigned long long) * MZH_NP_COUNT));
-    MZH_HIP(ctx->conj_cnt.ensure(sizeof(uint64_t) * C));
-    MZH_HIP(ctx->conj_start.ensure(sizeof(uint64_t) * (C + 1)));
-    MZH_HIP(ctx->conj_cursor.ensure(sizeof(uint32_t) * C));
-    MZH_HIP(ctx->conj_grand.ensure(sizeof(uint64_t)));
 
     MZH_HIP(hipMemcpyAsync(ctx->cd_begin.p, ctx->h_cd_begin.data(), sizeof(double) * C, hipMemcpyHostToDevice, st));
     MZH_HIP(hipMemcpyAsync(ctx->cd_end.p, ctx->h_cd_end.data(), sizeof(double) * C, hipMemcpyHostToDevice, st));
@@ -403,8 +437,6 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     ctx->n_conj = 0;
     if (!skip_cd) {
         // stage 4: broad phase
-        if ((uint64_t)N * C >= 0xffffffffull)
-            throw StatusError{MZH_EINVAL, "Too many objects x conjunction steps for one screening"};
         uint32_t n_ovf = 0;
         for (int attempt = 0; attempt < 3; ++attempt) {
             MZH_HIP(launch_bp_reset(ctx->cnt.as<uint32_t>(), ctx->head.as<uint32_t>(), ctx->ovf_count.as<uint32_t>(), N,
@@ -454,23 +486,27 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
         na.np_report = ctx->np_report.as<unsigned long long>();
         na.conj_count = ctx->conj_count.as<unsigned long long>();
         na.err = derr;
-        uint64_t pass_pairs = 64ull << 20;  // bounds the queues to a few GB
-        if (const char *env = std::getenv("MIZUBA_AMD_NARROW_PASS_PAIRS")) {
-            const unsigned long long v = std::strtoull(env, nullptr, 10);
-            if (v > 0) pass_pairs = v;
-        }
+        // bounds the queues to a few GB
+        const uint64_t pass_pairs = env_u64("MIZUBA_AMD_NARROW_PASS_PAIRS", 64ull << 20);
         const uint64_t chunk = std::min<uint64_t>(total, pass_pairs);
-        if (ctx->conj_cap == 0) ctx->conj_cap = 1u << 16;
+        if (ctx->conj_cap == 0) ctx->conj_cap = env_u64("MIZUBA_AMD_INIT_CONJ_CAP", 1u << 16);
         // queued root-finding candidates: about a fifth of the pairs on LEO catalogs
         const size_t cand_bytes = narrow_cand_bytes(ctx->order);
-        ctx->cand_cap = std::max<uint64_t>(ctx->cand_cap, chunk / 2 + 65536);
+        ctx->cand_cap = std::max<uint64_t>(ctx->cand_cap, env_u64("MIZUBA_AMD_INIT_CAND_CAP", chunk / 2 + 65536));
         // overlapping trajectory-step pairs: about 1.5 per pair for T/8 steps, 2.2 for the T/20 steps
         // of SGP4 polyjectories, with 3-minute conjunction steps (a larger need re-runs the pass)
-        ctx->steps_cap = std::max<uint64_t>(ctx->steps_cap, 3 * chunk + 65536);
+        ctx->steps_cap = std::max<uint64_t>(ctx->steps_cap, env_u64("MIZUBA_AMD_INIT_STEPS_CAP", 3 * chunk + 65536));
+        // candidates whose root isolation outgrows the private work list (rare)
+        ctx->slow_cap = std::max<uint64_t>(ctx->slow_cap, env_u64("MIZUBA_AMD_INIT_SLOW_CAP", 4096));
+        na.wl_fast_cap = (int)std::min<uint64_t>(env_u64("MIZUBA_AMD_WL_FAST_CAP", kNarrowWlFastCap), kNarrowWlFastCap);
         MZH_HIP(ctx->cand_count.ensure(sizeof(unsigned long long)));
         MZH_HIP(ctx->steps_count.ensure(sizeof(unsigned long long)));
+        MZH_HIP(ctx->slow_count.ensure(sizeof(unsigned long long)));
+        MZH_HIP(ctx->wl_slow.ensure(narrow_wl_slow_bytes(ctx->order)));
         na.cand_count = ctx->cand_count.as<unsigned long long>();
         na.steps_count = ctx->steps_count.as<unsigned long long>();
+        na.slow_count = ctx->slow_count.as<unsigned long long>();
+        na.wl_slow = ctx->wl_slow.p;
         MZH_HIP(ctx->conj_raw.grow(sizeof(mzh_conj) * ctx->conj_cap, 0, st));
         MZH_HIP(ctx->conj_step.grow(sizeof(uint32_t) * ctx->conj_cap, 0, st));
         MZH_HIP(hipMemsetAsync(ctx->conj_count.p, 0, sizeof(unsigned long long), st));
@@ -482,11 +518,12 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
             na.pair_begin = pb;
             na.pair_end = std::min<uint64_t>(total, pb + chunk);
             bool ok = false;
-            for (int attempt = 0; attempt < 4 && !ok; ++attempt) {
+            for (int attempt = 0; attempt < 8 && !ok; ++attempt) {
                 MZH_HIP(ctx->conj_raw.grow(sizeof(mzh_conj) * ctx->conj_cap, sizeof(mzh_conj) * done_conj, st));
                 MZH_HIP(ctx->conj_step.grow(sizeof(uint32_t) * ctx->conj_cap, sizeof(uint32_t) * done_conj, st));
                 MZH_HIP(ctx->cand.ensure(cand_bytes * ctx->cand_cap));
                 MZH_HIP(ctx->steps.ensure(sizeof(StepPair) * ctx->steps_cap));
+                MZH_HIP(ctx->slow.ensure(sizeof(uint64_t) * ctx->slow_cap));
                 if (attempt > 0) {
                     // undo the overflowed pass
                     MZH_HIP(hipMemcpyAsync(ctx->conj_count.p, &done_conj, sizeof(done_conj), hipMemcpyHostToDevice,
@@ -495,6 +532,9 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
                 }
                 MZH_HIP(hipMemsetAsync(ctx->cand_count.p, 0, sizeof(unsigned long long), st));
                 MZH_HIP(hipMemsetAsync(ctx->steps_count.p, 0, sizeof(unsigned long long), st));
+                MZH_HIP(hipMemsetAsync(ctx->slow_count.p, 0, sizeof(unsigned long long), st));
+                na.slow = ctx->slow.as<uint64_t>();
+                na.slow_cap = ctx->slow_cap;
                 na.steps = ctx->steps.p;
                 na.steps_cap = ctx->steps_cap;
                 na.conj = ctx->conj_raw.as<mzh_conj>();
@@ -503,18 +543,20 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
                 na.cand = ctx->cand.p;
                 na.cand_cap = ctx->cand_cap;
                 MZH_HIP(launch_narrow(na, st));
-                unsigned long long cnts[3] = {0, 0, 0}, rep[MZH_NP_COUNT];
+                unsigned long long cnts[4] = {0, 0, 0, 0}, rep[MZH_NP_COUNT];
                 MZH_HIP(hipMemcpyAsync(&cnts[0], ctx->conj_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                        st));
                 MZH_HIP(hipMemcpyAsync(&cnts[1], ctx->cand_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                        st));
                 MZH_HIP(hipMemcpyAsync(&cnts[2], ctx->steps_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                        st));
+                MZH_HIP(hipMemcpyAsync(&cnts[3], ctx->slow_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                       st));
                 MZH_HIP(hipMemcpyAsync(rep, ctx->np_report.p, sizeof(rep), hipMemcpyDeviceToHost, st));
                 MZH_HIP(hipStreamSynchronize(st));
                 const bool conj_ok = cnts[0] <= ctx->conj_cap, cand_ok = cnts[1] <= ctx->cand_cap,
-                           steps_ok = cnts[2] <= ctx->steps_cap;
-                ok = conj_ok && cand_ok && steps_ok;
+                           steps_ok = cnts[2] <= ctx->steps_cap, slow_ok = cnts[3] <= ctx->slow_cap;
+                ok = conj_ok && cand_ok && steps_ok && slow_ok;
                 if (ok) {
                     done_conj = cnts[0];
                     std::memcpy(done_rep, rep, sizeof(rep));
@@ -523,22 +565,19 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
                 if (!conj_ok) ctx->conj_cap = cnts[0] + cnts[0] / 4 + 1024;
                 if (!cand_ok) ctx->cand_cap = cnts[1] + cnts[1] / 4 + 1024;
                 if (!steps_ok) ctx->steps_cap = cnts[2] + cnts[2] / 4 + 1024;
+                if (!slow_ok) ctx->slow_cap = cnts[3] + cnts[3] / 4 + 1024;
             }
             if (!ok) throw StatusError{MZH_ERUNTIME, "narrow-phase buffers overflowed after resizing"};
         }
         MZH_HIP(hipEventRecord(ctx->ev[9], st));
         ctx->n_conj = done_conj;
         const uint64_t M = ctx->n_conj;
+        if (M >= 0xffffffffull) throw StatusError{MZH_ERUNTIME, "Too many conjunctions for one screening"};
         MZH_HIP(ctx->conj_sorted.ensure(sizeof(mzh_conj) * (M ? M : 1)));
-        MZH_HIP(ctx->conj_pos.ensure(sizeof(uint32_t) * (M ? M : 1)));
-        MZH_HIP(ctx->conj_inv.ensure(sizeof(uint32_t) * (M ? M : 1)));
-        MZH_HIP(launch_conj_hist(ctx->conj_step.as<uint32_t>(), M, ctx->conj_cnt.as<uint64_t>(), (uint32_t)C, st));
-        MZH_HIP(launch_scan_steps(ctx->conj_cnt.as<uint64_t>(), ctx->conj_start.as<uint64_t>(),
-                                  ctx->conj_grand.as<uint64_t>(), (uint32_t)C, st));
-        MZH_HIP(launch_conj_order(ctx->conj_raw.as<mzh_conj>(), ctx->conj_step.as<uint32_t>(), M,
-                                  ctx->conj_start.as<uint64_t>(), ctx->conj_cnt.as<uint64_t>(),
-                                  ctx->conj_cursor.as<uint32_t>(), ctx->conj_pos.as<uint32_t>(),
-                                  ctx->conj_inv.as<uint32_t>(), ctx->conj_sorted.as<mzh_conj>(), (uint32_t)C, st));
+        MZH_HIP(ctx->conj_keys.ensure(2u * sizeof(ConjKey) * (M ? M : 1)));
+        MZH_HIP(launch_conj_sort(ctx->conj_raw.as<mzh_conj>(), ctx->conj_step.as<uint32_t>(), M,
+                                 ctx->conj_keys.as<ConjKey>(), ctx->conj_keys.as<ConjKey>() + M,
+                                 ctx->conj_sorted.as<mzh_conj>(), st));
     } else {
         MZH_HIP(hipMemsetAsync(ctx->step_off.p, 0, sizeof(uint64_t) * (C + 1), st));
         MZH_HIP(hipEventRecord(ctx->ev[4], st));
@@ -1241,6 +1280,77 @@ int mzh_get_eval_time(const mzh_ctx *ctx, double *seconds)
     if (!ctx || !seconds) return MZH_EINVAL;
     *seconds = ctx->eval_s;
     return MZH_OK;
+}
+
+int mzh_check_screen(uint64_t n_objs, double maxT, double conj_thresh, double conj_det_interval,
+                     const int32_t *otypes, int mode, uint64_t *n_cd_steps)
+{
+    try {
+        if (n_objs > 0xffffffffull)
+            throw StatusError{MZH_EINVAL, "Too many objects for one screening: " + std::to_string(n_objs)};
+        if (!std::isfinite(maxT) || maxT <= 0) throw StatusError{MZH_EINVAL, "Invalid maxT " + fmt_double(maxT)};
+        const ScreenPlan pl = plan_screen((uint32_t)n_objs, maxT, conj_thresh, conj_det_interval, otypes, mode);
+        if (n_cd_steps) *n_cd_steps = pl.C;
+        return MZH_OK;
+    } catch (const StatusError &e) {
+        g_global_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_global_error = "host memory allocation failed";
+        return MZH_ENOMEM;
+    }
+}
+
+int mzh_root_finding(mzh_ctx *ctx, uint32_t order, uint64_t n, const double *polys, const double *rf_int,
+                     double *roots, uint32_t *n_roots, uint32_t *flags)
+{
+    return guarded(ctx, [&] {
+        if (order < (uint32_t)kMinOrder || order > (uint32_t)kMaxOrder)
+            throw StatusError{MZH_EINVAL, "Polynomial order " + std::to_string(order) + " is not supported"};
+        if (n == 0) return;
+        if (!polys || !rf_int || !roots || !n_roots) throw StatusError{MZH_EINVAL, "Null pointer passed to mzh_root_finding"};
+        for (uint64_t k = 0; k < n; ++k)
+            if (!std::isfinite(rf_int[k]) || !(rf_int[k] > 0))
+                throw StatusError{MZH_EINVAL, "Invalid root-finding interval " + fmt_double(rf_int[k])};
+        const uint32_t op1 = order + 1, nmax = 2u * order + 2u;
+        hipStream_t st = ctx->stream;
+        DevBuf d_p, d_rf, d_roots, d_info, d_slow, d_cnt, d_wl, d_err;
+        MZH_HIP(d_p.ensure(sizeof(double) * op1 * n));
+        MZH_HIP(d_rf.ensure(sizeof(double) * n));
+        MZH_HIP(d_roots.ensure(sizeof(double) * nmax * n));
+        MZH_HIP(d_info.ensure(sizeof(uint32_t) * n));
+        MZH_HIP(d_slow.ensure(sizeof(uint64_t) * n));
+        MZH_HIP(d_cnt.ensure(sizeof(unsigned long long)));
+        MZH_HIP(d_wl.ensure(narrow_wl_slow_bytes(order)));
+        MZH_HIP(d_err.ensure(sizeof(DevError)));
+        MZH_HIP(hipMemcpyAsync(d_p.p, polys, sizeof(double) * op1 * n, hipMemcpyHostToDevice, st));
+        MZH_HIP(hipMemcpyAsync(d_rf.p, rf_int, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        MZH_HIP(hipMemsetAsync(d_cnt.p, 0, sizeof(unsigned long long), st));
+        MZH_HIP(hipMemsetAsync(d_err.p, 0, sizeof(DevError), st));
+        RootFindArgs a;
+        a.polys = d_p.as<const double>();
+        a.rf_int = d_rf.as<const double>();
+        a.n = n;
+        a.roots = d_roots.as<double>();
+        a.info = d_info.as<uint32_t>();
+        a.slow = d_slow.as<uint64_t>();
+        a.slow_count = d_cnt.as<unsigned long long>();
+        a.wl_slow = d_wl.p;
+        a.wl_fast_cap = (int)std::min<uint64_t>(env_u64("MIZUBA_AMD_WL_FAST_CAP", kNarrowWlFastCap), kNarrowWlFastCap);
+        a.err = d_err.as<DevError>();
+        MZH_HIP(launch_rootfind(order, a, st));
+        std::vector<uint32_t> info(n);
+        DevError herr;
+        MZH_HIP(hipMemcpyAsync(info.data(), d_info.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+        MZH_HIP(hipMemcpyAsync(roots, d_roots.p, sizeof(double) * nmax * n, hipMemcpyDeviceToHost, st));
+        MZH_HIP(hipMemcpyAsync(&herr, d_err.p, sizeof(DevError), hipMemcpyDeviceToHost, st));
+        MZH_HIP(hipStreamSynchronize(st));
+        if (herr.code != kErrNone) throw StatusError{MZH_EINVAL, dev_error_message(herr)};
+        for (uint64_t k = 0; k < n; ++k) {
+            n_roots[k] = info[k] & 0xffu;
+            if (flags) flags[k] = (info[k] >> 8) & 3u;
+        }
+    });
 }
 
 int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode)

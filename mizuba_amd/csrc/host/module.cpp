@@ -849,6 +849,19 @@ void check_otypes_size(const std::optional<std::vector<int32_t>> &otypes, uint64
                                     + std::to_string(otypes->size()) + " instead");
 }
 
+// The constructor's checks in the reference's order (conjunctions.cpp:267-332:
+// threshold, interval, then the size and values of otypes), before any device
+// context is made; then this build's size limit (mzh_check_screen).
+void validate_screen(const PjData &pd, double conj_thresh, double conj_det_interval,
+                     const std::optional<std::vector<int32_t>> &otypes, int mode)
+{
+    check(mzh_check_screen(pd.n_objs, pd.maxT, conj_thresh, conj_det_interval, nullptr, mode, nullptr), nullptr);
+    check_otypes_size(otypes, pd.n_objs);
+    if (otypes)
+        check(mzh_check_screen(pd.n_objs, pd.maxT, conj_thresh, conj_det_interval, otypes->data(), mode, nullptr),
+              nullptr);
+}
+
 Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double conj_det_interval,
                                std::optional<std::vector<int32_t>> otypes,
                                std::optional<std::filesystem::path> data_dir,
@@ -858,7 +871,8 @@ Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double
         throw std::invalid_argument(
             "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time");
     const int m = parse_mode(mode);
-    (void)attached(pj);
+    const PjData &pd = attached(pj);
+    validate_screen(pd, conj_thresh, conj_det_interval, otypes, m);
     auto im = std::make_shared<ConjImpl>();
     // the result files are written when a data dir or a temporary dir is
     // given; the dir is made first, as the reference does (conjunctions.cpp:336-341)
@@ -894,8 +908,10 @@ std::vector<Conjunctions> make_conjunctions_batch(const std::vector<Polyjectory>
     std::vector<std::shared_ptr<ConjImpl>> ims(n);
     std::vector<mzh_ctx *> ctxs(n);
     std::vector<const int32_t *> ots(n, nullptr);
+    for (size_t k = 0; k < n; ++k)
+        validate_screen(attached(pjs[k]), conj_threshs[k], conj_det_intervals[k],
+                        otypes ? (*otypes)[k] : std::optional<std::vector<int32_t>>{}, m);
     for (size_t k = 0; k < n; ++k) {
-        if (otypes) check_otypes_size((*otypes)[k], attached(pjs[k]).n_objs);
         ims[k] = std::make_shared<ConjImpl>();
         ims[k]->ctx = std::make_unique<Ctx>(device);
         ims[k]->ctx->upload(pjs[k].d);
@@ -1270,6 +1286,29 @@ PYBIND11_MODULE(_core, m)
                  check(mzh_get_eval_time(c.ctx, &s), c.ctx);
                  return s;
              })
+        .def(
+            "root_finding",
+            [](Ctx &c, const py::array_t<double, py::array::c_style | py::array::forcecast> &polys,
+               const py::array_t<double, py::array::c_style | py::array::forcecast> &rf_int) {
+                // the narrow phase's root finder on free-standing polynomials
+                // (mzh_root_finding): returns (roots [n][2 order + 2], counts, flags)
+                if (polys.ndim() != 2 || rf_int.ndim() != 1 || rf_int.shape(0) != polys.shape(0))
+                    throw std::invalid_argument("root_finding: polys must be [n, order + 1] and rf_int [n]");
+                const uint64_t n = polys.shape(0);
+                const uint32_t order = (uint32_t)polys.shape(1) - 1u;
+                py::array_t<double> roots({(py::ssize_t)n, (py::ssize_t)(2 * order + 2)});
+                py::array_t<uint32_t> cnt((py::ssize_t)n), flags((py::ssize_t)n);
+                std::fill(roots.mutable_data(), roots.mutable_data() + roots.size(), 0.0);
+                int st;
+                {
+                    py::gil_scoped_release rel;
+                    st = mzh_root_finding(c.ctx, order, n, polys.data(), rf_int.data(), roots.mutable_data(),
+                                          cnt.mutable_data(), flags.mutable_data());
+                }
+                check(st, c.ctx);
+                return py::make_tuple(roots, cnt, flags);
+            },
+            "polys"_a, "rf_int"_a)
         .def("conjunctions", [](Ctx &c) {
             uint64_t n;
             check(mzh_get_n_conjunctions(c.ctx, &n), c.ctx);
@@ -1306,6 +1345,22 @@ PYBIND11_MODULE(_core, m)
           "otypes"_a = py::none(), "mode"_a = "fp64", "device"_a = 0,
           "Screen several independent polyjectories together on one GPU; returns one conjunctions object per "
           "problem, identical to constructing them one by one.");
+
+    m.def(
+        "check_screen",
+        [](uint64_t n_objs, double maxT, double conj_thresh, double conj_det_interval,
+           std::optional<std::vector<int32_t>> otypes, const std::string &mode) {
+            // the constructor's validation without a device (mzh_check_screen)
+            const int m = parse_mode(mode);
+            uint64_t C = 0;
+            check(mzh_check_screen(n_objs, maxT, conj_thresh, conj_det_interval, nullptr, m, &C), nullptr);
+            check_otypes_size(otypes, n_objs);
+            if (otypes)
+                check(mzh_check_screen(n_objs, maxT, conj_thresh, conj_det_interval, otypes->data(), m, &C), nullptr);
+            return C;
+        },
+        "n_objs"_a, "maxT"_a, "conj_thresh"_a, "conj_det_interval"_a, "otypes"_a = py::none(), "mode"_a = "fp64",
+        "Validate screening arguments as conjunctions() does and return the number of conjunction steps.");
 
     m.def("device_count", []() {
         int n = 0;

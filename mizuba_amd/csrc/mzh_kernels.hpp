@@ -92,6 +92,14 @@ struct NarrowArgs {
     void *cand;                   // records of narrow_cand_bytes(order) bytes
     uint64_t cand_cap;
     unsigned long long *cand_count;
+    // root isolation: entries of the private work list (<= 48), and the
+    // candidates that outgrow it (indices into cand), isolated again with
+    // kNarrowSlowLanes work lists of 251 entries in wl_slow
+    int wl_fast_cap;
+    uint64_t *slow;
+    uint64_t slow_cap;
+    unsigned long long *slow_count;
+    void *wl_slow;                // narrow_wl_slow_bytes(order)
 };
 // One overlapping pair of trajectory steps of a broad-phase pair.
 struct StepPair {
@@ -99,17 +107,50 @@ struct StepPair {
 };
 // Bytes of one queued root-finding candidate for a polynomial order.
 size_t narrow_cand_bytes(uint32_t order);
+// Root isolation work lists: an entry holds an interval and a polynomial; the
+// slow pass runs kNarrowSlowLanes lanes, each with a list of 251 entries.
+constexpr size_t narrow_wl_entry_bytes(uint32_t order) { return 8u * (order + 3u); }
+constexpr uint32_t kNarrowSlowLanes = 256;
+inline size_t narrow_wl_slow_bytes(uint32_t order) { return kNarrowSlowLanes * 251u * narrow_wl_entry_bytes(order); }
+constexpr int kNarrowWlFastCap = 48;
+
+// Root finding of free-standing polynomials (run_poly_root_finding with
+// direction +1, poly_utils.cpp:272-460), as the narrow phase does it.
+//   polys [n][order + 1], rf_int [n] (device)
+//   roots [n][2 * order + 2]: the roots found, info[k] = number of roots |
+//   (fast exclusion) << 8 | (isolated on the slow path) << 9, 0xfffffffe on
+//   error (the first error in err, with i = the polynomial's index).
+//   slow: n entries of scratch; wl_slow: narrow_wl_slow_bytes(order).
+struct RootFindArgs {
+    const double *polys;
+    const double *rf_int;
+    uint64_t n;
+    double *roots;
+    uint32_t *info;
+    uint64_t *slow;
+    unsigned long long *slow_count;
+    void *wl_slow;
+    int wl_fast_cap;
+    DevError *err;
+};
+hipError_t launch_rootfind(uint32_t order, const RootFindArgs &a, hipStream_t stream);
+template <int ORDER>
+hipError_t launch_rootfind_order(const RootFindArgs &a, hipStream_t stream);
 hipError_t launch_narrow(const NarrowArgs &a, hipStream_t stream);
 template <int ORDER>
 hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream);
 
-// Group conjunction records by step and sort each step by (tca, lexicographic):
-// launch_conj_hist counts records per step into cnt[C]; the caller scans cnt
-// into start[C + 1] (launch_scan_steps); launch_conj_order scatters and ranks.
-hipError_t launch_conj_hist(const uint32_t *in_step, uint64_t n, uint64_t *cnt, uint32_t n_steps, hipStream_t stream);
-hipError_t launch_conj_order(const mzh_conj *in, const uint32_t *in_step, uint64_t n, const uint64_t *start,
-                             const uint64_t *cnt, uint32_t *cursor, uint32_t *pos, uint32_t *inv, mzh_conj *out,
-                             uint32_t n_steps, hipStream_t stream);
+// Group conjunction records by step and sort each step by (tca, then every
+// field lexicographically): a merge sort of 32-byte keys, ping-ponging between
+// keys0 and keys1 (n entries each); the sorted records go to out.
+struct ConjKey {
+    uint32_t step, idx;
+    uint64_t tca;  // bits of tca + 0.0
+    uint64_t ij;   // i << 32 | j
+    uint64_t pad;
+};
+hipError_t launch_conj_sort(const mzh_conj *in, const uint32_t *in_step, uint64_t n, ConjKey *keys0, ConjKey *keys1,
+                            mzh_conj *out, hipStream_t stream);
 
 // ---- SGP4 (near-earth model) ------------------------------------------------
 // Deep-space (SDP4) terms of an element set with a period of 225 minutes or

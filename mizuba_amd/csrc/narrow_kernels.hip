@@ -34,7 +34,17 @@ namespace
 constexpr BinomTable kBinomN = make_binom_table();
 __constant__ BinomTable c_binom_np = kBinomN;
 
-constexpr int kWlCap = 48;  // device work-list capacity of the root isolation
+// Work list of the root isolation. A lane first isolates with a private list
+// of kWlCap entries; a candidate whose list outgrows it is isolated again by
+// the slow pass with a list of kWlSlowCap entries in global memory, so the
+// reference's limit (more than 250 pending intervals is an error,
+// poly_utils.cpp:417-421) is the only one that can fail a screening. With
+// the bound checked after every split, 250 entries plus the two children of
+// the last split never exceed kWlSlowCap.
+constexpr int kWlCap = 48;
+constexpr int kWlSlowCap = 251;
+// find_minima results
+constexpr int kFmOk = 0, kFmError = 1, kFmListFull = 2;
 // capacity of the per-interval root list (isolated roots plus roots found
 // exactly at a work-list interval's lower end)
 template <int ORDER>
@@ -383,20 +393,24 @@ struct WLEntry {
     double lb, ub;
     double p[ORDER + 1];
 };
+static_assert(sizeof(WLEntry<8>) == narrow_wl_entry_bytes(8), "work-list entry size");
+static_assert(sizeof(WLEntry<20>) == narrow_wl_entry_bytes(20), "work-list entry size");
 
-// run_poly_root_finding with direction +1. Returns false (and reports) on error.
+// run_poly_root_finding with direction +1 (poly_utils.cpp:272-460), with the
+// work list wl of `cap` entries. Returns kFmOk, kFmError (reported to err) or
+// kFmListFull when a split would take the list beyond cap (nothing reported:
+// the caller runs it again with a larger list).
 template <int ORDER>
-__device__ bool find_minima(const double *poly, double rf_int, double *roots, int &nroots, bool &fex, uint32_t i,
-                            uint32_t j, DevError *err)
+__device__ int find_minima(const double *poly, double rf_int, double *roots, int &nroots, bool &fex, uint32_t i,
+                           uint32_t j, DevError *err, WLEntry<ORDER> *wl, int cap)
 {
     constexpr int OP1 = ORDER + 1;
     nroots = 0;
     fex = false;
     if (fex_excludes<ORDER>(poly, rf_int)) {
         fex = true;
-        return true;
+        return kFmOk;
     }
-    WLEntry<ORDER> wl[kWlCap];
     double iso_lb[ORDER + 1], iso_ub[ORDER + 1];
     int nwl = 0, niso = 0;
 
@@ -441,22 +455,19 @@ __device__ bool find_minima(const double *poly, double rf_int, double *roots, in
             bool allf = true;
 #pragma unroll
             for (int k = 1; k < OP1; ++k) allf = allf && is_finite_d(q[k]);
-            if (allf && !add_root(lb * rf_int)) return false;
+            if (allf && !add_root(lb * rf_int)) return kFmError;
         }
         const uint32_t nsc = rtscc<ORDER>(q);
         if (nsc == 1) {
             if (niso >= ORDER) {
                 report_error(err, kErrNpIsolation, (uint32_t)nwl, (uint32_t)(niso + 1), 0.0);
-                return false;
+                return kFmError;
             }
             iso_lb[niso] = lb;
             iso_ub[niso] = ub;
             ++niso;
         } else if (nsc > 1) {
-            if (nwl + 2 > kWlCap) {
-                report_error(err, kErrNpWorkList, i, j, (double)(nwl + 2));
-                return false;
-            }
+            if (nwl + 2 > cap) return kFmListFull;
             double r[OP1], s[OP1];
             double cf = 1.0;
 #pragma unroll
@@ -479,7 +490,7 @@ __device__ bool find_minima(const double *poly, double rf_int, double *roots, in
         }
         if (nwl > 250) {
             report_error(err, kErrNpIsolation, (uint32_t)nwl, (uint32_t)niso, 0.0);
-            return false;
+            return kFmError;
         }
     } while (nwl != 0);
 
@@ -500,16 +511,16 @@ __device__ bool find_minima(const double *poly, double rf_int, double *roots, in
             const int e = solver.solve(lb, ub, 100, ra, rb, it);
             if (e != 0) {
                 report_error(err, kErrNpErrno, i, j, (double)e);
-                return false;
+                return kFmError;
             }
             if (it >= 100) {
                 report_error(err, kErrNpIters, i, j, 0.0);
-                return false;
+                return kFmError;
             }
-            if (!add_root(((ra / 2) + (rb / 2)) * rf_int)) return false;
+            if (!add_root(((ra / 2) + (rb / 2)) * rf_int)) return kFmError;
         }
     }
-    return true;
+    return kFmOk;
 }
 
 // Bjorck-Pereyra interpolation in place: f -> monomial coefficients
@@ -773,86 +784,86 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
     }
 }
 
-// Narrow phase, part 2: one lane per queued candidate (grid-stride over the
-// queue, whose length is read on the device). Minima of dist^2 from the roots
-// of its derivative (narrow_phase.cpp:519-605), boundary minima (:629-714).
+// Minima of dist^2 for one queued candidate (narrow_phase.cpp:519-605) and
+// its boundary minima (:629-714), with the root-isolation work list wl of cap
+// entries. rep accumulates np_report[2..6]. Returns kFmListFull, before
+// anything is counted or emitted, when the work list is too small.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_np_roots(NarrowArgs a)
+__device__ inline int np_root_cand(const NarrowArgs &a, uint64_t q, unsigned long long *rep, WLEntry<ORDER> *wl,
+                                   int cap)
 {
     constexpr int OP1 = ORDER + 1;
-    const unsigned long long avail = *a.cand_count;
-    const uint64_t n = avail < a.cand_cap ? avail : a.cand_cap;
     const double thresh2 = a.conj_thresh * a.conj_thresh;
-    unsigned long long rep[5] = {0, 0, 0, 0, 0};
-    const uint64_t stride = (uint64_t)gridDim.x * 256u;
-    for (uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x; q < n; q += stride) {
-        const NpCand<ORDER> &cd = static_cast<const NpCand<ORDER> *>(a.cand)[q];
-        double d2[OP1];
+    const NpCand<ORDER> &cd = static_cast<const NpCand<ORDER> *>(a.cand)[q];
+    double d2[OP1];
 #pragma unroll
-        for (int k = 0; k < OP1; ++k) d2[k] = cd.d2[k];
-        const double rf_int = cd.rf_int, lb_rf = cd.lb_rf;
-        const uint32_t i = cd.i, j = cd.j, c = cd.c, flags = cd.flags;
-        const double *Pi = a.pj.traj + (a.pj.step0[i] + cd.ki) * (uint64_t)OP1 * 7u;
-        const double *Pj = a.pj.traj + (a.pj.step0[j] + cd.kj) * (uint64_t)OP1 * 7u;
-        double d2p[OP1];
+    for (int k = 0; k < OP1; ++k) d2[k] = cd.d2[k];
+    const double rf_int = cd.rf_int, lb_rf = cd.lb_rf;
+    const uint32_t i = cd.i, j = cd.j, c = cd.c, flags = cd.flags;
+    const double *Pi = a.pj.traj + (a.pj.step0[i] + cd.ki) * (uint64_t)OP1 * 7u;
+    const double *Pj = a.pj.traj + (a.pj.step0[j] + cd.kj) * (uint64_t)OP1 * 7u;
+    double d2p[OP1];
 #pragma unroll
-        for (int k = 0; k < ORDER; ++k) d2p[k] = (double)(k + 1) * d2[k + 1];
-        d2p[ORDER] = 0;
-        double roots[kMaxRoots<ORDER>];
-        int nr = 0;
-        bool fex = false;
-        if (!find_minima<ORDER>(d2p, rf_int, roots, nr, fex, i, j, a.err)) continue;
-        ++rep[0];
-        rep[1] += fex ? 1 : 0;
-        rep[2] += (nr == 0) ? 1 : 0;
-        rep[3] += (unsigned long long)nr;
-        bool failed = false;
-        for (int r = 0; r < nr; ++r) {
-            const double cd2 = horner<ORDER>(d2, roots[r]);
-            if (!is_finite_d(cd2)) {
-                report_error(a.err, kErrNpConjDist2, i, j, cd2);
-                failed = true;
-                break;
-            }
-            if (cd2 < thresh2) {
-                emit_conj<ORDER>(a, c, i, j, lb_rf, roots[r], Pi, cd.di, Pj, cd.dj);
-            } else {
-                ++rep[4];
-            }
+    for (int k = 0; k < ORDER; ++k) d2p[k] = (double)(k + 1) * d2[k + 1];
+    d2p[ORDER] = 0;
+    double roots[kMaxRoots<ORDER>];
+    int nr = 0;
+    bool fex = false;
+    const int fm = find_minima<ORDER>(d2p, rf_int, roots, nr, fex, i, j, a.err, wl, cap);
+    if (fm != kFmOk) return fm;
+    ++rep[0];
+    rep[1] += fex ? 1 : 0;
+    rep[2] += (nr == 0) ? 1 : 0;
+    rep[3] += (unsigned long long)nr;
+    for (int r = 0; r < nr; ++r) {
+        const double cd2 = horner<ORDER>(d2, roots[r]);
+        if (!is_finite_d(cd2)) {
+            report_error(a.err, kErrNpConjDist2, i, j, cd2);
+            return kFmError;
         }
-        if (failed) continue;
-        if (flags & kCandEnd) {
-            const double tmc = next_toward_d(rf_int, -1.0);
-            const double dv = horner<ORDER>(d2p, tmc);
-            if (!is_finite_d(dv)) {
-                report_error(a.err, kErrNpBndDerUb, i, j, dv);
-                continue;
-            }
-            if (dv < 0) {
-                const double dd = horner<ORDER>(d2, tmc);
-                if (!is_finite_d(dd)) {
-                    report_error(a.err, kErrNpBndDist2Ub, i, j, dd);
-                    continue;
-                }
-                if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, tmc, Pi, cd.di, Pj, cd.dj);
-            }
-        }
-        if (flags & kCandBegin) {
-            const double dv = horner<ORDER>(d2p, 0.0);
-            if (!is_finite_d(dv)) {
-                report_error(a.err, kErrNpBndDerLb, i, j, dv);
-                continue;
-            }
-            if (dv > 0) {
-                const double dd = horner<ORDER>(d2, 0.0);
-                if (!is_finite_d(dd)) {
-                    report_error(a.err, kErrNpBndDist2Lb, i, j, dd);
-                    continue;
-                }
-                if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, 0.0, Pi, cd.di, Pj, cd.dj);
-            }
+        if (cd2 < thresh2) {
+            emit_conj<ORDER>(a, c, i, j, lb_rf, roots[r], Pi, cd.di, Pj, cd.dj);
+        } else {
+            ++rep[4];
         }
     }
+    if (flags & kCandEnd) {
+        const double tmc = next_toward_d(rf_int, -1.0);
+        const double dv = horner<ORDER>(d2p, tmc);
+        if (!is_finite_d(dv)) {
+            report_error(a.err, kErrNpBndDerUb, i, j, dv);
+            return kFmError;
+        }
+        if (dv < 0) {
+            const double dd = horner<ORDER>(d2, tmc);
+            if (!is_finite_d(dd)) {
+                report_error(a.err, kErrNpBndDist2Ub, i, j, dd);
+                return kFmError;
+            }
+            if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, tmc, Pi, cd.di, Pj, cd.dj);
+        }
+    }
+    if (flags & kCandBegin) {
+        const double dv = horner<ORDER>(d2p, 0.0);
+        if (!is_finite_d(dv)) {
+            report_error(a.err, kErrNpBndDerLb, i, j, dv);
+            return kFmError;
+        }
+        if (dv > 0) {
+            const double dd = horner<ORDER>(d2, 0.0);
+            if (!is_finite_d(dd)) {
+                report_error(a.err, kErrNpBndDist2Lb, i, j, dd);
+                return kFmError;
+            }
+            if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, 0.0, Pi, cd.di, Pj, cd.dj);
+        }
+    }
+    return kFmOk;
+}
+
+template <int ORDER>
+__device__ inline void np_flush_report(const NarrowArgs &a, const unsigned long long *rep)
+{
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const unsigned long long s = wave_reduce_add_u64(rep[k]);
@@ -860,27 +871,101 @@ __global__ __launch_bounds__(256) void k_np_roots(NarrowArgs a)
     }
 }
 
-#ifdef MZH_NARROW_HOST
-// ---- conjunction ordering ---------------------------------------------------
-
-__global__ void k_conj_hist(const uint32_t *__restrict__ step, uint64_t n, uint64_t *__restrict__ cnt)
+// Narrow phase, part 2: one lane per queued candidate (grid-stride over the
+// queue, whose length is read on the device), with a private work list of
+// a.wl_fast_cap (<= kWlCap) entries. A candidate that needs a longer list is
+// handed to the slow queue.
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_np_roots(NarrowArgs a)
 {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) atomicAdd((unsigned long long *)&cnt[step[k]], 1ull);
+    const unsigned long long avail = *a.cand_count;
+    const uint64_t n = avail < a.cand_cap ? avail : a.cand_cap;
+    unsigned long long rep[5] = {0, 0, 0, 0, 0};
+    WLEntry<ORDER> wl[kWlCap];
+    const int cap = a.wl_fast_cap < kWlCap ? a.wl_fast_cap : kWlCap;
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x; q < n; q += stride) {
+        if (np_root_cand<ORDER>(a, q, rep, wl, cap) == kFmListFull) {
+            const unsigned long long s = atomicAdd(a.slow_count, 1ull);
+            if (s < a.slow_cap) a.slow[s] = q;
+        }
+    }
+    np_flush_report<ORDER>(a, rep);
 }
 
-__global__ void k_conj_scatter(const uint32_t *__restrict__ step, uint64_t n, const uint64_t *__restrict__ start,
-                               uint32_t *__restrict__ cursor, uint32_t *__restrict__ pos)
+// Narrow phase, part 2b: the candidates of the slow queue, each lane with its
+// own work list of kWlSlowCap entries in a.wl_slow (gridDim.x * 256 lists).
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_np_roots_slow(NarrowArgs a)
 {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) {
-        const uint32_t c = step[k];
-        pos[k] = (uint32_t)(start[c] + atomicAdd(&cursor[c], 1u));
+    const unsigned long long avail = *a.slow_count;
+    const uint64_t n = avail < a.slow_cap ? avail : a.slow_cap;
+    unsigned long long rep[5] = {0, 0, 0, 0, 0};
+    const uint64_t lane = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    WLEntry<ORDER> *wl = static_cast<WLEntry<ORDER> *>(a.wl_slow) + lane * kWlSlowCap;
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t s = lane; s < n; s += stride) (void)np_root_cand<ORDER>(a, a.slow[s], rep, wl, kWlSlowCap);
+    np_flush_report<ORDER>(a, rep);
+}
+
+// Root finding of free-standing polynomials (run_poly_root_finding,
+// poly_utils.cpp:272-460, direction +1): one lane per polynomial, the
+// private list first; a polynomial whose list outgrows it is done again by
+// k_rootfind_slow with a kWlSlowCap-entry list in global memory.
+// Outputs as documented with RootFindArgs.
+template <int ORDER>
+__device__ inline int rootfind_one(const RootFindArgs &a, uint64_t k, WLEntry<ORDER> *wl, int cap, uint32_t slow)
+{
+    double poly[ORDER + 1];
+#pragma unroll
+    for (int m = 0; m <= ORDER; ++m) poly[m] = a.polys[k * (ORDER + 1) + m];
+    double roots[kMaxRoots<ORDER>];
+    int nr = 0;
+    bool fex = false;
+    const int fm = find_minima<ORDER>(poly, a.rf_int[k], roots, nr, fex, (uint32_t)k, 0u, a.err, wl, cap);
+    if (fm == kFmOk) {
+        for (int r = 0; r < nr; ++r) a.roots[k * kMaxRoots<ORDER> + r] = roots[r];
+        a.info[k] = (uint32_t)nr | (fex ? 256u : 0u) | (slow << 9);
+    } else if (fm == kFmError) {
+        a.info[k] = 0xfffffffeu;
+    }
+    return fm;
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_rootfind(RootFindArgs a)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (k >= a.n) return;
+    WLEntry<ORDER> wl[kWlCap];
+    const int cap = a.wl_fast_cap < kWlCap ? a.wl_fast_cap : kWlCap;
+    if (rootfind_one<ORDER>(a, k, wl, cap, 0u) == kFmListFull) {
+        a.info[k] = 0xffffffffu;
+        a.slow[atomicAdd(a.slow_count, 1ull)] = k;
     }
 }
 
-// strict weak order (tca, then all fields lexicographically), with the input
-// index as the final tie-break so that equal records get distinct ranks
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_rootfind_slow(RootFindArgs a)
+{
+    const uint64_t n = *a.slow_count;
+    const uint64_t lane = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    WLEntry<ORDER> *wl = static_cast<WLEntry<ORDER> *>(a.wl_slow) + lane * kWlSlowCap;
+    for (uint64_t s = lane; s < n; s += (uint64_t)gridDim.x * 256u)
+        (void)rootfind_one<ORDER>(a, a.slow[s], wl, kWlSlowCap, 1u);
+}
+
+#ifdef MZH_NARROW_HOST
+// ---- conjunction ordering ---------------------------------------------------
+// Records are appended in any order; the output is grouped by conjunction
+// step and each step sorted by (tca, then every field lexicographically), the
+// order of detect_conjunctions_narrow_phase.cpp:751-757. A merge sort over
+// compact keys (step, tca, i, j, input index): 1024-key tiles sorted in LDS,
+// then passes that merge runs of doubling width, where each key finds its
+// place in the sibling run by binary search. O(M log M) whatever the number
+// of conjunctions per step.
+
+// strict weak order (tca, then all fields lexicographically)
 __device__ inline bool conj_less(const mzh_conj &x, const mzh_conj &y)
 {
     if (x.tca != y.tca) return x.tca < y.tca;
@@ -898,30 +983,109 @@ __device__ inline bool conj_less(const mzh_conj &x, const mzh_conj &y)
     return false;
 }
 
-// rank of every record inside its step segment
-__global__ void k_conj_rank(const mzh_conj *__restrict__ in, const uint32_t *__restrict__ step, uint64_t n,
-                            const uint64_t *__restrict__ start, const uint64_t *__restrict__ cnt,
-                            const uint32_t *__restrict__ pos, uint32_t *__restrict__ inv, mzh_conj *__restrict__ out)
+constexpr uint32_t kConjPad = 0xffffffffu;  // step of the padding keys of a partial tile
+constexpr uint32_t kConjTile = 1024;
+
+// Total order of the keys: (step, tca, i, j), then the remaining fields of
+// the records, then the input index. tca >= 0, so its bits order as integers
+// (tca + 0.0 turns a -0.0 into +0.0).
+__device__ inline bool ck_less(const ConjKey &x, const ConjKey &y, const mzh_conj *__restrict__ rec)
 {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t c = step[k];
-    const uint64_t s0 = start[c], m = cnt[c];
-    const mzh_conj me = in[k];
-    uint64_t rank = 0;
-    for (uint64_t q = 0; q < m; ++q) {
-        const uint32_t other = inv[s0 + q];
-        if (other == (uint32_t)k) continue;
-        const mzh_conj &y = in[other];
-        if (conj_less(y, me) || (!conj_less(me, y) && other < (uint32_t)k)) ++rank;
-    }
-    out[s0 + rank] = me;
+    if (x.step != y.step) return x.step < y.step;
+    if (x.step == kConjPad) return false;
+    if (x.tca != y.tca) return x.tca < y.tca;
+    if (x.ij != y.ij) return x.ij < y.ij;
+    if (x.idx == y.idx) return false;
+    const mzh_conj &a = rec[x.idx], &b = rec[y.idx];
+    if (conj_less(a, b)) return true;
+    if (conj_less(b, a)) return false;
+    return x.idx < y.idx;
 }
 
-__global__ void k_conj_inv(const uint32_t *__restrict__ pos, uint64_t n, uint32_t *__restrict__ inv)
+__global__ __launch_bounds__(256) void k_conj_keys(const mzh_conj *__restrict__ in, const uint32_t *__restrict__ step,
+                                                   uint64_t n, ConjKey *__restrict__ keys)
 {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) inv[pos[k]] = (uint32_t)k;
+    const uint64_t k = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (k >= n) return;
+    const mzh_conj &r = in[k];
+    ConjKey ck;
+    ck.step = step[k];
+    ck.idx = (uint32_t)k;
+    ck.tca = (uint64_t)__double_as_longlong(r.tca + 0.0);
+    ck.ij = ((uint64_t)r.i << 32) | r.j;
+    keys[k] = ck;
+}
+
+// one workgroup sorts one tile of kConjTile keys in place (bitonic network in LDS)
+__global__ __launch_bounds__(256) void k_conj_tiles(ConjKey *__restrict__ keys, uint64_t n,
+                                                    const mzh_conj *__restrict__ rec)
+{
+    __shared__ ConjKey s[kConjTile];
+    const uint64_t base = (uint64_t)blockIdx.x * kConjTile;
+    for (uint32_t t = threadIdx.x; t < kConjTile; t += 256) {
+        if (base + t < n) {
+            s[t] = keys[base + t];
+        } else {
+            ConjKey pad;
+            pad.step = kConjPad;
+            pad.idx = kConjPad;
+            pad.tca = ~0ull;
+            pad.ij = ~0ull;
+            s[t] = pad;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= kConjTile; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < kConjTile / 2; t += 256) {
+                const uint32_t a = 2u * t - (t & (j - 1u)), b = a + j;
+                const bool up = (a & k) == 0;
+                if (ck_less(s[b], s[a], rec) == up) {
+                    const ConjKey x = s[a];
+                    s[a] = s[b];
+                    s[b] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t t = threadIdx.x; t < kConjTile; t += 256)
+        if (base + t < n) keys[base + t] = s[t];
+}
+
+// merge the sorted runs of width w pairwise: in -> out
+__global__ __launch_bounds__(256) void k_conj_merge(const ConjKey *__restrict__ in, ConjKey *__restrict__ out,
+                                                    uint64_t n, uint64_t w, const mzh_conj *__restrict__ rec)
+{
+    const uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t r = e / w, base = (r & ~1ull) * w;
+    const uint64_t sb = (r ^ 1ull) * w;
+    const ConjKey me = in[e];
+    if (sb >= n) {
+        out[e] = me;  // no sibling run
+        return;
+    }
+    const uint64_t se = sb + w < n ? sb + w : n;
+    // number of keys of the sibling run below me (all keys are distinct)
+    uint64_t lo = sb, len = se - sb;
+    while (len > 0) {
+        const uint64_t h = len / 2;
+        if (ck_less(in[lo + h], me, rec)) {
+            lo += h + 1;
+            len -= h + 1;
+        } else {
+            len = h;
+        }
+    }
+    out[base + (e - r * w) + (lo - sb)] = me;
+}
+
+__global__ __launch_bounds__(256) void k_conj_gather(const ConjKey *__restrict__ keys, const mzh_conj *__restrict__ in,
+                                                     uint64_t n, mzh_conj *__restrict__ out)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (k < n) out[k] = in[keys[k].idx];
 }
 #endif // MZH_NARROW_HOST
 
@@ -947,12 +1111,29 @@ hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream)
     // for the narrow phase of the 30 000-object day, 1024: 4.10, 512: 4.06, 256: 4.19)
     const uint64_t qblocks = std::min<uint64_t>((a.cand_cap + 255) / 256, 512);
     hipLaunchKernelGGL(k_np_roots<ORDER>, dim3((unsigned)qblocks), dim3(256), 0, stream, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the candidates whose root isolation outgrew the private work list
+    hipLaunchKernelGGL(k_np_roots_slow<ORDER>, dim3(kNarrowSlowLanes / 256), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+template <int ORDER>
+hipError_t launch_rootfind_order(const RootFindArgs &a, hipStream_t stream)
+{
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rootfind<ORDER>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rootfind_slow<ORDER>, dim3(kNarrowSlowLanes / 256), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 
 // Explicit instantiations for the orders compiled into this object (the build
 // splits the order range over several objects so that they compile in parallel).
-#define MZH_INST(N) template hipError_t launch_narrow_order<N>(const NarrowArgs &, hipStream_t);
+#define MZH_INST(N)                                                                                                    \
+    template hipError_t launch_narrow_order<N>(const NarrowArgs &, hipStream_t);                                      \
+    template hipError_t launch_rootfind_order<N>(const RootFindArgs &, hipStream_t);
 #if MZH_NARROW_LO <= 2 && 2 <= MZH_NARROW_HI
 MZH_INST(2)
 #endif
@@ -1059,6 +1240,22 @@ size_t narrow_cand_bytes(uint32_t order)
     }
 }
 
+template <int ORDER>
+hipError_t dispatch_rootfind(uint32_t order, const RootFindArgs &a, hipStream_t stream)
+{
+    if (order == ORDER) return launch_rootfind_order<ORDER>(a, stream);
+    if constexpr (ORDER < kMaxOrder) {
+        return dispatch_rootfind<ORDER + 1>(order, a, stream);
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_rootfind(uint32_t order, const RootFindArgs &a, hipStream_t stream)
+{
+    return dispatch_rootfind<kMinOrder>(order, a, stream);
+}
+
 hipError_t launch_narrow(const NarrowArgs &a, hipStream_t stream)
 {
     switch (a.pj.order) {
@@ -1105,27 +1302,24 @@ hipError_t launch_narrow(const NarrowArgs &a, hipStream_t stream)
     }
 }
 
-// Full conjunction ordering used by the orchestrator: histogram, scan (done by
-// the caller between the two halves), scatter, rank.
-hipError_t launch_conj_hist(const uint32_t *in_step, uint64_t n, uint64_t *cnt, uint32_t n_steps, hipStream_t stream)
-{
-    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint64_t) * n_steps, stream);
-    if (e != hipSuccess || n == 0) return e;
-    hipLaunchKernelGGL(k_conj_hist, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, in_step, n, cnt);
-    return hipGetLastError();
-}
-
-hipError_t launch_conj_order(const mzh_conj *in, const uint32_t *in_step, uint64_t n, const uint64_t *start,
-                             const uint64_t *cnt, uint32_t *cursor, uint32_t *pos, uint32_t *inv, mzh_conj *out,
-                             uint32_t n_steps, hipStream_t stream)
+// Conjunction ordering used by the orchestrator (see k_conj_keys .. k_conj_gather).
+hipError_t launch_conj_sort(const mzh_conj *in, const uint32_t *in_step, uint64_t n, ConjKey *keys0, ConjKey *keys1,
+                            mzh_conj *out, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
-    hipError_t e = hipMemsetAsync(cursor, 0, sizeof(uint32_t) * n_steps, stream);
-    if (e != hipSuccess) return e;
+    if (n >= 0xffffffffull) return hipErrorInvalidValue;
     const unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_conj_scatter, dim3(blocks), dim3(256), 0, stream, in_step, n, start, cursor, pos);
-    hipLaunchKernelGGL(k_conj_inv, dim3(blocks), dim3(256), 0, stream, pos, n, inv);
-    hipLaunchKernelGGL(k_conj_rank, dim3(blocks), dim3(256), 0, stream, in, in_step, n, start, cnt, pos, inv, out);
+    hipLaunchKernelGGL(k_conj_keys, dim3(blocks), dim3(256), 0, stream, in, in_step, n, keys0);
+    hipLaunchKernelGGL(k_conj_tiles, dim3((unsigned)((n + kConjTile - 1) / kConjTile)), dim3(256), 0, stream, keys0, n,
+                       in);
+    ConjKey *a = keys0, *b = keys1;
+    for (uint64_t w = kConjTile; w < n; w *= 2) {
+        hipLaunchKernelGGL(k_conj_merge, dim3(blocks), dim3(256), 0, stream, a, b, n, w, in);
+        ConjKey *t = a;
+        a = b;
+        b = t;
+    }
+    hipLaunchKernelGGL(k_conj_gather, dim3(blocks), dim3(256), 0, stream, a, in, n, out);
     return hipGetLastError();
 }
 #endif // MZH_NARROW_HOST

@@ -152,3 +152,32 @@ def test_bench_reads_the_committed_profiles():
         assert bench.hbm_traffic(k) > 0
     assert bench.eval_hbm_traffic() > 0
     assert bench.sgp4_flops_per_launch() > 1e9
+
+
+def test_check_screen_without_device():
+    """The constructor's validation runs on the host before any device is
+    touched: the reference's messages in the reference's order
+    (conjunctions.cpp:267-332), then this build's size limit (objects times
+    conjunction steps below 2^32)."""
+    assert mz.check_screen(1000, 1.0, 5.0, 3.0 / 1440.0) == 480
+    assert mz.check_screen(3, 1.0, 5.0, 0.3) == 4
+    # threshold before interval before object types
+    with pytest.raises(ValueError, match="The conjunction threshold must be finite and positive"):
+        mz.check_screen(2, 1.0, -1.0, float("nan"), otypes=[7])
+    with pytest.raises(ValueError, match="is too large and results in an overflow error"):
+        mz.check_screen(2, 1.0, 1e200, 0.1)
+    with pytest.raises(ValueError, match="The conjunction detection interval must be finite and positive"):
+        mz.check_screen(2, 1.0, 1.0, 0.0, otypes=[7])
+    with pytest.raises(ValueError, match="the expected size is 2, but the actual size is 1 instead"):
+        mz.check_screen(2, 1.0, 1.0, 0.1, otypes=[7])
+    with pytest.raises(ValueError, match="must be one of \\[1, 2, 4\\], but a value of 7 was detected"):
+        mz.check_screen(2, 1.0, 1.0, 0.1, otypes=[1, 7])
+    with pytest.raises(ValueError, match="must not exceed 4294967295"):
+        mz.check_screen(100000, 30.0, 5.0, 1.0 / 86400.0)
+    # conjunctions() raises the same before it needs a device
+    tcs, tms = H.planar_circ()
+    pj = mz.polyjectory([tcs, tcs], [tms, tms], [0, 0])
+    with pytest.raises(ValueError, match="2 objects over 3000000000 conjunction steps is too large"):
+        mz.conjunctions(pj, 0.1, pj.maxT / 3.0e9)
+    with pytest.raises(ValueError, match="must not exceed 4294967295"):
+        mz.conjunctions_batch([pj], [0.1], [pj.maxT / 3.0e9])
