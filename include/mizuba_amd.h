@@ -312,6 +312,31 @@ int mzh_get_stage_times(mzh_ctx *ctx, double out[MZH_STAGE_COUNT]);
  *        5 bvh nodes (fixed capacity per step), 6 pairs, 7 conjunctions. */
 int mzh_device_view(mzh_ctx *ctx, int which, void **ptr, uint64_t *bytes);
 
+/* Make dst use the polyjectory of src (same device), without a copy: the
+ * device arrays are shared, read-only, and stay allocated while any context
+ * uses them (a later upload into either context gets arrays of its own). Lets
+ * several screenings of one polyjectory (conjunctions() called again with
+ * other thresholds, intervals or object types) skip the upload. No reference
+ * counterpart: the reference's polyjectory is shared memory-mapped files
+ * (mizuba/polyjectory.cpp:91-236). */
+int mzh_share_polyjectory(mzh_ctx *dst, const mzh_ctx *src);
+
+/* Free device memory of a context (the reference's hint_release, which
+ * madvises its memory-mapped results away, mizuba/conjunctions.cpp:466-477):
+ *   MZH_RELEASE_SCRATCH      buffers that no accessor reads (queues, counts,
+ *                            sort and build scratch); a later mzh_screen
+ *                            allocates them again;
+ *   MZH_RELEASE_RESULTS      the device copies of the results: the copy
+ *                            accessors and mzh_device_view then fail, while
+ *                            the sizes, counters, step times and n_cd_steps
+ *                            stay available;
+ *   MZH_RELEASE_POLYJECTORY  the context's reference to its polyjectory.
+ * Waits for the context's stream first. */
+#define MZH_RELEASE_SCRATCH 1
+#define MZH_RELEASE_RESULTS 2
+#define MZH_RELEASE_POLYJECTORY 4
+int mzh_release(mzh_ctx *ctx, int what);
+
 /* Wait for all work queued on the context's stream. */
 int mzh_synchronize(mzh_ctx *ctx);
 

@@ -18,6 +18,7 @@
 #include <string>
 #include <thread>
 #include <limits>
+#include <memory>
 #include <vector>
 
 #include "../../include/mizuba_amd.h"
@@ -84,6 +85,12 @@ struct DevBuf {
         bytes = alloc;
         return hipSuccess;
     }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
     // Grow to at least b bytes, keeping the first `keep` bytes.
     hipError_t grow(size_t b, size_t keep, hipStream_t st)
     {
@@ -116,6 +123,12 @@ struct StatusError {
     std::string msg;
 };
 
+// Device arrays of a polyjectory: the AoS coefficients, the times, the
+// step-major transposed positions and the per-object layout.
+struct PjBufs {
+    DevBuf traj, time, cf4, step0, nsteps, time_off;
+};
+
 } // namespace
 
 struct mzh_ctx {
@@ -123,12 +136,13 @@ struct mzh_ctx {
     hipStream_t stream = nullptr;
     std::string last_error;
 
-    // polyjectory
+    // polyjectory: the device arrays may be shared by several contexts
+    // (mzh_share_polyjectory); they are read-only once uploaded
     bool has_pj = false;
     uint32_t n_objs = 0, order = 0;
     double maxT = 0;
     uint64_t n_steps_total = 0;
-    DevBuf traj, time, cf4, step0, nsteps, time_off;
+    std::shared_ptr<PjBufs> pjb;
     // host copies of its layout, statuses and epoch (TAI Julian date) for copy-out
     std::vector<uint64_t> h_traj_off, h_nsteps, h_time_off;
     std::vector<int32_t> h_status;
@@ -137,6 +151,7 @@ struct mzh_ctx {
 
     // problem state
     bool screened = false;
+    bool results_released = false;  // mzh_release(MZH_RELEASE_RESULTS) after the screen
     bool skip_cd = false;
     uint64_t C = 0;
     double conj_thresh = 0, conj_det_interval = 0;
@@ -263,15 +278,22 @@ std::string dev_error_message(const DevError &e)
     }
 }
 
+// Device arrays for a new polyjectory: the context's own, or fresh ones when
+// other contexts share the current arrays (they keep those).
+void own_pj_bufs(mzh_ctx *ctx)
+{
+    if (!ctx->pjb || ctx->pjb.use_count() > 1) ctx->pjb = std::make_shared<PjBufs>();
+}
+
 PjView pj_view(const mzh_ctx *ctx)
 {
     PjView v;
-    v.traj = ctx->traj.as<const double>();
-    v.time = ctx->time.as<const double>();
-    v.cf4 = ctx->cf4.as<const double>();
-    v.step0 = ctx->step0.as<const uint64_t>();
-    v.nsteps = ctx->nsteps.as<const uint64_t>();
-    v.time_off = ctx->time_off.as<const uint64_t>();
+    v.traj = ctx->pjb->traj.as<const double>();
+    v.time = ctx->pjb->time.as<const double>();
+    v.cf4 = ctx->pjb->cf4.as<const double>();
+    v.step0 = ctx->pjb->step0.as<const uint64_t>();
+    v.nsteps = ctx->pjb->nsteps.as<const uint64_t>();
+    v.time_off = ctx->pjb->time_off.as<const uint64_t>();
     v.n_objs = ctx->n_objs;
     v.order = ctx->order;
     return v;
@@ -360,6 +382,7 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     const size_t NC = (size_t)N * C;
     hipStream_t st = ctx->stream;
     ctx->screened = false;
+    ctx->results_released = false;
     ctx->C = C;
     ctx->conj_thresh = conj_thresh;
     ctx->conj_det_interval = conj_det_interval;
@@ -624,6 +647,14 @@ void require_screened(const mzh_ctx *ctx)
     if (!ctx->screened) throw StatusError{MZH_EINVAL, "No screening results are available in this context"};
 }
 
+// For the accessors that read the device copies of the results.
+void require_results(const mzh_ctx *ctx)
+{
+    require_screened(ctx);
+    if (ctx->results_released)
+        throw StatusError{MZH_EINVAL, "The device copies of the results of this screening have been released"};
+}
+
 // The trees are kept in the broad phase's layout; the reference's 48-byte
 // bvh_node records are produced on the first request after a screen.
 void export_nodes(mzh_ctx *ctx)
@@ -743,18 +774,19 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         hipStream_t st = ctx->stream;
         ctx->has_pj = false;
         ctx->screened = false;
-        MZH_HIP(ctx->traj.ensure(sizeof(double) * (traj_len ? traj_len : 1)));
-        MZH_HIP(ctx->time.ensure(sizeof(double) * (time_len ? time_len : 1)));
-        MZH_HIP(ctx->cf4.ensure(sizeof(double) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
-        MZH_HIP(ctx->step0.ensure(sizeof(uint64_t) * n_objs));
-        MZH_HIP(ctx->nsteps.ensure(sizeof(uint64_t) * n_objs));
-        MZH_HIP(ctx->time_off.ensure(sizeof(uint64_t) * n_objs));
-        if (traj_len) MZH_HIP(hipMemcpyAsync(ctx->traj.p, traj, sizeof(double) * traj_len, hipMemcpyHostToDevice, st));
-        if (time_len) MZH_HIP(hipMemcpyAsync(ctx->time.p, time, sizeof(double) * time_len, hipMemcpyHostToDevice, st));
-        MZH_HIP(hipMemcpyAsync(ctx->step0.p, step0.data(), sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
-        MZH_HIP(hipMemcpyAsync(ctx->nsteps.p, nsteps, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
-        MZH_HIP(hipMemcpyAsync(ctx->time_off.p, time_off, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
-        MZH_HIP(launch_transpose_cf4(ctx->traj.as<double>(), ctx->cf4.as<double>(), steps_total, order, st));
+        own_pj_bufs(ctx);
+        MZH_HIP(ctx->pjb->traj.ensure(sizeof(double) * (traj_len ? traj_len : 1)));
+        MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * (time_len ? time_len : 1)));
+        MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
+        MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_objs));
+        MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_objs));
+        MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_objs));
+        if (traj_len) MZH_HIP(hipMemcpyAsync(ctx->pjb->traj.p, traj, sizeof(double) * traj_len, hipMemcpyHostToDevice, st));
+        if (time_len) MZH_HIP(hipMemcpyAsync(ctx->pjb->time.p, time, sizeof(double) * time_len, hipMemcpyHostToDevice, st));
+        MZH_HIP(hipMemcpyAsync(ctx->pjb->step0.p, step0.data(), sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
+        MZH_HIP(hipMemcpyAsync(ctx->pjb->nsteps.p, nsteps, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
+        MZH_HIP(hipMemcpyAsync(ctx->pjb->time_off.p, time_off, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
+        MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), steps_total, order, st));
         MZH_HIP(hipStreamSynchronize(st));
         ctx->n_objs = n_objs;
         ctx->order = order;
@@ -1027,23 +1059,24 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
                           "allowed"};
     ctx->has_pj = false;
     ctx->screened = false;
-    MZH_HIP(ctx->traj.ensure(sizeof(double) * steps_total * op1 * 7u));
-    MZH_HIP(ctx->time.ensure(sizeof(double) * time_len));
-    MZH_HIP(ctx->cf4.ensure(sizeof(double) * 4u * op1 * steps_total));
-    MZH_HIP(ctx->step0.ensure(sizeof(uint64_t) * n_sats));
-    MZH_HIP(ctx->nsteps.ensure(sizeof(uint64_t) * n_sats));
-    MZH_HIP(ctx->time_off.ensure(sizeof(uint64_t) * n_sats));
-    MZH_HIP(hipMemcpyAsync(ctx->step0.p, step0.data(), sizeof(uint64_t) * n_sats, hipMemcpyHostToDevice, st));
-    MZH_HIP(hipMemcpyAsync(ctx->nsteps.p, nsteps.data(), sizeof(uint64_t) * n_sats, hipMemcpyHostToDevice, st));
-    MZH_HIP(hipMemcpyAsync(ctx->time_off.p, time_off.data(), sizeof(uint64_t) * n_sats, hipMemcpyHostToDevice, st));
-    g.traj = ctx->traj.as<double>();
-    g.time = ctx->time.as<double>();
-    g.sat_poly0 = ctx->step0.as<const uint64_t>();
-    g.sat_time0 = ctx->time_off.as<const uint64_t>();
+    own_pj_bufs(ctx);
+    MZH_HIP(ctx->pjb->traj.ensure(sizeof(double) * steps_total * op1 * 7u));
+    MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * time_len));
+    MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * op1 * steps_total));
+    MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_sats));
+    MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_sats));
+    MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_sats));
+    MZH_HIP(hipMemcpyAsync(ctx->pjb->step0.p, step0.data(), sizeof(uint64_t) * n_sats, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemcpyAsync(ctx->pjb->nsteps.p, nsteps.data(), sizeof(uint64_t) * n_sats, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemcpyAsync(ctx->pjb->time_off.p, time_off.data(), sizeof(uint64_t) * n_sats, hipMemcpyHostToDevice, st));
+    g.traj = ctx->pjb->traj.as<double>();
+    g.time = ctx->pjb->time.as<double>();
+    g.sat_poly0 = ctx->pjb->step0.as<const uint64_t>();
+    g.sat_time0 = ctx->pjb->time_off.as<const uint64_t>();
     MZH_HIP(hipEventRecord(E[5], st));
     MZH_HIP(launch_sgp4_write(g, order, n_redo, st));
     MZH_HIP(hipEventRecord(E[6], st));
-    MZH_HIP(launch_transpose_cf4(ctx->traj.as<double>(), ctx->cf4.as<double>(), steps_total, order, st));
+    MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), steps_total, order, st));
     MZH_HIP(hipEventRecord(E[7], st));
     MZH_HIP(hipStreamSynchronize(st));
     {
@@ -1124,9 +1157,9 @@ int mzh_copy_polyjectory(mzh_ctx *ctx, double *traj, double *time, uint64_t *tra
     return guarded(ctx, [&] {
         if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
         if (traj && ctx->traj_len)
-            MZH_HIP(hipMemcpy(traj, ctx->traj.p, sizeof(double) * ctx->traj_len, hipMemcpyDeviceToHost));
+            MZH_HIP(hipMemcpy(traj, ctx->pjb->traj.p, sizeof(double) * ctx->traj_len, hipMemcpyDeviceToHost));
         if (time && ctx->time_len)
-            MZH_HIP(hipMemcpy(time, ctx->time.p, sizeof(double) * ctx->time_len, hipMemcpyDeviceToHost));
+            MZH_HIP(hipMemcpy(time, ctx->pjb->time.p, sizeof(double) * ctx->time_len, hipMemcpyDeviceToHost));
         const size_t n = ctx->n_objs;
         if (traj_off) std::copy(ctx->h_traj_off.begin(), ctx->h_traj_off.end(), traj_off);
         if (nsteps) std::copy(ctx->h_nsteps.begin(), ctx->h_nsteps.end(), nsteps);
@@ -1418,7 +1451,7 @@ namespace
 int copy_dev(mzh_ctx *ctx, void *dst, const DevBuf &src, size_t bytes)
 {
     return guarded(ctx, [&] {
-        require_screened(ctx);
+        require_results(ctx);
         if (!dst && bytes) throw StatusError{MZH_EINVAL, "Null destination"};
         if (bytes) MZH_HIP(hipMemcpy(dst, src.p, bytes, hipMemcpyDeviceToHost));
     });
@@ -1477,6 +1510,7 @@ int mzh_copy_bvh_tree(mzh_ctx *ctx, uint64_t step, mzh_bvh_node *dst)
                                               + std::to_string(ctx->C)};
         const size_t n = ctx->h_tree_size[step];
         if (n) {
+            require_results(ctx);
             export_nodes(ctx);
             MZH_HIP(hipMemcpy(dst, ctx->nodes.as<mzh_bvh_node>() + step * ctx->node_cap, sizeof(mzh_bvh_node) * n,
                               hipMemcpyDeviceToHost));
@@ -1507,6 +1541,7 @@ int mzh_copy_aabb_collisions(mzh_ctx *ctx, uint64_t step, mzh_aabb_collision *ds
                                   + std::to_string(step) + ": the total number of conjunction steps is only "
                                   + std::to_string(ctx->C)};
         const uint64_t a = ctx->h_step_off[step], n = ctx->h_step_off[step + 1] - a;
+        if (n) require_results(ctx);
         if (n)
             MZH_HIP(hipMemcpy(dst, ctx->pairs.as<mzh_aabb_collision>() + a, sizeof(mzh_aabb_collision) * n,
                               hipMemcpyDeviceToHost));
@@ -1536,6 +1571,7 @@ int mzh_copy_conjunctions(mzh_ctx *ctx, mzh_conj *dst)
 {
     return guarded(ctx, [&] {
         require_screened(ctx);
+        if (ctx->n_conj) require_results(ctx);
         if (ctx->n_conj)
             MZH_HIP(hipMemcpy(dst, ctx->conj_sorted.p, sizeof(mzh_conj) * ctx->n_conj, hipMemcpyDeviceToHost));
     });
@@ -1560,7 +1596,7 @@ int mzh_get_stage_times(mzh_ctx *ctx, double out[MZH_STAGE_COUNT])
 int mzh_device_view(mzh_ctx *ctx, int which, void **ptr, uint64_t *bytes)
 {
     return guarded(ctx, [&] {
-        require_screened(ctx);
+        require_results(ctx);
         const size_t NC = (size_t)ctx->n_objs * ctx->C;
         switch (which) {
             case 0:
@@ -1598,6 +1634,59 @@ int mzh_device_view(mzh_ctx *ctx, int which, void **ptr, uint64_t *bytes)
                 break;
             default:
                 throw StatusError{MZH_EINVAL, "Invalid device view index " + std::to_string(which)};
+        }
+    });
+}
+
+int mzh_share_polyjectory(mzh_ctx *dst, const mzh_ctx *src)
+{
+    return guarded(dst, [&] {
+        if (!src || !src->has_pj) throw StatusError{MZH_EINVAL, "The source context holds no polyjectory"};
+        if (src == dst) return;
+        if (src->device != dst->device)
+            throw StatusError{MZH_EINVAL, "Contexts on different devices cannot share a polyjectory"};
+        dst->screened = false;
+        dst->pjb = src->pjb;
+        dst->n_objs = src->n_objs;
+        dst->order = src->order;
+        dst->maxT = src->maxT;
+        dst->n_steps_total = src->n_steps_total;
+        dst->h_traj_off = src->h_traj_off;
+        dst->h_nsteps = src->h_nsteps;
+        dst->h_time_off = src->h_time_off;
+        dst->h_status = src->h_status;
+        dst->traj_len = src->traj_len;
+        dst->time_len = src->time_len;
+        dst->pj_epoch[0] = src->pj_epoch[0];
+        dst->pj_epoch[1] = src->pj_epoch[1];
+        dst->has_pj = true;
+    });
+}
+
+int mzh_release(mzh_ctx *ctx, int what)
+{
+    return guarded(ctx, [&] {
+        MZH_HIP(hipStreamSynchronize(ctx->stream));
+        if (what & MZH_RELEASE_SCRATCH) {
+            for (DevBuf *b : {&ctx->kbke, &ctx->keys_tmp, &ctx->vals_tmp, &ctx->n_eff, &ctx->sort_todo, &ctx->gbox,
+                              &ctx->bvh_hdr, &ctx->bvh_scr, &ctx->otypes, &ctx->cnt, &ctx->head, &ctx->off,
+                              &ctx->step_total, &ctx->grand, &ctx->slots, &ctx->ovf, &ctx->ovf_count, &ctx->conj_raw,
+                              &ctx->conj_step, &ctx->conj_keys, &ctx->conj_count, &ctx->cand, &ctx->cand_count,
+                              &ctx->steps, &ctx->steps_count, &ctx->slow, &ctx->slow_count, &ctx->wl_slow,
+                              &ctx->ev_tm, &ctx->ev_sel, &ctx->ev_out, &ctx->ev_bad})
+                b->release();
+        }
+        if (what & MZH_RELEASE_RESULTS) {
+            for (DevBuf *b : {&ctx->aabbs, &ctx->srt_aabbs, &ctx->mcodes, &ctx->srt_mcodes, &ctx->srt_idx, &ctx->nodes,
+                              &ctx->trav, &ctx->tree_size, &ctx->pairs, &ctx->conj_sorted, &ctx->step_off,
+                              &ctx->cd_begin, &ctx->cd_end})
+                b->release();
+            ctx->nodes_ready = false;
+            if (ctx->screened) ctx->results_released = true;
+        }
+        if (what & MZH_RELEASE_POLYJECTORY) {
+            ctx->pjb.reset();
+            ctx->has_pj = false;
         }
     });
 }

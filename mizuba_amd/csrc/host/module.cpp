@@ -108,8 +108,9 @@ py::array ro_view_dtype(const py::object &base, const py::dtype &dt, const void 
 // polyjectory: minimal in-memory equivalent of mizuba::polyjectory
 // ---------------------------------------------------------------------------
 
-// Device copy of a polyjectory for state_eval / state_meval, made on the
-// first evaluation and kept for the polyjectory's lifetime.
+// Device copy of a polyjectory, made on its first use on a device
+// (state_eval / state_meval, conjunctions) and kept until the polyjectory is
+// released; screenings share its arrays (mzh_share_polyjectory).
 struct PjDevice {
     mzh_ctx *ctx = nullptr;
     PjDevice() = default;
@@ -127,8 +128,9 @@ struct PjData {
     std::vector<double> traj, time;
     std::vector<uint64_t> traj_off, nsteps, time_off, time_len;
     std::vector<int32_t> status;
-    std::mutex eval_m;
-    std::unique_ptr<PjDevice> eval_dev;
+    std::mutex eval_m;  // evaluations on the device copies
+    std::mutex dev_m;   // the map of device copies
+    std::map<int, std::unique_ptr<PjDevice>> devs;
     // data dir holding the polyjectory's files (empty: in memory only); a
     // non-persistent one is removed with the data (polyjectory.cpp:245-290)
     std::filesystem::path data_dir;
@@ -148,6 +150,22 @@ struct PjData {
 struct Polyjectory {
     std::shared_ptr<PjData> d;
 };
+
+// The context holding the device copy of d on `device`, uploaded on first use.
+mzh_ctx *pj_device(PjData &d, int device)
+{
+    std::lock_guard<std::mutex> lk(d.dev_m);
+    auto &slot = d.devs[device];
+    if (!slot) {
+        auto dev = std::make_unique<PjDevice>();
+        check(mzh_create(device, &dev->ctx), nullptr);
+        check(mzh_upload_polyjectory(dev->ctx, d.n_objs, d.order, d.traj.data(), d.traj.size(), d.time.data(),
+                                     d.time.size(), d.traj_off.data(), d.nsteps.data(), d.time_off.data(), d.maxT),
+              dev->ctx);
+        slot = std::move(dev);
+    }
+    return slot->ctx;
+}
 
 // polyjectory::check_attached (polyjectory.cpp:333-338).
 PjData &attached(const Polyjectory &p)
@@ -403,6 +421,19 @@ struct Ctx {
         pj = d;
     }
 
+    // use the polyjectory's cached device copy instead of uploading it
+    void share(const std::shared_ptr<PjData> &d, int device)
+    {
+        int st;
+        {
+            py::gil_scoped_release rel;
+            mzh_ctx *src = pj_device(*d, device);
+            st = mzh_share_polyjectory(ctx, src);
+        }
+        check(st, ctx);
+        pj = d;
+    }
+
     void screen(double thresh, double interval, const std::optional<std::vector<int32_t>> &otypes, int mode)
     {
         if (otypes && otypes->size() != pj->n_objs)
@@ -430,16 +461,9 @@ void pj_state_eval(PjData &d, const double *tm, int mode, const uint64_t *sel, u
 {
     if (n_rows == 0 || n_times == 0) return;
     py::gil_scoped_release rel;
+    mzh_ctx *ctx = pj_device(d, 0);
     std::lock_guard<std::mutex> lk(d.eval_m);
-    if (!d.eval_dev) {
-        auto dev = std::make_unique<PjDevice>();
-        check(mzh_create(0, &dev->ctx), nullptr);
-        check(mzh_upload_polyjectory(dev->ctx, d.n_objs, d.order, d.traj.data(), d.traj.size(), d.time.data(),
-                                     d.time.size(), d.traj_off.data(), d.nsteps.data(), d.time_off.data(), d.maxT),
-              dev->ctx);
-        d.eval_dev = std::move(dev);
-    }
-    check(mzh_state_eval(d.eval_dev->ctx, tm, mode, sel, n_rows, n_times, out), d.eval_dev->ctx);
+    check(mzh_state_eval(ctx, tm, mode, sel, n_rows, n_times, out), ctx);
 }
 
 void check_cc_aligned(const py::array &a, const char *msg)
@@ -773,6 +797,9 @@ void finish_impl(ConjImpl &im, const Polyjectory &pj, double conj_thresh, double
     im.otypes = otypes ? *otypes : std::vector<int32_t>(pj.d->n_objs, 1);
     im.cd_end_times.resize(im.C);
     check(mzh_copy_cd_end_times(im.ctx->ctx, im.cd_end_times.data()), im.ctx->ctx);
+    // the object keeps only what its accessors read (the polyjectory's device
+    // copy stays with the polyjectory)
+    check(mzh_release(im.ctx->ctx, MZH_RELEASE_SCRATCH | MZH_RELEASE_POLYJECTORY), im.ctx->ctx);
 }
 
 // The result files of detect_conjunctions (detect_conjunctions.cpp:91-126,
@@ -840,6 +867,43 @@ void store_conjunctions(ConjImpl &im, const std::filesystem::path &dir)
     }
 }
 
+// hint_release (conjunctions.cpp:466-477 madvises the memory-mapped results
+// away): every result is copied into the object's host caches, which the
+// accessors read from then on, and the device copies are freed.
+void conj_hint_release(ConjImpl &im)
+{
+    mzh_ctx *ctx = im.ctx->ctx;
+    lazy<float>(im, im.aabbs, im.C * (im.N + 1) * 8, [&](float *p) { return mzh_copy_aabbs(ctx, p); });
+    lazy<float>(im, im.srt_aabbs, im.C * (im.N + 1) * 8, [&](float *p) { return mzh_copy_srt_aabbs(ctx, p); });
+    lazy<uint64_t>(im, im.mcodes, im.C * im.N, [&](uint64_t *p) { return mzh_copy_mcodes(ctx, p); });
+    lazy<uint64_t>(im, im.srt_mcodes, im.C * im.N, [&](uint64_t *p) { return mzh_copy_srt_mcodes(ctx, p); });
+    lazy<uint32_t>(im, im.srt_idx, im.C * im.N, [&](uint32_t *p) { return mzh_copy_srt_idx(ctx, p); });
+    uint64_t nc = 0;
+    check(mzh_get_n_conjunctions(ctx, &nc), ctx);
+    lazy<mzh_conj>(im, im.conjs, nc, [&](mzh_conj *p) { return mzh_copy_conjunctions(ctx, p); });
+    std::lock_guard<std::mutex> lk(im.m);
+    py::gil_scoped_release rel;
+    for (uint64_t k = 0; k < im.C; ++k) {
+        if (!im.trees.count(k)) {
+            uint64_t n = 0;
+            check(mzh_get_bvh_tree_size(ctx, k, &n), ctx);
+            std::vector<mzh_bvh_node> v(n ? n : 1);
+            check(mzh_copy_bvh_tree(ctx, k, v.data()), ctx);
+            v.resize(n);
+            im.trees.emplace(k, std::move(v));
+        }
+        if (!im.bps.count(k)) {
+            uint64_t n = 0;
+            check(mzh_get_aabb_collisions_size(ctx, k, &n), ctx);
+            std::vector<mzh_aabb_collision> v(n ? n : 1);
+            check(mzh_copy_aabb_collisions(ctx, k, v.data()), ctx);
+            v.resize(n);
+            im.bps.emplace(k, std::move(v));
+        }
+    }
+    check(mzh_release(ctx, MZH_RELEASE_RESULTS), ctx);
+}
+
 void check_otypes_size(const std::optional<std::vector<int32_t>> &otypes, uint64_t n_objs)
 {
     if (otypes && otypes->size() != n_objs)
@@ -882,7 +946,7 @@ Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double
         im->data_dir = mzh_host::create_temp_dir(tmpdir, "mizuba_conjunctions");
     }
     im->ctx = std::make_unique<Ctx>(device);
-    im->ctx->upload(pj.d);
+    im->ctx->share(pj.d, device);
     im->ctx->screen(conj_thresh, conj_det_interval, otypes, m);
     finish_impl(*im, pj, conj_thresh, conj_det_interval, otypes, m);
     if (!im->data_dir.empty()) {
@@ -914,7 +978,7 @@ std::vector<Conjunctions> make_conjunctions_batch(const std::vector<Polyjectory>
     for (size_t k = 0; k < n; ++k) {
         ims[k] = std::make_shared<ConjImpl>();
         ims[k]->ctx = std::make_unique<Ctx>(device);
-        ims[k]->ctx->upload(pjs[k].d);
+        ims[k]->ctx->share(pjs[k].d, device);
         ctxs[k] = ims[k]->ctx->ctx;
         if (otypes && (*otypes)[k]) ots[k] = (*otypes)[k]->data();
     }
@@ -1029,8 +1093,13 @@ PYBIND11_MODULE(_core, m)
     pt.def("detach", [](Polyjectory &p) { p.d.reset(); });
     pt.def_property_readonly("is_detached", [](const Polyjectory &p) { return !p.d; });
     pt.def("hint_release", [](const Polyjectory &p) {
-        // the data is held in memory, not in mapped files: nothing to release
-        (void)attached(p);
+        // the data is held in host memory, not in mapped files: the device
+        // copies are what can be released (screenings made from them keep
+        // what they need)
+        auto &d = attached(p);
+        std::lock_guard<std::mutex> le(d.eval_m);
+        std::lock_guard<std::mutex> lk(d.dev_m);
+        d.devs.clear();
     });
     pt.def_property_readonly("status", [](const py::object &self) {
         const auto &d = attached(self.cast<const Polyjectory &>());
@@ -1071,15 +1140,19 @@ PYBIND11_MODULE(_core, m)
                 throw std::invalid_argument(
                     "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time");
             const auto gpes = gpes_from_array(gpes_);
-            Ctx c(device);
+            // the context that generates the polyjectory keeps it in HBM and
+            // becomes the polyjectory's device copy on this device
+            auto dev = std::make_unique<PjDevice>();
+            check(mzh_create(device, &dev->ctx), nullptr);
             {
                 py::gil_scoped_release rel;
-                check(mzh_make_sgp4_polyjectory(c.ctx, gpes.data(), gpes.size(), jd_begin, jd_end, reentry_radius,
+                check(mzh_make_sgp4_polyjectory(dev->ctx, gpes.data(), gpes.size(), jd_begin, jd_end, reentry_radius,
                                                 exit_radius, order),
-                      c.ctx);
+                      dev->ctx);
             }
-            auto d = pj_from_ctx(c.ctx);
+            auto d = pj_from_ctx(dev->ctx);
             if ((data_dir && !data_dir->empty()) || tmpdir || persist) store_pj(*d, data_dir, tmpdir, persist);
+            d->devs[device] = std::move(dev);
             return Polyjectory{d};
         },
         "gpes"_a, "jd_begin"_a, "jd_end"_a, "reentry_radius"_a, "exit_radius"_a, "data_dir"_a, "persist"_a,
@@ -1451,7 +1524,7 @@ PYBIND11_MODULE(_core, m)
     });
     cj.def_property_readonly("conj_thresh", [](const Conjunctions &c) { return c.impl->conj_thresh; });
     cj.def_property_readonly("conj_det_interval", [](const Conjunctions &c) { return c.impl->conj_det_interval; });
-    cj.def("hint_release", [](const Conjunctions &) {});
+    cj.def("hint_release", [](const Conjunctions &c) { conj_hint_release(*c.impl); });
     cj.def_property_readonly("data_dir", [](const Conjunctions &c) -> py::object {
         if (c.impl->data_dir.empty()) return py::none();
         return py::cast(c.impl->data_dir);

@@ -164,3 +164,43 @@ def test_many_conjunctions_in_one_step():
     assert len(g) == len(r)
     assert np.array_equal(_bits(g), _bits(r))
     assert np.all(np.diff(g["tca"]) >= 0)
+
+
+def test_shared_device_copy_and_hint_release():
+    """Screenings of one polyjectory share its device copy; hint_release moves
+    a screening's results to host memory and frees the device copies, after
+    which every accessor still returns the same arrays; releasing the
+    polyjectory's device copy makes the next screening upload it again."""
+    pj = mz.make_synthetic_polyjectory(600, 0.3, seed=17, frac_high=0.2)
+    c1 = mz.conjunctions(pj, 10.0, 3.0 / 1440.0)
+    c2 = mz.conjunctions(pj, 10.0, 3.0 / 1440.0, mode="mixed")
+    ref = {
+        name: getattr(c1, name).copy()
+        for name in ("aabbs", "srt_aabbs", "mcodes", "srt_mcodes", "srt_idx", "conjunctions", "cd_end_times")
+    }
+    trees = [c1.get_bvh_tree(k).copy() for k in range(c1.n_cd_steps)]
+    pairs = [c1.get_aabb_collisions(k).copy() for k in range(c1.n_cd_steps)]
+    rep = c1.np_report
+    c3 = mz.conjunctions(pj, 10.0, 3.0 / 1440.0)
+    c3.hint_release()
+    for name, a in ref.items():
+        assert np.array_equal(_bits(getattr(c3, name)), _bits(a)), name
+    for k in range(c3.n_cd_steps):
+        assert np.array_equal(_bits(c3.get_bvh_tree(k)), _bits(trees[k]))
+        assert np.array_equal(_bits(c3.get_aabb_collisions(k)), _bits(pairs[k]))
+    assert c3.np_report == rep
+    with pytest.raises(IndexError):
+        c3.get_bvh_tree(c3.n_cd_steps)
+    c3.hint_release()  # again: nothing left to release
+    assert np.array_equal(_bits(c2.conjunctions), _bits(ref["conjunctions"]))
+    pj.hint_release()
+    c4 = mz.conjunctions(pj, 10.0, 3.0 / 1440.0)
+    assert np.array_equal(_bits(c4.conjunctions), _bits(ref["conjunctions"]))
+    # the device copy left by make_sgp4_polyjectory serves the screening
+    g = mz.make_synthetic_gpes(2000, seed=3, frac_high=0.2, epoch_jd=2460496.5)
+    spj = mz.make_sgp4_polyjectory(g, 2460496.5, 2460496.75)[0]
+    a = mz.conjunctions(spj, 10.0, 3.0 / 1440.0).conjunctions
+    scr = mz.Screener(0)
+    scr.upload(spj)  # an upload of the host copy
+    scr.screen(10.0, 3.0 / 1440.0)
+    assert len(a) > 0 and np.array_equal(_bits(a), _bits(scr.conjunctions()))
