@@ -250,6 +250,9 @@ std::string dev_error_message(const DevError &e)
         case kErrNpIsolation:
             return "The polynomial root isolation algorithm failed: the working list size is " + i
                    + " and the number of isolating intervals is " + j;
+        case kErrRfInterval:
+            return "Invalid root-finding interval " + v + " for the polynomial at index " + i
+                   + ": it must be finite and positive";
         case kErrNpIters:
             return "Polynomial root finding failed due to too many iterations";
         case kErrNpErrno:
@@ -1342,47 +1345,40 @@ int mzh_root_finding(mzh_ctx *ctx, uint32_t order, uint64_t n, const double *pol
             throw StatusError{MZH_EINVAL, "Polynomial order " + std::to_string(order) + " is not supported"};
         if (n == 0) return;
         if (!polys || !rf_int || !roots || !n_roots) throw StatusError{MZH_EINVAL, "Null pointer passed to mzh_root_finding"};
-        for (uint64_t k = 0; k < n; ++k)
-            if (!std::isfinite(rf_int[k]) || !(rf_int[k] > 0))
-                throw StatusError{MZH_EINVAL, "Invalid root-finding interval " + fmt_double(rf_int[k])};
         const uint32_t op1 = order + 1, nmax = 2u * order + 2u;
         hipStream_t st = ctx->stream;
-        DevBuf d_p, d_rf, d_roots, d_info, d_slow, d_cnt, d_wl, d_err;
+        DevBuf d_p, d_rf, d_roots, d_cnt_r, d_flags, d_slow, d_cnt, d_wl, d_err;
         MZH_HIP(d_p.ensure(sizeof(double) * op1 * n));
         MZH_HIP(d_rf.ensure(sizeof(double) * n));
         MZH_HIP(d_roots.ensure(sizeof(double) * nmax * n));
-        MZH_HIP(d_info.ensure(sizeof(uint32_t) * n));
+        MZH_HIP(d_cnt_r.ensure(sizeof(uint32_t) * n));
+        MZH_HIP(d_flags.ensure(sizeof(uint32_t) * n));
         MZH_HIP(d_slow.ensure(sizeof(uint64_t) * n));
         MZH_HIP(d_cnt.ensure(sizeof(unsigned long long)));
         MZH_HIP(d_wl.ensure(narrow_wl_slow_bytes(order)));
         MZH_HIP(d_err.ensure(sizeof(DevError)));
         MZH_HIP(hipMemcpyAsync(d_p.p, polys, sizeof(double) * op1 * n, hipMemcpyHostToDevice, st));
         MZH_HIP(hipMemcpyAsync(d_rf.p, rf_int, sizeof(double) * n, hipMemcpyHostToDevice, st));
-        MZH_HIP(hipMemsetAsync(d_cnt.p, 0, sizeof(unsigned long long), st));
-        MZH_HIP(hipMemsetAsync(d_err.p, 0, sizeof(DevError), st));
         RootFindArgs a;
         a.polys = d_p.as<const double>();
         a.rf_int = d_rf.as<const double>();
         a.n = n;
         a.roots = d_roots.as<double>();
-        a.info = d_info.as<uint32_t>();
+        a.n_roots = d_cnt_r.as<uint32_t>();
+        a.flags = d_flags.as<uint32_t>();
         a.slow = d_slow.as<uint64_t>();
         a.slow_count = d_cnt.as<unsigned long long>();
         a.wl_slow = d_wl.p;
         a.wl_fast_cap = (int)std::min<uint64_t>(env_u64("MIZUBA_AMD_WL_FAST_CAP", kNarrowWlFastCap), kNarrowWlFastCap);
         a.err = d_err.as<DevError>();
         MZH_HIP(launch_rootfind(order, a, st));
-        std::vector<uint32_t> info(n);
         DevError herr;
-        MZH_HIP(hipMemcpyAsync(info.data(), d_info.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
-        MZH_HIP(hipMemcpyAsync(roots, d_roots.p, sizeof(double) * nmax * n, hipMemcpyDeviceToHost, st));
         MZH_HIP(hipMemcpyAsync(&herr, d_err.p, sizeof(DevError), hipMemcpyDeviceToHost, st));
         MZH_HIP(hipStreamSynchronize(st));
         if (herr.code != kErrNone) throw StatusError{MZH_EINVAL, dev_error_message(herr)};
-        for (uint64_t k = 0; k < n; ++k) {
-            n_roots[k] = info[k] & 0xffu;
-            if (flags) flags[k] = (info[k] >> 8) & 3u;
-        }
+        MZH_HIP(hipMemcpy(n_roots, d_cnt_r.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        MZH_HIP(hipMemcpy(roots, d_roots.p, sizeof(double) * nmax * n, hipMemcpyDeviceToHost));
+        if (flags) MZH_HIP(hipMemcpy(flags, d_flags.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
     });
 }
 

@@ -104,7 +104,7 @@ enum DevErrCode : int32_t {
     kErrNpBndDist2Ub = 19,
     kErrNpBndDerLb = 20,
     kErrNpBndDist2Lb = 21,
-    kErrNpWorkList = 22,   // device work-list capacity exceeded
+    kErrRfInterval = 23,   // invalid root-finding interval (value, i = polynomial)
 };
 
 __device__ inline void report_error(DevError *e, int32_t code, uint32_t i, uint32_t j, double v)

@@ -117,16 +117,19 @@ constexpr int kNarrowWlFastCap = 48;
 // Root finding of free-standing polynomials (run_poly_root_finding with
 // direction +1, poly_utils.cpp:272-460), as the narrow phase does it.
 //   polys [n][order + 1], rf_int [n] (device)
-//   roots [n][2 * order + 2]: the roots found, info[k] = number of roots |
-//   (fast exclusion) << 8 | (isolated on the slow path) << 9, 0xfffffffe on
-//   error (the first error in err, with i = the polynomial's index).
-//   slow: n entries of scratch; wl_slow: narrow_wl_slow_bytes(order).
+//   roots [n][2 * order + 2]: the roots found; n_roots[k]: their number
+//   (0xfffffffe on error: the first error is in err, with i = the
+//   polynomial's index); flags[k] (may be null): bit 0 the fast exclusion
+//   check fired, bit 1 the isolation ran on the slow path.
+//   Scratch: slow (n entries), slow_count and err (cleared by the launch),
+//   wl_slow (narrow_wl_slow_bytes(order)).
 struct RootFindArgs {
     const double *polys;
     const double *rf_int;
     uint64_t n;
     double *roots;
-    uint32_t *info;
+    uint32_t *n_roots;
+    uint32_t *flags;
     uint64_t *slow;
     unsigned long long *slow_count;
     void *wl_slow;

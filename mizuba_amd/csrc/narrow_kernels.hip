@@ -916,18 +916,25 @@ __global__ __launch_bounds__(256) void k_np_roots_slow(NarrowArgs a)
 template <int ORDER>
 __device__ inline int rootfind_one(const RootFindArgs &a, uint64_t k, WLEntry<ORDER> *wl, int cap, uint32_t slow)
 {
+    const double rf_int = a.rf_int[k];
+    if (!is_finite_d(rf_int) || !(rf_int > 0)) {
+        report_error(a.err, kErrRfInterval, (uint32_t)k, 0u, rf_int);
+        a.n_roots[k] = 0xfffffffeu;
+        return kFmError;
+    }
     double poly[ORDER + 1];
 #pragma unroll
     for (int m = 0; m <= ORDER; ++m) poly[m] = a.polys[k * (ORDER + 1) + m];
     double roots[kMaxRoots<ORDER>];
     int nr = 0;
     bool fex = false;
-    const int fm = find_minima<ORDER>(poly, a.rf_int[k], roots, nr, fex, (uint32_t)k, 0u, a.err, wl, cap);
+    const int fm = find_minima<ORDER>(poly, rf_int, roots, nr, fex, (uint32_t)k, 0u, a.err, wl, cap);
     if (fm == kFmOk) {
-        for (int r = 0; r < nr; ++r) a.roots[k * kMaxRoots<ORDER> + r] = roots[r];
-        a.info[k] = (uint32_t)nr | (fex ? 256u : 0u) | (slow << 9);
+        for (int r = 0; r < kMaxRoots<ORDER>; ++r) a.roots[k * kMaxRoots<ORDER> + r] = r < nr ? roots[r] : 0.0;
+        a.n_roots[k] = (uint32_t)nr;
+        if (a.flags) a.flags[k] = (fex ? 1u : 0u) | (slow << 1);
     } else if (fm == kFmError) {
-        a.info[k] = 0xfffffffeu;
+        a.n_roots[k] = 0xfffffffeu;
     }
     return fm;
 }
@@ -940,8 +947,19 @@ __global__ __launch_bounds__(256) void k_rootfind(RootFindArgs a)
     WLEntry<ORDER> wl[kWlCap];
     const int cap = a.wl_fast_cap < kWlCap ? a.wl_fast_cap : kWlCap;
     if (rootfind_one<ORDER>(a, k, wl, cap, 0u) == kFmListFull) {
-        a.info[k] = 0xffffffffu;
+        a.n_roots[k] = 0xffffffffu;
         a.slow[atomicAdd(a.slow_count, 1ull)] = k;
+    }
+}
+
+// zero the scratch counters of a root-finding launch
+__global__ void k_rootfind_clear(unsigned long long *slow_count, DevError *err)
+{
+    if (threadIdx.x == 0) {
+        *slow_count = 0ull;
+        err->code = 0;
+        err->i = err->j = 0u;
+        err->value = 0.0;
     }
 }
 
@@ -1122,6 +1140,7 @@ template <int ORDER>
 hipError_t launch_rootfind_order(const RootFindArgs &a, hipStream_t stream)
 {
     if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rootfind_clear, dim3(1), dim3(64), 0, stream, a.slow_count, a.err);
     hipLaunchKernelGGL(k_rootfind<ORDER>, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
