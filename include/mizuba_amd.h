@@ -269,6 +269,24 @@ int mzh_screen_batch(mzh_ctx *const *ctxs, uint64_t n, const double *conj_thresh
 int mzh_root_finding(mzh_ctx *ctx, uint32_t order, uint64_t n, const double *polys, const double *rf_int,
                      double *roots, uint32_t *n_roots, uint32_t *flags);
 
+/* mzh_root_finding with device pointers (polys, rf_int, roots, n_roots and
+ * flags in device memory on the context's device; flags may be NULL) that
+ * allocates no memory: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * reference counterpart. Its scratch (the slow path's counter, queue and
+ * 251-entry work lists, and the error word) is a fixed 16 MiB array of the
+ * library on each device, held until the process ends; n may not exceed what
+ * it holds ((16 MiB - 256 B - the work lists) / 8 polynomials: about 1.39
+ * million at order 8, 0.62 million at order 20), else MZH_EINVAL with a
+ * message saying so. The call clears the scratch it uses, waits for its
+ * kernels and returns MZH_EINVAL with the reference's message on an
+ * isolation failure. Calls on one device share that scratch: calls that
+ * overlap on one device, from any contexts, race. Outputs as in
+ * mzh_root_finding: n_roots[k] = 0xfffffffe for a polynomial whose root
+ * finding failed. */
+int mzh_root_finding_noalloc(mzh_ctx *ctx, uint32_t order, uint64_t n, const double *polys, const double *rf_int,
+                             double *roots, uint32_t *n_roots, uint32_t *flags);
+
 /* ---- accessors (mizuba/conjunctions.cpp:394-509) ------------------------ */
 
 /* conjunctions::get_n_cd_steps (conjunctions.cpp:457-460). */

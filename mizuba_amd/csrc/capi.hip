@@ -19,6 +19,7 @@
 #include <thread>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mizuba_amd.h"
@@ -192,6 +193,43 @@ namespace
 {
 
 thread_local std::string g_global_error;
+
+// ---- scratch of the allocation-free root finder ---------------------------
+// mzh_root_finding_noalloc keeps its scratch (slow-path counter, error word,
+// the slow pass's work lists and its queue) in this array; every device the
+// library runs on holds its own copy until the process ends. Host code must
+// not pass `scratch` itself to a kernel: it stands for a device address only
+// on the device. Its per-device address comes from hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = 16u << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_m;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current;
+// resolved once per device.
+hipError_t scratch_address(int device, void **ptr)
+{
+    *ptr = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_m);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[device] = p;
+    }
+    *ptr = g_scratch_dev[device];
+    return hipSuccess;
+}
+
+// Layout of the scratch for a root-finding call at `order`: slow-path
+// counter at 0, error word at 64, the slow pass's work lists from 256, then
+// the slow queue (8 bytes per polynomial).
+constexpr size_t kScrCount = 0, kScrErr = 64, kScrLists = 256;
+size_t scr_queue_offset(uint32_t order) { return kScrLists + (narrow_wl_slow_bytes(order) + 255u) / 256u * 256u; }
+uint64_t noalloc_max_polys(uint32_t order) { return (kNoAllocScratchBytes - scr_queue_offset(order)) / 8u; }
 
 int fail(mzh_ctx *ctx, int code, const std::string &msg)
 {
@@ -712,6 +750,12 @@ int mzh_create(int device, mzh_ctx **out)
     }
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
+    // set-up of the device's scratch address (once per device); a failure is
+    // reported by mzh_root_finding_noalloc, which tries again
+    if (e == hipSuccess) {
+        void *scr = nullptr;
+        (void)scratch_address(device, &scr);
+    }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     for (int k = 0; k < 12 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev[k]);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev_eval[k]);
@@ -1379,6 +1423,45 @@ int mzh_root_finding(mzh_ctx *ctx, uint32_t order, uint64_t n, const double *pol
         MZH_HIP(hipMemcpy(n_roots, d_cnt_r.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
         MZH_HIP(hipMemcpy(roots, d_roots.p, sizeof(double) * nmax * n, hipMemcpyDeviceToHost));
         if (flags) MZH_HIP(hipMemcpy(flags, d_flags.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int mzh_root_finding_noalloc(mzh_ctx *ctx, uint32_t order, uint64_t n, const double *polys, const double *rf_int,
+                             double *roots, uint32_t *n_roots, uint32_t *flags)
+{
+    return guarded(ctx, [&] {
+        if (order < (uint32_t)kMinOrder || order > (uint32_t)kMaxOrder)
+            throw StatusError{MZH_EINVAL, "Polynomial order " + std::to_string(order) + " is not supported"};
+        if (n == 0) return;
+        if (!polys || !rf_int || !roots || !n_roots)
+            throw StatusError{MZH_EINVAL, "Null pointer passed to mzh_root_finding_noalloc"};
+        if (n > noalloc_max_polys(order))
+            throw StatusError{MZH_EINVAL, "mzh_root_finding_noalloc: " + std::to_string(n)
+                                              + " polynomials of order " + std::to_string(order)
+                                              + " exceed the " + std::to_string(noalloc_max_polys(order))
+                                              + " that its fixed scratch of "
+                                              + std::to_string(kNoAllocScratchBytes >> 20) + " MiB holds"};
+        uint8_t *scr = nullptr;
+        MZH_HIP(scratch_address(ctx->device, reinterpret_cast<void **>(&scr)));
+        hipStream_t st = ctx->stream;
+        RootFindArgs a;
+        a.polys = polys;
+        a.rf_int = rf_int;
+        a.n = n;
+        a.roots = roots;
+        a.n_roots = n_roots;
+        a.flags = flags;
+        a.slow_count = reinterpret_cast<unsigned long long *>(scr + kScrCount);
+        a.err = reinterpret_cast<DevError *>(scr + kScrErr);
+        a.wl_slow = scr + kScrLists;
+        a.slow = reinterpret_cast<uint64_t *>(scr + scr_queue_offset(order));
+        a.wl_fast_cap = (int)std::min<uint64_t>(env_u64("MIZUBA_AMD_WL_FAST_CAP", kNarrowWlFastCap), kNarrowWlFastCap);
+        // the launch clears the counter and the error word first
+        MZH_HIP(launch_rootfind(order, a, st));
+        DevError herr;
+        MZH_HIP(hipMemcpyAsync(&herr, a.err, sizeof(DevError), hipMemcpyDeviceToHost, st));
+        MZH_HIP(hipStreamSynchronize(st));
+        if (herr.code != kErrNone) throw StatusError{MZH_EINVAL, dev_error_message(herr)};
     });
 }
 
