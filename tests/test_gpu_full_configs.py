@@ -1,9 +1,11 @@
-"""GPU runs at the full sizes of the larger configurations: 30 000 objects over
-7 days (3 360 conjunction steps) and the dense-shell stress case (100 000
-objects, 70 000 of them in five near-circular shells, 10 km, 2-minute steps);
-and, with catalogs made the configurations' way (seeded SGP4 mean elements
-through make_sgp4_polyjectory), the FP64 drop-in at 20 000 objects checked
-against the oracle on every step, and the dense-shell case again.
+"""GPU runs of the configurations at their full sizes, on catalogs made the
+configurations' way (seeded SGP4 mean elements through make_sgp4_polyjectory)
+and, for the dense-shell case, also on the Keplerian stand-in catalog:
+1 000 LEO objects over a day and the FP64 drop-in at 20 000 objects, checked
+against the oracle on every step; 30 000 objects over 7 days (3 360
+conjunction steps), checked against the oracle on every step window by
+window; the dense-shell stress case (100 000 objects, 70 000 of them in five
+near-circular shells, 10 km, 2-minute steps).
 
 The CPU oracle cannot screen these in a test's time, so they are checked
 through properties that hold at any size, and against the oracle on windows
@@ -51,6 +53,7 @@ def check_window(c, pj, thresh, interval, k0, k1):
     upper = (cj["tca"] <= t1) if k1 == c.n_cd_steps else (cj["tca"] < t1)
     g = cj[(cj["tca"] >= t0) & upper]
     assert np.array_equal(_bits(g), _bits(ref["conjunctions"])), (k0, k1, len(g), len(ref["conjunctions"]))
+    return len(g)
 
 
 def check_properties(c64, cmx, thresh):
@@ -76,16 +79,52 @@ def check_properties(c64, cmx, thresh):
         assert np.all(np.diff(key.astype(np.int64)) > 0)
 
 
-def test_config_30k_7days():
-    pj = mz.make_synthetic_polyjectory(30000, 7.0, seed=2026, frac_high=0.2)
+def check_all_steps(c, pj, thresh, interval, window):
+    """The whole screening against the oracle, window by window (the oracle
+    screens any step range on its own): every step's boxes, codes, sort
+    order, tree and pairs, and every conjunction."""
+    C = c.n_cd_steps
+    n_conj = 0
+    for k0 in range(0, C, window):
+        k1 = min(C, k0 + window)
+        n_conj += check_window(c, pj, thresh, interval, k0, k1)
+    assert n_conj == len(c.conjunctions)
+
+
+def test_config_1000_sgp4_full_day():
+    # the CPU configuration: 1 000 LEO objects from seeded SGP4 mean elements,
+    # 1 day, 5 km, 3 minutes; every step and counter against the oracle
+    from oracle import pyoracle as po
+
+    pj = _sgp4_catalog(1000, 1.0, seed=2024, frac_high=0.0)
+    thresh, interval = 5.0, 3.0 / 1440.0
+    c64 = mz.conjunctions(pj, thresh, interval, mode="fp64")
+    cmx = mz.conjunctions(pj, thresh, interval, mode="mixed")
+    assert c64.n_cd_steps == 480
+    ref = po.run(H.flat_of(pj), thresh, interval)
+    for name in ("aabbs", "srt_aabbs", "mcodes", "srt_mcodes", "srt_idx"):
+        assert np.array_equal(_bits(getattr(c64, name)), _bits(ref[name])), name
+    for k in range(c64.n_cd_steps):
+        assert np.array_equal(_bits(c64.get_bvh_tree(k)), _bits(po.get_bvh_tree(ref, k))), k
+        assert np.array_equal(_bits(c64.get_aabb_collisions(k)), _bits(po.get_aabb_collisions(ref, k))), k
+    assert np.array_equal(_bits(c64.conjunctions), _bits(ref["conjunctions"]))
+    assert np.array_equal(_bits(cmx.conjunctions), _bits(ref["conjunctions"]))
+    assert c64.np_report == ref["np_report"]
+
+
+def test_config_30k_7days_sgp4_every_step():
+    # the precision configuration: 30 000 objects (80% LEO, 20% higher) from
+    # seeded SGP4 mean elements over 7 days (3 360 steps of 3 minutes), 5 km;
+    # the FP64 screen equals the oracle on every step, mixed mode gives the
+    # same conjunctions
+    pj = _sgp4_catalog(30000, 7.0, seed=2026)
     thresh, interval = 5.0, 3.0 / 1440.0
     c64 = mz.conjunctions(pj, thresh, interval, mode="fp64")
     cmx = mz.conjunctions(pj, thresh, interval, mode="mixed")
     assert c64.n_cd_steps == 3360
     check_properties(c64, cmx, thresh)
-    check_window(c64, pj, thresh, interval, 0, 3)
-    check_window(c64, pj, thresh, interval, 1680, 1683)
-    check_window(c64, pj, thresh, interval, 3357, 3360)
+    del cmx
+    check_all_steps(c64, pj, thresh, interval, 420)
 
 
 def test_config_dense_shells_100k():
