@@ -246,13 +246,14 @@ int mzh_check_screen(uint64_t n_objs, double maxT, double conj_thresh, double co
                      const int32_t *otypes, int mode, uint64_t *n_cd_steps);
 
 /* Screen n independent problems together: ctxs[k] (each with its own uploaded
- * polyjectory) is screened with conj_thresh[k], conj_det_interval[k] and
- * otypes[k] (otypes may be NULL, as may any otypes[k]), exactly as
- * mzh_screen would. The problems run concurrently, each on its own context's
- * stream, so that many small problems fill the GPU together. status[k]
- * receives each problem's return code (its message through mzh_last_error on
- * ctxs[k]); the function returns the first non-zero code, MZH_OK if none, or
- * MZH_EINVAL if the arguments are malformed (a null or repeated context).
+ * polyjectory, all on one device) is screened with conj_thresh[k],
+ * conj_det_interval[k] and otypes[k] (otypes may be NULL, as may any
+ * otypes[k]), exactly as mzh_screen would. Every stage runs as one launch for
+ * all problems, on the first context's stream, so that many small problems
+ * fill the GPU together. status[k] receives each problem's return code (its
+ * message through mzh_last_error on ctxs[k]); the function returns the first
+ * non-zero code, MZH_OK if none, or MZH_EINVAL if the arguments are malformed
+ * (a null or repeated context, contexts on different devices).
  * The reference has no batched entry: it is conjunctions::conjunctions
  * (mizuba/conjunctions.cpp:263-360) called once per problem. */
 int mzh_screen_batch(mzh_ctx *const *ctxs, uint64_t n, const double *conj_thresh, const double *conj_det_interval,
@@ -286,6 +287,44 @@ int mzh_root_finding(mzh_ctx *ctx, uint32_t order, uint64_t n, const double *pol
  * finding failed. */
 int mzh_root_finding_noalloc(mzh_ctx *ctx, uint32_t order, uint64_t n, const double *polys, const double *rf_int,
                              double *roots, uint32_t *n_roots, uint32_t *flags);
+
+/* ---- a batch handle: the same problems screened again and again ---------- */
+
+/* A batch of n problems (as mzh_screen_batch takes them), kept for repeated
+ * screening: the serving loop of a catalogue whose polyjectories are
+ * refreshed in place, or the benchmark's repeated steps. The reference has no
+ * counterpart: conjunctions::conjunctions (mizuba/conjunctions.cpp:263-360)
+ * is called once per problem. Every stage runs as one launch for all
+ * problems; the sort and the BVH build are persistent kernels over
+ * (problem, conjunction step) work items. flags:
+ *   MZH_BATCH_GRAPH  after the first (eager) run, the whole launch sequence
+ *                    is captured into a HIP graph and later runs replay it,
+ *                    with the buffer sizes of the previous run plus a quarter;
+ *                    a replay whose sizes are too small is redone eagerly
+ *                    and captured again;
+ *   MZH_BATCH_TUNE   the first run first times the launch shapes (bucket-sort
+ *                    width, persistent grid sizes, broad-phase workgroup,
+ *                    root-finding grid) on this batch and keeps the fastest,
+ *                    also for later batches of the same size class in the
+ *                    process; the results do not depend on them.
+ * The contexts (on one device, each with a polyjectory) must outlive the
+ * batch; the arguments are validated as mzh_screen would (MZH_EINVAL,
+ * message in mzh_global_error). A context changed between runs (a new
+ * upload, a release, a screening by another call) is described again on the
+ * next run. Results are read through the contexts' accessors, as after
+ * mzh_screen. mzh_batch_run returns the first non-zero status; status[k]
+ * receives each problem's. */
+#define MZH_BATCH_GRAPH 1
+#define MZH_BATCH_TUNE 2
+typedef struct mzh_batch mzh_batch;
+int mzh_batch_create(mzh_ctx *const *ctxs, uint64_t n, const double *conj_thresh, const double *conj_det_interval,
+                     const int32_t *const *otypes, int mode, int flags, mzh_batch **out);
+int mzh_batch_run(mzh_batch *batch, int *status);
+/* Graph replays so far, device seconds of the last run, and the launch shape
+ * in use: bucket-sort bits, sort and BVH workgroups, broad-phase workgroup
+ * size, root-finding workgroups (any pointer may be NULL). */
+int mzh_batch_info(const mzh_batch *batch, uint64_t *replays, double *seconds, uint32_t shape[5]);
+int mzh_batch_destroy(mzh_batch *batch);
 
 /* ---- accessors (mizuba/conjunctions.cpp:394-509) ------------------------ */
 

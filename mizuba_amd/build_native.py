@@ -25,7 +25,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libmizuba_amd.so"
 
-HIP_SOURCES = ["capi.hip", "screen_kernels.hip", "narrow_kernels.hip", "eval_kernels.hip", "sgp4_kernels.hip"]
+HIP_SOURCES = ["capi.hip", "batch.hip", "screen_kernels.hip", "narrow_kernels.hip", "eval_kernels.hip", "sgp4_kernels.hip"]
 HOST_SOURCES = ["host/module.cpp", "host/synth.cpp", "host/pj_store.cpp"]
 
 ARCH = os.environ.get("MIZUBA_AMD_ARCH", "gfx950")

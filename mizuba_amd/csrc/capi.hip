@@ -23,171 +23,12 @@
 #include <vector>
 
 #include "../../include/mizuba_amd.h"
+#include "host_ctx.hpp"
 #include "mzh_device.hpp"
 #include "mzh_kernels.hpp"
 
-namespace
-{
-
 using namespace mzh;
-
-// Shortest decimal text that round-trips a double (the {} formatting of the
-// reference's fmt messages).
-std::string fmt_double(double v)
-{
-    if (std::isnan(v)) return "nan";
-    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
-    char buf[64];
-    for (int prec = 1; prec <= 17; ++prec) {
-        std::snprintf(buf, sizeof(buf), "%.*g", prec, v);
-        if (std::strtod(buf, nullptr) == v) break;
-    }
-    std::string s(buf);
-    // fmt prints integral doubles without exponent and without a trailing '.0'
-    return s;
-}
-
-// Positive integer from the environment, or dflt. Test hooks: the initial
-// sizes of the growable screening buffers can be made small so that the
-// resize-and-rerun paths run on small problems.
-uint64_t env_u64(const char *name, uint64_t dflt)
-{
-    if (const char *v = std::getenv(name)) {
-        const unsigned long long x = std::strtoull(v, nullptr, 10);
-        if (x > 0) return x;
-    }
-    return dflt;
-}
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t ensure(size_t b)
-    {
-        if (b <= bytes && p) return hipSuccess;
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-            bytes = 0;
-        }
-        const size_t alloc = b ? b : 256;
-        hipError_t e = hipMalloc(&p, alloc);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return e;
-        }
-        bytes = alloc;
-        return hipSuccess;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    // Grow to at least b bytes, keeping the first `keep` bytes.
-    hipError_t grow(size_t b, size_t keep, hipStream_t st)
-    {
-        if (b <= bytes && p) return hipSuccess;
-        void *q = nullptr;
-        hipError_t e = hipMalloc(&q, b);
-        if (e != hipSuccess) return e;
-        if (p && keep) {
-            e = hipMemcpyAsync(q, p, std::min(keep, bytes), hipMemcpyDeviceToDevice, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e != hipSuccess) {
-                (void)hipFree(q);
-                return e;
-            }
-        }
-        if (p) (void)hipFree(p);
-        p = q;
-        bytes = b;
-        return hipSuccess;
-    }
-    template <typename T>
-    T *as() const
-    {
-        return static_cast<T *>(p);
-    }
-};
-
-struct StatusError {
-    int code;
-    std::string msg;
-};
-
-// Device arrays of a polyjectory: the AoS coefficients, the times, the
-// step-major transposed positions and the per-object layout.
-struct PjBufs {
-    DevBuf traj, time, cf4, step0, nsteps, time_off;
-};
-
-} // namespace
-
-struct mzh_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string last_error;
-
-    // polyjectory: the device arrays may be shared by several contexts
-    // (mzh_share_polyjectory); they are read-only once uploaded
-    bool has_pj = false;
-    uint32_t n_objs = 0, order = 0;
-    double maxT = 0;
-    uint64_t n_steps_total = 0;
-    std::shared_ptr<PjBufs> pjb;
-    // host copies of its layout, statuses and epoch (TAI Julian date) for copy-out
-    std::vector<uint64_t> h_traj_off, h_nsteps, h_time_off;
-    std::vector<int32_t> h_status;
-    uint64_t traj_len = 0, time_len = 0;
-    double pj_epoch[2] = {0, 0};
-
-    // problem state
-    bool screened = false;
-    bool results_released = false;  // mzh_release(MZH_RELEASE_RESULTS) after the screen
-    bool skip_cd = false;
-    uint64_t C = 0;
-    double conj_thresh = 0, conj_det_interval = 0;
-    int mode = MZH_MODE_FP64;
-    std::vector<double> h_cd_begin, h_cd_end;
-    DevBuf cd_begin, cd_end;
-    DevBuf aabbs, kbke, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, sort_todo, gbox;
-    uint64_t node_cap = 0;
-    DevBuf nodes, trav, bvh_hdr, bvh_scr, tree_size;
-    bool nodes_ready = false;  // nodes holds the reference-layout export of trav
-    DevBuf otypes, cnt, head, off, step_total, step_off, grand, slots, ovf, ovf_count;
-    uint32_t ovf_cap = 0;
-    DevBuf pairs;
-    uint64_t n_pairs = 0;
-    DevBuf conj_raw, conj_step, conj_sorted, conj_count, np_report, conj_keys;
-    uint64_t conj_cap = 0, n_conj = 0;
-    DevBuf cand, cand_count, steps, steps_count, slow, slow_count, wl_slow;
-    uint64_t cand_cap = 0, steps_cap = 0, slow_cap = 0;
-    DevBuf err;
-    // polyjectory evaluation: staged times, selector and results, first bad element
-    DevBuf ev_tm, ev_sel, ev_out, ev_bad;
-    hipEvent_t ev_eval[2] = {};
-    double eval_s = 0;  // device time of the last evaluation kernel
-    // SGP4 generation: device times of its kernels (segment set-up, count,
-    // scan, write, transpose) in the last mzh_make_sgp4_polyjectory
-    hipEvent_t ev_sgp4[8] = {};
-    double sgp4_s[5] = {0, 0, 0, 0, 0};
-    hipEvent_t ev[12] = {};
-    double stage_s[MZH_STAGE_COUNT] = {};
-    uint64_t h_np[MZH_NP_COUNT] = {0, 0, 0, 0, 0, 0, 0};
-
-    // host caches of per-step sizes (filled after a screen)
-    std::vector<uint32_t> h_tree_size;
-    std::vector<uint64_t> h_step_off;  // [C + 1]
-};
+using namespace mzh::host;
 
 namespace
 {
@@ -237,14 +78,6 @@ int fail(mzh_ctx *ctx, int code, const std::string &msg)
     return code;
 }
 
-#define MZH_HIP(call)                                                                                                  \
-    do {                                                                                                               \
-        const hipError_t e_ = (call);                                                                                  \
-        if (e_ != hipSuccess)                                                                                          \
-            throw StatusError{e_ == hipErrorOutOfMemory ? MZH_ENOMEM : MZH_ERUNTIME,                                   \
-                              std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_)};                    \
-    } while (0)
-
 template <typename F>
 int guarded(mzh_ctx *ctx, F &&f)
 {
@@ -261,6 +94,11 @@ int guarded(mzh_ctx *ctx, F &&f)
         return fail(ctx, MZH_ERUNTIME, "unknown error");
     }
 }
+
+} // namespace
+
+namespace mzh::host
+{
 
 std::string dev_error_message(const DevError &e)
 {
@@ -319,12 +157,22 @@ std::string dev_error_message(const DevError &e)
     }
 }
 
+} // namespace mzh::host
+
+namespace
+{
+
 // Device arrays for a new polyjectory: the context's own, or fresh ones when
 // other contexts share the current arrays (they keep those).
 void own_pj_bufs(mzh_ctx *ctx)
 {
     if (!ctx->pjb || ctx->pjb.use_count() > 1) ctx->pjb = std::make_shared<PjBufs>();
 }
+
+} // namespace
+
+namespace mzh::host
+{
 
 PjView pj_view(const mzh_ctx *ctx)
 {
@@ -340,13 +188,16 @@ PjView pj_view(const mzh_ctx *ctx)
     return v;
 }
 
-// The arguments of one screening after validation.
-struct ScreenPlan {
-    uint64_t C = 0;
-    bool skip_cd = true;
-    std::vector<int32_t> ot;
-    std::vector<double> cd_begin, cd_end;
-};
+
+} // namespace mzh::host
+
+namespace
+{
+
+} // namespace
+
+namespace mzh::host
+{
 
 // conjunctions::conjunctions validation (conjunctions.cpp:279-335) with the
 // reference's messages, the step ranges of get_cd_begin_end
@@ -408,279 +259,21 @@ ScreenPlan plan_screen(uint32_t N, double maxT, double conj_thresh, double conj_
     return pl;
 }
 
-// conjunctions::conjunctions validation + detect_conjunctions, batched.
+} // namespace mzh::host
+
+namespace
+{
+
+// conjunctions::conjunctions validation + detect_conjunctions: a batch of one.
 void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes_in, int mode)
 {
     if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
-    const uint32_t N = ctx->n_objs;
-    ScreenPlan pl = plan_screen(N, ctx->maxT, conj_thresh, conj_det_interval, otypes_in, mode);
-    const uint64_t C = pl.C;
-    const bool skip_cd = pl.skip_cd;
-    const std::vector<int32_t> &ot = pl.ot;
-    ctx->h_cd_begin = std::move(pl.cd_begin);
-    ctx->h_cd_end = std::move(pl.cd_end);
-    const size_t boxes_bytes = sizeof(float) * 8u * (size_t)(N + 1) * C;
-    const size_t NC = (size_t)N * C;
-    hipStream_t st = ctx->stream;
-    ctx->screened = false;
-    ctx->results_released = false;
-    ctx->C = C;
-    ctx->conj_thresh = conj_thresh;
-    ctx->conj_det_interval = conj_det_interval;
-    ctx->mode = mode;
-    ctx->skip_cd = skip_cd;
-    ctx->node_cap = 2ull * (N ? N : 1);
-
-    MZH_HIP(ctx->cd_begin.ensure(sizeof(double) * C));
-    MZH_HIP(ctx->cd_end.ensure(sizeof(double) * C));
-    MZH_HIP(ctx->aabbs.ensure(boxes_bytes));
-    MZH_HIP(ctx->kbke.ensure(sizeof(uint2) * NC));
-    MZH_HIP(ctx->srt_aabbs.ensure(boxes_bytes));
-    MZH_HIP(ctx->mcodes.ensure(sizeof(uint64_t) * NC));
-    MZH_HIP(ctx->srt_mcodes.ensure(sizeof(uint64_t) * NC));
-    MZH_HIP(ctx->keys_tmp.ensure(sizeof(uint64_t) * NC));
-    MZH_HIP(ctx->srt_idx.ensure(sizeof(uint32_t) * NC));
-    MZH_HIP(ctx->vals_tmp.ensure(sizeof(uint32_t) * NC));
-    MZH_HIP(ctx->n_eff.ensure(sizeof(uint32_t) * C));
-    MZH_HIP(ctx->sort_todo.ensure(sizeof(uint32_t) * C));
-    MZH_HIP(ctx->gbox.ensure(sizeof(uint32_t) * 8u * C));
-    MZH_HIP(ctx->trav.ensure(sizeof(TravNode) * ctx->node_cap * C));
-    MZH_HIP(ctx->bvh_hdr.ensure(sizeof(uint4) * ctx->node_cap * C));
-    MZH_HIP(ctx->bvh_scr.ensure(bvh_scratch_stride(N) * C));
-    ctx->nodes_ready = false;
-    MZH_HIP(ctx->tree_size.ensure(sizeof(uint32_t) * C));
-    MZH_HIP(ctx->otypes.ensure(sizeof(int32_t) * (N ? N : 1)));
-    MZH_HIP(ctx->cnt.ensure(sizeof(uint32_t) * NC));
-    MZH_HIP(ctx->head.ensure(sizeof(uint32_t) * NC));
-    MZH_HIP(ctx->slots.ensure(sizeof(uint32_t) * 4u * NC));
-    if (ctx->ovf_cap == 0)
-        ctx->ovf_cap = (uint32_t)std::min<uint64_t>(env_u64("MIZUBA_AMD_INIT_OVF_CAP", NC / 8 + 4096), 0x7fffffffu);
-    MZH_HIP(ctx->ovf.ensure(sizeof(uint2) * ctx->ovf_cap));
-    MZH_HIP(ctx->ovf_count.ensure(sizeof(uint32_t)));
-    MZH_HIP(ctx->off.ensure(sizeof(uint64_t) * NC));
-    MZH_HIP(ctx->step_total.ensure(sizeof(uint64_t) * C));
-    MZH_HIP(ctx->step_off.ensure(sizeof(uint64_t) * (C + 1)));
-    MZH_HIP(ctx->grand.ensure(sizeof(uint64_t)));
-    MZH_HIP(ctx->err.ensure(sizeof(DevError)));
-    MZH_HIP(ctx->conj_count.ensure(sizeof(unsigned long long)));
-    MZH_HIP(ctx->np_report.ensure(sizeof(unsigned long long) * MZH_NP_COUNT));
-
-    MZH_HIP(hipMemcpyAsync(ctx->cd_begin.p, ctx->h_cd_begin.data(), sizeof(double) * C, hipMemcpyHostToDevice, st));
-    MZH_HIP(hipMemcpyAsync(ctx->cd_end.p, ctx->h_cd_end.data(), sizeof(double) * C, hipMemcpyHostToDevice, st));
-    MZH_HIP(hipMemcpyAsync(ctx->otypes.p, ot.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
-    MZH_HIP(hipMemsetAsync(ctx->err.p, 0, sizeof(DevError), st));
-    MZH_HIP(hipMemsetAsync(ctx->np_report.p, 0, sizeof(unsigned long long) * MZH_NP_COUNT, st));
-
-    const PjView pj = pj_view(ctx);
-    StepGrid sg;
-    sg.cd_begin = ctx->cd_begin.as<const double>();
-    sg.cd_end = ctx->cd_end.as<const double>();
-    sg.n_steps = (uint32_t)C;
-    DevError *derr = ctx->err.as<DevError>();
-
-    MZH_HIP(hipEventRecord(ctx->ev[0], st));
-    // stage 1: AABBs
-    MZH_HIP(hipEventRecord(ctx->ev[6], st));
-    MZH_HIP(launch_aabbs(pj, sg, conj_thresh, mode, ctx->aabbs.as<float>(), ctx->kbke.as<uint2>(),
-                         ctx->gbox.as<uint32_t>(), derr, st));
-    MZH_HIP(hipEventRecord(ctx->ev[7], st));
-    MZH_HIP(launch_gbox_finalize(ctx->gbox.as<uint32_t>(), ctx->aabbs.as<float>(), N, (uint32_t)C, derr, st));
-    MZH_HIP(hipEventRecord(ctx->ev[1], st));
-    // stage 2: Morton codes, sort, gather
-    MZH_HIP(launch_morton_sort(ctx->aabbs.as<float>(), ctx->mcodes.as<uint64_t>(), ctx->keys_tmp.as<uint64_t>(),
-                               ctx->vals_tmp.as<uint32_t>(), ctx->srt_mcodes.as<uint64_t>(),
-                               ctx->srt_idx.as<uint32_t>(), ctx->srt_aabbs.as<float>(), ctx->n_eff.as<uint32_t>(),
-                               ctx->sort_todo.as<uint32_t>(), N, (uint32_t)C, st));
-    MZH_HIP(hipEventRecord(ctx->ev[2], st));
-    // stage 3: BVH
-    MZH_HIP(launch_bvh(ctx->srt_mcodes.as<uint64_t>(), ctx->srt_aabbs.as<float>(), ctx->n_eff.as<uint32_t>(),
-                       ctx->trav.as<TravNode>(), ctx->bvh_hdr.as<uint4>(), ctx->bvh_scr.as<uint8_t>(),
-                       ctx->tree_size.as<uint32_t>(), N, ctx->node_cap, (uint32_t)C, st));
-    MZH_HIP(hipEventRecord(ctx->ev[3], st));
-    ctx->n_pairs = 0;
-    ctx->n_conj = 0;
-    if (!skip_cd) {
-        // stage 4: broad phase
-        uint32_t n_ovf = 0;
-        for (int attempt = 0; attempt < 3; ++attempt) {
-            MZH_HIP(launch_bp_reset(ctx->cnt.as<uint32_t>(), ctx->head.as<uint32_t>(), ctx->ovf_count.as<uint32_t>(), N,
-                                    (uint32_t)C, st));
-            MZH_HIP(hipEventRecord(ctx->ev[10], st));
-            MZH_HIP(launch_bp_count(ctx->trav.as<TravNode>(), ctx->tree_size.as<uint32_t>(),
-                                    ctx->srt_aabbs.as<float>(), ctx->srt_mcodes.as<uint64_t>(),
-                                    ctx->srt_idx.as<uint32_t>(), ctx->n_eff.as<uint32_t>(), ctx->otypes.as<int32_t>(),
-                                    ctx->cnt.as<uint32_t>(), ctx->slots.as<uint32_t>(), ctx->ovf.as<uint2>(),
-                                    ctx->head.as<uint32_t>(), ctx->ovf_count.as<uint32_t>(), ctx->ovf_cap, N,
-                                    ctx->node_cap, (uint32_t)C, st));
-            MZH_HIP(hipEventRecord(ctx->ev[11], st));
-            MZH_HIP(launch_scan_counts(ctx->cnt.as<uint32_t>(), ctx->off.as<uint64_t>(),
-                                       ctx->step_total.as<uint64_t>(), N, (uint32_t)C, st));
-            MZH_HIP(launch_scan_steps(ctx->step_total.as<uint64_t>(), ctx->step_off.as<uint64_t>(),
-                                      ctx->grand.as<uint64_t>(), (uint32_t)C, st));
-            MZH_HIP(hipMemcpyAsync(&n_ovf, ctx->ovf_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-            MZH_HIP(hipStreamSynchronize(st));
-            if (n_ovf <= ctx->ovf_cap) break;
-            ctx->ovf_cap = n_ovf + n_ovf / 4 + 4096;
-            MZH_HIP(ctx->ovf.ensure(sizeof(uint2) * ctx->ovf_cap));
-        }
-        if (n_ovf > ctx->ovf_cap) throw StatusError{MZH_ERUNTIME, "broad-phase overflow list capacity exceeded"};
-        uint64_t total = 0;
-        MZH_HIP(hipMemcpy(&total, ctx->grand.p, sizeof(uint64_t), hipMemcpyDeviceToHost));
-        ctx->n_pairs = total;
-        MZH_HIP(ctx->pairs.ensure(sizeof(mzh_aabb_collision) * (total ? total : 1)));
-        MZH_HIP(launch_bp_place(ctx->cnt.as<uint32_t>(), ctx->slots.as<uint32_t>(), ctx->ovf.as<uint2>(),
-                                ctx->head.as<uint32_t>(), ctx->ovf_cap, ctx->off.as<uint64_t>(),
-                                ctx->step_off.as<uint64_t>(), ctx->pairs.as<mzh_aabb_collision>(), N, (uint32_t)C, st));
-        MZH_HIP(hipEventRecord(ctx->ev[4], st));
-        // stage 5: narrow phase, in passes over at most pass_pairs broad-phase
-        // pairs (64 Mi, or MIZUBA_AMD_NARROW_PASS_PAIRS) so that the step-pair
-        // and candidate queues stay bounded; a pass whose queues or output
-        // overflow is re-run with larger buffers
-        NarrowArgs na;
-        na.pj = pj;
-        na.sg = sg;
-        na.kbke = ctx->kbke.as<const uint2>();
-        na.pairs = ctx->pairs.as<const mzh_aabb_collision>();
-        na.step_off = ctx->step_off.as<const uint64_t>();
-        na.conj_thresh = conj_thresh;
-        const uint32_t op1 = ctx->order + 1;
-        const double pi = 3.141592653589793238462643383279502884;
-        for (int m = 0; m <= kMaxOrder; ++m) na.cheby[m] = 0;
-        for (uint32_t m = 0; m < op1; ++m) na.cheby[m] = (std::cos((2. * m + 1.) / (2. * op1) * pi) + 1.) / 2.;
-        na.np_report = ctx->np_report.as<unsigned long long>();
-        na.conj_count = ctx->conj_count.as<unsigned long long>();
-        na.err = derr;
-        // bounds the queues to a few GB
-        const uint64_t pass_pairs = env_u64("MIZUBA_AMD_NARROW_PASS_PAIRS", 64ull << 20);
-        const uint64_t chunk = std::min<uint64_t>(total, pass_pairs);
-        if (ctx->conj_cap == 0) ctx->conj_cap = env_u64("MIZUBA_AMD_INIT_CONJ_CAP", 1u << 16);
-        // queued root-finding candidates: about a fifth of the pairs on LEO catalogs
-        const size_t cand_bytes = narrow_cand_bytes(ctx->order);
-        ctx->cand_cap = std::max<uint64_t>(ctx->cand_cap, env_u64("MIZUBA_AMD_INIT_CAND_CAP", chunk / 2 + 65536));
-        // overlapping trajectory-step pairs: about 1.5 per pair for T/8 steps, 2.2 for the T/20 steps
-        // of SGP4 polyjectories, with 3-minute conjunction steps (a larger need re-runs the pass)
-        ctx->steps_cap = std::max<uint64_t>(ctx->steps_cap, env_u64("MIZUBA_AMD_INIT_STEPS_CAP", 3 * chunk + 65536));
-        // candidates whose root isolation outgrows the private work list (rare)
-        ctx->slow_cap = std::max<uint64_t>(ctx->slow_cap, env_u64("MIZUBA_AMD_INIT_SLOW_CAP", 4096));
-        na.wl_fast_cap = (int)std::min<uint64_t>(env_u64("MIZUBA_AMD_WL_FAST_CAP", kNarrowWlFastCap), kNarrowWlFastCap);
-        MZH_HIP(ctx->cand_count.ensure(sizeof(unsigned long long)));
-        MZH_HIP(ctx->steps_count.ensure(sizeof(unsigned long long)));
-        MZH_HIP(ctx->slow_count.ensure(sizeof(unsigned long long)));
-        MZH_HIP(ctx->wl_slow.ensure(narrow_wl_slow_bytes(ctx->order)));
-        na.cand_count = ctx->cand_count.as<unsigned long long>();
-        na.steps_count = ctx->steps_count.as<unsigned long long>();
-        na.slow_count = ctx->slow_count.as<unsigned long long>();
-        na.wl_slow = ctx->wl_slow.p;
-        MZH_HIP(ctx->conj_raw.grow(sizeof(mzh_conj) * ctx->conj_cap, 0, st));
-        MZH_HIP(ctx->conj_step.grow(sizeof(uint32_t) * ctx->conj_cap, 0, st));
-        MZH_HIP(hipMemsetAsync(ctx->conj_count.p, 0, sizeof(unsigned long long), st));
-        MZH_HIP(hipMemsetAsync(ctx->np_report.p, 0, sizeof(unsigned long long) * MZH_NP_COUNT, st));
-        // state after the last completed pass: conjunction count, counters
-        unsigned long long done_conj = 0, done_rep[MZH_NP_COUNT] = {0, 0, 0, 0, 0, 0, 0};
-        MZH_HIP(hipEventRecord(ctx->ev[8], st));
-        for (uint64_t pb = 0; pb < total; pb += chunk) {
-            na.pair_begin = pb;
-            na.pair_end = std::min<uint64_t>(total, pb + chunk);
-            bool ok = false;
-            for (int attempt = 0; attempt < 8 && !ok; ++attempt) {
-                MZH_HIP(ctx->conj_raw.grow(sizeof(mzh_conj) * ctx->conj_cap, sizeof(mzh_conj) * done_conj, st));
-                MZH_HIP(ctx->conj_step.grow(sizeof(uint32_t) * ctx->conj_cap, sizeof(uint32_t) * done_conj, st));
-                MZH_HIP(ctx->cand.ensure(cand_bytes * ctx->cand_cap));
-                MZH_HIP(ctx->steps.ensure(sizeof(StepPair) * ctx->steps_cap));
-                MZH_HIP(ctx->slow.ensure(sizeof(uint64_t) * ctx->slow_cap));
-                if (attempt > 0) {
-                    // undo the overflowed pass
-                    MZH_HIP(hipMemcpyAsync(ctx->conj_count.p, &done_conj, sizeof(done_conj), hipMemcpyHostToDevice,
-                                           st));
-                    MZH_HIP(hipMemcpyAsync(ctx->np_report.p, done_rep, sizeof(done_rep), hipMemcpyHostToDevice, st));
-                }
-                MZH_HIP(hipMemsetAsync(ctx->cand_count.p, 0, sizeof(unsigned long long), st));
-                MZH_HIP(hipMemsetAsync(ctx->steps_count.p, 0, sizeof(unsigned long long), st));
-                MZH_HIP(hipMemsetAsync(ctx->slow_count.p, 0, sizeof(unsigned long long), st));
-                na.slow = ctx->slow.as<uint64_t>();
-                na.slow_cap = ctx->slow_cap;
-                na.steps = ctx->steps.p;
-                na.steps_cap = ctx->steps_cap;
-                na.conj = ctx->conj_raw.as<mzh_conj>();
-                na.conj_step = ctx->conj_step.as<uint32_t>();
-                na.conj_cap = ctx->conj_cap;
-                na.cand = ctx->cand.p;
-                na.cand_cap = ctx->cand_cap;
-                MZH_HIP(launch_narrow(na, st));
-                unsigned long long cnts[4] = {0, 0, 0, 0}, rep[MZH_NP_COUNT];
-                MZH_HIP(hipMemcpyAsync(&cnts[0], ctx->conj_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                       st));
-                MZH_HIP(hipMemcpyAsync(&cnts[1], ctx->cand_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                       st));
-                MZH_HIP(hipMemcpyAsync(&cnts[2], ctx->steps_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                       st));
-                MZH_HIP(hipMemcpyAsync(&cnts[3], ctx->slow_count.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                       st));
-                MZH_HIP(hipMemcpyAsync(rep, ctx->np_report.p, sizeof(rep), hipMemcpyDeviceToHost, st));
-                MZH_HIP(hipStreamSynchronize(st));
-                const bool conj_ok = cnts[0] <= ctx->conj_cap, cand_ok = cnts[1] <= ctx->cand_cap,
-                           steps_ok = cnts[2] <= ctx->steps_cap, slow_ok = cnts[3] <= ctx->slow_cap;
-                ok = conj_ok && cand_ok && steps_ok && slow_ok;
-                if (ok) {
-                    done_conj = cnts[0];
-                    std::memcpy(done_rep, rep, sizeof(rep));
-                    break;
-                }
-                if (!conj_ok) ctx->conj_cap = cnts[0] + cnts[0] / 4 + 1024;
-                if (!cand_ok) ctx->cand_cap = cnts[1] + cnts[1] / 4 + 1024;
-                if (!steps_ok) ctx->steps_cap = cnts[2] + cnts[2] / 4 + 1024;
-                if (!slow_ok) ctx->slow_cap = cnts[3] + cnts[3] / 4 + 1024;
-            }
-            if (!ok) throw StatusError{MZH_ERUNTIME, "narrow-phase buffers overflowed after resizing"};
-        }
-        MZH_HIP(hipEventRecord(ctx->ev[9], st));
-        ctx->n_conj = done_conj;
-        const uint64_t M = ctx->n_conj;
-        if (M >= 0xffffffffull) throw StatusError{MZH_ERUNTIME, "Too many conjunctions for one screening"};
-        MZH_HIP(ctx->conj_sorted.ensure(sizeof(mzh_conj) * (M ? M : 1)));
-        MZH_HIP(ctx->conj_keys.ensure(2u * sizeof(ConjKey) * (M ? M : 1)));
-        MZH_HIP(launch_conj_sort(ctx->conj_raw.as<mzh_conj>(), ctx->conj_step.as<uint32_t>(), M,
-                                 ctx->conj_keys.as<ConjKey>(), ctx->conj_keys.as<ConjKey>() + M,
-                                 ctx->conj_sorted.as<mzh_conj>(), st));
-    } else {
-        MZH_HIP(hipMemsetAsync(ctx->step_off.p, 0, sizeof(uint64_t) * (C + 1), st));
-        MZH_HIP(hipEventRecord(ctx->ev[4], st));
-        MZH_HIP(hipEventRecord(ctx->ev[8], st));
-        MZH_HIP(hipEventRecord(ctx->ev[9], st));
-    }
-    MZH_HIP(hipEventRecord(ctx->ev[5], st));
-    MZH_HIP(hipStreamSynchronize(st));
-
-    DevError herr;
-    MZH_HIP(hipMemcpy(&herr, ctx->err.p, sizeof(DevError), hipMemcpyDeviceToHost));
-    if (herr.code != kErrNone) throw StatusError{MZH_EINVAL, dev_error_message(herr)};
-
-    float ms[5];
-    for (int k = 0; k < 5; ++k) MZH_HIP(hipEventElapsedTime(&ms[k], ctx->ev[k], ctx->ev[k + 1]));
-    for (int k = 0; k < 5; ++k) ctx->stage_s[k] = ms[k] * 1e-3;
-    float tot = 0;
-    MZH_HIP(hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[5]));
-    ctx->stage_s[5] = tot * 1e-3;
-    float ka = 0, kn = 0;
-    MZH_HIP(hipEventElapsedTime(&ka, ctx->ev[6], ctx->ev[7]));
-    MZH_HIP(hipEventElapsedTime(&kn, ctx->ev[8], ctx->ev[9]));
-    ctx->stage_s[6] = ka * 1e-3;
-    ctx->stage_s[7] = kn * 1e-3;
-    float kb = 0;
-    if (!skip_cd) MZH_HIP(hipEventElapsedTime(&kb, ctx->ev[10], ctx->ev[11]));
-    ctx->stage_s[8] = kb * 1e-3;
-    unsigned long long rep[MZH_NP_COUNT];
-    MZH_HIP(hipMemcpy(rep, ctx->np_report.p, sizeof(rep), hipMemcpyDeviceToHost));
-    for (int k = 0; k < MZH_NP_COUNT; ++k) ctx->h_np[k] = rep[k];
-
-    ctx->h_tree_size.resize(C);
-    MZH_HIP(hipMemcpy(ctx->h_tree_size.data(), ctx->tree_size.p, sizeof(uint32_t) * C, hipMemcpyDeviceToHost));
-    for (uint64_t k = 0; k < C; ++k) {
-        if (ctx->h_tree_size[k] > ctx->node_cap) throw StatusError{MZH_ERUNTIME, "BVH node capacity exceeded"};
-    }
-    ctx->h_step_off.resize(C + 1);
-    MZH_HIP(hipMemcpy(ctx->h_step_off.data(), ctx->step_off.p, sizeof(uint64_t) * (C + 1), hipMemcpyDeviceToHost));
-    ctx->screened = true;
+    set_plan(ctx, plan_screen(ctx->n_objs, ctx->maxT, conj_thresh, conj_det_interval, otypes_in, mode), conj_thresh,
+             conj_det_interval, mode);
+    if (!ctx->solo) ctx->solo.reset(batch_new(ctx->stream));
+    int status = MZH_OK;
+    batch_run(ctx->solo.get(), {ctx}, false, false, &status);
+    if (status != MZH_OK) throw StatusError{status, ctx->last_error};
 }
 
 void require_screened(const mzh_ctx *ctx)
@@ -757,7 +350,6 @@ int mzh_create(int device, mzh_ctx **out)
         (void)scratch_address(device, &scr);
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    for (int k = 0; k < 12 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev[k]);
     for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev_eval[k]);
     for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&ctx->ev_sgp4[k]);
     if (e != hipSuccess) {
@@ -774,8 +366,6 @@ int mzh_destroy(mzh_ctx *ctx)
     if (!ctx) return MZH_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto &e : ctx->ev)
-        if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->ev_eval)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->ev_sgp4)
@@ -822,6 +412,7 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         ctx->has_pj = false;
         ctx->screened = false;
         own_pj_bufs(ctx);
+        ++ctx->generation;
         MZH_HIP(ctx->pjb->traj.ensure(sizeof(double) * (traj_len ? traj_len : 1)));
         MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * (time_len ? time_len : 1)));
         MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
@@ -1107,6 +698,7 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     ctx->has_pj = false;
     ctx->screened = false;
     own_pj_bufs(ctx);
+    ++ctx->generation;
     MZH_HIP(ctx->pjb->traj.ensure(sizeof(double) * steps_total * op1 * 7u));
     MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * time_len));
     MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * op1 * steps_total));
@@ -1470,35 +1062,206 @@ int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const
     return guarded(ctx, [&] { run_screen(ctx, conj_thresh, conj_det_interval, otypes, mode); });
 }
 
+} // extern "C"
+
+// A batch handle: the problems, their screening arguments and the batch that
+// screens them (with its captured graph and launch shapes).
+struct mzh_batch {
+    std::vector<mzh_ctx *> ctxs;
+    std::vector<double> thresh, interval;
+    std::vector<std::vector<int32_t>> ot;
+    std::vector<char> has_ot;
+    std::vector<uint64_t> gen_after_run;
+    int mode = MZH_MODE_FP64;
+    int flags = 0;
+    bool tuned = false;
+    std::unique_ptr<ScreenBatch, BatchDeleter> b;
+};
+
+namespace
+{
+
+// The arguments of a batch: contexts distinct, non-null, on one device, each
+// with a polyjectory. Returns MZH_OK or MZH_EINVAL with g_global_error set.
+int check_batch_args(mzh_ctx *const *ctxs, uint64_t n, const double *conj_thresh, const double *conj_det_interval)
+{
+    if (!ctxs || !conj_thresh || !conj_det_interval) {
+        g_global_error = "Null pointer passed to a batched screening";
+        return MZH_EINVAL;
+    }
+    std::vector<const mzh_ctx *> seen(ctxs, ctxs + n);
+    std::sort(seen.begin(), seen.end());
+    if (!seen.front() || std::adjacent_find(seen.begin(), seen.end()) != seen.end()) {
+        g_global_error = "The contexts of a batched screening must be distinct and non-null";
+        return MZH_EINVAL;
+    }
+    for (uint64_t k = 0; k < n; ++k)
+        if (ctxs[k]->device != ctxs[0]->device) {
+            g_global_error = "The contexts of a batched screening must be on one device";
+            return MZH_EINVAL;
+        }
+    return MZH_OK;
+}
+
+// Validate and set the plan of problem k; its status (message in the context).
+int plan_one(mzh_ctx *ctx, double thresh, double interval, const int32_t *ot, int mode)
+{
+    try {
+        if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
+        set_plan(ctx, plan_screen(ctx->n_objs, ctx->maxT, thresh, interval, ot, mode), thresh, interval, mode);
+        return MZH_OK;
+    } catch (const StatusError &e) {
+        return fail(ctx, e.code, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, MZH_ENOMEM, "host memory allocation failed");
+    }
+}
+
+// Screen the planned problems among ctxs (status[k] == MZH_OK) with batch b.
+void run_planned(ScreenBatch *b, mzh_ctx *const *ctxs, uint64_t n, bool graph, bool tune, int *status)
+{
+    std::vector<mzh_ctx *> todo;
+    std::vector<uint64_t> where;
+    for (uint64_t k = 0; k < n; ++k)
+        if (status[k] == MZH_OK) {
+            todo.push_back(ctxs[k]);
+            where.push_back(k);
+        }
+    if (todo.empty()) return;
+    std::vector<int> st(todo.size(), MZH_OK);
+    try {
+        MZH_HIP(hipSetDevice(todo[0]->device));
+        batch_run(b, todo, graph, tune, st.data());
+    } catch (const StatusError &e) {
+        for (size_t k = 0; k < todo.size(); ++k) st[k] = fail(todo[k], e.code, e.msg);
+    } catch (const std::bad_alloc &) {
+        for (size_t k = 0; k < todo.size(); ++k) st[k] = fail(todo[k], MZH_ENOMEM, "host memory allocation failed");
+    }
+    for (size_t k = 0; k < todo.size(); ++k) status[where[k]] = st[k];
+}
+
+} // namespace
+
+extern "C" {
+
 int mzh_screen_batch(mzh_ctx *const *ctxs, uint64_t n, const double *conj_thresh, const double *conj_det_interval,
                      const int32_t *const *otypes, int mode, int *status)
 {
     if (n == 0) return MZH_OK;
-    if (!ctxs || !conj_thresh || !conj_det_interval || !status) return MZH_EINVAL;
-    {
-        std::vector<const mzh_ctx *> seen(ctxs, ctxs + n);
-        std::sort(seen.begin(), seen.end());
-        if (!seen.front() || std::adjacent_find(seen.begin(), seen.end()) != seen.end()) return MZH_EINVAL;
-    }
-    // One host thread per problem in flight: each drives its own context's
-    // stream, including the two synchronisations that size the pair and
-    // conjunction buffers, so the problems overlap on the device.
-    std::atomic<uint64_t> next{0};
-    auto worker = [&] {
-        for (uint64_t k = next.fetch_add(1); k < n; k = next.fetch_add(1))
-            status[k] = mzh_screen(ctxs[k], conj_thresh[k], conj_det_interval[k], otypes ? otypes[k] : nullptr, mode);
-    };
-    const uint64_t nthreads = std::min<uint64_t>(n, 8);
-    std::vector<std::thread> pool;
+    if (!status) return MZH_EINVAL;
+    const int chk = check_batch_args(ctxs, n, conj_thresh, conj_det_interval);
+    if (chk != MZH_OK) return chk;
+    for (uint64_t k = 0; k < n; ++k)
+        status[k] = plan_one(ctxs[k], conj_thresh[k], conj_det_interval[k], otypes ? otypes[k] : nullptr, mode);
+    // every valid problem in one batch on the first context's stream
+    std::unique_ptr<ScreenBatch, BatchDeleter> b;
     try {
-        for (uint64_t t = 1; t < nthreads; ++t) pool.emplace_back(worker);
-    } catch (...) {
-        // fewer threads: the remaining problems are taken by those that started
+        MZH_HIP(hipSetDevice(ctxs[0]->device));
+        b.reset(batch_new(ctxs[0]->stream));
+    } catch (const StatusError &e) {
+        for (uint64_t k = 0; k < n; ++k)
+            if (status[k] == MZH_OK) status[k] = fail(ctxs[k], e.code, e.msg);
     }
-    worker();
-    for (auto &th : pool) th.join();
+    if (b) run_planned(b.get(), ctxs, n, false, false, status);
     for (uint64_t k = 0; k < n; ++k)
         if (status[k] != MZH_OK) return status[k];
+    return MZH_OK;
+}
+
+int mzh_batch_create(mzh_ctx *const *ctxs, uint64_t n, const double *conj_thresh, const double *conj_det_interval,
+                     const int32_t *const *otypes, int mode, int flags, mzh_batch **out)
+{
+    if (!out) return MZH_EINVAL;
+    *out = nullptr;
+    if (n == 0) {
+        g_global_error = "A batch needs at least one problem";
+        return MZH_EINVAL;
+    }
+    const int chk = check_batch_args(ctxs, n, conj_thresh, conj_det_interval);
+    if (chk != MZH_OK) return chk;
+    if (mode != MZH_MODE_FP64 && mode != MZH_MODE_MIXED) {
+        g_global_error = "Invalid screening mode " + std::to_string(mode);
+        return MZH_EINVAL;
+    }
+    try {
+        std::unique_ptr<mzh_batch> bt(new mzh_batch());
+        bt->ctxs.assign(ctxs, ctxs + n);
+        bt->thresh.assign(conj_thresh, conj_thresh + n);
+        bt->interval.assign(conj_det_interval, conj_det_interval + n);
+        bt->ot.resize(n);
+        bt->has_ot.assign(n, 0);
+        bt->gen_after_run.assign(n, ~0ull);
+        for (uint64_t k = 0; k < n; ++k) {
+            if (!ctxs[k]->has_pj) {
+                g_global_error = "No polyjectory has been uploaded to the context of problem " + std::to_string(k);
+                return MZH_EINVAL;
+            }
+            const int32_t *ot = otypes ? otypes[k] : nullptr;
+            if (ot) {
+                bt->ot[k].assign(ot, ot + ctxs[k]->n_objs);
+                bt->has_ot[k] = 1;
+            }
+            // the arguments are checked now, as conjunctions() would
+            (void)plan_screen(ctxs[k]->n_objs, ctxs[k]->maxT, conj_thresh[k], conj_det_interval[k], ot, mode);
+        }
+        bt->mode = mode;
+        bt->flags = flags;
+        MZH_HIP(hipSetDevice(ctxs[0]->device));
+        bt->b.reset(batch_new(ctxs[0]->stream));
+        *out = bt.release();
+        return MZH_OK;
+    } catch (const StatusError &e) {
+        g_global_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_global_error = "host memory allocation failed";
+        return MZH_ENOMEM;
+    }
+}
+
+int mzh_batch_run(mzh_batch *bt, int *status)
+{
+    if (!bt || !status) return MZH_EINVAL;
+    const uint64_t n = bt->ctxs.size();
+    for (uint64_t k = 0; k < n; ++k) {
+        mzh_ctx *ctx = bt->ctxs[k];
+        status[k] = MZH_OK;
+        // plan again only when the context was touched since this batch's run
+        if (ctx->generation != bt->gen_after_run[k] || ctx->last_batch != bt->b.get())
+            status[k] = plan_one(ctx, bt->thresh[k], bt->interval[k], bt->has_ot[k] ? bt->ot[k].data() : nullptr,
+                                 bt->mode);
+    }
+    const bool tune = (bt->flags & MZH_BATCH_TUNE) && !bt->tuned;
+    run_planned(bt->b.get(), bt->ctxs.data(), n, (bt->flags & MZH_BATCH_GRAPH) != 0, tune, status);
+    bt->tuned = bt->tuned || tune;
+    for (uint64_t k = 0; k < n; ++k) bt->gen_after_run[k] = bt->ctxs[k]->generation;
+    for (uint64_t k = 0; k < n; ++k)
+        if (status[k] != MZH_OK) return status[k];
+    return MZH_OK;
+}
+
+int mzh_batch_info(const mzh_batch *bt, uint64_t *replays, double *seconds, uint32_t shape[5])
+{
+    if (!bt) return MZH_EINVAL;
+    if (replays) *replays = batch_replays(bt->b.get());
+    if (seconds) *seconds = batch_seconds(bt->b.get());
+    if (shape) {
+        const LaunchShape ls = batch_shape(bt->b.get());
+        shape[0] = ls.bucket_bits;
+        shape[1] = ls.sort_wgs;
+        shape[2] = ls.bvh_wgs;
+        shape[3] = ls.bp_block;
+        shape[4] = ls.roots_wgs;
+    }
+    return MZH_OK;
+}
+
+int mzh_batch_destroy(mzh_batch *bt)
+{
+    if (!bt) return MZH_OK;
+    if (!bt->ctxs.empty()) (void)hipSetDevice(bt->ctxs[0]->device);
+    if (!bt->ctxs.empty()) (void)hipStreamSynchronize(bt->ctxs[0]->stream);
+    delete bt;
     return MZH_OK;
 }
 
@@ -1725,6 +1488,7 @@ int mzh_share_polyjectory(mzh_ctx *dst, const mzh_ctx *src)
         if (src->device != dst->device)
             throw StatusError{MZH_EINVAL, "Contexts on different devices cannot share a polyjectory"};
         dst->screened = false;
+        ++dst->generation;
         dst->pjb = src->pjb;
         dst->n_objs = src->n_objs;
         dst->order = src->order;
@@ -1746,14 +1510,15 @@ int mzh_release(mzh_ctx *ctx, int what)
 {
     return guarded(ctx, [&] {
         MZH_HIP(hipStreamSynchronize(ctx->stream));
+        ++ctx->generation;
         if (what & MZH_RELEASE_SCRATCH) {
             for (DevBuf *b : {&ctx->kbke, &ctx->keys_tmp, &ctx->vals_tmp, &ctx->n_eff, &ctx->sort_todo, &ctx->gbox,
                               &ctx->bvh_hdr, &ctx->bvh_scr, &ctx->otypes, &ctx->cnt, &ctx->head, &ctx->off,
-                              &ctx->step_total, &ctx->grand, &ctx->slots, &ctx->ovf, &ctx->ovf_count, &ctx->conj_raw,
-                              &ctx->conj_step, &ctx->conj_keys, &ctx->conj_count, &ctx->cand, &ctx->cand_count,
-                              &ctx->steps, &ctx->steps_count, &ctx->slow, &ctx->slow_count, &ctx->wl_slow,
-                              &ctx->ev_tm, &ctx->ev_sel, &ctx->ev_out, &ctx->ev_bad})
+                              &ctx->step_total, &ctx->slots, &ctx->ovf, &ctx->ovf_count, &ctx->ev_tm, &ctx->ev_sel,
+                              &ctx->ev_out, &ctx->ev_bad})
                 b->release();
+            // the narrow phase's queues of the context's own screenings
+            ctx->solo.reset();
         }
         if (what & MZH_RELEASE_RESULTS) {
             for (DevBuf *b : {&ctx->aabbs, &ctx->srt_aabbs, &ctx->mcodes, &ctx->srt_mcodes, &ctx->srt_idx, &ctx->nodes,

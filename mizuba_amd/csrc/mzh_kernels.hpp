@@ -16,74 +16,160 @@ namespace mzh
 hipError_t launch_transpose_cf4(const double *traj, double *cf4, uint64_t n_steps_total, uint32_t order,
                                 hipStream_t stream);
 
-// Stage 1: per-(step, object) AABBs plus the per-step global box.
-// mode: MZH_MODE_FP64 or MZH_MODE_MIXED.
-// Also stores, per (step, object), the range [kb, ke) of the object's
-// trajectory steps overlapping the conjunction step, for the narrow phase.
-hipError_t launch_aabbs(const PjView &pj, const StepGrid &sg, double conj_thresh, int mode, float *aabbs,
-                        uint2 *kbke, uint32_t *gbox_ord, DevError *err, hipStream_t stream);
-hipError_t launch_gbox_finalize(const uint32_t *gbox_ord, float *aabbs, uint32_t n_objs, uint32_t n_steps,
-                                DevError *err, hipStream_t stream);
+// ---- a batch of screening problems ------------------------------------------
+// Conjunction steps handled by one lane of the AABB kernel.
+constexpr uint32_t kAabbStepGroup = 16;
 
-// Stage 2: Morton codes, sort by (no data, code, index), gather of the boxes
-// in sorted order; n_eff = number of objects with data per step. todo
-// (one word per step) marks the steps left to the radix sort fallback.
-hipError_t launch_morton_sort(const float *aabbs, uint64_t *mcodes, uint64_t *keys_tmp, uint32_t *vals_tmp,
-                              uint64_t *srt_mcodes, uint32_t *srt_idx, float *srt_aabbs, uint32_t *n_eff,
-                              uint32_t *todo, uint32_t n_objs, uint32_t n_steps, hipStream_t stream);
+// Every stage runs as one launch over all problems of a batch (a single
+// screening is a batch of one). Each problem is described to the kernels by a
+// Prob record in device memory; a launch table (the exclusive prefix of the
+// blocks, or work items, each problem contributes) maps a block to its
+// problem and its block within the problem.
 
-// Stage 3: BVH, one workgroup per step. nodes has node_cap entries per step.
-// Also writes the depth-first traversal copy `trav` (node_cap per step);
-// nolc_buf, ps_buf and pos_buf are node_cap-sized scratch per step.
-// Bytes of per-step BVH scratch in HBM (used when it does not fit in LDS).
-size_t bvh_scratch_stride(uint32_t n_objs);
-// hdr: node_cap uint4 headers per step, scratch of the build.
-hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, TravNode *trav,
-                      uint4 *hdr, uint8_t *scr_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap,
-                      uint32_t n_steps, hipStream_t stream);
-// The reference's bvh_node records of every step's tree, from the device layout.
-hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mzh_bvh_node *nodes, uint64_t node_cap,
-                             uint32_t n_steps, hipStream_t stream);
+// Overflow flags of a problem (Prob::flags), set on the device when a
+// capacity chosen before the launch is too small; the screening is then run
+// again with the sizes the kernels counted.
+constexpr uint32_t kFlagPairs = 1u;  // more pairs than pair_cap
+constexpr uint32_t kFlagOvf = 2u;    // broad-phase overflow pool full
+constexpr uint32_t kFlagConj = 4u;   // more conjunctions than conj_out_cap
 
-// Stage 4: broad phase. launch_bp_count finds every pair once (each query
-// walks the tree to the right of its own code run) and counts it for the
-// pair's lower original index i, keeping i's first 4 partners inline (slots: 4
-// per object and step) and pushing the others on i's list in the overflow
-// pool (ovf entries {partner, next}, head per object and step). After the
-// scans, launch_bp_place writes every object's pairs sorted by partner.
-// launch_bp_reset clears the counts, list heads and pool counter first.
-hipError_t launch_bp_reset(uint32_t *cnt, uint32_t *head, uint32_t *ovf_count, uint32_t n_objs, uint32_t n_steps,
-                           hipStream_t stream);
-hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
-                           const uint64_t *srt_mcodes, const uint32_t *srt_idx, const uint32_t *n_eff,
-                           const int32_t *otypes, uint32_t *cnt, uint32_t *slots, uint2 *ovf, uint32_t *head,
-                           uint32_t *ovf_count, uint32_t ovf_cap, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
-                           hipStream_t stream);
-hipError_t launch_scan_counts(const uint32_t *cnt, uint64_t *off, uint64_t *step_total, uint32_t n_objs,
-                              uint32_t n_steps, hipStream_t stream);
-hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uint64_t *grand_total, uint32_t n_steps,
-                             hipStream_t stream);
-hipError_t launch_bp_place(const uint32_t *cnt, const uint32_t *slots, const uint2 *ovf, const uint32_t *head,
-                           uint32_t ovf_cap, const uint64_t *off, const uint64_t *step_off, mzh_aabb_collision *pairs,
-                           uint32_t n_objs, uint32_t n_steps, hipStream_t stream);
-
-// Stage 5: narrow phase over the broad-phase pairs [pair_begin, pair_end) of
-// the concatenated pair list (all steps).
-struct NarrowArgs {
+struct Prob {
     PjView pj;
-    StepGrid sg;
-    const uint2 *kbke;  // [C][N] overlapping trajectory-step ranges from the AABB stage
-    const mzh_aabb_collision *pairs;
-    const uint64_t *step_off;  // [C + 1], exclusive prefix of the per-step pair counts
-    uint64_t pair_begin, pair_end;
-    double conj_thresh;
-    double cheby[kMaxOrder + 1];  // (cos((2i+1)pi/(2 op1)) + 1) / 2, i = 0..order
-    mzh_conj *conj;               // output records (capacity conj_cap)
-    uint32_t *conj_step;          // step index of each record
-    uint64_t conj_cap;
-    unsigned long long *conj_count;
+    const double *cd_begin, *cd_end;  // [C]
+    uint32_t C, N;
+    uint32_t step0;     // index of its first conjunction step among the batch's
+    int32_t mode;       // MZH_MODE_FP64 / MZH_MODE_MIXED
+    double radius;      // conj_thresh / 2
+    double thresh;
+    uint64_t node_cap;  // BVH nodes per step
+    uint64_t bvh_scr_stride;
+    // the problem's buffers (its context's)
+    float *aabbs, *srt_aabbs;  // [C][N + 1][2][4]
+    uint2 *kbke;               // [C][N]
+    uint32_t *gbox;            // [C][8] order-preserving images of the global box
+    uint64_t *mcodes, *srt_mcodes, *keys_tmp;
+    uint32_t *srt_idx, *vals_tmp, *n_eff, *sort_todo;
+    TravNode *trav;            // [C][node_cap]
+    uint4 *bvh_hdr;            // [C][node_cap]
+    uint8_t *bvh_scr;          // [C][bvh_scr_stride], when the build scratch is not in LDS
+    uint32_t *tree_size;       // [C]
+    const int32_t *otypes;     // [N]
+    uint32_t *cnt, *head, *slots;  // [C][N], [C][N], [C][N][4]
+    uint2 *ovf;                // overflow pool {partner, next}
+    uint32_t *ovf_count;
+    uint32_t ovf_cap;
+    uint64_t *off;             // [C][N] offset of each object's pairs in its step
+    uint64_t *step_total;      // [C]
+    uint64_t *step_off;        // [C + 1]: exclusive prefix of the steps' pairs; [C] = all pairs
+    mzh_aabb_collision *pairs; // pair_cap entries
+    uint64_t pair_cap;
+    mzh_conj *conj_out;        // the problem's conjunctions, sorted (conj_out_cap entries)
+    uint64_t conj_out_cap;
     unsigned long long *np_report;  // [MZH_NP_COUNT]
     DevError *err;
+    uint32_t *flags;           // kFlag*
+    uint64_t *counts;          // [4]: pairs, pool entries, conjunctions, first conjunction in the batch's list
+};
+
+// Work counters shared by the launches of one batch run (device memory),
+// cleared by launch_batch_init.
+struct BatchCounters {
+    uint32_t next_sort, next_bvh, next_bvh_hbm, pad;
+};
+
+// Exclusive prefix arrays (np + 1 entries, device memory) of one batch.
+struct BatchTables {
+    const uint32_t *steps;       // conjunction steps (Prob::step0)
+    const uint32_t *aabb_blk;    // blocks of the AABB kernel
+    const uint32_t *nc_blk;      // blocks of 256 (step, object) entries
+    const uint32_t *bp_blk;      // blocks of the broad-phase walk
+    const uint32_t *gather_blk;  // blocks of 256 (step, sorted position) entries of the radix-sort gather
+    const uint2 *sort_items;     // (problem, step) work items of the sort, largest problems first
+    const uint2 *bvh_items;      // same, problems whose BVH scratch fits in LDS
+    const uint2 *bvh_hbm_items;  // same, the others
+    uint32_t n_sort_items, n_bvh_items, n_bvh_hbm_items;
+    uint32_t bvh_lds_bytes;      // dynamic LDS of the LDS build (largest problem of its list)
+};
+
+// Launch shapes that leave the results unchanged (chosen by the tuner).
+struct LaunchShape {
+    uint32_t bp_block = 256;       // broad-phase walk workgroup: 128, 256 or 512
+    uint32_t sort_wgs = 512;       // persistent sort workgroups
+    uint32_t bvh_wgs = 512;        // persistent BVH workgroups
+    uint32_t roots_wgs = 512;      // root-finding workgroups
+    uint32_t bucket_bits = 12;     // bucket-sort width: 10, 11 or 12 bits
+};
+
+// Clear the per-problem counters, flags and error words, the global boxes
+// and the batch's work counters. One thread per conjunction step.
+hipError_t launch_batch_init(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
+                             BatchCounters *bc, hipStream_t stream);
+// Stage 1: per-(step, object) AABBs plus the per-step global box; also
+// stores, per (step, object), the range [kb, ke) of the object's trajectory
+// steps overlapping the conjunction step, for the narrow phase. One launch
+// per polynomial order present in the batch: problems of another order
+// return at once.
+hipError_t launch_aabbs(uint32_t order, const Prob *P, uint32_t np, const BatchTables &t, uint32_t blk_begin,
+                        uint32_t n_blocks, hipStream_t stream);
+hipError_t launch_gbox_finalize(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
+                                hipStream_t stream);
+// Stage 2: Morton codes, sort by (no data, code, index), gather of the boxes
+// in sorted order (persistent workgroups over the sort items); steps with a
+// bucket too large for the bucket sort go to the radix sort and gather.
+hipError_t launch_morton_sort(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
+                              uint32_t n_gather_blocks, BatchCounters *bc, const LaunchShape &ls, hipStream_t stream);
+// Bytes of per-step BVH scratch (LDS when it fits kBvhLdsMax, else HBM).
+size_t bvh_scratch_bytes_host(uint32_t n_objs);
+size_t bvh_scratch_stride(uint32_t n_objs);
+constexpr size_t kBvhLdsMaxHost = 64u * 1024u;
+// Stage 3: BVH (persistent workgroups over the BVH items).
+hipError_t launch_bvh(const Prob *P, uint32_t np, const BatchTables &t, BatchCounters *bc, const LaunchShape &ls,
+                      hipStream_t stream);
+// The reference's bvh_node records of every step's tree, from the device
+// layout (one problem).
+hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mzh_bvh_node *nodes, uint64_t node_cap,
+                             uint32_t n_steps, hipStream_t stream);
+// Stage 4: broad phase. launch_bp_count finds every pair once (each query
+// walks the tree to the right of its own code run) and counts it for the
+// pair's lower original index i, keeping i's first 4 partners inline and
+// pushing the others on i's list in the overflow pool. After the scans
+// (per-step offsets, then the step prefix whose last entry is the problem's
+// pair count), launch_bp_place writes every object's pairs sorted by partner.
+// launch_bp_reset clears the counts and list heads first.
+hipError_t launch_bp_reset(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_nc_blocks,
+                           hipStream_t stream);
+hipError_t launch_bp_count(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_bp_blocks,
+                           const LaunchShape &ls, hipStream_t stream);
+hipError_t launch_scan_counts(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
+                              hipStream_t stream);
+hipError_t launch_scan_steps(const Prob *P, uint32_t np, hipStream_t stream);
+// Per problem: pair and pool counts into counts[0..1], kFlagPairs /
+// kFlagOvf when they exceed the capacities; pair_pre (np + 1 entries) =
+// exclusive prefix of the problems' placed pairs, the narrow phase's global
+// pair index.
+hipError_t launch_pair_prefix(const Prob *P, uint32_t np, uint64_t *pair_pre, hipStream_t stream);
+hipError_t launch_bp_place(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_nc_blocks,
+                           hipStream_t stream);
+
+// Stage 5: narrow phase over the placed pairs of problems [p_first, p_end)
+// (one order group: all of polynomial order `order`), pairs [pair_begin,
+// pair_end) counted from the group's first, of the batch's global pair index
+// (problem p's pairs are [pair_pre[p], pair_pre[p + 1]); the range is clamped
+// on the device to the group's pairs). Conjunction records of
+// all problems go to one list, with the batch-wide index of their step.
+struct NarrowArgs {
+    const Prob *P;
+    uint32_t np;
+    uint32_t order;
+    const uint64_t *pair_pre;     // [np + 1]
+    uint32_t p_first, p_end;      // the problems of the pass's order group
+    uint64_t pair_begin, pair_end;  // relative to the group's first pair
+    uint32_t roots_wgs;           // root-finding workgroups
+    double cheby[kMaxOrder + 1];  // (cos((2i+1)pi/(2 op1)) + 1) / 2, i = 0..order
+    mzh_conj *conj;               // output records (capacity conj_cap)
+    uint32_t *conj_step;          // batch-wide step index of each record (Prob::step0 + step)
+    uint64_t conj_cap;
+    unsigned long long *conj_count;
     // overlapping trajectory-step pairs of every broad-phase pair
     void *steps;                  // StepPair records
     uint64_t steps_cap;
@@ -100,11 +186,19 @@ struct NarrowArgs {
     uint64_t slow_cap;
     unsigned long long *slow_count;
     void *wl_slow;                // narrow_wl_slow_bytes(order)
+    // largest queue lengths of the passes (steps, cand, slow), for sizing
+    unsigned long long *peak;
 };
-// One overlapping pair of trajectory steps of a broad-phase pair.
+// One overlapping pair of trajectory steps of a broad-phase pair of problem p.
 struct StepPair {
-    uint32_t i, j, c, ki, kj;
+    uint32_t p, i, j, c, ki, kj;
 };
+// Narrow phase of the pairs [pair_begin, pair_end): enumeration, screen, root
+// finding (and its slow pass), then the pass's queue lengths folded into
+// peak and the queue counters cleared for the next pass.
+// launch_narrow_reset clears the problems' counters, the conjunction count,
+// the queue counters and peak before the first pass.
+hipError_t launch_narrow_reset(const NarrowArgs &a, hipStream_t stream);
 // Bytes of one queued root-finding candidate for a polynomial order.
 size_t narrow_cand_bytes(uint32_t order);
 // Root isolation work lists: an entry holds an interval and a polynomial; the
@@ -145,15 +239,21 @@ hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream);
 
 // Group conjunction records by step and sort each step by (tca, then every
 // field lexicographically): a merge sort of 32-byte keys, ping-ponging between
-// keys0 and keys1 (n entries each); the sorted records go to out.
+// keys0 and keys1 (n entries each). n_dev (may be null) holds the number of
+// records on the device, at most n (the launches are sized by n). Each
+// problem then takes its range of the sorted list (its steps are contiguous
+// in the batch's step index) into Prob::conj_out, with counts[2..3] = its
+// number of conjunctions and its first index, and kFlagConj where conj_out
+// is too small.
 struct ConjKey {
     uint32_t step, idx;
     uint64_t tca;  // bits of tca + 0.0
     uint64_t ij;   // i << 32 | j
     uint64_t pad;
 };
-hipError_t launch_conj_sort(const mzh_conj *in, const uint32_t *in_step, uint64_t n, ConjKey *keys0, ConjKey *keys1,
-                            mzh_conj *out, hipStream_t stream);
+hipError_t launch_conj_sort(const Prob *P, uint32_t np, const uint32_t *step_pre, const mzh_conj *in,
+                            const uint32_t *in_step, uint64_t n, const unsigned long long *n_dev, ConjKey *keys0,
+                            ConjKey *keys1, hipStream_t stream);
 
 // ---- SGP4 (near-earth model) ------------------------------------------------
 // Deep-space (SDP4) terms of an element set with a period of 225 minutes or

@@ -45,6 +45,47 @@ constexpr int kWlCap = 48;
 constexpr int kWlSlowCap = 251;
 // find_minima results
 constexpr int kFmOk = 0, kFmError = 1, kFmListFull = 2;
+
+// The problem of global pair index g: the largest p with pre[p] <= g.
+__device__ inline uint32_t find_prob64(const uint64_t *__restrict__ pre, uint32_t np, uint64_t g)
+{
+    uint32_t lo = 0, n = np;
+    while (n > 1) {
+        const uint32_t h = n / 2;
+        if (pre[lo + h] <= g) {
+            lo += h;
+            n -= h;
+        } else {
+            n = h;
+        }
+    }
+    return lo;
+}
+
+// Add v[0..K) of every active lane to np_report[k0..k0+K) of the lane's problem
+// p. The lanes of a wave nearly always belong to one problem: the wave's
+// sums are then added by one lane. Call from all lanes of the wave.
+template <int K>
+__device__ inline void add_prob_report(const Prob *__restrict__ P, uint32_t p, bool active, int k0,
+                                       const unsigned long long *v)
+{
+    const uint64_t act = __ballot(active);
+    if (act == 0) return;
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const uint32_t p0 = __shfl(p, leader, 64);
+    const bool same = __ballot(active && p != p0) == 0;
+    if (same) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const unsigned long long s = wave_reduce_add_u64(active ? v[k] : 0ull);
+            if ((int)__lane_id() == leader && s) atomicAdd(&P[p0].np_report[k0 + k], s);
+        }
+    } else if (active) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (v[k]) atomicAdd(&P[p].np_report[k0 + k], v[k]);
+    }
+}
 // capacity of the per-interval root list (isolated roots plus roots found
 // exactly at a work-list interval's lower end)
 template <int ORDER>
@@ -542,7 +583,7 @@ __device__ inline void vm_interp(const double *alpha, double *c)
 }
 
 template <int ORDER>
-__device__ inline void emit_conj(const NarrowArgs &a, uint32_t c, uint32_t i, uint32_t j, double lb_rf, double t,
+__device__ inline void emit_conj(const NarrowArgs &a, uint32_t gstep, uint32_t i, uint32_t j, double lb_rf, double t,
                                  const double *Pi, double di, const double *Pj, double dj)
 {
     constexpr int OP1 = ORDER + 1;
@@ -567,7 +608,7 @@ __device__ inline void emit_conj(const NarrowArgs &a, uint32_t c, uint32_t i, ui
     const unsigned long long slot = atomicAdd(a.conj_count, 1ull);
     if (slot < a.conj_cap) {
         a.conj[slot] = rec;
-        a.conj_step[slot] = c;
+        a.conj_step[slot] = gstep;
     }
 }
 
@@ -579,6 +620,7 @@ struct NpCand {
     double lb_rf, rf_int, di, dj;
     uint32_t c, i, j, flags;  // flags: bit 0 trajectory end reached, bit 1 trajectory begin
     uint64_t ki, kj;          // trajectory-step indices of i and j
+    uint32_t p, pad;          // problem
 };
 
 constexpr uint32_t kCandEnd = 1u, kCandBegin = 2u;
@@ -613,16 +655,23 @@ __device__ inline void translate1s(const double *__restrict__ src, double a, dou
 // appended to the step-pair list (wave-aggregated atomics).
 __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
 {
-    const uint64_t p = a.pair_begin + (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t g0 = a.pair_pre[a.p_first], g1 = a.pair_pre[a.p_end];
+    const uint64_t g = g0 + a.pair_begin + (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t g_end = g0 + a.pair_end < g1 ? g0 + a.pair_end : g1;
     uint64_t kbi = 0, kbj = 0, kei = 0, kej = 0;
-    uint32_t i = 0, j = 0, c = 0;
+    uint32_t i = 0, j = 0, c = 0, p = 0;
     const double *ti = nullptr, *tj = nullptr;
     double cb = 0, ce = 0;
-    if (p < a.pair_end) {
-        uint32_t lo = 0, n = a.sg.n_steps;
+    const bool live = g < g_end;
+    if (live) {
+        p = find_prob64(a.pair_pre, a.np, g);
+        const Prob &q = a.P[p];
+        const uint64_t l = g - a.pair_pre[p];
+        // the pair's conjunction step: the last c with step_off[c] <= l
+        uint32_t lo = 0, n = q.C;
         while (n > 0) {
             const uint32_t h = n / 2;
-            if (a.step_off[lo + h + 1] <= p) {
+            if (q.step_off[lo + h + 1] <= l) {
                 lo += h + 1;
                 n -= h + 1;
             } else {
@@ -630,14 +679,14 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
             }
         }
         c = lo;
-        i = a.pairs[p].i;
-        j = a.pairs[p].j;
-        cb = a.sg.cd_begin[c];
-        ce = a.sg.cd_end[c];
-        ti = a.pj.time + a.pj.time_off[i];
-        tj = a.pj.time + a.pj.time_off[j];
+        i = q.pairs[l].i;
+        j = q.pairs[l].j;
+        cb = q.cd_begin[c];
+        ce = q.cd_end[c];
+        ti = q.pj.time + q.pj.time_off[i];
+        tj = q.pj.time + q.pj.time_off[j];
         // overlapping trajectory steps of both objects, as found by the AABB stage
-        const uint2 ri = a.kbke[(size_t)c * a.pj.n_objs + i], rj = a.kbke[(size_t)c * a.pj.n_objs + j];
+        const uint2 ri = q.kbke[(size_t)c * q.N + i], rj = q.kbke[(size_t)c * q.N + j];
         kbi = ri.x;
         kei = ri.y;
         kbj = rj.x;
@@ -658,18 +707,20 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
     };
     uint32_t cnt = 0;
     walk([&](uint64_t, uint64_t) { ++cnt; });
+    {
+        const unsigned long long v = cnt;
+        add_prob_report<1>(a.P, p, live, 0, &v);
+    }
     __shared__ uint32_t s_w[17];
     __shared__ unsigned long long s_base;
     uint32_t total;
     const uint32_t ex = block_excl_scan_u32(cnt, s_w, total);
-    if (threadIdx.x == 0) {
-        s_base = total ? atomicAdd(a.steps_count, (unsigned long long)total) : 0ull;
-        if (total) atomicAdd(&a.np_report[0], (unsigned long long)total);
-    }
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(a.steps_count, (unsigned long long)total) : 0ull;
     __syncthreads();
     uint64_t slot = s_base + ex;
     walk([&](uint64_t ki, uint64_t kj) {
-        if (slot < a.steps_cap) static_cast<StepPair *>(a.steps)[slot] = StepPair{i, j, c, (uint32_t)ki, (uint32_t)kj};
+        if (slot < a.steps_cap)
+            static_cast<StepPair *>(a.steps)[slot] = StepPair{p, i, j, c, (uint32_t)ki, (uint32_t)kj};
         ++slot;
     });
 }
@@ -687,22 +738,26 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
     const uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     unsigned long long n_rej = 0;
     bool keep = false;
-    uint32_t flags = 0, i = 0, j = 0, c = 0;
+    uint32_t flags = 0, i = 0, j = 0, c = 0, p = 0;
     uint64_t ki = 0, kj = 0;
     double lb_rf = 0, rf_int = 0, di = 0, dj = 0;
     double d2[OP1];
-    if (q < nsp) {
+    const bool live = q < nsp;
+    if (live) {
         const StepPair sp = static_cast<const StepPair *>(a.steps)[q];
+        p = sp.p;
         i = sp.i;
         j = sp.j;
         c = sp.c;
         ki = sp.ki;
         kj = sp.kj;
-        const double cb = a.sg.cd_begin[c], ce = a.sg.cd_end[c];
-        const double thresh2 = a.conj_thresh * a.conj_thresh;
-        const uint64_t nsi = a.pj.nsteps[i], nsj = a.pj.nsteps[j];
-        const double *ti = a.pj.time + a.pj.time_off[i];
-        const double *tj = a.pj.time + a.pj.time_off[j];
+        const Prob &pb = a.P[p];
+        const PjView &pj = pb.pj;
+        const double cb = pb.cd_begin[c], ce = pb.cd_end[c];
+        const double thresh2 = pb.thresh * pb.thresh;
+        const uint64_t nsi = pj.nsteps[i], nsj = pj.nsteps[j];
+        const double *ti = pj.time + pj.time_off[i];
+        const double *tj = pj.time + pj.time_off[j];
         const double tsi = ti[ki], tei = ti[ki + 1], tsj = tj[kj], tej = tj[kj + 1];
         const double lb_i = cb > tsi ? cb : tsi, ub_i = ce < tei ? ce : tei;
         const double lb_j = cb > tsj ? cb : tsj, ub_j = ce < tej ? ce : tej;
@@ -713,13 +768,13 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
         rf_int = ub_rf - lb_rf;
         bool ok = true;
         if (!is_finite_d(di) || !is_finite_d(dj) || !is_finite_d(rf_int) || di < 0 || dj < 0 || rf_int < 0) {
-            report_error(a.err, kErrNpInterval, i, j, rf_int);
+            report_error(pb.err, kErrNpInterval, i, j, rf_int);
             ok = false;
         }
         if (ok) {
             // positions from the step-major copy: channel ch of a step at cf4[(s * 4 + ch) * OP1]
-            const double *Qi = a.pj.cf4 + (a.pj.step0[i] + ki) * 4u * OP1;
-            const double *Qj = a.pj.cf4 + (a.pj.step0[j] + kj) * 4u * OP1;
+            const double *Qi = pj.cf4 + (pj.step0[i] + ki) * 4u * OP1;
+            const double *Qj = pj.cf4 + (pj.step0[j] + kj) * 4u * OP1;
             double nodes[OP1];
 #pragma unroll
             for (int m = 0; m < OP1; ++m) nodes[OP1 - 1 - m] = a.cheby[m] * rf_int;
@@ -740,7 +795,7 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
             double elo, ehi;
             cs_enc0<ORDER>(d2, rf_int, elo, ehi);
             if (!is_finite_d(elo) || !is_finite_d(ehi)) {
-                report_error(a.err, kErrNpDist2, i, j, elo);
+                report_error(pb.err, kErrNpDist2, i, j, elo);
             } else if (elo < thresh2) {
                 flags = 0x80000000u;
                 if ((ki + 1 == nsi && tei == ub_rf) || (kj + 1 == nsj && tej == ub_rf)) flags |= kCandEnd;
@@ -751,17 +806,13 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
             }
         }
     }
+    add_prob_report<1>(a.P, p, live, 1, &n_rej);
     // block-aggregated append to the candidate queue
     __shared__ uint32_t s_w[17];
     __shared__ unsigned long long s_base;
-    __shared__ unsigned long long s_rw[4];
     uint32_t total;
     const uint32_t ex = block_excl_scan_u32(keep ? 1u : 0u, s_w, total);
-    const unsigned long long rej = block_reduce_add_u64(n_rej, s_rw);
-    if (threadIdx.x == 0) {
-        s_base = total ? atomicAdd(a.cand_count, (unsigned long long)total) : 0ull;
-        if (rej) atomicAdd(&a.np_report[1], rej);
-    }
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(a.cand_count, (unsigned long long)total) : 0ull;
     __syncthreads();
     if (keep) {
         const unsigned long long slot = s_base + ex;
@@ -780,6 +831,8 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
                     cd.flags = flags;
                     cd.ki = ki;
                     cd.kj = kj;
+                    cd.p = p;
+                    cd.pad = 0;
         }
     }
 }
@@ -788,20 +841,35 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
 // its boundary minima (:629-714), with the root-isolation work list wl of cap
 // entries. rep accumulates np_report[2..6]. Returns kFmListFull, before
 // anything is counted or emitted, when the work list is too small.
+// The candidate's problem is cd.p; rep accumulates for the problem rep_p and
+// is added to it (and restarted) when the candidate belongs to another one.
 template <int ORDER>
-__device__ inline int np_root_cand(const NarrowArgs &a, uint64_t q, unsigned long long *rep, WLEntry<ORDER> *wl,
-                                   int cap)
+__device__ inline int np_root_cand(const NarrowArgs &a, uint64_t q, unsigned long long *rep, uint32_t &rep_p,
+                                   WLEntry<ORDER> *wl, int cap)
 {
     constexpr int OP1 = ORDER + 1;
-    const double thresh2 = a.conj_thresh * a.conj_thresh;
     const NpCand<ORDER> &cd = static_cast<const NpCand<ORDER> *>(a.cand)[q];
+    const Prob &pb = a.P[cd.p];
+    if (cd.p != rep_p) {
+        if (rep_p != 0xffffffffu) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                if (rep[k]) atomicAdd(&a.P[rep_p].np_report[2 + k], rep[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rep[k] = 0;
+        rep_p = cd.p;
+    }
+    const double thresh2 = pb.thresh * pb.thresh;
+    DevError *err = pb.err;
     double d2[OP1];
 #pragma unroll
     for (int k = 0; k < OP1; ++k) d2[k] = cd.d2[k];
     const double rf_int = cd.rf_int, lb_rf = cd.lb_rf;
-    const uint32_t i = cd.i, j = cd.j, c = cd.c, flags = cd.flags;
-    const double *Pi = a.pj.traj + (a.pj.step0[i] + cd.ki) * (uint64_t)OP1 * 7u;
-    const double *Pj = a.pj.traj + (a.pj.step0[j] + cd.kj) * (uint64_t)OP1 * 7u;
+    const uint32_t i = cd.i, j = cd.j, flags = cd.flags;
+    const uint32_t gstep = pb.step0 + cd.c;
+    const double *Pi = pb.pj.traj + (pb.pj.step0[i] + cd.ki) * (uint64_t)OP1 * 7u;
+    const double *Pj = pb.pj.traj + (pb.pj.step0[j] + cd.kj) * (uint64_t)OP1 * 7u;
     double d2p[OP1];
 #pragma unroll
     for (int k = 0; k < ORDER; ++k) d2p[k] = (double)(k + 1) * d2[k + 1];
@@ -809,7 +877,7 @@ __device__ inline int np_root_cand(const NarrowArgs &a, uint64_t q, unsigned lon
     double roots[kMaxRoots<ORDER>];
     int nr = 0;
     bool fex = false;
-    const int fm = find_minima<ORDER>(d2p, rf_int, roots, nr, fex, i, j, a.err, wl, cap);
+    const int fm = find_minima<ORDER>(d2p, rf_int, roots, nr, fex, i, j, err, wl, cap);
     if (fm != kFmOk) return fm;
     ++rep[0];
     rep[1] += fex ? 1 : 0;
@@ -818,11 +886,11 @@ __device__ inline int np_root_cand(const NarrowArgs &a, uint64_t q, unsigned lon
     for (int r = 0; r < nr; ++r) {
         const double cd2 = horner<ORDER>(d2, roots[r]);
         if (!is_finite_d(cd2)) {
-            report_error(a.err, kErrNpConjDist2, i, j, cd2);
+            report_error(err, kErrNpConjDist2, i, j, cd2);
             return kFmError;
         }
         if (cd2 < thresh2) {
-            emit_conj<ORDER>(a, c, i, j, lb_rf, roots[r], Pi, cd.di, Pj, cd.dj);
+            emit_conj<ORDER>(a, gstep, i, j, lb_rf, roots[r], Pi, cd.di, Pj, cd.dj);
         } else {
             ++rep[4];
         }
@@ -831,44 +899,40 @@ __device__ inline int np_root_cand(const NarrowArgs &a, uint64_t q, unsigned lon
         const double tmc = next_toward_d(rf_int, -1.0);
         const double dv = horner<ORDER>(d2p, tmc);
         if (!is_finite_d(dv)) {
-            report_error(a.err, kErrNpBndDerUb, i, j, dv);
+            report_error(err, kErrNpBndDerUb, i, j, dv);
             return kFmError;
         }
         if (dv < 0) {
             const double dd = horner<ORDER>(d2, tmc);
             if (!is_finite_d(dd)) {
-                report_error(a.err, kErrNpBndDist2Ub, i, j, dd);
+                report_error(err, kErrNpBndDist2Ub, i, j, dd);
                 return kFmError;
             }
-            if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, tmc, Pi, cd.di, Pj, cd.dj);
+            if (dd < thresh2) emit_conj<ORDER>(a, gstep, i, j, lb_rf, tmc, Pi, cd.di, Pj, cd.dj);
         }
     }
     if (flags & kCandBegin) {
         const double dv = horner<ORDER>(d2p, 0.0);
         if (!is_finite_d(dv)) {
-            report_error(a.err, kErrNpBndDerLb, i, j, dv);
+            report_error(err, kErrNpBndDerLb, i, j, dv);
             return kFmError;
         }
         if (dv > 0) {
             const double dd = horner<ORDER>(d2, 0.0);
             if (!is_finite_d(dd)) {
-                report_error(a.err, kErrNpBndDist2Lb, i, j, dd);
+                report_error(err, kErrNpBndDist2Lb, i, j, dd);
                 return kFmError;
             }
-            if (dd < thresh2) emit_conj<ORDER>(a, c, i, j, lb_rf, 0.0, Pi, cd.di, Pj, cd.dj);
+            if (dd < thresh2) emit_conj<ORDER>(a, gstep, i, j, lb_rf, 0.0, Pi, cd.di, Pj, cd.dj);
         }
     }
     return kFmOk;
 }
 
 template <int ORDER>
-__device__ inline void np_flush_report(const NarrowArgs &a, const unsigned long long *rep)
+__device__ inline void np_flush_report(const NarrowArgs &a, const unsigned long long *rep, uint32_t rep_p)
 {
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const unsigned long long s = wave_reduce_add_u64(rep[k]);
-        if (__lane_id() == 0 && s) atomicAdd(&a.np_report[2 + k], s);
-    }
+    add_prob_report<5>(a.P, rep_p, rep_p != 0xffffffffu, 2, rep);
 }
 
 // Narrow phase, part 2: one lane per queued candidate (grid-stride over the
@@ -881,16 +945,17 @@ __global__ __launch_bounds__(256) void k_np_roots(NarrowArgs a)
     const unsigned long long avail = *a.cand_count;
     const uint64_t n = avail < a.cand_cap ? avail : a.cand_cap;
     unsigned long long rep[5] = {0, 0, 0, 0, 0};
+    uint32_t rep_p = 0xffffffffu;
     WLEntry<ORDER> wl[kWlCap];
     const int cap = a.wl_fast_cap < kWlCap ? a.wl_fast_cap : kWlCap;
     const uint64_t stride = (uint64_t)gridDim.x * 256u;
     for (uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x; q < n; q += stride) {
-        if (np_root_cand<ORDER>(a, q, rep, wl, cap) == kFmListFull) {
+        if (np_root_cand<ORDER>(a, q, rep, rep_p, wl, cap) == kFmListFull) {
             const unsigned long long s = atomicAdd(a.slow_count, 1ull);
             if (s < a.slow_cap) a.slow[s] = q;
         }
     }
-    np_flush_report<ORDER>(a, rep);
+    np_flush_report<ORDER>(a, rep, rep_p);
 }
 
 // Narrow phase, part 2b: the candidates of the slow queue, each lane with its
@@ -901,11 +966,44 @@ __global__ __launch_bounds__(256) void k_np_roots_slow(NarrowArgs a)
     const unsigned long long avail = *a.slow_count;
     const uint64_t n = avail < a.slow_cap ? avail : a.slow_cap;
     unsigned long long rep[5] = {0, 0, 0, 0, 0};
+    uint32_t rep_p = 0xffffffffu;
     const uint64_t lane = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     WLEntry<ORDER> *wl = static_cast<WLEntry<ORDER> *>(a.wl_slow) + lane * kWlSlowCap;
     const uint64_t stride = (uint64_t)gridDim.x * 256u;
-    for (uint64_t s = lane; s < n; s += stride) (void)np_root_cand<ORDER>(a, a.slow[s], rep, wl, kWlSlowCap);
-    np_flush_report<ORDER>(a, rep);
+    for (uint64_t s = lane; s < n; s += stride) (void)np_root_cand<ORDER>(a, a.slow[s], rep, rep_p, wl, kWlSlowCap);
+    np_flush_report<ORDER>(a, rep, rep_p);
+}
+
+// End of a narrow-phase pass: the queue lengths folded into peak, the queue
+// counters cleared for the next pass.
+__global__ void k_np_pass_end(unsigned long long *steps_count, unsigned long long *cand_count,
+                              unsigned long long *slow_count, unsigned long long *peak)
+{
+    if (threadIdx.x != 0) return;
+    peak[0] = peak[0] > *steps_count ? peak[0] : *steps_count;
+    peak[1] = peak[1] > *cand_count ? peak[1] : *cand_count;
+    peak[2] = peak[2] > *slow_count ? peak[2] : *slow_count;
+    *steps_count = 0;
+    *cand_count = 0;
+    *slow_count = 0;
+}
+
+// Before the first pass: the problems' narrow-phase counters, the
+// conjunction count, the queue counters and their peaks start at zero.
+__global__ void k_np_reset(const Prob *__restrict__ P, uint32_t np, unsigned long long *conj_count,
+                           unsigned long long *steps_count, unsigned long long *cand_count,
+                           unsigned long long *slow_count, unsigned long long *peak)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < np)
+        for (int k = 0; k < MZH_NP_COUNT; ++k) P[t].np_report[k] = 0ull;
+    if (t == 0) {
+        *conj_count = 0;
+        *steps_count = 0;
+        *cand_count = 0;
+        *slow_count = 0;
+        peak[0] = peak[1] = peak[2] = 0;
+    }
 }
 
 // Root finding of free-standing polynomials (run_poly_root_finding,
@@ -1020,9 +1118,18 @@ __device__ inline bool ck_less(const ConjKey &x, const ConjKey &y, const mzh_con
     return x.idx < y.idx;
 }
 
-__global__ __launch_bounds__(256) void k_conj_keys(const mzh_conj *__restrict__ in, const uint32_t *__restrict__ step,
-                                                   uint64_t n, ConjKey *__restrict__ keys)
+__device__ inline uint64_t dev_count(uint64_t n, const unsigned long long *n_dev)
 {
+    if (!n_dev) return n;
+    const uint64_t m = *n_dev;
+    return m < n ? m : n;
+}
+
+__global__ __launch_bounds__(256) void k_conj_keys(const mzh_conj *__restrict__ in, const uint32_t *__restrict__ step,
+                                                   uint64_t n, const unsigned long long *n_dev,
+                                                   ConjKey *__restrict__ keys)
+{
+    n = dev_count(n, n_dev);
     const uint64_t k = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (k >= n) return;
     const mzh_conj &r = in[k];
@@ -1036,10 +1143,12 @@ __global__ __launch_bounds__(256) void k_conj_keys(const mzh_conj *__restrict__ 
 
 // one workgroup sorts one tile of kConjTile keys in place (bitonic network in LDS)
 __global__ __launch_bounds__(256) void k_conj_tiles(ConjKey *__restrict__ keys, uint64_t n,
-                                                    const mzh_conj *__restrict__ rec)
+                                                    const unsigned long long *n_dev, const mzh_conj *__restrict__ rec)
 {
     __shared__ ConjKey s[kConjTile];
+    n = dev_count(n, n_dev);
     const uint64_t base = (uint64_t)blockIdx.x * kConjTile;
+    if (base >= n) return;
     for (uint32_t t = threadIdx.x; t < kConjTile; t += 256) {
         if (base + t < n) {
             s[t] = keys[base + t];
@@ -1073,8 +1182,10 @@ __global__ __launch_bounds__(256) void k_conj_tiles(ConjKey *__restrict__ keys, 
 
 // merge the sorted runs of width w pairwise: in -> out
 __global__ __launch_bounds__(256) void k_conj_merge(const ConjKey *__restrict__ in, ConjKey *__restrict__ out,
-                                                    uint64_t n, uint64_t w, const mzh_conj *__restrict__ rec)
+                                                    uint64_t n, const unsigned long long *n_dev, uint64_t w,
+                                                    const mzh_conj *__restrict__ rec)
 {
+    n = dev_count(n, n_dev);
     const uint64_t e = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (e >= n) return;
     const uint64_t r = e / w, base = (r & ~1ull) * w;
@@ -1099,11 +1210,58 @@ __global__ __launch_bounds__(256) void k_conj_merge(const ConjKey *__restrict__ 
     out[base + (e - r * w) + (lo - sb)] = me;
 }
 
-__global__ __launch_bounds__(256) void k_conj_gather(const ConjKey *__restrict__ keys, const mzh_conj *__restrict__ in,
-                                                     uint64_t n, mzh_conj *__restrict__ out)
+// Every problem's range of the sorted keys (its batch steps are
+// [step0, step0 + C)), counts[2..3], and kFlagConj if conj_out is too small.
+// One thread per problem.
+__global__ void k_conj_bounds(const Prob *__restrict__ P, uint32_t np, const ConjKey *__restrict__ keys,
+                              uint64_t n, const unsigned long long *n_dev)
 {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    n = dev_count(n, n_dev);
+    const Prob &q = P[p];
+    auto lower = [&](uint32_t st) {
+        uint64_t lo = 0, len = n;
+        while (len > 0) {
+            const uint64_t h = len / 2;
+            if (keys[lo + h].step < st) {
+                lo += h + 1;
+                len -= h + 1;
+            } else {
+                len = h;
+            }
+        }
+        return lo;
+    };
+    const uint64_t b = lower(q.step0), e = lower(q.step0 + q.C);
+    q.counts[2] = e - b;
+    q.counts[3] = b;
+    if (e - b > q.conj_out_cap) atomicOr(q.flags, kFlagConj);
+}
+
+// The sorted records into their problems' outputs (thread per record).
+__global__ __launch_bounds__(256) void k_conj_gather(const Prob *__restrict__ P, uint32_t np,
+                                                     const uint32_t *__restrict__ step_pre,
+                                                     const ConjKey *__restrict__ keys, const mzh_conj *__restrict__ in,
+                                                     uint64_t n, const unsigned long long *n_dev)
+{
+    n = dev_count(n, n_dev);
     const uint64_t k = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    if (k < n) out[k] = in[keys[k].idx];
+    if (k >= n) return;
+    const ConjKey key = keys[k];
+    uint32_t lo = 0, m = np;
+    while (m > 1) {
+        const uint32_t h = m / 2;
+        if (step_pre[lo + h] <= key.step) {
+            lo += h;
+            m -= h;
+        } else {
+            m = h;
+        }
+    }
+    const Prob &q = P[lo];
+    const uint64_t at = k - q.counts[3];
+    if (at < q.conj_out_cap) q.conj_out[at] = in[key.idx];
 }
 #endif // MZH_NARROW_HOST
 
@@ -1114,6 +1272,8 @@ hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream)
 {
     const uint64_t blocks = (a.pair_end - a.pair_begin + 255) / 256;
     if (blocks == 0) return hipSuccess;
+    // the pass's pairs (fewer when the problems placed fewer: the kernel reads
+    // the batch's count)
     hipLaunchKernelGGL(k_np_enum, dim3((unsigned)blocks), dim3(256), 0, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1127,12 +1287,15 @@ hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream)
     // takes several candidates, which evens out their uneven root-finding work,
     // and fewer workgroups add their report counters atomically (4096: 4.48 ms
     // for the narrow phase of the 30 000-object day, 1024: 4.10, 512: 4.06, 256: 4.19)
-    const uint64_t qblocks = std::min<uint64_t>((a.cand_cap + 255) / 256, 512);
+    const uint64_t qblocks = std::min<uint64_t>((a.cand_cap + 255) / 256, a.roots_wgs ? a.roots_wgs : 512);
     hipLaunchKernelGGL(k_np_roots<ORDER>, dim3((unsigned)qblocks), dim3(256), 0, stream, a);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     // the candidates whose root isolation outgrew the private work list
     hipLaunchKernelGGL(k_np_roots_slow<ORDER>, dim3(kNarrowSlowLanes / 256), dim3(256), 0, stream, a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_np_pass_end, dim3(1), dim3(64), 0, stream, a.steps_count, a.cand_count, a.slow_count, a.peak);
     return hipGetLastError();
 }
 
@@ -1277,7 +1440,7 @@ hipError_t launch_rootfind(uint32_t order, const RootFindArgs &a, hipStream_t st
 
 hipError_t launch_narrow(const NarrowArgs &a, hipStream_t stream)
 {
-    switch (a.pj.order) {
+    switch (a.order) {
         case 2:
             return launch_narrow_order<2>(a, stream);
         case 3:
@@ -1321,24 +1484,38 @@ hipError_t launch_narrow(const NarrowArgs &a, hipStream_t stream)
     }
 }
 
-// Conjunction ordering used by the orchestrator (see k_conj_keys .. k_conj_gather).
-hipError_t launch_conj_sort(const mzh_conj *in, const uint32_t *in_step, uint64_t n, ConjKey *keys0, ConjKey *keys1,
-                            mzh_conj *out, hipStream_t stream)
+// Conjunction ordering used by the orchestrator (see k_conj_keys ..
+// k_conj_gather): sized by the capacity n, bounded on the device by n_dev.
+hipError_t launch_conj_sort(const Prob *P, uint32_t np, const uint32_t *step_pre, const mzh_conj *in,
+                            const uint32_t *in_step, uint64_t n, const unsigned long long *n_dev, ConjKey *keys0,
+                            ConjKey *keys1, hipStream_t stream)
 {
-    if (n == 0) return hipSuccess;
     if (n >= 0xffffffffull) return hipErrorInvalidValue;
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_conj_keys, dim3(blocks), dim3(256), 0, stream, in, in_step, n, keys0);
-    hipLaunchKernelGGL(k_conj_tiles, dim3((unsigned)((n + kConjTile - 1) / kConjTile)), dim3(256), 0, stream, keys0, n,
-                       in);
+    if (n > 0) {
+        const unsigned blocks = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_conj_keys, dim3(blocks), dim3(256), 0, stream, in, in_step, n, n_dev, keys0);
+        hipLaunchKernelGGL(k_conj_tiles, dim3((unsigned)((n + kConjTile - 1) / kConjTile)), dim3(256), 0, stream,
+                           keys0, n, n_dev, in);
+    }
     ConjKey *a = keys0, *b = keys1;
     for (uint64_t w = kConjTile; w < n; w *= 2) {
-        hipLaunchKernelGGL(k_conj_merge, dim3(blocks), dim3(256), 0, stream, a, b, n, w, in);
+        hipLaunchKernelGGL(k_conj_merge, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a, b, n, n_dev, w,
+                           in);
         ConjKey *t = a;
         a = b;
         b = t;
     }
-    hipLaunchKernelGGL(k_conj_gather, dim3(blocks), dim3(256), 0, stream, a, in, n, out);
+    hipLaunchKernelGGL(k_conj_bounds, dim3((np + 255) / 256), dim3(256), 0, stream, P, np, a, n, n_dev);
+    if (n > 0)
+        hipLaunchKernelGGL(k_conj_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, P, np, step_pre, a,
+                           in, n, n_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_narrow_reset(const NarrowArgs &a, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_np_reset, dim3((a.np + 255) / 256), dim3(256), 0, stream, a.P, a.np, a.conj_count,
+                       a.steps_count, a.cand_count, a.slow_count, a.peak);
     return hipGetLastError();
 }
 #endif // MZH_NARROW_HOST

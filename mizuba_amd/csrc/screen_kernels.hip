@@ -6,6 +6,7 @@
 // of launches instead of per-step loops.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "mzh_device.hpp"
@@ -19,6 +20,23 @@ namespace
 
 constexpr BinomTable kBinom = make_binom_table();
 __constant__ BinomTable c_binom = kBinom;
+
+// The problem of a block or work item: the largest p with pre[p] <= x (pre is
+// an exclusive prefix of np + 1 entries).
+__device__ inline uint32_t find_prob(const uint32_t *__restrict__ pre, uint32_t np, uint32_t x)
+{
+    uint32_t lo = 0, n = np;
+    while (n > 1) {
+        const uint32_t h = n / 2;
+        if (pre[lo + h] <= x) {
+            lo += h;
+            n -= h;
+        } else {
+            n = h;
+        }
+    }
+    return lo;
+}
 
 // ---------------------------------------------------------------------------
 // upload-time transpose
@@ -188,22 +206,35 @@ __device__ inline void cs_enclosure_fp32(const double *__restrict__ cf, double l
 // Conjunction steps handled by one thread of the AABB kernel: the thread walks
 // its object's time array once across the group instead of searching it per
 // step, and consecutive steps reuse the trajectory pieces it has just read.
-constexpr uint32_t kAabbGroup = 16;
+constexpr uint32_t kAabbGroup = kAabbStepGroup;
 
 // Stage 1: per (object, conjunction step) boxes (detect_conjunctions_aabbs.cpp:64-252)
 // and the per-step global box (:314-397). One thread per object and group of
 // kAabbGroup consecutive steps; the 256 lanes of a block are 256 consecutive
 // objects, so each step's boxes are written coalesced and reduced per block.
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double radius, int mode,
-                                               float *__restrict__ aabbs, uint2 *__restrict__ kbke,
-                                               uint32_t *__restrict__ gbox_ord, DevError *err)
+__global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint32_t np,
+                                               const uint32_t *__restrict__ blk_pre, uint32_t blk_begin)
 {
+    const uint32_t b = blk_begin + blockIdx.x;
+    const uint32_t pi = find_prob(blk_pre, np, b);
+    const Prob &q = P[pi];
+    if (q.pj.order != (uint32_t)ORDER) return;
+    const uint32_t lb = b - blk_pre[pi];
+    const PjView pj = q.pj;
+    const double *cd_begin = q.cd_begin, *cd_end = q.cd_end;
+    const uint32_t n_steps = q.C;
+    const double radius = q.radius;
+    const int mode = q.mode;
+    float *__restrict__ aabbs = q.aabbs;
+    uint2 *__restrict__ kbke = q.kbke;
+    uint32_t *__restrict__ gbox_ord = q.gbox;
+    DevError *err = q.err;
     const uint32_t N = pj.n_objs;
     const uint32_t bpo = (N + 255u) / 256u;
-    const uint32_t g = blockIdx.x / bpo;
-    const uint32_t o = (blockIdx.x - g * bpo) * 256u + threadIdx.x;
-    const uint32_t c0 = g * kAabbGroup, c1 = min(sg.n_steps, c0 + kAabbGroup);
+    const uint32_t g = lb / bpo;
+    const uint32_t o = (lb - g * bpo) * 256u + threadIdx.x;
+    const uint32_t c0 = g * kAabbGroup, c1 = min(n_steps, c0 + kAabbGroup);
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     __shared__ uint32_t s_l[4][4], s_u[4][4];
     // The lane's current trajectory piece (4 channels x (ORDER + 1)
@@ -223,7 +254,7 @@ __global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double ra
             t = pj.time + pj.time_off[o];
             cfb = pj.cf4 + (pj.step0[o] * 4u) * (ORDER + 1);
             // upper_bound(t + 1, t + ns + 1, cd_begin[c0]): first piece ending after the group's start
-            const double cb = sg.cd_begin[c0];
+            const double cb = cd_begin[c0];
             uint64_t lo = 0, n = ns;
             while (n > 0) {
                 const uint64_t h = n / 2;
@@ -241,7 +272,7 @@ __global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double ra
         float lb4[4] = {__builtin_inff(), __builtin_inff(), __builtin_inff(), __builtin_inff()};
         float ub4[4] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
         if (o < N) {
-            const double cb = sg.cd_begin[c], ce = sg.cd_end[c];
+            const double cb = cd_begin[c], ce = cd_end[c];
             uint2 range = make_uint2(0u, 0u);
             if (ns > 0 && !(t[0] >= ce || t[ns] <= cb)) {
                 // kb: upper_bound of cb (monotone in c); ke: lower_bound of ce from kb, plus one
@@ -345,26 +376,50 @@ __global__ __launch_bounds__(256) void k_aabbs(PjView pj, StepGrid sg, double ra
     }
 }
 
-__global__ void k_gbox_finalize(const uint32_t *__restrict__ gbox_ord, float *__restrict__ aabbs, uint32_t N,
-                                uint32_t C, DevError *err)
+// one thread per conjunction step of the batch
+__global__ void k_gbox_finalize(const Prob *__restrict__ P, uint32_t np, const uint32_t *__restrict__ step_pre,
+                                uint32_t n_all)
 {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
-    float *row = aabbs + ((size_t)c * (N + 1) + N) * 8u;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_all) return;
+    const Prob &q = P[find_prob(step_pre, np, s)];
+    const uint32_t c = s - q.step0, N = q.N;
+    const uint32_t *gbox_ord = q.gbox;
+    float *row = q.aabbs + ((size_t)c * (N + 1) + N) * 8u;
     for (int d = 0; d < 4; ++d) {
         const float l = ord2f(gbox_ord[c * 8u + d]), u = ord2f(gbox_ord[c * 8u + 4u + d]);
-        if (is_finite_f(l) && !is_finite_f(u - l)) report_error(err, kErrGlobalBox, c, 0, 0.0);
+        if (is_finite_f(l) && !is_finite_f(u - l)) report_error(q.err, kErrGlobalBox, c, 0, 0.0);
         row[d] = l;
         row[4 + d] = u;
     }
 }
 
-__global__ void k_init_gbox(uint32_t *gbox_ord, uint32_t C)
+// One thread per conjunction step of the batch: the step's global box starts
+// empty; the first step of a problem also clears the problem's counters,
+// flags and error word, and thread 0 the batch's work counters.
+__global__ void k_batch_init(const Prob *__restrict__ P, uint32_t np, const uint32_t *__restrict__ step_pre,
+                             uint32_t n_all, BatchCounters *bc)
 {
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= C * 8u) return;
-    const uint32_t d = idx & 7u;
-    gbox_ord[idx] = (d < 4) ? f2ord(__builtin_inff()) : f2ord(-__builtin_inff());
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) {
+        bc->next_sort = 0;
+        bc->next_bvh = 0;
+        bc->next_bvh_hbm = 0;
+    }
+    if (s >= n_all) return;
+    const Prob &q = P[find_prob(step_pre, np, s)];
+    const uint32_t c = s - q.step0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) q.gbox[c * 8u + d] = (d < 4) ? f2ord(__builtin_inff()) : f2ord(-__builtin_inff());
+    if (c == 0) {
+        for (int k = 0; k < MZH_NP_COUNT; ++k) q.np_report[k] = 0ull;
+        q.err->code = 0;
+        q.err->i = q.err->j = 0u;
+        q.err->value = 0.0;
+        *q.flags = 0u;
+        *q.ovf_count = 0u;
+        for (int k = 0; k < 4; ++k) q.counts[k] = 0ull;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -428,14 +483,20 @@ __device__ inline uint32_t block_excl_scan_1024(uint32_t v, uint32_t *s_w, uint3
     return r;
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_sort(const uint64_t *__restrict__ mcodes,
-                                                       const float *__restrict__ aabbs, uint64_t *__restrict__ keys_tmp,
-                                                       uint32_t *__restrict__ vals_tmp, uint64_t *__restrict__ keys_out,
-                                                       uint32_t *__restrict__ vals_out, uint32_t *__restrict__ n_eff,
-                                                       const uint32_t *__restrict__ todo, uint32_t N)
+__global__ __launch_bounds__(kSortBlock) void k_sort(const Prob *__restrict__ P, uint32_t np,
+                                                       const uint32_t *__restrict__ step_pre)
 {
-    const uint32_t c = blockIdx.x;
-    if (!todo[c]) return;  // sorted by the bucket sort
+    const Prob &q = P[find_prob(step_pre, np, blockIdx.x)];
+    const uint32_t c = blockIdx.x - q.step0;
+    const uint32_t N = q.N;
+    const uint64_t *__restrict__ mcodes = q.mcodes;
+    const float *__restrict__ aabbs = q.aabbs;
+    uint64_t *__restrict__ keys_tmp = q.keys_tmp;
+    uint32_t *__restrict__ vals_tmp = q.vals_tmp;
+    uint64_t *__restrict__ keys_out = q.srt_mcodes;
+    uint32_t *__restrict__ vals_out = q.srt_idx;
+    uint32_t *__restrict__ n_eff = q.n_eff;
+    if (!q.sort_todo[c]) return;  // sorted by the bucket sort
     const size_t base = (size_t)c * N;
     const uint64_t *kin = mcodes + base;
     uint64_t *kt = keys_tmp + base;
@@ -581,14 +642,20 @@ __global__ __launch_bounds__(kSortBlock) void k_sort(const uint64_t *__restrict_
     for (uint32_t e = t0 + tid; e < N; e += kSortBlock) vo[e] = vt[e];
 }
 
-__global__ __launch_bounds__(256) void k_gather(const float *__restrict__ aabbs, const uint32_t *__restrict__ srt_idx,
-                                                float *__restrict__ srt_aabbs, const uint32_t *__restrict__ todo,
-                                                uint32_t N)
+__global__ __launch_bounds__(256) void k_gather(const Prob *__restrict__ P, uint32_t np,
+                                                const uint32_t *__restrict__ blk_pre)
 {
+    const uint32_t pi = find_prob(blk_pre, np, blockIdx.x);
+    const Prob &q = P[pi];
+    const uint32_t lb = blockIdx.x - blk_pre[pi];
+    const uint32_t N = q.N;
+    const float *__restrict__ aabbs = q.aabbs;
+    const uint32_t *__restrict__ srt_idx = q.srt_idx;
+    float *__restrict__ srt_aabbs = q.srt_aabbs;
     const uint32_t bps = (N + 1u + 255u) / 256u;
-    const uint32_t c = blockIdx.x / bps;
-    if (!todo[c]) return;  // gathered by the bucket sort
-    const uint32_t p = (blockIdx.x - c * bps) * 256u + threadIdx.x;
+    const uint32_t c = lb / bps;
+    if (!q.sort_todo[c]) return;  // gathered by the bucket sort
+    const uint32_t p = (lb - c * bps) * 256u + threadIdx.x;
     if (p > N) return;
     const size_t sb = (size_t)c * (N + 1) * 8u;
     const uint32_t src = (p == N) ? N : srt_idx[(size_t)c * N + p];
@@ -611,8 +678,6 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ aabbs,
 // (k_sort + k_gather), flagged in todo.
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t kBktBits = 12;
-constexpr uint32_t kBkts = 1u << kBktBits;
 constexpr uint32_t kBktMax = 1024;
 constexpr uint32_t kBktWin = 1024 + 2 * kBktMax;  // a window and the buckets it cuts
 
@@ -621,26 +686,39 @@ __device__ inline bool key_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t 
     return ka < kb || (ka == kb && va < vb);
 }
 
-__global__ __launch_bounds__(1024) void k_bucket_sort(const float *__restrict__ aabbs, uint64_t *__restrict__ mcodes,
-                                                      uint64_t *__restrict__ keys_tmp, uint32_t *__restrict__ vals_tmp,
-                                                      uint64_t *__restrict__ srt_mcodes, uint32_t *__restrict__ srt_idx,
-                                                      float *__restrict__ srt_aabbs, uint32_t *__restrict__ n_eff,
-                                                      uint32_t *__restrict__ todo, uint32_t N)
+// LDS of the bucket sort of one step.
+template <uint32_t BITS>
+struct BucketLds {
+    uint32_t off[1u << BITS];
+    uint64_t key[kBktWin];
+    uint32_t val[kBktWin];
+    uint32_t w[65];
+    uint32_t big, win[2];
+};
+
+// The bucket sort of step c of problem q by one 1024-thread workgroup.
+template <uint32_t BITS>
+__device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
 {
-    const uint32_t c = blockIdx.x;
+    constexpr uint32_t kBkts = 1u << BITS;
+    constexpr uint32_t kBktBits = BITS;
+    const uint32_t N = q.N;
     const uint32_t tid = threadIdx.x;
-    const float *boxes = aabbs + (size_t)c * (N + 1) * 8u;
-    float *sboxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
-    uint64_t *codes = mcodes + (size_t)c * N;
-    uint64_t *kt = keys_tmp + (size_t)c * N;
-    uint32_t *vt = vals_tmp + (size_t)c * N;
-    uint64_t *ko = srt_mcodes + (size_t)c * N;
-    uint32_t *vo = srt_idx + (size_t)c * N;
-    __shared__ uint32_t s_off[kBkts];
-    __shared__ uint64_t s_key[kBktWin];
-    __shared__ uint32_t s_val[kBktWin];
-    __shared__ uint32_t s_w[65];
-    __shared__ uint32_t s_big, s_win[2];
+    uint32_t *__restrict__ todo = q.sort_todo;
+    uint32_t *__restrict__ n_eff = q.n_eff;
+    const float *boxes = q.aabbs + (size_t)c * (N + 1) * 8u;
+    float *sboxes = q.srt_aabbs + (size_t)c * (N + 1) * 8u;
+    uint64_t *codes = q.mcodes + (size_t)c * N;
+    uint64_t *kt = q.keys_tmp + (size_t)c * N;
+    uint32_t *vt = q.vals_tmp + (size_t)c * N;
+    uint64_t *ko = q.srt_mcodes + (size_t)c * N;
+    uint32_t *vo = q.srt_idx + (size_t)c * N;
+    uint32_t *s_off = L.off;
+    uint64_t *s_key = L.key;
+    uint32_t *s_val = L.val;
+    uint32_t *s_w = L.w;
+    uint32_t &s_big = L.big;
+    uint32_t *s_win = L.win;
     const float4 glo = reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[0];
     const float4 ghi = reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[1];
 
@@ -656,7 +734,7 @@ __global__ __launch_bounds__(1024) void k_bucket_sort(const float *__restrict__ 
         if (is_finite_f(blo.x)) atomicAdd(&s_off[(uint32_t)(code >> (64u - kBktBits))], 1u);
     }
     __syncthreads();
-    // exclusive offsets of the buckets (16 per thread), the largest bucket
+    // exclusive offsets of the buckets (kBkts / 1024 per thread), the largest bucket
     constexpr uint32_t kPer = kBkts / 1024u;
     uint32_t cnt[kPer], sum = 0, mx = 0;
 #pragma unroll
@@ -750,6 +828,32 @@ __global__ __launch_bounds__(1024) void k_bucket_sort(const float *__restrict__ 
                 reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
         }
         __syncthreads();
+    }
+}
+
+// Next work item of a persistent workgroup: one atomic per workgroup on the
+// batch's counter. Every workgroup leaves once the counter passes n_items,
+// which it does after a finite number of fetches: no workgroup ever waits.
+__device__ inline uint32_t next_item(uint32_t *counter, uint32_t *s_item)
+{
+    __syncthreads();  // the previous item's LDS is no longer read
+    if (threadIdx.x == 0) *s_item = atomicAdd(counter, 1u);
+    __syncthreads();
+    return *s_item;
+}
+
+// Stage 2, persistent: each workgroup takes (problem, step) items in the
+// order of the list (largest problems first, so that the long items start
+// early and the short ones fill the end).
+template <uint32_t BITS>
+__global__ __launch_bounds__(1024) void k_bucket_sort(const Prob *__restrict__ P, const uint2 *__restrict__ items,
+                                                      uint32_t n_items, uint32_t *counter)
+{
+    __shared__ BucketLds<BITS> L;
+    __shared__ uint32_t s_item;
+    for (uint32_t it = next_item(counter, &s_item); it < n_items; it = next_item(counter, &s_item)) {
+        const uint2 w = items[it];
+        bucket_sort_step<BITS>(P[w.x], w.y, L);
     }
 }
 
@@ -867,29 +971,23 @@ __device__ inline int bvh_split(const BvhScratch &sc, uint32_t lo, uint32_t hi, 
 // an internal node's in the refit, so that consecutive lanes write full cache
 // lines instead of revisiting parts of records.
 template <bool LDS>
-__global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ srt_mcodes,
-                                                     const float *__restrict__ srt_aabbs,
-                                                     const uint32_t *__restrict__ n_eff, TravNode *__restrict__ trav,
-                                                     uint4 *__restrict__ hdr, uint8_t *__restrict__ scr_buf,
-                                                     size_t scr_stride, uint32_t *__restrict__ tree_size, uint32_t N,
-                                                     uint64_t node_cap)
+__device__ void bvh_step(const Prob &pb, uint32_t c, uint8_t *s_dyn, uint32_t *lvl, uint32_t *s_w)
 {
-    extern __shared__ __align__(16) uint8_t s_dyn[];
-    const uint32_t c = blockIdx.x;
-    const uint32_t n = n_eff[c];
+    const uint32_t N = pb.N;
+    const uint64_t node_cap = pb.node_cap;
+    uint32_t *__restrict__ tree_size = pb.tree_size;
+    const uint32_t n = pb.n_eff[c];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     if (n == 0) {
         if (tid == 0) tree_size[c] = 0;
         return;
     }
-    const uint64_t *codes = srt_mcodes + (size_t)c * N;
-    const float *boxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
-    TravNode *T = trav + (size_t)c * node_cap;
-    uint4 *H = hdr + (size_t)c * node_cap;
-    const BvhScratch sc = bvh_scratch_at(LDS ? s_dyn : scr_buf + (size_t)c * scr_stride, n);
+    const uint64_t *codes = pb.srt_mcodes + (size_t)c * N;
+    const float *boxes = pb.srt_aabbs + (size_t)c * (N + 1) * 8u;
+    TravNode *T = pb.trav + (size_t)c * node_cap;
+    uint4 *H = pb.bvh_hdr + (size_t)c * node_cap;
+    const BvhScratch sc = bvh_scratch_at(LDS ? s_dyn : pb.bvh_scr + (size_t)c * pb.bvh_scr_stride, n);
     const uint32_t nch = (n + 63u) / 64u;
-    __shared__ uint32_t lvl[kBvhMaxLevels + 2];
-    __shared__ uint32_t s_w[65];
 
     // ---- boundary strengths and chunk maxima (one wave per chunk) ----------
     for (uint32_t q = tid; q < nch * 64u; q += kBvhBlock) {
@@ -986,6 +1084,23 @@ __global__ __launch_bounds__(kBvhBlock) void k_bvh(const uint64_t *__restrict__ 
     if (tid == 0) tree_size[c] = le;
 }
 
+// Stage 3, persistent: each workgroup takes (problem, step) items in list
+// order (largest problems first). LDS: the build scratch of the largest step
+// of the list (dynamic) when LDS, else the scratch is in HBM.
+template <bool LDS>
+__global__ __launch_bounds__(kBvhBlock) void k_bvh(const Prob *__restrict__ P, const uint2 *__restrict__ items,
+                                                     uint32_t n_items, uint32_t *counter)
+{
+    extern __shared__ __align__(16) uint8_t s_dyn[];
+    __shared__ uint32_t lvl[kBvhMaxLevels + 2];
+    __shared__ uint32_t s_w[65];
+    __shared__ uint32_t s_item;
+    for (uint32_t it = next_item(counter, &s_item); it < n_items; it = next_item(counter, &s_item)) {
+        const uint2 w = items[it];
+        bvh_step<LDS>(P[w.x], w.y, s_dyn, lvl, s_w);
+    }
+}
+
 // The reference's node records (conjunctions.hpp:65-72) from the device
 // layout, for the steps' trees as exposed by get_bvh_tree.
 __global__ __launch_bounds__(256) void k_bvh_export(const TravNode *__restrict__ trav,
@@ -1071,18 +1186,33 @@ __device__ inline void bp_emit(uint32_t c, uint32_t i, uint32_t j, uint32_t N, u
 // emitted by o for the members after it. Each pair is counted for its lower
 // index i (bp_emit); the order within i's partners is fixed when they are
 // placed.
-__global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav, const uint32_t *__restrict__ tree_size,
-                                            const float *__restrict__ srt_aabbs, const uint64_t *__restrict__ srt_mcodes,
-                                            const uint32_t *__restrict__ srt_idx, const uint32_t *__restrict__ n_eff,
-                                            const int32_t *__restrict__ otypes, uint32_t *__restrict__ cnt,
-                                            uint32_t *__restrict__ slots, uint2 *__restrict__ ovf,
-                                            uint32_t *__restrict__ head, uint32_t *__restrict__ ovf_count,
-                                            uint32_t ovf_cap, uint32_t N, uint64_t node_cap)
+template <uint32_t BS>
+__global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t np,
+                                           const uint32_t *__restrict__ blk_pre)
 {
-    const uint32_t bps = (N + 255u) / 256u;
-    const uint32_t lblk = xcd_block(blockIdx.x, gridDim.x);
+    // logical block: consecutive logical blocks (one step of one problem) on one XCD
+    const uint32_t gblk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t pi = find_prob(blk_pre, np, gblk);
+    const Prob &q = P[pi];
+    const uint32_t lblk = gblk - blk_pre[pi];
+    const uint32_t N = q.N;
+    const uint64_t node_cap = q.node_cap;
+    const TravNode *__restrict__ trav = q.trav;
+    const uint32_t *__restrict__ tree_size = q.tree_size;
+    const float *__restrict__ srt_aabbs = q.srt_aabbs;
+    const uint64_t *__restrict__ srt_mcodes = q.srt_mcodes;
+    const uint32_t *__restrict__ srt_idx = q.srt_idx;
+    const uint32_t *__restrict__ n_eff = q.n_eff;
+    const int32_t *__restrict__ otypes = q.otypes;
+    uint32_t *__restrict__ cnt = q.cnt;
+    uint32_t *__restrict__ slots = q.slots;
+    uint2 *__restrict__ ovf = q.ovf;
+    uint32_t *__restrict__ head = q.head;
+    uint32_t *__restrict__ ovf_count = q.ovf_count;
+    const uint32_t ovf_cap = q.ovf_cap;
+    const uint32_t bps = (N + BS - 1u) / BS;
     const uint32_t c = lblk / bps;
-    const uint32_t o = (lblk - c * bps) * 256u + threadIdx.x;
+    const uint32_t o = (lblk - c * bps) * BS + threadIdx.x;
     const uint32_t n = n_eff[c];
     const uint32_t nn = tree_size[c];
     if (o >= n || nn == 0) return;
@@ -1176,18 +1306,27 @@ __global__ __launch_bounds__(256) void k_bp(const TravNode *__restrict__ trav, c
 
 // Place every object's pairs at its offset in (i, j) order: the inline
 // partners and, past kBpSlots, those on its overflow list, sorted by j.
-__global__ __launch_bounds__(256) void k_bp_place(const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ slots,
-                                                  const uint2 *__restrict__ ovf, const uint32_t *__restrict__ head,
-                                                  uint32_t ovf_cap, const uint64_t *__restrict__ off,
-                                                  const uint64_t *__restrict__ step_off,
-                                                  mzh_aabb_collision *__restrict__ pairs, uint32_t N, uint32_t C)
+__global__ __launch_bounds__(256) void k_bp_place(const Prob *__restrict__ P, uint32_t np,
+                                                  const uint32_t *__restrict__ blk_pre)
 {
-    const uint64_t g = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint32_t pi = find_prob(blk_pre, np, blockIdx.x);
+    const Prob &q = P[pi];
+    const uint32_t N = q.N, C = q.C;
+    const uint32_t *__restrict__ cnt = q.cnt;
+    const uint32_t *__restrict__ slots = q.slots;
+    const uint2 *__restrict__ ovf = q.ovf;
+    const uint32_t *__restrict__ head = q.head;
+    const uint32_t ovf_cap = q.ovf_cap;
+    const uint64_t *__restrict__ off = q.off;
+    const uint64_t *__restrict__ step_off = q.step_off;
+    const uint64_t g = (uint64_t)(blockIdx.x - blk_pre[pi]) * 256u + threadIdx.x;
     if (g >= (uint64_t)N * C) return;
     const uint32_t n = cnt[g];
     if (n == 0) return;
     const uint32_t c = (uint32_t)(g / N), orig = (uint32_t)(g - (uint64_t)c * N);
-    mzh_aabb_collision *out = pairs + step_off[c] + off[g];
+    const uint64_t at = step_off[c] + off[g];
+    if (at + n > q.pair_cap) return;  // too small a pair buffer: flagged, the screening runs again
+    mzh_aabb_collision *out = q.pairs + at;
     if (n <= kBpSlots) {
         uint32_t v[kBpSlots];
 #pragma unroll
@@ -1223,11 +1362,17 @@ __global__ __launch_bounds__(256) void k_bp_place(const uint32_t *__restrict__ c
     }
 }
 
-// per-step exclusive scan of cnt[step][0..N) -> off, step totals
-__global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t *__restrict__ cnt, uint64_t *__restrict__ off,
-                                                      uint64_t *__restrict__ step_total, uint32_t N)
+// per-step exclusive scan of cnt[step][0..N) -> off, step totals (one
+// workgroup per conjunction step of the batch)
+__global__ __launch_bounds__(1024) void k_scan_counts(const Prob *__restrict__ P, uint32_t np,
+                                                      const uint32_t *__restrict__ step_pre)
 {
-    const uint32_t c = blockIdx.x;
+    const Prob &q = P[find_prob(step_pre, np, blockIdx.x)];
+    const uint32_t c = blockIdx.x - q.step0;
+    const uint32_t N = q.N;
+    const uint32_t *__restrict__ cnt = q.cnt;
+    uint64_t *__restrict__ off = q.off;
+    uint64_t *__restrict__ step_total = q.step_total;
     __shared__ uint32_t s_w[65];
     uint64_t carry = 0;
     for (uint32_t cb = 0; cb < N; cb += 1024) {
@@ -1241,11 +1386,14 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t *__restrict
     if (threadIdx.x == 0) step_total[c] = carry;
 }
 
-// exclusive scan over the step totals -> step_off[0..C], grand total
-__global__ __launch_bounds__(1024) void k_scan_steps(const uint64_t *__restrict__ step_total,
-                                                     uint64_t *__restrict__ step_off, uint64_t *__restrict__ grand,
-                                                     uint32_t C)
+// exclusive scan over the step totals -> step_off[0..C], step_off[C] = the
+// problem's pair count (one workgroup per problem)
+__global__ __launch_bounds__(1024) void k_scan_steps(const Prob *__restrict__ P)
 {
+    const Prob &q = P[blockIdx.x];
+    const uint64_t *__restrict__ step_total = q.step_total;
+    uint64_t *__restrict__ step_off = q.step_off;
+    const uint32_t C = q.C;
     __shared__ uint64_t s_w[17];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     uint64_t carry = 0;
@@ -1274,28 +1422,54 @@ __global__ __launch_bounds__(1024) void k_scan_steps(const uint64_t *__restrict_
         carry += s_w[16];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        step_off[C] = carry;
-        *grand = carry;
+    if (threadIdx.x == 0) step_off[C] = carry;
+}
+
+// Pair and pool counts of every problem, their overflow flags, and the
+// exclusive prefix of the placed pairs over the batch (one thread).
+__global__ void k_pair_prefix(const Prob *__restrict__ P, uint32_t np, uint64_t *__restrict__ pair_pre)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t run = 0;
+    for (uint32_t p = 0; p < np; ++p) {
+        const Prob &q = P[p];
+        const uint64_t n = q.step_off[q.C];
+        const uint32_t nov = *q.ovf_count;
+        q.counts[0] = n;
+        q.counts[1] = nov;
+        uint32_t f = 0;
+        if (n > q.pair_cap) f |= kFlagPairs;
+        if (nov > q.ovf_cap) f |= kFlagOvf;
+        if (f) atomicOr(q.flags, f);
+        pair_pre[p] = run;
+        // an incomplete pair list is not screened
+        run += (f == 0) ? n : 0;
     }
+    pair_pre[np] = run;
+}
+
+// Broad-phase counts and list heads start empty (one thread per (step, object)).
+__global__ __launch_bounds__(256) void k_bp_reset(const Prob *__restrict__ P, uint32_t np,
+                                                  const uint32_t *__restrict__ blk_pre)
+{
+    const uint32_t pi = find_prob(blk_pre, np, blockIdx.x);
+    const Prob &q = P[pi];
+    const uint64_t g = (uint64_t)(blockIdx.x - blk_pre[pi]) * 256u + threadIdx.x;
+    if (g >= (uint64_t)q.N * q.C) return;
+    q.cnt[g] = 0u;
+    q.head[g] = 0xffffffffu;
 }
 
 template <int ORDER>
-hipError_t launch_aabbs_order(const PjView &pj, const StepGrid &sg, double radius, int mode, float *aabbs,
-                              uint2 *kbke, uint32_t *gbox_ord, DevError *err, hipStream_t stream)
+hipError_t dispatch_aabbs(uint32_t order, const Prob *P, uint32_t np, const BatchTables &t, uint32_t blk_begin,
+                          uint32_t n_blocks, hipStream_t stream)
 {
-    const dim3 grid(((pj.n_objs + 255u) / 256u) * ((sg.n_steps + kAabbGroup - 1u) / kAabbGroup));
-    hipLaunchKernelGGL(k_aabbs<ORDER>, grid, dim3(256), 0, stream, pj, sg, radius, mode, aabbs, kbke, gbox_ord, err);
-    return hipGetLastError();
-}
-
-template <int ORDER>
-hipError_t dispatch_aabbs(int order, const PjView &pj, const StepGrid &sg, double radius, int mode, float *aabbs,
-                          uint2 *kbke, uint32_t *gbox_ord, DevError *err, hipStream_t stream)
-{
-    if (order == ORDER) return launch_aabbs_order<ORDER>(pj, sg, radius, mode, aabbs, kbke, gbox_ord, err, stream);
+    if (order == (uint32_t)ORDER) {
+        hipLaunchKernelGGL(k_aabbs<ORDER>, dim3(n_blocks), dim3(256), 0, stream, P, np, t.aabb_blk, blk_begin);
+        return hipGetLastError();
+    }
     if constexpr (ORDER < kMaxOrder) {
-        return dispatch_aabbs<ORDER + 1>(order, pj, sg, radius, mode, aabbs, kbke, gbox_ord, err, stream);
+        return dispatch_aabbs<ORDER + 1>(order, P, np, t, blk_begin, n_blocks, stream);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1313,59 +1487,78 @@ hipError_t launch_transpose_cf4(const double *traj, double *cf4, uint64_t n_step
     return hipGetLastError();
 }
 
-hipError_t launch_aabbs(const PjView &pj, const StepGrid &sg, double conj_thresh, int mode, float *aabbs,
-                        uint2 *kbke, uint32_t *gbox_ord, DevError *err, hipStream_t stream)
+hipError_t launch_batch_init(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
+                             BatchCounters *bc, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_init_gbox, dim3((sg.n_steps * 8u + 255u) / 256u), dim3(256), 0, stream, gbox_ord,
-                       sg.n_steps);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return dispatch_aabbs<kMinOrder>((int)pj.order, pj, sg, conj_thresh / 2, mode, aabbs, kbke, gbox_ord, err,
-                                     stream);
-}
-
-hipError_t launch_gbox_finalize(const uint32_t *gbox_ord, float *aabbs, uint32_t n_objs, uint32_t n_steps,
-                                DevError *err, hipStream_t stream)
-{
-    hipLaunchKernelGGL(k_gbox_finalize, dim3((n_steps + 255u) / 256u), dim3(256), 0, stream, gbox_ord, aabbs, n_objs,
-                       n_steps, err);
+    hipLaunchKernelGGL(k_batch_init, dim3((n_steps_all + 255u) / 256u), dim3(256), 0, stream, P, np, t.steps,
+                       n_steps_all, bc);
     return hipGetLastError();
 }
 
-hipError_t launch_morton_sort(const float *aabbs, uint64_t *mcodes, uint64_t *keys_tmp, uint32_t *vals_tmp,
-                              uint64_t *srt_mcodes, uint32_t *srt_idx, float *srt_aabbs, uint32_t *n_eff,
-                              uint32_t *todo, uint32_t n_objs, uint32_t n_steps, hipStream_t stream)
+hipError_t launch_aabbs(uint32_t order, const Prob *P, uint32_t np, const BatchTables &t, uint32_t blk_begin,
+                        uint32_t n_blocks, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_bucket_sort, dim3(n_steps), dim3(1024), 0, stream, aabbs, mcodes, keys_tmp, vals_tmp,
-                       srt_mcodes, srt_idx, srt_aabbs, n_eff, todo, n_objs);
+    if (n_blocks == 0) return hipSuccess;
+    return dispatch_aabbs<kMinOrder>(order, P, np, t, blk_begin, n_blocks, stream);
+}
+
+hipError_t launch_gbox_finalize(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
+                                hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_gbox_finalize, dim3((n_steps_all + 255u) / 256u), dim3(256), 0, stream, P, np, t.steps,
+                       n_steps_all);
+    return hipGetLastError();
+}
+
+hipError_t launch_morton_sort(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
+                              uint32_t n_gather_blocks, BatchCounters *bc, const LaunchShape &ls, hipStream_t stream)
+{
+    const dim3 g(std::max(1u, std::min(ls.sort_wgs, t.n_sort_items)));
+    switch (ls.bucket_bits) {
+        case 10:
+            hipLaunchKernelGGL(k_bucket_sort<10>, g, dim3(1024), 0, stream, P, t.sort_items, t.n_sort_items,
+                               &bc->next_sort);
+            break;
+        case 11:
+            hipLaunchKernelGGL(k_bucket_sort<11>, g, dim3(1024), 0, stream, P, t.sort_items, t.n_sort_items,
+                               &bc->next_sort);
+            break;
+        default:
+            hipLaunchKernelGGL(k_bucket_sort<12>, g, dim3(1024), 0, stream, P, t.sort_items, t.n_sort_items,
+                               &bc->next_sort);
+            break;
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // steps with an oversized bucket: stable radix sort and gather
-    hipLaunchKernelGGL(k_sort, dim3(n_steps), dim3(kSortBlock), 0, stream, mcodes, aabbs, keys_tmp, vals_tmp,
-                       srt_mcodes, srt_idx, n_eff, todo, n_objs);
+    hipLaunchKernelGGL(k_sort, dim3(n_steps_all), dim3(kSortBlock), 0, stream, P, np, t.steps);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gather, dim3(((n_objs + 1u + 255u) / 256u) * n_steps), dim3(256), 0, stream, aabbs, srt_idx,
-                       srt_aabbs, todo, n_objs);
+    hipLaunchKernelGGL(k_gather, dim3(n_gather_blocks), dim3(256), 0, stream, P, np, t.gather_blk);
     return hipGetLastError();
 }
 
 size_t bvh_scratch_stride(uint32_t n_objs) { return (bvh_scratch_bytes(n_objs) + 255u) / 256u * 256u; }
+size_t bvh_scratch_bytes_host(uint32_t n_objs) { return bvh_scratch_bytes(n_objs); }
+static_assert(kBvhLdsMaxHost == kBvhLdsMax, "BVH LDS bound");
 
-hipError_t launch_bvh(const uint64_t *srt_mcodes, const float *srt_aabbs, const uint32_t *n_eff, TravNode *trav,
-                      uint4 *hdr, uint8_t *scr_buf, uint32_t *tree_size, uint32_t n_objs, uint64_t node_cap,
-                      uint32_t n_steps, hipStream_t stream)
+hipError_t launch_bvh(const Prob *P, uint32_t np, const BatchTables &t, BatchCounters *bc, const LaunchShape &ls,
+                      hipStream_t stream)
 {
-    // LDS when two workgroups per CU still fit; otherwise the scratch is in HBM
-    const size_t bytes = bvh_scratch_bytes(n_objs);
-    if (bytes <= kBvhLdsMax) {
-        hipLaunchKernelGGL(k_bvh<true>, dim3(n_steps), dim3(kBvhBlock), bytes, stream, srt_mcodes, srt_aabbs, n_eff,
-                           trav, hdr, scr_buf, bvh_scratch_stride(n_objs), tree_size, n_objs, node_cap);
-    } else {
-        hipLaunchKernelGGL(k_bvh<false>, dim3(n_steps), dim3(kBvhBlock), 0, stream, srt_mcodes, srt_aabbs, n_eff,
-                           trav, hdr, scr_buf, bvh_scratch_stride(n_objs), tree_size, n_objs, node_cap);
+    (void)np;
+    // LDS when the scratch of the largest step of the list fits; otherwise in HBM
+    if (t.n_bvh_items) {
+        hipLaunchKernelGGL(k_bvh<true>, dim3(std::max(1u, std::min(ls.bvh_wgs, t.n_bvh_items))), dim3(kBvhBlock),
+                           t.bvh_lds_bytes, stream, P, t.bvh_items, t.n_bvh_items, &bc->next_bvh);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-    return hipGetLastError();
+    if (t.n_bvh_hbm_items) {
+        hipLaunchKernelGGL(k_bvh<false>, dim3(std::max(1u, std::min(ls.bvh_wgs, t.n_bvh_hbm_items))),
+                           dim3(kBvhBlock), 0, stream, P, t.bvh_hbm_items, t.n_bvh_hbm_items, &bc->next_bvh_hbm);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mzh_bvh_node *nodes, uint64_t node_cap,
@@ -1377,49 +1570,56 @@ hipError_t launch_bvh_export(const TravNode *trav, const uint32_t *tree_size, mz
     return hipGetLastError();
 }
 
-hipError_t launch_bp_reset(uint32_t *cnt, uint32_t *head, uint32_t *ovf_count, uint32_t n_objs, uint32_t n_steps,
+hipError_t launch_bp_reset(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_nc_blocks,
                            hipStream_t stream)
 {
-    const size_t nq = (size_t)n_objs * n_steps;
-    hipError_t e = hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nq, stream);
-    if (e == hipSuccess) e = hipMemsetAsync(head, 0xff, sizeof(uint32_t) * nq, stream);
-    if (e == hipSuccess) e = hipMemsetAsync(ovf_count, 0, sizeof(uint32_t), stream);
-    return e;
+    if (n_nc_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bp_reset, dim3(n_nc_blocks), dim3(256), 0, stream, P, np, t.nc_blk);
+    return hipGetLastError();
 }
 
-hipError_t launch_bp_count(const TravNode *nodes, const uint32_t *tree_size, const float *srt_aabbs,
-                           const uint64_t *srt_mcodes, const uint32_t *srt_idx, const uint32_t *n_eff,
-                           const int32_t *otypes, uint32_t *cnt, uint32_t *slots, uint2 *ovf, uint32_t *head,
-                           uint32_t *ovf_count, uint32_t ovf_cap, uint32_t n_objs, uint64_t node_cap, uint32_t n_steps,
+hipError_t launch_bp_count(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_bp_blocks,
+                           const LaunchShape &ls, hipStream_t stream)
+{
+    if (n_bp_blocks == 0) return hipSuccess;
+    switch (ls.bp_block) {
+        case 128:
+            hipLaunchKernelGGL(k_bp<128>, dim3(n_bp_blocks), dim3(128), 0, stream, P, np, t.bp_blk);
+            break;
+        case 512:
+            hipLaunchKernelGGL(k_bp<512>, dim3(n_bp_blocks), dim3(512), 0, stream, P, np, t.bp_blk);
+            break;
+        default:
+            hipLaunchKernelGGL(k_bp<256>, dim3(n_bp_blocks), dim3(256), 0, stream, P, np, t.bp_blk);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_counts(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
+                              hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_scan_counts, dim3(n_steps_all), dim3(1024), 0, stream, P, np, t.steps);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_steps(const Prob *P, uint32_t np, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_scan_steps, dim3(np), dim3(1024), 0, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_prefix(const Prob *P, uint32_t np, uint64_t *pair_pre, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_pair_prefix, dim3(1), dim3(64), 0, stream, P, np, pair_pre);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_place(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_nc_blocks,
                            hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_bp, dim3(((n_objs + 255u) / 256u) * n_steps), dim3(256), 0, stream, nodes, tree_size,
-                       srt_aabbs, srt_mcodes, srt_idx, n_eff, otypes, cnt, slots, ovf, head, ovf_count, ovf_cap,
-                       n_objs, node_cap);
-    return hipGetLastError();
-}
-
-hipError_t launch_scan_counts(const uint32_t *cnt, uint64_t *off, uint64_t *step_total, uint32_t n_objs,
-                              uint32_t n_steps, hipStream_t stream)
-{
-    hipLaunchKernelGGL(k_scan_counts, dim3(n_steps), dim3(1024), 0, stream, cnt, off, step_total, n_objs);
-    return hipGetLastError();
-}
-
-hipError_t launch_scan_steps(const uint64_t *step_total, uint64_t *step_off, uint64_t *grand_total, uint32_t n_steps,
-                             hipStream_t stream)
-{
-    hipLaunchKernelGGL(k_scan_steps, dim3(1), dim3(1024), 0, stream, step_total, step_off, grand_total, n_steps);
-    return hipGetLastError();
-}
-
-hipError_t launch_bp_place(const uint32_t *cnt, const uint32_t *slots, const uint2 *ovf, const uint32_t *head,
-                           uint32_t ovf_cap, const uint64_t *off, const uint64_t *step_off, mzh_aabb_collision *pairs,
-                           uint32_t n_objs, uint32_t n_steps, hipStream_t stream)
-{
-    const uint64_t nq = (uint64_t)n_objs * n_steps;
-    hipLaunchKernelGGL(k_bp_place, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, stream, cnt, slots, ovf, head,
-                       ovf_cap, off, step_off, pairs, n_objs, n_steps);
+    if (n_nc_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bp_place, dim3(n_nc_blocks), dim3(256), 0, stream, P, np, t.nc_blk);
     return hipGetLastError();
 }
 
