@@ -109,71 +109,159 @@ def copy_calibration(mib):
 def batch_workload(args):
     """The batching configuration: 64 independent problems, object counts
     log-uniform in [500, 20000], conj_thresh U[1, 10] km, horizons U[0.5, 3]
-    days, 3-minute conjunction steps, same element mixture. Times the mixed
-    screening of all problems through mizuba_amd.screen_batch against
-    the same problems screened one by one (polyjectories resident in HBM)."""
+    days, 3-minute conjunction steps, seeded SGP4 mean elements (same mixture)
+    made into polyjectories by make_sgp4_polyjectory, mixed precision. One
+    step = one screening of all 64 problems through a batch handle: one launch
+    per stage for all problems, replayed as a captured HIP graph, launch
+    shapes tuned on the first run. Timed beside it: the same batch run eagerly
+    (host sizing between phases) and the problems screened one by one."""
     import mizuba_amd as mz
 
     rng = np.random.default_rng(args.seed)
     n_prob = 64
     ns = np.exp(rng.uniform(np.log(500), np.log(20000), n_prob)).astype(int)
     threshs = [float(x) for x in rng.uniform(1.0, 10.0, n_prob)]
-    days = rng.uniform(0.5, 3.0, n_prob)
-    intervals = [3.0 / 1440.0] * n_prob
+    days = [float(x) for x in rng.uniform(0.5, 3.0, n_prob)]
+    interval = 3.0 / 1440.0
+    jd0 = 2460496.5
     t0 = time.perf_counter()
-    pjs = [
-        mz.make_synthetic_polyjectory(int(n), float(d), seed=args.seed + 1000 + k, frac_high=0.2, order=8)
-        for k, (n, d) in enumerate(zip(ns, days))
-    ]
-    print(f"[bench] {n_prob} synthetic problems ({int(ns.sum())} objects) in {time.perf_counter() - t0:.1f}s",
+    pjs = []
+    for k in range(n_prob):
+        g = mz.make_synthetic_gpes(int(ns[k]), seed=args.seed + 1000 + k, frac_high=0.2, epoch_jd=jd0)
+        pjs.append(mz.make_sgp4_polyjectory(g, jd0, jd0 + days[k])[0])
+    print(f"[bench] {n_prob} SGP4 problems ({int(ns.sum())} objects) in {time.perf_counter() - t0:.1f}s",
           file=sys.stderr, flush=True)
     obj_days = float(sum(pj.n_objs * pj.maxT for pj in pjs))
-    # polyjectories resident in HBM, one context each; the same contexts are
-    # screened together and then one by one
     scrs = [mz.Screener(0) for _ in pjs]
     for scr, pj in zip(scrs, pjs):
         scr.upload(pj)
-    for _ in range(args.warmup):
-        mz.screen_batch(scrs, threshs, intervals, mode="mixed")
+    intervals = [interval] * n_prob
+    # graph replays (tuned shapes, captured after the first run)
+    bg = mz.Batch(scrs, threshs, intervals, mode="mixed", graph=True, tune=True)
+    t = time.perf_counter()
+    bg.run()
+    tune_s = time.perf_counter() - t
+    for _ in range(max(args.warmup, 1)):
+        bg.run()
+    dev = []
     t = time.perf_counter()
     for _ in range(args.steps):
-        mz.screen_batch(scrs, threshs, intervals, mode="mixed")
-    dt_batch = (time.perf_counter() - t) / args.steps
-    batch = [scr.conjunctions() for scr in scrs]
+        bg.run()
+        dev.append(bg.seconds)
+    dt_graph = (time.perf_counter() - t) / args.steps
+    graph_res = [scr.conjunctions() for scr in scrs]
+    # the same batch run eagerly, with the same shapes
+    be = mz.Batch(scrs, threshs, intervals, mode="mixed", graph=False)
+    be.run()
     t = time.perf_counter()
     for _ in range(args.steps):
-        for scr, th, iv in zip(scrs, threshs, intervals):
-            scr.screen(th, iv, mode="mixed")
+        be.run()
+    dt_eager = (time.perf_counter() - t) / args.steps
+    stages = scrs[0].stage_times()
+    eager_res = [scr.conjunctions() for scr in scrs]
+    n_pairs = sum(scr.n_pairs() for scr in scrs)
+    # one by one
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        for scr, th in zip(scrs, threshs):
+            scr.screen(th, interval, mode="mixed")
     dt_single = (time.perf_counter() - t) / args.steps
     single = [scr.conjunctions() for scr in scrs]
-    same = all(np.array_equal(b.view(np.uint8), s.view(np.uint8)) for b, s in zip(batch, single))
+    same = all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) and np.array_equal(a.view(np.uint8),
+                                                                                        c.view(np.uint8))
+               for a, b, c in zip(graph_res, eager_res, single))
+    # roofline: the broad-phase walk of the eager batch run, the batch's
+    # dominant kernel (algorithmic bytes summed over the problems)
+    kb = stages["broad_kernel"]
+    C_all = [int(np.ceil(pj.maxT / interval)) for pj in pjs]
+    bp_bytes = sum(broad_bytes_per_launch(pj.n_objs, C, 0) for pj, C in zip(pjs, C_all)) + 4 * n_pairs
     out = {
         "metric": "screened object-days/s, 64 independent problems screened together",
-        "value": round(obj_days / dt_batch, 1),
+        "value": round(obj_days / dt_graph, 1),
         "unit": "object-days/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt_batch * 1e3, 3),
+        "ms_per_step": round(dt_graph * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 screen + f64 narrow phase",
-        "data": "synthetic (seeded J2-secular Keplerian catalogs, order-8 Chebyshev-fitted polyjectories)",
+        "data": "synthetic (seeded SGP4 mean elements, polyjectories from this library's GPU make_sgp4_polyjectory, "
+                "order 8)",
         "config": {
             "workload": "64 problems, N log-uniform in [500, 20000], conj_thresh U[1, 10] km, horizon U[0.5, 3] d, "
-            "conj_det_interval 3 min",
+            "conj_det_interval 3 min, mixed precision",
             "n_problems": n_prob,
             "n_objs_total": int(ns.sum()),
             "object_days": round(obj_days, 1),
             "seed": args.seed,
         },
+        "graph_device_ms": round(float(np.median(dev)) * 1e3, 3),
+        "eager_batch": {"value": round(obj_days / dt_eager, 1), "ms_per_step": round(dt_eager * 1e3, 3),
+                        "stages_ms": {k: round(v * 1e3, 3) for k, v in stages.items()}},
         "one_by_one": {"value": round(obj_days / dt_single, 1), "ms_per_step": round(dt_single * 1e3, 3)},
-        "batch_over_one_by_one": round(dt_single / dt_batch, 3),
+        "graph_over_one_by_one": round(dt_single / dt_graph, 3),
+        "graph_over_eager": round(dt_eager / dt_graph, 3),
+        "tuned_shape": bg.shape,
+        "first_run_with_tuning_s": round(tune_s, 3),
         "identical_conj_lists": bool(same),
-        "n_conj": int(sum(len(b) for b in batch)),
+        "n_conj": int(sum(len(b) for b in graph_res)),
+        "n_pairs": int(n_pairs),
+        "roofline": {
+            "kernel": "k_bp",
+            "bound": "hbm",
+            "achieved": round(bp_bytes / kb / 1e9, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(bp_bytes / kb / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes": int(bp_bytes),
+            "avg_launch_ms": round(kb * 1e3, 4),
+        },
     }
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+
+        # the oracle on every problem's first steps (a bounded sample), and
+        # the GPU list against it there
+        threads = po.n_threads_default()
+        nsamp = args.cpu_sample_steps if args.cpu_sample_steps > 0 else 24
+        t = time.perf_counter()
+        refs = []
+        samp_days = 0.0
+        for pj, th in zip(pjs, threshs):
+            traj, tm, toff, nst, tmo = pj._flat()
+            flat = po.Flat.from_arrays(pj.poly_order, traj, tm, nst, toff, tmo, pj.maxT)
+            C = int(np.ceil(pj.maxT / interval))
+            k1 = min(nsamp, C)
+            refs.append(po.run(flat, th, interval, step_end=k1, n_threads=threads))
+            samp_days += pj.n_objs * min(k1 * interval, pj.maxT)
+        dt_cpu = time.perf_counter() - t
+        ok = True
+        for g, r in zip(graph_res, refs):
+            t_end = r["cd_end_times"][-1]
+            gg = g[g["tca"] < t_end]
+            ok = ok and np.array_equal(gg.view(np.uint8), r["conjunctions"].view(np.uint8))
+        out["cpu_baseline"] = {"value": round(samp_days / dt_cpu, 2), "unit": "object-days/s", "cores": threads,
+                               "kind": "port", "cpu": cpu_model(),
+                               "sample": f"first {nsamp} conjunction steps of each of the 64 problems "
+                                         f"({samp_days:.0f} object-days), oracle parallel over steps only, "
+                                         f"wall {dt_cpu:.2f}s"}
+        out["oracle_check"] = {"steps_per_problem": nsamp, "identical_bits": bool(ok),
+                               "n_conj": int(sum(len(r["conjunctions"]) for r in refs))}
     print(json.dumps(out))
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the CPU baseline."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def eval_bytes_per_launch(pj, tgrid):
@@ -651,8 +739,10 @@ def main():
             "unit": "object-days/s",
             "cores": threads,
             "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"first {nsamp} of {C} conjunction steps ({sample_days:.4f} days) of the same "
-            f"{pj.n_objs}-object polyjectory, wall {dt_cpu:.2f}s",
+            f"{pj.n_objs}-object polyjectory, wall {dt_cpu:.2f}s; the oracle runs its steps in parallel "
+            "(OpenMP over steps, each step's objects and pairs in one thread)",
         }
         # TCA/DCA error of the mixed GPU pipeline against the FP64 oracle on the sample
         t_end = ref["cd_end_times"][-1]

@@ -36,6 +36,7 @@ polyjectory = _core.polyjectory
 conjunctions = _core.conjunctions
 conjunctions_batch = _core.conjunctions_batch
 Screener = _core.Screener
+Batch = _core.Batch
 screen_batch = _core.screen_batch
 make_synthetic_polyjectory = _core.make_synthetic_polyjectory
 device_count = _core.device_count
@@ -61,6 +62,7 @@ __all__ = [
     "conjunctions_batch",
     "otype",
     "Screener",
+    "Batch",
     "screen_batch",
     "make_synthetic_polyjectory",
     "device_count",

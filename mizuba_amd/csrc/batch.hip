@@ -195,7 +195,7 @@ void ScreenBatch::prepare()
     hP.assign(n, Prob{});
     MZH_HIP(d_counts.ensure(sizeof(uint64_t) * 4u * n));
     MZH_HIP(d_flags.ensure(sizeof(uint32_t) * n));
-    MZH_HIP(d_np.ensure(sizeof(unsigned long long) * MZH_NP_COUNT * n));
+    MZH_HIP(d_np.ensure(sizeof(unsigned long long) * kReportWords * n));
     MZH_HIP(d_err.ensure(sizeof(DevError) * n));
     MZH_HIP(d_bc.ensure(sizeof(BatchCounters)));
     MZH_HIP(d_pair_pre.ensure(sizeof(uint64_t) * (n + 1)));
@@ -248,8 +248,8 @@ void ScreenBatch::prepare()
         MZH_HIP(hipMemcpyAsync(ctx->otypes.p, ctx->h_otypes.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
         Prob &q = hP[k];
         q.pj = pj_view(ctx);
-        q.cd_begin = ctx->cd_begin.as<const double>();
-        q.cd_end = ctx->cd_end.as<const double>();
+        q.cd_begin = gl(ctx->cd_begin.as<const double>());
+        q.cd_end = gl(ctx->cd_end.as<const double>());
         q.C = (uint32_t)C;
         q.N = N;
         q.step0 = step_pre[k];
@@ -258,39 +258,39 @@ void ScreenBatch::prepare()
         q.thresh = ctx->conj_thresh;
         q.node_cap = ctx->node_cap;
         q.bvh_scr_stride = bvh_scratch_stride(N);
-        q.aabbs = ctx->aabbs.as<float>();
-        q.srt_aabbs = ctx->srt_aabbs.as<float>();
-        q.kbke = ctx->kbke.as<uint2>();
-        q.gbox = ctx->gbox.as<uint32_t>();
-        q.mcodes = ctx->mcodes.as<uint64_t>();
-        q.srt_mcodes = ctx->srt_mcodes.as<uint64_t>();
-        q.keys_tmp = ctx->keys_tmp.as<uint64_t>();
-        q.srt_idx = ctx->srt_idx.as<uint32_t>();
-        q.vals_tmp = ctx->vals_tmp.as<uint32_t>();
-        q.n_eff = ctx->n_eff.as<uint32_t>();
-        q.sort_todo = ctx->sort_todo.as<uint32_t>();
-        q.trav = ctx->trav.as<TravNode>();
-        q.bvh_hdr = ctx->bvh_hdr.as<uint4>();
-        q.bvh_scr = ctx->bvh_scr.as<uint8_t>();
-        q.tree_size = ctx->tree_size.as<uint32_t>();
-        q.otypes = ctx->otypes.as<const int32_t>();
-        q.cnt = ctx->cnt.as<uint32_t>();
-        q.head = ctx->head.as<uint32_t>();
-        q.slots = ctx->slots.as<uint32_t>();
-        q.ovf = ctx->ovf.as<uint2>();
-        q.ovf_count = ctx->ovf_count.as<uint32_t>();
+        q.aabbs = gl(ctx->aabbs.as<float>());
+        q.srt_aabbs = gl(ctx->srt_aabbs.as<float>());
+        q.kbke = gl(ctx->kbke.as<uint2>());
+        q.gbox = gl(ctx->gbox.as<uint32_t>());
+        q.mcodes = gl(ctx->mcodes.as<uint64_t>());
+        q.srt_mcodes = gl(ctx->srt_mcodes.as<uint64_t>());
+        q.keys_tmp = gl(ctx->keys_tmp.as<uint64_t>());
+        q.srt_idx = gl(ctx->srt_idx.as<uint32_t>());
+        q.vals_tmp = gl(ctx->vals_tmp.as<uint32_t>());
+        q.n_eff = gl(ctx->n_eff.as<uint32_t>());
+        q.sort_todo = gl(ctx->sort_todo.as<uint32_t>());
+        q.trav = gl(ctx->trav.as<TravNode>());
+        q.bvh_hdr = gl(ctx->bvh_hdr.as<uint4>());
+        q.bvh_scr = gl(ctx->bvh_scr.as<uint8_t>());
+        q.tree_size = gl(ctx->tree_size.as<uint32_t>());
+        q.otypes = gl(ctx->otypes.as<const int32_t>());
+        q.cnt = gl(ctx->cnt.as<uint32_t>());
+        q.head = gl(ctx->head.as<uint32_t>());
+        q.slots = gl(ctx->slots.as<uint32_t>());
+        q.ovf = gl(ctx->ovf.as<uint2>());
+        q.ovf_count = gl(ctx->ovf_count.as<uint32_t>());
         q.ovf_cap = ctx->ovf_cap;
-        q.off = ctx->off.as<uint64_t>();
-        q.step_total = ctx->step_total.as<uint64_t>();
-        q.step_off = ctx->step_off.as<uint64_t>();
-        q.pairs = ctx->pairs.as<mzh_aabb_collision>();
+        q.off = gl(ctx->off.as<uint64_t>());
+        q.step_total = gl(ctx->step_total.as<uint64_t>());
+        q.step_off = gl(ctx->step_off.as<uint64_t>());
+        q.pairs = gl(ctx->pairs.as<mzh_aabb_collision>());
         q.pair_cap = ctx->pair_cap;
-        q.conj_out = ctx->conj_sorted.as<mzh_conj>();
+        q.conj_out = gl(ctx->conj_sorted.as<mzh_conj>());
         q.conj_out_cap = ctx->conj_out_cap;
-        q.np_report = d_np.as<unsigned long long>() + (size_t)k * MZH_NP_COUNT;
-        q.err = d_err.as<DevError>() + k;
-        q.flags = d_flags.as<uint32_t>() + k;
-        q.counts = d_counts.as<uint64_t>() + 4u * k;
+        q.np_report = gl(d_np.as<unsigned long long>() + (size_t)k * kReportWords);
+        q.err = gl(d_err.as<DevError>() + k);
+        q.flags = gl(d_flags.as<uint32_t>() + k);
+        q.counts = gl(d_counts.as<uint64_t>() + 4u * k);
         step_pre[k + 1] = step_pre[k] + (uint32_t)C;
         aabb_pre[k + 1] = aabb_pre[k] + ((N + 255u) / 256u) * (uint32_t)((C + kAabbStepGroup - 1) / kAabbStepGroup);
         nc_pre[k + 1] = nc_pre[k] + (uint32_t)((NC + 255u) / 256u);
@@ -485,14 +485,15 @@ void ScreenBatch::run_eager(bool timed)
             if (flags[k] & kFlagOvf) {
                 ctx->ovf_cap = (uint32_t)std::min<uint64_t>(grown(counts[4u * k + 1]), 0x7fffffffu);
                 MZH_HIP(ctx->ovf.ensure(sizeof(uint2) * ctx->ovf_cap));
-                hP[k].ovf = ctx->ovf.as<uint2>();
+                hP[k].ovf = gl(ctx->ovf.as<uint2>());
                 hP[k].ovf_cap = ctx->ovf_cap;
                 redo = true;
             }
             if (counts[4u * k] > ctx->pair_cap) {
-                ctx->pair_cap = counts[4u * k];
+                // exact, or with the headroom that later graph replays need
+                ctx->pair_cap = use_graph ? grown(counts[4u * k]) : counts[4u * k];
                 MZH_HIP(ctx->pairs.ensure(sizeof(mzh_aabb_collision) * ctx->pair_cap));
-                hP[k].pairs = ctx->pairs.as<mzh_aabb_collision>();
+                hP[k].pairs = gl(ctx->pairs.as<mzh_aabb_collision>());
                 hP[k].pair_cap = ctx->pair_cap;
                 resized = true;
             }
@@ -552,9 +553,9 @@ void ScreenBatch::run_eager(bool timed)
         for (uint32_t k = 0; k < n; ++k) {
             mzh_ctx *ctx = probs[k];
             if (counts[4u * k + 2] > ctx->conj_out_cap) {
-                ctx->conj_out_cap = counts[4u * k + 2];
+                ctx->conj_out_cap = use_graph ? grown(counts[4u * k + 2]) : counts[4u * k + 2];
                 MZH_HIP(ctx->conj_sorted.ensure(sizeof(mzh_conj) * ctx->conj_out_cap));
-                hP[k].conj_out = ctx->conj_sorted.as<mzh_conj>();
+                hP[k].conj_out = gl(ctx->conj_sorted.as<mzh_conj>());
                 hP[k].conj_out_cap = ctx->conj_out_cap;
                 resized = true;
             }
@@ -576,28 +577,18 @@ void ScreenBatch::capture()
     read_back(counts, flags);
     unsigned long long ctr[8];
     MZH_HIP(hipMemcpy(ctr, d_ctr.p, sizeof(ctr), hipMemcpyDeviceToHost));
+    // the results of the eager run stay where they are: the buffers that hold
+    // them already have the headroom (run_eager sized them so); only the
+    // overflow pool, scratch, may grow here
     group_pair_ub.assign(groups.size(), 0);
     for (uint32_t k = 0; k < n; ++k) {
         mzh_ctx *ctx = probs[k];
-        const uint64_t pc = grown(counts[4u * k]), cc = grown(counts[4u * k + 2]);
-        if (pc > ctx->pair_cap) {
-            ctx->pair_cap = pc;
-            MZH_HIP(ctx->pairs.ensure(sizeof(mzh_aabb_collision) * pc));
-        }
-        if (cc > ctx->conj_out_cap) {
-            ctx->conj_out_cap = cc;
-            MZH_HIP(ctx->conj_sorted.ensure(sizeof(mzh_conj) * cc));
-        }
         const uint32_t oc = (uint32_t)std::min<uint64_t>(grown(counts[4u * k + 1]), 0x7fffffffu);
         if (oc > ctx->ovf_cap) {
             ctx->ovf_cap = oc;
             MZH_HIP(ctx->ovf.ensure(sizeof(uint2) * oc));
         }
-        hP[k].pairs = ctx->pairs.as<mzh_aabb_collision>();
-        hP[k].pair_cap = ctx->pair_cap;
-        hP[k].conj_out = ctx->conj_sorted.as<mzh_conj>();
-        hP[k].conj_out_cap = ctx->conj_out_cap;
-        hP[k].ovf = ctx->ovf.as<uint2>();
+        hP[k].ovf = gl(ctx->ovf.as<uint2>());
         hP[k].ovf_cap = ctx->ovf_cap;
     }
     for (size_t gi = 0; gi < groups.size(); ++gi)
@@ -661,7 +652,7 @@ void ScreenBatch::finalize(int *status, bool replayed)
     std::vector<uint64_t> counts(4u * n);
     std::vector<uint32_t> flags(n);
     std::vector<DevError> errs(n);
-    std::vector<unsigned long long> rep(MZH_NP_COUNT * (size_t)n);
+    std::vector<unsigned long long> rep(kReportWords * (size_t)n);
     MZH_HIP(hipMemcpyAsync(counts.data(), d_counts.p, sizeof(uint64_t) * 4u * n, hipMemcpyDeviceToHost, st));
     MZH_HIP(hipMemcpyAsync(errs.data(), d_err.p, sizeof(DevError) * n, hipMemcpyDeviceToHost, st));
     MZH_HIP(hipMemcpyAsync(rep.data(), d_np.p, sizeof(unsigned long long) * rep.size(), hipMemcpyDeviceToHost, st));
@@ -712,7 +703,11 @@ void ScreenBatch::finalize(int *status, bool replayed)
         }
         ctx->n_pairs = ctx->skip_cd ? 0 : counts[4u * k];
         ctx->n_conj = counts[4u * k + 2];
-        for (int m = 0; m < MZH_NP_COUNT; ++m) ctx->h_np[m] = rep[(size_t)k * MZH_NP_COUNT + m];
+        for (int m = 0; m < MZH_NP_COUNT; ++m) {
+            uint64_t sum = 0;
+            for (uint32_t sl = 0; sl < kReportSlots; ++sl) sum += rep[(size_t)k * kReportWords + sl * kReportStride + m];
+            ctx->h_np[m] = sum;
+        }
         for (int m = 0; m < MZH_STAGE_COUNT; ++m) ctx->stage_s[m] = stage[m];
         ctx->nodes_ready = false;
         ctx->screened = true;

@@ -177,12 +177,12 @@ namespace mzh::host
 PjView pj_view(const mzh_ctx *ctx)
 {
     PjView v;
-    v.traj = ctx->pjb->traj.as<const double>();
-    v.time = ctx->pjb->time.as<const double>();
-    v.cf4 = ctx->pjb->cf4.as<const double>();
-    v.step0 = ctx->pjb->step0.as<const uint64_t>();
-    v.nsteps = ctx->pjb->nsteps.as<const uint64_t>();
-    v.time_off = ctx->pjb->time_off.as<const uint64_t>();
+    v.traj = gl(ctx->pjb->traj.as<const double>());
+    v.time = gl(ctx->pjb->time.as<const double>());
+    v.cf4 = gl(ctx->pjb->cf4.as<const double>());
+    v.step0 = gl(ctx->pjb->step0.as<const uint64_t>());
+    v.nsteps = gl(ctx->pjb->nsteps.as<const uint64_t>());
+    v.time_off = gl(ctx->pjb->time_off.as<const uint64_t>());
     v.n_objs = ctx->n_objs;
     v.order = ctx->order;
     return v;

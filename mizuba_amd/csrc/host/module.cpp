@@ -395,8 +395,9 @@ std::shared_ptr<PjData> pj_from_synth(mzh_host::SynthOut &&s)
 
 struct Ctx {
     mzh_ctx *ctx = nullptr;
+    int device = 0;
     std::shared_ptr<PjData> pj;
-    explicit Ctx(int device)
+    explicit Ctx(int device_) : device(device_)
     {
         const int st = mzh_create(device, &ctx);
         check(st, nullptr);
@@ -408,6 +409,7 @@ struct Ctx {
     Ctx(const Ctx &) = delete;
     Ctx &operator=(const Ctx &) = delete;
 
+    // a new host-to-device copy of the polyjectory in this context
     void upload(const std::shared_ptr<PjData> &d)
     {
         int st;
@@ -999,6 +1001,48 @@ std::vector<Conjunctions> make_conjunctions_batch(const std::vector<Polyjectory>
     return out;
 }
 
+// A batch handle over Screener contexts, which it keeps alive.
+struct BatchHandle {
+    mzh_batch *b = nullptr;
+    std::vector<std::shared_ptr<Ctx>> scrs;
+    BatchHandle() = default;
+    BatchHandle(const BatchHandle &) = delete;
+    BatchHandle &operator=(const BatchHandle &) = delete;
+    ~BatchHandle()
+    {
+        if (b) mzh_batch_destroy(b);
+    }
+};
+
+std::shared_ptr<BatchHandle> make_batch_handle(const std::vector<std::shared_ptr<Ctx>> &scrs,
+                                               const std::vector<double> &threshs,
+                                               const std::vector<double> &intervals,
+                                               std::optional<std::vector<std::optional<std::vector<int32_t>>>> otypes,
+                                               const std::string &mode, bool graph, bool tune)
+{
+    const size_t n = scrs.size();
+    if (threshs.size() != n || intervals.size() != n || (otypes && otypes->size() != n))
+        throw std::invalid_argument("Batch: the screeners, thresholds, intervals and object types must come in lists "
+                                    "of the same length");
+    std::vector<mzh_ctx *> ctxs(n);
+    std::vector<const int32_t *> ots(n, nullptr);
+    for (size_t k = 0; k < n; ++k) {
+        if (!scrs[k] || !scrs[k]->pj) throw std::invalid_argument("No polyjectory has been uploaded");
+        ctxs[k] = scrs[k]->ctx;
+        if (otypes && (*otypes)[k]) {
+            check_otypes_size((*otypes)[k], scrs[k]->pj->n_objs);
+            ots[k] = (*otypes)[k]->data();
+        }
+    }
+    auto h = std::make_shared<BatchHandle>();
+    h->scrs = scrs;
+    const int flags = (graph ? MZH_BATCH_GRAPH : 0) | (tune ? MZH_BATCH_TUNE : 0);
+    check(mzh_batch_create(ctxs.data(), n, threshs.data(), intervals.data(), ots.data(), parse_mode(mode), flags,
+                           &h->b),
+          nullptr);
+    return h;
+}
+
 } // namespace
 
 PYBIND11_MODULE(_core, m)
@@ -1233,11 +1277,19 @@ PYBIND11_MODULE(_core, m)
 
     py::class_<Ctx, std::shared_ptr<Ctx>>(m, "Screener")
         .def(py::init<int>(), "device"_a = 0)
-        .def("upload",
-             [](Ctx &c, const Polyjectory &p) {
-                 (void)attached(p);
-                 c.upload(p.d);
-             })
+        .def(
+            "upload",
+            [](Ctx &c, const Polyjectory &p, bool copy) {
+                // the polyjectory's device copy on this context's device is
+                // shared (made on first use); copy=True uploads a new one
+                (void)attached(p);
+                if (copy) {
+                    c.upload(p.d);
+                } else {
+                    c.share(p.d, c.device);
+                }
+            },
+            "pj"_a, "copy"_a = false)
         .def(
             "screen",
             [](Ctx &c, double thresh, double interval, std::optional<std::vector<int32_t>> otypes,
@@ -1413,6 +1465,46 @@ PYBIND11_MODULE(_core, m)
         },
         "screeners"_a, "conj_threshs"_a, "conj_det_intervals"_a, "mode"_a = "fp64",
         "Screen the polyjectories resident in several Screener objects together (mzh_screen_batch).");
+
+    // ---- a batch handle over Screener contexts ----------------------------
+    py::class_<BatchHandle, std::shared_ptr<BatchHandle>>(m, "Batch")
+        .def(py::init(&make_batch_handle), "screeners"_a, "conj_threshs"_a, "conj_det_intervals"_a,
+             "otypes"_a = py::none(), "mode"_a = "fp64", "graph"_a = true, "tune"_a = false,
+             "The same problems screened again and again (mzh_batch_create): one launch per stage for all of "
+             "them; graph: replay the captured launch sequence after the first run; tune: time the launch shapes "
+             "on the first run.")
+        .def("run",
+             [](BatchHandle &b) {
+                 std::vector<int> st(b.scrs.size(), MZH_OK);
+                 {
+                     py::gil_scoped_release rel;
+                     (void)mzh_batch_run(b.b, st.data());
+                 }
+                 for (size_t k = 0; k < st.size(); ++k) check(st[k], b.scrs[k]->ctx);
+             })
+        .def_property_readonly("replays",
+                               [](const BatchHandle &b) {
+                                   uint64_t r = 0;
+                                   mzh_batch_info(b.b, &r, nullptr, nullptr);
+                                   return r;
+                               })
+        .def_property_readonly("seconds",
+                               [](const BatchHandle &b) {
+                                   double t = 0;
+                                   mzh_batch_info(b.b, nullptr, &t, nullptr);
+                                   return t;
+                               })
+        .def_property_readonly("shape", [](const BatchHandle &b) {
+            uint32_t sh[5] = {0, 0, 0, 0, 0};
+            mzh_batch_info(b.b, nullptr, nullptr, sh);
+            py::dict d;
+            d["bucket_bits"] = sh[0];
+            d["sort_wgs"] = sh[1];
+            d["bvh_wgs"] = sh[2];
+            d["bp_block"] = sh[3];
+            d["roots_wgs"] = sh[4];
+            return d;
+        });
 
     m.def("conjunctions_batch", &make_conjunctions_batch, "pjs"_a, "conj_threshs"_a, "conj_det_intervals"_a,
           "otypes"_a = py::none(), "mode"_a = "fp64", "device"_a = 0,

@@ -48,14 +48,33 @@ struct StepGrid {
     uint32_t n_steps;       // C
 };
 
+// Device buffers reached through a pointer held in device memory (a Prob
+// record, a PjView inside one) are declared global: a pointer loaded from
+// memory is otherwise generic (flat) to the compiler, and a flat access also
+// counts against the LDS wait counter, so kernels that mix LDS work with such
+// loads would wait on both. gl() converts a host-side device pointer for
+// these fields (a no-op in the host compilation).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MZH_GLOBAL __attribute__((address_space(1)))
+#else
+#define MZH_GLOBAL
+#endif
+template <class T>
+using gbuf = T MZH_GLOBAL *;
+template <class T>
+__host__ __device__ inline gbuf<T> gl(T *p)
+{
+    return (gbuf<T>)p;
+}
+
 // Polyjectory as laid out in HBM.
 struct PjView {
-    const double *traj;      // AoS [S][op1][7]
-    const double *time;      // per object nsteps + 1 values
-    const double *cf4;       // step-major transposed [S][4][op1], channels x y z r
-    const uint64_t *step0;   // global index of the object's first step
-    const uint64_t *nsteps;  // steps per object
-    const uint64_t *time_off;
+    gbuf<const double> traj;      // AoS [S][op1][7]
+    gbuf<const double> time;      // per object nsteps + 1 values
+    gbuf<const double> cf4;       // step-major transposed [S][4][op1], channels x y z r
+    gbuf<const uint64_t> step0;   // global index of the object's first step
+    gbuf<const uint64_t> nsteps;  // steps per object
+    gbuf<const uint64_t> time_off;
     uint32_t n_objs;
     uint32_t order;
 };

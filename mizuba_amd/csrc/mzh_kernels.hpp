@@ -33,9 +33,19 @@ constexpr uint32_t kFlagPairs = 1u;  // more pairs than pair_cap
 constexpr uint32_t kFlagOvf = 2u;    // broad-phase overflow pool full
 constexpr uint32_t kFlagConj = 4u;   // more conjunctions than conj_out_cap
 
+// Narrow-phase report counters of a problem (Prob::np_report): kReportSlots
+// copies of the MZH_NP_COUNT counters, one 128-byte line each. A wave adds its
+// sums to the copy its index picks, so that the additions of a launch spread
+// over memory channels instead of queueing on one address; the host adds the
+// copies up.
+constexpr uint32_t kReportSlots = 64;
+constexpr uint32_t kReportStride = 16;
+constexpr uint32_t kReportWords = kReportSlots * kReportStride;
+static_assert(MZH_NP_COUNT <= kReportStride, "report counters per copy");
+
 struct Prob {
     PjView pj;
-    const double *cd_begin, *cd_end;  // [C]
+    gbuf<const double> cd_begin, cd_end;  // [C]
     uint32_t C, N;
     uint32_t step0;     // index of its first conjunction step among the batch's
     int32_t mode;       // MZH_MODE_FP64 / MZH_MODE_MIXED
@@ -44,31 +54,31 @@ struct Prob {
     uint64_t node_cap;  // BVH nodes per step
     uint64_t bvh_scr_stride;
     // the problem's buffers (its context's)
-    float *aabbs, *srt_aabbs;  // [C][N + 1][2][4]
-    uint2 *kbke;               // [C][N]
-    uint32_t *gbox;            // [C][8] order-preserving images of the global box
-    uint64_t *mcodes, *srt_mcodes, *keys_tmp;
-    uint32_t *srt_idx, *vals_tmp, *n_eff, *sort_todo;
-    TravNode *trav;            // [C][node_cap]
-    uint4 *bvh_hdr;            // [C][node_cap]
-    uint8_t *bvh_scr;          // [C][bvh_scr_stride], when the build scratch is not in LDS
-    uint32_t *tree_size;       // [C]
-    const int32_t *otypes;     // [N]
-    uint32_t *cnt, *head, *slots;  // [C][N], [C][N], [C][N][4]
-    uint2 *ovf;                // overflow pool {partner, next}
-    uint32_t *ovf_count;
+    gbuf<float> aabbs, srt_aabbs;  // [C][N + 1][2][4]
+    gbuf<uint2> kbke;  // [C][N]
+    gbuf<uint32_t> gbox;  // [C][8] order-preserving images of the global box
+    gbuf<uint64_t> mcodes, srt_mcodes, keys_tmp;
+    gbuf<uint32_t> srt_idx, vals_tmp, n_eff, sort_todo;
+    gbuf<TravNode> trav;  // [C][node_cap]
+    gbuf<uint4> bvh_hdr;  // [C][node_cap]
+    gbuf<uint8_t> bvh_scr;  // [C][bvh_scr_stride], when the build scratch is not in LDS
+    gbuf<uint32_t> tree_size;  // [C]
+    gbuf<const int32_t> otypes;  // [N]
+    gbuf<uint32_t> cnt, head, slots;  // [C][N], [C][N], [C][N][4]
+    gbuf<uint2> ovf;  // overflow pool {partner, next}
+    gbuf<uint32_t> ovf_count;
     uint32_t ovf_cap;
-    uint64_t *off;             // [C][N] offset of each object's pairs in its step
-    uint64_t *step_total;      // [C]
-    uint64_t *step_off;        // [C + 1]: exclusive prefix of the steps' pairs; [C] = all pairs
-    mzh_aabb_collision *pairs; // pair_cap entries
+    gbuf<uint64_t> off;  // [C][N] offset of each object's pairs in its step
+    gbuf<uint64_t> step_total;  // [C]
+    gbuf<uint64_t> step_off;  // [C + 1]: exclusive prefix of the steps' pairs; [C] = all pairs
+    gbuf<mzh_aabb_collision> pairs;  // pair_cap entries
     uint64_t pair_cap;
-    mzh_conj *conj_out;        // the problem's conjunctions, sorted (conj_out_cap entries)
+    gbuf<mzh_conj> conj_out;  // the problem's conjunctions, sorted (conj_out_cap entries)
     uint64_t conj_out_cap;
-    unsigned long long *np_report;  // [MZH_NP_COUNT]
-    DevError *err;
-    uint32_t *flags;           // kFlag*
-    uint64_t *counts;          // [4]: pairs, pool entries, conjunctions, first conjunction in the batch's list
+    gbuf<unsigned long long> np_report;  // [kReportSlots][kReportStride], MZH_NP_COUNT used of each
+    gbuf<DevError> err;
+    gbuf<uint32_t> flags;  // kFlag*
+    gbuf<uint64_t> counts;  // [4]: pairs, pool entries, conjunctions, first conjunction in the batch's list
 };
 
 // Work counters shared by the launches of one batch run (device memory),

@@ -62,6 +62,12 @@ __device__ inline uint32_t find_prob64(const uint64_t *__restrict__ pre, uint32_
     return lo;
 }
 
+// Offset of the report copy this wave adds to (see kReportSlots).
+__device__ inline uint32_t report_slot()
+{
+    return ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kReportSlots - 1u)) * kReportStride;
+}
+
 // Add v[0..K) of every active lane to np_report[k0..k0+K) of the lane's problem
 // p. The lanes of a wave nearly always belong to one problem: the wave's
 // sums are then added by one lane. Call from all lanes of the wave.
@@ -74,16 +80,17 @@ __device__ inline void add_prob_report(const Prob *__restrict__ P, uint32_t p, b
     const int leader = __ffsll((unsigned long long)act) - 1;
     const uint32_t p0 = __shfl(p, leader, 64);
     const bool same = __ballot(active && p != p0) == 0;
+    const uint32_t slot = report_slot() + (uint32_t)k0;
     if (same) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const unsigned long long s = wave_reduce_add_u64(active ? v[k] : 0ull);
-            if ((int)__lane_id() == leader && s) atomicAdd(&P[p0].np_report[k0 + k], s);
+            if ((int)__lane_id() == leader && s) atomicAdd(&P[p0].np_report[slot + k], s);
         }
     } else if (active) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            if (v[k]) atomicAdd(&P[p].np_report[k0 + k], v[k]);
+            if (v[k]) atomicAdd(&P[p].np_report[slot + k], v[k]);
     }
 }
 // capacity of the per-interval root list (isolated roots plus roots found
@@ -854,7 +861,7 @@ __device__ inline int np_root_cand(const NarrowArgs &a, uint64_t q, unsigned lon
         if (rep_p != 0xffffffffu) {
 #pragma unroll
             for (int k = 0; k < 5; ++k)
-                if (rep[k]) atomicAdd(&a.P[rep_p].np_report[2 + k], rep[k]);
+                if (rep[k]) atomicAdd(&a.P[rep_p].np_report[report_slot() + 2 + k], rep[k]);
         }
 #pragma unroll
         for (int k = 0; k < 5; ++k) rep[k] = 0;
@@ -995,8 +1002,7 @@ __global__ void k_np_reset(const Prob *__restrict__ P, uint32_t np, unsigned lon
                            unsigned long long *slow_count, unsigned long long *peak)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < np)
-        for (int k = 0; k < MZH_NP_COUNT; ++k) P[t].np_report[k] = 0ull;
+    if (t < np * kReportWords) P[t / kReportWords].np_report[t % kReportWords] = 0ull;
     if (t == 0) {
         *conj_count = 0;
         *steps_count = 0;
@@ -1514,7 +1520,7 @@ hipError_t launch_conj_sort(const Prob *P, uint32_t np, const uint32_t *step_pre
 
 hipError_t launch_narrow_reset(const NarrowArgs &a, hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_np_reset, dim3((a.np + 255) / 256), dim3(256), 0, stream, a.P, a.np, a.conj_count,
+    hipLaunchKernelGGL(k_np_reset, dim3((a.np * kReportWords + 255) / 256), dim3(256), 0, stream, a.P, a.np, a.conj_count,
                        a.steps_count, a.cand_count, a.slow_count, a.peak);
     return hipGetLastError();
 }

@@ -411,8 +411,8 @@ __global__ void k_batch_init(const Prob *__restrict__ P, uint32_t np, const uint
     const uint32_t c = s - q.step0;
 #pragma unroll
     for (int d = 0; d < 8; ++d) q.gbox[c * 8u + d] = (d < 4) ? f2ord(__builtin_inff()) : f2ord(-__builtin_inff());
+    for (uint32_t k = c; k < kReportWords; k += q.C) q.np_report[k] = 0ull;
     if (c == 0) {
-        for (int k = 0; k < MZH_NP_COUNT; ++k) q.np_report[k] = 0ull;
         q.err->code = 0;
         q.err->i = q.err->j = 0u;
         q.err->value = 0.0;

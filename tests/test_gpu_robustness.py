@@ -201,6 +201,6 @@ def test_shared_device_copy_and_hint_release():
     spj = mz.make_sgp4_polyjectory(g, 2460496.5, 2460496.75)[0]
     a = mz.conjunctions(spj, 10.0, 3.0 / 1440.0).conjunctions
     scr = mz.Screener(0)
-    scr.upload(spj)  # an upload of the host copy
+    scr.upload(spj, copy=True)  # an upload of the host copy
     scr.screen(10.0, 3.0 / 1440.0)
     assert len(a) > 0 and np.array_equal(_bits(a), _bits(scr.conjunctions()))
