@@ -93,17 +93,33 @@ def hbm_traffic(kernel_prefix):
 
 
 def copy_calibration(mib):
-    """Plain streaming kernel of known size for the PMC calibration: reads and
-    writes `mib` MiB three times (an elementwise torch kernel, not a DMA copy)."""
+    """Plain streaming kernels of known size for the PMC calibration: this
+    library's own HIP copy kernel (k_stream_copy through mzh_stream_copy, in
+    this process, 3 launches) and a torch elementwise kernel (3 launches), each
+    reading and writing `mib` MiB per launch."""
+    import ctypes
+
     import torch
 
-    n = mib * 1024 * 1024 // 4
+    nbytes = mib * 1024 * 1024
+    lib = ctypes.CDLL(os.path.join(ROOT, "mizuba_amd", "libmizuba_amd.so"))
+    ctx = ctypes.c_void_p()
+    if lib.mzh_create(0, ctypes.byref(ctx)) != 0:
+        raise RuntimeError("mzh_create failed")
+    sec = ctypes.c_double()
+    rc = lib.mzh_stream_copy(ctx, ctypes.c_uint64(nbytes), 3, ctypes.byref(sec))
+    lib.mzh_destroy(ctx)
+    if rc != 0:
+        raise RuntimeError(f"mzh_stream_copy failed ({rc})")
+    n = nbytes // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     for _ in range(3):
         torch.mul(a, 1.0, out=b)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n, "launches": 3}))
+    print(json.dumps({"copy_calibration_bytes_read": nbytes, "copy_calibration_bytes_written": nbytes, "launches": 3,
+                      "k_stream_copy_ms": round(sec.value * 1e3, 4),
+                      "k_stream_copy_gbs": round(2 * nbytes / sec.value / 1e9, 1)}))
 
 
 def batch_workload(args):

@@ -397,6 +397,13 @@ int mzh_release(mzh_ctx *ctx, int what);
 /* Wait for all work queued on the context's stream. */
 int mzh_synchronize(mzh_ctx *ctx);
 
+/* Measurement helper, no counterpart in the reference: `launches` launches of
+ * a plain streaming-copy kernel that reads `bytes` (a multiple of 16) and
+ * writes `bytes` on the context's device, into two scratch buffers allocated
+ * and freed here; *seconds receives the average launch duration. Its hardware
+ * counters (FETCH_SIZE, WRITE_SIZE) calibrate the HBM traffic figures. */
+int mzh_stream_copy(mzh_ctx *ctx, uint64_t bytes, int launches, double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

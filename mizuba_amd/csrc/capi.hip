@@ -301,6 +301,14 @@ void export_nodes(mzh_ctx *ctx)
     ctx->nodes_ready = true;
 }
 
+// Streaming copy of n 16-byte words (the traffic calibration of
+// mzh_stream_copy): every word read once and written once.
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t k = (uint64_t)blockIdx.x * 256u + threadIdx.x; k < n; k += stride) dst[k] = src[k];
+}
+
 } // namespace
 
 extern "C" {
@@ -1538,6 +1546,34 @@ int mzh_release(mzh_ctx *ctx, int what)
 int mzh_synchronize(mzh_ctx *ctx)
 {
     return guarded(ctx, [&] { MZH_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int mzh_stream_copy(mzh_ctx *ctx, uint64_t bytes, int launches, double *seconds)
+{
+    if (!ctx || bytes == 0 || bytes % 16u != 0 || launches < 1) return MZH_EINVAL;
+    return guarded(ctx, [&] {
+        DevBuf src, dst;
+        MZH_HIP(src.ensure(bytes));
+        MZH_HIP(dst.ensure(bytes));
+        MZH_HIP(hipMemsetAsync(src.p, 1, bytes, ctx->stream));
+        hipEvent_t e0, e1;
+        MZH_HIP(hipEventCreate(&e0));
+        MZH_HIP(hipEventCreate(&e1));
+        const uint64_t n = bytes / 16u;
+        const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255u) / 256u, 8192u);
+        MZH_HIP(hipEventRecord(e0, ctx->stream));
+        for (int k = 0; k < launches; ++k)
+            hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, ctx->stream, src.as<const uint4>(),
+                               dst.as<uint4>(), n);
+        MZH_HIP(hipGetLastError());
+        MZH_HIP(hipEventRecord(e1, ctx->stream));
+        MZH_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MZH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        MZH_HIP(hipEventDestroy(e0));
+        MZH_HIP(hipEventDestroy(e1));
+        if (seconds) *seconds = 1e-3 * ms / launches;
+    });
 }
 
 } // extern "C"

@@ -10,36 +10,12 @@
 #include <cstdint>
 
 #include "../../include/mizuba_amd.h"
+#include "mzh_enclosure.hpp"
 
 namespace mzh
 {
 
-constexpr int kMaxOrder = 20;  // highest polynomial order with a compiled kernel
-constexpr int kMinOrder = 2;   // polyjectory.cpp:434-438 rejects order < 2
 constexpr uint32_t kBvhMaxLevels = 72;
-
-// Binomial coefficients C(n, k) for n, k <= kMaxOrder, exact in double.
-struct BinomTable {
-    double v[kMaxOrder + 1][kMaxOrder + 1];
-};
-
-constexpr BinomTable make_binom_table()
-{
-    BinomTable t{};
-    for (int n = 0; n <= kMaxOrder; ++n) {
-        for (int k = 0; k <= kMaxOrder; ++k) {
-            if (k > n) {
-                t.v[n][k] = 0.0;
-            } else {
-                unsigned long long r = 1;
-                const int kk = (k > n - k) ? n - k : k;
-                for (int i = 1; i <= kk; ++i) r = r * (unsigned long long)(n - kk + i) / (unsigned long long)i;
-                t.v[n][k] = (double)r;
-            }
-        }
-    }
-    return t;
-}
 
 // Per-problem parameters passed by value to the kernels.
 struct StepGrid {
