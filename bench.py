@@ -269,6 +269,133 @@ def batch_workload(args):
     print(json.dumps(out))
 
 
+LARGE_CONFIGS = {
+    # name: (objects, of them in the dense shells, days, conj_thresh km, interval min, seed, oracle windows of steps)
+    "config2": (30000, 0, 7.0, 5.0, 3.0, 2026, 48),
+    "config4": (100000, 70000, 1.0, 10.0, 2.0, 2029, 8),
+}
+
+
+def large_config_workload(args):
+    """The precision and dense-shell configurations on their own data: seeded
+    SGP4 mean elements made into order-8 polyjectories by
+    make_sgp4_polyjectory. config2: 30 000 objects (80% LEO, 20% higher) over
+    7 days, 5 km, 3-minute steps (3 360 steps); config4: 100 000 objects, 70 000
+    of them in five near-circular shells (540-580 km, 72 planes, 53 / 97.6 deg)
+    and 30 000 LEO debris, 1 day, 10 km, 2-minute steps. The FP64 drop-in and
+    the mixed pipeline are timed on the resident polyjectory; the oracle
+    screens three windows of steps (start, middle, end), against which both
+    lists are compared: identical (i, j) and the largest TCA / DCA error."""
+    import mizuba_amd as mz
+
+    n, n_shell, days, thresh, minutes, seed, win = LARGE_CONFIGS[args.workload]
+    interval = minutes / 1440.0
+    jd0 = 2460496.5
+    t0 = time.perf_counter()
+    gpes = mz.make_synthetic_gpes(n, seed=seed, frac_high=0.0 if n_shell else 0.2, n_shell=n_shell, epoch_jd=jd0)
+    pj = mz.make_sgp4_polyjectory(gpes, jd0, jd0 + days)[0]
+    print(f"[bench] {args.workload}: {pj.n_objs} objects x {days:g} d in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    scr = mz.Screener(0)
+    scr.upload(pj)
+    res = {}
+    for mode in ("fp64", "mixed"):
+        for _ in range(max(args.warmup, 1)):
+            scr.screen(thresh, interval, mode=mode)
+        t = time.perf_counter()
+        stages = []
+        for _ in range(args.steps):
+            scr.screen(thresh, interval, mode=mode)
+            stages.append(scr.stage_times())
+        dt = (time.perf_counter() - t) / args.steps
+        res[mode] = {
+            "dt": dt,
+            "conj": scr.conjunctions(),
+            "n_pairs": int(scr.n_pairs()),
+            "stages_ms": {k: round(float(np.mean([s_[k] for s_ in stages])) * 1e3, 3) for k in stages[0]},
+        }
+        print(f"[bench] {mode} {dt * 1e3:.2f} ms/step", file=sys.stderr, flush=True)
+    C = int(np.ceil(pj.maxT / interval))
+    obj_days = pj.n_objs * pj.maxT
+    c64, cmx = res["fp64"]["conj"], res["mixed"]["conj"]
+    same_ij = len(c64) == len(cmx) and bool(np.array_equal(c64["i"], cmx["i"]) and np.array_equal(c64["j"], cmx["j"]))
+    out = {
+        "metric": "screened object-days/s; mixed vs FP64 GPU at identical conj. list",
+        "value": round(obj_days / res["mixed"]["dt"], 1),
+        "unit": "object-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["mixed"]["dt"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 screen + f64 narrow phase",
+        "data": "synthetic (seeded SGP4 mean elements, polyjectory from this library's GPU make_sgp4_polyjectory, "
+                "order 8)",
+        "config": {
+            "workload": f"{args.workload}: {pj.n_objs} objects ({n_shell} in dense shells) x {days:g} d, "
+                        f"conj_thresh {thresh:g} km, conj_det_interval {minutes:g} min",
+            "n_objs": pj.n_objs,
+            "n_cd_steps": C,
+            "seed": seed,
+        },
+        "fp64_dropin": {"value": round(obj_days / res["fp64"]["dt"], 1),
+                        "ms_per_step": round(res["fp64"]["dt"] * 1e3, 3),
+                        "stages_ms": res["fp64"]["stages_ms"], "n_pairs": res["fp64"]["n_pairs"]},
+        "mixed": {"stages_ms": res["mixed"]["stages_ms"], "n_pairs": res["mixed"]["n_pairs"]},
+        "mixed_over_fp64": round(res["fp64"]["dt"] / res["mixed"]["dt"], 3),
+        "identical_conj_list": same_ij,
+        "identical_conj_bits": bool(same_ij and np.array_equal(c64.view(np.uint8), cmx.view(np.uint8))),
+        "n_conj": int(len(cmx)),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+
+        traj, tm, toff, ns, tmo = pj._flat()
+        flat = po.Flat.from_arrays(pj.poly_order, traj, tm, ns, toff, tmo, pj.maxT)
+        threads = po.n_threads_default()
+        cd_end = np.minimum(interval * np.arange(1, C + 1, dtype=np.float64), pj.maxT)
+        cd_end[-1] = pj.maxT
+        windows = [(0, win), (C // 2 - win // 2, C // 2 - win // 2 + win), (C - win, C)]
+        err = {"fp64": [0.0, 0.0, True, True], "mixed": [0.0, 0.0, True, True]}
+        n_ref = 0
+        t_cpu = 0.0
+        for k0, k1 in windows:
+            t = time.perf_counter()
+            ref = po.run(flat, thresh, interval, step_begin=k0, step_end=k1, n_threads=threads)
+            t_cpu += time.perf_counter() - t
+            r = ref["conjunctions"]
+            n_ref += len(r)
+            t0w = 0.0 if k0 == 0 else cd_end[k0 - 1]
+            t1w = cd_end[k1 - 1]
+            for mode, cj in (("fp64", c64), ("mixed", cmx)):
+                upper = (cj["tca"] <= t1w) if k1 == C else (cj["tca"] < t1w)
+                g = cj[(cj["tca"] >= t0w) & upper]
+                ok = len(g) == len(r) and np.array_equal(g["i"], r["i"]) and np.array_equal(g["j"], r["j"])
+                e = err[mode]
+                e[2] = e[2] and bool(ok)
+                e[3] = e[3] and bool(ok and np.array_equal(g.view(np.uint8), r.view(np.uint8)))
+                if ok and len(r):
+                    e[0] = max(e[0], float(np.max(np.abs(g["tca"] - r["tca"]))) * 86400.0)
+                    e[1] = max(e[1], float(np.max(np.abs(g["dca"] - r["dca"]))))
+        samp_days = sum(min(k1 * interval, pj.maxT) - k0 * interval for k0, k1 in windows)
+        out["oracle_check"] = {
+            "windows": windows,
+            "n_conj": int(n_ref),
+            **{f"{m}_identical_ij": err[m][2] for m in err},
+            **{f"{m}_bitwise": err[m][3] for m in err},
+            **{f"{m}_max_dtca_s": err[m][0] for m in err},
+            **{f"{m}_max_ddca_km": err[m][1] for m in err},
+        }
+        out["cpu_baseline"] = {"value": round(pj.n_objs * samp_days / t_cpu, 2), "unit": "object-days/s",
+                               "cores": threads, "kind": "port", "cpu": cpu_model(),
+                               "sample": f"3 windows of {win} conjunction steps ({samp_days:.4f} days) of the same "
+                                         f"polyjectory, wall {t_cpu:.2f}s; the oracle runs its steps in parallel "
+                                         "(OpenMP over steps)"}
+    print(json.dumps(out))
+
+
 def cpu_model():
     """The host CPU's model name (/proc/cpuinfo), for the CPU baseline."""
     try:
@@ -595,7 +722,8 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=0, help="conjunction steps timed on the CPU (0: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0, help="only run the PMC copy calibration")
-    ap.add_argument("--workload", choices=("headline", "batch64", "state_eval", "sgp4"), default="headline")
+    ap.add_argument("--workload", choices=("headline", "batch64", "state_eval", "sgp4", "config2", "config4"),
+                    default="headline")
     ap.add_argument("--catalog", choices=("kepler", "sgp4"), default="sgp4",
                     help="headline catalog: J2-secular Keplerian fits, or SGP4 mean elements through make_sgp4_polyjectory")
     args = ap.parse_args()
@@ -619,6 +747,9 @@ def main():
         return
     if args.workload == "state_eval":
         state_eval_workload(args)
+        return
+    if args.workload in LARGE_CONFIGS:
+        large_config_workload(args)
         return
 
     import mizuba_amd as mz

@@ -236,7 +236,7 @@ void ScreenBatch::prepare()
         if (ctx->ovf_cap == 0)
             ctx->ovf_cap = (uint32_t)std::min<uint64_t>(env_u64("MIZUBA_AMD_INIT_OVF_CAP", NC / 8 + 4096), 0x7fffffffu);
         MZH_HIP(ctx->ovf.ensure(sizeof(uint2) * ctx->ovf_cap));
-        MZH_HIP(ctx->ovf_count.ensure(sizeof(uint32_t)));
+        MZH_HIP(ctx->ovf_count.ensure(sizeof(uint32_t) * kPoolShards * kPoolStride));
         MZH_HIP(ctx->off.ensure(sizeof(uint64_t) * NC));
         MZH_HIP(ctx->step_total.ensure(sizeof(uint64_t) * C));
         MZH_HIP(ctx->step_off.ensure(sizeof(uint64_t) * (C + 1)));

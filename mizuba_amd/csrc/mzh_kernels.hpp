@@ -33,6 +33,15 @@ constexpr uint32_t kFlagPairs = 1u;  // more pairs than pair_cap
 constexpr uint32_t kFlagOvf = 2u;    // broad-phase overflow pool full
 constexpr uint32_t kFlagConj = 4u;   // more conjunctions than conj_out_cap
 
+// The broad-phase overflow pool is split into kPoolShards equal parts
+// (ovf_cap / kPoolShards entries each) with a counter each, one 128-byte line
+// apart: a block takes entries from the part its index picks, one atomic per
+// wave, so that a step with many pairs per object does not queue every entry
+// on one address. The pool a screening needs is kPoolShards times its most
+// used part.
+constexpr uint32_t kPoolShards = 64;
+constexpr uint32_t kPoolStride = 32;
+
 // Narrow-phase report counters of a problem (Prob::np_report): kReportSlots
 // copies of the MZH_NP_COUNT counters, one 128-byte line each. A wave adds its
 // sums to the copy its index picks, so that the additions of a launch spread
@@ -66,7 +75,7 @@ struct Prob {
     gbuf<const int32_t> otypes;  // [N]
     gbuf<uint32_t> cnt, head, slots;  // [C][N], [C][N], [C][N][4]
     gbuf<uint2> ovf;  // overflow pool {partner, next}
-    gbuf<uint32_t> ovf_count;
+    gbuf<uint32_t> ovf_count;  // [kPoolShards][kPoolStride]: entries taken from each part of the pool
     uint32_t ovf_cap;
     gbuf<uint64_t> off;  // [C][N] offset of each object's pairs in its step
     gbuf<uint64_t> step_total;  // [C]

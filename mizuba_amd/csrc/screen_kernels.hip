@@ -268,12 +268,12 @@ __global__ void k_batch_init(const Prob *__restrict__ P, uint32_t np, const uint
 #pragma unroll
     for (int d = 0; d < 8; ++d) q.gbox[c * 8u + d] = (d < 4) ? f2ord(__builtin_inff()) : f2ord(-__builtin_inff());
     for (uint32_t k = c; k < kReportWords; k += q.C) q.np_report[k] = 0ull;
+    for (uint32_t k = c; k < kPoolShards; k += q.C) q.ovf_count[k * kPoolStride] = 0u;
     if (c == 0) {
         q.err->code = 0;
         q.err->i = q.err->j = 0u;
         q.err->value = 0.0;
         *q.flags = 0u;
-        *q.ovf_count = 0u;
         for (int k = 0; k < 4; ++k) q.counts[k] = 0ull;
     }
 }
@@ -1009,16 +1009,35 @@ __device__ inline bool box_overlap(const float4 &ql, const float4 &qu, const flo
 // Record the broad-phase pair (i, j), i < j: count it for i, keep the first
 // kBpSlots partners inline, and push the others on i's list in the overflow
 // pool (entry = partner, next entry; head[step][i] = latest entry).
-__device__ inline void bp_emit(uint32_t c, uint32_t i, uint32_t j, uint32_t N, uint32_t *cnt, uint32_t *slots,
-                               uint2 *ovf, uint32_t *head, uint32_t *ovf_count, uint32_t ovf_cap)
+//
+// Called by the lanes of a wave that record a pair (any subset of the wave):
+// the pool entries of the lanes that need one are taken with one atomic on the
+// counter of the block's part of the pool.
+__device__ inline void bp_emit(bool has, uint32_t c, uint32_t i, uint32_t j, uint32_t N, uint32_t *cnt,
+                               uint32_t *slots, uint2 *ovf, uint32_t *head, uint32_t *ovf_count, uint32_t ovf_cap)
 {
     const size_t g = (size_t)c * N + i;
-    const uint32_t k = atomicAdd(&cnt[g], 1u);
-    if (k < kBpSlots) {
-        slots[g * kBpSlots + k] = j;
-    } else {
-        const uint32_t e = atomicAdd(ovf_count, 1u);
-        if (e < ovf_cap) {
+    bool need = false;
+    if (has) {
+        const uint32_t k = atomicAdd(&cnt[g], 1u);
+        if (k < kBpSlots) {
+            slots[g * kBpSlots + k] = j;
+        } else {
+            need = true;
+        }
+    }
+    const uint64_t m = __ballot(need);
+    if (m == 0) return;
+    const uint32_t part = blockIdx.x & (kPoolShards - 1u);
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(&ovf_count[part * kPoolStride], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (need) {
+        const uint32_t loc = base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+        const uint32_t part_cap = ovf_cap / kPoolShards;
+        if (loc < part_cap) {
+            const uint32_t e = part * part_cap + loc;
             const uint32_t next = atomicExch(&head[g], e);
             ovf[e] = make_uint2(j, next);
         }
@@ -1090,16 +1109,16 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
     // partners found during the walk wait in registers and are recorded
     // together (all lanes at once) when the buffer fills and at the end
     constexpr uint32_t kBuf = 8;
-    uint32_t buf[kBuf];
+    uint32_t buf[kBuf] = {};
     uint32_t nbuf = 0;
     auto flush = [&]() {
 #pragma unroll
-        for (uint32_t k = 0; k < kBuf; ++k)
-            if (k < nbuf) {
-                const uint32_t oi = buf[k];
-                bp_emit(c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, head, ovf_count,
-                        ovf_cap);
-            }
+        for (uint32_t k = 0; k < kBuf; ++k) {
+            if (__ballot(k < nbuf) == 0) break;  // the calling lanes' buffers are done
+            const uint32_t oi = buf[k];
+            bp_emit(k < nbuf, c, orig < oi ? orig : oi, orig < oi ? oi : orig, N, cnt, slots, ovf, head, ovf_count,
+                    ovf_cap);
+        }
         nbuf = 0;
     };
     auto add = [&](uint32_t oi) {
@@ -1282,15 +1301,23 @@ __global__ __launch_bounds__(1024) void k_scan_steps(const Prob *__restrict__ P)
 }
 
 // Pair and pool counts of every problem, their overflow flags, and the
-// exclusive prefix of the placed pairs over the batch (one thread).
+// exclusive prefix of the placed pairs over the batch (one wave: the pool
+// count a problem needs is kPoolShards times the count of its most used part).
 __global__ void k_pair_prefix(const Prob *__restrict__ P, uint32_t np, uint64_t *__restrict__ pair_pre)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (blockIdx.x != 0 || threadIdx.x >= 64) return;
     uint64_t run = 0;
     for (uint32_t p = 0; p < np; ++p) {
         const Prob &q = P[p];
+        uint32_t mp = threadIdx.x < kPoolShards ? q.ovf_count[threadIdx.x * kPoolStride] : 0u;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = __shfl_xor(mp, off, 64);
+            mp = mp > o ? mp : o;
+        }
+        if (threadIdx.x != 0) continue;
         const uint64_t n = q.step_off[q.C];
-        const uint32_t nov = *q.ovf_count;
+        const uint64_t nov = (uint64_t)mp * kPoolShards;
         q.counts[0] = n;
         q.counts[1] = nov;
         uint32_t f = 0;
@@ -1301,7 +1328,7 @@ __global__ void k_pair_prefix(const Prob *__restrict__ P, uint32_t np, uint64_t 
         // an incomplete pair list is not screened
         run += (f == 0) ? n : 0;
     }
-    pair_pre[np] = run;
+    if (threadIdx.x == 0) pair_pre[np] = run;
 }
 
 // Broad-phase counts and list heads start empty (one thread per (step, object)).
