@@ -42,6 +42,10 @@ extern "C" {
 #define MZH_ERANGE 2
 #define MZH_ERUNTIME 3
 #define MZH_ENOMEM 4
+/* A screening too large for one batch (objects x conjunction steps above
+ * 2^32 - 1): std::invalid_argument (ValueError); mzh_screen_steps screens
+ * such a problem in ranges of steps. */
+#define MZH_ESIZE 5
 
 /* Precision modes of the screen (AABB -> broad phase).
  * MZH_MODE_FP64 : drop-in arithmetic of the reference -- FP64 Cargo-Shisha
@@ -239,11 +243,39 @@ int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const
  * without a context or a device: the threshold, the interval, the object
  * types (otypes may be NULL) and the number of conjunction steps
  * ceil(maxT / conj_det_interval), returned in *n_cd_steps. Also checks this
- * build's size limit: n_objs * n_cd_steps must not exceed 2^32 - 1. Same
- * status and message (mzh_global_error) as mzh_screen for the same
- * arguments. */
+ * build's one-batch size limit: n_objs * n_cd_steps must not exceed
+ * 2^32 - 1 (MZH_ESIZE, with *n_cd_steps set). Same status and message
+ * (mzh_global_error) as mzh_screen for the same arguments. */
 int mzh_check_screen(uint64_t n_objs, double maxT, double conj_thresh, double conj_det_interval,
                      const int32_t *otypes, int mode, uint64_t *n_cd_steps);
+
+/* mzh_screen restricted to the conjunction steps [step_begin, step_end) of
+ * the screening (0 <= step_begin < step_end <= n_cd_steps): the same step
+ * times (get_cd_begin_end with the whole screening's step count), and the
+ * results of those steps only, as if their rows had been cut out of a whole
+ * screening: the per-step arrays hold step_end - step_begin steps, the
+ * conjunctions are those found in these steps. Screening a problem range by
+ * range and concatenating gives the whole screening's results bit for bit;
+ * the one-batch size limit applies to the range. Replaces the chunked step
+ * loop of detect_conjunctions (mizuba/detect_conjunctions.cpp:61-79, the
+ * chunks of cd_chunk_size steps; :184-242, :313-424, each chunk computed and
+ * then written out before the next), so that a screening needs device memory
+ * for one range of steps only. A range outside the screening is MZH_ERANGE. */
+int mzh_screen_steps(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode,
+                     uint64_t step_begin, uint64_t step_end);
+
+/* Free and total memory of the context's device, in bytes. */
+int mzh_device_memory(mzh_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* Conjunction steps per range for a screening of n_objs objects over
+ * n_cd_steps steps whose device buffers (the per-step arrays, trees, pair and
+ * sort scratch: about MZH_SCREEN_BYTES_PER_OBJECT_STEP bytes per object and
+ * step) must fit in device_bytes: at least 1, at most n_cd_steps, and small
+ * enough for the one-batch size limit. n_cd_steps itself means one batch. The
+ * reference's counterpart is its fixed chunk of 128 steps
+ * (mizuba/detect_conjunctions.cpp:79). */
+#define MZH_SCREEN_BYTES_PER_OBJECT_STEP 400
+int mzh_chunk_steps(uint64_t n_objs, uint64_t n_cd_steps, uint64_t device_bytes, uint64_t *chunk_steps);
 
 /* Screen n independent problems together: ctxs[k] (each with its own uploaded
  * polyjectory, all on one device) is screened with conj_thresh[k],

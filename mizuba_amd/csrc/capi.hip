@@ -199,13 +199,22 @@ namespace
 namespace mzh::host
 {
 
+// This build's size limit of one batch: the per-step stages index (step,
+// object) with 32-bit counts, so N * C must stay below 2^32 (MZH_ESIZE).
+void check_batch_size(uint64_t N, uint64_t C)
+{
+    if (N * C > 0xffffffffull)
+        throw StatusError{MZH_ESIZE, "A screening of " + std::to_string(N) + " objects over " + std::to_string(C)
+                                         + " conjunction steps is too large: the number of objects times the number "
+                                           "of conjunction steps must not exceed 4294967295"};
+}
+
 // conjunctions::conjunctions validation (conjunctions.cpp:279-335) with the
 // reference's messages, the step ranges of get_cd_begin_end
-// (conjunctions.cpp:373-392), and this build's size limit: the per-step
-// stages index (step, object) with 32-bit counts, so N * C must stay below
-// 2^32. Everything is checked before any device memory is touched.
+// (conjunctions.cpp:373-392), and (one_batch) the size limit of one batch.
+// Everything is checked before any device memory is touched.
 ScreenPlan plan_screen(uint32_t N, double maxT, double conj_thresh, double conj_det_interval,
-                       const int32_t *otypes_in, int mode)
+                       const int32_t *otypes_in, int mode, bool one_batch, uint64_t k0, uint64_t k1)
 {
     if (mode != MZH_MODE_FP64 && mode != MZH_MODE_MIXED)
         throw StatusError{MZH_EINVAL, "Invalid screening mode " + std::to_string(mode)};
@@ -240,21 +249,20 @@ ScreenPlan plan_screen(uint32_t N, double maxT, double conj_thresh, double conj_
     if (!(cs >= 1) || cs > 4.0e9)
         throw StatusError{MZH_EINVAL, "The number of conjunction steps " + fmt_double(cs) + " is not representable"};
     pl.C = (uint64_t)cs;
-    if ((uint64_t)N * pl.C > 0xffffffffull)
-        throw StatusError{MZH_EINVAL, "A screening of " + std::to_string(N) + " objects over " + std::to_string(pl.C)
-                                          + " conjunction steps is too large: the number of objects times the number "
-                                            "of conjunction steps must not exceed 4294967295"};
-    // get_cd_begin_end for every step (conjunctions.cpp:373-392)
-    pl.cd_begin.resize(pl.C);
-    pl.cd_end.resize(pl.C);
-    for (uint64_t k = 0; k < pl.C; ++k) {
+    if (one_batch) check_batch_size(N, pl.C);
+    // get_cd_begin_end (conjunctions.cpp:373-392) for the steps [k0, k1)
+    k1 = std::min(k1, pl.C);
+    k0 = std::min(k0, k1);
+    pl.cd_begin.resize(k1 - k0);
+    pl.cd_end.resize(k1 - k0);
+    for (uint64_t k = k0; k < k1; ++k) {
         const double b = conj_det_interval * static_cast<double>(k);
         const double e = (k == pl.C - 1) ? maxT : conj_det_interval * static_cast<double>(k + 1);
         if (!std::isfinite(b) || !std::isfinite(e) || !(e > b) || b < 0 || e > maxT)
             throw StatusError{MZH_EINVAL, "Invalid conjunction step time range: [" + fmt_double(b) + ", "
                                               + fmt_double(e) + ")"};
-        pl.cd_begin[k] = b;
-        pl.cd_end[k] = e;
+        pl.cd_begin[k - k0] = b;
+        pl.cd_end[k - k0] = e;
     }
     return pl;
 }
@@ -270,6 +278,29 @@ void run_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, cons
     if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
     set_plan(ctx, plan_screen(ctx->n_objs, ctx->maxT, conj_thresh, conj_det_interval, otypes_in, mode), conj_thresh,
              conj_det_interval, mode);
+    if (!ctx->solo) ctx->solo.reset(batch_new(ctx->stream));
+    int status = MZH_OK;
+    batch_run(ctx->solo.get(), {ctx}, false, false, &status);
+    if (status != MZH_OK) throw StatusError{status, ctx->last_error};
+}
+
+// The conjunction steps [k0, k1) of a screening: the whole screening's plan
+// cut to the range (its step times and object types), a batch of one.
+void run_screen_steps(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes_in, int mode,
+                      uint64_t k0, uint64_t k1)
+{
+    if (!ctx->has_pj) throw StatusError{MZH_EINVAL, "No polyjectory has been uploaded to this context"};
+    // the range's step times only (a screening beyond one batch may have more
+    // steps than host memory holds step times for)
+    ScreenPlan pl =
+        plan_screen(ctx->n_objs, ctx->maxT, conj_thresh, conj_det_interval, otypes_in, mode, false, k0, k1);
+    if (!(k0 < k1 && k1 <= pl.C))
+        throw StatusError{MZH_ERANGE, "Invalid range of conjunction steps [" + std::to_string(k0) + ", "
+                                          + std::to_string(k1) + ") for a screening of " + std::to_string(pl.C)
+                                          + " conjunction steps"};
+    check_batch_size(ctx->n_objs, k1 - k0);
+    pl.C = k1 - k0;
+    set_plan(ctx, std::move(pl), conj_thresh, conj_det_interval, mode);
     if (!ctx->solo) ctx->solo.reset(batch_new(ctx->stream));
     int status = MZH_OK;
     batch_run(ctx->solo.get(), {ctx}, false, false, &status);
@@ -969,8 +1000,13 @@ int mzh_check_screen(uint64_t n_objs, double maxT, double conj_thresh, double co
         if (n_objs > 0xffffffffull)
             throw StatusError{MZH_EINVAL, "Too many objects for one screening: " + std::to_string(n_objs)};
         if (!std::isfinite(maxT) || maxT <= 0) throw StatusError{MZH_EINVAL, "Invalid maxT " + fmt_double(maxT)};
-        const ScreenPlan pl = plan_screen((uint32_t)n_objs, maxT, conj_thresh, conj_det_interval, otypes, mode);
+        // the arguments and the step count, then the limit of one batch, then
+        // (within it) every step's time range
+        const ScreenPlan pl =
+            plan_screen((uint32_t)n_objs, maxT, conj_thresh, conj_det_interval, otypes, mode, false, 0, 0);
         if (n_cd_steps) *n_cd_steps = pl.C;
+        check_batch_size(n_objs, pl.C);
+        (void)plan_screen((uint32_t)n_objs, maxT, conj_thresh, conj_det_interval, otypes, mode);
         return MZH_OK;
     } catch (const StatusError &e) {
         g_global_error = e.msg;
@@ -1068,6 +1104,37 @@ int mzh_root_finding_noalloc(mzh_ctx *ctx, uint32_t order, uint64_t n, const dou
 int mzh_screen(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode)
 {
     return guarded(ctx, [&] { run_screen(ctx, conj_thresh, conj_det_interval, otypes, mode); });
+}
+
+int mzh_screen_steps(mzh_ctx *ctx, double conj_thresh, double conj_det_interval, const int32_t *otypes, int mode,
+                     uint64_t step_begin, uint64_t step_end)
+{
+    return guarded(ctx, [&] {
+        run_screen_steps(ctx, conj_thresh, conj_det_interval, otypes, mode, step_begin, step_end);
+    });
+}
+
+int mzh_device_memory(mzh_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes)
+{
+    return guarded(ctx, [&] {
+        if (!free_bytes || !total_bytes) throw StatusError{MZH_EINVAL, "Null pointer"};
+        MZH_HIP(hipSetDevice(ctx->device));
+        size_t f = 0, t = 0;
+        MZH_HIP(hipMemGetInfo(&f, &t));
+        *free_bytes = f;
+        *total_bytes = t;
+    });
+}
+
+int mzh_chunk_steps(uint64_t n_objs, uint64_t n_cd_steps, uint64_t device_bytes, uint64_t *chunk_steps)
+{
+    if (!chunk_steps || n_cd_steps == 0) return MZH_EINVAL;
+    const uint64_t n = n_objs ? n_objs : 1;
+    uint64_t k = n_cd_steps;
+    k = std::min<uint64_t>(k, 0xffffffffull / n);
+    k = std::min<uint64_t>(k, device_bytes / ((uint64_t)MZH_SCREEN_BYTES_PER_OBJECT_STEP * n));
+    *chunk_steps = std::max<uint64_t>(k, 1);
+    return MZH_OK;
 }
 
 } // extern "C"

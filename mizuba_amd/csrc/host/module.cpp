@@ -9,8 +9,12 @@
 #include <pybind11/stl.h>
 #include <pybind11/stl/filesystem.h>
 
+#include <unistd.h>
+
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <limits>
@@ -56,7 +60,7 @@ void check(int st, const mzh_ctx *ctx)
         mzh_global_error(buf, sizeof(buf));
     }
     const std::string msg(buf);
-    if (st == MZH_EINVAL) throw std::invalid_argument(msg);
+    if (st == MZH_EINVAL || st == MZH_ESIZE) throw std::invalid_argument(msg);
     if (st == MZH_ERANGE) throw std::out_of_range(msg);
     if (st == MZH_ENOMEM) throw std::bad_alloc();
     throw std::runtime_error(msg);
@@ -753,6 +757,11 @@ struct ConjImpl {
     // data dir holding the result files, removed with the object
     // (conjunctions.cpp:200-233); empty: results in HBM and host caches only
     std::filesystem::path data_dir;
+    // screened in ranges of chunk_steps conjunction steps: every result is in
+    // the host caches, and the counters and stage times are the ranges' sums
+    uint64_t chunk_steps = 0;
+    uint64_t np[MZH_NP_COUNT] = {};
+    double stage[MZH_STAGE_COUNT] = {};
     ConjImpl() = default;
     ConjImpl(const ConjImpl &) = delete;
     ConjImpl &operator=(const ConjImpl &) = delete;
@@ -813,6 +822,32 @@ void store_conjunctions(ConjImpl &im, const std::filesystem::path &dir)
 {
     mzh_ctx *ctx = im.ctx->ctx;
     const uint64_t C = im.C, N = im.N;
+    if (im.chunk_steps) {
+        // a screening in ranges of steps keeps every result in the host caches
+        mzh_host::write_file_ro(dir / "aabbs", im.aabbs->data(), C * (N + 1) * 8 * sizeof(float));
+        mzh_host::write_file_ro(dir / "srt_aabbs", im.srt_aabbs->data(), C * (N + 1) * 8 * sizeof(float));
+        mzh_host::write_file_ro(dir / "mcodes", im.mcodes->data(), C * N * sizeof(uint64_t));
+        mzh_host::write_file_ro(dir / "srt_mcodes", im.srt_mcodes->data(), C * N * sizeof(uint64_t));
+        mzh_host::write_file_ro(dir / "vidx", im.srt_idx->data(), C * N * sizeof(uint32_t));
+        std::vector<mzh_bvh_node> all_t;
+        std::vector<mzh_aabb_collision> all_p;
+        for (uint64_t k = 0; k < C; ++k) {
+            const auto &t = im.trees.at(k);
+            all_t.insert(all_t.end(), t.begin(), t.end());
+            const auto &b = im.bps.at(k);
+            all_p.insert(all_p.end(), b.begin(), b.end());
+        }
+        mzh_host::write_file_ro(dir / "bvh", all_t.data(), all_t.size() * sizeof(mzh_bvh_node));
+        if (all_p.empty()) all_p.push_back(mzh_aabb_collision{});
+        mzh_host::write_file_ro(dir / "bp", all_p.data(), all_p.size() * sizeof(mzh_aabb_collision));
+        if (!im.conjs->empty()) {
+            mzh_host::write_file_ro(dir / "conjunctions", im.conjs->data(), im.conjs->size() * sizeof(mzh_conj));
+        } else {
+            const char empty = 0;
+            mzh_host::write_file_ro(dir / "conjunctions", &empty, 1);
+        }
+        return;
+    }
     {
         std::vector<float> v(C * (N + 1) * 8);
         check(mzh_copy_aabbs(ctx, v.data()), ctx);
@@ -874,6 +909,7 @@ void store_conjunctions(ConjImpl &im, const std::filesystem::path &dir)
 // accessors read from then on, and the device copies are freed.
 void conj_hint_release(ConjImpl &im)
 {
+    if (im.chunk_steps) return;  // already in host memory
     mzh_ctx *ctx = im.ctx->ctx;
     lazy<float>(im, im.aabbs, im.C * (im.N + 1) * 8, [&](float *p) { return mzh_copy_aabbs(ctx, p); });
     lazy<float>(im, im.srt_aabbs, im.C * (im.N + 1) * 8, [&](float *p) { return mzh_copy_srt_aabbs(ctx, p); });
@@ -917,15 +953,89 @@ void check_otypes_size(const std::optional<std::vector<int32_t>> &otypes, uint64
 
 // The constructor's checks in the reference's order (conjunctions.cpp:267-332:
 // threshold, interval, then the size and values of otypes), before any device
-// context is made; then this build's size limit (mzh_check_screen).
-void validate_screen(const PjData &pd, double conj_thresh, double conj_det_interval,
-                     const std::optional<std::vector<int32_t>> &otypes, int mode)
+// context is made; then this build's size limit of one batch
+// (mzh_check_screen), an error unless `ranges` (then the screening runs in
+// ranges of steps). Returns the number of conjunction steps.
+uint64_t validate_screen(const PjData &pd, double conj_thresh, double conj_det_interval,
+                         const std::optional<std::vector<int32_t>> &otypes, int mode, bool ranges = false)
 {
-    check(mzh_check_screen(pd.n_objs, pd.maxT, conj_thresh, conj_det_interval, nullptr, mode, nullptr), nullptr);
+    uint64_t C = 0;
+    auto run = [&](const int32_t *ot) {
+        const int st = mzh_check_screen(pd.n_objs, pd.maxT, conj_thresh, conj_det_interval, ot, mode, &C);
+        if (!(st == MZH_ESIZE && ranges)) check(st, nullptr);
+    };
+    run(nullptr);
     check_otypes_size(otypes, pd.n_objs);
-    if (otypes)
-        check(mzh_check_screen(pd.n_objs, pd.maxT, conj_thresh, conj_det_interval, otypes->data(), mode, nullptr),
-              nullptr);
+    if (otypes) run(otypes->data());
+    return C;
+}
+
+// Screen the problem in ranges of K conjunction steps (mzh_screen_steps),
+// each range's results copied into the host caches at their place before the
+// next range is screened: the device holds one range at a time, the
+// reference's chunked loop (detect_conjunctions.cpp:61-79, 184-242), with the
+// results in host memory instead of files. The results are the one-batch
+// screening's, bit for bit.
+void screen_in_ranges(ConjImpl &im, const PjData &pd, double conj_thresh, double conj_det_interval,
+                      const std::optional<std::vector<int32_t>> &otypes, int m, uint64_t C, uint64_t K)
+{
+    mzh_ctx *ctx = im.ctx->ctx;
+    const uint64_t N = pd.n_objs;
+    im.C = C;
+    im.N = N;
+    im.chunk_steps = K;
+    im.aabbs = std::make_unique<std::vector<float>>(C * (N + 1) * 8);
+    im.srt_aabbs = std::make_unique<std::vector<float>>(C * (N + 1) * 8);
+    im.mcodes = std::make_unique<std::vector<uint64_t>>(C * N ? C * N : 1);
+    im.srt_mcodes = std::make_unique<std::vector<uint64_t>>(C * N ? C * N : 1);
+    im.srt_idx = std::make_unique<std::vector<uint32_t>>(C * N ? C * N : 1);
+    im.conjs = std::make_unique<std::vector<mzh_conj>>();
+    im.cd_end_times.resize(C);
+    const int32_t *ot = otypes ? otypes->data() : nullptr;
+    py::gil_scoped_release rel;
+    for (uint64_t k0 = 0; k0 < C; k0 += K) {
+        const uint64_t k1 = std::min(C, k0 + K), nk = k1 - k0;
+        check(mzh_screen_steps(ctx, conj_thresh, conj_det_interval, ot, m, k0, k1), ctx);
+        check(mzh_copy_cd_end_times(ctx, im.cd_end_times.data() + k0), ctx);
+        check(mzh_copy_aabbs(ctx, im.aabbs->data() + k0 * (N + 1) * 8), ctx);
+        check(mzh_copy_srt_aabbs(ctx, im.srt_aabbs->data() + k0 * (N + 1) * 8), ctx);
+        check(mzh_copy_mcodes(ctx, im.mcodes->data() + k0 * N), ctx);
+        check(mzh_copy_srt_mcodes(ctx, im.srt_mcodes->data() + k0 * N), ctx);
+        check(mzh_copy_srt_idx(ctx, im.srt_idx->data() + k0 * N), ctx);
+        for (uint64_t k = 0; k < nk; ++k) {
+            uint64_t n = 0;
+            check(mzh_get_bvh_tree_size(ctx, k, &n), ctx);
+            std::vector<mzh_bvh_node> t(n ? n : 1);
+            if (n) check(mzh_copy_bvh_tree(ctx, k, t.data()), ctx);
+            t.resize(n);
+            im.trees[k0 + k] = std::move(t);
+            check(mzh_get_aabb_collisions_size(ctx, k, &n), ctx);
+            std::vector<mzh_aabb_collision> b(n ? n : 1);
+            if (n) check(mzh_copy_aabb_collisions(ctx, k, b.data()), ctx);
+            b.resize(n);
+            im.bps[k0 + k] = std::move(b);
+        }
+        uint64_t nc = 0;
+        check(mzh_get_n_conjunctions(ctx, &nc), ctx);
+        const size_t at = im.conjs->size();
+        im.conjs->resize(at + nc);
+        if (nc) check(mzh_copy_conjunctions(ctx, im.conjs->data() + at), ctx);
+        uint64_t r[MZH_NP_COUNT];
+        check(mzh_get_np_report(ctx, r), ctx);
+        for (int j = 0; j < MZH_NP_COUNT; ++j) im.np[j] += r[j];
+        double t[MZH_STAGE_COUNT];
+        check(mzh_get_stage_times(ctx, t), ctx);
+        for (int j = 0; j < MZH_STAGE_COUNT; ++j) im.stage[j] += t[j];
+    }
+    check(mzh_release(ctx, MZH_RELEASE_SCRATCH | MZH_RELEASE_RESULTS | MZH_RELEASE_POLYJECTORY), ctx);
+}
+
+// Host bytes that a screening's results take in host memory: the per-step
+// arrays, about two 48-byte tree nodes per object and step, and the pairs
+// (estimated at one per object and step).
+uint64_t result_host_bytes(uint64_t N, uint64_t C)
+{
+    return C * ((N + 1) * 64 + N * (8 + 8 + 4 + 96 + 8));
 }
 
 Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double conj_det_interval,
@@ -938,7 +1048,19 @@ Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double
             "The 'data_dir' and 'tmpdir' construction arguments cannot be provided at the same time");
     const int m = parse_mode(mode);
     const PjData &pd = attached(pj);
-    validate_screen(pd, conj_thresh, conj_det_interval, otypes, m);
+    const uint64_t C = validate_screen(pd, conj_thresh, conj_det_interval, otypes, m, true);
+    // a screening beyond one batch's limit or the device's free memory runs
+    // in ranges of steps, its results in host memory, which must hold them
+    auto check_host = [&]() {
+        const uint64_t need = result_host_bytes(pd.n_objs, C);
+        const uint64_t avail = (uint64_t)sysconf(_SC_AVPHYS_PAGES) * (uint64_t)sysconf(_SC_PAGESIZE);
+        if (need > avail)
+            throw std::invalid_argument("A screening of " + std::to_string(pd.n_objs) + " objects over "
+                                        + std::to_string(C) + " conjunction steps is too large: its results need about "
+                                        + std::to_string(need) + " bytes of host memory, more than the "
+                                        + std::to_string(avail) + " bytes available");
+    };
+    if (C > 0xffffffffull / (pd.n_objs ? pd.n_objs : 1)) check_host();
     auto im = std::make_shared<ConjImpl>();
     // the result files are written when a data dir or a temporary dir is
     // given; the dir is made first, as the reference does (conjunctions.cpp:336-341)
@@ -949,8 +1071,30 @@ Conjunctions make_conjunctions(const Polyjectory &pj, double conj_thresh, double
     }
     im->ctx = std::make_unique<Ctx>(device);
     im->ctx->share(pj.d, device);
-    im->ctx->screen(conj_thresh, conj_det_interval, otypes, m);
-    finish_impl(*im, pj, conj_thresh, conj_det_interval, otypes, m);
+    // conjunction steps per range: what one batch's limit and half the
+    // device's free memory allow (MIZUBA_AMD_CHUNK_STEPS caps it)
+    uint64_t K = C;
+    {
+        uint64_t free_b = 0, total_b = 0;
+        check(mzh_device_memory(im->ctx->ctx, &free_b, &total_b), im->ctx->ctx);
+        check(mzh_chunk_steps(pd.n_objs, C, free_b / 2, &K), nullptr);
+        if (const char *e = std::getenv("MIZUBA_AMD_CHUNK_STEPS"); e && *e) {
+            const uint64_t cap = std::strtoull(e, nullptr, 10);
+            if (cap) K = std::min(K, cap);
+        }
+    }
+    if (K < C) {
+        check_host();
+        screen_in_ranges(*im, pd, conj_thresh, conj_det_interval, otypes, m, C, K);
+        im->pj = pj.d;
+        im->conj_thresh = conj_thresh;
+        im->conj_det_interval = conj_det_interval;
+        im->mode = m;
+        im->otypes = otypes ? *otypes : std::vector<int32_t>(pd.n_objs, 1);
+    } else {
+        im->ctx->screen(conj_thresh, conj_det_interval, otypes, m);
+        finish_impl(*im, pj, conj_thresh, conj_det_interval, otypes, m);
+    }
     if (!im->data_dir.empty()) {
         py::gil_scoped_release rel;
         store_conjunctions(*im, im->data_dir);
@@ -1293,11 +1437,29 @@ PYBIND11_MODULE(_core, m)
         .def(
             "screen",
             [](Ctx &c, double thresh, double interval, std::optional<std::vector<int32_t>> otypes,
-               const std::string &mode) {
+               const std::string &mode, std::optional<std::pair<uint64_t, uint64_t>> steps) {
                 if (!c.pj) throw std::invalid_argument("No polyjectory has been uploaded");
-                c.screen(thresh, interval, otypes, parse_mode(mode));
+                if (!steps) {
+                    c.screen(thresh, interval, otypes, parse_mode(mode));
+                    return;
+                }
+                // the conjunction steps [steps[0], steps[1]) only (mzh_screen_steps)
+                if (otypes && otypes->size() != c.pj->n_objs)
+                    throw std::invalid_argument("Invalid array of object types passed to the constructor of a "
+                                                "conjunctions objects: the expected size is "
+                                                + std::to_string(c.pj->n_objs) + ", but the actual size is "
+                                                + std::to_string(otypes->size()) + " instead");
+                const int m = parse_mode(mode);
+                int st;
+                {
+                    py::gil_scoped_release rel;
+                    st = mzh_screen_steps(c.ctx, thresh, interval, otypes ? otypes->data() : nullptr, m, steps->first,
+                                          steps->second);
+                }
+                check(st, c.ctx);
             },
-            "conj_thresh"_a, "conj_det_interval"_a, "otypes"_a = py::none(), "mode"_a = "fp64")
+            "conj_thresh"_a, "conj_det_interval"_a, "otypes"_a = py::none(), "mode"_a = "fp64",
+            "steps"_a = py::none())
         .def("stage_times",
              [](Ctx &c) {
                  double t[MZH_STAGE_COUNT];
@@ -1579,6 +1741,10 @@ PYBIND11_MODULE(_core, m)
         auto &im = *self.cast<const Conjunctions &>().impl;
         std::lock_guard<std::mutex> lk(im.m);
         auto it = im.trees.find(i);
+        if (it == im.trees.end() && im.chunk_steps)
+            throw std::out_of_range("Cannot fetch the BVH tree for the conjunction timestep at index "
+                                    + std::to_string(i) + ": the total number of conjunction steps is only "
+                                    + std::to_string(im.C));
         if (it == im.trees.end()) {
             uint64_t n = 0;
             check(mzh_get_bvh_tree_size(im.ctx->ctx, i, &n), im.ctx->ctx);
@@ -1593,6 +1759,10 @@ PYBIND11_MODULE(_core, m)
         auto &im = *self.cast<const Conjunctions &>().impl;
         std::lock_guard<std::mutex> lk(im.m);
         auto it = im.bps.find(i);
+        if (it == im.bps.end() && im.chunk_steps)
+            throw std::out_of_range("Cannot fetch the list of AABB collisions for the conjunction timestep at index "
+                                    + std::to_string(i) + ": the total number of conjunction steps is only "
+                                    + std::to_string(im.C));
         if (it == im.bps.end()) {
             uint64_t n = 0;
             check(mzh_get_aabb_collisions_size(im.ctx->ctx, i, &n), im.ctx->ctx);
@@ -1606,7 +1776,11 @@ PYBIND11_MODULE(_core, m)
     cj.def_property_readonly("conjunctions", [](const py::object &self) {
         auto &im = *self.cast<const Conjunctions &>().impl;
         uint64_t n = 0;
-        check(mzh_get_n_conjunctions(im.ctx->ctx, &n), im.ctx->ctx);
+        if (im.chunk_steps) {
+            n = im.conjs->size();
+        } else {
+            check(mzh_get_n_conjunctions(im.ctx->ctx, &n), im.ctx->ctx);
+        }
         const auto &v = lazy<mzh_conj>(im, im.conjs, n, [&](mzh_conj *p) { return mzh_copy_conjunctions(im.ctx->ctx, p); });
         return ro_view_dtype(self, conj_dtype(), v.data(), (py::ssize_t)n);
     });
@@ -1623,7 +1797,11 @@ PYBIND11_MODULE(_core, m)
     });
     cj.def_property_readonly("np_report", [](const Conjunctions &c) {
         uint64_t r[MZH_NP_COUNT];
-        check(mzh_get_np_report(c.impl->ctx->ctx, r), c.impl->ctx->ctx);
+        if (c.impl->chunk_steps) {
+            std::copy(c.impl->np, c.impl->np + MZH_NP_COUNT, r);
+        } else {
+            check(mzh_get_np_report(c.impl->ctx->ctx, r), c.impl->ctx->ctx);
+        }
         py::dict d;
         const char *names[] = {"n_tot_conj_candidates", "n_dist2_check",     "n_poly_roots",
                                "n_fex_check",           "n_poly_no_roots",   "n_tot_dist_minima",
@@ -1631,9 +1809,15 @@ PYBIND11_MODULE(_core, m)
         for (int k = 0; k < MZH_NP_COUNT; ++k) d[names[k]] = r[k];
         return d;
     });
+    cj.def_property_readonly("chunk_steps", [](const Conjunctions &c) { return c.impl->chunk_steps; },
+                             "Conjunction steps per range of a screening run in ranges (0: one batch).");
     cj.def_property_readonly("stage_times", [](const Conjunctions &c) {
         double t[MZH_STAGE_COUNT];
-        check(mzh_get_stage_times(c.impl->ctx->ctx, t), c.impl->ctx->ctx);
+        if (c.impl->chunk_steps) {
+            std::copy(c.impl->stage, c.impl->stage + MZH_STAGE_COUNT, t);
+        } else {
+            check(mzh_get_stage_times(c.impl->ctx->ctx, t), c.impl->ctx->ctx);
+        }
         py::dict d;
         const char *names[] = {"aabbs", "morton", "bvh", "broad", "narrow", "total", "aabb_kernel", "narrow_kernel", "broad_kernel"};
         for (int k = 0; k < MZH_STAGE_COUNT; ++k) d[names[k]] = t[k];

@@ -219,7 +219,10 @@ struct ScreenPlan {
 // reference's messages, the step ranges of get_cd_begin_end
 // (conjunctions.cpp:373-392), and this build's size limit.
 ScreenPlan plan_screen(uint32_t N, double maxT, double conj_thresh, double conj_det_interval,
-                       const int32_t *otypes_in, int mode);
+                       const int32_t *otypes_in, int mode, bool one_batch = true, uint64_t k0 = 0,
+                       uint64_t k1 = ~0ull);
+// The size limit of one batch (MZH_ESIZE): N * C below 2^32.
+void check_batch_size(uint64_t N, uint64_t C);
 
 // Store a validated plan in its context.
 void set_plan(mzh_ctx *ctx, ScreenPlan &&pl, double conj_thresh, double conj_det_interval, int mode);

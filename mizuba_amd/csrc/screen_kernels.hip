@@ -942,9 +942,12 @@ __device__ void bvh_step(const Prob &pb, uint32_t c, uint8_t *s_dyn, uint32_t *l
 
 // Stage 3, persistent: each workgroup takes (problem, step) items in list
 // order (largest problems first). LDS: the build scratch of the largest step
-// of the list (dynamic) when LDS, else the scratch is in HBM.
+// of the list (dynamic) when LDS, else the scratch is in HBM. Eight waves per
+// SIMD (at most 64 VGPRs) let two workgroups share a CU: a build is a chain of
+// dependent level passes, and the second workgroup's passes fill the first's
+// waits.
 template <bool LDS>
-__global__ __launch_bounds__(kBvhBlock) void k_bvh(const Prob *__restrict__ P, const uint2 *__restrict__ items,
+__global__ __launch_bounds__(kBvhBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bvh(const Prob *__restrict__ P, const uint2 *__restrict__ items,
                                                      uint32_t n_items, uint32_t *counter)
 {
     extern __shared__ __align__(16) uint8_t s_dyn[];
