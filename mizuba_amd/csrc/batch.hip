@@ -241,6 +241,7 @@ void ScreenBatch::prepare()
         MZH_HIP(ctx->step_total.ensure(sizeof(uint64_t) * C));
         MZH_HIP(ctx->step_off.ensure(sizeof(uint64_t) * (C + 1)));
         MZH_HIP(ctx->pairs.ensure(sizeof(mzh_aabb_collision) * (ctx->pair_cap ? ctx->pair_cap : 1)));
+        MZH_HIP(ctx->pair_c.ensure(sizeof(uint32_t) * (ctx->pair_cap ? ctx->pair_cap : 1)));
         MZH_HIP(ctx->conj_sorted.ensure(sizeof(mzh_conj) * (ctx->conj_out_cap ? ctx->conj_out_cap : 1)));
         ctx->nodes_ready = false;
         MZH_HIP(hipMemcpyAsync(ctx->cd_begin.p, ctx->h_cd_begin.data(), sizeof(double) * C, hipMemcpyHostToDevice, st));
@@ -284,6 +285,7 @@ void ScreenBatch::prepare()
         q.step_total = gl(ctx->step_total.as<uint64_t>());
         q.step_off = gl(ctx->step_off.as<uint64_t>());
         q.pairs = gl(ctx->pairs.as<mzh_aabb_collision>());
+        q.pair_c = gl(ctx->pair_c.as<uint32_t>());
         q.pair_cap = ctx->pair_cap;
         q.conj_out = gl(ctx->conj_sorted.as<mzh_conj>());
         q.conj_out_cap = ctx->conj_out_cap;
@@ -493,7 +495,9 @@ void ScreenBatch::run_eager(bool timed)
                 // exact, or with the headroom that later graph replays need
                 ctx->pair_cap = use_graph ? grown(counts[4u * k]) : counts[4u * k];
                 MZH_HIP(ctx->pairs.ensure(sizeof(mzh_aabb_collision) * ctx->pair_cap));
+                MZH_HIP(ctx->pair_c.ensure(sizeof(uint32_t) * ctx->pair_cap));
                 hP[k].pairs = gl(ctx->pairs.as<mzh_aabb_collision>());
+                hP[k].pair_c = gl(ctx->pair_c.as<uint32_t>());
                 hP[k].pair_cap = ctx->pair_cap;
                 resized = true;
             }

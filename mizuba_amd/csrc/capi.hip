@@ -1589,7 +1589,8 @@ int mzh_release(mzh_ctx *ctx, int what)
         if (what & MZH_RELEASE_SCRATCH) {
             for (DevBuf *b : {&ctx->kbke, &ctx->keys_tmp, &ctx->vals_tmp, &ctx->n_eff, &ctx->sort_todo, &ctx->gbox,
                               &ctx->bvh_hdr, &ctx->bvh_scr, &ctx->otypes, &ctx->cnt, &ctx->head, &ctx->off,
-                              &ctx->step_total, &ctx->slots, &ctx->ovf, &ctx->ovf_count, &ctx->ev_tm, &ctx->ev_sel,
+                              &ctx->step_total, &ctx->slots, &ctx->ovf, &ctx->ovf_count, &ctx->pair_c, &ctx->ev_tm,
+                              &ctx->ev_sel,
                               &ctx->ev_out, &ctx->ev_bad})
                 b->release();
             // the narrow phase's queues of the context's own screenings

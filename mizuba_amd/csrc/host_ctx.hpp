@@ -163,7 +163,7 @@ struct mzh_ctx {
     bool nodes_ready = false;  // nodes holds the reference-layout export of trav
     mzh::host::DevBuf otypes, cnt, head, off, step_total, step_off, slots, ovf, ovf_count;
     uint32_t ovf_cap = 0;
-    mzh::host::DevBuf pairs;
+    mzh::host::DevBuf pairs, pair_c;
     uint64_t n_pairs = 0, pair_cap = 0;
     mzh::host::DevBuf conj_sorted;
     uint64_t n_conj = 0, conj_out_cap = 0;

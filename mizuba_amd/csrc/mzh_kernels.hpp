@@ -81,6 +81,7 @@ struct Prob {
     gbuf<uint64_t> step_total;  // [C]
     gbuf<uint64_t> step_off;  // [C + 1]: exclusive prefix of the steps' pairs; [C] = all pairs
     gbuf<mzh_aabb_collision> pairs;  // pair_cap entries
+    gbuf<uint32_t> pair_c;           // pair_cap entries: the conjunction step of each pair
     uint64_t pair_cap;
     gbuf<mzh_conj> conj_out;  // the problem's conjunctions, sorted (conj_out_cap entries)
     uint64_t conj_out_cap;

@@ -674,18 +674,7 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
         p = find_prob64(a.pair_pre, a.np, g);
         const Prob &q = a.P[p];
         const uint64_t l = g - a.pair_pre[p];
-        // the pair's conjunction step: the last c with step_off[c] <= l
-        uint32_t lo = 0, n = q.C;
-        while (n > 0) {
-            const uint32_t h = n / 2;
-            if (q.step_off[lo + h + 1] <= l) {
-                lo += h + 1;
-                n -= h + 1;
-            } else {
-                n = h;
-            }
-        }
-        c = lo;
+        c = q.pair_c[l];  // the pair's conjunction step, recorded when the pair was placed
         i = q.pairs[l].i;
         j = q.pairs[l].j;
         cb = q.cd_begin[c];
@@ -699,8 +688,9 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
         kbj = rj.x;
         kej = rj.y;
     }
-    // the merge-walk visits step pairs in order; pass 1 counts the overlapping
-    // ones, pass 2 writes them at this lane's offset in the block's range
+    // the merge-walk visits step pairs in order; the first kEnumBuf
+    // overlapping ones wait in registers while the block's offsets are
+    // found, the rare rest is found by walking again
     auto walk = [&](auto &&emit) {
         for (uint64_t ki = kbi, kj = kbj; ki < kei && kj < kej;) {
             const double tsi = ti[ki], tei = ti[ki + 1], tsj = tj[kj], tej = tj[kj + 1];
@@ -712,8 +702,18 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
             kj += inc_j ? 1 : 0;
         }
     };
+    constexpr uint32_t kEnumBuf = 4;
+    uint32_t bki[kEnumBuf] = {}, bkj[kEnumBuf] = {};
     uint32_t cnt = 0;
-    walk([&](uint64_t, uint64_t) { ++cnt; });
+    walk([&](uint64_t ki, uint64_t kj) {
+#pragma unroll
+        for (uint32_t m = 0; m < kEnumBuf; ++m)
+            if (m == cnt) {
+                bki[m] = (uint32_t)ki;
+                bkj[m] = (uint32_t)kj;
+            }
+        ++cnt;
+    });
     {
         const unsigned long long v = cnt;
         add_prob_report<1>(a.P, p, live, 0, &v);
@@ -724,12 +724,18 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
     const uint32_t ex = block_excl_scan_u32(cnt, s_w, total);
     if (threadIdx.x == 0) s_base = total ? atomicAdd(a.steps_count, (unsigned long long)total) : 0ull;
     __syncthreads();
-    uint64_t slot = s_base + ex;
-    walk([&](uint64_t ki, uint64_t kj) {
-        if (slot < a.steps_cap)
-            static_cast<StepPair *>(a.steps)[slot] = StepPair{p, i, j, c, (uint32_t)ki, (uint32_t)kj};
-        ++slot;
-    });
+    const uint64_t slot = s_base + ex;
+    StepPair *out = static_cast<StepPair *>(a.steps);
+#pragma unroll
+    for (uint32_t m = 0; m < kEnumBuf; ++m)
+        if (m < cnt && slot + m < a.steps_cap) out[slot + m] = StepPair{p, i, j, c, bki[m], bkj[m]};
+    if (cnt > kEnumBuf) {
+        uint32_t m = 0;
+        walk([&](uint64_t ki, uint64_t kj) {
+            if (m >= kEnumBuf && slot + m < a.steps_cap) out[slot + m] = StepPair{p, i, j, c, (uint32_t)ki, (uint32_t)kj};
+            ++m;
+        });
+    }
 }
 
 // Narrow phase, part 1b: one lane per overlapping step-pair. Root-finding

@@ -1205,6 +1205,9 @@ __global__ __launch_bounds__(256) void k_bp_place(const Prob *__restrict__ P, ui
     const uint64_t at = step_off[c] + off[g];
     if (at + n > q.pair_cap) return;  // too small a pair buffer: flagged, the screening runs again
     mzh_aabb_collision *out = q.pairs + at;
+    // the step of every pair, for the narrow phase
+    uint32_t *out_c = q.pair_c + at;
+    for (uint32_t k = 0; k < n; ++k) out_c[k] = c;
     if (n <= kBpSlots) {
         uint32_t v[kBpSlots];
 #pragma unroll
