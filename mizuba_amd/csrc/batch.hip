@@ -133,6 +133,7 @@ private:
     uint32_t n_steps_all = 0, n_nc_blocks = 0, n_bp_blocks = 0, n_gather_blocks = 0;
     struct Group {
         uint32_t order, p_first, p_end, blk_begin, n_blk;
+        bool any_fp64, any_mixed;  // the modes of its problems (one AABB launch each)
     };
     std::vector<Group> groups;
     bool any_bp = false;
@@ -302,8 +303,9 @@ void ScreenBatch::prepare()
         max_order = std::max(max_order, ctx->order);
         max_cand_bytes = std::max(max_cand_bytes, narrow_cand_bytes(ctx->order));
         if (groups.empty() || groups.back().order != ctx->order)
-            groups.push_back(Group{ctx->order, k, k + 1, aabb_pre[k], 0});
+            groups.push_back(Group{ctx->order, k, k + 1, aabb_pre[k], 0, false, false});
         groups.back().p_end = k + 1;
+        (ctx->mode == MZH_MODE_MIXED ? groups.back().any_mixed : groups.back().any_fp64) = true;
         groups.back().n_blk = aabb_pre[k + 1] - groups.back().blk_begin;
     }
     n_steps_all = step_pre[n];
@@ -369,7 +371,10 @@ void ScreenBatch::enqueue_stages(bool timed)
     MZH_HIP(launch_batch_init(P, n, tabs, n_steps_all, bc, st));
     // stage 1: AABBs
     if (timed) MZH_HIP(hipEventRecord(ev[6], st));
-    for (const Group &g : groups) MZH_HIP(launch_aabbs(g.order, P, n, tabs, g.blk_begin, g.n_blk, st));
+    for (const Group &g : groups) {
+        if (g.any_fp64) MZH_HIP(launch_aabbs(g.order, MZH_MODE_FP64, P, n, tabs, g.blk_begin, g.n_blk, st));
+        if (g.any_mixed) MZH_HIP(launch_aabbs(g.order, MZH_MODE_MIXED, P, n, tabs, g.blk_begin, g.n_blk, st));
+    }
     if (timed) MZH_HIP(hipEventRecord(ev[7], st));
     MZH_HIP(launch_gbox_finalize(P, n, tabs, n_steps_all, st));
     if (timed) MZH_HIP(hipEventRecord(ev[1], st));

@@ -180,6 +180,7 @@ PjView pj_view(const mzh_ctx *ctx)
     v.traj = gl(ctx->pjb->traj.as<const double>());
     v.time = gl(ctx->pjb->time.as<const double>());
     v.cf4 = gl(ctx->pjb->cf4.as<const double>());
+    v.cf4f = gl(ctx->pjb->cf4f.as<const float>());
     v.step0 = gl(ctx->pjb->step0.as<const uint64_t>());
     v.nsteps = gl(ctx->pjb->nsteps.as<const uint64_t>());
     v.time_off = gl(ctx->pjb->time_off.as<const uint64_t>());
@@ -455,6 +456,7 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         MZH_HIP(ctx->pjb->traj.ensure(sizeof(double) * (traj_len ? traj_len : 1)));
         MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * (time_len ? time_len : 1)));
         MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
+        MZH_HIP(ctx->pjb->cf4f.ensure(sizeof(float) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
         MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_objs));
         MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_objs));
         MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_objs));
@@ -463,7 +465,8 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         MZH_HIP(hipMemcpyAsync(ctx->pjb->step0.p, step0.data(), sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
         MZH_HIP(hipMemcpyAsync(ctx->pjb->nsteps.p, nsteps, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
         MZH_HIP(hipMemcpyAsync(ctx->pjb->time_off.p, time_off, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
-        MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), steps_total, order, st));
+        MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), ctx->pjb->cf4f.as<float>(),
+                                 steps_total, order, st));
         MZH_HIP(hipStreamSynchronize(st));
         ctx->n_objs = n_objs;
         ctx->order = order;
@@ -741,6 +744,7 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     MZH_HIP(ctx->pjb->traj.ensure(sizeof(double) * steps_total * op1 * 7u));
     MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * time_len));
     MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * op1 * steps_total));
+    MZH_HIP(ctx->pjb->cf4f.ensure(sizeof(float) * 4u * op1 * steps_total));
     MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_sats));
     MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_sats));
     MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_sats));
@@ -754,7 +758,8 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     MZH_HIP(hipEventRecord(E[5], st));
     MZH_HIP(launch_sgp4_write(g, order, n_redo, st));
     MZH_HIP(hipEventRecord(E[6], st));
-    MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), steps_total, order, st));
+    MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), ctx->pjb->cf4f.as<float>(),
+                                 steps_total, order, st));
     MZH_HIP(hipEventRecord(E[7], st));
     MZH_HIP(hipStreamSynchronize(st));
     {

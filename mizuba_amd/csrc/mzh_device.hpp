@@ -48,6 +48,7 @@ struct PjView {
     gbuf<const double> traj;      // AoS [S][op1][7]
     gbuf<const double> time;      // per object nsteps + 1 values
     gbuf<const double> cf4;       // step-major transposed [S][4][op1], channels x y z r
+    gbuf<const float> cf4f;       // the same rounded to float (the mixed-precision AABBs)
     gbuf<const uint64_t> step0;   // global index of the object's first step
     gbuf<const uint64_t> nsteps;  // steps per object
     gbuf<const uint64_t> time_off;

@@ -134,8 +134,10 @@ MZH_HD inline void cs_enclosure_fp64(const double *__restrict__ cf, double lb, d
 // enclosure's own (smaller) errors, so the float interval [b - e, b + e]
 // contains the FP64 Bernstein coefficient. The margin is evaluated from
 // magnitudes accumulated alongside the values and inflated by a factor 2.
-template <int ORDER, int STRIDE = 1>
-MZH_HD inline void cs_enclosure_fp32(const double *__restrict__ cf, double lb_d, double ub_d, float &emin,
+// The coefficients may come as doubles or already rounded to float (CT =
+// float): each is used as (float)cf[i], so both give the same result.
+template <int ORDER, int STRIDE = 1, typename CT = double>
+MZH_HD inline void cs_enclosure_fp32(const CT *__restrict__ cf, double lb_d, double ub_d, float &emin,
                                      float &emax, const BinomTable &B)
 {
     constexpr int OP1 = ORDER + 1;

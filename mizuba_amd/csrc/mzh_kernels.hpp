@@ -12,8 +12,8 @@ namespace mzh
 {
 
 // Upload-time layout transform: AoS [S][op1][7] -> step-major [S][4][op1]
-// holding channels x, y, z, r.
-hipError_t launch_transpose_cf4(const double *traj, double *cf4, uint64_t n_steps_total, uint32_t order,
+// holding channels x, y, z, r, in FP64 (cf4) and rounded to FP32 (cf4f).
+hipError_t launch_transpose_cf4(const double *traj, double *cf4, float *cf4f, uint64_t n_steps_total, uint32_t order,
                                 hipStream_t stream);
 
 // ---- a batch of screening problems ------------------------------------------
@@ -129,8 +129,10 @@ hipError_t launch_batch_init(const Prob *P, uint32_t np, const BatchTables &t, u
 // steps overlapping the conjunction step, for the narrow phase. One launch
 // per polynomial order present in the batch: problems of another order
 // return at once.
-hipError_t launch_aabbs(uint32_t order, const Prob *P, uint32_t np, const BatchTables &t, uint32_t blk_begin,
-                        uint32_t n_blocks, hipStream_t stream);
+// mode: the problems of that mode (MZH_MODE_FP64 or MZH_MODE_MIXED) are
+// handled, the others return at once.
+hipError_t launch_aabbs(uint32_t order, int mode, const Prob *P, uint32_t np, const BatchTables &t,
+                        uint32_t blk_begin, uint32_t n_blocks, hipStream_t stream);
 hipError_t launch_gbox_finalize(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
                                 hipStream_t stream);
 // Stage 2: Morton codes, sort by (no data, code, index), gather of the boxes

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "mzh_device.hpp"
 #include "mzh_kernels.hpp"
@@ -42,8 +43,8 @@ __device__ inline uint32_t find_prob(const uint32_t *__restrict__ pre, uint32_t 
 // upload-time transpose
 // ---------------------------------------------------------------------------
 
-__global__ void k_transpose_cf4(const double *__restrict__ traj, double *__restrict__ cf4, uint64_t n_steps_total,
-                                uint32_t op1)
+__global__ void k_transpose_cf4(const double *__restrict__ traj, double *__restrict__ cf4, float *__restrict__ cf4f,
+                                uint64_t n_steps_total, uint32_t op1)
 {
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t per_step = 4ull * op1;
@@ -52,7 +53,9 @@ __global__ void k_transpose_cf4(const double *__restrict__ traj, double *__restr
     const uint32_t r = (uint32_t)(idx - s * per_step);
     const uint32_t ch = r / op1, k = r - ch * op1;
     const uint32_t src_ch = (ch == 3) ? 6u : ch;
-    cf4[idx] = traj[(s * op1 + k) * 7u + src_ch];
+    const double v = traj[(s * op1 + k) * 7u + src_ch];
+    cf4[idx] = v;
+    cf4f[idx] = (float)v;  // the rounding the FP32 enclosure applies to each coefficient
 }
 
 // ---------------------------------------------------------------------------
@@ -68,14 +71,21 @@ constexpr uint32_t kAabbGroup = kAabbStepGroup;
 // and the per-step global box (:314-397). One thread per object and group of
 // kAabbGroup consecutive steps; the 256 lanes of a block are 256 consecutive
 // objects, so each step's boxes are written coalesced and reduced per block.
-template <int ORDER>
+//
+// F32: the mixed-precision problems (the FP64 ones return), whose enclosures
+// are computed in FP32 from the float copy of the coefficients (cf4f): half
+// the bytes read and half the LDS of the piece cache. The others: the FP64
+// problems.
+template <int ORDER, bool F32>
 __global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint32_t np,
                                                const uint32_t *__restrict__ blk_pre, uint32_t blk_begin)
 {
+    using CT = typename std::conditional<F32, float, double>::type;
     const uint32_t b = blk_begin + blockIdx.x;
     const uint32_t pi = find_prob(blk_pre, np, b);
     const Prob &q = P[pi];
     if (q.pj.order != (uint32_t)ORDER) return;
+    if ((q.mode == MZH_MODE_MIXED) != F32) return;
     const uint32_t lb = b - blk_pre[pi];
     const PjView pj = q.pj;
     const double *cd_begin = q.cd_begin, *cd_end = q.cd_end;
@@ -97,18 +107,24 @@ __global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint3
     // coefficients) is kept in LDS, lane-minor: a piece spans several
     // conjunction steps and is read from HBM once. Up to order 8 (72 KB).
     constexpr int kCf = 4 * (ORDER + 1);
-    constexpr bool kCache = kCf * 256 * 8 <= 80 * 1024;
-    __shared__ double s_cf[kCache ? kCf : 1][256];
+    constexpr bool kCache = kCf * 256 * sizeof(CT) <= 80 * 1024;
+    __shared__ CT s_cf[kCache ? kCf : 1][256];
     uint64_t k_cached = ~0ull;
 
     uint64_t ns = 0, kb = 0;
     const double *t = nullptr;
-    const double *cfb = nullptr;
+    const CT *cfb = nullptr;
+    const double *cfb64 = nullptr;
     if (o < N) {
         ns = pj.nsteps[o];
         if (ns > 0) {
             t = pj.time + pj.time_off[o];
-            cfb = pj.cf4 + (pj.step0[o] * 4u) * (ORDER + 1);
+            cfb64 = pj.cf4 + (pj.step0[o] * 4u) * (ORDER + 1);
+            if constexpr (F32) {
+                cfb = pj.cf4f + (pj.step0[o] * 4u) * (ORDER + 1);
+            } else {
+                cfb = cfb64;
+            }
             // upper_bound(t + 1, t + ns + 1, cd_begin[c0]): first piece ending after the group's start
             const double cb = cd_begin[c0];
             uint64_t lo = 0, n = ns;
@@ -142,7 +158,7 @@ __global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint3
                     const double ev_lb = cb > ss ? cb : ss;
                     const double ev_ub = ce < t[k + 1] ? ce : t[k + 1];
                     const double h_lb = ev_lb - ss, h_ub = ev_ub - ss;
-                    const double *cf = cfb + k * (uint64_t)kCf;
+                    const CT *cf = cfb + k * (uint64_t)kCf;
                     constexpr int kStride = kCache ? 256 : 1;
                     if constexpr (kCache) {
                         if (k != k_cached) {
@@ -155,8 +171,8 @@ __global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint3
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
                         float fl, fu;
-                        const double *cfd = cf + d * (ORDER + 1) * kStride;
-                        if (mode == MZH_MODE_FP64) {
+                        const CT *cfd = cf + d * (ORDER + 1) * kStride;
+                        if constexpr (!F32) {
                             double emin, emax;
                             cs_enclosure_fp64<ORDER, kStride>(cfd, h_lb, h_ub, emin, emax, c_binom);
                             emin = emin - radius;
@@ -170,8 +186,9 @@ __global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint3
                             if (!is_finite_f(emin) || !is_finite_f(emax)) {
                                 // coefficients or intermediates outside the float
                                 // range (ill-conditioned short steps): use the FP64
-                                // enclosure for this piece
-                                cs_enclosure_fp64<ORDER, kStride>(cfd, h_lb, h_ub, dmin, dmax, c_binom);
+                                // enclosure for this piece, from the FP64 copy
+                                cs_enclosure_fp64<ORDER, 1>(cfb64 + k * (uint64_t)kCf + d * (ORDER + 1), h_lb, h_ub,
+                                                            dmin, dmax, c_binom);
                             }
                             // dmin <= FP64 enclosure minimum (as reals); the same
                             // monotone double subtraction and outward float
@@ -1350,15 +1367,21 @@ __global__ __launch_bounds__(256) void k_bp_reset(const Prob *__restrict__ P, ui
 }
 
 template <int ORDER>
-hipError_t dispatch_aabbs(uint32_t order, const Prob *P, uint32_t np, const BatchTables &t, uint32_t blk_begin,
-                          uint32_t n_blocks, hipStream_t stream)
+hipError_t dispatch_aabbs(uint32_t order, int mode, const Prob *P, uint32_t np, const BatchTables &t,
+                          uint32_t blk_begin, uint32_t n_blocks, hipStream_t stream)
 {
     if (order == (uint32_t)ORDER) {
-        hipLaunchKernelGGL(k_aabbs<ORDER>, dim3(n_blocks), dim3(256), 0, stream, P, np, t.aabb_blk, blk_begin);
+        if (mode == MZH_MODE_MIXED) {
+            hipLaunchKernelGGL((k_aabbs<ORDER, true>), dim3(n_blocks), dim3(256), 0, stream, P, np, t.aabb_blk,
+                               blk_begin);
+        } else {
+            hipLaunchKernelGGL((k_aabbs<ORDER, false>), dim3(n_blocks), dim3(256), 0, stream, P, np, t.aabb_blk,
+                               blk_begin);
+        }
         return hipGetLastError();
     }
     if constexpr (ORDER < kMaxOrder) {
-        return dispatch_aabbs<ORDER + 1>(order, P, np, t, blk_begin, n_blocks, stream);
+        return dispatch_aabbs<ORDER + 1>(order, mode, P, np, t, blk_begin, n_blocks, stream);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1366,12 +1389,12 @@ hipError_t dispatch_aabbs(uint32_t order, const Prob *P, uint32_t np, const Batc
 
 } // namespace
 
-hipError_t launch_transpose_cf4(const double *traj, double *cf4, uint64_t n_steps_total, uint32_t order,
+hipError_t launch_transpose_cf4(const double *traj, double *cf4, float *cf4f, uint64_t n_steps_total, uint32_t order,
                                 hipStream_t stream)
 {
     const uint64_t n = n_steps_total * 4ull * (order + 1u);
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_transpose_cf4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, traj, cf4,
+    hipLaunchKernelGGL(k_transpose_cf4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, traj, cf4, cf4f,
                        n_steps_total, order + 1u);
     return hipGetLastError();
 }
@@ -1384,11 +1407,11 @@ hipError_t launch_batch_init(const Prob *P, uint32_t np, const BatchTables &t, u
     return hipGetLastError();
 }
 
-hipError_t launch_aabbs(uint32_t order, const Prob *P, uint32_t np, const BatchTables &t, uint32_t blk_begin,
-                        uint32_t n_blocks, hipStream_t stream)
+hipError_t launch_aabbs(uint32_t order, int mode, const Prob *P, uint32_t np, const BatchTables &t,
+                        uint32_t blk_begin, uint32_t n_blocks, hipStream_t stream)
 {
     if (n_blocks == 0) return hipSuccess;
-    return dispatch_aabbs<kMinOrder>(order, P, np, t, blk_begin, n_blocks, stream);
+    return dispatch_aabbs<kMinOrder>(order, mode, P, np, t, blk_begin, n_blocks, stream);
 }
 
 hipError_t launch_gbox_finalize(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
