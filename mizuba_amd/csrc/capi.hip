@@ -181,6 +181,7 @@ PjView pj_view(const mzh_ctx *ctx)
     v.time = gl(ctx->pjb->time.as<const double>());
     v.cf4 = gl(ctx->pjb->cf4.as<const double>());
     v.cf4f = gl(ctx->pjb->cf4f.as<const float>());
+    v.dbnd = gl(ctx->pjb->dbnd.as<const float>());
     v.step0 = gl(ctx->pjb->step0.as<const uint64_t>());
     v.nsteps = gl(ctx->pjb->nsteps.as<const uint64_t>());
     v.time_off = gl(ctx->pjb->time_off.as<const uint64_t>());
@@ -457,6 +458,7 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * (time_len ? time_len : 1)));
         MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
         MZH_HIP(ctx->pjb->cf4f.ensure(sizeof(float) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
+        MZH_HIP(ctx->pjb->dbnd.ensure(sizeof(float) * order * (steps_total ? steps_total : 1)));
         MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_objs));
         MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_objs));
         MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_objs));
@@ -467,6 +469,10 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         MZH_HIP(hipMemcpyAsync(ctx->pjb->time_off.p, time_off, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
         MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), ctx->pjb->cf4f.as<float>(),
                                  steps_total, order, st));
+        MZH_HIP(launch_deriv_bounds(ctx->pjb->cf4.as<double>(), ctx->pjb->time.as<double>(),
+                                    ctx->pjb->step0.as<uint64_t>(), ctx->pjb->nsteps.as<uint64_t>(),
+                                    ctx->pjb->time_off.as<uint64_t>(), n_objs, steps_total, order,
+                                    ctx->pjb->dbnd.as<float>(), st));
         MZH_HIP(hipStreamSynchronize(st));
         ctx->n_objs = n_objs;
         ctx->order = order;
@@ -745,6 +751,7 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * time_len));
     MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * op1 * steps_total));
     MZH_HIP(ctx->pjb->cf4f.ensure(sizeof(float) * 4u * op1 * steps_total));
+    MZH_HIP(ctx->pjb->dbnd.ensure(sizeof(float) * order * steps_total));
     MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_sats));
     MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_sats));
     MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_sats));
@@ -760,6 +767,9 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     MZH_HIP(hipEventRecord(E[6], st));
     MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), ctx->pjb->cf4f.as<float>(),
                                  steps_total, order, st));
+    MZH_HIP(launch_deriv_bounds(ctx->pjb->cf4.as<double>(), ctx->pjb->time.as<double>(), ctx->pjb->step0.as<uint64_t>(),
+                                ctx->pjb->nsteps.as<uint64_t>(), ctx->pjb->time_off.as<uint64_t>(), n_sats, steps_total,
+                                order, ctx->pjb->dbnd.as<float>(), st));
     MZH_HIP(hipEventRecord(E[7], st));
     MZH_HIP(hipStreamSynchronize(st));
     {

@@ -117,7 +117,7 @@ struct StatusError {
 // Device arrays of a polyjectory: the AoS coefficients, the times, the
 // step-major transposed positions and the per-object layout.
 struct PjBufs {
-    mzh::host::DevBuf traj, time, cf4, cf4f, step0, nsteps, time_off;
+    mzh::host::DevBuf traj, time, cf4, cf4f, dbnd, step0, nsteps, time_off;
 };
 
 } // namespace mzh::host

@@ -49,6 +49,9 @@ struct PjView {
     gbuf<const double> time;      // per object nsteps + 1 values
     gbuf<const double> cf4;       // step-major transposed [S][4][op1], channels x y z r
     gbuf<const float> cf4f;       // the same rounded to float (the mixed-precision AABBs)
+    // [S][order]: D_a, a = 1..order, upper bounds on |d^a r / dt^a| (Euclidean
+    // norm of the position's a-th derivative, days) over each whole step
+    gbuf<const float> dbnd;
     gbuf<const uint64_t> step0;   // global index of the object's first step
     gbuf<const uint64_t> nsteps;  // steps per object
     gbuf<const uint64_t> time_off;

@@ -16,6 +16,14 @@ namespace mzh
 hipError_t launch_transpose_cf4(const double *traj, double *cf4, float *cf4f, uint64_t n_steps_total, uint32_t order,
                                 hipStream_t stream);
 
+// Upload-time bounds on the position derivatives of every trajectory step
+// (PjView::dbnd): for a = 1..order, D_a >= max over the step of the Euclidean
+// norm of the a-th derivative of (x, y, z), from the coefficients of cf4 and
+// the step lengths, rounded up.
+hipError_t launch_deriv_bounds(const double *cf4, const double *time, const uint64_t *step0, const uint64_t *nsteps,
+                               const uint64_t *time_off, uint32_t n_objs, uint64_t n_steps_total, uint32_t order,
+                               float *dbnd, hipStream_t stream);
+
 // ---- a batch of screening problems ------------------------------------------
 // Conjunction steps handled by one lane of the AABB kernel.
 constexpr uint32_t kAabbStepGroup = 16;
@@ -196,6 +204,10 @@ struct NarrowArgs {
     void *steps;                  // StepPair records
     uint64_t steps_cap;
     unsigned long long *steps_count;
+    // with mixed-precision problems (else null): the step pairs the
+    // pre-screen keeps (steps_cap records), which the screen reads
+    void *steps2;
+    unsigned long long *steps2_count;
     // queue of step-pairs whose dist^2 enclosure reaches below thresh^2
     void *cand;                   // records of narrow_cand_bytes(order) bytes
     uint64_t cand_cap;

@@ -58,6 +58,79 @@ __global__ void k_transpose_cf4(const double *__restrict__ traj, double *__restr
     cf4f[idx] = (float)v;  // the rounding the FP32 enclosure applies to each coefficient
 }
 
+// Upper bounds on the position derivatives of one trajectory step (one lane
+// per step of the concatenated steps). Over the step, tau in [0, H],
+//   |d^a x / dtau^a| <= B_a = sum_{k >= a} k! / (k - a)! |c_k| H^(k - a)
+// per channel (Horner in H on positive terms, so the rounding is at most
+// 2 order ulps, covered by the factor 1 + 2^-40), and D_a = |(B_a,x, B_a,y,
+// B_a,z)| rounded up to float.
+template <int ORDER>
+__global__ void k_deriv_bounds(const double *__restrict__ cf4, const double *__restrict__ time,
+                               const uint64_t *__restrict__ step0, const uint64_t *__restrict__ nsteps,
+                               const uint64_t *__restrict__ time_off, uint32_t n_objs, uint64_t n_steps_total,
+                               float *__restrict__ dbnd)
+{
+    constexpr int OP1 = ORDER + 1;
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_steps_total) return;
+    // the object owning step s: the last o with step0[o] <= s (objects without
+    // steps share their successor's step0 and come before it)
+    uint32_t lo = 0, n = n_objs;
+    while (n > 1) {
+        const uint32_t h = n / 2;
+        if (step0[lo + h] <= s) {
+            lo += h;
+            n -= h;
+        } else {
+            n = h;
+        }
+    }
+    const uint64_t k = s - step0[lo];
+    if (k >= nsteps[lo]) return;  // not reached: every step belongs to an object
+    const double *t = time + time_off[lo];
+    const double H = fabs(t[k + 1] - t[k]);
+    double acc[ORDER];
+#pragma unroll
+    for (int a = 0; a < ORDER; ++a) acc[a] = 0.0;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const double *c = cf4 + (s * 4u + (uint64_t)ch) * OP1;
+        double fa = 1.0;  // a!
+#pragma unroll
+        for (int a = 1; a <= ORDER; ++a) {
+            fa *= (double)a;
+            // k! / (k - a)! = C(k, a) a!, for k = ORDER down to a, in Horner order
+            double b = 0.0;
+#pragma unroll
+            for (int kk = ORDER; kk >= a; --kk) b = b * H + (c_binom.v[kk][a] * fa) * fabs(c[kk]);
+            acc[a - 1] += b * b;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < ORDER; ++a) {
+        const double d = sqrt(acc[a]) * (1.0 + 0x1p-40);
+        dbnd[s * ORDER + a] = next_up_f((float)d);
+    }
+}
+
+template <int ORDER>
+hipError_t dispatch_deriv_bounds(uint32_t order, const double *cf4, const double *time, const uint64_t *step0,
+                                 const uint64_t *nsteps, const uint64_t *time_off, uint32_t n_objs,
+                                 uint64_t n_steps_total, float *dbnd, hipStream_t stream)
+{
+    if (order == (uint32_t)ORDER) {
+        hipLaunchKernelGGL(k_deriv_bounds<ORDER>, dim3((unsigned)((n_steps_total + 255) / 256)), dim3(256), 0, stream,
+                           cf4, time, step0, nsteps, time_off, n_objs, n_steps_total, dbnd);
+        return hipGetLastError();
+    }
+    if constexpr (ORDER < kMaxOrder) {
+        return dispatch_deriv_bounds<ORDER + 1>(order, cf4, time, step0, nsteps, time_off, n_objs, n_steps_total, dbnd,
+                                                stream);
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // stage 1: AABBs (detect_conjunctions_aabbs.cpp:64-252, conjunctions_jit.cpp:179-298)
 // ---------------------------------------------------------------------------
@@ -1397,6 +1470,15 @@ hipError_t launch_transpose_cf4(const double *traj, double *cf4, float *cf4f, ui
     hipLaunchKernelGGL(k_transpose_cf4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, traj, cf4, cf4f,
                        n_steps_total, order + 1u);
     return hipGetLastError();
+}
+
+hipError_t launch_deriv_bounds(const double *cf4, const double *time, const uint64_t *step0, const uint64_t *nsteps,
+                               const uint64_t *time_off, uint32_t n_objs, uint64_t n_steps_total, uint32_t order,
+                               float *dbnd, hipStream_t stream)
+{
+    if (n_steps_total == 0 || n_objs == 0) return hipSuccess;
+    return dispatch_deriv_bounds<kMinOrder>(order, cf4, time, step0, nsteps, time_off, n_objs, n_steps_total, dbnd,
+                                            stream);
 }
 
 hipError_t launch_batch_init(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,

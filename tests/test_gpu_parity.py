@@ -342,6 +342,43 @@ def test_mixed_mode_superset(leo1000):
     assert_conj_equal(cmx.conjunctions, c64.conjunctions)
 
 
+def test_mixed_prescreen_near_threshold():
+    """Mixed mode's narrow-phase pre-screen drops step pairs only where the
+    FP64 narrow phase cannot find a conjunction: SGP4 objects with copies
+    shifted along track (mean anomaly) and across (node) so that their
+    distances sit around the threshold, from 0.6 to 1.4 times it. The mixed
+    list must equal the FP64 list bit for bit, with minima just under the
+    threshold among them."""
+    thresh = 5.0
+    jd0 = 2460496.5
+    base = mz.make_synthetic_gpes(150, seed=77, frac_high=0.2, epoch_jd=jd0)
+    a = (398600.8 / (base["n0"] / 60.0) ** 2) ** (1.0 / 3.0)
+    parts = [base]
+    nid = base["norad_id"].max() + 1
+    for f in (0.6, 0.9, 0.97, 0.995, 1.0, 1.005, 1.03, 1.1, 1.4):
+        g = base.copy()
+        g["m0"] = np.mod(g["m0"] + f * thresh / a, 2 * np.pi)
+        g["norad_id"] = np.arange(nid, nid + len(g))
+        nid += len(g)
+        parts.append(g)
+        h = base.copy()
+        h["node0"] = np.mod(h["node0"] + f * thresh / (a * np.maximum(np.sin(h["i0"]), 0.05)), 2 * np.pi)
+        h["norad_id"] = np.arange(nid, nid + len(h))
+        nid += len(h)
+        parts.append(h)
+    gpes = np.concatenate(parts)
+    pj = mz.make_sgp4_polyjectory(gpes, jd0, jd0 + 0.25)[0]
+    c64 = mz.conjunctions(pj, thresh, 3.0 / 1440.0, mode="fp64")
+    cmx = mz.conjunctions(pj, thresh, 3.0 / 1440.0, mode="mixed")
+    assert_conj_equal(cmx.conjunctions, c64.conjunctions)
+    cj = c64.conjunctions
+    assert np.count_nonzero(cj["dca"] > 0.95 * thresh) > 10
+    r64, rmx = c64.np_report, cmx.np_report
+    # the pre-screen's drops are counted as distance rejections
+    assert rmx["n_dist2_check"] >= r64["n_dist2_check"] - (rmx["n_tot_conj_candidates"] - r64["n_tot_conj_candidates"])
+    assert rmx["n_tot_conj_candidates"] == rmx["n_dist2_check"] + rmx["n_poly_roots"]
+
+
 def test_mixed_mode_ill_conditioned_step():
     """A sliver trajectory step whose coefficients overflow FP32 takes the FP64
     enclosure in mixed mode: boxes still contain the FP64 ones."""
