@@ -36,10 +36,11 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 
 
-def aabb_bytes_per_launch(pj, interval):
+def aabb_bytes_per_launch(pj, interval, coef_bytes=8):
     """Algorithmic bytes of one AABB launch: per (object, conjunction step)
     304 B per overlapping trajectory step (4 channels x 9 FP64 coefficients +
-    2 times) + 32 B box write."""
+    2 times; 160 B with the mixed-precision kernel's float coefficients,
+    coef_bytes=4) + 32 B box write."""
     traj, tm, toff, ns, tmo = pj._flat()
     op1 = pj.poly_order + 1
     C = int(np.ceil(pj.maxT / interval))
@@ -57,7 +58,7 @@ def aabb_bytes_per_launch(pj, interval):
         ke = np.searchsorted(t[1:], ce, side="left")
         ke = ke + (ke != n)
         pieces += int(np.sum(np.where(ov, ke - kb, 0)))
-    per_piece = 4 * op1 * 8 + 16
+    per_piece = 4 * op1 * coef_bytes + 16
     return pieces * per_piece + C * pj.n_objs * 32, pieces / (C * pj.n_objs)
 
 
@@ -810,8 +811,10 @@ def main():
     st_avg_fp64 = {k: avg(st_fp64, k) for k in stage_keys + ("total",) + kern_keys}
     C = int(np.ceil(pj.maxT / interval))
     aabb_bytes, kbar = aabb_bytes_per_launch(pj, interval)
+    aabb_bytes_f32, _ = aabb_bytes_per_launch(pj, interval, coef_bytes=4)
     kernels = {
-        "k_aabbs<8>": (st_avg_fp64["aabb_kernel"], aabb_bytes),
+        "k_aabbs<8, false>": (st_avg_fp64["aabb_kernel"], aabb_bytes),
+        "k_aabbs<8, true>": (st_avg["aabb_kernel"], aabb_bytes_f32),
         "k_bvh": (st_avg_fp64["bvh"], bvh_bytes_per_launch(pj.n_objs, C)),
         "k_bp": (st_avg_fp64["broad_kernel"], broad_bytes_per_launch(pj.n_objs, C, pairs_fp64)),
     }
