@@ -71,8 +71,10 @@ def broad_bytes_per_launch(n_objs, n_steps, n_pairs):
 
 
 def bvh_bytes_per_launch(n_objs, n_steps):
-    """Algorithmic bytes of one BVH launch: per (object, conjunction step) the
-    sorted code (8 B) and box (32 B) read once and two 48 B nodes written."""
+    """Algorithmic bytes of one build of the broad phase's walk tree
+    (k_lbvh_build, then k_lbvh_top for the nodes that span blocks): per
+    (object, conjunction step) the sorted code (8 B) and box (32 B) read once
+    and two 48 B nodes (a leaf and an internal node) written."""
     return n_steps * n_objs * (8 + 32 + 96)
 
 
@@ -815,7 +817,7 @@ def main():
     kernels = {
         "k_aabbs<8, false>": (st_avg_fp64["aabb_kernel"], aabb_bytes),
         "k_aabbs<8, true>": (st_avg["aabb_kernel"], aabb_bytes_f32),
-        "k_bvh": (st_avg_fp64["bvh"], bvh_bytes_per_launch(pj.n_objs, C)),
+        "k_lbvh_build+k_lbvh_top": (st_avg_fp64["bvh"], bvh_bytes_per_launch(pj.n_objs, C)),
         "k_bp": (st_avg_fp64["broad_kernel"], broad_bytes_per_launch(pj.n_objs, C, pairs_fp64)),
     }
     per_kernel = {

@@ -130,7 +130,7 @@ private:
     uint64_t steps_cap = 0, cand_cap = 0, slow_cap = 0, conj_cap = 0;
     std::vector<Prob> hP;
     BatchTables tabs{};
-    uint32_t n_steps_all = 0, n_nc_blocks = 0, n_bp_blocks = 0, n_gather_blocks = 0;
+    uint32_t n_steps_all = 0, n_nc_blocks = 0, n_bp_blocks = 0, n_gather_blocks = 0, n_lbvh_blocks = 0;
     struct Group {
         uint32_t order, p_first, p_end, blk_begin, n_blk;
         bool any_fp64, any_mixed;  // the modes of its problems (one AABB launch each)
@@ -178,6 +178,47 @@ private:
     void read_back(std::vector<uint64_t> &counts, std::vector<uint32_t> &flags);
 };
 
+void build_bfs_trees(mzh_ctx *ctx)
+{
+    const uint32_t N = ctx->n_objs;
+    const uint64_t C = ctx->C;
+    if (C == 0) return;
+    hipStream_t st = ctx->stream;
+    MZH_HIP(ctx->bvh_hdr.ensure(sizeof(uint4) * ctx->node_cap * C));
+    const bool in_lds = bvh_scratch_bytes_host(N) <= kBvhLdsMaxHost;
+    if (!in_lds) MZH_HIP(ctx->bvh_scr.ensure(bvh_scratch_stride(N) * C));
+    Prob q{};
+    q.C = (uint32_t)C;
+    q.N = N;
+    q.node_cap = ctx->node_cap;
+    q.bvh_scr_stride = bvh_scratch_stride(N);
+    q.srt_aabbs = gl(ctx->srt_aabbs.as<float>());
+    q.srt_mcodes = gl(ctx->srt_mcodes.as<uint64_t>());
+    q.n_eff = gl(ctx->n_eff.as<uint32_t>());
+    q.trav = gl(ctx->trav.as<TravNode>());
+    q.bvh_hdr = gl(ctx->bvh_hdr.as<uint4>());
+    q.bvh_scr = gl(ctx->bvh_scr.as<uint8_t>());
+    q.tree_size = gl(ctx->tree_size.as<uint32_t>());
+    std::vector<uint2> items(C);
+    for (uint64_t c = 0; c < C; ++c) items[c] = make_uint2(0u, (uint32_t)c);
+    DevBuf dP, dI, dC;
+    MZH_HIP(dP.ensure(sizeof(Prob)));
+    MZH_HIP(dI.ensure(sizeof(uint2) * C));
+    MZH_HIP(dC.ensure(sizeof(BatchCounters)));
+    MZH_HIP(hipMemcpyAsync(dP.p, &q, sizeof(Prob), hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemcpyAsync(dI.p, items.data(), sizeof(uint2) * C, hipMemcpyHostToDevice, st));
+    MZH_HIP(hipMemsetAsync(dC.p, 0, sizeof(BatchCounters), st));
+    BatchTables t{};
+    t.bvh_items = in_lds ? dI.as<uint2>() : nullptr;
+    t.bvh_hbm_items = in_lds ? nullptr : dI.as<uint2>();
+    t.n_bvh_items = in_lds ? (uint32_t)C : 0u;
+    t.n_bvh_hbm_items = in_lds ? 0u : (uint32_t)C;
+    t.bvh_lds_bytes = in_lds ? (uint32_t)bvh_scratch_bytes_host(N) : 0u;
+    MZH_HIP(launch_bvh(dP.as<Prob>(), 1u, t, dC.as<BatchCounters>(), LaunchShape{}, st));
+    // the temporaries are freed on return
+    MZH_HIP(hipStreamSynchronize(st));
+}
+
 void ScreenBatch::prepare()
 {
     // only the compiled variants (the block tables below follow bp_block)
@@ -202,7 +243,8 @@ void ScreenBatch::prepare()
     MZH_HIP(d_bc.ensure(sizeof(BatchCounters)));
     MZH_HIP(d_pair_pre.ensure(sizeof(uint64_t) * (n + 1)));
     MZH_HIP(d_ctr.ensure(sizeof(unsigned long long) * 8u));
-    std::vector<uint32_t> step_pre(n + 1), aabb_pre(n + 1), nc_pre(n + 1), bp_pre(n + 1), gather_pre(n + 1);
+    std::vector<uint32_t> step_pre(n + 1), aabb_pre(n + 1), nc_pre(n + 1), bp_pre(n + 1), gather_pre(n + 1),
+        lbvh_pre(n + 1);
     max_order = 0;
     max_cand_bytes = 0;
     any_bp = false;
@@ -301,6 +343,7 @@ void ScreenBatch::prepare()
         nc_pre[k + 1] = nc_pre[k] + (uint32_t)((NC + 255u) / 256u);
         bp_pre[k + 1] = bp_pre[k] + (ctx->skip_cd ? 0u : ((N + ls.bp_block - 1u) / ls.bp_block) * (uint32_t)C);
         gather_pre[k + 1] = gather_pre[k] + ((N + 1u + 255u) / 256u) * (uint32_t)C;
+        lbvh_pre[k + 1] = lbvh_pre[k] + ((N + kLbvhBlock - 1u) / kLbvhBlock) * (uint32_t)C;
         any_bp = any_bp || !ctx->skip_cd;
         max_order = std::max(max_order, ctx->order);
         max_cand_bytes = std::max(max_cand_bytes, narrow_cand_bytes(ctx->order));
@@ -315,6 +358,7 @@ void ScreenBatch::prepare()
     n_nc_blocks = nc_pre[n];
     n_bp_blocks = bp_pre[n];
     n_gather_blocks = gather_pre[n];
+    n_lbvh_blocks = lbvh_pre[n];
     // work items of the persistent sort and build, largest problems first
     std::vector<uint32_t> by_size(n);
     std::iota(by_size.begin(), by_size.end(), 0u);
@@ -330,12 +374,13 @@ void ScreenBatch::prepare()
             (in_lds ? bvh_lds : bvh_hbm).push_back(make_uint2(k, c));
         }
     }
-    const size_t n_tab = 5u * (n + 1u);
+    const size_t n_tab = 6u * (n + 1u);
     MZH_HIP(d_tab.ensure(sizeof(uint32_t) * n_tab));
     MZH_HIP(d_items.ensure(sizeof(uint2) * (items.size() + bvh_lds.size() + bvh_hbm.size() + 1u)));
     std::vector<uint32_t> tab;
     tab.reserve(n_tab);
-    for (const auto *v : {&step_pre, &aabb_pre, &nc_pre, &bp_pre, &gather_pre}) tab.insert(tab.end(), v->begin(), v->end());
+    for (const auto *v : {&step_pre, &aabb_pre, &nc_pre, &bp_pre, &gather_pre, &lbvh_pre})
+        tab.insert(tab.end(), v->begin(), v->end());
     MZH_HIP(hipMemcpyAsync(d_tab.p, tab.data(), sizeof(uint32_t) * n_tab, hipMemcpyHostToDevice, st));
     std::vector<uint2> all(items);
     all.insert(all.end(), bvh_lds.begin(), bvh_lds.end());
@@ -347,6 +392,7 @@ void ScreenBatch::prepare()
     tabs.nc_blk = T + 2 * (n + 1);
     tabs.bp_blk = T + 3 * (n + 1);
     tabs.gather_blk = T + 4 * (n + 1);
+    tabs.lbvh_blk = T + 5 * (n + 1);
     const uint2 *I = d_items.as<uint2>();
     tabs.sort_items = I;
     tabs.bvh_items = I + items.size();
@@ -384,8 +430,8 @@ void ScreenBatch::enqueue_stages(bool timed)
     // stage 2: Morton codes, sort, gather
     MZH_HIP(launch_morton_sort(P, n, tabs, n_steps_all, n_gather_blocks, bc, ls, st));
     if (timed) MZH_HIP(hipEventRecord(ev[2], st));
-    // stage 3: BVH
-    MZH_HIP(launch_bvh(P, n, tabs, bc, ls, st));
+    // stage 3: the walk tree
+    MZH_HIP(launch_lbvh(P, n, tabs, n_lbvh_blocks, n_steps_all, st));
     if (timed) MZH_HIP(hipEventRecord(ev[3], st));
     // stage 4: broad phase (counts, scans and the pair prefix)
     MZH_HIP(launch_bp_reset(P, n, tabs, n_nc_blocks, st));
@@ -798,7 +844,6 @@ void ScreenBatch::tune()
     const std::vector<Knob> knobs = {
         {&LaunchShape::bucket_bits, {10, 11, 12}, 1},
         {&LaunchShape::sort_wgs, {256, 512, 1024}, 1},
-        {&LaunchShape::bvh_wgs, {256, 512, 1024}, 2},
         {&LaunchShape::bp_block, {128, 256, 512}, 8},
         {&LaunchShape::roots_wgs, {256, 512, 1024}, 7},
     };

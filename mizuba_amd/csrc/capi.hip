@@ -152,6 +152,12 @@ std::string dev_error_message(const DevError &e)
             return "An invalid value of " + v
                    + " was computed for a candidate minimum of the distance square function at the lower boundary "
                      "of a trajectory";
+        case kErrBvhWalk:
+            return "Internal error: the broad-phase walk of the object at sorted position " + j
+                   + " in the conjunction step " + i + " did not end within its tree";
+        case kErrBvhBuild:
+            return "Internal error: the broad-phase tree of the conjunction step " + i
+                   + " links its node " + j + " outside the tree";
         default:
             return "device error code " + std::to_string(e.code);
     }
@@ -322,11 +328,13 @@ void require_results(const mzh_ctx *ctx)
         throw StatusError{MZH_EINVAL, "The device copies of the results of this screening have been released"};
 }
 
-// The trees are kept in the broad phase's layout; the reference's 48-byte
-// bvh_node records are produced on the first request after a screen.
+// The screening walks a tree of its own numbering; the reference's
+// breadth-first trees and their 48-byte bvh_node records are made on the
+// first request after a screen.
 void export_nodes(mzh_ctx *ctx)
 {
     if (ctx->nodes_ready) return;
+    build_bfs_trees(ctx);
     MZH_HIP(ctx->nodes.ensure(sizeof(mzh_bvh_node) * ctx->node_cap * ctx->C));
     MZH_HIP(launch_bvh_export(ctx->trav.as<TravNode>(), ctx->tree_size.as<uint32_t>(), ctx->nodes.as<mzh_bvh_node>(),
                               ctx->node_cap, (uint32_t)ctx->C, ctx->stream));
@@ -1602,7 +1610,7 @@ int mzh_release(mzh_ctx *ctx, int what)
         MZH_HIP(hipStreamSynchronize(ctx->stream));
         ++ctx->generation;
         if (what & MZH_RELEASE_SCRATCH) {
-            for (DevBuf *b : {&ctx->kbke, &ctx->keys_tmp, &ctx->vals_tmp, &ctx->n_eff, &ctx->sort_todo, &ctx->gbox,
+            for (DevBuf *b : {&ctx->kbke, &ctx->keys_tmp, &ctx->vals_tmp, &ctx->sort_todo, &ctx->gbox,
                               &ctx->bvh_hdr, &ctx->bvh_scr, &ctx->otypes, &ctx->cnt, &ctx->head, &ctx->off,
                               &ctx->step_total, &ctx->slots, &ctx->ovf, &ctx->ovf_count, &ctx->pair_c, &ctx->ev_tm,
                               &ctx->ev_sel,
@@ -1613,6 +1621,7 @@ int mzh_release(mzh_ctx *ctx, int what)
         }
         if (what & MZH_RELEASE_RESULTS) {
             for (DevBuf *b : {&ctx->aabbs, &ctx->srt_aabbs, &ctx->mcodes, &ctx->srt_mcodes, &ctx->srt_idx, &ctx->nodes,
+                              &ctx->n_eff,
                               &ctx->trav, &ctx->tree_size, &ctx->pairs, &ctx->conj_sorted, &ctx->step_off,
                               &ctx->cd_begin, &ctx->cd_end})
                 b->release();

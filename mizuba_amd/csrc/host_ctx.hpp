@@ -234,6 +234,11 @@ void set_plan(mzh_ctx *ctx, ScreenPlan &&pl, double conj_thresh, double conj_det
 // receives each problem's outcome, its message in ctxs[k]->last_error.
 ScreenBatch *batch_new(hipStream_t stream);
 void batch_run(ScreenBatch *b, const std::vector<mzh_ctx *> &ctxs, bool graph, bool tune, int *status);
+// The reference's breadth-first trees (detect_conjunctions_bvh.cpp:267-699)
+// of the context's last screening, into ctx->trav and ctx->tree_size, from
+// its sorted codes and boxes; the screening itself walks a tree of its own
+// numbering. Synchronous on ctx->stream.
+void build_bfs_trees(mzh_ctx *ctx);
 // Graph replays so far, device time of the last run (seconds), launch shape.
 uint64_t batch_replays(const ScreenBatch *b);
 double batch_seconds(const ScreenBatch *b);

@@ -11,6 +11,7 @@
 
 #include "../../include/mizuba_amd.h"
 #include "mzh_enclosure.hpp"
+#include "mzh_lbvh.hpp"
 
 namespace mzh
 {
@@ -59,14 +60,17 @@ struct PjView {
     uint32_t order;
 };
 
-// BVH node as stored on the device: the reference's breadth-first order
-// (detect_conjunctions_bvh.cpp:521-580) with an escape index (the node that
-// follows the subtree in a depth-first walk), so that the broad phase walks the
-// tree without a stack. left is the index of the left child (the right child
-// is left + 1) and 0 for a leaf, since the root is nobody's child. A leaf
-// holds its sorted range [begin, end); an internal node holds begin and, in
-// place of its end, its split (the first position of its right child). The
-// escape of the root's right spine is kTravEnd.
+// BVH node as stored on the device, in one of two numberings of the same
+// tree. The broad phase walks the radix tree of mzh_lbvh.hpp: internal node
+// k at 2k, leaf p at 2p + 1; an internal node holds {escape, right child,
+// left child, 1}, a leaf or a run's root {escape, begin, end, 0}. The
+// reference's breadth-first order (detect_conjunctions_bvh.cpp:521-580),
+// made on request for the tree accessors, holds {escape, begin, end or split,
+// left}: left is the index of the left child (the right child is left + 1)
+// and 0 for a leaf, since the root is nobody's child; an internal node holds
+// its split (the first position of its right child) in place of its end. The
+// escape index is the node that follows the subtree in a depth-first walk;
+// that of the root's right spine is kTravEnd.
 constexpr uint32_t kTravEnd = 0xffffffffu;
 struct alignas(16) TravNode {
     float lb[4];
@@ -104,6 +108,8 @@ enum DevErrCode : int32_t {
     kErrNpBndDerLb = 20,
     kErrNpBndDist2Lb = 21,
     kErrRfInterval = 23,   // invalid root-finding interval (value, i = polynomial)
+    kErrBvhWalk = 24,      // a broad-phase walk longer than its tree (i = step, j = sorted position)
+    kErrBvhBuild = 25,     // a walk-tree node with a parent or sibling outside the tree (i = step, j = node)
 };
 
 __device__ inline void report_error(DevError *e, int32_t code, uint32_t i, uint32_t j, double v)

@@ -112,6 +112,7 @@ struct BatchTables {
     const uint32_t *nc_blk;      // blocks of 256 (step, object) entries
     const uint32_t *bp_blk;      // blocks of the broad-phase walk
     const uint32_t *gather_blk;  // blocks of 256 (step, sorted position) entries of the radix-sort gather
+    const uint32_t *lbvh_blk;    // blocks of kLbvhBlock sorted positions of one step (walk-tree build)
     const uint2 *sort_items;     // (problem, step) work items of the sort, largest problems first
     const uint2 *bvh_items;      // same, problems whose BVH scratch fits in LDS
     const uint2 *bvh_hbm_items;  // same, the others
@@ -152,7 +153,15 @@ hipError_t launch_morton_sort(const Prob *P, uint32_t np, const BatchTables &t, 
 size_t bvh_scratch_bytes_host(uint32_t n_objs);
 size_t bvh_scratch_stride(uint32_t n_objs);
 constexpr size_t kBvhLdsMaxHost = 64u * 1024u;
-// Stage 3: BVH (persistent workgroups over the BVH items).
+// Stage 3: the broad phase's walk tree (radix tree over runs, Karras'
+// construction, mzh_lbvh.hpp) of every step: n_blocks blocks of kLbvhBlock
+// positions (the lbvh_blk table), then one workgroup per step for the nodes
+// that span blocks. Also counts each step's tree size.
+constexpr uint32_t kLbvhBlock = 512;
+hipError_t launch_lbvh(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_blocks, uint32_t n_steps_all,
+                       hipStream_t stream);
+// The reference's breadth-first trees (persistent workgroups over the BVH
+// items), made when a caller asks for a tree.
 hipError_t launch_bvh(const Prob *P, uint32_t np, const BatchTables &t, BatchCounters *bc, const LaunchShape &ls,
                       hipStream_t stream);
 // The reference's bvh_node records of every step's tree, from the device

@@ -340,6 +340,24 @@ __global__ void k_gbox_finalize(const Prob *__restrict__ P, uint32_t np, const u
     }
 }
 
+// The per-step build words of the walk tree in the BVH scratch (node_cap
+// 16-byte words per step, used as 4 node_cap 32-bit words): the parent of each
+// node whose parent's range leaves its block; at node_cap + 2k the arrival
+// counter of such a spanning internal node k and at node_cap + 2k + 1 its
+// split and child flags; the frontier list (nodes finished inside their block
+// whose parent spans blocks); its length, the spanning nodes' count and
+// their list.
+struct LbvhWords {
+    uint32_t *par, *flag, *front, *nfront, *nspan, *span;
+};
+
+__device__ inline LbvhWords lbvh_words(const Prob &q, uint32_t c)
+{
+    uint32_t *h = reinterpret_cast<uint32_t *>(q.bvh_hdr + (size_t)c * q.node_cap);
+    const uint64_t nc = q.node_cap;
+    return LbvhWords{h, h + nc, h + 2u * nc, h + 3u * nc, h + 3u * nc + 1u, h + 3u * nc + 2u};
+}
+
 // One thread per conjunction step of the batch: the step's global box starts
 // empty; the first step of a problem also clears the problem's counters,
 // flags and error word, and thread 0 the batch's work counters.
@@ -357,6 +375,9 @@ __global__ void k_batch_init(const Prob *__restrict__ P, uint32_t np, const uint
     const uint32_t c = s - q.step0;
 #pragma unroll
     for (int d = 0; d < 8; ++d) q.gbox[c * 8u + d] = (d < 4) ? f2ord(__builtin_inff()) : f2ord(-__builtin_inff());
+    q.tree_size[c] = 0u;  // counted by the walk-tree build
+    *lbvh_words(q, c).nfront = 0u;
+    *lbvh_words(q, c).nspan = 0u;
     for (uint32_t k = c; k < kReportWords; k += q.C) q.np_report[k] = 0ull;
     for (uint32_t k = c; k < kPoolShards; k += q.C) q.ovf_count[k * kPoolStride] = 0u;
     if (c == 0) {
@@ -1084,6 +1105,227 @@ __global__ __launch_bounds__(256) void k_bvh_export(const TravNode *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// stage 3, the broad phase's walk tree: the radix tree of mzh_lbvh.hpp (the
+// reference's tree over runs of equal codes, numbered internal node k at 2k,
+// leaf p at 2p + 1), built for every step at once. The reference's
+// breadth-first records (k_bvh) are made only when a caller asks for a tree.
+// ---------------------------------------------------------------------------
+
+__device__ inline void box_min_max(float4 &l, float4 &u, const float4 &bl, const float4 &bu)
+{
+    l.x = l.x < bl.x ? l.x : bl.x;
+    l.y = l.y < bl.y ? l.y : bl.y;
+    l.z = l.z < bl.z ? l.z : bl.z;
+    l.w = l.w < bl.w ? l.w : bl.w;
+    u.x = u.x > bu.x ? u.x : bu.x;
+    u.y = u.y > bu.y ? u.y : bu.y;
+    u.z = u.z > bu.z ? u.z : bu.z;
+    u.w = u.w > bu.w ? u.w : bu.w;
+}
+
+// Split and child flags of an internal node: split << 2 | left_leaf | right_leaf << 1.
+__device__ inline uint32_t lbvh_info(const LbvhNode &nd)
+{
+    return (nd.split << 2) | (nd.left_leaf ? 1u : 0u) | (nd.right_leaf ? 2u : 0u);
+}
+
+__device__ inline uint32_t lbvh_sibling(uint32_t info, uint32_t self)
+{
+    const uint32_t split = info >> 2;
+    const uint32_t lc = lbvh_left(split, (info & 1u) != 0u), rc = lbvh_right(split, (info & 2u) != 0u);
+    return self == lc ? rc : lc;
+}
+
+// The walk's fields of internal node k: escape, then the right and left
+// children (the right child's index halved is the split); a run's root is
+// the walk's leaf for the run: escape, begin, end, 0.
+__device__ inline void lbvh_store_meta(TravNode *T, uint32_t k, const LbvhNode &nd, uint32_t esc)
+{
+    *reinterpret_cast<uint4 *>(&T[2u * k].escape) =
+        nd.run ? make_uint4(esc, nd.lo, nd.hi + 1u, 0u)
+               : make_uint4(esc, lbvh_right(nd.split, nd.right_leaf), lbvh_left(nd.split, nd.left_leaf), 1u);
+}
+
+// The codes of a block's positions [B0 - 1, B1 + 2) staged in LDS.
+struct LdsCodes {
+    const uint64_t *s;
+    uint32_t b0;
+    __device__ uint64_t operator()(uint32_t i) const { return s[i + 1u - b0]; }
+};
+
+// One workgroup per block of BS consecutive sorted positions of one step, one
+// position per thread, with the block's codes and a halo of one position
+// before and two after in LDS. The thread of position p writes leaf p (the
+// object's box) and counts the runs of equal codes (the step's node total,
+// 2 runs - 1, is the reference's tree size); for internal node k = p it finds
+// whether the node's range leaves the block (one code beyond the block,
+// lbvh_leaves), and if not, the node from the block's codes alone
+// (lbvh_node); nodes that leave go on the step's spanning list for
+// k_lbvh_top. Internal boxes are then made bottom-up inside the block: from
+// each leaf a thread climbs to the parent, and of the two children's threads
+// the second to arrive (an LDS counter per node) makes the parent's box from
+// both children's boxes in LDS and climbs on. A node whose parent spans goes
+// on the step's frontier list instead. Boxes are unions (min / max), so the
+// order does not matter.
+template <uint32_t BS>
+__global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, uint32_t np,
+                                                   const uint32_t *__restrict__ blk_pre)
+{
+    __shared__ float4 s_box[2u * BS][2];  // node boxes by local index 2 (k - B0) + leaf
+    __shared__ uint64_t s_code[BS + 3u];  // codes of the positions [B0 - 1, B1 + 2)
+    __shared__ uint16_t s_par[2u * BS];   // parent's slot when the parent lies inside the block
+    __shared__ uint32_t s_info[BS];       // split and child flags of internal node B0 + slot
+    __shared__ uint32_t s_flag[BS];       // arrivals at internal node B0 + slot
+    constexpr uint16_t kNoParent = 0xffffu;
+    static_assert(2u * BS <= kNoParent, "LDS slots");
+    const uint32_t gblk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t pi = find_prob(blk_pre, np, gblk);
+    const Prob &q = P[pi];
+    const uint32_t lblk = gblk - blk_pre[pi];
+    const uint32_t bps = (q.N + BS - 1u) / BS;
+    const uint32_t c = lblk / bps;
+    const uint32_t B0 = (lblk - c * bps) * BS;
+    const uint32_t n = q.n_eff[c];
+    if (B0 >= n) return;  // the whole workgroup
+    const uint32_t B1 = n - B0 < BS ? n : B0 + BS;
+    const uint32_t t = threadIdx.x, p = B0 + t;
+    const bool live = p < B1;
+    const uint64_t *codes = q.srt_mcodes + (size_t)c * q.N;
+    const float *boxes = q.srt_aabbs + (size_t)c * (q.N + 1) * 8u;
+    TravNode *T = q.trav + (size_t)c * q.node_cap;
+    const LbvhWords W = lbvh_words(q, c);
+    const LdsCodes code{s_code, B0};
+    auto local = [&](uint32_t g) { return 2u * ((g >> 1) - B0) + (g & 1u); };
+    for (uint32_t x = t; x < BS + 3u; x += BS) {
+        const int64_t pos = (int64_t)B0 - 1 + x;
+        s_code[x] = pos >= 0 && pos < (int64_t)n ? codes[pos] : 0ull;
+    }
+    s_par[2u * t] = kNoParent;
+    s_par[2u * t + 1u] = kNoParent;
+    s_flag[t] = 0u;
+    __syncthreads();
+    float4 bl = make_float4(0.f, 0.f, 0.f, 0.f), bu = bl;
+    uint32_t runs = 0;
+    if (live) {
+        bl = reinterpret_cast<const float4 *>(boxes + (size_t)p * 8u)[0];
+        bu = reinterpret_cast<const float4 *>(boxes + (size_t)p * 8u)[1];
+        s_box[2u * t + 1u][0] = bl;
+        s_box[2u * t + 1u][1] = bu;
+        uint4 *leaf = reinterpret_cast<uint4 *>(&T[2u * p + 1u]);
+        leaf[0] = make_uint4(__float_as_uint(bl.x), __float_as_uint(bl.y), __float_as_uint(bl.z), __float_as_uint(bl.w));
+        leaf[1] = make_uint4(__float_as_uint(bu.x), __float_as_uint(bu.y), __float_as_uint(bu.z), __float_as_uint(bu.w));
+        leaf[2] = make_uint4(lbvh_escape(code, n, p), p, p + 1u, 0u);
+        runs = p == 0u ? 1u : (code(p) != code(p - 1u) ? 2u : 0u);
+        if (p + 1u < n) {
+            if (lbvh_leaves(code, n, p, B0, B1)) {
+                W.span[atomicAdd(W.nspan, 1u)] = p;
+            } else {
+                const LbvhNode nd = lbvh_node(code, n, p, B0, B1);
+                lbvh_store_meta(T, p, nd, lbvh_escape(code, n, nd.hi));
+                s_info[t] = lbvh_info(nd);
+                // inside the block, so are its children
+                s_par[local(lbvh_left(nd.split, nd.left_leaf))] = (uint16_t)t;
+                s_par[local(lbvh_right(nd.split, nd.right_leaf))] = (uint16_t)t;
+            }
+        }
+    }
+    {
+        uint32_t r = runs;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+        if ((t & 63u) == 0u && r) atomicAdd(&q.tree_size[c], r);
+    }
+    __syncthreads();
+    if (!live) return;
+    const uint32_t root = lbvh_root(n);
+    uint32_t self = 2u * p + 1u;
+    float4 l = bl, u = bu;
+    // a climb is at most as long as the tree is deep (96 key bits)
+    for (uint32_t guard = 0; guard < 128u; ++guard) {
+        const uint32_t pa = s_par[local(self)];
+        if (pa == kNoParent) {
+            if (self != root) W.front[atomicAdd(W.nfront, 1u)] = self;
+            break;
+        }
+        // the first child to arrive leaves; its box is in LDS for the second
+        const uint32_t old = __hip_atomic_fetch_add(&s_flag[pa], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0u) break;
+        const uint32_t sib = local(lbvh_sibling(s_info[pa], self));
+        box_min_max(l, u, s_box[sib][0], s_box[sib][1]);
+        s_box[2u * pa][0] = l;
+        s_box[2u * pa][1] = u;
+        float4 *nb = reinterpret_cast<float4 *>(&T[2u * (B0 + pa)]);
+        nb[0] = l;
+        nb[1] = u;
+        self = 2u * (B0 + pa);
+    }
+}
+
+// The nodes that span blocks, one workgroup per step: first the spanning
+// nodes themselves (from the step's codes in global memory, each with its own
+// thread), their children's parent links and counters; then the same
+// bottom-up climb as inside the blocks, from the step's frontier list, with
+// the counters and split words in the BVH scratch and the boxes in the node
+// records.
+__global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, uint32_t np,
+                                                   const uint32_t *__restrict__ step_pre)
+{
+    const uint32_t s = blockIdx.x;
+    const Prob &q = P[find_prob(step_pre, np, s)];
+    const uint32_t c = s - q.step0;
+    const uint32_t n = q.n_eff[c];
+    if (n < 2u) return;
+    const LbvhWords W = lbvh_words(q, c);
+    const uint64_t *codes = q.srt_mcodes + (size_t)c * q.N;
+    TravNode *T = q.trav + (size_t)c * q.node_cap;
+    const uint32_t ns = min(*W.nspan, n - 1u);
+    for (uint32_t x = threadIdx.x; x < ns; x += blockDim.x) {
+        const uint32_t k = W.span[x];
+        if (k + 1u >= n) {
+            report_error(q.err, kErrBvhBuild, c, 2u * k, 0.0);
+            continue;
+        }
+        const LbvhNode nd = lbvh_node(codes, n, k);
+        lbvh_store_meta(T, k, nd, lbvh_escape(codes, n, nd.hi));
+        W.par[lbvh_left(nd.split, nd.left_leaf)] = 2u * k;
+        W.par[lbvh_right(nd.split, nd.right_leaf)] = 2u * k;
+        W.flag[2u * k] = 0u;
+        W.flag[2u * k + 1u] = lbvh_info(nd);
+    }
+    __syncthreads();
+    const uint32_t nf = min(*W.nfront, 2u * n);
+    for (uint32_t f = threadIdx.x; f < nf; f += blockDim.x) {
+        uint32_t self = W.front[f];
+        if (self >= 2u * n) {
+            report_error(q.err, kErrBvhBuild, c, self, 0.0);
+            break;
+        }
+        float4 l = *reinterpret_cast<const float4 *>(T[self].lb);
+        float4 u = *reinterpret_cast<const float4 *>(T[self].ub);
+        for (uint32_t guard = 0; guard < 128u && self != 0u; ++guard) {
+            const uint32_t pa = W.par[self];
+            if (pa >= 2u * n || (pa & 1u)) {
+                report_error(q.err, kErrBvhBuild, c, self, 0.0);
+                break;
+            }
+            const uint32_t old =
+                __hip_atomic_fetch_add(&W.flag[pa], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) break;
+            const uint32_t sib = lbvh_sibling(W.flag[pa + 1u], self);
+            if (sib >= 2u * n) {
+                report_error(q.err, kErrBvhBuild, c, pa, 0.0);
+                break;
+            }
+            box_min_max(l, u, *reinterpret_cast<const float4 *>(T[sib].lb), *reinterpret_cast<const float4 *>(T[sib].ub));
+            float4 *nb = reinterpret_cast<float4 *>(&T[pa]);
+            nb[0] = l;
+            nb[1] = u;
+            self = pa;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // stage 4: broad phase (detect_conjunctions_broad_phase.cpp:153-339)
 // One lane per (step, Morton-sorted query object); stack traversal with
 // closed-interval 4-D overlap; a pair (i, j) is emitted from the query of the
@@ -1166,7 +1408,6 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
     const uint32_t N = q.N;
     const uint64_t node_cap = q.node_cap;
     const TravNode *__restrict__ trav = q.trav;
-    const uint32_t *__restrict__ tree_size = q.tree_size;
     const float *__restrict__ srt_aabbs = q.srt_aabbs;
     const uint64_t *__restrict__ srt_mcodes = q.srt_mcodes;
     const uint32_t *__restrict__ srt_idx = q.srt_idx;
@@ -1182,8 +1423,7 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
     const uint32_t c = lblk / bps;
     const uint32_t o = (lblk - c * bps) * BS + threadIdx.x;
     const uint32_t n = n_eff[c];
-    const uint32_t nn = tree_size[c];
-    if (o >= n || nn == 0) return;
+    if (o >= n) return;
     const TravNode *T = trav + (size_t)c * node_cap;
     const uint32_t *vidx = srt_idx + (size_t)c * N;
     const float *boxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
@@ -1232,21 +1472,27 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
             if (ot + otypes[oi] <= 3) add(oi);
         }
     }
-    uint32_t i = 0;
-    while (i < nn) {
+    // the walk tree (mzh_lbvh.hpp): a depth-first walk visits each of its
+    // 2n - 1 reachable nodes at most once
+    const uint32_t lim = 2u * n;
+    uint32_t i = lbvh_root(n), visits = 0;
+    while (i < lim) {
+        if (++visits > lim) {
+            report_error(q.err, kErrBvhWalk, c, o, 0.0);
+            break;
+        }
         const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
         const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
         const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
-        if (!box_overlap(ql, qu, nl, nu)) {
-            i = meta.x;
-            continue;
-        }
-        if (meta.w != 0u) {
-            // internal node: skip the left child when it ends at or before the run
-            i = meta.z <= re ? meta.w + 1u : meta.w;
-            continue;
-        }
-        if (meta.y >= re) {
+        // the next node is chosen from all four words on every path, so that
+        // the record is read with one load: an internal node {escape, right,
+        // left, 1} is entered, skipping the left child when it ends at or
+        // before the run (the split is the right child's index halved); a
+        // leaf {escape, begin, end, 0}, or a node missed, is left by its escape
+        const bool hit_node = box_overlap(ql, qu, nl, nu);
+        const bool enter = hit_node && meta.w != 0u;
+        const uint32_t next = enter ? ((meta.y >> 1) <= re ? meta.y : meta.z) : meta.x;
+        if (hit_node && meta.w == 0u && meta.y >= re) {
             const bool single = meta.z - meta.y == 1u;
             // box_o against the leaf box, needed when the query box is larger than box_o
             const bool o_hits_leaf = !multi || box_overlap(bl, bu, nl, nu);
@@ -1267,7 +1513,7 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
                 if (hit) add(oi);
             }
         }
-        i = meta.x;
+        i = next;
     }
     flush();
 }
@@ -1535,6 +1781,19 @@ hipError_t launch_morton_sort(const Prob *P, uint32_t np, const BatchTables &t, 
 size_t bvh_scratch_stride(uint32_t n_objs) { return (bvh_scratch_bytes(n_objs) + 255u) / 256u * 256u; }
 size_t bvh_scratch_bytes_host(uint32_t n_objs) { return bvh_scratch_bytes(n_objs); }
 static_assert(kBvhLdsMaxHost == kBvhLdsMax, "BVH LDS bound");
+
+hipError_t launch_lbvh(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_blocks, uint32_t n_steps_all,
+                       hipStream_t stream)
+{
+    if (n_blocks) {
+        hipLaunchKernelGGL(k_lbvh_build<kLbvhBlock>, dim3(n_blocks), dim3(kLbvhBlock), 0, stream, P, np, t.lbvh_blk);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (n_steps_all == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lbvh_top, dim3(n_steps_all), dim3(1024), 0, stream, P, np, t.steps);
+    return hipGetLastError();
+}
 
 hipError_t launch_bvh(const Prob *P, uint32_t np, const BatchTables &t, BatchCounters *bc, const LaunchShape &ls,
                       hipStream_t stream)
