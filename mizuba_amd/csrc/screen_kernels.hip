@@ -1136,14 +1136,13 @@ __device__ inline uint32_t lbvh_sibling(uint32_t info, uint32_t self)
     return self == lc ? rc : lc;
 }
 
-// The walk's fields of internal node k: escape, then the right and left
+// The walk's fields of an internal node: escape, then the right and left
 // children (the right child's index halved is the split); a run's root is
 // the walk's leaf for the run: escape, begin, end, 0.
-__device__ inline void lbvh_store_meta(TravNode *T, uint32_t k, const LbvhNode &nd, uint32_t esc)
+__device__ inline uint4 lbvh_meta(const LbvhNode &nd, uint32_t esc)
 {
-    *reinterpret_cast<uint4 *>(&T[2u * k].escape) =
-        nd.run ? make_uint4(esc, nd.lo, nd.hi + 1u, 0u)
-               : make_uint4(esc, lbvh_right(nd.split, nd.right_leaf), lbvh_left(nd.split, nd.left_leaf), 1u);
+    return nd.run ? make_uint4(esc, nd.lo, nd.hi + 1u, 0u)
+                  : make_uint4(esc, lbvh_right(nd.split, nd.right_leaf), lbvh_left(nd.split, nd.left_leaf), 1u);
 }
 
 // The codes of a block's positions [B0 - 1, B1 + 2) staged in LDS.
@@ -1164,7 +1163,9 @@ struct LdsCodes {
 // k_lbvh_top. Internal boxes are then made bottom-up inside the block: from
 // each leaf a thread climbs to the parent, and of the two children's threads
 // the second to arrive (an LDS counter per node) makes the parent's box from
-// both children's boxes in LDS and climbs on. A node whose parent spans goes
+// both children's boxes in LDS, writes the parent's whole record (walk fields
+// kept in LDS until then, so that each record is written once) and climbs on.
+// A node whose parent spans goes
 // on the step's frontier list instead. Boxes are unions (min / max), so the
 // order does not matter.
 template <uint32_t BS>
@@ -1174,6 +1175,7 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
     __shared__ float4 s_box[2u * BS][2];  // node boxes by local index 2 (k - B0) + leaf
     __shared__ uint64_t s_code[BS + 3u];  // codes of the positions [B0 - 1, B1 + 2)
     __shared__ uint16_t s_par[2u * BS];   // parent's slot when the parent lies inside the block
+    __shared__ uint4 s_meta[BS];          // walk fields of internal node B0 + slot
     __shared__ uint32_t s_info[BS];       // split and child flags of internal node B0 + slot
     __shared__ uint32_t s_flag[BS];       // arrivals at internal node B0 + slot
     constexpr uint16_t kNoParent = 0xffffu;
@@ -1211,6 +1213,7 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
         bu = reinterpret_cast<const float4 *>(boxes + (size_t)p * 8u)[1];
         s_box[2u * t + 1u][0] = bl;
         s_box[2u * t + 1u][1] = bu;
+        // the leaf's whole record (the walk reads a node's box and fields together)
         uint4 *leaf = reinterpret_cast<uint4 *>(&T[2u * p + 1u]);
         leaf[0] = make_uint4(__float_as_uint(bl.x), __float_as_uint(bl.y), __float_as_uint(bl.z), __float_as_uint(bl.w));
         leaf[1] = make_uint4(__float_as_uint(bu.x), __float_as_uint(bu.y), __float_as_uint(bu.z), __float_as_uint(bu.w));
@@ -1221,7 +1224,7 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
                 W.span[atomicAdd(W.nspan, 1u)] = p;
             } else {
                 const LbvhNode nd = lbvh_node(code, n, p, B0, B1);
-                lbvh_store_meta(T, p, nd, lbvh_escape(code, n, nd.hi));
+                s_meta[t] = lbvh_meta(nd, lbvh_escape(code, n, nd.hi));
                 s_info[t] = lbvh_info(nd);
                 // inside the block, so are its children
                 s_par[local(lbvh_left(nd.split, nd.left_leaf))] = (uint16_t)t;
@@ -1248,15 +1251,21 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
             break;
         }
         // the first child to arrive leaves; its box is in LDS for the second
-        const uint32_t old = __hip_atomic_fetch_add(&s_flag[pa], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // (the climb exchanges data through LDS only: its fences order LDS
+        // accesses and leave the node records' global stores in flight)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        const uint32_t old = __hip_atomic_fetch_add(&s_flag[pa], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0u) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         const uint32_t sib = local(lbvh_sibling(s_info[pa], self));
         box_min_max(l, u, s_box[sib][0], s_box[sib][1]);
         s_box[2u * pa][0] = l;
         s_box[2u * pa][1] = u;
-        float4 *nb = reinterpret_cast<float4 *>(&T[2u * (B0 + pa)]);
-        nb[0] = l;
-        nb[1] = u;
+        // the whole record at once
+        uint4 *nb = reinterpret_cast<uint4 *>(&T[2u * (B0 + pa)]);
+        nb[0] = make_uint4(__float_as_uint(l.x), __float_as_uint(l.y), __float_as_uint(l.z), __float_as_uint(l.w));
+        nb[1] = make_uint4(__float_as_uint(u.x), __float_as_uint(u.y), __float_as_uint(u.z), __float_as_uint(u.w));
+        nb[2] = s_meta[pa];
         self = 2u * (B0 + pa);
     }
 }
@@ -1286,7 +1295,7 @@ __global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, u
             continue;
         }
         const LbvhNode nd = lbvh_node(codes, n, k);
-        lbvh_store_meta(T, k, nd, lbvh_escape(codes, n, nd.hi));
+        *reinterpret_cast<uint4 *>(&T[2u * k].escape) = lbvh_meta(nd, lbvh_escape(codes, n, nd.hi));
         W.par[lbvh_left(nd.split, nd.left_leaf)] = 2u * k;
         W.par[lbvh_right(nd.split, nd.right_leaf)] = 2u * k;
         W.flag[2u * k] = 0u;
