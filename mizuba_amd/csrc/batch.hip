@@ -126,7 +126,7 @@ private:
 
     // device tables and batch-wide state
     DevBuf d_P, d_tab, d_items, d_bc, d_pair_pre, d_ctr, d_counts, d_flags, d_np, d_err;
-    DevBuf steps, steps2, cand, slow, wl_slow, conj_raw, conj_step, conj_keys;
+    DevBuf steps, cand, slow, wl_slow, conj_raw, conj_step, conj_keys;
     uint64_t steps_cap = 0, cand_cap = 0, slow_cap = 0, conj_cap = 0;
     std::vector<Prob> hP;
     BatchTables tabs{};
@@ -137,7 +137,6 @@ private:
     };
     std::vector<Group> groups;
     bool any_bp = false;
-    bool any_mixed = false;  // a mixed-precision problem: the narrow phase's pre-screen runs
     uint32_t max_order = 0;
     size_t max_cand_bytes = 0;
 
@@ -248,7 +247,6 @@ void ScreenBatch::prepare()
     max_order = 0;
     max_cand_bytes = 0;
     any_bp = false;
-    any_mixed = false;
     groups.clear();
     for (uint32_t k = 0; k < n; ++k) {
         mzh_ctx *ctx = probs[k];
@@ -260,6 +258,7 @@ void ScreenBatch::prepare()
         MZH_HIP(ctx->cd_end.ensure(sizeof(double) * C));
         MZH_HIP(ctx->aabbs.ensure(boxes_bytes));
         MZH_HIP(ctx->kbke.ensure(sizeof(uint2) * NC));
+        if (ctx->mode == MZH_MODE_MIXED) MZH_HIP(ctx->nprec.ensure(sizeof(float4) * 2u * NC));
         MZH_HIP(ctx->srt_aabbs.ensure(boxes_bytes));
         MZH_HIP(ctx->mcodes.ensure(sizeof(uint64_t) * NC));
         MZH_HIP(ctx->srt_mcodes.ensure(sizeof(uint64_t) * NC));
@@ -307,6 +306,7 @@ void ScreenBatch::prepare()
         q.aabbs = gl(ctx->aabbs.as<float>());
         q.srt_aabbs = gl(ctx->srt_aabbs.as<float>());
         q.kbke = gl(ctx->kbke.as<uint2>());
+        q.nprec = gl(ctx->nprec.as<float4>());
         q.gbox = gl(ctx->gbox.as<uint32_t>());
         q.mcodes = gl(ctx->mcodes.as<uint64_t>());
         q.srt_mcodes = gl(ctx->srt_mcodes.as<uint64_t>());
@@ -351,7 +351,6 @@ void ScreenBatch::prepare()
             groups.push_back(Group{ctx->order, k, k + 1, aabb_pre[k], 0, false, false});
         groups.back().p_end = k + 1;
         (ctx->mode == MZH_MODE_MIXED ? groups.back().any_mixed : groups.back().any_fp64) = true;
-        any_mixed = any_mixed || ctx->mode == MZH_MODE_MIXED;
         groups.back().n_blk = aabb_pre[k + 1] - groups.back().blk_begin;
     }
     n_steps_all = step_pre[n];
@@ -461,8 +460,6 @@ NarrowArgs ScreenBatch::narrow_args(const Group &g) const
     na.steps = steps.p;
     na.steps_cap = steps_cap;
     na.steps_count = ctr + 1;
-    na.steps2 = any_mixed ? steps2.p : nullptr;
-    na.steps2_count = ctr + 7;
     na.cand = cand.p;
     na.cand_cap = cand_cap;
     na.cand_count = ctr + 2;
@@ -498,7 +495,6 @@ void ScreenBatch::enqueue_narrow(const std::vector<uint64_t> &group_pairs)
 void ScreenBatch::ensure_queues()
 {
     MZH_HIP(steps.ensure(sizeof(StepPair) * (steps_cap ? steps_cap : 1)));
-    if (any_mixed) MZH_HIP(steps2.ensure(sizeof(StepPair) * (steps_cap ? steps_cap : 1)));
     MZH_HIP(cand.ensure(max_cand_bytes * (cand_cap ? cand_cap : 1)));
     MZH_HIP(slow.ensure(sizeof(uint64_t) * (slow_cap ? slow_cap : 1)));
     MZH_HIP(conj_raw.grow(sizeof(mzh_conj) * (conj_cap ? conj_cap : 1), 0, st));

@@ -187,7 +187,8 @@ PjView pj_view(const mzh_ctx *ctx)
     v.time = gl(ctx->pjb->time.as<const double>());
     v.cf4 = gl(ctx->pjb->cf4.as<const double>());
     v.cf4f = gl(ctx->pjb->cf4f.as<const float>());
-    v.dbnd = gl(ctx->pjb->dbnd.as<const float>());
+    v.sbnd = gl(ctx->pjb->sbnd.as<const float4>());
+    v.lam = gl(ctx->pjb->lam.as<const uint32_t>());
     v.step0 = gl(ctx->pjb->step0.as<const uint64_t>());
     v.nsteps = gl(ctx->pjb->nsteps.as<const uint64_t>());
     v.time_off = gl(ctx->pjb->time_off.as<const uint64_t>());
@@ -466,7 +467,8 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * (time_len ? time_len : 1)));
         MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
         MZH_HIP(ctx->pjb->cf4f.ensure(sizeof(float) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
-        MZH_HIP(ctx->pjb->dbnd.ensure(sizeof(float) * order * (steps_total ? steps_total : 1)));
+        MZH_HIP(ctx->pjb->sbnd.ensure(sizeof(float4) * (steps_total ? steps_total : 1)));
+        MZH_HIP(ctx->pjb->lam.ensure(sizeof(uint32_t)));
         MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_objs));
         MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_objs));
         MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_objs));
@@ -477,10 +479,10 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         MZH_HIP(hipMemcpyAsync(ctx->pjb->time_off.p, time_off, sizeof(uint64_t) * n_objs, hipMemcpyHostToDevice, st));
         MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), ctx->pjb->cf4f.as<float>(),
                                  steps_total, order, st));
-        MZH_HIP(launch_deriv_bounds(ctx->pjb->cf4.as<double>(), ctx->pjb->time.as<double>(),
-                                    ctx->pjb->step0.as<uint64_t>(), ctx->pjb->nsteps.as<uint64_t>(),
-                                    ctx->pjb->time_off.as<uint64_t>(), n_objs, steps_total, order,
-                                    ctx->pjb->dbnd.as<float>(), st));
+        MZH_HIP(launch_step_bounds(ctx->pjb->cf4.as<double>(), ctx->pjb->time.as<double>(),
+                                   ctx->pjb->step0.as<uint64_t>(), ctx->pjb->nsteps.as<uint64_t>(),
+                                   ctx->pjb->time_off.as<uint64_t>(), n_objs, steps_total, order,
+                                   ctx->pjb->sbnd.as<float4>(), ctx->pjb->lam.as<uint32_t>(), st));
         MZH_HIP(hipStreamSynchronize(st));
         ctx->n_objs = n_objs;
         ctx->order = order;
@@ -759,7 +761,8 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     MZH_HIP(ctx->pjb->time.ensure(sizeof(double) * time_len));
     MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * op1 * steps_total));
     MZH_HIP(ctx->pjb->cf4f.ensure(sizeof(float) * 4u * op1 * steps_total));
-    MZH_HIP(ctx->pjb->dbnd.ensure(sizeof(float) * order * steps_total));
+    MZH_HIP(ctx->pjb->sbnd.ensure(sizeof(float4) * steps_total));
+    MZH_HIP(ctx->pjb->lam.ensure(sizeof(uint32_t)));
     MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_sats));
     MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_sats));
     MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_sats));
@@ -775,9 +778,9 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     MZH_HIP(hipEventRecord(E[6], st));
     MZH_HIP(launch_transpose_cf4(ctx->pjb->traj.as<double>(), ctx->pjb->cf4.as<double>(), ctx->pjb->cf4f.as<float>(),
                                  steps_total, order, st));
-    MZH_HIP(launch_deriv_bounds(ctx->pjb->cf4.as<double>(), ctx->pjb->time.as<double>(), ctx->pjb->step0.as<uint64_t>(),
-                                ctx->pjb->nsteps.as<uint64_t>(), ctx->pjb->time_off.as<uint64_t>(), n_sats, steps_total,
-                                order, ctx->pjb->dbnd.as<float>(), st));
+    MZH_HIP(launch_step_bounds(ctx->pjb->cf4.as<double>(), ctx->pjb->time.as<double>(), ctx->pjb->step0.as<uint64_t>(),
+                               ctx->pjb->nsteps.as<uint64_t>(), ctx->pjb->time_off.as<uint64_t>(), n_sats, steps_total,
+                               order, ctx->pjb->sbnd.as<float4>(), ctx->pjb->lam.as<uint32_t>(), st));
     MZH_HIP(hipEventRecord(E[7], st));
     MZH_HIP(hipStreamSynchronize(st));
     {
@@ -1610,7 +1613,7 @@ int mzh_release(mzh_ctx *ctx, int what)
         MZH_HIP(hipStreamSynchronize(ctx->stream));
         ++ctx->generation;
         if (what & MZH_RELEASE_SCRATCH) {
-            for (DevBuf *b : {&ctx->kbke, &ctx->keys_tmp, &ctx->vals_tmp, &ctx->sort_todo, &ctx->gbox,
+            for (DevBuf *b : {&ctx->kbke, &ctx->nprec, &ctx->keys_tmp, &ctx->vals_tmp, &ctx->sort_todo, &ctx->gbox,
                               &ctx->bvh_hdr, &ctx->bvh_scr, &ctx->otypes, &ctx->cnt, &ctx->head, &ctx->off,
                               &ctx->step_total, &ctx->slots, &ctx->ovf, &ctx->ovf_count, &ctx->pair_c, &ctx->ev_tm,
                               &ctx->ev_sel,

@@ -16,13 +16,16 @@ namespace mzh
 hipError_t launch_transpose_cf4(const double *traj, double *cf4, float *cf4f, uint64_t n_steps_total, uint32_t order,
                                 hipStream_t stream);
 
-// Upload-time bounds on the position derivatives of every trajectory step
-// (PjView::dbnd): for a = 1..order, D_a >= max over the step of the Euclidean
-// norm of the a-th derivative of (x, y, z), from the coefficients of cf4 and
-// the step lengths, rounded up.
-hipError_t launch_deriv_bounds(const double *cf4, const double *time, const uint64_t *step0, const uint64_t *nsteps,
-                               const uint64_t *time_off, uint32_t n_objs, uint64_t n_steps_total, uint32_t order,
-                               float *dbnd, hipStream_t stream);
+// Upload-time bounds of every trajectory step for the mixed-precision
+// pre-screen (PjView::sbnd, PjView::lam), from the coefficients of cf4 and
+// the step lengths, rounded up: D2 >= max over the step of |d^2 (x, y, z) /
+// dt^2|; A with |d^a (x, y, z) / dt^a| <= A lam^a for a = 1..order, lam (one
+// float for the polyjectory, as its bits) the largest of the steps' growth
+// rates; Jv and Jd >= the jumps of position and velocity from the object's
+// previous step.
+hipError_t launch_step_bounds(const double *cf4, const double *time, const uint64_t *step0, const uint64_t *nsteps,
+                              const uint64_t *time_off, uint32_t n_objs, uint64_t n_steps_total, uint32_t order,
+                              float4 *sbnd, uint32_t *lam_bits, hipStream_t stream);
 
 // ---- a batch of screening problems ------------------------------------------
 // Conjunction steps handled by one lane of the AABB kernel.
@@ -73,6 +76,7 @@ struct Prob {
     // the problem's buffers (its context's)
     gbuf<float> aabbs, srt_aabbs;  // [C][N + 1][2][4]
     gbuf<uint2> kbke;  // [C][N]
+    gbuf<float4> nprec;  // [C][N][2] mixed-precision pre-screen model of each (step, object), mixed problems
     gbuf<uint32_t> gbox;  // [C][8] order-preserving images of the global box
     gbuf<uint64_t> mcodes, srt_mcodes, keys_tmp;
     gbuf<uint32_t> srt_idx, vals_tmp, n_eff, sort_todo;
@@ -213,10 +217,6 @@ struct NarrowArgs {
     void *steps;                  // StepPair records
     uint64_t steps_cap;
     unsigned long long *steps_count;
-    // with mixed-precision problems (else null): the step pairs the
-    // pre-screen keeps (steps_cap records), which the screen reads
-    void *steps2;
-    unsigned long long *steps2_count;
     // queue of step-pairs whose dist^2 enclosure reaches below thresh^2
     void *cand;                   // records of narrow_cand_bytes(order) bytes
     uint64_t cand_cap;

@@ -656,72 +656,31 @@ __device__ inline void translate1s(const double *__restrict__ src, double a, dou
     }
 }
 
-// Narrow phase, part 1a: one lane per broad-phase pair. Merge-walk of the
-// trajectory steps of i and j over the conjunction step
-// (narrow_phase.cpp:357-420); every temporally overlapping pair of steps is
-// appended to the step-pair list (wave-aggregated atomics).
-// Mixed-precision pre-screen of one overlapping step pair (i, ki), (j, kj)
-// on the interval [lb, ub]: true when no conjunction of the FP64 narrow phase
-// can come from it, so that it need not be screened. Mixed mode owes only the
-// FP64 conjunction list, and every conjunction (a minimum, or a boundary
-// value, of the dist^2 interpolant p below thresh^2) needs p < thresh^2 on the
-// interval. The test bounds the true distance from below and p from the true
-// dist^2 F:
-//  - d(t) = r_i - r_j around the midpoint tm: d(tm) + d'(tm) s plus a
-//    remainder of at most (D2_i + D2_j) s^2 / 2 (|s| <= w, the step's
-//    derivative bounds dbnd), with d(tm) and d'(tm) from the float copy of the
-//    coefficients in FP64 (error at most 2^-24 of the magnitude sums, plus the
-//    FP64 Horner error, covered by 6.1e-8 of them);
-//  - the segment's least norm: |d0 x v| / |v| (a bound for the whole line),
-//    or the end point when the line's closest point is clearly outside;
-//  - |p - F| <= 2 (h/4)^n1 / n1! max|F^(n1)| for the n1 = order + 1
-//    Chebyshev nodes of the first kind on [0, h], with F^(n1) = sum_{a=1..order}
-//    C(n1, a) d^(a) . d^(n1 - a) bounded by the D_a of both steps, plus a
-//    relative 1e-9 of the magnitudes for the FP64 path's rounding.
-// Non-finite values never reject.
-template <int ORDER>
-__device__ inline bool np_far(const PjView &pj, uint32_t i, uint32_t j, uint64_t ki, uint64_t kj, double tsi,
-                              double tsj, double lb, double ub, double thresh)
+// Mixed-precision pre-screen of a broad-phase pair (i, j) over its
+// conjunction step, from the two objects' models (k_aabbs): true when no
+// conjunction of the FP64 narrow phase can come from any of the pair's step
+// pairs. Mixed mode owes only the FP64 conjunction list, and every conjunction
+// (a minimum, or a boundary value, of a step pair's dist^2 interpolant p
+// below thresh^2) needs p < thresh^2 somewhere in the step. Over the step,
+// with s = t - tm in [-w, w], r_i(t) - r_j(t) is within R_i + R_j of the
+// segment d0 + v s (d0 = p0_i - p0_j, v = v0_i - v0_j), whose least norm is
+// |d0 x v| / |v| (a bound for the whole line), or an end point's when the
+// line's closest point is clearly outside; and |p - F| <= E_i + E_j for the
+// true dist^2 F, plus a relative 1e-9 of the magnitudes for the FP64 path's
+// rounding. Non-finite values never reject.
+__device__ inline bool np_pair_far(const float4 &ai, const float4 &bi, const float4 &aj, const float4 &bj, double w,
+                                   double thresh)
 {
-    constexpr int OP1 = ORDER + 1;
-    const double h = ub - lb;
-    const double tm = lb + 0.5 * h;
-    const double w = fmax(tm - lb, ub - tm);
-    const uint64_t si = pj.step0[i] + ki, sj = pj.step0[j] + kj;
-    const float *ci = pj.cf4f + si * 4u * OP1;
-    const float *cj = pj.cf4f + sj * 4u * OP1;
-    const double taui = tm - tsi, tauj = tm - tsj;
-    const double ai = fabs(taui), aj = fabs(tauj);
-    double dd = 0, vv = 0, dv = 0, cx = 0, cy = 0, cz = 0, e2 = 0, dn2 = 0;
-    double d0[3], v[3];
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        double pi = (double)ci[ch * OP1 + ORDER], dpi = 0.0, mi = fabs(pi), dmi = 0.0;
-        double pj_ = (double)cj[ch * OP1 + ORDER], dpj = 0.0, mj = fabs(pj_), dmj = 0.0;
-#pragma unroll
-        for (int k = ORDER - 1; k >= 0; --k) {
-            const double xi = (double)ci[ch * OP1 + k], xj = (double)cj[ch * OP1 + k];
-            dpi = dpi * taui + pi;
-            dmi = dmi * ai + mi;
-            pi = pi * taui + xi;
-            mi = mi * ai + fabs(xi);
-            dpj = dpj * tauj + pj_;
-            dmj = dmj * aj + mj;
-            pj_ = pj_ * tauj + xj;
-            mj = mj * aj + fabs(xj);
-        }
-        d0[ch] = pi - pj_;
-        v[ch] = dpi - dpj;
-        const double e = 6.1e-8 * ((mi + mj) + (dmi + dmj) * w);
-        e2 += e * e;
-        dn2 += d0[ch] * d0[ch];
-    }
-    dd = dn2;
-    vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-    dv = d0[0] * v[0] + d0[1] * v[1] + d0[2] * v[2];
-    cx = d0[1] * v[2] - d0[2] * v[1];
-    cy = d0[2] * v[0] - d0[0] * v[2];
-    cz = d0[0] * v[1] - d0[1] * v[0];
+    const double R = (double)ai.w + (double)aj.w, E = (double)bi.w + (double)bj.w;
+    if (!(R < 1e300) || !(E < 1e300)) return false;
+    const double d0[3] = {(double)ai.x - (double)aj.x, (double)ai.y - (double)aj.y, (double)ai.z - (double)aj.z};
+    const double v[3] = {(double)bi.x - (double)bj.x, (double)bi.y - (double)bj.y, (double)bi.z - (double)bj.z};
+    const double dd = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2];
+    const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double dv = d0[0] * v[0] + d0[1] * v[1] + d0[2] * v[2];
+    const double cx = d0[1] * v[2] - d0[2] * v[1];
+    const double cy = d0[2] * v[0] - d0[0] * v[2];
+    const double cz = d0[0] * v[1] - d0[1] * v[0];
     const double dn = sqrt(dd), vn = sqrt(vv);
     double m;
     const double s0 = vv > 0.0 ? -dv / vv : 0.0;
@@ -737,25 +696,18 @@ __device__ inline bool np_far(const PjView &pj, uint32_t i, uint32_t j, uint64_t
     } else {
         m = dn;
     }
-    const float *bi = pj.dbnd + si * ORDER, *bj = pj.dbnd + sj * ORDER;
-    double D[ORDER + 1];
-    D[0] = 0.0;
-#pragma unroll
-    for (int a = 1; a <= ORDER; ++a) D[a] = (double)bi[a - 1] + (double)bj[a - 1];
-    const double mag = dn + vn * w + D[2] * w * w;
-    const double m_lb = m - sqrt(e2) - 0.5 * D[2] * w * w - 1e-12 * mag;
-    // the interpolation error of the n1-node Chebyshev interpolant of dist^2
-    double f9 = 0.0;
-#pragma unroll
-    for (int a = 1; a <= ORDER; ++a) f9 += c_binom_np.v[OP1][a] * D[a] * D[OP1 - a];
-    double q = 2.0;
-#pragma unroll
-    for (int k = 1; k <= OP1; ++k) q *= (h * 0.25) / (double)k;
-    const double e_int = q * f9;
-    const double lim = thresh * thresh + e_int + 1e-9 * mag * mag + 1e-9;
+    const double mag = dn + vn * w + R;
+    const double m_lb = m - R - 1e-12 * mag;
+    const double lim = thresh * thresh + E + 1e-9 * mag * mag + 1e-9;
     return m_lb > 0.0 && m_lb * m_lb * (1.0 - 1e-12) > lim;
 }
 
+// Narrow phase, part 1a: one lane per broad-phase pair. Merge-walk of the
+// trajectory steps of i and j over the conjunction step
+// (narrow_phase.cpp:357-420); every temporally overlapping pair of steps is
+// appended to the step-pair list (wave-aggregated atomics). The step pairs of
+// a mixed-precision problem's pair that np_pair_far rules out are counted as
+// candidates rejected by the distance check and not appended.
 __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
 {
     const uint64_t g0 = a.pair_pre[a.p_first], g1 = a.pair_pre[a.p_end];
@@ -765,6 +717,7 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
     uint32_t i = 0, j = 0, c = 0, p = 0;
     const double *ti = nullptr, *tj = nullptr;
     double cb = 0, ce = 0;
+    bool far = false;
     const bool live = g < g_end;
     if (live) {
         p = find_prob64(a.pair_pre, a.np, g);
@@ -783,6 +736,11 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
         kei = ri.y;
         kbj = rj.x;
         kej = rj.y;
+        if (q.mode == MZH_MODE_MIXED) {
+            const float4 *mi = q.nprec + ((size_t)c * q.N + i) * 2u, *mj = q.nprec + ((size_t)c * q.N + j) * 2u;
+            const double tm = cb + 0.5 * (ce - cb);
+            far = np_pair_far(mi[0], mi[1], mj[0], mj[1], fmax(tm - cb, ce - tm), q.thresh);
+        }
     }
     // the merge-walk visits step pairs in order; the first kEnumBuf
     // overlapping ones wait in registers while the block's offsets are
@@ -811,9 +769,11 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
         ++cnt;
     });
     {
-        const unsigned long long v = cnt;
-        add_prob_report<1>(a.P, p, live, 0, &v);
+        // candidates, and those the pre-screen rejects
+        const unsigned long long v[2] = {cnt, far ? cnt : 0u};
+        add_prob_report<2>(a.P, p, live, 0, v);
     }
+    if (far) cnt = 0;
     __shared__ uint32_t s_w[17];
     __shared__ unsigned long long s_base;
     uint32_t total;
@@ -834,56 +794,6 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
     }
 }
 
-// Narrow phase, part 1a' (batches with mixed-precision problems): one lane per
-// queued step pair. The step pairs of mixed-precision problems that np_far
-// rules out are counted as rejected by the distance check; the others (and
-// every step pair of an FP64 problem) go on to the second queue, which the
-// screen then reads. Order within the queue does not matter: candidates and
-// conjunctions are sorted later.
-template <int ORDER>
-__global__ __launch_bounds__(256) void k_np_prescreen(NarrowArgs a)
-{
-    const unsigned long long avail = *a.steps_count;
-    const uint64_t nsp = avail < a.steps_cap ? avail : a.steps_cap;
-    const uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-    unsigned long long n_rej = 0;
-    bool keep = false;
-    uint32_t p = 0;
-    StepPair sp{};
-    const bool live = q < nsp;
-    if (live) {
-        sp = static_cast<const StepPair *>(a.steps)[q];
-        p = sp.p;
-        const Prob &pb = a.P[p];
-        keep = true;
-        if (pb.mode == MZH_MODE_MIXED) {
-            const PjView &pj = pb.pj;
-            const double cb = pb.cd_begin[sp.c], ce = pb.cd_end[sp.c];
-            const double *ti = pj.time + pj.time_off[sp.i];
-            const double *tj = pj.time + pj.time_off[sp.j];
-            const double tsi = ti[sp.ki], tei = ti[sp.ki + 1], tsj = tj[sp.kj], tej = tj[sp.kj + 1];
-            const double lb_i = cb > tsi ? cb : tsi, ub_i = ce < tei ? ce : tei;
-            const double lb_j = cb > tsj ? cb : tsj, ub_j = ce < tej ? ce : tej;
-            const double lb = lb_i > lb_j ? lb_i : lb_j, ub = ub_i < ub_j ? ub_i : ub_j;
-            if (ub > lb && np_far<ORDER>(pj, sp.i, sp.j, sp.ki, sp.kj, tsi, tsj, lb, ub, pb.thresh)) {
-                keep = false;
-                n_rej = 1;
-            }
-        }
-    }
-    add_prob_report<1>(a.P, p, live, 1, &n_rej);
-    __shared__ uint32_t s_w[17];
-    __shared__ unsigned long long s_base;
-    uint32_t total;
-    const uint32_t ex = block_excl_scan_u32(keep ? 1u : 0u, s_w, total);
-    if (threadIdx.x == 0) s_base = total ? atomicAdd(a.steps2_count, (unsigned long long)total) : 0ull;
-    __syncthreads();
-    if (keep) {
-        const unsigned long long slot = s_base + ex;
-        if (slot < a.steps_cap) static_cast<StepPair *>(a.steps2)[slot] = sp;
-    }
-}
-
 // Narrow phase, part 1b: one lane per overlapping step-pair. Root-finding
 // interval and translations (narrow_phase.cpp:404-488), dist^2
 // interpolation (:95-152) and the Cargo-Shisha screen (:502-513); survivors
@@ -892,7 +802,7 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
 {
     constexpr int OP1 = ORDER + 1;
-    const unsigned long long avail = a.steps2 ? *a.steps2_count : *a.steps_count;
+    const unsigned long long avail = *a.steps_count;
     const uint64_t nsp = avail < a.steps_cap ? avail : a.steps_cap;
     const uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     unsigned long long n_rej = 0;
@@ -903,7 +813,7 @@ __global__ __launch_bounds__(256) void k_np_screen(NarrowArgs a)
     double d2[OP1];
     const bool live = q < nsp;
     if (live) {
-        const StepPair sp = static_cast<const StepPair *>(a.steps2 ? a.steps2 : a.steps)[q];
+        const StepPair sp = static_cast<const StepPair *>(a.steps)[q];
         p = sp.p;
         i = sp.i;
         j = sp.j;
@@ -1136,8 +1046,7 @@ __global__ __launch_bounds__(256) void k_np_roots_slow(NarrowArgs a)
 // End of a narrow-phase pass: the queue lengths folded into peak, the queue
 // counters cleared for the next pass.
 __global__ void k_np_pass_end(unsigned long long *steps_count, unsigned long long *cand_count,
-                              unsigned long long *slow_count, unsigned long long *peak,
-                              unsigned long long *steps2_count)
+                              unsigned long long *slow_count, unsigned long long *peak)
 {
     if (threadIdx.x != 0) return;
     peak[0] = peak[0] > *steps_count ? peak[0] : *steps_count;
@@ -1146,22 +1055,19 @@ __global__ void k_np_pass_end(unsigned long long *steps_count, unsigned long lon
     *steps_count = 0;
     *cand_count = 0;
     *slow_count = 0;
-    *steps2_count = 0;
 }
 
 // Before the first pass: the problems' narrow-phase counters, the
 // conjunction count, the queue counters and their peaks start at zero.
 __global__ void k_np_reset(const Prob *__restrict__ P, uint32_t np, unsigned long long *conj_count,
                            unsigned long long *steps_count, unsigned long long *cand_count,
-                           unsigned long long *slow_count, unsigned long long *peak,
-                           unsigned long long *steps2_count)
+                           unsigned long long *slow_count, unsigned long long *peak)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < np * kReportWords) P[t / kReportWords].np_report[t % kReportWords] = 0ull;
     if (t == 0) {
         *conj_count = 0;
         *steps_count = 0;
-        *steps2_count = 0;
         *cand_count = 0;
         *slow_count = 0;
         peak[0] = peak[1] = peak[2] = 0;
@@ -1441,11 +1347,6 @@ hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream)
     if (e != hipSuccess) return e;
     // one lane per step-pair; the list length is read on the device
     const uint64_t sblocks = (a.steps_cap + 255) / 256;
-    if (a.steps2) {
-        hipLaunchKernelGGL(k_np_prescreen<ORDER>, dim3((unsigned)sblocks), dim3(256), 0, stream, a);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
     hipLaunchKernelGGL(k_np_screen<ORDER>, dim3((unsigned)sblocks), dim3(256), 0, stream, a);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1462,8 +1363,7 @@ hipError_t launch_narrow_order(const NarrowArgs &a, hipStream_t stream)
     hipLaunchKernelGGL(k_np_roots_slow<ORDER>, dim3(kNarrowSlowLanes / 256), dim3(256), 0, stream, a);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_np_pass_end, dim3(1), dim3(64), 0, stream, a.steps_count, a.cand_count, a.slow_count, a.peak,
-                       a.steps2_count);
+    hipLaunchKernelGGL(k_np_pass_end, dim3(1), dim3(64), 0, stream, a.steps_count, a.cand_count, a.slow_count, a.peak);
     return hipGetLastError();
 }
 
@@ -1683,7 +1583,7 @@ hipError_t launch_conj_sort(const Prob *P, uint32_t np, const uint32_t *step_pre
 hipError_t launch_narrow_reset(const NarrowArgs &a, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_np_reset, dim3((a.np * kReportWords + 255) / 256), dim3(256), 0, stream, a.P, a.np, a.conj_count,
-                       a.steps_count, a.cand_count, a.slow_count, a.peak, a.steps2_count);
+                       a.steps_count, a.cand_count, a.slow_count, a.peak);
     return hipGetLastError();
 }
 #endif // MZH_NARROW_HOST
