@@ -258,7 +258,7 @@ void ScreenBatch::prepare()
         MZH_HIP(ctx->cd_end.ensure(sizeof(double) * C));
         MZH_HIP(ctx->aabbs.ensure(boxes_bytes));
         MZH_HIP(ctx->kbke.ensure(sizeof(uint2) * NC));
-        if (ctx->mode == MZH_MODE_MIXED) MZH_HIP(ctx->nprec.ensure(sizeof(float4) * 2u * NC));
+        if (ctx->mode == MZH_MODE_MIXED) MZH_HIP(ctx->nprec.ensure(sizeof(float4) * 3u * NC));
         MZH_HIP(ctx->srt_aabbs.ensure(boxes_bytes));
         MZH_HIP(ctx->mcodes.ensure(sizeof(uint64_t) * NC));
         MZH_HIP(ctx->srt_mcodes.ensure(sizeof(uint64_t) * NC));

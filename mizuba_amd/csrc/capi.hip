@@ -188,7 +188,7 @@ PjView pj_view(const mzh_ctx *ctx)
     v.cf4 = gl(ctx->pjb->cf4.as<const double>());
     v.cf4f = gl(ctx->pjb->cf4f.as<const float>());
     v.sbnd = gl(ctx->pjb->sbnd.as<const float4>());
-    v.lam = gl(ctx->pjb->lam.as<const uint32_t>());
+    v.scal = gl(ctx->pjb->scal.as<const uint32_t>());
     v.step0 = gl(ctx->pjb->step0.as<const uint64_t>());
     v.nsteps = gl(ctx->pjb->nsteps.as<const uint64_t>());
     v.time_off = gl(ctx->pjb->time_off.as<const uint64_t>());
@@ -468,7 +468,7 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
         MZH_HIP(ctx->pjb->cf4f.ensure(sizeof(float) * 4u * (order + 1) * (steps_total ? steps_total : 1)));
         MZH_HIP(ctx->pjb->sbnd.ensure(sizeof(float4) * (steps_total ? steps_total : 1)));
-        MZH_HIP(ctx->pjb->lam.ensure(sizeof(uint32_t)));
+        MZH_HIP(ctx->pjb->scal.ensure(2 * sizeof(uint32_t)));
         MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_objs));
         MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_objs));
         MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_objs));
@@ -482,7 +482,7 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
         MZH_HIP(launch_step_bounds(ctx->pjb->cf4.as<double>(), ctx->pjb->time.as<double>(),
                                    ctx->pjb->step0.as<uint64_t>(), ctx->pjb->nsteps.as<uint64_t>(),
                                    ctx->pjb->time_off.as<uint64_t>(), n_objs, steps_total, order,
-                                   ctx->pjb->sbnd.as<float4>(), ctx->pjb->lam.as<uint32_t>(), st));
+                                   ctx->pjb->sbnd.as<float4>(), ctx->pjb->scal.as<uint32_t>(), st));
         MZH_HIP(hipStreamSynchronize(st));
         ctx->n_objs = n_objs;
         ctx->order = order;
@@ -762,7 +762,7 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
     MZH_HIP(ctx->pjb->cf4.ensure(sizeof(double) * 4u * op1 * steps_total));
     MZH_HIP(ctx->pjb->cf4f.ensure(sizeof(float) * 4u * op1 * steps_total));
     MZH_HIP(ctx->pjb->sbnd.ensure(sizeof(float4) * steps_total));
-    MZH_HIP(ctx->pjb->lam.ensure(sizeof(uint32_t)));
+    MZH_HIP(ctx->pjb->scal.ensure(2 * sizeof(uint32_t)));
     MZH_HIP(ctx->pjb->step0.ensure(sizeof(uint64_t) * n_sats));
     MZH_HIP(ctx->pjb->nsteps.ensure(sizeof(uint64_t) * n_sats));
     MZH_HIP(ctx->pjb->time_off.ensure(sizeof(uint64_t) * n_sats));
@@ -780,7 +780,7 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
                                  steps_total, order, st));
     MZH_HIP(launch_step_bounds(ctx->pjb->cf4.as<double>(), ctx->pjb->time.as<double>(), ctx->pjb->step0.as<uint64_t>(),
                                ctx->pjb->nsteps.as<uint64_t>(), ctx->pjb->time_off.as<uint64_t>(), n_sats, steps_total,
-                               order, ctx->pjb->sbnd.as<float4>(), ctx->pjb->lam.as<uint32_t>(), st));
+                               order, ctx->pjb->sbnd.as<float4>(), ctx->pjb->scal.as<uint32_t>(), st));
     MZH_HIP(hipEventRecord(E[7], st));
     MZH_HIP(hipStreamSynchronize(st));
     {

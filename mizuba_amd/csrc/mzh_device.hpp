@@ -50,12 +50,12 @@ struct PjView {
     gbuf<const double> time;      // per object nsteps + 1 values
     gbuf<const double> cf4;       // step-major transposed [S][4][op1], channels x y z r
     gbuf<const float> cf4f;       // the same rounded to float (the mixed-precision AABBs)
-    // [S]: {D2, A, Jv, Jd} of each step (launch_step_bounds): a bound on
-    // |d^2 (x, y, z) / dt^2| over the step, A with |d^a (x, y, z) / dt^a| <=
-    // A lam^a for a = 1..order, and the jumps of position and velocity from
-    // the object's previous step; lam: one float (as its bits)
+    // [S]: {D3, Jv, Jd, Ja} of each step (launch_step_bounds): a bound on
+    // |d^3 (x, y, z) / dt^3| over the step and the jumps of position,
+    // velocity and acceleration from the object's previous step; scal: the
+    // polyjectory's lam and A (float bits), |d^a (x, y, z) / dt^a| <= A lam^a
     gbuf<const float4> sbnd;
-    gbuf<const uint32_t> lam;
+    gbuf<const uint32_t> scal;
     gbuf<const uint64_t> step0;   // global index of the object's first step
     gbuf<const uint64_t> nsteps;  // steps per object
     gbuf<const uint64_t> time_off;

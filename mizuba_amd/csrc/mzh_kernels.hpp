@@ -17,15 +17,15 @@ hipError_t launch_transpose_cf4(const double *traj, double *cf4, float *cf4f, ui
                                 hipStream_t stream);
 
 // Upload-time bounds of every trajectory step for the mixed-precision
-// pre-screen (PjView::sbnd, PjView::lam), from the coefficients of cf4 and
-// the step lengths, rounded up: D2 >= max over the step of |d^2 (x, y, z) /
-// dt^2|; A with |d^a (x, y, z) / dt^a| <= A lam^a for a = 1..order, lam (one
-// float for the polyjectory, as its bits) the largest of the steps' growth
-// rates; Jv and Jd >= the jumps of position and velocity from the object's
-// previous step.
+// pre-screen (PjView::sbnd, PjView::scal), from the coefficients of cf4 and
+// the step lengths, rounded up: D3 >= max over the step of |d^3 (x, y, z) /
+// dt^3|; Jv, Jd and Ja >= the jumps of position, velocity and acceleration
+// from the object's previous step; and for the polyjectory (two floats, as
+// their bits) lam and A with |d^a (x, y, z) / dt^a| <= A lam^a over every
+// step, a = 1..order.
 hipError_t launch_step_bounds(const double *cf4, const double *time, const uint64_t *step0, const uint64_t *nsteps,
                               const uint64_t *time_off, uint32_t n_objs, uint64_t n_steps_total, uint32_t order,
-                              float4 *sbnd, uint32_t *lam_bits, hipStream_t stream);
+                              float4 *sbnd, uint32_t *scal, hipStream_t stream);
 
 // ---- a batch of screening problems ------------------------------------------
 // Conjunction steps handled by one lane of the AABB kernel.
@@ -76,7 +76,7 @@ struct Prob {
     // the problem's buffers (its context's)
     gbuf<float> aabbs, srt_aabbs;  // [C][N + 1][2][4]
     gbuf<uint2> kbke;  // [C][N]
-    gbuf<float4> nprec;  // [C][N][2] mixed-precision pre-screen model of each (step, object), mixed problems
+    gbuf<float4> nprec;  // [C][N][3] mixed-precision pre-screen model of each (step, object), mixed problems
     gbuf<uint32_t> gbox;  // [C][8] order-preserving images of the global box
     gbuf<uint64_t> mcodes, srt_mcodes, keys_tmp;
     gbuf<uint32_t> srt_idx, vals_tmp, n_eff, sort_todo;

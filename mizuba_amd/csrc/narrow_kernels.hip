@@ -662,19 +662,22 @@ __device__ inline void translate1s(const double *__restrict__ src, double a, dou
 // pairs. Mixed mode owes only the FP64 conjunction list, and every conjunction
 // (a minimum, or a boundary value, of a step pair's dist^2 interpolant p
 // below thresh^2) needs p < thresh^2 somewhere in the step. Over the step,
-// with s = t - tm in [-w, w], r_i(t) - r_j(t) is within R_i + R_j of the
-// segment d0 + v s (d0 = p0_i - p0_j, v = v0_i - v0_j), whose least norm is
-// |d0 x v| / |v| (a bound for the whole line), or an end point's when the
-// line's closest point is clearly outside; and |p - F| <= E_i + E_j for the
-// true dist^2 F, plus a relative 1e-9 of the magnitudes for the FP64 path's
-// rounding. Non-finite values never reject.
-__device__ inline bool np_pair_far(const float4 &ai, const float4 &bi, const float4 &aj, const float4 &bj, double w,
-                                   double thresh)
+// with s = t - tm in [-w, w], r_i(t) - r_j(t) is within R_i + R_j of
+// d0 + v s + a s^2 / 2 (the differences of the models' position, velocity and
+// acceleration), hence within R_i + R_j + |a| w^2 / 2 of the segment d0 + v s,
+// whose least norm is |d0 x v| / |v| (a bound for the whole line), or an end
+// point's when the line's closest point is clearly outside; and |p - F| <=
+// E_i + E_j for the true dist^2 F, plus a relative 1e-9 of the magnitudes
+// for the FP64 path's rounding. Non-finite values never reject.
+__device__ inline bool np_pair_far(const float4 *mi, const float4 *mj, double w, double thresh)
 {
+    const float4 ai = mi[0], bi = mi[1], ci = mi[2], aj = mj[0], bj = mj[1], cj = mj[2];
     const double R = (double)ai.w + (double)aj.w, E = (double)bi.w + (double)bj.w;
     if (!(R < 1e300) || !(E < 1e300)) return false;
     const double d0[3] = {(double)ai.x - (double)aj.x, (double)ai.y - (double)aj.y, (double)ai.z - (double)aj.z};
     const double v[3] = {(double)bi.x - (double)bj.x, (double)bi.y - (double)bj.y, (double)bi.z - (double)bj.z};
+    const double ax = (double)ci.x - (double)cj.x, ay = (double)ci.y - (double)cj.y, az = (double)ci.z - (double)cj.z;
+    const double an = sqrt(ax * ax + ay * ay + az * az);
     const double dd = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2];
     const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     const double dv = d0[0] * v[0] + d0[1] * v[1] + d0[2] * v[2];
@@ -696,8 +699,9 @@ __device__ inline bool np_pair_far(const float4 &ai, const float4 &bi, const flo
     } else {
         m = dn;
     }
-    const double mag = dn + vn * w + R;
-    const double m_lb = m - R - 1e-12 * mag;
+    const double curv = 0.5 * an * w * w;
+    const double mag = dn + vn * w + curv + R;
+    const double m_lb = m - curv - R - 1e-12 * mag;
     const double lim = thresh * thresh + E + 1e-9 * mag * mag + 1e-9;
     return m_lb > 0.0 && m_lb * m_lb * (1.0 - 1e-12) > lim;
 }
@@ -737,9 +741,9 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
         kbj = rj.x;
         kej = rj.y;
         if (q.mode == MZH_MODE_MIXED) {
-            const float4 *mi = q.nprec + ((size_t)c * q.N + i) * 2u, *mj = q.nprec + ((size_t)c * q.N + j) * 2u;
+            const float4 *mi = q.nprec + ((size_t)c * q.N + i) * 3u, *mj = q.nprec + ((size_t)c * q.N + j) * 3u;
             const double tm = cb + 0.5 * (ce - cb);
-            far = np_pair_far(mi[0], mi[1], mj[0], mj[1], fmax(tm - cb, ce - tm), q.thresh);
+            far = np_pair_far(mi, mj, fmax(tm - cb, ce - tm), q.thresh);
         }
     }
     // the merge-walk visits step pairs in order; the first kEnumBuf
@@ -759,15 +763,26 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
     constexpr uint32_t kEnumBuf = 4;
     uint32_t bki[kEnumBuf] = {}, bkj[kEnumBuf] = {};
     uint32_t cnt = 0;
-    walk([&](uint64_t ki, uint64_t kj) {
+    const uint64_t ni = kei - kbi, nj = kej - kbj;
+    if (far && (ni == 1u || nj == 1u)) {
+        // a pair the pre-screen rules out is only counted: both objects'
+        // pieces cover the whole step (their models exist), so every piece
+        // of one overlaps the other's single piece
+        cnt = (uint32_t)(ni > nj ? ni : nj);
+    } else if (far && ni == 2u && nj == 2u) {
+        // three overlapping step pairs, or two when the inner boundaries meet
+        cnt = ti[kbi + 1u] == tj[kbj + 1u] ? 2u : 3u;
+    } else {
+        walk([&](uint64_t ki, uint64_t kj) {
 #pragma unroll
-        for (uint32_t m = 0; m < kEnumBuf; ++m)
-            if (m == cnt) {
-                bki[m] = (uint32_t)ki;
-                bkj[m] = (uint32_t)kj;
-            }
-        ++cnt;
-    });
+            for (uint32_t m = 0; m < kEnumBuf; ++m)
+                if (m == cnt) {
+                    bki[m] = (uint32_t)ki;
+                    bkj[m] = (uint32_t)kj;
+                }
+            ++cnt;
+        });
+    }
     {
         // candidates, and those the pre-screen rejects
         const unsigned long long v[2] = {cnt, far ? cnt : 0u};

@@ -60,23 +60,24 @@ __global__ void k_transpose_cf4(const double *__restrict__ traj, double *__restr
 
 // Bounds of one trajectory step for the mixed-precision pre-screen (one lane
 // per step of the concatenated steps), each rounded up to float:
-// sbnd = {D2, A, Jv, Jd}. Over the step, tau in [0, H],
+// sbnd = {D3, Jv, Jd, Ja}. Over the step, tau in [0, H],
 //   |d^a x / dtau^a| <= B_a = sum_{k >= a} k! / (k - a)! |c_k| H^(k - a)
 // per channel (Horner in H on positive terms, so the rounding is at most
 // 2 order ulps, covered by the factor 1 + 2^-40), and D_a = |(B_a,x, B_a,y,
-// B_a,z)| bounds the a-th derivative of the position; D2 = D_2. Jv and Jd:
-// the jumps of position and velocity from the end of the object's previous
-// step to the start of this one (0 for its first step), from FP64
-// evaluations bounded with 1e-14 of their magnitude sums. A: with lam the
-// polyjectory's growth rate, D_a <= A lam^a for a = 1..order. Two passes:
-// the first (AMP false) writes D2, Jv, Jd and raises lam to the step's own
-// rate max_a (D_a / D_1)^(1 / (a - 1)); the second (AMP true) sets
-// A = max_a D_a / lam^a, which holds whatever lam is.
+// B_a,z)| bounds the a-th derivative of the position; D3 = D_3 (0 for order
+// 2). Jv, Jd and Ja: the jumps of position, velocity and acceleration from
+// the end of the object's previous step to the start of this one (0 for its
+// first step), from FP64 evaluations bounded with 1e-14 of their magnitude
+// sums. Two passes for the polyjectory's scalars lam and A (as float bits),
+// with D_a <= A lam^a for every step and a = 1..order: the first (AMP false)
+// raises lam to each step's own rate max_a (D_a / D_1)^(1 / (a - 1)); the
+// second (AMP true) raises A to each step's max_a D_a / lam^a, which holds
+// whatever lam is.
 template <int ORDER, bool AMP>
 __global__ void k_step_bounds(const double *__restrict__ cf4, const double *__restrict__ time,
                               const uint64_t *__restrict__ step0, const uint64_t *__restrict__ nsteps,
                               const uint64_t *__restrict__ time_off, uint32_t n_objs, uint64_t n_steps_total,
-                              float4 *__restrict__ sbnd, uint32_t *__restrict__ lam_bits)
+                              float4 *__restrict__ sbnd, uint32_t *__restrict__ scal)
 {
     constexpr int OP1 = ORDER + 1;
     constexpr double up = 1.0 + 0x1p-40;
@@ -101,7 +102,7 @@ __global__ void k_step_bounds(const double *__restrict__ cf4, const double *__re
     double acc[ORDER];
 #pragma unroll
     for (int a = 0; a < ORDER; ++a) acc[a] = 0.0;
-    double jv2 = 0.0, jd2 = 0.0;
+    double jv2 = 0.0, jd2 = 0.0, ja2 = 0.0;
     const double Hp = k > 0 ? fabs(t[k] - t[k - 1]) : 0.0;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
@@ -117,11 +118,13 @@ __global__ void k_step_bounds(const double *__restrict__ cf4, const double *__re
             acc[a - 1] += b * b;
         }
         if (!AMP && k > 0) {
-            // the previous step's position and velocity at its end
+            // the previous step's position, velocity and half acceleration at its end
             const double *cp = cf4 + ((s - 1u) * 4u + (uint64_t)ch) * OP1;
-            double v = cp[ORDER], dv = 0.0, mv = fabs(cp[ORDER]), md = 0.0;
+            double v = cp[ORDER], dv = 0.0, hv = 0.0, mv = fabs(cp[ORDER]), md = 0.0, mh = 0.0;
 #pragma unroll
             for (int kk = ORDER - 1; kk >= 0; --kk) {
+                hv = hv * Hp + dv;
+                mh = mh * Hp + md;
                 dv = dv * Hp + v;
                 md = md * Hp + mv;
                 v = v * Hp + cp[kk];
@@ -129,8 +132,10 @@ __global__ void k_step_bounds(const double *__restrict__ cf4, const double *__re
             }
             const double ev = fabs(c[0] - v) + 1e-14 * (mv + fabs(c[0]));
             const double ed = fabs(c[1] - dv) + 1e-14 * (md + fabs(c[1]));
+            const double ea = 2.0 * (fabs(c[2] - hv) + 1e-14 * (mh + fabs(c[2])));
             jv2 += ev * ev;
             jd2 += ed * ed;
+            ja2 += ea * ea;
         }
     }
     if constexpr (!AMP) {
@@ -144,11 +149,12 @@ __global__ void k_step_bounds(const double *__restrict__ cf4, const double *__re
             }
         }
         // non-negative floats order as their bit patterns
-        if (rate > 0.0) atomicMax(lam_bits, __float_as_uint(next_up_f((float)rate)));
-        sbnd[s] = make_float4(next_up_f((float)(sqrt(acc[1]) * up)), 0.0f, next_up_f((float)(sqrt(jv2) * up)),
-                              next_up_f((float)(sqrt(jd2) * up)));
+        if (rate > 0.0) atomicMax(&scal[0], __float_as_uint(next_up_f((float)rate)));
+        const double d3 = ORDER >= 3 ? sqrt(acc[ORDER >= 3 ? 2 : 0]) : 0.0;
+        sbnd[s] = make_float4(next_up_f((float)(d3 * up)), next_up_f((float)(sqrt(jv2) * up)),
+                              next_up_f((float)(sqrt(jd2) * up)), next_up_f((float)(sqrt(ja2) * up)));
     } else {
-        const double lam = (double)__uint_as_float(*lam_bits);
+        const double lam = (double)__uint_as_float(scal[0]);
         double amp = 0.0, la = 1.0;
 #pragma unroll
         for (int a = 1; a <= ORDER; ++a) {
@@ -156,28 +162,28 @@ __global__ void k_step_bounds(const double *__restrict__ cf4, const double *__re
             const double d = sqrt(acc[a - 1]) * up;
             if (d > 0.0) amp = fmax(amp, la > 0.0 ? d / la : __builtin_inf());
         }
-        sbnd[s].y = next_up_f((float)(amp * (1.0 + 1e-12)));
+        if (amp > 0.0) atomicMax(&scal[1], __float_as_uint(next_up_f((float)(amp * (1.0 + 1e-12)))));
     }
 }
 
 template <int ORDER>
 hipError_t dispatch_step_bounds(uint32_t order, const double *cf4, const double *time, const uint64_t *step0,
                                 const uint64_t *nsteps, const uint64_t *time_off, uint32_t n_objs,
-                                uint64_t n_steps_total, float4 *sbnd, uint32_t *lam_bits, hipStream_t stream)
+                                uint64_t n_steps_total, float4 *sbnd, uint32_t *scal, hipStream_t stream)
 {
     if (order == (uint32_t)ORDER) {
         const dim3 g((unsigned)((n_steps_total + 255) / 256));
         hipLaunchKernelGGL((k_step_bounds<ORDER, false>), g, dim3(256), 0, stream, cf4, time, step0, nsteps, time_off,
-                           n_objs, n_steps_total, sbnd, lam_bits);
+                           n_objs, n_steps_total, sbnd, scal);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((k_step_bounds<ORDER, true>), g, dim3(256), 0, stream, cf4, time, step0, nsteps, time_off,
-                           n_objs, n_steps_total, sbnd, lam_bits);
+                           n_objs, n_steps_total, sbnd, scal);
         return hipGetLastError();
     }
     if constexpr (ORDER < kMaxOrder) {
         return dispatch_step_bounds<ORDER + 1>(order, cf4, time, step0, nsteps, time_off, n_objs, n_steps_total, sbnd,
-                                               lam_bits, stream);
+                                               scal, stream);
     } else {
         return hipErrorInvalidValue;
     }
@@ -204,12 +210,14 @@ constexpr uint32_t kAabbGroup = kAabbStepGroup;
 //
 // The mixed-precision problems also get each (object, step)'s pre-screen
 // model (nprec, read by the narrow phase's np_pair_far): over the step
-// [cb, ce] with midpoint tm and half-width w, the position p0 and velocity v0
-// at tm, and R >= |r(t) - p0 - v0 (t - tm)| for every t in [cb, ce]:
-//  - on the piece holding tm, Taylor: (D2 / 2) (t - tm)^2;
-//  - across each piece boundary tb inside the step, the same with D2 the
-//    largest of the pieces', plus the jumps there, Jv + Jd |t - tb|
-//    (|t - tb| <= 2w);
+// [cb, ce] with midpoint tm and half-width w, the position p0, velocity v0 and
+// acceleration a0 at tm, and R >= |r(t) - p0 - v0 s - a0 s^2 / 2| for every
+// t in [cb, ce], s = t - tm:
+//  - on the piece holding tm, Taylor: (D3 / 6) |s|^3;
+//  - across each piece boundary tb inside the step, the same with D3 the
+//    largest of the pieces', plus the jumps there, Jv + Jd |t - tb| +
+//    Ja |t - tb|^2 / 2 (|t - tb| <= 2w; each jump of the value or a
+//    derivative carries on as a Taylor polynomial past the boundary);
 //  - p0, v0 evaluated in FP32 from the float coefficients, within
 //    (4 order + 8) 6.1e-8 of their magnitude sums of the FP64 polynomial's
 //    values (coefficient rounding, FP32 Horner and the rounding of
@@ -217,12 +225,18 @@ constexpr uint32_t kAabbGroup = kAabbStepGroup;
 // and E, the object's share of the FP64 path's Chebyshev interpolation error
 // for any step pair inside the step, 2 (h'/4)^n1 / n1! max |F^(n1)| with
 // h' <= h = ce - cb and F^(n1) = sum_{a=1..order} C(n1, a) d^(a) . d^(n1-a),
-// |d^(a)| <= (A_i + A_j) lam^a (A the largest of the pieces'): at most
-// 2 (h/4)^n1 / n1! (2^n1 - 2) lam^n1 (A_i + A_j)^2 <= E_i + E_j with
-// E = 4 (2^n1 - 2) (h/4)^n1 / n1! lam^n1 A^2. Objects whose data do not cover
-// the whole step get R = infinity (never screened out).
+// |d^(a)| <= 2 A lam^a (the polyjectory's A and lam): at most
+// 8 (2^n1 - 2) (h/4)^n1 / n1! (lam^n1 A^2) = E_i + E_j with E half of it.
+// Objects whose data do not cover the whole step get R = infinity (never
+// screened out).
+// Two waves per SIMD for the mixed-precision kernels up to order 9, whose
+// registers then just fit (none spilled at order 9, 12 bytes at order 8);
+// the others as their registers allow.
 template <int ORDER, bool F32>
-__global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint32_t np,
+constexpr unsigned kAabbWaves = (F32 && ORDER <= 9) ? 2u : 1u;
+
+template <int ORDER, bool F32>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAabbWaves<ORDER, F32>))) void k_aabbs(const Prob *__restrict__ P, uint32_t np,
                                                const uint32_t *__restrict__ blk_pre, uint32_t blk_begin)
 {
     using CT = typename std::conditional<F32, float, double>::type;
@@ -263,12 +277,13 @@ __global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint3
     float4 sb_cached = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // the step bounds of the cached piece (mixed)
     // error of the FP32 model at tm off the FP64 polynomial, per unit of the
     // magnitude sums (value: sum |c_k| |tau|^k; velocity: sum k |c_k|
-    // |tau|^(k-1)), in units of 2^-24: the coefficients' float rounding (1),
-    // the FP32 Horner evaluation (2 order) and tau = tm - t_k rounded to float
-    // (|p'| |tau| 2^-24 <= order 2^-24 of the value's sum, and likewise for the
-    // velocity); with room to spare
+    // |tau|^(k-1); half acceleration: sum C(k, 2) |c_k| |tau|^(k-2)), in units
+    // of 2^-24: the coefficients' float rounding (1), the FP32 Horner
+    // evaluation (2 order) and tau = tm - t_k rounded to float (|p'| |tau|
+    // 2^-24 <= order 2^-24 of the value's sum, and likewise for the
+    // derivatives); with room to spare
     constexpr double kModelErr = (4.0 * ORDER + 8.0) * 6.1e-8;
-    // 4 (2^n1 - 2) / n1!, rounded up
+    // 4 (2^n1 - 2) / n1!, rounded up (each object's half of the pair's bound)
     constexpr double kInterpConst = [] {
         double f = 1.0;
         for (int kk = 2; kk <= ORDER + 1; ++kk) f *= (double)kk;
@@ -309,9 +324,9 @@ __global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint3
             // the pre-screen model (mixed precision)
             const double tm = cb + 0.5 * (ce - cb), mw = fmax(tm - cb, ce - tm);
             const bool full = F32 && ns > 0 && t[0] <= cb && t[ns] >= ce;
-            float d2m = 0.0f, amx = 0.0f, esum = 0.0f;
+            float d3m = 0.0f, esum = 0.0f;
             double jsum = 0.0;
-            float p0[3] = {0.0f, 0.0f, 0.0f}, v0[3] = {0.0f, 0.0f, 0.0f};
+            float4 *dm = F32 ? q.nprec + ((size_t)c * N + o) * 3u : nullptr;
             if (ns > 0 && !(t[0] >= ce || t[ns] <= cb)) {
                 // kb: upper_bound of cb (monotone in c); ke: lower_bound of ce from kb, plus one
                 while (kb < ns && !(cb < t[1 + kb])) ++kb;
@@ -379,21 +394,27 @@ __global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint3
                     if constexpr (F32) {
                         if (full) {
                             const float4 sb = sb_cached;
-                            d2m = fmaxf(d2m, sb.x);
-                            amx = fmaxf(amx, sb.y);
-                            if (k > kb) jsum += (double)sb.z + 2.0 * mw * (double)sb.w;
+                            d3m = fmaxf(d3m, sb.x);
+                            if (k > kb) {
+                                const double w2 = 2.0 * mw;  // |t - tb| <= 2 w
+                                jsum += (double)sb.y + w2 * (double)sb.z + 0.5 * w2 * w2 * (double)sb.w;
+                            }
                             if (ss <= tm && tm < t[k + 1]) {
-                                // position, velocity and their magnitude sums
-                                // at tm, in FP32 from the piece's float
-                                // coefficients
-                                const float tau = (float)(tm - ss), at = fabsf(tau);
+                                // position, velocity, half acceleration and
+                                // their magnitude sums at tm, in FP32 from
+                                // the piece's float coefficients
+                                const float tau = (float)(tm - ss), at = fabsf(tau), fw = (float)mw;
+                                float p0[3], v0[3], a0[3];
 #pragma unroll
                                 for (int ch = 0; ch < 3; ++ch) {
                                     const CT *cc = cf + ch * (ORDER + 1) * kStride;
-                                    float pv = cc[ORDER * kStride], dv = 0.0f, mp = fabsf(pv), md = 0.0f;
+                                    float pv = cc[ORDER * kStride], dv = 0.0f, hv = 0.0f;
+                                    float mp = fabsf(pv), md = 0.0f, mh = 0.0f;
 #pragma unroll
                                     for (int kk = ORDER - 1; kk >= 0; --kk) {
                                         const float x = cc[kk * kStride];
+                                        hv = hv * tau + dv;
+                                        mh = mh * at + md;
                                         dv = dv * tau + pv;
                                         md = md * at + mp;
                                         pv = pv * tau + x;
@@ -401,30 +422,37 @@ __global__ __launch_bounds__(256) void k_aabbs(const Prob *__restrict__ P, uint3
                                     }
                                     p0[ch] = pv;
                                     v0[ch] = dv;
-                                    esum += mp + md * (float)mw;
+                                    a0[ch] = 2.0f * hv;
+                                    // value, velocity x w and acceleration x w^2 / 2
+                                    esum += mp + md * fw + mh * fw * fw;
                                 }
+                                // stored at once (R follows the loop): 4 (2^n1 - 2) / n1!
+                                // (h lam / 4)^n1 A^2
+                                const double hl = (ce - cb) * 0.25 * (double)__uint_as_float(pj.scal[0]);
+                                const double A = (double)__uint_as_float(pj.scal[1]);
+                                double qi = kInterpConst;
+#pragma unroll
+                                for (int kk = 1; kk <= ORDER + 1; ++kk) qi *= hl;
+                                dm[0] = make_float4(p0[0], p0[1], p0[2], __builtin_inff());
+                                dm[1] = make_float4(v0[0], v0[1], v0[2], next_up_f((float)(qi * A * A * (1.0 + 1e-9))));
+                                dm[2] = make_float4(a0[0], a0[1], a0[2], 0.0f);
                             }
                         }
                     }
                 }
             }
             if constexpr (F32) {
-                float4 m0 = make_float4(0.0f, 0.0f, 0.0f, __builtin_inff()), m1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (full) {
                     // the model's error: the channels' sum bounds their norm,
                     // and 1e-3 covers the float sums' rounding
-                    const double R = 0.5 * (double)d2m * mw * mw + jsum + kModelErr * 1.001 * (double)esum;
-                    // 4 (2^n1 - 2) / n1! (h lam / 4)^n1 A^2
-                    const double hl = (ce - cb) * 0.25 * (double)__uint_as_float(*pj.lam);
-                    double qi = kInterpConst;
-#pragma unroll
-                    for (int kk = 1; kk <= ORDER + 1; ++kk) qi *= hl;
-                    m0 = make_float4(p0[0], p0[1], p0[2], next_up_f((float)(R * (1.0 + 1e-9))));
-                    m1 = make_float4(v0[0], v0[1], v0[2], next_up_f((float)(qi * (double)amx * (double)amx * (1.0 + 1e-9))));
+                    const double R = (1.0 / 6.0) * (double)d3m * mw * mw * mw + jsum + kModelErr * 1.001 * (double)esum;
+                    dm[0].w = next_up_f((float)(R * (1.0 + 1e-9)));
+                } else {
+                    // never screened out
+                    dm[0] = make_float4(0.0f, 0.0f, 0.0f, __builtin_inff());
+                    dm[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    dm[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
-                float4 *dm = q.nprec + ((size_t)c * N + o) * 2u;
-                dm[0] = m0;
-                dm[1] = m1;
             }
             float4 *dst = reinterpret_cast<float4 *>(aabbs + ((size_t)c * (N + 1) + o) * 8u);
             dst[0] = make_float4(lb4[0], lb4[1], lb4[2], lb4[3]);
@@ -1873,12 +1901,12 @@ hipError_t launch_transpose_cf4(const double *traj, double *cf4, float *cf4f, ui
 
 hipError_t launch_step_bounds(const double *cf4, const double *time, const uint64_t *step0, const uint64_t *nsteps,
                               const uint64_t *time_off, uint32_t n_objs, uint64_t n_steps_total, uint32_t order,
-                              float4 *sbnd, uint32_t *lam_bits, hipStream_t stream)
+                              float4 *sbnd, uint32_t *scal, hipStream_t stream)
 {
-    hipError_t e = hipMemsetAsync(lam_bits, 0, sizeof(uint32_t), stream);
+    hipError_t e = hipMemsetAsync(scal, 0, 2 * sizeof(uint32_t), stream);
     if (e != hipSuccess || n_steps_total == 0) return e;
     return dispatch_step_bounds<kMinOrder>(order, cf4, time, step0, nsteps, time_off, n_objs, n_steps_total, sbnd,
-                                           lam_bits, stream);
+                                           scal, stream);
 }
 
 hipError_t launch_batch_init(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
