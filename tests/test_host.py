@@ -148,7 +148,7 @@ def test_bench_reads_the_committed_profiles():
     sys.path.insert(0, str(ROOT))
     import bench
 
-    for k in ("k_bp", "k_aabbs", "k_bvh"):
+    for k in ("k_bp", "k_aabbs", "k_lbvh_build"):
         assert bench.hbm_traffic(k) > 0
     assert bench.eval_hbm_traffic() > 0
     assert bench.sgp4_flops_per_launch() > 1e9
