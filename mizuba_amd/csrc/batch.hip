@@ -177,6 +177,25 @@ private:
     void read_back(std::vector<uint64_t> &counts, std::vector<uint32_t> &flags);
 };
 
+// The mixed-precision pre-screen's bound on the FP64 narrow phase's dist^2
+// interpolation error for any step pair of a problem: with n1 = order + 1
+// Chebyshev nodes on an interval of h <= conj_det_interval, at most
+// 2 (h/4)^n1 / n1! max |F^(n1)|, F^(n1) = sum_{a=1..order} C(n1, a) d^(a) .
+// d^(n1-a), |d^(a)| <= 2 A lam^a (the polyjectory's A and lam): at most
+// 8 (2^n1 - 2) / n1! (h lam / 4)^n1 A^2, rounded up.
+double mixed_interp_bound(const mzh_ctx *ctx)
+{
+    float lam = 0.0f, amp = 0.0f;
+    std::memcpy(&lam, &ctx->pjb->h_scal[0], sizeof(float));
+    std::memcpy(&amp, &ctx->pjb->h_scal[1], sizeof(float));
+    const uint32_t n1 = ctx->order + 1u;
+    const double hl = ctx->conj_det_interval * (1.0 + 1e-12) * 0.25 * (double)lam;
+    double e = 8.0 * (std::ldexp(1.0, (int)n1) - 2.0);
+    for (uint32_t k = 1; k <= n1; ++k) e *= hl / (double)k;
+    e *= (double)amp * (double)amp * (1.0 + 1e-9);
+    return std::isfinite(e) ? e : HUGE_VAL;
+}
+
 void build_bfs_trees(mzh_ctx *ctx)
 {
     const uint32_t N = ctx->n_objs;
@@ -258,7 +277,7 @@ void ScreenBatch::prepare()
         MZH_HIP(ctx->cd_end.ensure(sizeof(double) * C));
         MZH_HIP(ctx->aabbs.ensure(boxes_bytes));
         MZH_HIP(ctx->kbke.ensure(sizeof(uint2) * NC));
-        if (ctx->mode == MZH_MODE_MIXED) MZH_HIP(ctx->nprec.ensure(sizeof(float4) * 3u * NC));
+        if (ctx->mode == MZH_MODE_MIXED) MZH_HIP(ctx->nprec.ensure(sizeof(float4) * 2u * NC));
         MZH_HIP(ctx->srt_aabbs.ensure(boxes_bytes));
         MZH_HIP(ctx->mcodes.ensure(sizeof(uint64_t) * NC));
         MZH_HIP(ctx->srt_mcodes.ensure(sizeof(uint64_t) * NC));
@@ -269,7 +288,7 @@ void ScreenBatch::prepare()
         MZH_HIP(ctx->sort_todo.ensure(sizeof(uint32_t) * C));
         MZH_HIP(ctx->gbox.ensure(sizeof(uint32_t) * 8u * C));
         MZH_HIP(ctx->trav.ensure(sizeof(TravNode) * ctx->node_cap * C));
-        MZH_HIP(ctx->bvh_hdr.ensure(sizeof(uint4) * ctx->node_cap * C));
+        MZH_HIP(ctx->bvh_hdr.ensure(sizeof(uint32_t) * lbvh_step_words(N) * C));
         const size_t scr_b = bvh_scratch_bytes_host(N);
         if (scr_b > kBvhLdsMaxHost) MZH_HIP(ctx->bvh_scr.ensure(bvh_scratch_stride(N) * C));
         MZH_HIP(ctx->tree_size.ensure(sizeof(uint32_t) * C));
@@ -303,10 +322,12 @@ void ScreenBatch::prepare()
         q.thresh = ctx->conj_thresh;
         q.node_cap = ctx->node_cap;
         q.bvh_scr_stride = bvh_scratch_stride(N);
+        q.lbvh_stride = lbvh_step_words(N);
         q.aabbs = gl(ctx->aabbs.as<float>());
         q.srt_aabbs = gl(ctx->srt_aabbs.as<float>());
         q.kbke = gl(ctx->kbke.as<uint2>());
         q.nprec = gl(ctx->nprec.as<float4>());
+        q.np_e = mixed_interp_bound(ctx);
         q.gbox = gl(ctx->gbox.as<uint32_t>());
         q.mcodes = gl(ctx->mcodes.as<uint64_t>());
         q.srt_mcodes = gl(ctx->srt_mcodes.as<uint64_t>());

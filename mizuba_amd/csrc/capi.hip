@@ -483,6 +483,7 @@ int mzh_upload_polyjectory(mzh_ctx *ctx, uint32_t n_objs, uint32_t order, const 
                                    ctx->pjb->step0.as<uint64_t>(), ctx->pjb->nsteps.as<uint64_t>(),
                                    ctx->pjb->time_off.as<uint64_t>(), n_objs, steps_total, order,
                                    ctx->pjb->sbnd.as<float4>(), ctx->pjb->scal.as<uint32_t>(), st));
+        MZH_HIP(hipMemcpyAsync(ctx->pjb->h_scal, ctx->pjb->scal.p, sizeof(ctx->pjb->h_scal), hipMemcpyDeviceToHost, st));
         MZH_HIP(hipStreamSynchronize(st));
         ctx->n_objs = n_objs;
         ctx->order = order;
@@ -782,6 +783,7 @@ void make_sgp4_impl(mzh_ctx *ctx, const mzh_gpe *gpes, uint64_t n_gpes, double j
                                ctx->pjb->nsteps.as<uint64_t>(), ctx->pjb->time_off.as<uint64_t>(), n_sats, steps_total,
                                order, ctx->pjb->sbnd.as<float4>(), ctx->pjb->scal.as<uint32_t>(), st));
     MZH_HIP(hipEventRecord(E[7], st));
+    MZH_HIP(hipMemcpyAsync(ctx->pjb->h_scal, ctx->pjb->scal.p, sizeof(ctx->pjb->h_scal), hipMemcpyDeviceToHost, st));
     MZH_HIP(hipStreamSynchronize(st));
     {
         const int pairs[5][2] = {{0, 1}, {2, 3}, {3, 4}, {5, 6}, {6, 7}};

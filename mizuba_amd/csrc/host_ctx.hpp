@@ -118,6 +118,7 @@ struct StatusError {
 // step-major transposed positions and the per-object layout.
 struct PjBufs {
     mzh::host::DevBuf traj, time, cf4, cf4f, sbnd, scal, step0, nsteps, time_off;
+    uint32_t h_scal[2] = {0u, 0u};  // the polyjectory's lam and A (float bits), read back at upload
 };
 
 } // namespace mzh::host
@@ -158,7 +159,7 @@ struct mzh_ctx {
     std::vector<double> h_cd_begin, h_cd_end;
     mzh::host::DevBuf cd_begin, cd_end;
     mzh::host::DevBuf aabbs, kbke, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, sort_todo, gbox;
-    mzh::host::DevBuf nprec;  // mixed-precision pre-screen models, [C][N][3] float4
+    mzh::host::DevBuf nprec;  // mixed-precision pre-screen models, [C][N][2] float4
     uint64_t node_cap = 0;
     mzh::host::DevBuf nodes, trav, bvh_hdr, bvh_scr, tree_size;
     bool nodes_ready = false;  // nodes holds the reference-layout export of trav

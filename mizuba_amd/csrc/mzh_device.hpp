@@ -126,6 +126,20 @@ __device__ inline void report_error(DevError *e, int32_t code, uint32_t i, uint3
 
 // ---- IEEE helpers --------------------------------------------------------
 
+// bfloat16 (the top 16 bits of a float): rounded to nearest (ties to even),
+// rounded up (for x >= 0), and back to float.
+__device__ inline uint32_t bf16_rn(float x)
+{
+    const uint32_t b = __float_as_uint(x);
+    return (b + 0x7fffu + ((b >> 16) & 1u)) >> 16;
+}
+__device__ inline uint32_t bf16_up(float x)
+{
+    const uint32_t b = __float_as_uint(x);
+    return (b >> 16) + ((b & 0xffffu) ? 1u : 0u);
+}
+__device__ inline float bf16_f(uint32_t h) { return __uint_as_float(h << 16); }
+
 // nextafter(x, -inf) / nextafter(x, +inf) for float, exact bit stepping.
 __device__ inline float next_down_f(float x)
 {

@@ -73,15 +73,17 @@ struct Prob {
     double thresh;
     uint64_t node_cap;  // BVH nodes per step
     uint64_t bvh_scr_stride;
+    uint64_t lbvh_stride;  // 32-bit words per step of the walk-tree build's scratch (lbvh_step_words)
     // the problem's buffers (its context's)
     gbuf<float> aabbs, srt_aabbs;  // [C][N + 1][2][4]
     gbuf<uint2> kbke;  // [C][N]
-    gbuf<float4> nprec;  // [C][N][3] mixed-precision pre-screen model of each (step, object), mixed problems
+    gbuf<float4> nprec;  // [C][N][2] mixed-precision pre-screen model of each (step, object), mixed problems
+    double np_e;         // mixed problems: the bound on the FP64 path's dist^2 interpolation error of a pair
     gbuf<uint32_t> gbox;  // [C][8] order-preserving images of the global box
     gbuf<uint64_t> mcodes, srt_mcodes, keys_tmp;
     gbuf<uint32_t> srt_idx, vals_tmp, n_eff, sort_todo;
     gbuf<TravNode> trav;  // [C][node_cap]
-    gbuf<uint4> bvh_hdr;  // [C][node_cap]
+    gbuf<uint4> bvh_hdr;  // walk-tree build: [C][lbvh_stride] 32-bit words; breadth-first build: [C][node_cap]
     gbuf<uint8_t> bvh_scr;  // [C][bvh_scr_stride], when the build scratch is not in LDS
     gbuf<uint32_t> tree_size;  // [C]
     gbuf<const int32_t> otypes;  // [N]
@@ -162,6 +164,18 @@ constexpr size_t kBvhLdsMaxHost = 64u * 1024u;
 // positions (the lbvh_blk table), then one workgroup per step for the nodes
 // that span blocks. Also counts each step's tree size.
 constexpr uint32_t kLbvhBlock = 512;
+// 32-bit words per step of the walk-tree build's scratch (in the BVH scratch
+// buffer) for steps of at most n_objs objects: the parent of every node, two
+// counts, and the lists of spanning and frontier nodes sized by their bound.
+// A node spans the boundary between two blocks only if it holds both
+// positions next to it, so each boundary has at most 96 of them (one per
+// prefix length of the 96-bit keys); a frontier node is a child of a spanning
+// node.
+__host__ __device__ inline uint64_t lbvh_span_cap(uint32_t n_objs) { return 96ull * ((n_objs + kLbvhBlock - 1u) / kLbvhBlock) + 1u; }
+__host__ __device__ inline uint64_t lbvh_step_words(uint32_t n_objs)
+{
+    return 2ull * n_objs + 2u + lbvh_span_cap(n_objs) + (2u * lbvh_span_cap(n_objs) + 2u);
+}
 hipError_t launch_lbvh(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_blocks, uint32_t n_steps_all,
                        hipStream_t stream);
 // The reference's breadth-first trees (persistent workgroups over the BVH

@@ -666,17 +666,23 @@ __device__ inline void translate1s(const double *__restrict__ src, double a, dou
 // d0 + v s + a s^2 / 2 (the differences of the models' position, velocity and
 // acceleration), hence within R_i + R_j + |a| w^2 / 2 of the segment d0 + v s,
 // whose least norm is |d0 x v| / |v| (a bound for the whole line), or an end
-// point's when the line's closest point is clearly outside; and |p - F| <=
-// E_i + E_j for the true dist^2 F, plus a relative 1e-9 of the magnitudes
-// for the FP64 path's rounding. Non-finite values never reject.
-__device__ inline bool np_pair_far(const float4 *mi, const float4 *mj, double w, double thresh)
+// point's when the line's closest point is clearly outside; and |p - F| <= E
+// for the true dist^2 F (the problem's bound, Prob::np_e), plus a relative
+// 1e-9 of the magnitudes for the FP64 path's rounding. Non-finite values
+// never reject.
+__device__ inline bool np_pair_far(const float4 *mi, const float4 *mj, double w, double thresh, double E)
 {
-    const float4 ai = mi[0], bi = mi[1], ci = mi[2], aj = mj[0], bj = mj[1], cj = mj[2];
-    const double R = (double)ai.w + (double)aj.w, E = (double)bi.w + (double)bj.w;
+    // {p0.x, p0.y, p0.z, v0.x}, {v0.y, v0.z, a0.x | a0.y << 16, a0.z | R << 16} (a0 and R bfloat16)
+    const float4 ai = mi[0], bi = mi[1], aj = mj[0], bj = mj[1];
+    const uint32_t zi = __float_as_uint(bi.z), wi = __float_as_uint(bi.w);
+    const uint32_t zj = __float_as_uint(bj.z), wj = __float_as_uint(bj.w);
+    const double R = (double)bf16_f(wi >> 16) + (double)bf16_f(wj >> 16);
     if (!(R < 1e300) || !(E < 1e300)) return false;
     const double d0[3] = {(double)ai.x - (double)aj.x, (double)ai.y - (double)aj.y, (double)ai.z - (double)aj.z};
-    const double v[3] = {(double)bi.x - (double)bj.x, (double)bi.y - (double)bj.y, (double)bi.z - (double)bj.z};
-    const double ax = (double)ci.x - (double)cj.x, ay = (double)ci.y - (double)cj.y, az = (double)ci.z - (double)cj.z;
+    const double v[3] = {(double)ai.w - (double)aj.w, (double)bi.x - (double)bj.x, (double)bi.y - (double)bj.y};
+    const double ax = (double)bf16_f(zi & 0xffffu) - (double)bf16_f(zj & 0xffffu);
+    const double ay = (double)bf16_f(zi >> 16) - (double)bf16_f(zj >> 16);
+    const double az = (double)bf16_f(wi & 0xffffu) - (double)bf16_f(wj & 0xffffu);
     const double an = sqrt(ax * ax + ay * ay + az * az);
     const double dd = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2];
     const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
@@ -741,9 +747,9 @@ __global__ __launch_bounds__(256) void k_np_enum(NarrowArgs a)
         kbj = rj.x;
         kej = rj.y;
         if (q.mode == MZH_MODE_MIXED) {
-            const float4 *mi = q.nprec + ((size_t)c * q.N + i) * 3u, *mj = q.nprec + ((size_t)c * q.N + j) * 3u;
+            const float4 *mi = q.nprec + ((size_t)c * q.N + i) * 2u, *mj = q.nprec + ((size_t)c * q.N + j) * 2u;
             const double tm = cb + 0.5 * (ce - cb);
-            far = np_pair_far(mi, mj, fmax(tm - cb, ce - tm), q.thresh);
+            far = np_pair_far(mi, mj, fmax(tm - cb, ce - tm), q.thresh, q.np_e);
         }
     }
     // the merge-walk visits step pairs in order; the first kEnumBuf

@@ -209,10 +209,10 @@ constexpr uint32_t kAabbGroup = kAabbStepGroup;
 // problems.
 //
 // The mixed-precision problems also get each (object, step)'s pre-screen
-// model (nprec, read by the narrow phase's np_pair_far): over the step
-// [cb, ce] with midpoint tm and half-width w, the position p0, velocity v0 and
-// acceleration a0 at tm, and R >= |r(t) - p0 - v0 s - a0 s^2 / 2| for every
-// t in [cb, ce], s = t - tm:
+// model (nprec, read by the narrow phase's np_pair_far, 32 bytes: p0 and v0 in
+// float, a0 and R in bfloat16): over the step [cb, ce] with midpoint tm and
+// half-width w, the position p0, velocity v0 and acceleration a0 at tm, and
+// R >= |r(t) - p0 - v0 s - a0 s^2 / 2| for every t in [cb, ce], s = t - tm:
 //  - on the piece holding tm, Taylor: (D3 / 6) |s|^3;
 //  - across each piece boundary tb inside the step, the same with D3 the
 //    largest of the pieces', plus the jumps there, Jv + Jd |t - tb| +
@@ -221,19 +221,15 @@ constexpr uint32_t kAabbGroup = kAabbStepGroup;
 //  - p0, v0 evaluated in FP32 from the float coefficients, within
 //    (4 order + 8) 6.1e-8 of their magnitude sums of the FP64 polynomial's
 //    values (coefficient rounding, FP32 Horner and the rounding of
-//    tm - t_k; stored as computed);
-// and E, the object's share of the FP64 path's Chebyshev interpolation error
-// for any step pair inside the step, 2 (h'/4)^n1 / n1! max |F^(n1)| with
-// h' <= h = ce - cb and F^(n1) = sum_{a=1..order} C(n1, a) d^(a) . d^(n1-a),
-// |d^(a)| <= 2 A lam^a (the polyjectory's A and lam): at most
-// 8 (2^n1 - 2) (h/4)^n1 / n1! (lam^n1 A^2) = E_i + E_j with E half of it.
+//    tm - t_k; stored as computed, the acceleration rounded to bfloat16 with
+//    the rounding added, R rounded up).
 // Objects whose data do not cover the whole step get R = infinity (never
 // screened out).
-// Two waves per SIMD for the mixed-precision kernels up to order 9, whose
-// registers then just fit (none spilled at order 9, 12 bytes at order 8);
-// the others as their registers allow.
+// Two waves per SIMD for the mixed-precision kernels up to order 8, whose
+// registers then just fit (20 bytes spilled at order 8); the others as their
+// registers allow.
 template <int ORDER, bool F32>
-constexpr unsigned kAabbWaves = (F32 && ORDER <= 9) ? 2u : 1u;
+constexpr unsigned kAabbWaves = (F32 && ORDER <= 8) ? 2u : 1u;
 
 template <int ORDER, bool F32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAabbWaves<ORDER, F32>))) void k_aabbs(const Prob *__restrict__ P, uint32_t np,
@@ -283,12 +279,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAabbWaves<
     // 2^-24 <= order 2^-24 of the value's sum, and likewise for the
     // derivatives); with room to spare
     constexpr double kModelErr = (4.0 * ORDER + 8.0) * 6.1e-8;
-    // 4 (2^n1 - 2) / n1!, rounded up (each object's half of the pair's bound)
-    constexpr double kInterpConst = [] {
-        double f = 1.0;
-        for (int kk = 2; kk <= ORDER + 1; ++kk) f *= (double)kk;
-        return 4.0 * ((double)(1u << (ORDER + 1)) - 2.0) / f * (1.0 + 1e-12);
-    }();
     if (o < N) {
         ns = pj.nsteps[o];
         sbase = pj.step0[o];
@@ -324,9 +314,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAabbWaves<
             // the pre-screen model (mixed precision)
             const double tm = cb + 0.5 * (ce - cb), mw = fmax(tm - cb, ce - tm);
             const bool full = F32 && ns > 0 && t[0] <= cb && t[ns] >= ce;
-            float d3m = 0.0f, esum = 0.0f;
+            float d3m = 0.0f, esum = 0.0f, aerr = 0.0f;
             double jsum = 0.0;
-            float4 *dm = F32 ? q.nprec + ((size_t)c * N + o) * 3u : nullptr;
+            uint32_t az = 0u;  // the acceleration's z, as bfloat16
+            float4 *dm = F32 ? q.nprec + ((size_t)c * N + o) * 2u : nullptr;
             if (ns > 0 && !(t[0] >= ce || t[ns] <= cb)) {
                 // kb: upper_bound of cb (monotone in c); ke: lower_bound of ce from kb, plus one
                 while (kb < ns && !(cb < t[1 + kb])) ++kb;
@@ -426,16 +417,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAabbWaves<
                                     // value, velocity x w and acceleration x w^2 / 2
                                     esum += mp + md * fw + mh * fw * fw;
                                 }
-                                // stored at once (R follows the loop): 4 (2^n1 - 2) / n1!
-                                // (h lam / 4)^n1 A^2
-                                const double hl = (ce - cb) * 0.25 * (double)__uint_as_float(pj.scal[0]);
-                                const double A = (double)__uint_as_float(pj.scal[1]);
-                                double qi = kInterpConst;
-#pragma unroll
-                                for (int kk = 1; kk <= ORDER + 1; ++kk) qi *= hl;
-                                dm[0] = make_float4(p0[0], p0[1], p0[2], __builtin_inff());
-                                dm[1] = make_float4(v0[0], v0[1], v0[2], next_up_f((float)(qi * A * A * (1.0 + 1e-9))));
-                                dm[2] = make_float4(a0[0], a0[1], a0[2], 0.0f);
+                                // stored at once (R follows the loop); the
+                                // acceleration as bfloat16, its rounding (exact
+                                // in float) part of the model's error
+                                const uint32_t ax = bf16_rn(a0[0]), ay = bf16_rn(a0[1]);
+                                az = bf16_rn(a0[2]);
+                                const float ex = a0[0] - bf16_f(ax), ey = a0[1] - bf16_f(ay), ez = a0[2] - bf16_f(az);
+                                aerr = fabsf(ex) + fabsf(ey) + fabsf(ez);
+                                dm[0] = make_float4(p0[0], p0[1], p0[2], v0[0]);
+                                dm[1] = make_float4(v0[1], v0[2], __uint_as_float(ax | (ay << 16)), __uint_as_float(az));
                             }
                         }
                     }
@@ -445,13 +435,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kAabbWaves<
                 if (full) {
                     // the model's error: the channels' sum bounds their norm,
                     // and 1e-3 covers the float sums' rounding
-                    const double R = (1.0 / 6.0) * (double)d3m * mw * mw * mw + jsum + kModelErr * 1.001 * (double)esum;
-                    dm[0].w = next_up_f((float)(R * (1.0 + 1e-9)));
+                    const double R = (1.0 / 6.0) * (double)d3m * mw * mw * mw + jsum + kModelErr * 1.001 * (double)esum
+                                     + 0.5 * mw * mw * 1.001 * (double)aerr;
+                    dm[1].w = __uint_as_float(az | (bf16_up(next_up_f((float)(R * (1.0 + 1e-9)))) << 16));
                 } else {
-                    // never screened out
-                    dm[0] = make_float4(0.0f, 0.0f, 0.0f, __builtin_inff());
-                    dm[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    dm[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    // never screened out (R infinite)
+                    dm[0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    dm[1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0x7f800000u));
                 }
             }
             float4 *dst = reinterpret_cast<float4 *>(aabbs + ((size_t)c * (N + 1) + o) * 8u);
@@ -512,22 +502,23 @@ __global__ void k_gbox_finalize(const Prob *__restrict__ P, uint32_t np, const u
     }
 }
 
-// The per-step build words of the walk tree in the BVH scratch (node_cap
-// 16-byte words per step, used as 4 node_cap 32-bit words): the parent of each
-// node whose parent's range leaves its block; at node_cap + 2k the arrival
-// counter of such a spanning internal node k and at node_cap + 2k + 1 its
-// split and child flags; the frontier list (nodes finished inside their block
-// whose parent spans blocks); its length, the spanning nodes' count and
-// their list.
+// The per-step build words of the walk tree in the BVH scratch
+// (lbvh_step_words): the parent of each node whose parent spans blocks, the
+// frontier list (nodes finished inside their block whose parent spans
+// blocks), its length, and the spanning nodes' list and count. A spanning
+// node's arrival counter and split word live in its record's box words until
+// the climb writes its box there.
 struct LbvhWords {
-    uint32_t *par, *flag, *front, *nfront, *nspan, *span;
+    uint32_t *par, *nfront, *nspan, *span, *front;
+    uint32_t span_cap, front_cap;
 };
 
 __device__ inline LbvhWords lbvh_words(const Prob &q, uint32_t c)
 {
-    uint32_t *h = reinterpret_cast<uint32_t *>(q.bvh_hdr + (size_t)c * q.node_cap);
-    const uint64_t nc = q.node_cap;
-    return LbvhWords{h, h + nc, h + 2u * nc, h + 3u * nc, h + 3u * nc + 1u, h + 3u * nc + 2u};
+    uint32_t *h = reinterpret_cast<uint32_t *>(q.bvh_hdr + 0) + (size_t)c * q.lbvh_stride;
+    const uint32_t np = 2u * q.N;
+    const uint32_t sc = (uint32_t)lbvh_span_cap(q.N);
+    return LbvhWords{h, h + np, h + np + 1u, h + np + 2u, h + np + 2u + sc, sc, 2u * sc + 2u};
 }
 
 // One thread per conjunction step of the batch: the step's global box starts
@@ -1393,7 +1384,12 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
         runs = p == 0u ? 1u : (code(p) != code(p - 1u) ? 2u : 0u);
         if (p + 1u < n) {
             if (lbvh_leaves(code, n, p, B0, B1)) {
-                W.span[atomicAdd(W.nspan, 1u)] = p;
+                const uint32_t at = atomicAdd(W.nspan, 1u);
+                if (at < W.span_cap) {
+                    W.span[at] = p;
+                } else {
+                    report_error(q.err, kErrBvhBuild, c, 2u * p, 0.0);
+                }
             } else {
                 const LbvhNode nd = lbvh_node(code, n, p, B0, B1);
                 s_meta[t] = lbvh_meta(nd, lbvh_escape(code, n, nd.hi));
@@ -1419,7 +1415,14 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
     for (uint32_t guard = 0; guard < 128u; ++guard) {
         const uint32_t pa = s_par[local(self)];
         if (pa == kNoParent) {
-            if (self != root) W.front[atomicAdd(W.nfront, 1u)] = self;
+            if (self != root) {
+                const uint32_t at = atomicAdd(W.nfront, 1u);
+                if (at < W.front_cap) {
+                    W.front[at] = self;
+                } else {
+                    report_error(q.err, kErrBvhBuild, c, self, 0.0);
+                }
+            }
             break;
         }
         // the first child to arrive leaves; its box is in LDS for the second
@@ -1446,8 +1449,8 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
 // nodes themselves (from the step's codes in global memory, each with its own
 // thread), their children's parent links and counters; then the same
 // bottom-up climb as inside the blocks, from the step's frontier list, with
-// the counters and split words in the BVH scratch and the boxes in the node
-// records.
+// the counters and split words in the spanning nodes' box words (the second
+// child to arrive reads the split word, then writes the box there).
 __global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, uint32_t np,
                                                    const uint32_t *__restrict__ step_pre)
 {
@@ -1459,7 +1462,7 @@ __global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, u
     const LbvhWords W = lbvh_words(q, c);
     const uint64_t *codes = q.srt_mcodes + (size_t)c * q.N;
     TravNode *T = q.trav + (size_t)c * q.node_cap;
-    const uint32_t ns = min(*W.nspan, n - 1u);
+    const uint32_t ns = min(min(*W.nspan, n - 1u), W.span_cap);
     for (uint32_t x = threadIdx.x; x < ns; x += blockDim.x) {
         const uint32_t k = W.span[x];
         if (k + 1u >= n) {
@@ -1468,13 +1471,13 @@ __global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, u
         }
         const LbvhNode nd = lbvh_node(codes, n, k);
         *reinterpret_cast<uint4 *>(&T[2u * k].escape) = lbvh_meta(nd, lbvh_escape(codes, n, nd.hi));
+        // the arrival counter and split word, in the box words until the box
+        *reinterpret_cast<uint4 *>(T[2u * k].lb) = make_uint4(0u, lbvh_info(nd), 0u, 0u);
         W.par[lbvh_left(nd.split, nd.left_leaf)] = 2u * k;
         W.par[lbvh_right(nd.split, nd.right_leaf)] = 2u * k;
-        W.flag[2u * k] = 0u;
-        W.flag[2u * k + 1u] = lbvh_info(nd);
     }
     __syncthreads();
-    const uint32_t nf = min(*W.nfront, 2u * n);
+    const uint32_t nf = min(*W.nfront, W.front_cap);
     for (uint32_t f = threadIdx.x; f < nf; f += blockDim.x) {
         uint32_t self = W.front[f];
         if (self >= 2u * n) {
@@ -1489,10 +1492,10 @@ __global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, u
                 report_error(q.err, kErrBvhBuild, c, self, 0.0);
                 break;
             }
-            const uint32_t old =
-                __hip_atomic_fetch_add(&W.flag[pa], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            uint32_t *cw = reinterpret_cast<uint32_t *>(T[pa].lb);
+            const uint32_t old = __hip_atomic_fetch_add(&cw[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0u) break;
-            const uint32_t sib = lbvh_sibling(W.flag[pa + 1u], self);
+            const uint32_t sib = lbvh_sibling(cw[1], self);
             if (sib >= 2u * n) {
                 report_error(q.err, kErrBvhBuild, c, pa, 0.0);
                 break;
