@@ -202,6 +202,7 @@ void build_bfs_trees(mzh_ctx *ctx)
     const uint64_t C = ctx->C;
     if (C == 0) return;
     hipStream_t st = ctx->stream;
+    MZH_HIP(ctx->trav.ensure(sizeof(TravNode) * ctx->node_cap * C));
     MZH_HIP(ctx->bvh_hdr.ensure(sizeof(uint4) * ctx->node_cap * C));
     const bool in_lds = bvh_scratch_bytes_host(N) <= kBvhLdsMaxHost;
     if (!in_lds) MZH_HIP(ctx->bvh_scr.ensure(bvh_scratch_stride(N) * C));
@@ -287,7 +288,7 @@ void ScreenBatch::prepare()
         MZH_HIP(ctx->n_eff.ensure(sizeof(uint32_t) * C));
         MZH_HIP(ctx->sort_todo.ensure(sizeof(uint32_t) * C));
         MZH_HIP(ctx->gbox.ensure(sizeof(uint32_t) * 8u * C));
-        MZH_HIP(ctx->trav.ensure(sizeof(TravNode) * ctx->node_cap * C));
+        MZH_HIP(ctx->wtree.ensure(sizeof(WalkNode) * ctx->node_cap * C));
         MZH_HIP(ctx->bvh_hdr.ensure(sizeof(uint32_t) * lbvh_step_words(N) * C));
         const size_t scr_b = bvh_scratch_bytes_host(N);
         if (scr_b > kBvhLdsMaxHost) MZH_HIP(ctx->bvh_scr.ensure(bvh_scratch_stride(N) * C));
@@ -336,7 +337,7 @@ void ScreenBatch::prepare()
         q.vals_tmp = gl(ctx->vals_tmp.as<uint32_t>());
         q.n_eff = gl(ctx->n_eff.as<uint32_t>());
         q.sort_todo = gl(ctx->sort_todo.as<uint32_t>());
-        q.trav = gl(ctx->trav.as<TravNode>());
+        q.wtree = gl(ctx->wtree.as<WalkNode>());
         q.bvh_hdr = gl(ctx->bvh_hdr.as<uint4>());
         q.bvh_scr = gl(ctx->bvh_scr.as<uint8_t>());
         q.tree_size = gl(ctx->tree_size.as<uint32_t>());

@@ -12,6 +12,7 @@
 #include "../../include/mizuba_amd.h"
 #include "mzh_enclosure.hpp"
 #include "mzh_lbvh.hpp"
+#include "mzh_qbox.hpp"
 
 namespace mzh
 {
@@ -63,17 +64,14 @@ struct PjView {
     uint32_t order;
 };
 
-// BVH node as stored on the device, in one of two numberings of the same
-// tree. The broad phase walks the radix tree of mzh_lbvh.hpp: internal node
-// k at 2k, leaf p at 2p + 1; an internal node holds {escape, right child,
-// left child, 1}, a leaf or a run's root {escape, begin, end, 0}. The
-// reference's breadth-first order (detect_conjunctions_bvh.cpp:521-580),
-// made on request for the tree accessors, holds {escape, begin, end or split,
-// left}: left is the index of the left child (the right child is left + 1)
-// and 0 for a leaf, since the root is nobody's child; an internal node holds
-// its split (the first position of its right child) in place of its end. The
-// escape index is the node that follows the subtree in a depth-first walk;
-// that of the root's right spine is kTravEnd.
+// BVH node of the reference's breadth-first order
+// (detect_conjunctions_bvh.cpp:521-580), made on request for the tree
+// accessors: {escape, begin, end or split, left}, where left is the index of
+// the left child (the right child is left + 1) and 0 for a leaf, since the
+// root is nobody's child; an internal node holds its split (the first
+// position of its right child) in place of its end. The escape index is the
+// node that follows the subtree in a depth-first walk; that of the root's
+// right spine is kTravEnd.
 constexpr uint32_t kTravEnd = 0xffffffffu;
 struct alignas(16) TravNode {
     float lb[4];
@@ -84,6 +82,18 @@ struct alignas(16) TravNode {
     uint32_t left;
 };
 static_assert(sizeof(TravNode) == 48, "TravNode layout");
+
+// Node of the broad phase's walk tree (mzh_lbvh.hpp: internal node k at 2k,
+// leaf p at 2p + 1), read by the walk with two 16-byte loads: the node's box
+// on the step's grid (mzh_qbox.hpp) and {escape, right child, left
+// child, 1} for an internal node, {escape, begin, end, 0} for a leaf or a
+// run's root.
+struct alignas(16) WalkNode {
+    uint32_t qb[4];
+    uint32_t escape, a, b, internal;
+};
+static_assert(sizeof(WalkNode) == 32, "WalkNode layout");
+
 
 // Error word written by kernels; the host turns it into the reference's
 // exception message. Only the first error (lowest code wins ties) is kept.

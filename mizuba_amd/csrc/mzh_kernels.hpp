@@ -82,7 +82,8 @@ struct Prob {
     gbuf<uint32_t> gbox;  // [C][8] order-preserving images of the global box
     gbuf<uint64_t> mcodes, srt_mcodes, keys_tmp;
     gbuf<uint32_t> srt_idx, vals_tmp, n_eff, sort_todo;
-    gbuf<TravNode> trav;  // [C][node_cap]
+    gbuf<TravNode> trav;  // [C][node_cap], the breadth-first trees, made on request
+    gbuf<WalkNode> wtree;  // [C][node_cap], the broad phase's walk trees
     gbuf<uint4> bvh_hdr;  // walk-tree build: [C][lbvh_stride] 32-bit words; breadth-first build: [C][node_cap]
     gbuf<uint8_t> bvh_scr;  // [C][bvh_scr_stride], when the build scratch is not in LDS
     gbuf<uint32_t> tree_size;  // [C]

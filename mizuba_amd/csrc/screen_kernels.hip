@@ -1317,8 +1317,8 @@ struct LdsCodes {
 
 // One workgroup per block of BS consecutive sorted positions of one step, one
 // position per thread, with the block's codes and a halo of one position
-// before and two after in LDS. The thread of position p writes leaf p (the
-// object's box) and counts the runs of equal codes (the step's node total,
+// before and two after in LDS. Boxes are on the step's quantization grid
+// (qbox). The thread of position p writes leaf p (the object's box) and counts the runs of equal codes (the step's node total,
 // 2 runs - 1, is the reference's tree size); for internal node k = p it finds
 // whether the node's range leaves the block (one code beyond the block,
 // lbvh_leaves), and if not, the node from the block's codes alone
@@ -1335,7 +1335,7 @@ template <uint32_t BS>
 __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, uint32_t np,
                                                    const uint32_t *__restrict__ blk_pre)
 {
-    __shared__ float4 s_box[2u * BS][2];  // node boxes by local index 2 (k - B0) + leaf
+    __shared__ uint4 s_box[2u * BS];      // node grid boxes by local index 2 (k - B0) + leaf
     __shared__ uint64_t s_code[BS + 3u];  // codes of the positions [B0 - 1, B1 + 2)
     __shared__ uint16_t s_par[2u * BS];   // parent's slot when the parent lies inside the block
     __shared__ uint4 s_meta[BS];          // walk fields of internal node B0 + slot
@@ -1357,7 +1357,7 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
     const bool live = p < B1;
     const uint64_t *codes = q.srt_mcodes + (size_t)c * q.N;
     const float *boxes = q.srt_aabbs + (size_t)c * (q.N + 1) * 8u;
-    TravNode *T = q.trav + (size_t)c * q.node_cap;
+    WalkNode *T = q.wtree + (size_t)c * q.node_cap;
     const LbvhWords W = lbvh_words(q, c);
     const LdsCodes code{s_code, B0};
     auto local = [&](uint32_t g) { return 2u * ((g >> 1) - B0) + (g & 1u); };
@@ -1369,18 +1369,18 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
     s_par[2u * t + 1u] = kNoParent;
     s_flag[t] = 0u;
     __syncthreads();
-    float4 bl = make_float4(0.f, 0.f, 0.f, 0.f), bu = bl;
+    uint4 qb = make_uint4(0u, 0u, 0u, 0u);
     uint32_t runs = 0;
     if (live) {
-        bl = reinterpret_cast<const float4 *>(boxes + (size_t)p * 8u)[0];
-        bu = reinterpret_cast<const float4 *>(boxes + (size_t)p * 8u)[1];
-        s_box[2u * t + 1u][0] = bl;
-        s_box[2u * t + 1u][1] = bu;
+        const QGrid g = qgrid(reinterpret_cast<const float4 *>(boxes + (size_t)q.N * 8u)[0],
+                              reinterpret_cast<const float4 *>(boxes + (size_t)q.N * 8u)[1]);
+        qb = qbox(g, reinterpret_cast<const float4 *>(boxes + (size_t)p * 8u)[0],
+                  reinterpret_cast<const float4 *>(boxes + (size_t)p * 8u)[1]);
+        s_box[2u * t + 1u] = qb;
         // the leaf's whole record (the walk reads a node's box and fields together)
         uint4 *leaf = reinterpret_cast<uint4 *>(&T[2u * p + 1u]);
-        leaf[0] = make_uint4(__float_as_uint(bl.x), __float_as_uint(bl.y), __float_as_uint(bl.z), __float_as_uint(bl.w));
-        leaf[1] = make_uint4(__float_as_uint(bu.x), __float_as_uint(bu.y), __float_as_uint(bu.z), __float_as_uint(bu.w));
-        leaf[2] = make_uint4(lbvh_escape(code, n, p), p, p + 1u, 0u);
+        leaf[0] = qb;
+        leaf[1] = make_uint4(lbvh_escape(code, n, p), p, p + 1u, 0u);
         runs = p == 0u ? 1u : (code(p) != code(p - 1u) ? 2u : 0u);
         if (p + 1u < n) {
             if (lbvh_leaves(code, n, p, B0, B1)) {
@@ -1410,7 +1410,6 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
     if (!live) return;
     const uint32_t root = lbvh_root(n);
     uint32_t self = 2u * p + 1u;
-    float4 l = bl, u = bu;
     // a climb is at most as long as the tree is deep (96 key bits)
     for (uint32_t guard = 0; guard < 128u; ++guard) {
         const uint32_t pa = s_par[local(self)];
@@ -1433,14 +1432,12 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
         if (old == 0u) break;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         const uint32_t sib = local(lbvh_sibling(s_info[pa], self));
-        box_min_max(l, u, s_box[sib][0], s_box[sib][1]);
-        s_box[2u * pa][0] = l;
-        s_box[2u * pa][1] = u;
+        qb = qbox_union(qb, s_box[sib]);
+        s_box[2u * pa] = qb;
         // the whole record at once
         uint4 *nb = reinterpret_cast<uint4 *>(&T[2u * (B0 + pa)]);
-        nb[0] = make_uint4(__float_as_uint(l.x), __float_as_uint(l.y), __float_as_uint(l.z), __float_as_uint(l.w));
-        nb[1] = make_uint4(__float_as_uint(u.x), __float_as_uint(u.y), __float_as_uint(u.z), __float_as_uint(u.w));
-        nb[2] = s_meta[pa];
+        nb[0] = qb;
+        nb[1] = s_meta[pa];
         self = 2u * (B0 + pa);
     }
 }
@@ -1461,7 +1458,7 @@ __global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, u
     if (n < 2u) return;
     const LbvhWords W = lbvh_words(q, c);
     const uint64_t *codes = q.srt_mcodes + (size_t)c * q.N;
-    TravNode *T = q.trav + (size_t)c * q.node_cap;
+    WalkNode *T = q.wtree + (size_t)c * q.node_cap;
     const uint32_t ns = min(min(*W.nspan, n - 1u), W.span_cap);
     for (uint32_t x = threadIdx.x; x < ns; x += blockDim.x) {
         const uint32_t k = W.span[x];
@@ -1472,7 +1469,7 @@ __global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, u
         const LbvhNode nd = lbvh_node(codes, n, k);
         *reinterpret_cast<uint4 *>(&T[2u * k].escape) = lbvh_meta(nd, lbvh_escape(codes, n, nd.hi));
         // the arrival counter and split word, in the box words until the box
-        *reinterpret_cast<uint4 *>(T[2u * k].lb) = make_uint4(0u, lbvh_info(nd), 0u, 0u);
+        *reinterpret_cast<uint4 *>(T[2u * k].qb) = make_uint4(0u, lbvh_info(nd), 0u, 0u);
         W.par[lbvh_left(nd.split, nd.left_leaf)] = 2u * k;
         W.par[lbvh_right(nd.split, nd.right_leaf)] = 2u * k;
     }
@@ -1484,15 +1481,14 @@ __global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, u
             report_error(q.err, kErrBvhBuild, c, self, 0.0);
             break;
         }
-        float4 l = *reinterpret_cast<const float4 *>(T[self].lb);
-        float4 u = *reinterpret_cast<const float4 *>(T[self].ub);
+        uint4 qb = *reinterpret_cast<const uint4 *>(T[self].qb);
         for (uint32_t guard = 0; guard < 128u && self != 0u; ++guard) {
             const uint32_t pa = W.par[self];
             if (pa >= 2u * n || (pa & 1u)) {
                 report_error(q.err, kErrBvhBuild, c, self, 0.0);
                 break;
             }
-            uint32_t *cw = reinterpret_cast<uint32_t *>(T[pa].lb);
+            uint32_t *cw = T[pa].qb;
             const uint32_t old = __hip_atomic_fetch_add(&cw[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0u) break;
             const uint32_t sib = lbvh_sibling(cw[1], self);
@@ -1500,10 +1496,8 @@ __global__ __launch_bounds__(1024) void k_lbvh_top(const Prob *__restrict__ P, u
                 report_error(q.err, kErrBvhBuild, c, pa, 0.0);
                 break;
             }
-            box_min_max(l, u, *reinterpret_cast<const float4 *>(T[sib].lb), *reinterpret_cast<const float4 *>(T[sib].ub));
-            float4 *nb = reinterpret_cast<float4 *>(&T[pa]);
-            nb[0] = l;
-            nb[1] = u;
+            qb = qbox_union(qb, *reinterpret_cast<const uint4 *>(T[sib].qb));
+            *reinterpret_cast<uint4 *>(T[pa].qb) = qb;
             self = pa;
         }
     }
@@ -1591,7 +1585,7 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
     const uint32_t lblk = gblk - blk_pre[pi];
     const uint32_t N = q.N;
     const uint64_t node_cap = q.node_cap;
-    const TravNode *__restrict__ trav = q.trav;
+    const WalkNode *__restrict__ wtree = q.wtree;
     const float *__restrict__ srt_aabbs = q.srt_aabbs;
     const uint64_t *__restrict__ srt_mcodes = q.srt_mcodes;
     const uint32_t *__restrict__ srt_idx = q.srt_idx;
@@ -1608,7 +1602,7 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
     const uint32_t o = (lblk - c * bps) * BS + threadIdx.x;
     const uint32_t n = n_eff[c];
     if (o >= n) return;
-    const TravNode *T = trav + (size_t)c * node_cap;
+    const WalkNode *T = wtree + (size_t)c * node_cap;
     const uint32_t *vidx = srt_idx + (size_t)c * N;
     const float *boxes = srt_aabbs + (size_t)c * (N + 1) * 8u;
     const float4 bl = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[0];
@@ -1657,7 +1651,11 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
         }
     }
     // the walk tree (mzh_lbvh.hpp): a depth-first walk visits each of its
-    // 2n - 1 reachable nodes at most once
+    // 2n - 1 reachable nodes at most once. Its boxes are on the step's grid
+    // (qbox), which holds every box that meets the query box on the grid too.
+    const uint4 qq = qbox(qgrid(reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[0],
+                                reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[1]),
+                          ql, qu);
     const uint32_t lim = 2u * n;
     uint32_t i = lbvh_root(n), visits = 0;
     while (i < lim) {
@@ -1665,22 +1663,29 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
             report_error(q.err, kErrBvhWalk, c, o, 0.0);
             break;
         }
-        const float4 nl = *reinterpret_cast<const float4 *>(T[i].lb);
-        const float4 nu = *reinterpret_cast<const float4 *>(T[i].ub);
+        const uint4 nq = *reinterpret_cast<const uint4 *>(T[i].qb);
         const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
         // the next node is chosen from all four words on every path, so that
         // the record is read with one load: an internal node {escape, right,
         // left, 1} is entered, skipping the left child when it ends at or
         // before the run (the split is the right child's index halved); a
         // leaf {escape, begin, end, 0}, or a node missed, is left by its escape
-        const bool hit_node = box_overlap(ql, qu, nl, nu);
+        const bool hit_node = qbox_overlap(qq, nq);
         const bool enter = hit_node && meta.w != 0u;
         const uint32_t next = enter ? ((meta.y >> 1) <= re ? meta.y : meta.z) : meta.x;
         if (hit_node && meta.w == 0u && meta.y >= re) {
             const bool single = meta.z - meta.y == 1u;
+            // the leaf's float box: its object's, or the union of its run's
+            float4 nl = reinterpret_cast<const float4 *>(boxes + (size_t)meta.y * 8u)[0];
+            float4 nu = reinterpret_cast<const float4 *>(boxes + (size_t)meta.y * 8u)[1];
+#pragma unroll 1
+            for (uint32_t m = meta.y + 1u; m < meta.z; ++m)
+                box_min_max(nl, nu, reinterpret_cast<const float4 *>(boxes + (size_t)m * 8u)[0],
+                            reinterpret_cast<const float4 *>(boxes + (size_t)m * 8u)[1]);
             // box_o against the leaf box, needed when the query box is larger than box_o
             const bool o_hits_leaf = !multi || box_overlap(bl, bu, nl, nu);
-            for (uint32_t m = meta.y; m != meta.z; ++m) {
+            const uint32_t m_end = box_overlap(ql, qu, nl, nu) ? meta.z : meta.y;
+            for (uint32_t m = meta.y; m != m_end; ++m) {
                 const uint32_t oi = vidx[m];
                 if (ot + otypes[oi] > 3) continue;
                 bool hit;

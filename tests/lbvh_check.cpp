@@ -1,10 +1,12 @@
 // Host build of the broad phase's walk tree (mizuba_amd/csrc/mzh_lbvh.hpp:
 // Karras' construction over (code, position) keys, runs of equal codes as
-// leaves, escape indices), driven by tests/test_lbvh_host.py.
+// leaves, escape indices) and of its grid boxes (mzh_qbox.hpp), driven by
+// tests/test_lbvh_host.py.
 #include <cstdint>
 #include <vector>
 
 #include "../mizuba_amd/csrc/mzh_lbvh.hpp"
+#include "../mizuba_amd/csrc/mzh_qbox.hpp"
 
 namespace
 {
@@ -123,5 +125,42 @@ extern "C" int64_t lbvh_block_check(uint32_t n, const uint64_t *codes, uint32_t 
         }
         if (w.outside) bad += (int64_t)w.outside;
     }
+    return bad;
+}
+
+// The grid boxes of n float boxes (lo[4], hi[4] each, inside or on the global
+// box glo, ghi) against the float tests, over every pair (a, b): a pair whose
+// closed boxes meet must meet on the grid, and the grid box of the union of
+// two boxes must be the union of their grid boxes. Returns the number of
+// violations; meets_grid counts the pairs that meet on the grid only.
+extern "C" int64_t qbox_check(uint32_t n, const float *boxes, const float *g, int64_t *meets_float, int64_t *meets_grid)
+{
+    const float4 glo = make_float4(g[0], g[1], g[2], g[3]), ghi = make_float4(g[4], g[5], g[6], g[7]);
+    const mzh::QGrid grid = mzh::qgrid(glo, ghi);
+    auto lo = [&](uint32_t i) { return make_float4(boxes[8 * i], boxes[8 * i + 1], boxes[8 * i + 2], boxes[8 * i + 3]); };
+    auto hi = [&](uint32_t i) {
+        return make_float4(boxes[8 * i + 4], boxes[8 * i + 5], boxes[8 * i + 6], boxes[8 * i + 7]);
+    };
+    int64_t bad = 0, mf = 0, mg = 0;
+    for (uint32_t a = 0; a < n; ++a) {
+        const uint4 qa = mzh::qbox(grid, lo(a), hi(a));
+        for (uint32_t b = 0; b < n; ++b) {
+            const uint4 qb = mzh::qbox(grid, lo(b), hi(b));
+            bool meet = true;
+            for (int d = 0; d < 4; ++d) meet = meet && boxes[8 * a + 4 + d] >= boxes[8 * b + d] && boxes[8 * a + d] <= boxes[8 * b + 4 + d];
+            const bool gm = mzh::qbox_overlap(qa, qb);
+            mf += meet;
+            mg += gm && !meet;
+            if (meet && !gm) ++bad;
+            float4 ul = lo(a), uh = hi(a);
+            const float4 bl = lo(b), bh = hi(b);
+            ul = make_float4(fminf(ul.x, bl.x), fminf(ul.y, bl.y), fminf(ul.z, bl.z), fminf(ul.w, bl.w));
+            uh = make_float4(fmaxf(uh.x, bh.x), fmaxf(uh.y, bh.y), fmaxf(uh.z, bh.z), fmaxf(uh.w, bh.w));
+            const uint4 qu = mzh::qbox(grid, ul, uh), qv = mzh::qbox_union(qa, qb);
+            if (qu.x != qv.x || qu.y != qv.y || qu.z != qv.z || qu.w != qv.w) ++bad;
+        }
+    }
+    *meets_float = mf;
+    *meets_grid = mg;
     return bad;
 }

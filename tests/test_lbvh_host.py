@@ -5,7 +5,10 @@ of equal codes as leaves must give the reference's radix tree over runs
 every node's escape index must name the node that follows its subtree. The
 trees are compared with a recursive build from the definition (split where the
 highest differing bit of the range flips) on adversarial code arrays, and with
-the oracle's trees of a synthetic screening."""
+the oracle's trees of a synthetic screening. The walk tree's grid boxes
+(mizuba_amd/csrc/mzh_qbox.hpp) are checked against the float overlap test:
+boxes that meet must meet on the grid, and the grid box of a union must be the
+union of the grid boxes."""
 
 import ctypes
 import pathlib
@@ -34,6 +37,7 @@ def lib(tmp_path_factory):
     L = ctypes.CDLL(str(out))
     L.lbvh_walk.restype = ctypes.c_int64
     L.lbvh_block_check.restype = ctypes.c_int64
+    L.qbox_check.restype = ctypes.c_int64
     return L
 
 
@@ -177,3 +181,58 @@ def test_oracle_trees(lib, oracle):
                 stack.append(int(nd["left"]))
         assert _walk(lib, codes) == dfs
         _check(lib, codes)
+
+
+def _qbox_check(lib, boxes, g):
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 8)
+    g = np.ascontiguousarray(g, dtype=np.float32)
+    mf, mg = ctypes.c_int64(0), ctypes.c_int64(0)
+    F32 = ctypes.POINTER(ctypes.c_float)
+    bad = lib.qbox_check(ctypes.c_uint32(len(boxes)), boxes.ctypes.data_as(F32), g.ctypes.data_as(F32),
+                         ctypes.byref(mf), ctypes.byref(mg))
+    return bad, mf.value, mg.value
+
+
+def _global(boxes):
+    b = np.asarray(boxes, np.float32).reshape(-1, 8)
+    return np.concatenate([b[:, :4].min(axis=0), b[:, 4:].max(axis=0)])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_grid_boxes_keep_every_overlap(lib, seed):
+    """Random boxes of the sizes a screening makes (LEO positions over a
+    3-minute step, km) and boxes sharing faces, edges and corners exactly: no
+    pair that meets is lost on the grid, and grid unions are exact."""
+    rng = np.random.default_rng(seed)
+    n = 300
+    c = rng.uniform(-8000.0, 8000.0, (n, 4)).astype(np.float32)
+    c[:, 3] = rng.uniform(6700.0, 8400.0, n)
+    h = rng.uniform(1.0, 900.0, (n, 4)).astype(np.float32)
+    boxes = np.concatenate([c - h, c + h], axis=1).astype(np.float32)
+    # boxes that touch: the low face of one at the high face of another
+    for k in range(0, 60, 2):
+        boxes[k + 1, :4] = boxes[k, 4:]
+        boxes[k + 1, 4:] = boxes[k, 4:] + np.float32(1.0)
+    # boxes on the global box's faces, and points
+    boxes[100, :] = np.concatenate([boxes[:, :4].min(axis=0), boxes[:, :4].min(axis=0)])
+    boxes[101, :] = np.concatenate([boxes[:, 4:].max(axis=0), boxes[:, 4:].max(axis=0)])
+    boxes[102, 4:] = boxes[102, :4]
+    bad, mf, mg = _qbox_check(lib, boxes, _global(boxes))
+    assert bad == 0
+    assert mf > n  # the pairs include overlaps beyond the diagonal
+    assert mg < 0.05 * n * n  # the grid adds few pairs
+
+
+def test_grid_boxes_degenerate_extents(lib):
+    """A global box of zero width in some dimensions (all objects at one
+    radius, or one object), and extents near the float limits."""
+    rng = np.random.default_rng(5)
+    boxes = rng.uniform(-1.0, 1.0, (50, 8)).astype(np.float32)
+    boxes[:, 4:] = boxes[:, :4] + np.float32(0.5)
+    boxes[:, 3] = boxes[:, 7] = 7000.0
+    assert _qbox_check(lib, boxes, _global(boxes))[0] == 0
+    one = np.array([[1.0, 2.0, 3.0, 4.0, 1.0, 2.0, 3.0, 4.0]], np.float32)
+    assert _qbox_check(lib, one, _global(one))[0] == 0
+    big = rng.uniform(-3e38, 3e38, (40, 4)).astype(np.float32)
+    bb = np.concatenate([big, big + np.abs(big) * np.float32(1e-3)], axis=1)
+    assert _qbox_check(lib, bb, _global(bb))[0] == 0
