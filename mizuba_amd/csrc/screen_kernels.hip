@@ -801,7 +801,7 @@ __global__ __launch_bounds__(256) void k_gather(const Prob *__restrict__ P, uint
 // data, code, object index): the reference's comparator (:179-196) with its
 // unspecified tie order fixed to the object index. Objects are distributed
 // into 2^12 buckets by the top code bits (LDS counters); then each window of
-// 1024 sorted positions, widened to whole buckets, is staged in LDS and every
+// 2048 sorted positions, widened to whole buckets, is staged in LDS and every
 // object finds its place by counting the smaller keys of its bucket. Objects
 // without data go after the finite ones in index order. A step with a bucket
 // larger than kBktMax (a tight cluster of objects) is left to the radix sort
@@ -809,12 +809,8 @@ __global__ __launch_bounds__(256) void k_gather(const Prob *__restrict__ P, uint
 // ---------------------------------------------------------------------------
 
 constexpr uint32_t kBktMax = 1024;
-constexpr uint32_t kBktWin = 1024 + 2 * kBktMax;  // a window and the buckets it cuts
-
-__device__ inline bool key_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb)
-{
-    return ka < kb || (ka == kb && va < vb);
-}
+constexpr uint32_t kBktWinPos = 2048;                   // sorted positions per window, two per thread
+constexpr uint32_t kBktWin = kBktWinPos + 2 * kBktMax;  // a window and the buckets it cuts
 
 // LDS of the bucket sort of one step.
 template <uint32_t BITS>
@@ -823,7 +819,7 @@ struct BucketLds {
     uint64_t key[kBktWin];
     uint32_t val[kBktWin];
     uint32_t w[65];
-    uint32_t big, win[2];
+    uint32_t big;
 };
 
 // The bucket sort of step c of problem q by one 1024-thread workgroup.
@@ -848,20 +844,31 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
     uint32_t *s_val = L.val;
     uint32_t *s_w = L.w;
     uint32_t &s_big = L.big;
-    uint32_t *s_win = L.win;
     const float4 glo = reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[0];
     const float4 ghi = reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[1];
 
     for (uint32_t k = tid; k < kBkts; k += 1024u) s_off[k] = 0;
     if (tid == 0) s_big = 0;
     __syncthreads();
-    // codes and bucket sizes
-    for (uint32_t o = tid; o < N; o += 1024u) {
+    // codes and bucket sizes (two objects per thread at a time, their boxes
+    // read together)
+    for (uint32_t o = tid; o < N; o += 2048u) {
+        const uint32_t o2 = o + 1024u;
         const float4 blo = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[0];
         const float4 bhi = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[1];
+        float4 blo2 = blo, bhi2 = bhi;
+        if (o2 < N) {
+            blo2 = reinterpret_cast<const float4 *>(boxes + (size_t)o2 * 8u)[0];
+            bhi2 = reinterpret_cast<const float4 *>(boxes + (size_t)o2 * 8u)[1];
+        }
         const uint64_t code = morton_code(blo, bhi, glo, ghi);
         codes[o] = code;
         if (is_finite_f(blo.x)) atomicAdd(&s_off[(uint32_t)(code >> (64u - kBktBits))], 1u);
+        if (o2 < N) {
+            const uint64_t code2 = morton_code(blo2, bhi2, glo, ghi);
+            codes[o2] = code2;
+            if (is_finite_f(blo2.x)) atomicAdd(&s_off[(uint32_t)(code2 >> (64u - kBktBits))], 1u);
+        }
     }
     __syncthreads();
     // exclusive offsets of the buckets (kBkts / 1024 per thread), the largest bucket
@@ -895,67 +902,107 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
     }
     // finite objects into their buckets (any order inside a bucket); objects
     // without data straight to their final place, in index order
-    uint32_t inf_carry = 0;
-    for (uint32_t base = 0; base < N; base += 1024u) {
-        const uint32_t o = base + tid;
-        bool fin = false, inf = false;
-        float4 blo, bhi;
-        if (o < N) {
-            blo = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[0];
-            bhi = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[1];
-            fin = is_finite_f(blo.x);
-            inf = !fin;
+    if (nfin == N) {
+        // every object has data: each thread moves the codes it made, four
+        // loads in flight
+        uint32_t o = tid;
+        for (; o + 3u * 1024u < N; o += 4u * 1024u) {
+            uint64_t cd[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k) cd[k] = codes[o + k * 1024u];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k) {
+                const uint32_t pos = atomicAdd(&s_off[(uint32_t)(cd[k] >> (64u - kBktBits))], 1u);
+                kt[pos] = cd[k];
+                vt[pos] = o + k * 1024u;
+            }
         }
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan_1024(inf ? 1u : 0u, s_w, tot);
-        if (fin) {
+        for (; o < N; o += 1024u) {
             const uint64_t code = codes[o];
             const uint32_t pos = atomicAdd(&s_off[(uint32_t)(code >> (64u - kBktBits))], 1u);
             kt[pos] = code;
             vt[pos] = o;
-        } else if (inf) {
-            const uint32_t p = nfin + inf_carry + ex;
-            ko[p] = ~0ull;
-            vo[p] = o;
-            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] = blo;
-            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] = bhi;
         }
-        inf_carry += tot;
+    } else {
+        uint32_t inf_carry = 0;
+        for (uint32_t base = 0; base < N; base += 1024u) {
+            const uint32_t o = base + tid;
+            bool fin = false, inf = false;
+            float4 blo, bhi;
+            if (o < N) {
+                blo = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[0];
+                bhi = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[1];
+                fin = is_finite_f(blo.x);
+                inf = !fin;
+            }
+            uint32_t tot;
+            const uint32_t ex = block_excl_scan_1024(inf ? 1u : 0u, s_w, tot);
+            if (fin) {
+                const uint64_t code = codes[o];
+                const uint32_t pos = atomicAdd(&s_off[(uint32_t)(code >> (64u - kBktBits))], 1u);
+                kt[pos] = code;
+                vt[pos] = o;
+            } else if (inf) {
+                const uint32_t p = nfin + inf_carry + ex;
+                ko[p] = ~0ull;
+                vo[p] = o;
+                reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] = blo;
+                reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] = bhi;
+            }
+            inf_carry += tot;
+        }
     }
     __syncthreads();
-    __syncthreads();
-    // s_off[b] is now the end of bucket b. Windows of 1024 positions, widened
-    // to the buckets cut by their ends, are ranked from LDS.
-    for (uint32_t base = 0; base < nfin; base += 1024u) {
-        if (tid == 0) {
-            const uint32_t last = min(base + 1023u, nfin - 1u);
-            const uint32_t b0 = (uint32_t)(kt[base] >> (64u - kBktBits));
-            const uint32_t b1 = (uint32_t)(kt[last] >> (64u - kBktBits));
-            s_win[0] = b0 ? s_off[b0 - 1u] : 0u;
-            s_win[1] = s_off[b1];
+    // s_off[b] is now the end of bucket b. Windows of kBktWinPos positions,
+    // widened to the buckets cut by their ends, are ranked from LDS: an
+    // object's place is its bucket's start plus the number of smaller keys
+    // (code, then index) in its bucket.
+    auto bucket_at = [&](uint32_t pos) {
+        // the first bucket ending after pos (the ends do not decrease)
+        uint32_t lo = 0, hi = kBkts - 1u;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_off[mid] > pos) hi = mid;
+            else lo = mid + 1u;
         }
-        __syncthreads();
-        const uint32_t wb = s_win[0], we = s_win[1];
+        return lo;
+    };
+    for (uint32_t base = 0; base < nfin; base += kBktWinPos) {
+        const uint32_t b0 = bucket_at(base), b1 = bucket_at(min(base + kBktWinPos - 1u, nfin - 1u));
+        const uint32_t wb = b0 ? s_off[b0 - 1u] : 0u, we = s_off[b1];
         for (uint32_t e = wb + tid; e < we; e += 1024u) {
             s_key[e - wb] = kt[e];
             s_val[e - wb] = vt[e];
         }
         __syncthreads();
-        const uint32_t e = base + tid;
-        if (e < nfin) {
-            const uint64_t key = s_key[e - wb];
-            const uint32_t val = s_val[e - wb];
-            const uint32_t b = (uint32_t)(key >> (64u - kBktBits));
-            const uint32_t beg = b ? s_off[b - 1u] : 0u, end = s_off[b];
-            uint32_t rank = 0;
-            for (uint32_t m = beg; m < end; ++m) rank += key_less(s_key[m - wb], s_val[m - wb], key, val) ? 1u : 0u;
-            const uint32_t p = beg + rank;
-            ko[p] = key;
-            vo[p] = val;
-            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] =
-                reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[0];
-            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] =
-                reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
+#pragma unroll
+        for (uint32_t h = 0; h < 2u; ++h) {
+            const uint32_t e = base + h * 1024u + tid;
+            if (e < nfin) {
+                const uint64_t key = s_key[e - wb];
+                const uint32_t val = s_val[e - wb];
+                // the object's box, read while the rank is counted
+                const float4 glo = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[0];
+                const float4 ghi = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
+                const uint32_t b = (uint32_t)(key >> (64u - kBktBits));
+                const uint32_t beg = b ? s_off[b - 1u] : 0u, end = s_off[b];
+                uint32_t rank = 0, eq = 0;
+                for (uint32_t m = beg; m < end; ++m) {
+                    const uint64_t km = s_key[m - wb];
+                    rank += km < key ? 1u : 0u;
+                    eq += km == key ? 1u : 0u;
+                }
+                if (eq > 1u) {
+                    // equal codes: the lower object index first
+                    for (uint32_t m = beg; m < end; ++m)
+                        rank += (s_key[m - wb] == key && s_val[m - wb] < val) ? 1u : 0u;
+                }
+                const uint32_t p = beg + rank;
+                ko[p] = key;
+                vo[p] = val;
+                reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] = glo;
+                reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] = ghi;
+            }
         }
         __syncthreads();
     }
