@@ -850,24 +850,24 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
     for (uint32_t k = tid; k < kBkts; k += 1024u) s_off[k] = 0;
     if (tid == 0) s_big = 0;
     __syncthreads();
-    // codes and bucket sizes (two objects per thread at a time, their boxes
-    // read together)
-    for (uint32_t o = tid; o < N; o += 2048u) {
-        const uint32_t o2 = o + 1024u;
-        const float4 blo = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[0];
-        const float4 bhi = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[1];
-        float4 blo2 = blo, bhi2 = bhi;
-        if (o2 < N) {
-            blo2 = reinterpret_cast<const float4 *>(boxes + (size_t)o2 * 8u)[0];
-            bhi2 = reinterpret_cast<const float4 *>(boxes + (size_t)o2 * 8u)[1];
+    // codes and bucket sizes; the next object's box is read while the
+    // current one's code is made
+    {
+        const float4 *bx = reinterpret_cast<const float4 *>(boxes);
+        float4 blo = make_float4(0.f, 0.f, 0.f, 0.f), bhi = blo;
+        if (tid < N) {
+            blo = bx[(size_t)tid * 2u];
+            bhi = bx[(size_t)tid * 2u + 1u];
         }
-        const uint64_t code = morton_code(blo, bhi, glo, ghi);
-        codes[o] = code;
-        if (is_finite_f(blo.x)) atomicAdd(&s_off[(uint32_t)(code >> (64u - kBktBits))], 1u);
-        if (o2 < N) {
-            const uint64_t code2 = morton_code(blo2, bhi2, glo, ghi);
-            codes[o2] = code2;
-            if (is_finite_f(blo2.x)) atomicAdd(&s_off[(uint32_t)(code2 >> (64u - kBktBits))], 1u);
+        for (uint32_t o = tid; o < N; o += 1024u) {
+            // (the last object's box again past the end)
+            const uint32_t on = min(o + 1024u, N - 1u);
+            const float4 nlo = bx[(size_t)on * 2u], nhi = bx[(size_t)on * 2u + 1u];
+            const uint64_t code = morton_code(blo, bhi, glo, ghi);
+            codes[o] = code;
+            if (is_finite_f(blo.x)) atomicAdd(&s_off[(uint32_t)(code >> (64u - kBktBits))], 1u);
+            blo = nlo;
+            bhi = nhi;
         }
     }
     __syncthreads();
@@ -902,33 +902,33 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
     }
     // finite objects into their buckets (any order inside a bucket); objects
     // without data straight to their final place, in index order
-    if (nfin == N) {
+    if (nfin == N && N > 0u) {
         // every object has data: each thread moves the codes it made, four
-        // loads in flight
-        uint32_t o = tid;
-        for (; o + 3u * 1024u < N; o += 4u * 1024u) {
-            uint64_t cd[4];
+        // at a time, the next four read while the current ones are placed
+        // (the last object's code again past the end)
+        uint64_t cd[4];
 #pragma unroll
-            for (uint32_t k = 0; k < 4u; ++k) cd[k] = codes[o + k * 1024u];
+        for (uint32_t k = 0; k < 4u; ++k) cd[k] = codes[min(tid + k * 1024u, N - 1u)];
+        for (uint32_t o = tid; o < N; o += 4u * 1024u) {
+            uint64_t nx[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k) nx[k] = codes[min(o + (4u + k) * 1024u, N - 1u)];
 #pragma unroll
             for (uint32_t k = 0; k < 4u; ++k) {
-                const uint32_t pos = atomicAdd(&s_off[(uint32_t)(cd[k] >> (64u - kBktBits))], 1u);
-                kt[pos] = cd[k];
-                vt[pos] = o + k * 1024u;
+                if (o + k * 1024u < N) {
+                    const uint32_t pos = atomicAdd(&s_off[(uint32_t)(cd[k] >> (64u - kBktBits))], 1u);
+                    kt[pos] = cd[k];
+                    vt[pos] = o + k * 1024u;
+                }
+                cd[k] = nx[k];
             }
         }
-        for (; o < N; o += 1024u) {
-            const uint64_t code = codes[o];
-            const uint32_t pos = atomicAdd(&s_off[(uint32_t)(code >> (64u - kBktBits))], 1u);
-            kt[pos] = code;
-            vt[pos] = o;
-        }
-    } else {
+    } else if (nfin < N) {
         uint32_t inf_carry = 0;
         for (uint32_t base = 0; base < N; base += 1024u) {
             const uint32_t o = base + tid;
             bool fin = false, inf = false;
-            float4 blo, bhi;
+            float4 blo = make_float4(0.f, 0.f, 0.f, 0.f), bhi = blo;
             if (o < N) {
                 blo = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[0];
                 bhi = reinterpret_cast<const float4 *>(boxes + (size_t)o * 8u)[1];
@@ -967,14 +967,53 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
         }
         return lo;
     };
-    for (uint32_t base = 0; base < nfin; base += kBktWinPos) {
+    // the window starting at sorted position base: [wb, we)
+    auto window = [&](uint32_t base, uint32_t &wb, uint32_t &we) {
         const uint32_t b0 = bucket_at(base), b1 = bucket_at(min(base + kBktWinPos - 1u, nfin - 1u));
-        const uint32_t wb = b0 ? s_off[b0 - 1u] : 0u, we = s_off[b1];
-        for (uint32_t e = wb + tid; e < we; e += 1024u) {
-            s_key[e - wb] = kt[e];
-            s_val[e - wb] = vt[e];
+        wb = b0 ? s_off[b0 - 1u] : 0u;
+        we = s_off[b1];
+    };
+    // LDS-only barriers: the sorted outputs' global stores stay in flight
+    auto lds_barrier = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    };
+    // each window is read into registers during the previous window's ranking
+    constexpr uint32_t kStage = kBktWin / 1024u;
+    uint64_t pk[kStage];
+    uint32_t pv[kStage];
+    uint32_t wb = 0, we = 0;
+    if (nfin > 0u) {
+        window(0u, wb, we);
+        // (the window's last entry again past its end)
+#pragma unroll
+        for (uint32_t j = 0; j < kStage; ++j) {
+            const uint32_t x = min(wb + tid + j * 1024u, we - 1u);
+            pk[j] = kt[x];
+            pv[j] = vt[x];
         }
-        __syncthreads();
+    }
+    for (uint32_t base = 0; base < nfin; base += kBktWinPos) {
+#pragma unroll
+        for (uint32_t j = 0; j < kStage; ++j) {
+            const uint32_t x = wb + tid + j * 1024u;
+            if (x < we) {
+                s_key[x - wb] = pk[j];
+                s_val[x - wb] = pv[j];
+            }
+        }
+        lds_barrier();
+        uint32_t nwb = 0, nwe = 0;
+        if (base + kBktWinPos < nfin) {
+            window(base + kBktWinPos, nwb, nwe);
+#pragma unroll
+            for (uint32_t j = 0; j < kStage; ++j) {
+                const uint32_t x = min(nwb + tid + j * 1024u, nwe - 1u);
+                pk[j] = kt[x];
+                pv[j] = vt[x];
+            }
+        }
 #pragma unroll
         for (uint32_t h = 0; h < 2u; ++h) {
             const uint32_t e = base + h * 1024u + tid;
@@ -1004,7 +1043,9 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
                 reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] = ghi;
             }
         }
-        __syncthreads();
+        lds_barrier();
+        wb = nwb;
+        we = nwe;
     }
 }
 
