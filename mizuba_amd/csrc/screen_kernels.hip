@@ -994,6 +994,42 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
             pv[j] = vt[x];
         }
     }
+    // each object's box is gathered during one window's ranking and stored
+    // during the next one's, so that its read does not hold the window up
+    float4 dlo0 = make_float4(0.f, 0.f, 0.f, 0.f), dhi0 = dlo0, dlo1 = dlo0, dhi1 = dlo0;
+    uint32_t dp0 = ~0u, dp1 = ~0u;
+    auto store_box = [&](uint32_t p, const float4 &lo, const float4 &hi) {
+        if (p != ~0u) {
+            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] = lo;
+            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] = hi;
+        }
+    };
+    // the rank of the object at sorted position e of the window [wb, we)
+    auto rank_one = [&](uint32_t e, uint32_t wb, float4 &lo, float4 &hi, uint32_t &dp) {
+        dp = ~0u;
+        if (e >= nfin) return;
+        const uint64_t key = s_key[e - wb];
+        const uint32_t val = s_val[e - wb];
+        // the object's box, stored in the next window
+        lo = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[0];
+        hi = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
+        const uint32_t b = (uint32_t)(key >> (64u - kBktBits));
+        const uint32_t beg = b ? s_off[b - 1u] : 0u, end = s_off[b];
+        uint32_t rank = 0, eq = 0;
+        for (uint32_t m = beg; m < end; ++m) {
+            const uint64_t km = s_key[m - wb];
+            rank += km < key ? 1u : 0u;
+            eq += km == key ? 1u : 0u;
+        }
+        if (eq > 1u) {
+            // equal codes: the lower object index first
+            for (uint32_t m = beg; m < end; ++m) rank += (s_key[m - wb] == key && s_val[m - wb] < val) ? 1u : 0u;
+        }
+        const uint32_t p = beg + rank;
+        ko[p] = key;
+        vo[p] = val;
+        dp = p;
+    };
     for (uint32_t base = 0; base < nfin; base += kBktWinPos) {
 #pragma unroll
         for (uint32_t j = 0; j < kStage; ++j) {
@@ -1004,6 +1040,8 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
             }
         }
         lds_barrier();
+        store_box(dp0, dlo0, dhi0);
+        store_box(dp1, dlo1, dhi1);
         uint32_t nwb = 0, nwe = 0;
         if (base + kBktWinPos < nfin) {
             window(base + kBktWinPos, nwb, nwe);
@@ -1014,39 +1052,14 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
                 pv[j] = vt[x];
             }
         }
-#pragma unroll
-        for (uint32_t h = 0; h < 2u; ++h) {
-            const uint32_t e = base + h * 1024u + tid;
-            if (e < nfin) {
-                const uint64_t key = s_key[e - wb];
-                const uint32_t val = s_val[e - wb];
-                // the object's box, read while the rank is counted
-                const float4 glo = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[0];
-                const float4 ghi = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
-                const uint32_t b = (uint32_t)(key >> (64u - kBktBits));
-                const uint32_t beg = b ? s_off[b - 1u] : 0u, end = s_off[b];
-                uint32_t rank = 0, eq = 0;
-                for (uint32_t m = beg; m < end; ++m) {
-                    const uint64_t km = s_key[m - wb];
-                    rank += km < key ? 1u : 0u;
-                    eq += km == key ? 1u : 0u;
-                }
-                if (eq > 1u) {
-                    // equal codes: the lower object index first
-                    for (uint32_t m = beg; m < end; ++m)
-                        rank += (s_key[m - wb] == key && s_val[m - wb] < val) ? 1u : 0u;
-                }
-                const uint32_t p = beg + rank;
-                ko[p] = key;
-                vo[p] = val;
-                reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] = glo;
-                reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] = ghi;
-            }
-        }
+        rank_one(base + tid, wb, dlo0, dhi0, dp0);
+        rank_one(base + 1024u + tid, wb, dlo1, dhi1, dp1);
         lds_barrier();
         wb = nwb;
         we = nwe;
     }
+    store_box(dp0, dlo0, dhi0);
+    store_box(dp1, dlo1, dhi1);
 }
 
 // Next work item of a persistent workgroup: one atomic per workgroup on the
@@ -1064,7 +1077,9 @@ __device__ inline uint32_t next_item(uint32_t *counter, uint32_t *s_item)
 // order of the list (largest problems first, so that the long items start
 // early and the short ones fill the end).
 template <uint32_t BITS>
-__global__ __launch_bounds__(1024) void k_bucket_sort(const Prob *__restrict__ P, const uint2 *__restrict__ items,
+// (one 1024-thread workgroup runs per CU whatever the LDS size, so the
+// registers are sized for 4 waves per SIMD)
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_bucket_sort(const Prob *__restrict__ P, const uint2 *__restrict__ items,
                                                       uint32_t n_items, uint32_t *counter)
 {
     __shared__ BucketLds<BITS> L;
