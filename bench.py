@@ -67,16 +67,18 @@ def broad_bytes_per_launch(n_objs, n_steps, n_pairs):
     """Algorithmic bytes of one broad-phase traversal launch: per (object,
     conjunction step) the query box (32 B), its sorted-to-original index and
     object type (8 B), its pair count (4 B) and its share of the step's tree
-    read once (2 nodes x 32 B), plus one 4 B partner index per pair."""
-    return n_steps * n_objs * (32 + 8 + 4 + 64) + 4 * n_pairs
+    read once (2 nodes of the reference's 48 B layout; the walk tree's own
+    records are 32 B), plus one 4 B partner index per pair."""
+    return n_steps * n_objs * (32 + 8 + 4 + 96) + 4 * n_pairs
 
 
 def bvh_bytes_per_launch(n_objs, n_steps):
     """Algorithmic bytes of one build of the broad phase's walk tree
     (k_lbvh_build, then k_lbvh_top for the nodes that span blocks): per
     (object, conjunction step) the sorted code (8 B) and box (32 B) read once
-    and two 32 B nodes (a leaf and an internal node) written."""
-    return n_steps * n_objs * (8 + 32 + 64)
+    and two nodes (a leaf and an internal node) of the reference's 48 B layout
+    written: 136 B per object-step (the walk tree's own records are 32 B)."""
+    return n_steps * n_objs * (8 + 32 + 96)
 
 
 def hbm_traffic(kernel_prefix):
