@@ -1672,8 +1672,10 @@ __device__ inline void bp_emit(bool has, uint32_t c, uint32_t i, uint32_t j, uin
 //
 // Here every pair is found once. The query at sorted position o walks only
 // the part of the tree to the right of its own run of equal codes [rb, re):
-// at an internal node it goes straight to the right child when the split is
-// at or before re, so no subtree left of the run is ever read. Its query box
+// it starts at the escape of its run's leaf, the node that follows the leaf
+// in depth-first order, so it visits the subtrees hanging to the right of the
+// path from the root to its leaf and never the path itself (whose nodes all
+// hold the query's leaf, so a walk from the root would enter each). Its query box
 // is the box of its own leaf, which contains box_o, so every pair between o
 // and a later leaf is reached from o whichever of the two has the lower
 // original index; the reference's condition is then checked exactly for the
@@ -1764,7 +1766,7 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
                                 reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[1]),
                           ql, qu);
     const uint32_t lim = 2u * n;
-    uint32_t i = lbvh_root(n), visits = 0;
+    uint32_t i = lbvh_escape(LbvhArray{codes}, n, re - 1u), visits = 0;
     while (i < lim) {
         if (++visits > lim) {
             report_error(q.err, kErrBvhWalk, c, o, 0.0);
@@ -1774,9 +1776,10 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
         const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
         // the next node is chosen from all four words on every path, so that
         // the record is read with one load: an internal node {escape, right,
-        // left, 1} is entered, skipping the left child when it ends at or
-        // before the run (the split is the right child's index halved); a
-        // leaf {escape, begin, end, 0}, or a node missed, is left by its escape
+        // left, 1} is entered (skipping the left child if it ended at or
+        // before the run, which no node after the run does: the split is the
+        // right child's index halved); a leaf {escape, begin, end, 0}, or a
+        // node missed, is left by its escape
         const bool hit_node = qbox_overlap(qq, nq);
         const bool enter = hit_node && meta.w != 0u;
         const uint32_t next = enter ? ((meta.y >> 1) <= re ? meta.y : meta.z) : meta.x;
