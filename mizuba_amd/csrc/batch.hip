@@ -288,6 +288,7 @@ void ScreenBatch::prepare()
         MZH_HIP(ctx->n_eff.ensure(sizeof(uint32_t) * C));
         MZH_HIP(ctx->sort_todo.ensure(sizeof(uint32_t) * C));
         MZH_HIP(ctx->gbox.ensure(sizeof(uint32_t) * 8u * C));
+        MZH_HIP(ctx->bkt_end.ensure(sizeof(uint32_t) * kBktEndStride * C));
         MZH_HIP(ctx->wtree.ensure(sizeof(WalkNode) * ctx->node_cap * C));
         MZH_HIP(ctx->bvh_hdr.ensure(sizeof(uint32_t) * lbvh_step_words(N) * C));
         const size_t scr_b = bvh_scratch_bytes_host(N);
@@ -335,6 +336,7 @@ void ScreenBatch::prepare()
         q.keys_tmp = gl(ctx->keys_tmp.as<uint64_t>());
         q.srt_idx = gl(ctx->srt_idx.as<uint32_t>());
         q.vals_tmp = gl(ctx->vals_tmp.as<uint32_t>());
+        q.bkt_end = gl(ctx->bkt_end.as<uint32_t>());
         q.n_eff = gl(ctx->n_eff.as<uint32_t>());
         q.sort_todo = gl(ctx->sort_todo.as<uint32_t>());
         q.wtree = gl(ctx->wtree.as<WalkNode>());
@@ -449,7 +451,7 @@ void ScreenBatch::enqueue_stages(bool timed)
     MZH_HIP(launch_gbox_finalize(P, n, tabs, n_steps_all, st));
     if (timed) MZH_HIP(hipEventRecord(ev[1], st));
     // stage 2: Morton codes, sort, gather
-    MZH_HIP(launch_morton_sort(P, n, tabs, n_steps_all, n_gather_blocks, bc, ls, st));
+    MZH_HIP(launch_morton_sort(P, n, tabs, n_steps_all, n_gather_blocks, n_lbvh_blocks, bc, ls, st));
     if (timed) MZH_HIP(hipEventRecord(ev[2], st));
     // stage 3: the walk tree
     MZH_HIP(launch_lbvh(P, n, tabs, n_lbvh_blocks, n_steps_all, st));

@@ -1616,7 +1616,7 @@ int mzh_release(mzh_ctx *ctx, int what)
         ++ctx->generation;
         if (what & MZH_RELEASE_SCRATCH) {
             for (DevBuf *b : {&ctx->kbke, &ctx->nprec, &ctx->keys_tmp, &ctx->vals_tmp, &ctx->sort_todo, &ctx->gbox,
-                              &ctx->wtree, &ctx->bvh_hdr, &ctx->bvh_scr, &ctx->otypes, &ctx->cnt, &ctx->head, &ctx->off,
+                              &ctx->wtree, &ctx->bkt_end, &ctx->bvh_hdr, &ctx->bvh_scr, &ctx->otypes, &ctx->cnt, &ctx->head, &ctx->off,
                               &ctx->step_total, &ctx->slots, &ctx->ovf, &ctx->ovf_count, &ctx->pair_c, &ctx->ev_tm,
                               &ctx->ev_sel,
                               &ctx->ev_out, &ctx->ev_bad})

@@ -161,7 +161,7 @@ struct mzh_ctx {
     mzh::host::DevBuf aabbs, kbke, srt_aabbs, mcodes, srt_mcodes, srt_idx, keys_tmp, vals_tmp, n_eff, sort_todo, gbox;
     mzh::host::DevBuf nprec;  // mixed-precision pre-screen models, [C][N][2] float4
     uint64_t node_cap = 0;
-    mzh::host::DevBuf nodes, trav, wtree, bvh_hdr, bvh_scr, tree_size;
+    mzh::host::DevBuf nodes, trav, wtree, bvh_hdr, bvh_scr, tree_size, bkt_end;
     bool nodes_ready = false;  // nodes holds the reference-layout export of trav
     mzh::host::DevBuf otypes, cnt, head, off, step_total, step_off, slots, ovf, ovf_count;
     uint32_t ovf_cap = 0;

@@ -82,6 +82,7 @@ struct Prob {
     gbuf<uint32_t> gbox;  // [C][8] order-preserving images of the global box
     gbuf<uint64_t> mcodes, srt_mcodes, keys_tmp;
     gbuf<uint32_t> srt_idx, vals_tmp, n_eff, sort_todo;
+    gbuf<uint32_t> bkt_end;  // [C][kBktEndStride]: the bucket sort's bucket ends per step
     gbuf<TravNode> trav;  // [C][node_cap], the breadth-first trees, made on request
     gbuf<WalkNode> wtree;  // [C][node_cap], the broad phase's walk trees
     gbuf<uint4> bvh_hdr;  // walk-tree build: [C][lbvh_stride] 32-bit words; breadth-first build: [C][node_cap]
@@ -155,7 +156,8 @@ hipError_t launch_gbox_finalize(const Prob *P, uint32_t np, const BatchTables &t
 // in sorted order (persistent workgroups over the sort items); steps with a
 // bucket too large for the bucket sort go to the radix sort and gather.
 hipError_t launch_morton_sort(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
-                              uint32_t n_gather_blocks, BatchCounters *bc, const LaunchShape &ls, hipStream_t stream);
+                              uint32_t n_gather_blocks, uint32_t n_rank_blocks, BatchCounters *bc,
+                              const LaunchShape &ls, hipStream_t stream);
 // Bytes of per-step BVH scratch (LDS when it fits kBvhLdsMax, else HBM).
 size_t bvh_scratch_bytes_host(uint32_t n_objs);
 size_t bvh_scratch_stride(uint32_t n_objs);
@@ -165,6 +167,7 @@ constexpr size_t kBvhLdsMaxHost = 64u * 1024u;
 // positions (the lbvh_blk table), then one workgroup per step for the nodes
 // that span blocks. Also counts each step's tree size.
 constexpr uint32_t kLbvhBlock = 512;
+constexpr uint32_t kBktEndStride = 4096;  // bucket ends per step (2^12, the widest bucket sort)
 // 32-bit words per step of the walk-tree build's scratch (in the BVH scratch
 // buffer) for steps of at most n_objs objects: the parent of every node, two
 // counts, and the lists of spanning and frontier nodes sized by their bound.

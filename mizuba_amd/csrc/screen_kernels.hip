@@ -800,24 +800,22 @@ __global__ __launch_bounds__(256) void k_gather(const Prob *__restrict__ P, uint
 // (detect_conjunctions_morton.cpp:70-223). The sort key is (box without
 // data, code, object index): the reference's comparator (:179-196) with its
 // unspecified tie order fixed to the object index. Objects are distributed
-// into 2^12 buckets by the top code bits (LDS counters); then each window of
-// 2048 sorted positions, widened to whole buckets, is staged in LDS and every
-// object finds its place by counting the smaller keys of its bucket. Objects
+// into 2^12 buckets by the top code bits (LDS counters) and scattered to them
+// by one workgroup per step; then each block of 512 sorted positions, widened
+// to whole buckets, is staged in LDS by a workgroup of its own
+// (k_bucket_rank) and every object finds its place by counting the smaller
+// keys of its bucket. Objects
 // without data go after the finite ones in index order. A step with a bucket
 // larger than kBktMax (a tight cluster of objects) is left to the radix sort
 // (k_sort + k_gather), flagged in todo.
 // ---------------------------------------------------------------------------
 
 constexpr uint32_t kBktMax = 1024;
-constexpr uint32_t kBktWinPos = 2048;                   // sorted positions per window, two per thread
-constexpr uint32_t kBktWin = kBktWinPos + 2 * kBktMax;  // a window and the buckets it cuts
 
 // LDS of the bucket sort of one step.
 template <uint32_t BITS>
 struct BucketLds {
     uint32_t off[1u << BITS];
-    uint64_t key[kBktWin];
-    uint32_t val[kBktWin];
     uint32_t w[65];
     uint32_t big;
 };
@@ -840,8 +838,6 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
     uint64_t *ko = q.srt_mcodes + (size_t)c * N;
     uint32_t *vo = q.srt_idx + (size_t)c * N;
     uint32_t *s_off = L.off;
-    uint64_t *s_key = L.key;
-    uint32_t *s_val = L.val;
     uint32_t *s_w = L.w;
     uint32_t &s_big = L.big;
     const float4 glo = reinterpret_cast<const float4 *>(boxes + (size_t)N * 8u)[0];
@@ -953,113 +949,87 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
         }
     }
     __syncthreads();
-    // s_off[b] is now the end of bucket b. Windows of kBktWinPos positions,
-    // widened to the buckets cut by their ends, are ranked from LDS: an
-    // object's place is its bucket's start plus the number of smaller keys
-    // (code, then index) in its bucket.
+    // s_off[b] is now the end of bucket b: for the ranking (k_bucket_rank)
+    uint32_t *ends = q.bkt_end + (size_t)c * kBktEndStride;
+    for (uint32_t k = tid; k < kBkts; k += 1024u) ends[k] = s_off[k];
+}
+
+// The ranking of the bucket sort, one workgroup per block of kLbvhBlock
+// sorted positions of a step (one thread each): the block's positions,
+// widened to the whole buckets they cut (found by a binary search of the
+// bucket ends), are staged in LDS, and each object's place is its bucket's
+// start plus the number of smaller codes in its bucket (object indices
+// compared only among equal codes). The final write gathers the box, so the
+// sorted boxes, codes and indices leave in one pass.
+template <uint32_t BITS>
+__global__ __launch_bounds__(kLbvhBlock) void k_bucket_rank(const Prob *__restrict__ P, uint32_t np,
+                                                           const uint32_t *__restrict__ blk_pre)
+{
+    constexpr uint32_t kBkts = 1u << BITS;
+    constexpr uint32_t kWin = kLbvhBlock + 2u * kBktMax;
+    __shared__ uint32_t s_end[kBkts];
+    __shared__ uint64_t s_key[kWin];
+    __shared__ uint32_t s_val[kWin];
+    const uint32_t gblk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t pi = find_prob(blk_pre, np, gblk);
+    const Prob &q = P[pi];
+    const uint32_t lblk = gblk - blk_pre[pi];
+    const uint32_t N = q.N;
+    const uint32_t bps = (N + kLbvhBlock - 1u) / kLbvhBlock;
+    const uint32_t c = lblk / bps;
+    const uint32_t B0 = (lblk - c * bps) * kLbvhBlock;
+    if (q.sort_todo[c]) return;  // left to the radix sort
+    const uint32_t nfin = q.n_eff[c];
+    if (B0 >= nfin) return;  // the whole workgroup
+    const uint32_t tid = threadIdx.x;
+    const uint32_t *ends = q.bkt_end + (size_t)c * kBktEndStride;
+    for (uint32_t k = tid; k < kBkts; k += kLbvhBlock) s_end[k] = ends[k];
+    __syncthreads();
     auto bucket_at = [&](uint32_t pos) {
         // the first bucket ending after pos (the ends do not decrease)
         uint32_t lo = 0, hi = kBkts - 1u;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (s_off[mid] > pos) hi = mid;
+            if (s_end[mid] > pos) hi = mid;
             else lo = mid + 1u;
         }
         return lo;
     };
-    // the window starting at sorted position base: [wb, we)
-    auto window = [&](uint32_t base, uint32_t &wb, uint32_t &we) {
-        const uint32_t b0 = bucket_at(base), b1 = bucket_at(min(base + kBktWinPos - 1u, nfin - 1u));
-        wb = b0 ? s_off[b0 - 1u] : 0u;
-        we = s_off[b1];
-    };
-    // LDS-only barriers: the sorted outputs' global stores stay in flight
-    auto lds_barrier = [] {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-        __builtin_amdgcn_s_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    };
-    // each window is read into registers during the previous window's ranking
-    constexpr uint32_t kStage = kBktWin / 1024u;
-    uint64_t pk[kStage];
-    uint32_t pv[kStage];
-    uint32_t wb = 0, we = 0;
-    if (nfin > 0u) {
-        window(0u, wb, we);
-        // (the window's last entry again past its end)
-#pragma unroll
-        for (uint32_t j = 0; j < kStage; ++j) {
-            const uint32_t x = min(wb + tid + j * 1024u, we - 1u);
-            pk[j] = kt[x];
-            pv[j] = vt[x];
-        }
+    const uint32_t b0 = bucket_at(B0), b1 = bucket_at(min(B0 + kLbvhBlock - 1u, nfin - 1u));
+    const uint32_t wb = b0 ? s_end[b0 - 1u] : 0u, we = s_end[b1];
+    const uint64_t *kt = q.keys_tmp + (size_t)c * N;
+    const uint32_t *vt = q.vals_tmp + (size_t)c * N;
+    for (uint32_t x = wb + tid; x < we; x += kLbvhBlock) {
+        s_key[x - wb] = kt[x];
+        s_val[x - wb] = vt[x];
     }
-    // each object's box is gathered during one window's ranking and stored
-    // during the next one's, so that its read does not hold the window up
-    float4 dlo0 = make_float4(0.f, 0.f, 0.f, 0.f), dhi0 = dlo0, dlo1 = dlo0, dhi1 = dlo0;
-    uint32_t dp0 = ~0u, dp1 = ~0u;
-    auto store_box = [&](uint32_t p, const float4 &lo, const float4 &hi) {
-        if (p != ~0u) {
-            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] = lo;
-            reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] = hi;
-        }
-    };
-    // the rank of the object at sorted position e of the window [wb, we)
-    auto rank_one = [&](uint32_t e, uint32_t wb, float4 &lo, float4 &hi, uint32_t &dp) {
-        dp = ~0u;
-        if (e >= nfin) return;
-        const uint64_t key = s_key[e - wb];
-        const uint32_t val = s_val[e - wb];
-        // the object's box, stored in the next window
-        lo = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[0];
-        hi = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
-        const uint32_t b = (uint32_t)(key >> (64u - kBktBits));
-        const uint32_t beg = b ? s_off[b - 1u] : 0u, end = s_off[b];
-        uint32_t rank = 0, eq = 0;
-        for (uint32_t m = beg; m < end; ++m) {
-            const uint64_t km = s_key[m - wb];
-            rank += km < key ? 1u : 0u;
-            eq += km == key ? 1u : 0u;
-        }
-        if (eq > 1u) {
-            // equal codes: the lower object index first
-            for (uint32_t m = beg; m < end; ++m) rank += (s_key[m - wb] == key && s_val[m - wb] < val) ? 1u : 0u;
-        }
-        const uint32_t p = beg + rank;
-        ko[p] = key;
-        vo[p] = val;
-        dp = p;
-    };
-    for (uint32_t base = 0; base < nfin; base += kBktWinPos) {
-#pragma unroll
-        for (uint32_t j = 0; j < kStage; ++j) {
-            const uint32_t x = wb + tid + j * 1024u;
-            if (x < we) {
-                s_key[x - wb] = pk[j];
-                s_val[x - wb] = pv[j];
-            }
-        }
-        lds_barrier();
-        store_box(dp0, dlo0, dhi0);
-        store_box(dp1, dlo1, dhi1);
-        uint32_t nwb = 0, nwe = 0;
-        if (base + kBktWinPos < nfin) {
-            window(base + kBktWinPos, nwb, nwe);
-#pragma unroll
-            for (uint32_t j = 0; j < kStage; ++j) {
-                const uint32_t x = min(nwb + tid + j * 1024u, nwe - 1u);
-                pk[j] = kt[x];
-                pv[j] = vt[x];
-            }
-        }
-        rank_one(base + tid, wb, dlo0, dhi0, dp0);
-        rank_one(base + 1024u + tid, wb, dlo1, dhi1, dp1);
-        lds_barrier();
-        wb = nwb;
-        we = nwe;
+    __syncthreads();
+    const uint32_t e = B0 + tid;
+    if (e >= nfin) return;
+    const uint64_t key = s_key[e - wb];
+    const uint32_t val = s_val[e - wb];
+    const float *boxes = q.aabbs + (size_t)c * (N + 1) * 8u;
+    // the object's box, read while the rank is counted
+    const float4 glo = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[0];
+    const float4 ghi = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
+    const uint32_t b = (uint32_t)(key >> (64u - BITS));
+    const uint32_t beg = b ? s_end[b - 1u] : 0u, end = s_end[b];
+    uint32_t rank = 0, eq = 0;
+    for (uint32_t m = beg; m < end; ++m) {
+        const uint64_t km = s_key[m - wb];
+        rank += km < key ? 1u : 0u;
+        eq += km == key ? 1u : 0u;
     }
-    store_box(dp0, dlo0, dhi0);
-    store_box(dp1, dlo1, dhi1);
+    if (eq > 1u) {
+        // equal codes: the lower object index first
+        for (uint32_t m = beg; m < end; ++m) rank += (s_key[m - wb] == key && s_val[m - wb] < val) ? 1u : 0u;
+    }
+    const uint32_t p = beg + rank;
+    q.srt_mcodes[(size_t)c * N + p] = key;
+    q.srt_idx[(size_t)c * N + p] = val;
+    float *sboxes = q.srt_aabbs + (size_t)c * (N + 1) * 8u;
+    reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[0] = glo;
+    reinterpret_cast<float4 *>(sboxes + (size_t)p * 8u)[1] = ghi;
 }
 
 // Next work item of a persistent workgroup: one atomic per workgroup on the
@@ -1077,9 +1047,7 @@ __device__ inline uint32_t next_item(uint32_t *counter, uint32_t *s_item)
 // order of the list (largest problems first, so that the long items start
 // early and the short ones fill the end).
 template <uint32_t BITS>
-// (one 1024-thread workgroup runs per CU whatever the LDS size, so the
-// registers are sized for 4 waves per SIMD)
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_bucket_sort(const Prob *__restrict__ P, const uint2 *__restrict__ items,
+__global__ __launch_bounds__(1024) void k_bucket_sort(const Prob *__restrict__ P, const uint2 *__restrict__ items,
                                                       uint32_t n_items, uint32_t *counter)
 {
     __shared__ BucketLds<BITS> L;
@@ -2051,7 +2019,8 @@ hipError_t launch_gbox_finalize(const Prob *P, uint32_t np, const BatchTables &t
 }
 
 hipError_t launch_morton_sort(const Prob *P, uint32_t np, const BatchTables &t, uint32_t n_steps_all,
-                              uint32_t n_gather_blocks, BatchCounters *bc, const LaunchShape &ls, hipStream_t stream)
+                              uint32_t n_gather_blocks, uint32_t n_rank_blocks, BatchCounters *bc,
+                              const LaunchShape &ls, hipStream_t stream)
 {
     const dim3 g(std::max(1u, std::min(ls.sort_wgs, t.n_sort_items)));
     switch (ls.bucket_bits) {
@@ -2070,6 +2039,24 @@ hipError_t launch_morton_sort(const Prob *P, uint32_t np, const BatchTables &t, 
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (n_rank_blocks) {
+        switch (ls.bucket_bits) {
+            case 10:
+                hipLaunchKernelGGL(k_bucket_rank<10>, dim3(n_rank_blocks), dim3(kLbvhBlock), 0, stream, P, np,
+                                   t.lbvh_blk);
+                break;
+            case 11:
+                hipLaunchKernelGGL(k_bucket_rank<11>, dim3(n_rank_blocks), dim3(kLbvhBlock), 0, stream, P, np,
+                                   t.lbvh_blk);
+                break;
+            default:
+                hipLaunchKernelGGL(k_bucket_rank<12>, dim3(n_rank_blocks), dim3(kLbvhBlock), 0, stream, P, np,
+                                   t.lbvh_blk);
+                break;
+        }
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     // steps with an oversized bucket: stable radix sort and gather
     hipLaunchKernelGGL(k_sort, dim3(n_steps_all), dim3(kSortBlock), 0, stream, P, np, t.steps);
     e = hipGetLastError();
