@@ -956,18 +956,18 @@ __device__ void bucket_sort_step(const Prob &q, uint32_t c, BucketLds<BITS> &L)
 
 // The ranking of the bucket sort, one workgroup per block of kLbvhBlock
 // sorted positions of a step (one thread each): the block's positions,
-// widened to the whole buckets they cut (found by a binary search of the
-// bucket ends), are staged in LDS, and each object's place is its bucket's
-// start plus the number of smaller codes in its bucket (object indices
-// compared only among equal codes). The final write gathers the box, so the
-// sorted boxes, codes and indices leave in one pass.
+// widened to the whole buckets they cut, are staged in LDS, and each object's
+// place is its bucket's start plus the number of smaller codes in its bucket
+// (object indices compared only among equal codes). The buckets of the
+// block's first and last positions are those of the keys scattered there; a
+// thread's own key, bucket range and box are read while the block is staged.
+// The final write gathers the box, so the sorted boxes, codes and indices
+// leave in one pass.
 template <uint32_t BITS>
 __global__ __launch_bounds__(kLbvhBlock) void k_bucket_rank(const Prob *__restrict__ P, uint32_t np,
                                                            const uint32_t *__restrict__ blk_pre)
 {
-    constexpr uint32_t kBkts = 1u << BITS;
     constexpr uint32_t kWin = kLbvhBlock + 2u * kBktMax;
-    __shared__ uint32_t s_end[kBkts];
     __shared__ uint64_t s_key[kWin];
     __shared__ uint32_t s_val[kWin];
     const uint32_t gblk = xcd_block(blockIdx.x, gridDim.x);
@@ -983,37 +983,36 @@ __global__ __launch_bounds__(kLbvhBlock) void k_bucket_rank(const Prob *__restri
     if (B0 >= nfin) return;  // the whole workgroup
     const uint32_t tid = threadIdx.x;
     const uint32_t *ends = q.bkt_end + (size_t)c * kBktEndStride;
-    for (uint32_t k = tid; k < kBkts; k += kLbvhBlock) s_end[k] = ends[k];
-    __syncthreads();
-    auto bucket_at = [&](uint32_t pos) {
-        // the first bucket ending after pos (the ends do not decrease)
-        uint32_t lo = 0, hi = kBkts - 1u;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_end[mid] > pos) hi = mid;
-            else lo = mid + 1u;
-        }
-        return lo;
-    };
-    const uint32_t b0 = bucket_at(B0), b1 = bucket_at(min(B0 + kLbvhBlock - 1u, nfin - 1u));
-    const uint32_t wb = b0 ? s_end[b0 - 1u] : 0u, we = s_end[b1];
     const uint64_t *kt = q.keys_tmp + (size_t)c * N;
     const uint32_t *vt = q.vals_tmp + (size_t)c * N;
+    const uint32_t e = B0 + tid;
+    const bool live = e < nfin;
+    // the thread's key (in scatter order: ranked among its bucket's keys)
+    const uint64_t key = kt[live ? e : B0];
+    const uint32_t val = vt[live ? e : B0];
+    const uint32_t b = (uint32_t)(key >> (64u - BITS));
+    const uint32_t beg = b ? ends[b - 1u] : 0u, end = ends[b];
+    const float *boxes = q.aabbs + (size_t)c * (N + 1) * 8u;
+    const float4 glo = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[0];
+    const float4 ghi = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
+    // the window: the buckets of the block's first and last positions
+    const uint32_t b0 = (uint32_t)(kt[B0] >> (64u - BITS));
+    const uint32_t b1 = (uint32_t)(kt[min(B0 + kLbvhBlock - 1u, nfin - 1u)] >> (64u - BITS));
+    const uint32_t wb = b0 ? ends[b0 - 1u] : 0u, we = ends[b1];
+    if (we < wb || we - wb > kWin) {  // (the whole workgroup: the window's bounds are its own)
+        report_error(q.err, kErrBvhBuild, c, B0, 0.0);
+        return;
+    }
     for (uint32_t x = wb + tid; x < we; x += kLbvhBlock) {
         s_key[x - wb] = kt[x];
         s_val[x - wb] = vt[x];
     }
     __syncthreads();
-    const uint32_t e = B0 + tid;
-    if (e >= nfin) return;
-    const uint64_t key = s_key[e - wb];
-    const uint32_t val = s_val[e - wb];
-    const float *boxes = q.aabbs + (size_t)c * (N + 1) * 8u;
-    // the object's box, read while the rank is counted
-    const float4 glo = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[0];
-    const float4 ghi = reinterpret_cast<const float4 *>(boxes + (size_t)val * 8u)[1];
-    const uint32_t b = (uint32_t)(key >> (64u - BITS));
-    const uint32_t beg = b ? s_end[b - 1u] : 0u, end = s_end[b];
+    if (!live) return;
+    if (beg < wb || end > we) {
+        report_error(q.err, kErrBvhBuild, c, e, 0.0);
+        return;
+    }
     uint32_t rank = 0, eq = 0;
     for (uint32_t m = beg; m < end; ++m) {
         const uint64_t km = s_key[m - wb];
