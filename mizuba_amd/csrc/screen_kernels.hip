@@ -1741,16 +1741,15 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
         }
         const uint4 nq = *reinterpret_cast<const uint4 *>(T[i].qb);
         const uint4 meta = *reinterpret_cast<const uint4 *>(&T[i].escape);
-        // the next node is chosen from all four words on every path, so that
-        // the record is read with one load: an internal node {escape, right,
-        // left, 1} is entered (skipping the left child if it ended at or
-        // before the run, which no node after the run does: the split is the
-        // right child's index halved); a leaf {escape, begin, end, 0}, or a
-        // node missed, is left by its escape
+        // an internal node {escape, right, left, 1} is entered, to its left
+        // child (every node the walk reaches lies after the run); a leaf
+        // {escape, begin, end, 0}, or a node missed, is left by its escape.
+        // The next node is chosen from the record's words on every path, so
+        // that the record is read with one load.
         const bool hit_node = qbox_overlap(qq, nq);
         const bool enter = hit_node && meta.w != 0u;
-        const uint32_t next = enter ? ((meta.y >> 1) <= re ? meta.y : meta.z) : meta.x;
-        if (hit_node && meta.w == 0u && meta.y >= re) {
+        const uint32_t next = enter ? meta.z : meta.x;
+        if (hit_node && meta.w == 0u) {
             const bool single = meta.z - meta.y == 1u;
             // the leaf's float box: its object's, or the union of its run's
             float4 nl = reinterpret_cast<const float4 *>(boxes + (size_t)meta.y * 8u)[0];
