@@ -158,6 +158,9 @@ std::string dev_error_message(const DevError &e)
         case kErrBvhBuild:
             return "Internal error: the broad-phase tree of the conjunction step " + i
                    + " links its node " + j + " outside the tree";
+        case kErrSortWindow:
+            return "Internal error: the Morton sort of the conjunction step " + i + " found the buckets at sorted position "
+                   + j + " outside its window";
         default:
             return "device error code " + std::to_string(e.code);
     }

@@ -123,6 +123,7 @@ enum DevErrCode : int32_t {
     kErrRfInterval = 23,   // invalid root-finding interval (value, i = polynomial)
     kErrBvhWalk = 24,      // a broad-phase walk longer than its tree (i = step, j = sorted position)
     kErrBvhBuild = 25,     // a walk-tree node with a parent or sibling outside the tree (i = step, j = node)
+    kErrSortWindow = 26,   // a bucket-sort ranking block whose buckets exceed its window (i = step, j = position)
 };
 
 __device__ inline void report_error(DevError *e, int32_t code, uint32_t i, uint32_t j, double v)

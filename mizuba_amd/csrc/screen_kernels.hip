@@ -1000,7 +1000,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_bucket_rank(const Prob *__restri
     const uint32_t b1 = (uint32_t)(kt[min(B0 + kLbvhBlock - 1u, nfin - 1u)] >> (64u - BITS));
     const uint32_t wb = b0 ? ends[b0 - 1u] : 0u, we = ends[b1];
     if (we < wb || we - wb > kWin) {  // (the whole workgroup: the window's bounds are its own)
-        report_error(q.err, kErrBvhBuild, c, B0, 0.0);
+        report_error(q.err, kErrSortWindow, c, B0, 0.0);
         return;
     }
     for (uint32_t x = wb + tid; x < we; x += kLbvhBlock) {
@@ -1010,7 +1010,7 @@ __global__ __launch_bounds__(kLbvhBlock) void k_bucket_rank(const Prob *__restri
     __syncthreads();
     if (!live) return;
     if (beg < wb || end > we) {
-        report_error(q.err, kErrBvhBuild, c, e, 0.0);
+        report_error(q.err, kErrSortWindow, c, e, 0.0);
         return;
     }
     uint32_t rank = 0, eq = 0;
