@@ -93,22 +93,29 @@ MZH_LBVH_HD inline int64_t lbvh_dir(const Code &code, uint32_t n, uint32_t k, in
 // than dmin bits with key k. Reads the codes at k - 1, k, k + 1 and s0 - 1 or
 // s1.
 template <class Code>
+MZH_LBVH_HD inline bool lbvh_leaves(const Code &code, uint32_t n, uint32_t k, int64_t s0, int64_t s1, int64_t d,
+                                    int dmin)
+{
+    return lbvh_delta(code, 0, n, k, d > 0 ? s1 : s0 - 1) > dmin;
+}
+
+template <class Code>
 MZH_LBVH_HD inline bool lbvh_leaves(const Code &code, uint32_t n, uint32_t k, int64_t s0, int64_t s1)
 {
     int dmin;
     const int64_t d = lbvh_dir(code, n, k, dmin);
-    return lbvh_delta(code, 0, n, k, d > 0 ? s1 : s0 - 1) > dmin;
+    return lbvh_leaves(code, n, k, s0, s1, d, dmin);
 }
 
 // Internal node k, its other end and split searched over [s0, s1) only:
 // exact when the node's range lies inside [s0, s1) (lbvh_leaves false), since
 // the common prefix with key k only shrinks away from k. Reads the codes at
 // k - 1, k + 1 and inside [s0, s1).
+// (the same with k's direction d and prefix dmin known, from lbvh_dir)
 template <class Code>
-MZH_LBVH_HD inline LbvhNode lbvh_node(const Code &code, uint32_t n, uint32_t k, int64_t s0, int64_t s1)
+MZH_LBVH_HD inline LbvhNode lbvh_node(const Code &code, uint32_t n, uint32_t k, int64_t s0, int64_t s1, int64_t d,
+                                      int dmin)
 {
-    int dmin;
-    const int64_t d = lbvh_dir(code, n, k, dmin);
     // the other end: exponential then binary search for the farthest key
     // sharing more than dmin bits with key k
     uint64_t lmax = 2;
@@ -134,6 +141,14 @@ MZH_LBVH_HD inline LbvhNode lbvh_node(const Code &code, uint32_t n, uint32_t k, 
     r.right_leaf = (int64_t)r.hi == gamma + 1;
     r.run = code(r.lo) == code(r.hi);
     return r;
+}
+
+template <class Code>
+MZH_LBVH_HD inline LbvhNode lbvh_node(const Code &code, uint32_t n, uint32_t k, int64_t s0, int64_t s1)
+{
+    int dmin;
+    const int64_t d = lbvh_dir(code, n, k, dmin);
+    return lbvh_node(code, n, k, s0, s1, d, dmin);
 }
 
 MZH_LBVH_HD inline LbvhNode lbvh_node(const uint64_t *codes, uint32_t n, uint32_t k)

@@ -1453,7 +1453,9 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
         leaf[1] = make_uint4(lbvh_escape(code, n, p), p, p + 1u, 0u);
         runs = p == 0u ? 1u : (code(p) != code(p - 1u) ? 2u : 0u);
         if (p + 1u < n) {
-            if (lbvh_leaves(code, n, p, B0, B1)) {
+            int dmin;
+            const int64_t dir = lbvh_dir(code, n, p, dmin);
+            if (lbvh_leaves(code, n, p, B0, B1, dir, dmin)) {
                 const uint32_t at = atomicAdd(W.nspan, 1u);
                 if (at < W.span_cap) {
                     W.span[at] = p;
@@ -1461,7 +1463,7 @@ __global__ __launch_bounds__(BS) void k_lbvh_build(const Prob *__restrict__ P, u
                     report_error(q.err, kErrBvhBuild, c, 2u * p, 0.0);
                 }
             } else {
-                const LbvhNode nd = lbvh_node(code, n, p, B0, B1);
+                const LbvhNode nd = lbvh_node(code, n, p, B0, B1, dir, dmin);
                 s_meta[t] = lbvh_meta(nd, lbvh_escape(code, n, nd.hi));
                 s_info[t] = lbvh_info(nd);
                 // inside the block, so are its children
