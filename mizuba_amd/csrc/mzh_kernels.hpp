@@ -130,7 +130,7 @@ struct BatchTables {
 
 // Launch shapes that leave the results unchanged (chosen by the tuner).
 struct LaunchShape {
-    uint32_t bp_block = 256;       // broad-phase walk workgroup: 128, 256 or 512
+    uint32_t bp_block = 128;       // broad-phase walk workgroup: 128, 256 or 512
     uint32_t sort_wgs = 512;       // persistent sort workgroups
     uint32_t bvh_wgs = 512;        // persistent BVH workgroups
     uint32_t roots_wgs = 512;      // root-finding workgroups
