@@ -1697,7 +1697,7 @@ __global__ __launch_bounds__(BS) void k_bp(const Prob *__restrict__ P, uint32_t 
     const bool multi = re - rb > 1u;
     // partners found during the walk wait in registers and are recorded
     // together (all lanes at once) when the buffer fills and at the end
-    constexpr uint32_t kBuf = 8;
+    constexpr uint32_t kBuf = 4;
     uint32_t buf[kBuf] = {};
     uint32_t nbuf = 0;
     auto flush = [&]() {
